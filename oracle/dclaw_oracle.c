@@ -1,0 +1,1251 @@
+/*
+ * dclaw_oracle.c -- CPU restatement of D-Claw's per-patch finite-volume step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Never linked into the product.
+ *
+ * Each routine restates the reference Fortran it cites, statement by
+ * statement and in the same floating-point evaluation order (Fortran
+ * left-to-right, `x**2` as x*x, unary minus outside the product), compiled
+ * with -ffp-contract=off and glibc libm, so that it reproduces the Fortran's
+ * arithmetic.  MAX/MIN are written as the x86 maxsd/minsd semantics
+ * ((a > b) ? a : b), SIGN(a,b) as copysign(|a|, b).
+ *
+ * Routines owned by D-Claw follow the files in src/2d/dig.  The GeoClaw/AMRClaw glue
+ * that is not part of the reference tree (step2, the inline MC limiter,
+ * stepgrid's update, bc2amr extrap/wall, the dt rule) is restated from
+ * Clawpack 5's published algorithm; those parts are pinned by invariants
+ * (conservation, lake at rest, symmetry, closed forms), not by reference
+ * source.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "dclaw_oracle.h"
+
+#define MEQN 7
+#define I_H 1
+#define I_HU 2
+#define I_HV 3
+#define I_HM 4
+#define I_PB 5
+#define I_HCHI 6
+#define I_BDIF 7
+#define I_PHI 2
+#define I_THETA 3
+#define I_FSPHI 4
+#define I_TAUDIR_X 5
+#define I_TAUDIR_Y 6
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline double dsign(double a, double b) { return copysign(fabs(a), b); }
+
+/* 1-D slice element (m 1-based, i in 1-mbc..) */
+#define S1(arr, ncomp, m, i) (arr)[((m) - 1) + (ncomp) * ((i) + mbc - 1)]
+/* 2-D patch element */
+#define P2(arr, ncomp, m, i, j) \
+    (arr)[((m) - 1) + (size_t)(ncomp) * ((size_t)((i) + mbc - 1) + (size_t)((j) + mbc - 1) * (size_t)(mx + 2 * mbc))]
+
+/* ------------------------------------------------------------------------ */
+/* qfix: digclaw_module.f90:285-345 */
+static void qfix(const oracle_params* P, double* h, double* hu, double* hv, double* hm,
+                 double* p, double* hchi, double* u, double* v, double* m, double* chi,
+                 double* rho, double gz)
+{
+    if (*h <= P->dry_tolerance) {
+        *h = 0.0; *hu = 0.0; *hv = 0.0; *hm = 0.0; *p = 0.0; *hchi = 0.0;
+        *u = 0.0; *v = 0.0; *m = 0.0; *chi = 0.0; *rho = 0.0;
+        return;
+    }
+    *u = *hu / *h;
+    *v = *hv / *h;
+    *m = *hm / *h;
+    *chi = *hchi / *h;
+    *m = dmax(*m, 0.0);
+    *m = dmin(*m, 1.0);
+    *hm = *h * *m;
+    *chi = dmax(*chi, 0.0);
+    *chi = dmin(*chi, 1.0);
+    *hchi = *h * *chi;
+    *rho = P->rho_s * *m + (1.0 - *m) * P->rho_f;
+    double pmax = *rho * gz * *h;
+    *p = dmax(0.0, *p);
+    *p = dmin(pmax, *p);
+}
+
+/* qfix_cmass: digclaw_module.f90:353-387 */
+static void qfix_cmass(const oracle_params* P, double* h, double* m, double* p, double* rho,
+                       double* p_exc, double* hu, double* hv, double* hm, double u, double v,
+                       double rhoh, double gz)
+{
+    *m = dmax(*m, 0.0);
+    *m = dmin(*m, 1.0);
+    *rho = *m * (P->rho_s - P->rho_f) + P->rho_f;
+    *h = rhoh / *rho;
+    double p_exc_max = rhoh * gz - P->rho_f * *h * gz;
+    double p_exc_min = 0.0 - P->rho_f * *h * gz;
+    *p_exc = dmax(p_exc_min, *p_exc);
+    *p_exc = dmin(p_exc_max, *p_exc);
+    *p = P->rho_f * gz * *h + *p_exc;
+    *hu = *h * u;
+    *hv = *h * v;
+    *hm = *h * *m;
+}
+
+/* setvars: digclaw_module.f90:394-491.  `phi` is the module global that
+ * calc_taudir overwrites (digclaw_module.f90:557); callers pass P with it. */
+static void setvars(const oracle_params* P, double h, double u, double v, double m, double p,
+                    double chi, double gz, double* rho, double* kperm, double* alphainv,
+                    double* m_eq, double* tanpsi, double* tau)
+{
+    *rho = m * (P->rho_s - P->rho_f) + P->rho_f;
+    double kr_chi = 1.0;
+    if (P->segregation == 1) kr_chi = pow(10.0, P->delta_kr_order * 2.0 * (chi - 0.5));
+    *kperm = kr_chi * P->kref * exp(-((m - P->mref) / 0.04));
+    double sig_eff = dmax(0.0, *rho * gz * h - p);
+    double sig_0 = P->sigma_0;
+    if (P->alphamethod == 1)
+        sig_0 = 0.50 * P->alpha_c * P->rho_f * (P->rho_s - P->rho_f) * gz * h / *rho;
+    *alphainv = m * (sig_eff + sig_0) / P->alpha_c;
+    double vnorm = sqrt(u * u + v * v);
+    double shear;
+    if (h > P->dry_tolerance) shear = 2.0 * vnorm / h;
+    else shear = 0.0;
+    double sd = shear * P->delta;
+    double Nden = P->rho_s * (sd * sd) + sig_eff;
+    double Nnum = P->mu * shear;
+    if (Nnum <= 0.0) *m_eq = P->m_crit;
+    else *m_eq = P->m_crit * (sqrt(Nden) / (sqrt(Nden) + sqrt(Nnum)));
+    *tanpsi = P->c1 * (m - *m_eq);
+    double psi = atan(P->c1 * (m - *m_eq));
+    *tau = sig_eff * tan(P->phi + psi);
+    *tau = *tau * 0.5 * (1.0 + tanh(100.0 * (m - 0.1)));
+}
+
+/* riemanntype: riemannsolvers_dclaw.f:526-637 */
+static void riemanntype(double hL, double hR, double uL, double uR, double* hm, double* s1m,
+                        double* s2m, int* rare1, int* rare2, int maxiter, double drytol,
+                        double g)
+{
+    double um, u1m, u2m, delu, h_max, h_min, h0, F_max, F_min, dfdh, F0, slope, gL, gR;
+    h_min = dmin(hR, hL);
+    h_max = dmax(hR, hL);
+    delu = uR - uL;
+    if (h_min <= drytol) {
+        *hm = 0.0;
+        um = 0.0;
+        *s1m = uR + uL - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * hL);
+        *s2m = uR + uL - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * hL);
+        if (hL <= 0.0) { *rare2 = 1; *rare1 = 0; }
+        else { *rare1 = 1; *rare2 = 0; }
+    } else {
+        F_min = delu + 2.0 * (sqrt(g * h_min) - sqrt(g * h_max));
+        F_max = delu + (h_max - h_min) * (sqrt(.5 * g * (h_max + h_min) / (h_max * h_min)));
+        if (F_min > 0.0) {
+            double t = dmax(0.0, -delu + 2.0 * (sqrt(g * hL) + sqrt(g * hR)));
+            *hm = (1.0 / (16.0 * g)) * (t * t);
+            um = dsign(1.0, *hm) * (uL + 2.0 * (sqrt(g * hL) - sqrt(g * *hm)));
+            *s1m = uL + 2.0 * sqrt(g * hL) - 3.0 * sqrt(g * *hm);
+            *s2m = uR - 2.0 * sqrt(g * hR) + 3.0 * sqrt(g * *hm);
+            *rare1 = 1;
+            *rare2 = 1;
+        } else if (F_max <= 0.0) {
+            h0 = h_max;
+            for (int iter = 1; iter <= maxiter; iter++) {
+                gL = sqrt(.5 * g * (1.0 / h0 + 1.0 / hL));
+                gR = sqrt(.5 * g * (1.0 / h0 + 1.0 / hR));
+                F0 = delu + (h0 - hL) * gL + (h0 - hR) * gR;
+                dfdh = gL - g * (h0 - hL) / (4.0 * (h0 * h0) * gL) + gR
+                       - g * (h0 - hR) / (4.0 * (h0 * h0) * gR);
+                slope = 2.0 * sqrt(h0) * dfdh;
+                double t = sqrt(h0) - F0 / slope;
+                h0 = t * t;
+            }
+            *hm = h0;
+            u1m = uL - (*hm - hL) * sqrt((.5 * g) * (1.0 / *hm + 1.0 / hL));
+            u2m = uR + (*hm - hR) * sqrt((.5 * g) * (1.0 / *hm + 1.0 / hR));
+            um = .5 * (u1m + u2m);
+            *s1m = u1m - sqrt(g * *hm);
+            *s2m = u2m + sqrt(g * *hm);
+            *rare1 = 0;
+            *rare2 = 0;
+        } else {
+            h0 = h_min;
+            for (int iter = 1; iter <= maxiter; iter++) {
+                F0 = delu + 2.0 * (sqrt(g * h0) - sqrt(g * h_max))
+                     + (h0 - h_min) * sqrt(.5 * g * (1.0 / h0 + 1.0 / h_min));
+                slope = (F_max - F0) / (h_max - h_min);
+                h0 = h0 - F0 / slope;
+            }
+            *hm = h0;
+            if (hL > hR) {
+                um = uL + 2.0 * sqrt(g * hL) - 2.0 * sqrt(g * *hm);
+                *s1m = uL + 2.0 * sqrt(g * hL) - 3.0 * sqrt(g * *hm);
+                *s2m = uL + 2.0 * sqrt(g * hL) - sqrt(g * *hm);
+                *rare1 = 1;
+                *rare2 = 0;
+            } else {
+                *s2m = uR - 2.0 * sqrt(g * hR) + 3.0 * sqrt(g * *hm);
+                *s1m = uR - 2.0 * sqrt(g * hR) + sqrt(g * *hm);
+                um = uR - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * *hm);
+                *rare2 = 1;
+                *rare1 = 0;
+            }
+        }
+    }
+    (void)um;
+}
+
+/* riemann_dig2_aug_sswave_ez: riemannsolvers_dclaw.f:3-521.
+ * fw is fw(6,3) column-major: fw[(m-1) + 6*(mw-1)]. */
+#define FW(m, mw) fw[((m) - 1) + 6 * ((mw) - 1)]
+static void riemann_dig2_aug_sswave_ez(const oracle_params* P, int ixy, double hL, double hR,
+        double huL, double huR, double hvL, double hvR, double hmL, double hmR, double pL,
+        double pR, double bL, double bR, double uL, double uR, double vL, double vR,
+        double mL, double mR, double thetaL, double thetaR, double* sw, double* fw,
+        int wallprob, double taudirL, double taudirR, double chiL, double chiR, double fsL,
+        double fsR)
+{
+    (void)ixy;
+    const double grav = P->grav, rho_f = P->rho_f, kappa = P->kappa;
+    double h, u, v, mbar, hustar_interface, u_interface;
+    double R[3][4];   /* R(0:2,1:3) */
+    double del[5];    /* del(0:4) */
+    double beta[4];   /* beta(1:3) */
+    double rho, rhoL, rhoR, tauL, tauR, tau, gzL, gzR, tanpsi, kperm, m_eq, alphainv;
+    double theta, gamma, eps, sL, sR, sRoe1, sRoe2, sE1, sE2, uhat, chat;
+    double delb, s1m, s2m, hm, criticaltol, criticaltol_2;
+    double s1s2bar, s1s2tilde, hbar, source2dx, veltol2, hstarHLL, deldelh, drytol, gz, geps;
+    double tausource, gammaL, gammaR, theta1, theta3, vnorm, alpha_seg, a, b, c;
+    double seg_L, seg_R;
+    int sonic, rare1, rare2, rarecorrector, static_, cwavetype;
+    const int delta_huvm_revert = 1;
+
+    veltol2 = 0.0;
+    drytol = P->dry_tolerance;
+    criticaltol = dmax(drytol * grav, 1e-6);
+    criticaltol_2 = sqrt(criticaltol);
+
+    gzL = grav * cos(thetaL);
+    gzR = grav * cos(thetaR);
+    theta = 0.5 * (thetaL + thetaR);
+    gz = grav * cos(theta);
+    alpha_seg = 1.0 - P->beta_seg;
+
+    if (hL >= drytol && hR >= drytol) {
+        setvars(P, hL, uL, vL, mL, pL, chiL, gzL, &rhoL, &kperm, &alphainv, &m_eq, &tanpsi, &tauL);
+        setvars(P, hR, uR, vR, mR, pR, chiR, gzR, &rhoR, &kperm, &alphainv, &m_eq, &tanpsi, &tauR);
+        h = 0.5 * (hL + hR);
+        v = 0.5 * (vL + vR);
+        mbar = 0.5 * (mL + mR);
+    } else if (hL >= drytol) {
+        setvars(P, hL, uL, vL, mL, pL, chiL, gzL, &rhoL, &kperm, &alphainv, &m_eq, &tanpsi, &tauL);
+        setvars(P, hL, uL, vL, mL, pL, chiL, gzL, &rhoR, &kperm, &alphainv, &m_eq, &tanpsi, &tauR);
+        tauR = 0.5 * tauL;
+        h = 0.5 * hL;
+        v = vL;
+        mbar = mL;
+    } else {
+        setvars(P, hR, uR, vR, mR, pR, chiR, gzR, &rhoL, &kperm, &alphainv, &m_eq, &tanpsi, &tauL);
+        setvars(P, hR, uR, vR, mR, pR, chiR, gzR, &rhoR, &kperm, &alphainv, &m_eq, &tanpsi, &tauR);
+        tauL = 0.5 * tauR;
+        h = 0.5 * hR;
+        v = vR;
+        mbar = mR;
+    }
+    (void)v; (void)mbar;
+
+    tauL = fsL * tauL;
+    tauR = fsR * tauR;
+    rho = 0.5 * (rhoL + rhoR);
+    tau = 0.5 * (tauL + tauR);
+    (void)tau;
+
+    if (P->src2method == -1) {
+        gamma = rho_f / rho;
+        gammaL = rho_f / rhoL;
+        gammaR = rho_f / rhoR;
+    } else {
+        gamma = 0.25 * (rho_f + 3.0 * rho) / rho;
+        gammaL = 0.25 * (rho_f + 3.0 * rhoL) / rhoL;
+        gammaR = 0.25 * (rho_f + 3.0 * rhoR) / rhoR;
+    }
+
+    eps = kappa + (1.0 - kappa) * gamma;
+    geps = gz * eps;
+
+    sL = uL - sqrt(geps * hL);
+    sR = uR + sqrt(geps * hR);
+    uhat = (sqrt(hL) * uL + sqrt(hR) * uR) / (sqrt(hR) + sqrt(hL));
+    chat = sqrt(geps * 0.5 * (hR + hL));
+    sRoe1 = uhat - chat;
+    sRoe2 = uhat + chat;
+
+    sE1 = dmin(sL, sRoe1);
+    sE2 = dmax(sR, sRoe2);
+    sw[0] = sE1;
+    sw[2] = sE2;
+    u = uhat;
+
+    riemanntype(hL, hR, uL, uR, &hm, &s1m, &s2m, &rare1, &rare2, 1, drytol, geps);
+    sw[0] = dmin(sw[0], s2m);
+    sw[2] = dmax(sw[2], s1m);
+    sw[1] = 0.5 * (sw[2] + sw[0]);
+
+    hstarHLL = dmax((huL - huR + sE2 * hR - sE1 * hL) / (sE2 - sE1), 0.0);
+
+    rarecorrector = 0;
+    if (P->rarecorrectortest) {
+        double sdelta = sw[2] - sw[0];
+        double raremin = 0.5, raremax = 0.9;
+        if (rare1 && sE1 * s1m < 0.0) raremin = 0.2;
+        if (rare2 && sE2 * s2m < 0.0) raremin = 0.2;
+        if (rare1 || rare2) {
+            double rare1st = 3.0 * (sqrt(geps * hL) - sqrt(geps * hm));
+            double rare2st = 3.0 * (sqrt(geps * hR) - sqrt(geps * hm));
+            if (dmax(rare1st, rare2st) > raremin * sdelta &&
+                dmax(rare1st, rare2st) < raremax * sdelta) {
+                rarecorrector = 1;
+                if (rare1st > rare2st) sw[1] = s1m;
+                else if (rare2st > rare1st) sw[1] = s2m;
+                else sw[1] = 0.5 * (s1m + s2m);
+            }
+        }
+        if (hstarHLL < dmin(hL, hR) / 5.0) rarecorrector = 0;
+    }
+
+    delb = (bR - bL);
+
+    hbar = 0.5 * (hL + hR);
+    {
+        double t = uL + uR;
+        s1s2bar = 0.25 * (t * t) - gz * hbar;
+    }
+    s1s2tilde = dmax(0.0, uL * uR) - gz * hbar;
+
+    sonic = 0;
+    if (fabs(s1s2bar) <= criticaltol) sonic = 1;
+    else if (s1s2bar * s1s2tilde <= criticaltol * criticaltol) sonic = 1;
+    else if (s1s2bar * sE1 * sE2 <= criticaltol * criticaltol) sonic = 1;
+    else if (dmin(fabs(sE1), fabs(sE2)) < criticaltol_2) sonic = 1;
+    else if (sE1 < criticaltol_2 && s1m > -criticaltol_2) sonic = 1;
+    else if (sE2 > -criticaltol_2 && s2m < criticaltol_2) sonic = 1;
+    else if ((uL + sqrt(geps * hL)) * (uR + sqrt(geps * hR)) < 0.0) sonic = 1;
+    else if ((uL - sqrt(geps * hL)) * (uR - sqrt(geps * hR)) < 0.0) sonic = 1;
+
+    if (sonic) source2dx = -(gz * hbar * delb);
+    else source2dx = -(gz * hbar * delb * s1s2tilde / s1s2bar);
+
+    source2dx = dmin(source2dx, gz * dmax(-(hL * delb), -(hR * delb)));
+    source2dx = dmax(source2dx, gz * dmin(-(hL * delb), -(hR * delb)));
+
+    if (fabs(u) <= veltol2) source2dx = -(hbar * gz * delb);
+
+    if (sonic) deldelh = -delb;
+    else deldelh = delb * gz * hbar / s1s2bar;
+
+    if (sE1 < -criticaltol && sE2 > criticaltol) {
+        deldelh = dmin(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+        deldelh = dmax(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+    } else if (sE1 >= criticaltol) {
+        deldelh = dmin(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+        deldelh = dmax(deldelh, -hL);
+    } else if (sE2 <= -criticaltol) {
+        deldelh = dmin(deldelh, hR);
+        deldelh = dmax(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+    }
+
+    R[0][2] = 0.0; R[1][2] = 0.0; R[2][2] = 1.0;
+    cwavetype = 1;
+    R[0][1] = 1.0; R[1][1] = sw[0]; R[2][1] = sw[0] * sw[0];
+    R[0][3] = 1.0; R[1][3] = sw[2]; R[2][3] = sw[2] * sw[2];
+    if (rarecorrector) {
+        R[0][2] = 1.0; R[1][2] = sw[1]; R[2][2] = sw[1] * sw[1];
+        cwavetype = 2;
+    }
+
+    del[0] = hR - hL - deldelh;
+    del[1] = huR - huL;
+    del[2] = hR * (uR * uR) + 0.5 * kappa * gz * (hR * hR)
+             - (hL * (uL * uL) + 0.5 * kappa * gz * (hL * hL));
+    del[2] = del[2] + (1.0 - kappa) * h * (pR - pL) / rho;
+    del[3] = pR - pL - gamma * rho * gz * deldelh;
+    del[4] = -(gamma * rho * gz * u * (hR - hL)) + gamma * rho * gz * del[1] + u * (pR - pL);
+
+    vnorm = sqrt(uR * uR + uL * uL + vR * vR + vL * vL);
+    static_ = 0;
+    if (vnorm > 0.0) {
+        tausource = 0.0;
+    } else if (0.5 * fabs(taudirR * tauR / rhoR + tauL * taudirR / rhoL) > fabs(del[2] - source2dx)) {
+        tausource = del[2] - source2dx;
+        del[1] = 0.0;
+        del[0] = 0.0;
+        del[3] = 0.0;
+        del[4] = 0.0;
+        static_ = 1;
+    } else {
+        tausource = 0.5 * ((taudirR * tauR / rhoR) + (tauL * taudirR / rhoL));
+        tausource = dsign(tausource, del[2] - source2dx);
+    }
+    if (wallprob) tausource = 0.0;
+
+    del[2] = del[2] - source2dx - tausource;
+
+    if (sw[0] >= 0.0) { theta1 = thetaR; theta3 = thetaR; }
+    else if (sw[2] <= 0.0) { theta1 = thetaL; theta3 = thetaL; }
+    else if (sw[1] >= 0.0) { theta1 = thetaL; theta3 = thetaR; }
+    else { theta1 = thetaL; theta3 = thetaR; }
+
+    a = sw[0];
+    b = sw[1];
+    c = sw[2];
+    if (cwavetype == 1) {
+        beta[1] = (c * del[0] - del[1]) / (c - a);
+        beta[2] = a * c * del[0] - (a + c) * del[1] + del[2];
+        beta[3] = (del[1] - a * del[0]) / (c - a);
+    } else {
+        beta[1] = (b * c * del[0] - (b + c) * del[1] + del[2]) / (a * a - a * b - a * c + b * c);
+        beta[2] = (-(a * c * del[0]) + (a + c) * del[1] - del[2]) / (a * b - a * c - b * b + b * c);
+        beta[3] = (a * b * del[0] - (a + b) * del[1] + del[2]) / (a * b - a * c - b * c + c * c);
+    }
+
+    for (int mw = 1; mw <= 3; mw++)
+        for (int m = 1; m <= 2; m++) FW(m, mw) = beta[mw] * R[m][mw];
+
+    FW(4, 1) = FW(1, 1) * mL;
+    FW(4, 3) = FW(1, 3) * mR;
+    FW(4, 2) = hmR * uR - hmL * uL - FW(4, 1) - FW(4, 3);
+
+    FW(3, 1) = FW(1, 1) * vL;
+    FW(3, 3) = FW(1, 3) * vR;
+    FW(3, 2) = hvR * uR - hvL * uL - FW(3, 1) - FW(3, 3);
+
+    FW(5, 1) = FW(1, 1) * gammaL * rhoL * grav * cos(theta1);
+    FW(5, 3) = FW(1, 3) * gammaR * rhoR * grav * cos(theta3);
+    FW(5, 2) = del[4] - FW(5, 3) - FW(5, 1);
+
+    seg_L = chiL * hL * uL * (1.0 + (1.0 - alpha_seg) * (1.0 - chiL));
+    seg_R = chiR * hR * uR * (1.0 + (1.0 - alpha_seg) * (1.0 - chiR));
+    FW(6, 1) = FW(1, 1) * chiL * (1.0 + (1.0 - alpha_seg) * (1.0 - chiL));
+    FW(6, 3) = FW(1, 3) * chiR * (1.0 + (1.0 - alpha_seg) * (1.0 - chiR));
+    FW(6, 2) = seg_R - seg_L - FW(6, 1) - FW(6, 3);
+
+    if (delta_huvm_revert) {
+        hustar_interface = huL + FW(1, 1);
+        if (!static_) {
+            u_interface = hustar_interface / hstarHLL;
+            del[4] = -(gamma * rho * gz * u_interface * (hR - hL)) + gamma * rho * gz * del[1]
+                     + u_interface * (pR - pL);
+            FW(5, 2) = del[4] - FW(5, 3) - FW(5, 1);
+        }
+        if (hustar_interface <= 0.0) {
+            FW(3, 1) = FW(3, 1) + FW(3, 2);
+            FW(4, 1) = FW(4, 1) + FW(4, 2);
+            FW(5, 1) = FW(5, 1) + FW(5, 2);
+            FW(6, 1) = FW(6, 1) + FW(6, 2);
+        } else {
+            FW(3, 3) = FW(3, 3) + FW(3, 2);
+            FW(4, 3) = FW(4, 3) + FW(4, 2);
+            FW(5, 3) = FW(5, 3) + FW(5, 2);
+            FW(6, 3) = FW(6, 3) + FW(6, 2);
+        }
+        FW(3, 2) = 0.0;
+        FW(4, 2) = 0.0;
+        FW(5, 2) = 0.0;
+        FW(6, 2) = 0.0;
+    }
+}
+#undef FW
+
+/* ------------------------------------------------------------------------ */
+/* rpn2: rpn2_dclaw.f:24-392 (mcapa = 0, bed_normal = 0).  ql = qr = q1d and
+ * qfix is applied to q1d IN PLACE (rpn2_dclaw.f:164-168), exactly as the
+ * reference does through its by-reference arguments. */
+void oracle_rpn2(const oracle_params* P, int ixy, int mx, int mbc, int maux, double* q1d,
+                 const double* aux2, double* fwave, double* s, double* amdq, double* apdq)
+{
+    const int mwaves = P->mwaves;
+    const double drytol = P->dry_tolerance, gmod = P->grav;
+    int mhu, nhv;
+    if (ixy == 1) { mhu = I_HU; nhv = I_HV; } else { mhu = I_HV; nhv = I_HU; }
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double wall[3], fw[18], sw[3];
+        for (int mw = 1; mw <= mwaves; mw++) {
+            s[(mw - 1) + mwaves * (i + mbc - 1)] = 0.0;
+            for (int m = 1; m <= MEQN; m++) fwave[(m - 1) + MEQN * ((mw - 1) + mwaves * (i + mbc - 1))] = 0.0;
+        }
+        for (int k = 0; k < 3; k++) sw[k] = 0.0;
+        for (int k = 0; k < 18; k++) fw[k] = 0.0;
+
+        if (S1(q1d, MEQN, I_H, i - 1) <= drytol && S1(q1d, MEQN, I_H, i) <= drytol) continue;
+
+        double taudirR, taudirL;
+        if (ixy == 1) {
+            taudirR = S1(aux2, maux, I_TAUDIR_X, i);
+            taudirL = S1(aux2, maux, I_TAUDIR_X, i - 1);
+        } else {
+            taudirR = S1(aux2, maux, I_TAUDIR_Y, i);
+            taudirL = S1(aux2, maux, I_TAUDIR_Y, i - 1);
+        }
+        double fsL = S1(aux2, maux, I_FSPHI, i - 1);
+        double fsR = S1(aux2, maux, I_FSPHI, i);
+        double thetaL = 0.0, thetaR = 0.0;
+        double gzL = P->grav, gzR = P->grav;
+
+        double uR, vR, mR, chiR, rhoR, uL, vL, mL, chiL, rhoL;
+        qfix(P, &S1(q1d, MEQN, I_H, i), &S1(q1d, MEQN, mhu, i), &S1(q1d, MEQN, nhv, i),
+             &S1(q1d, MEQN, I_HM, i), &S1(q1d, MEQN, I_PB, i), &S1(q1d, MEQN, I_HCHI, i),
+             &uR, &vR, &mR, &chiR, &rhoR, gzR);
+        qfix(P, &S1(q1d, MEQN, I_H, i - 1), &S1(q1d, MEQN, mhu, i - 1), &S1(q1d, MEQN, nhv, i - 1),
+             &S1(q1d, MEQN, I_HM, i - 1), &S1(q1d, MEQN, I_PB, i - 1), &S1(q1d, MEQN, I_HCHI, i - 1),
+             &uL, &vL, &mL, &chiL, &rhoL, gzL);
+
+        double hL = S1(q1d, MEQN, I_H, i - 1), hR = S1(q1d, MEQN, I_H, i);
+        double huL = S1(q1d, MEQN, mhu, i - 1), huR = S1(q1d, MEQN, mhu, i);
+        double hvL = S1(q1d, MEQN, nhv, i - 1), hvR = S1(q1d, MEQN, nhv, i);
+        double hmL = S1(q1d, MEQN, I_HM, i - 1), hmR = S1(q1d, MEQN, I_HM, i);
+        double pL = S1(q1d, MEQN, I_PB, i - 1), pR = S1(q1d, MEQN, I_PB, i);
+        double bL = S1(aux2, maux, 1, i - 1) - S1(q1d, MEQN, I_BDIF, i - 1);
+        double bR = S1(aux2, maux, 1, i) - S1(q1d, MEQN, I_BDIF, i);
+        double chiHL = S1(q1d, MEQN, I_HCHI, i - 1), chiHR = S1(q1d, MEQN, I_HCHI, i);
+        (void)chiHL; (void)chiHR;
+
+        for (int mw = 0; mw < 3; mw++) wall[mw] = 1.0;
+        int wallprob = 0;
+        double hstar, hstartest, s1m, s2m;
+        int rare1, rare2;
+        if (hR <= drytol) {
+            hR = 0.0;
+            pR = 0.0;
+            mR = mL;
+            chiR = chiL;
+            riemanntype(hL, hL, uL, -uL, &hstar, &s1m, &s2m, &rare1, &rare2, 1, drytol, gmod);
+            hstartest = dmax(hL, hstar);
+            if (hstartest + bL < bR) {
+                wall[1] = 0.0;
+                wall[2] = 0.0;
+                wallprob = 1;
+                hR = hL; huR = -huL; hvR = hvL; hmR = hmL; bR = bL;
+                uR = -uL; vR = vL; mR = mL; pR = pL; chiHR = chiHL; chiR = chiL;
+            }
+        } else if (hL <= drytol) {
+            hL = 0.0;
+            pL = 0.0;
+            mL = mR;
+            chiL = chiR;
+            riemanntype(hR, hR, -uR, uR, &hstar, &s1m, &s2m, &rare1, &rare2, 1, drytol, gmod);
+            hstartest = dmax(hR, hstar);
+            if (hstartest + bR < bL) {
+                wall[0] = 0.0;
+                wall[1] = 0.0;
+                wallprob = 1;
+                hL = hR; huL = -huR; hvL = hvR; hmL = hmR; mL = mR; bL = bR;
+                uL = -uR; vL = vR; pL = pR; chiHL = chiHR; chiL = chiR;
+            }
+        }
+
+        riemann_dig2_aug_sswave_ez(P, ixy, hL, hR, huL, huR, hvL, hvR, hmL, hmR, pL, pR, bL,
+                                   bR, uL, uR, vL, vR, mL, mR, thetaL, thetaR, sw, fw,
+                                   wallprob, taudirL, taudirR, chiL, chiR, fsL, fsR);
+
+        for (int mw = 0; mw < 3; mw++) {
+            sw[mw] = sw[mw] * wall[mw];
+            for (int m = 0; m < 6; m++) fw[m + 6 * mw] = fw[m + 6 * mw] * wall[mw];
+        }
+
+#define SS(mw) s[((mw) - 1) + mwaves * (i + mbc - 1)]
+#define FWV(m, mw) fwave[((m) - 1) + MEQN * (((mw) - 1) + mwaves * (i + mbc - 1))]
+#define F6(m, mw) fw[((m) - 1) + 6 * ((mw) - 1)]
+        SS(1) = sw[0];
+        SS(2) = sw[1];
+        SS(3) = sw[1];
+        SS(4) = sw[1];
+        SS(5) = sw[2];
+
+        FWV(I_H, 1) = F6(1, 1);
+        FWV(mhu, 1) = F6(2, 1);
+        FWV(nhv, 1) = F6(3, 1);
+        FWV(I_HM, 1) = F6(4, 1);
+        FWV(I_PB, 1) = F6(5, 1);
+        FWV(I_HCHI, 1) = F6(6, 1);
+
+        FWV(I_H, 5) = F6(1, 3);
+        FWV(mhu, 5) = F6(2, 3);
+        FWV(nhv, 5) = F6(3, 3);
+        FWV(I_HM, 5) = F6(4, 3);
+        FWV(I_PB, 5) = F6(5, 3);
+        FWV(I_HCHI, 5) = F6(6, 3);
+
+        FWV(I_H, 2) = F6(1, 2);
+        FWV(mhu, 2) = F6(2, 2);
+        FWV(nhv, 2) = F6(3, 2);
+        FWV(I_HM, 2) = 0.0;
+        FWV(I_PB, 2) = 0.0;
+        FWV(I_HCHI, 2) = F6(6, 2);
+
+        FWV(I_H, 3) = 0.0;
+        FWV(mhu, 3) = 0.0;
+        FWV(nhv, 3) = 0.0;
+        FWV(I_HM, 3) = F6(4, 2);
+        FWV(I_PB, 3) = 0.0;
+        FWV(I_HCHI, 3) = 0.0;
+
+        FWV(I_H, 4) = 0.0;
+        FWV(mhu, 4) = 0.0;
+        FWV(nhv, 4) = 0.0;
+        FWV(I_HM, 4) = 0.0;
+        FWV(I_PB, 4) = F6(5, 2);
+        FWV(I_HCHI, 4) = 0.0;
+#undef SS
+#undef FWV
+#undef F6
+    }
+
+    /* fluctuations: rpn2_dclaw.f:347-379 (s == 0 contributes to neither) */
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int m = 1; m <= MEQN; m++) {
+            S1(amdq, MEQN, m, i) = 0.0;
+            S1(apdq, MEQN, m, i) = 0.0;
+        }
+    for (int i = 2 - mbc; i <= mx + mbc; i++)
+        for (int mw = 1; mw <= mwaves; mw++) {
+            double sv = s[(mw - 1) + mwaves * (i + mbc - 1)];
+            const double* f = &fwave[MEQN * ((mw - 1) + mwaves * (i + mbc - 1))];
+            if (sv < 0.0) {
+                for (int m = 1; m <= MEQN; m++) S1(amdq, MEQN, m, i) = S1(amdq, MEQN, m, i) + f[m - 1];
+            } else if (sv > 0.0) {
+                for (int m = 1; m <= MEQN; m++) S1(apdq, MEQN, m, i) = S1(apdq, MEQN, m, i) + f[m - 1];
+            }
+        }
+}
+
+/* rpt2: rpt2_dclaw.f:16-266 (coordinate_system = 1, bed_normal = 0) */
+void oracle_rpt2(const oracle_params* P, int ixy, int imp, int mx, int mbc, int maux,
+                 const double* q1d, const double* aux1, const double* aux2, const double* aux3,
+                 const double* asdq, double* bmasdq, double* bpasdq)
+{
+    int mu, mv;
+    if (ixy == 1) { mu = I_HU; mv = I_HV; } else { mu = I_HV; mv = I_HU; }
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int m = 1; m <= MEQN; m++) {
+            S1(bmasdq, MEQN, m, i) = 0.0;
+            S1(bpasdq, MEQN, m, i) = 0.0;
+        }
+    const double g = P->grav;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double h, u, v, m, p, chi, eta, topo1, topo3;
+        int ic = (imp == 1) ? i - 1 : i;
+        h = S1(q1d, MEQN, I_H, ic);
+        if (h <= P->dry_tolerance) continue;
+        u = S1(q1d, MEQN, mu, ic) / h;
+        v = S1(q1d, MEQN, mv, ic) / h;
+        m = S1(q1d, MEQN, I_HM, ic) / h;
+        p = S1(q1d, MEQN, I_PB, ic);
+        chi = S1(q1d, MEQN, I_HCHI, ic) / h;
+        eta = h + S1(aux2, maux, 1, ic) - S1(q1d, MEQN, I_BDIF, ic);
+        topo1 = S1(aux1, maux, 1, ic);
+        topo3 = S1(aux3, maux, 1, ic);
+        (void)p;
+
+        double rho = m * P->rho_s + (1.0 - m) * P->rho_f;
+        double gamma;
+        if (P->src2method == -1) gamma = P->rho_f / rho;
+        else gamma = 0.25 * (P->rho_f + 3.0 * rho) / rho;
+
+        if (eta < dmin(topo1, topo3)) continue;
+
+        const double dxdcp = 1.0, dxdcm = 1.0;
+        double s[4], beta[4], r[7][4];
+        s[1] = v - sqrt(g * h);
+        s[2] = v;
+        s[3] = v + sqrt(g * h);
+
+        double delf1 = S1(asdq, MEQN, I_H, i);
+        double delf2 = S1(asdq, MEQN, mu, i);
+        double delf3 = S1(asdq, MEQN, mv, i);
+
+        beta[1] = (s[3] * delf1 - delf3) / (s[3] - s[1]);
+        beta[2] = -(u * delf1) + delf2;
+        beta[3] = (delf3 - s[1] * delf1) / (s[3] - s[1]);
+
+        r[1][1] = 1.0; r[2][1] = u; r[3][1] = s[1]; r[4][1] = m; r[5][1] = gamma * rho * g; r[6][1] = chi;
+        r[1][3] = 1.0; r[2][3] = u; r[3][3] = s[3]; r[4][3] = m; r[5][3] = gamma * rho * g; r[6][3] = chi;
+        r[1][2] = 0.0; r[2][2] = 1.0; r[3][2] = 0.0; r[4][2] = 0.0; r[5][2] = 0.0; r[6][2] = 0.0;
+
+        const int comp[7] = {0, I_H, mu, mv, I_HM, I_PB, I_HCHI};
+        for (int mw = 1; mw <= 3; mw++) {
+            if (s[mw] < 0.0 && eta >= topo1) {
+                for (int k = 1; k <= 6; k++)
+                    S1(bmasdq, MEQN, comp[k], i) = S1(bmasdq, MEQN, comp[k], i) + dxdcm * s[mw] * beta[mw] * r[k][mw];
+            } else if (s[mw] > 0.0 && eta >= topo3) {
+                for (int k = 1; k <= 6; k++)
+                    S1(bpasdq, MEQN, comp[k], i) = S1(bpasdq, MEQN, comp[k], i) + dxdcp * s[mw] * beta[mw] * r[k][mw];
+            }
+        }
+    }
+}
+
+/* limiter: AMRClaw inlinelimiter.f (Clawpack 5; not in the reference tree,
+ * called at flux2fw.f:188).  Waves are compared with their UNLIMITED
+ * neighbours (dotr is formed before wave i is scaled). */
+void oracle_limiter(const oracle_params* P, int mx, int mbc, double* fwave, const double* s)
+{
+    const int mwaves = P->mwaves;
+#define WV(m, mw, i) fwave[((m) - 1) + MEQN * (((mw) - 1) + mwaves * ((i) + mbc - 1))]
+    for (int mw = 1; mw <= mwaves; mw++) {
+        if (P->mthlim[mw - 1] == 0) continue;
+        double dotr = 0.0;
+        for (int i = 0; i <= mx + 1; i++) {
+            double wnorm2 = 0.0;
+            double dotl = dotr;
+            dotr = 0.0;
+            for (int m = 1; m <= MEQN; m++) {
+                wnorm2 = wnorm2 + WV(m, mw, i) * WV(m, mw, i);
+                dotr = dotr + WV(m, mw, i) * WV(m, mw, i + 1);
+            }
+            if (i == 0) continue;
+            if (wnorm2 == 0.0) continue;
+            double r;
+            if (s[(mw - 1) + mwaves * (i + mbc - 1)] > 0.0) r = dotl / wnorm2;
+            else r = dotr / wnorm2;
+            double wlimitr;
+            switch (P->mthlim[mw - 1]) {
+            case 1: wlimitr = dmax(0.0, dmin(1.0, r)); break;
+            case 2: wlimitr = dmax(dmax(0.0, dmin(1.0, 2.0 * r)), dmin(2.0, r)); break;
+            case 3: wlimitr = (r + fabs(r)) / (1.0 + fabs(r)); break;
+            case 4: {
+                double c = (1.0 + r) / 2.0;
+                wlimitr = dmax(0.0, dmin(dmin(c, 2.0), 2.0 * r));
+                break;
+            }
+            default: wlimitr = r; break;
+            }
+            for (int m = 1; m <= MEQN; m++) WV(m, mw, i) = wlimitr * WV(m, mw, i);
+        }
+    }
+#undef WV
+}
+
+/* flux2: flux2fw.f:8-286 (relimit = .false., coordinate_system = 1) */
+void oracle_flux2(const oracle_params* P, int ixy, int maxm, int mx, int mbc, int maux,
+                  double* q1d, double* dtdx1d, const double* aux1, const double* aux2,
+                  const double* aux3, double* faddm, double* faddp, double* gaddm,
+                  double* gaddp, double* cfl1d)
+{
+    const int mwaves = P->mwaves;
+    const int n = maxm + 2 * mbc;
+    double* fwave = (double*)malloc(sizeof(double) * MEQN * mwaves * n);
+    double* s = (double*)malloc(sizeof(double) * mwaves * n);
+    double* amdq = (double*)malloc(sizeof(double) * MEQN * n);
+    double* apdq = (double*)malloc(sizeof(double) * MEQN * n);
+    double* cqxx = (double*)calloc((size_t)MEQN * n, sizeof(double));
+    double* bmasdq = (double*)malloc(sizeof(double) * MEQN * n);
+    double* bpasdq = (double*)malloc(sizeof(double) * MEQN * n);
+#define GADD(arr, m, i, side) (arr)[((m) - 1) + MEQN * (((i) + mbc - 1) + n * ((side) - 1))]
+    int limit = 0;
+    for (int mw = 0; mw < mwaves; mw++) if (P->mthlim[mw] > 0) limit = 1;
+
+    for (int side = 1; side <= 2; side++)
+        for (int i = 1 - mbc; i <= mx + mbc; i++)
+            for (int m = 1; m <= MEQN; m++) {
+                S1(faddm, MEQN, m, i) = 0.0;
+                S1(faddp, MEQN, m, i) = 0.0;
+                GADD(gaddm, m, i, side) = 0.0;
+                GADD(gaddp, m, i, side) = 0.0;
+            }
+
+    oracle_rpn2(P, ixy, mx, mbc, maux, q1d, aux2, fwave, s, amdq, apdq);
+
+    for (int i = 1 - mbc + 1; i <= mx + mbc - 1; i++)
+        for (int m = 1; m <= MEQN; m++) {
+            S1(faddp, MEQN, m, i) = S1(faddp, MEQN, m, i) - S1(apdq, MEQN, m, i);
+            S1(faddm, MEQN, m, i) = S1(faddm, MEQN, m, i) + S1(amdq, MEQN, m, i);
+        }
+
+    *cfl1d = 0.0;
+    for (int mw = 1; mw <= mwaves; mw++)
+        for (int i = 1; i <= mx + 1; i++) {
+            double sv = s[(mw - 1) + mwaves * (i + mbc - 1)];
+            double a = dtdx1d[i + mbc - 1] * sv;
+            double b = -(dtdx1d[i - 1 + mbc - 1] * sv);
+            /* dmax1(cfl1d, a, b) evaluated left to right */
+            *cfl1d = dmax(dmax(*cfl1d, a), b);
+        }
+
+    if (P->order != 1) {
+        if (limit) oracle_limiter(P, mx, mbc, fwave, s);
+        for (int i = 1; i <= mx + 1; i++) {
+            dtdx1d[i - 1 + mbc - 1] = 0.5 * (dtdx1d[i - 1 + mbc - 1] + dtdx1d[i + mbc - 1]);
+            double hleft2 = S1(q1d, MEQN, I_H, i - 2);
+            double hleft = S1(q1d, MEQN, I_H, i - 1);
+            double hright = S1(q1d, MEQN, I_H, i);
+            double hright2 = S1(q1d, MEQN, I_H, i + 1);
+            for (int m = 1; m <= MEQN; m++) {
+                S1(cqxx, MEQN, m, i) = 0.0;
+                if (hleft * hright * hleft2 * hright2 > 0.0) {
+                    for (int mw = 1; mw <= mwaves; mw++) {
+                        double sv = s[(mw - 1) + mwaves * (i + mbc - 1)];
+                        double fv = fwave[(m - 1) + MEQN * ((mw - 1) + mwaves * (i + mbc - 1))];
+                        S1(cqxx, MEQN, m, i) = S1(cqxx, MEQN, m, i)
+                            + dsign(1.0, sv) * (1.0 - fabs(sv) * dtdx1d[i - 1 + mbc - 1]) * fv;
+                    }
+                }
+                S1(faddm, MEQN, m, i) = S1(faddm, MEQN, m, i) + 0.5 * S1(cqxx, MEQN, m, i);
+                S1(faddp, MEQN, m, i) = S1(faddp, MEQN, m, i) + 0.5 * S1(cqxx, MEQN, m, i);
+            }
+        }
+    }
+
+    if (P->transverse != 0) {
+        if (P->order > 1 && P->transverse == 2) {
+            for (int i = 1; i <= mx + 1; i++)
+                for (int m = 1; m <= MEQN; m++) {
+                    S1(amdq, MEQN, m, i) = S1(amdq, MEQN, m, i) + S1(cqxx, MEQN, m, i);
+                    S1(apdq, MEQN, m, i) = S1(apdq, MEQN, m, i) - S1(cqxx, MEQN, m, i);
+                }
+        }
+        oracle_rpt2(P, ixy, 1, mx, mbc, maux, q1d, aux1, aux2, aux3, amdq, bmasdq, bpasdq);
+        for (int i = 1; i <= mx + 1; i++)
+            for (int m = 1; m <= MEQN; m++) {
+                double gupdate = 0.5 * dtdx1d[i - 1 + mbc - 1] * S1(bmasdq, MEQN, m, i);
+                GADD(gaddm, m, i - 1, 1) = GADD(gaddm, m, i - 1, 1) - gupdate;
+                GADD(gaddp, m, i - 1, 1) = GADD(gaddp, m, i - 1, 1) - gupdate;
+                gupdate = 0.5 * dtdx1d[i - 1 + mbc - 1] * S1(bpasdq, MEQN, m, i);
+                GADD(gaddm, m, i - 1, 2) = GADD(gaddm, m, i - 1, 2) - gupdate;
+                GADD(gaddp, m, i - 1, 2) = GADD(gaddp, m, i - 1, 2) - gupdate;
+            }
+        oracle_rpt2(P, ixy, 2, mx, mbc, maux, q1d, aux1, aux2, aux3, apdq, bmasdq, bpasdq);
+        for (int i = 1; i <= mx + 1; i++)
+            for (int m = 1; m <= MEQN; m++) {
+                double gupdate = 0.5 * dtdx1d[i - 1 + mbc - 1] * S1(bmasdq, MEQN, m, i);
+                GADD(gaddm, m, i, 1) = GADD(gaddm, m, i, 1) - gupdate;
+                GADD(gaddp, m, i, 1) = GADD(gaddp, m, i, 1) - gupdate;
+                gupdate = 0.5 * dtdx1d[i - 1 + mbc - 1] * S1(bpasdq, MEQN, m, i);
+                GADD(gaddm, m, i, 2) = GADD(gaddm, m, i, 2) - gupdate;
+                GADD(gaddp, m, i, 2) = GADD(gaddp, m, i, 2) - gupdate;
+            }
+    }
+#undef GADD
+    free(fwave); free(s); free(amdq); free(apdq); free(cqxx); free(bmasdq); free(bpasdq);
+}
+
+/* ------------------------------------------------------------------------ */
+/* calc_taudir: digclaw_module.f90:500-719 (bed_normal = 0).  Overwrites the
+ * global phi with aux(i_phi,i,j) for every cell it visits (:557). */
+void oracle_calc_taudir(oracle_params* P, int mx, int my, int mbc, int maux, double dx,
+                        double dy, const double* q, double* aux)
+{
+    const double gz = P->grav, theta = 0.0, drytol = P->dry_tolerance;
+    for (int j = 2 - mbc; j <= my + mbc - 1; j++)
+        for (int i = 2 - mbc; i <= mx + mbc - 1; i++) {
+            double u, v, m, chi, rho, uL, vL, mL, chiL, rhoL, uR, vR, mR, chiR, rhoR;
+            double uB, vB, mB, chiB, rhoB, uT, vT, mT, chiT, rhoT;
+            double h = P2(q, MEQN, I_H, i, j), hu = P2(q, MEQN, I_HU, i, j), hv = P2(q, MEQN, I_HV, i, j);
+            double hm = P2(q, MEQN, I_HM, i, j), p = P2(q, MEQN, I_PB, i, j), hchi = P2(q, MEQN, I_HCHI, i, j);
+            qfix(P, &h, &hu, &hv, &hm, &p, &hchi, &u, &v, &m, &chi, &rho, gz);
+            double b = P2(aux, maux, 1, i, j) - P2(q, MEQN, I_BDIF, i, j);
+            double eta = h + b;
+            P->phi = P2(aux, maux, I_PHI, i, j);
+
+#define NB(SUF, ii, jj)                                                                        \
+    double h##SUF = P2(q, MEQN, I_H, ii, jj), hu##SUF = P2(q, MEQN, I_HU, ii, jj);            \
+    double hv##SUF = P2(q, MEQN, I_HV, ii, jj), hm##SUF = P2(q, MEQN, I_HM, ii, jj);          \
+    double p##SUF = P2(q, MEQN, I_PB, ii, jj), hchi##SUF = P2(q, MEQN, I_HCHI, ii, jj);       \
+    qfix(P, &h##SUF, &hu##SUF, &hv##SUF, &hm##SUF, &p##SUF, &hchi##SUF, &u##SUF, &v##SUF,      \
+         &m##SUF, &chi##SUF, &rho##SUF, gz);                                                   \
+    double b##SUF = P2(aux, maux, 1, ii, jj) - P2(q, MEQN, I_BDIF, ii, jj);                   \
+    double eta##SUF = h##SUF + b##SUF;                                                         \
+    if (h##SUF < drytol) eta##SUF = dmin(eta##SUF, eta);
+
+            NB(L, i - 1, j)
+            NB(R, i + 1, j)
+            NB(B, i, j - 1)
+            NB(T, i, j + 1)
+#undef NB
+            (void)huL; (void)hvL; (void)huR; (void)hvR; (void)huB; (void)hvB; (void)huT; (void)hvT;
+            (void)bL; (void)bR; (void)bB; (void)bT;
+
+            if (h < drytol) {
+                eta = dmin(etaL, eta);
+                eta = dmin(etaB, eta);
+            }
+            if ((h + hL + hB + hR + hT) < drytol) {
+                P2(aux, maux, I_TAUDIR_X, i, j) = 0.0;
+                P2(aux, maux, I_TAUDIR_Y, i, j) = 0.0;
+                P2(aux, maux, I_FSPHI, i, j) = 0.0;
+                continue;
+            }
+            double kperm, alphainv, m_eq, tanpsi, tau;
+            setvars(P, h, u, v, m, p, chi, gz, &rho, &kperm, &alphainv, &m_eq, &tanpsi, &tau);
+
+            const double sth = sin(theta);
+            double FxL = -(gz * 0.5 * (h + hL) * (eta - etaL) / dx) + gz * 0.5 * (h + hL) * sth;
+            double FyL = -(gz * 0.5 * (h + hB) * (eta - etaB) / dy);
+            double FxR = -(gz * 0.5 * (h + hR) * (etaR - eta) / dx) + gz * 0.5 * (h + hR) * sth;
+            double FyR = -(gz * 0.5 * (h + hT) * (etaT - eta) / dy);
+            double Fx, Fy;
+            if (FxL * FxR > 0.0) Fx = dsign(dmin(fabs(FxL), fabs(FxR)), FxL);
+            else Fx = 0.0;
+            if (FyL * FyR > 0.0) Fy = dsign(dmin(fabs(FyL), fabs(FyR)), FyL);
+            else Fy = 0.0;
+
+            double hvnorm = sqrt(hu * hu + hv * hv);
+            if (hvnorm > 0.0) {
+                P2(aux, maux, I_TAUDIR_X, i, j) = -(dx * hu / hvnorm);
+                P2(aux, maux, I_TAUDIR_Y, i, j) = -(dy * hv / hvnorm);
+                double dot = dmin(dmax(0.0, Fx * hu), dmax(0.0, Fy * hv));
+                if (dot > 0.0) {
+                    double Fproj = dot / hvnorm;
+                    P2(aux, maux, I_FSPHI, i, j) = dmin(1.0, Fproj * rho / dmax(tau, 1e-16));
+                } else {
+                    P2(aux, maux, I_FSPHI, i, j) = 0.0;
+                }
+            } else {
+                if ((FxL * FxL + Fy * Fy) > 0.0)
+                    P2(aux, maux, I_TAUDIR_X, i, j) = -(dx * FxL / sqrt(FxL * FxL + Fy * Fy));
+                else
+                    P2(aux, maux, I_TAUDIR_X, i, j) = dx * 1.0;
+                if ((Fx * Fx + FyL * FyL) > 0.0)
+                    P2(aux, maux, I_TAUDIR_Y, i, j) = -(dy * FyL / sqrt(Fx * Fx + FyL * FyL));
+                else
+                    P2(aux, maux, I_TAUDIR_Y, i, j) = dy * 1.0;
+                double tx = P2(aux, maux, I_TAUDIR_X, i, j), ty = P2(aux, maux, I_TAUDIR_Y, i, j);
+                if ((ty * ty + tx * tx) > 0.0) P2(aux, maux, I_FSPHI, i, j) = 1.0;
+                else P2(aux, maux, I_FSPHI, i, j) = 0.0;
+            }
+        }
+}
+
+/* b4step2: b4step2.f90:7-113 (bed_normal = 0, aux already finalised) */
+void oracle_b4step2(oracle_params* P, int mx, int my, int mbc, int maux, double dx, double dy,
+                    double* q, double* aux)
+{
+    const double gz = P->grav;
+    for (int j = 1 - mbc; j <= my + mbc; j++)
+        for (int i = 1 - mbc; i <= mx + mbc; i++) {
+            double u, v, m, chi, rho;
+            qfix(P, &P2(q, MEQN, I_H, i, j), &P2(q, MEQN, I_HU, i, j), &P2(q, MEQN, I_HV, i, j),
+                 &P2(q, MEQN, I_HM, i, j), &P2(q, MEQN, I_PB, i, j), &P2(q, MEQN, I_HCHI, i, j),
+                 &u, &v, &m, &chi, &rho, gz);
+            P2(q, MEQN, I_BDIF, i, j) = dmax(P2(q, MEQN, I_BDIF, i, j), 0.0);
+        }
+    for (int j = 1 - mbc; j <= my + mbc; j++)
+        for (int i = 1 - mbc; i <= mx + mbc; i++) {
+            if (P2(q, MEQN, 1, i, j) > 0.0) {
+                double a = P2(q, MEQN, 2, i, j), b = P2(q, MEQN, 3, i, j);
+                double s = sqrt(a * a + b * b) / P2(q, MEQN, 1, i, j);
+                if (s > P->speed_limit) {
+                    double sratio = P->speed_limit / s;
+                    P2(q, MEQN, 2, i, j) = P2(q, MEQN, 2, i, j) * sratio;
+                    P2(q, MEQN, 3, i, j) = P2(q, MEQN, 3, i, j) * sratio;
+                }
+            }
+        }
+    oracle_calc_taudir(P, mx, my, mbc, maux, dx, dy, q, aux);
+}
+
+/* step2: GeoClaw step2.f90 (Clawpack 5; not in the reference tree).
+ * x-sweeps over rows j = 0..my+1, then y-sweeps over columns i = 0..mx+1,
+ * accumulating into fm/fp/gm/gp in that order. */
+void oracle_step2(const oracle_params* P, int mx, int my, int mbc, int maux, const double* qold,
+                  const double* aux, double dx, double dy, double dt, double* cflgrid, double* fm,
+                  double* fp, double* gm, double* gp)
+{
+    const int maxm = mx > my ? mx : my;
+    const int n = maxm + 2 * mbc;
+    double* q1d = (double*)malloc(sizeof(double) * MEQN * n);
+    double* aux1 = (double*)malloc(sizeof(double) * maux * n);
+    double* aux2 = (double*)malloc(sizeof(double) * maux * n);
+    double* aux3 = (double*)malloc(sizeof(double) * maux * n);
+    double* dtdx1d = (double*)malloc(sizeof(double) * n);
+    double* faddm = (double*)malloc(sizeof(double) * MEQN * n);
+    double* faddp = (double*)malloc(sizeof(double) * MEQN * n);
+    double* gaddm = (double*)malloc(sizeof(double) * MEQN * n * 2);
+    double* gaddp = (double*)malloc(sizeof(double) * MEQN * n * 2);
+    const size_t ncell = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);
+    *cflgrid = 0.0;
+    const double dtdx = dt / dx, dtdy = dt / dy;
+    memset(fm, 0, sizeof(double) * MEQN * ncell);
+    memset(fp, 0, sizeof(double) * MEQN * ncell);
+    memset(gm, 0, sizeof(double) * MEQN * ncell);
+    memset(gp, 0, sizeof(double) * MEQN * ncell);
+#define GA(arr, m, i, side) (arr)[((m) - 1) + MEQN * (((i) + mbc - 1) + n * ((side) - 1))]
+
+    for (int j = 0; j <= my + 1; j++) {
+        for (int i = 1 - mbc; i <= mx + mbc; i++) {
+            for (int m = 1; m <= MEQN; m++) S1(q1d, MEQN, m, i) = P2(qold, MEQN, m, i, j);
+            for (int m = 1; m <= maux; m++) {
+                S1(aux1, maux, m, i) = P2(aux, maux, m, i, j - 1);
+                S1(aux2, maux, m, i) = P2(aux, maux, m, i, j);
+                S1(aux3, maux, m, i) = P2(aux, maux, m, i, j + 1);
+            }
+        }
+        for (int k = 0; k < n; k++) dtdx1d[k] = dtdx;
+        double cfl1d;
+        oracle_flux2(P, 1, maxm, mx, mbc, maux, q1d, dtdx1d, aux1, aux2, aux3, faddm, faddp, gaddm,
+                     gaddp, &cfl1d);
+        *cflgrid = dmax(*cflgrid, cfl1d);
+        for (int i = 1; i <= mx + 1; i++)
+            for (int m = 1; m <= MEQN; m++) {
+                P2(fm, MEQN, m, i, j) = P2(fm, MEQN, m, i, j) + S1(faddm, MEQN, m, i);
+                P2(fp, MEQN, m, i, j) = P2(fp, MEQN, m, i, j) + S1(faddp, MEQN, m, i);
+                P2(gm, MEQN, m, i, j) = P2(gm, MEQN, m, i, j) + GA(gaddm, m, i, 1);
+                P2(gp, MEQN, m, i, j) = P2(gp, MEQN, m, i, j) + GA(gaddp, m, i, 1);
+                P2(gm, MEQN, m, i, j + 1) = P2(gm, MEQN, m, i, j + 1) + GA(gaddm, m, i, 2);
+                P2(gp, MEQN, m, i, j + 1) = P2(gp, MEQN, m, i, j + 1) + GA(gaddp, m, i, 2);
+            }
+    }
+
+    for (int i = 0; i <= mx + 1; i++) {
+        for (int j = 1 - mbc; j <= my + mbc; j++) {
+            for (int m = 1; m <= MEQN; m++) S1(q1d, MEQN, m, j) = P2(qold, MEQN, m, i, j);
+            for (int m = 1; m <= maux; m++) {
+                S1(aux1, maux, m, j) = P2(aux, maux, m, i - 1, j);
+                S1(aux2, maux, m, j) = P2(aux, maux, m, i, j);
+                S1(aux3, maux, m, j) = P2(aux, maux, m, i + 1, j);
+            }
+        }
+        for (int k = 0; k < n; k++) dtdx1d[k] = dtdy;
+        double cfl1d;
+        oracle_flux2(P, 2, maxm, my, mbc, maux, q1d, dtdx1d, aux1, aux2, aux3, faddm, faddp, gaddm,
+                     gaddp, &cfl1d);
+        *cflgrid = dmax(*cflgrid, cfl1d);
+        for (int j = 1; j <= my + 1; j++)
+            for (int m = 1; m <= MEQN; m++) {
+                P2(gm, MEQN, m, i, j) = P2(gm, MEQN, m, i, j) + S1(faddm, MEQN, m, j);
+                P2(gp, MEQN, m, i, j) = P2(gp, MEQN, m, i, j) + S1(faddp, MEQN, m, j);
+                P2(fm, MEQN, m, i, j) = P2(fm, MEQN, m, i, j) + GA(gaddm, m, j, 1);
+                P2(fp, MEQN, m, i, j) = P2(fp, MEQN, m, i, j) + GA(gaddp, m, j, 1);
+                P2(fm, MEQN, m, i + 1, j) = P2(fm, MEQN, m, i + 1, j) + GA(gaddm, m, j, 2);
+                P2(fp, MEQN, m, i + 1, j) = P2(fp, MEQN, m, i + 1, j) + GA(gaddp, m, j, 2);
+            }
+    }
+#undef GA
+    free(q1d); free(aux1); free(aux2); free(aux3); free(dtdx1d);
+    free(faddm); free(faddp); free(gaddm); free(gaddp);
+}
+
+/* mp_update_relax_Dclaw4: mp_update.f90:710-789 */
+static void mp_update_relax_Dclaw4(const oracle_params* P, double dt, double* h, double* u,
+                                   double* v, double* m, double* p, double* chi, double rhoh,
+                                   double gz)
+{
+    const double rho_f = P->rho_f, rho_s = P->rho_s, mu = P->mu;
+    double rho, kperm, alphainv, m_eq, tanpsi, tau, hu, hv, hm, hchi;
+    setvars(P, *h, *u, *v, *m, *p, *chi, gz, &rho, &kperm, &alphainv, &m_eq, &tanpsi, &tau);
+    hu = *h * *u;
+    hv = *h * *v;
+    hm = *h * *m;
+    hchi = *h * *chi;
+    double vnorm = sqrt(hu * hu + hv * hv) / *h;
+    *p = *p - dt * 3.0 * vnorm * tanpsi * alphainv / *h;
+    qfix(P, h, &hu, &hv, &hm, p, &hchi, u, v, m, chi, &rho, gz);
+
+    double zeta = 3.0 * alphainv / (*h * 2.0) + (rho - rho_f) * rho_f * gz / (4.0 * rho);
+    double krate = -(zeta * 2.0 * kperm / (*h * dmax(mu, 1e-16)));
+    double p_eq = rho_f * *h * gz;
+    *p = p_eq + (*p - p_eq) * exp(krate * dt);
+
+    double p_exc = *p - rho_f * gz * *h;
+    qfix_cmass(P, h, m, p, &rho, &p_exc, &hu, &hv, &hm, *u, *v, rhoh, gz);
+    double D = -(2.0 * (kperm / (mu * *h)) * p_exc);
+    krate = D * (rho - rho_f) / rhoh;
+    hm = hm * exp(-(dt * D * rho_f / (rhoh)));
+
+    if (P->src2method == 0) {
+        *h = *h + *h * krate * dt;
+        hchi = *h * *chi;
+        hu = hu * exp(dt * krate);
+        hv = hv * exp(dt * krate);
+        qfix(P, h, &hu, &hv, &hm, p, &hchi, u, v, m, chi, &rho, gz);
+    } else if (P->src2method == 1) {
+        *h = (rhoh - hm * (rho_s - rho_f)) / rho_f;
+        hu = *h * *u;
+        hv = *h * *v;
+        hchi = *h * *chi;
+        qfix(P, h, &hu, &hv, &hm, p, &hchi, u, v, m, chi, &rho, gz);
+    }
+}
+
+/* src2: src2.f90:27-306 (bed_normal = curvature = entrainment = 0;
+ * src2method -1, 0, 1) */
+void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, double* q,
+                 const double* aux, double dt)
+{
+    (void)aux; (void)maux;
+    const double drytol = P->dry_tolerance, rho_s = P->rho_s, rho_f = P->rho_f, mu = P->mu;
+    double coeffmanning = P->friction_forcing ? P->manning_coefficient : 0.0;
+    const double gz = P->grav, gx = 0.0;
+    for (int j = 1; j <= my; j++)
+        for (int i = 1; i <= mx; i++) {
+            double h = P2(q, MEQN, I_H, i, j);
+            if (h <= drytol) {
+                for (int m = 1; m <= 6; m++) P2(q, MEQN, m, i, j) = 0.0;
+                continue;
+            }
+            double hu = P2(q, MEQN, I_HU, i, j), hv = P2(q, MEQN, I_HV, i, j);
+            double hm = P2(q, MEQN, I_HM, i, j), p = P2(q, MEQN, I_PB, i, j);
+            double hchi = P2(q, MEQN, I_HCHI, i, j);
+            double rhoh = hm * rho_s + (h - hm) * rho_f;
+            if (hm < 0.0) continue;
+
+            double u, v, m, chi, rho, kperm, alphainv, m_eq, tanpsi, tau;
+            qfix(P, &h, &hu, &hv, &hm, &p, &hchi, &u, &v, &m, &chi, &rho, gz);
+            setvars(P, h, u, v, m, p, chi, gz, &rho, &kperm, &alphainv, &m_eq, &tanpsi, &tau);
+
+            double hvnorm0 = sqrt(hu * hu + hv * hv);
+            double vnorm = hvnorm0 / h;
+            if (hvnorm0 > 0.0) {
+                vnorm = dmax(0.0, vnorm - dt * tau / rhoh);
+                vnorm = vnorm * exp(-((1.0 - m) * 2.0 * mu * dt / (h * rhoh)));
+                double hvnorm = h * vnorm;
+                hu = hvnorm * hu / hvnorm0 + gx * h * dt;
+                hv = hvnorm * hv / hvnorm0;
+                u = hu / h;
+                v = hv / h;
+                vnorm = sqrt(u * u + v * v);
+            }
+            setvars(P, h, u, v, m, p, chi, gz, &rho, &kperm, &alphainv, &m_eq, &tanpsi, &tau);
+
+            if (P->src2method == -1) {
+                hu = h * u; hv = h * v; hm = h * m; hchi = h * chi;
+            } else {
+                mp_update_relax_Dclaw4(P, dt, &h, &u, &v, &m, &p, &chi, rhoh, gz);
+                hu = h * u; hv = h * v; hm = h * m; hchi = h * chi;
+            }
+
+            if (h <= drytol) {
+                for (int mm = 1; mm <= 6; mm++) P2(q, MEQN, mm, i, j) = 0.0;
+                continue;
+            }
+            setvars(P, h, u, v, m, p, chi, gz, &rho, &kperm, &alphainv, &m_eq, &tanpsi, &tau);
+
+            if (P->friction_forcing && coeffmanning > 0.0) {
+                if (h <= P->friction_depth) {
+                    if (P->dd_manning)
+                        coeffmanning = P->manning_max + (P->manning_coefficient - P->manning_max)
+                                       * 0.5 * (1.0 + tanh(300.0 * (h - 0.05)));
+                    double beta = 1.0;
+                    double gamma = beta * sqrt(hu * hu + hv * hv) * (gz * (coeffmanning * coeffmanning))
+                                   / pow(h, 7.0 / 3.0);
+                    double dgamma = 1.0 + dt * gamma;
+                    hu = hu / dgamma;
+                    hv = hv / dgamma;
+                    qfix(P, &h, &hu, &hv, &hm, &p, &hchi, &u, &v, &m, &chi, &rho, gz);
+                }
+            }
+            P2(q, MEQN, I_H, i, j) = h;
+            P2(q, MEQN, I_HU, i, j) = hu;
+            P2(q, MEQN, I_HV, i, j) = hv;
+            P2(q, MEQN, I_HM, i, j) = hm;
+            P2(q, MEQN, I_PB, i, j) = p;
+            P2(q, MEQN, I_HCHI, i, j) = hchi;
+        }
+}
+
+/* stepgrid: GeoClaw stepgrid.f (Clawpack 5; not in the reference tree):
+ * b4step2, step2, conservative update of interior cells, Godunov-split src2. */
+double oracle_stepgrid(oracle_params* P, int mx, int my, int mbc, int maux, double dx,
+                       double dy, double dt, double* q, double* aux, double* work)
+{
+    const size_t ncell = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);
+    double* fm = work;
+    double* fp = work + MEQN * ncell;
+    double* gm = work + 2 * MEQN * ncell;
+    double* gp = work + 3 * MEQN * ncell;
+    double cflgrid;
+    oracle_b4step2(P, mx, my, mbc, maux, dx, dy, q, aux);
+    oracle_step2(P, mx, my, mbc, maux, q, aux, dx, dy, dt, &cflgrid, fm, fp, gm, gp);
+    const double dtdx = dt / dx, dtdy = dt / dy;
+    for (int j = 1; j <= my; j++)
+        for (int i = 1; i <= mx; i++)
+            for (int m = 1; m <= MEQN; m++)
+                P2(q, MEQN, m, i, j) = P2(q, MEQN, m, i, j)
+                    - dtdx * (P2(fm, MEQN, m, i + 1, j) - P2(fp, MEQN, m, i, j))
+                    - dtdy * (P2(gm, MEQN, m, i, j + 1) - P2(gp, MEQN, m, i, j));
+    oracle_src2(P, mx, my, mbc, maux, q, aux, dt);
+    return cflgrid;
+}
+
+/* Physical boundary conditions (GeoClaw bc2amr; not in the reference tree):
+ * extrap copies the nearest interior cell, wall mirrors with the normal
+ * momentum negated.  x sides first for interior rows, then y sides across
+ * the full width (corner values are the same in either order for these
+ * two rules). */
+void oracle_bc_fill(int mx, int my, int mbc, int meqn, double* q, const int* bc)
+{
+#define QQ(m, i, j) q[((m) - 1) + (size_t)meqn * ((size_t)((i) + mbc - 1) + (size_t)((j) + mbc - 1) * (size_t)(mx + 2 * mbc))]
+    for (int j = 1; j <= my; j++)
+        for (int k = 1; k <= mbc; k++)
+            for (int m = 1; m <= meqn; m++) {
+                if (bc[0] == 3) QQ(m, 1 - k, j) = (m == 2 ? -1.0 : 1.0) * QQ(m, k, j);
+                else QQ(m, 1 - k, j) = QQ(m, 1, j);
+                if (bc[1] == 3) QQ(m, mx + k, j) = (m == 2 ? -1.0 : 1.0) * QQ(m, mx + 1 - k, j);
+                else QQ(m, mx + k, j) = QQ(m, mx, j);
+            }
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int k = 1; k <= mbc; k++)
+            for (int m = 1; m <= meqn; m++) {
+                if (bc[2] == 3) QQ(m, i, 1 - k) = (m == 3 ? -1.0 : 1.0) * QQ(m, i, k);
+                else QQ(m, i, 1 - k) = QQ(m, i, 1);
+                if (bc[3] == 3) QQ(m, i, my + k) = (m == 3 ? -1.0 : 1.0) * QQ(m, i, my + 1 - k);
+                else QQ(m, i, my + k) = QQ(m, i, my);
+            }
+#undef QQ
+}
+
+int oracle_run(oracle_params* P, int mx, int my, int mbc, int maux, double dx, double dy,
+               const int* bc, double* q, double* aux, int nsteps, double* dt_inout,
+               double cfl_desired, double dt_max, double* cfl_hist, double* dt_hist,
+               int nthreads)
+{
+    (void)nthreads;
+    const size_t ncell = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);
+    double* work = (double*)malloc(sizeof(double) * 4 * MEQN * ncell);
+    double dt = *dt_inout;
+    int n;
+    for (n = 0; n < nsteps; n++) {
+        oracle_bc_fill(mx, my, mbc, MEQN, q, bc);
+        double cfl = oracle_stepgrid(P, mx, my, mbc, maux, dx, dy, dt, q, aux, work);
+        if (dt_hist) dt_hist[n] = dt;
+        if (cfl_hist) cfl_hist[n] = cfl;
+        double dtnew = (cfl > 0.0) ? dt * cfl_desired / cfl : INFINITY;
+        dt = dmin(dtnew, dt_max);
+    }
+    *dt_inout = dt;
+    free(work);
+    return n;
+}
+
+/* ---- point-routine entry points for tests ---- */
+void oracle_qfix(const oracle_params* P, double* q6, double gz, double* prim5)
+{
+    qfix(P, &q6[0], &q6[1], &q6[2], &q6[3], &q6[4], &q6[5], &prim5[0], &prim5[1], &prim5[2],
+         &prim5[3], &prim5[4], gz);
+}
+
+void oracle_setvars(const oracle_params* P, const double* in7, double* out6)
+{
+    setvars(P, in7[0], in7[1], in7[2], in7[3], in7[4], in7[5], in7[6], &out6[0], &out6[1],
+            &out6[2], &out6[3], &out6[4], &out6[5]);
+}
+
+void oracle_riemanntype(double hL, double hR, double uL, double uR, int maxiter, double drytol,
+                        double g, double* out3, int* rare2)
+{
+    int r1, r2;
+    riemanntype(hL, hR, uL, uR, &out3[0], &out3[1], &out3[2], &r1, &r2, maxiter, drytol, g);
+    rare2[0] = r1;
+    rare2[1] = r2;
+}
+
+/* in30: hL hR huL huR hvL hvR hmL hmR pL pR bL bR uL uR vL vR mL mR thetaL
+ * thetaR taudirL taudirR chiL chiR fsL fsR (26 used) */
+void oracle_riemann_solver(const oracle_params* P, int ixy, const double* x, int wallprob,
+                           double* sw3, double* fw18)
+{
+    riemann_dig2_aug_sswave_ez(P, ixy, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8],
+                               x[9], x[10], x[11], x[12], x[13], x[14], x[15], x[16], x[17],
+                               x[18], x[19], sw3, fw18, wallprob, x[20], x[21], x[22], x[23],
+                               x[24], x[25]);
+}

@@ -1,0 +1,96 @@
+/*
+ * dclaw_oracle.h -- CPU restatement of D-Claw's per-patch finite-volume step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
+ * against; it is never linked into, loaded by, or called from the product
+ * library (dclaw_amd/).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.
+ *
+ * Arrays follow the reference layout exactly: Fortran column-major, component
+ * fastest, q(meqn=7, 1-mbc:mx+mbc, 1-mbc:my+mbc), aux(maux, same).
+ * q components (digclaw_module.f90:96-102): 1 h, 2 hu, 3 hv, 4 hm, 5 pb,
+ * 6 hchi, 7 bdif.  aux components for coordinate_system=1
+ * (digclaw_module.f90:83-110): 1 b, 2 phi, 3 theta, 4 fsphi, 5 taudir_x,
+ * 6 taudir_y.
+ */
+#ifndef DCLAW_ORACLE_H
+#define DCLAW_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Module globals of geoclaw_module / digclaw_module / amr_module that the
+ * reference reads on the step path.  Field order is mirrored by the Python
+ * ctypes Structure in oracle/oracle.py. */
+typedef struct oracle_params {
+    double grav;                /* geoclaw_module grav */
+    double dry_tolerance;       /* geoclaw_module dry_tolerance */
+    double speed_limit;         /* geoclaw_module speed_limit (b4step2.f90:78) */
+    double rho_s, rho_f, m_crit, mref, kref;
+    double phi;                 /* radians (digclaw_module.f90:136) */
+    double delta, mu, alpha_c, c1, sigma_0;
+    double kappa;               /* 1 (digclaw_module.f90:113) */
+    double beta_seg, delta_kr_order;
+    double manning_coefficient; /* manning_coefficient(1) (src2.f90:82) */
+    double friction_depth, manning_max;
+    int src2method, alphamethod, rarecorrectortest, segregation;
+    int friction_forcing, dd_manning;
+    int order;                  /* method(2): 1 Godunov, 2 with limited corrections */
+    int transverse;             /* method(3): 0, 1, 2 */
+    int mthlim[5];              /* limiter per wave family */
+    int mwaves;                 /* 5 */
+} oracle_params;
+
+/* ---- point routines (for unit / known-answer tests) ---- */
+void oracle_qfix(const oracle_params* p, double* q6, double gz, double* prim5);
+void oracle_setvars(const oracle_params* p, const double* in7, double* out6);
+void oracle_riemanntype(double hL, double hR, double uL, double uR, int maxiter,
+                        double drytol, double g, double* out3, int* rare2);
+void oracle_riemann_solver(const oracle_params* p, int ixy, const double* in30,
+                           int wallprob, double* sw3, double* fw18);
+
+/* ---- slice routines (1-D arrays with 1-mbc:mx+mbc extents) ---- */
+void oracle_rpn2(const oracle_params* p, int ixy, int mx, int mbc, int maux,
+                 double* q1d, const double* aux2, double* fwave, double* s,
+                 double* amdq, double* apdq);
+void oracle_rpt2(const oracle_params* p, int ixy, int imp, int mx, int mbc, int maux,
+                 const double* q1d, const double* aux1, const double* aux2,
+                 const double* aux3, const double* asdq, double* bmasdq, double* bpasdq);
+void oracle_limiter(const oracle_params* p, int mx, int mbc, double* fwave, const double* s);
+void oracle_flux2(const oracle_params* p, int ixy, int maxm, int mx, int mbc, int maux,
+                  double* q1d, double* dtdx1d, const double* aux1, const double* aux2,
+                  const double* aux3, double* faddm, double* faddp, double* gaddm,
+                  double* gaddp, double* cfl1d);
+
+/* ---- patch routines ---- */
+void oracle_calc_taudir(oracle_params* p, int mx, int my, int mbc, int maux,
+                        double dx, double dy, const double* q, double* aux);
+void oracle_b4step2(oracle_params* p, int mx, int my, int mbc, int maux,
+                    double dx, double dy, double* q, double* aux);
+void oracle_step2(const oracle_params* p, int mx, int my, int mbc, int maux,
+                  const double* qold, const double* aux, double dx, double dy, double dt,
+                  double* cflgrid, double* fm, double* fp, double* gm, double* gp);
+void oracle_src2(const oracle_params* p, int mx, int my, int mbc, int maux,
+                 double* q, const double* aux, double dt);
+/* One stepgrid: b4step2, step2, conservative update, src2.  `work` must hold
+ * 4*7*(mx+2mbc)*(my+2mbc) doubles (fm, fp, gm, gp).  Returns cflgrid. */
+double oracle_stepgrid(oracle_params* p, int mx, int my, int mbc, int maux,
+                       double dx, double dy, double dt, double* q, double* aux,
+                       double* work);
+/* Physical boundary fill, bc codes as Clawpack: 1 extrap, 3 wall.
+ * bc[0..3] = xlower, xupper, ylower, yupper. */
+void oracle_bc_fill(int mx, int my, int mbc, int meqn, double* q, const int* bc);
+/* Fixed-patch time loop: bc fill then stepgrid, nsteps times, with
+ * dt_{n+1} = min(dt_n*cfl_desired/cfl_n, dt_max).  dt_inout holds dt_0 on
+ * entry and the next dt on return; cfl_hist (nsteps) and dt_hist (nsteps)
+ * may be NULL.  Returns the number of steps taken. */
+int oracle_run(oracle_params* p, int mx, int my, int mbc, int maux, double dx,
+               double dy, const int* bc, double* q, double* aux, int nsteps,
+               double* dt_inout, double cfl_desired, double dt_max,
+               double* cfl_hist, double* dt_hist, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

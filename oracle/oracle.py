@@ -1,0 +1,278 @@
+"""ctypes binding of the CPU oracle (oracle/dclaw_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg import this
+module; the product package ``dclaw_amd`` never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libdclaw_oracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class OracleParams(ctypes.Structure):
+    """Mirror of ``oracle_params`` in dclaw_oracle.h (same field order)."""
+
+    _fields_ = [
+        ("grav", ctypes.c_double),
+        ("dry_tolerance", ctypes.c_double),
+        ("speed_limit", ctypes.c_double),
+        ("rho_s", ctypes.c_double),
+        ("rho_f", ctypes.c_double),
+        ("m_crit", ctypes.c_double),
+        ("mref", ctypes.c_double),
+        ("kref", ctypes.c_double),
+        ("phi", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("mu", ctypes.c_double),
+        ("alpha_c", ctypes.c_double),
+        ("c1", ctypes.c_double),
+        ("sigma_0", ctypes.c_double),
+        ("kappa", ctypes.c_double),
+        ("beta_seg", ctypes.c_double),
+        ("delta_kr_order", ctypes.c_double),
+        ("manning_coefficient", ctypes.c_double),
+        ("friction_depth", ctypes.c_double),
+        ("manning_max", ctypes.c_double),
+        ("src2method", ctypes.c_int),
+        ("alphamethod", ctypes.c_int),
+        ("rarecorrectortest", ctypes.c_int),
+        ("segregation", ctypes.c_int),
+        ("friction_forcing", ctypes.c_int),
+        ("dd_manning", ctypes.c_int),
+        ("order", ctypes.c_int),
+        ("transverse", ctypes.c_int),
+        ("mthlim", ctypes.c_int * 5),
+        ("mwaves", ctypes.c_int),
+    ]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(OracleParams)
+        L.oracle_qfix.argtypes = [P, _dp, ctypes.c_double, _dp]
+        L.oracle_setvars.argtypes = [P, _dp, _dp]
+        L.oracle_riemanntype.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_double,
+                                                                   ctypes.c_double, _dp, _ip]
+        L.oracle_riemann_solver.argtypes = [P, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.oracle_rpn2.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_rpt2.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_limiter.argtypes = [P, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.oracle_calc_taudir.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.oracle_b4step2.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.oracle_step2.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   _dp, _dp, _dp, _dp, _dp]
+        L.oracle_src2.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  _dp, _dp, ctypes.c_double]
+        L.oracle_stepgrid.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                                      _dp, _dp]
+        L.oracle_stepgrid.restype = ctypes.c_double
+        L.oracle_bc_fill.argtypes = [ctypes.c_int] * 4 + [_dp, _ip]
+        L.oracle_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, _ip, _dp, _dp, ctypes.c_int,
+                                 _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
+        L.oracle_run.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def params_from(dp) -> OracleParams:
+    """Build the oracle's parameter struct from a ``DigParams``-like object
+    (any object with the same attribute names)."""
+    o = OracleParams()
+    for name, ctype in OracleParams._fields_:
+        if name == "mthlim":
+            arr = (ctypes.c_int * 5)(*[int(x) for x in dp.mthlim])
+            o.mthlim = arr
+        elif ctype is ctypes.c_int:
+            setattr(o, name, int(getattr(dp, name)))
+        else:
+            setattr(o, name, float(getattr(dp, name)))
+    return o
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags["F_CONTIGUOUS"] or a.flags["C_CONTIGUOUS"])
+    return a.ctypes.data_as(_dp)
+
+
+# ---------------------------------------------------------------- point API
+def qfix(params, q6, gz=None):
+    q = np.ascontiguousarray(q6, dtype=np.float64).copy()
+    prim = np.zeros(5)
+    lib().oracle_qfix(ctypes.byref(params), _ptr(q), params.grav if gz is None else gz, _ptr(prim))
+    return q, prim
+
+
+def setvars(params, h, u, v, m, p, chi, gz):
+    inp = np.array([h, u, v, m, p, chi, gz], dtype=np.float64)
+    out = np.zeros(6)
+    lib().oracle_setvars(ctypes.byref(params), _ptr(inp), _ptr(out))
+    return dict(zip(["rho", "kperm", "alphainv", "m_eq", "tanpsi", "tau"], out))
+
+
+def riemanntype(hL, hR, uL, uR, maxiter=1, drytol=1e-3, g=9.81):
+    out = np.zeros(3)
+    rare = (ctypes.c_int * 2)()
+    lib().oracle_riemanntype(hL, hR, uL, uR, maxiter, drytol, g, _ptr(out),
+                             ctypes.cast(rare, _ip))
+    return out[0], out[1], out[2], bool(rare[0]), bool(rare[1])
+
+
+SOLVER_INPUTS = ["hL", "hR", "huL", "huR", "hvL", "hvR", "hmL", "hmR", "pL", "pR", "bL", "bR",
+                 "uL", "uR", "vL", "vR", "mL", "mR", "thetaL", "thetaR", "taudirL", "taudirR",
+                 "chiL", "chiR", "fsL", "fsR"]
+
+
+def riemann_solver(params, ixy=1, wallprob=False, **kw):
+    x = np.zeros(30)
+    for k, name in enumerate(SOLVER_INPUTS):
+        x[k] = kw.get(name, 0.0)
+    sw = np.zeros(3)
+    fw = np.zeros(18)
+    lib().oracle_riemann_solver(ctypes.byref(params), ixy, _ptr(x), int(wallprob), _ptr(sw),
+                                _ptr(fw))
+    return sw, fw.reshape(3, 6).T.copy()   # fw[m, mw] with m 0..5, mw 0..2
+
+
+# ---------------------------------------------------------------- slice API
+def rpn2(params, ixy, q1d, aux2, mbc=2):
+    """q1d: (7, mx+2mbc) Fortran-ordered slice; aux2: (maux, mx+2mbc).
+    Returns (q1d_after, fwave (7,mwaves,n), s (mwaves,n), amdq, apdq)."""
+    q = np.array(q1d, dtype=np.float64, order="F", copy=True)
+    aux = np.asfortranarray(aux2, dtype=np.float64)
+    n = q.shape[1]
+    mx = n - 2 * mbc
+    mw = params.mwaves
+    fwave = np.zeros((7, mw, n), order="F")
+    s = np.zeros((mw, n), order="F")
+    amdq = np.zeros((7, n), order="F")
+    apdq = np.zeros((7, n), order="F")
+    lib().oracle_rpn2(ctypes.byref(params), ixy, mx, mbc, aux.shape[0], _ptr(q), _ptr(aux),
+                      _ptr(fwave), _ptr(s), _ptr(amdq), _ptr(apdq))
+    return q, fwave, s, amdq, apdq
+
+
+def rpt2(params, ixy, imp, q1d, aux1, aux2, aux3, asdq, mbc=2):
+    q = np.asfortranarray(q1d, dtype=np.float64)
+    n = q.shape[1]
+    mx = n - 2 * mbc
+    a1, a2, a3 = (np.asfortranarray(a, dtype=np.float64) for a in (aux1, aux2, aux3))
+    asdq = np.asfortranarray(asdq, dtype=np.float64)
+    bm = np.zeros((7, n), order="F")
+    bp = np.zeros((7, n), order="F")
+    lib().oracle_rpt2(ctypes.byref(params), ixy, imp, mx, mbc, a2.shape[0], _ptr(q), _ptr(a1),
+                      _ptr(a2), _ptr(a3), _ptr(asdq), _ptr(bm), _ptr(bp))
+    return bm, bp
+
+
+def limiter(params, fwave, s, mbc=2):
+    fw = np.array(fwave, dtype=np.float64, order="F", copy=True)
+    s = np.asfortranarray(s, dtype=np.float64)
+    n = fw.shape[2]
+    lib().oracle_limiter(ctypes.byref(params), n - 2 * mbc, mbc, _ptr(fw), _ptr(s))
+    return fw
+
+
+# ---------------------------------------------------------------- patch API
+def _dims(q, mbc):
+    return q.shape[1] - 2 * mbc, q.shape[2] - 2 * mbc
+
+
+def calc_taudir(params, q, aux, dx, dy, mbc=2):
+    aux = np.array(aux, dtype=np.float64, order="F", copy=True)
+    q = np.asfortranarray(q, dtype=np.float64)
+    mx, my = _dims(q, mbc)
+    lib().oracle_calc_taudir(ctypes.byref(params), mx, my, mbc, aux.shape[0], dx, dy, _ptr(q),
+                             _ptr(aux))
+    return aux
+
+
+def b4step2(params, q, aux, dx, dy, mbc=2):
+    q = np.array(q, dtype=np.float64, order="F", copy=True)
+    aux = np.array(aux, dtype=np.float64, order="F", copy=True)
+    mx, my = _dims(q, mbc)
+    lib().oracle_b4step2(ctypes.byref(params), mx, my, mbc, aux.shape[0], dx, dy, _ptr(q),
+                         _ptr(aux))
+    return q, aux
+
+
+def step2(params, q, aux, dx, dy, dt, mbc=2):
+    q = np.asfortranarray(q, dtype=np.float64)
+    aux = np.asfortranarray(aux, dtype=np.float64)
+    mx, my = _dims(q, mbc)
+    fm, fp, gm, gp = (np.zeros_like(q, order="F") for _ in range(4))
+    cfl = ctypes.c_double(0.0)
+    lib().oracle_step2(ctypes.byref(params), mx, my, mbc, aux.shape[0], _ptr(q), _ptr(aux), dx,
+                       dy, dt, ctypes.byref(cfl), _ptr(fm), _ptr(fp), _ptr(gm), _ptr(gp))
+    return cfl.value, fm, fp, gm, gp
+
+
+def src2(params, q, aux, dt, mbc=2):
+    q = np.array(q, dtype=np.float64, order="F", copy=True)
+    aux = np.asfortranarray(aux, dtype=np.float64)
+    mx, my = _dims(q, mbc)
+    lib().oracle_src2(ctypes.byref(params), mx, my, mbc, aux.shape[0], _ptr(q), _ptr(aux), dt)
+    return q
+
+
+def stepgrid(params, q, aux, dx, dy, dt, mbc=2):
+    """One step in place semantics on copies: returns (q, aux, cflgrid)."""
+    q = np.array(q, dtype=np.float64, order="F", copy=True)
+    aux = np.array(aux, dtype=np.float64, order="F", copy=True)
+    mx, my = _dims(q, mbc)
+    work = np.zeros(4 * q.size)
+    cfl = lib().oracle_stepgrid(ctypes.byref(params), mx, my, mbc, aux.shape[0], dx, dy, dt,
+                                _ptr(q), _ptr(aux), _ptr(work))
+    return q, aux, cfl
+
+
+def bc_fill(q, bc, mbc=2):
+    q = np.array(q, dtype=np.float64, order="F", copy=True)
+    mx, my = _dims(q, mbc)
+    bcs = (ctypes.c_int * 4)(*bc)
+    lib().oracle_bc_fill(mx, my, mbc, q.shape[0], _ptr(q), ctypes.cast(bcs, _ip))
+    return q
+
+
+def run(params, q, aux, dx, dy, nsteps, dt0, cfl_desired, dt_max, bc=(1, 1, 1, 1), mbc=2):
+    """Fixed-patch time loop.  Returns (q, aux, dt_next, cfl_hist, dt_hist)."""
+    q = np.array(q, dtype=np.float64, order="F", copy=True)
+    aux = np.array(aux, dtype=np.float64, order="F", copy=True)
+    mx, my = _dims(q, mbc)
+    bcs = (ctypes.c_int * 4)(*bc)
+    dt = ctypes.c_double(dt0)
+    cfl_hist = np.zeros(max(nsteps, 1))
+    dt_hist = np.zeros(max(nsteps, 1))
+    lib().oracle_run(ctypes.byref(params), mx, my, mbc, aux.shape[0], dx, dy,
+                     ctypes.cast(bcs, _ip), _ptr(q), _ptr(aux), nsteps, ctypes.byref(dt),
+                     cfl_desired, dt_max, _ptr(cfl_hist), _ptr(dt_hist), 1)
+    return q, aux, dt.value, cfl_hist[:nsteps], dt_hist[:nsteps]
