@@ -1,0 +1,87 @@
+"""Loader of the gfx950 library ``libdclaw_hip.so`` (ctypes, C-ABI only).
+
+torch is imported first on purpose: torch bundles its own HIP runtime whose
+SONAME is ``libamdhip64.so.7``, the same as the library's dependency, so the
+library binds to the runtime torch already loaded and device pointers from
+torch tensors are valid for it.  There is no fallback: if the library is
+missing the import fails with instructions to build it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+try:  # noqa: SIM105 - torch must be loaded before the HIP library
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
+from .params import CParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libdclaw_hip.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+class DClawError(RuntimeError):
+    pass
+
+
+class CPatch(ctypes.Structure):
+    """Mirror of ``dclaw_patch`` in include/dclaw_hip.h."""
+
+    _fields_ = [
+        ("q_in", _vp), ("q_out", _vp), ("aux", _vp),
+        ("mx", ctypes.c_int), ("my", ctypes.c_int), ("maux", ctypes.c_int),
+        ("member", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/dclaw_hip.h
+SIGNATURES = {
+    "dclaw_last_error": (ctypes.c_char_p, []),
+    "dclaw_version": (ctypes.c_int, []),
+    "dclaw_strip_width": (ctypes.c_int, []),
+    "dclaw_bc_fill": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip, _vp]),
+    "dclaw_stepgrid_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          _vp, _vp, _vp, _vp]),
+    "dclaw_worklist_size": (ctypes.c_int, [_ip, _ip, ctypes.c_int, ctypes.c_int]),
+    "dclaw_worklist_build": (ctypes.c_int, [_ip, _ip, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int32)]),
+    "dclaw_step_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+                                          _vp, _vp, _vp, _vp]),
+    "dclaw_dt_update": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, _vp]),
+    "dclaw_set_params": (ctypes.c_int, [ctypes.POINTER(CParams)]),
+    "stepgrid_dclaw_": (None, [_dp, _dp, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DClawError(
+                f"{LIB_PATH} not found: build it with `python -m dclaw_amd.build` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().dclaw_last_error()
+        raise DClawError(f"dclaw error {rc}: {msg.decode() if msg else ''}")

@@ -1,0 +1,392 @@
+// dclaw_kernels.hip -- gfx950 kernels and the C-ABI of the D-Claw step.
+//
+// Kernels:
+//   k_bc_fill        ghost-cell fill (GeoClaw bc2amr extrap / wall)
+//   k_stepgrid       fused stepgrid for one patch: one 64-lane wavefront per
+//                    (58-column strip, row segment); see dclaw_strip.h
+//   k_step_batched   the same body over a work list of (patch, strip,
+//                    segment) tiles of many patches in one launch
+//   k_dt_update      dt_{n+1} = min(dt_n*cfl_desired/cfl_n, dt_max) per member
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dclaw_hip.h"
+#include "dclaw_strip.h"
+
+using namespace dclaw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(DCLAW_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+struct GpuLane {
+    __device__ __forceinline__ int id() const { return (int)threadIdx.x; }
+    __device__ __forceinline__ double shfl(double v, int src) const { return __shfl(v, src, 64); }
+};
+
+__device__ __forceinline__ void wave_max_to(double v, double* dst)
+{
+    for (int off = 32; off >= 1; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+    if (threadIdx.x == 0 && v > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long*>(dst),
+                  (unsigned long long)__double_as_longlong(v));
+}
+
+constexpr int DEFAULT_SEG_ROWS = 32;
+
+__global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in,
+                                                 double* __restrict__ q_out,
+                                                 const double* __restrict__ aux,
+                                                 double* aux_out, int maux, int mx, int my,
+                                                 double dx, double dy,
+                                                 const double* __restrict__ dt_dev,
+                                                 const dclaw_params* __restrict__ params,
+                                                 double* cfl_dev, int seg_rows)
+{
+    const dclaw_params P = *params;
+    StripArgs A;
+    A.q_in = q_in;
+    A.q_out = q_out;
+    A.aux = aux;
+    A.aux_out = aux_out;
+    A.maux = maux;
+    A.mx = mx;
+    A.my = my;
+    A.dx = dx;
+    A.dy = dy;
+    A.dt = *dt_dev;
+    A.i0 = 1 + (int)blockIdx.x * STRIP_W;
+    A.jA = 1 + (int)blockIdx.y * seg_rows;
+    A.jB = min(my, A.jA + seg_rows - 1);
+    GpuLane L;
+    const double cfl = strip_step(L, P, A);
+    wave_max_to(cfl, cfl_dev);
+}
+
+__global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restrict__ patches,
+                                                     const int32_t* __restrict__ tiles,
+                                                     int seg_rows,
+                                                     const double* __restrict__ dt_dev,
+                                                     const dclaw_params* __restrict__ params,
+                                                     double* cfl_dev)
+{
+    const int t = (int)blockIdx.x;
+    const int pidx = tiles[4 * t + 0];
+    const dclaw_patch pt = patches[pidx];
+    const dclaw_params P = params[pt.member];
+    StripArgs A;
+    A.q_in = pt.q_in;
+    A.q_out = pt.q_out;
+    A.aux = pt.aux;
+    A.aux_out = nullptr;
+    A.maux = pt.maux;
+    A.mx = pt.mx;
+    A.my = pt.my;
+    A.dx = pt.dx;
+    A.dy = pt.dy;
+    A.dt = dt_dev[pt.member];
+    A.i0 = 1 + tiles[4 * t + 1] * STRIP_W;
+    A.jA = 1 + tiles[4 * t + 2] * seg_rows;
+    A.jB = min(pt.my, A.jA + seg_rows - 1);
+    GpuLane L;
+    const double cfl = strip_step(L, P, A);
+    wave_max_to(cfl, cfl_dev + pt.member);
+}
+
+// Ghost fill: each ghost cell copies its source cell directly (extrap: nearest
+// interior cell; wall: mirror cell with the normal momentum negated).  The
+// direct index map equals bc2amr's x-then-y fill for these two rules.
+__global__ void k_bc_fill(double* q, int mx, int my, int bxl, int bxh, int byl, int byh)
+{
+    const int nx = mx + 4, ny = my + 4;
+    const long n = (long)nx * ny;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int ii = (int)(t % nx) - 1;   // Fortran i in [-1, mx+2]
+        const int jj = (int)(t / nx) - 1;
+        if (ii >= 1 && ii <= mx && jj >= 1 && jj <= my) continue;
+        int si = ii, sj = jj;
+        double fu = 1.0, fv = 1.0;
+        if (ii < 1) {
+            if (bxl == DCLAW_BC_WALL) { si = 1 - ii; fu = -1.0; } else si = 1;
+        } else if (ii > mx) {
+            if (bxh == DCLAW_BC_WALL) { si = 2 * mx + 1 - ii; fu = -1.0; } else si = mx;
+        }
+        if (jj < 1) {
+            if (byl == DCLAW_BC_WALL) { sj = 1 - jj; fv = -1.0; } else sj = 1;
+        } else if (jj > my) {
+            if (byh == DCLAW_BC_WALL) { sj = 2 * my + 1 - jj; fv = -1.0; } else sj = my;
+        }
+        const size_t src = qidx(0, si, sj, mx), dst = qidx(0, ii, jj, mx);
+        for (int m = 0; m < 7; m++) {
+            double val = q[src + m];
+            if (m == 1) val = fu * val;
+            if (m == 2) val = fv * val;
+            q[dst + m] = val;
+        }
+    }
+}
+
+__global__ void k_dt_update(double* dt, double* cfl, int n, double cfl_desired, double dt_max)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double c = cfl[k];
+    double dtn;
+    if (c > 0.0) dtn = dt[k] * cfl_desired / c;
+    else dtn = dt_max;
+    dt[k] = dmin(dtn, dt_max);
+    cfl[k] = 0.0;
+}
+
+int check_patch(int mx, int my, int maux)
+{
+    if (mx < 1 || my < 1) return fail(DCLAW_EINVAL, "mx and my must be >= 1");
+    if (maux < 1) return fail(DCLAW_EINVAL, "maux must be >= 1");
+    if ((double)(mx + 4) * (double)(my + 4) * 7.0 > 9.0e18)
+        return fail(DCLAW_EINVAL, "patch too large");
+    return DCLAW_OK;
+}
+
+// Host drop-in state (library-owned device buffers, grown on demand).
+struct HostCtx {
+    std::mutex mu;
+    dclaw_params params;
+    bool have_params = false;
+    dclaw_params* d_params = nullptr;
+    double* d_scalars = nullptr;   // [0] dt, [1] cfl
+    double* d_qa = nullptr;
+    double* d_qb = nullptr;
+    double* d_aux = nullptr;
+    size_t cap_q = 0, cap_aux = 0;
+};
+HostCtx g_host;
+
+int grow(double** p, size_t* cap, size_t need)
+{
+    if (*cap >= need) return DCLAW_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    HIP_TRY(hipMalloc(p, need * sizeof(double)));
+    *cap = need;
+    return DCLAW_OK;
+}
+
+int validate_params(const dclaw_params& P)
+{
+    if (P.mwaves != 5) return fail(DCLAW_EINVAL, "mwaves must be 5 (rpn2_dclaw.f:276-315)");
+    if (P.order != 1 && P.order != 2) return fail(DCLAW_EINVAL, "order must be 1 or 2");
+    if (P.transverse < 0 || P.transverse > 2) return fail(DCLAW_EINVAL, "transverse must be 0..2");
+    if (P.src2method < -1 || P.src2method > 1)
+        return fail(DCLAW_EINVAL, "src2method must be -1, 0 or 1 on this path");
+    if (!(P.dry_tolerance >= 0.0) || !(P.grav > 0.0))
+        return fail(DCLAW_EINVAL, "grav must be > 0 and dry_tolerance >= 0");
+    return DCLAW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dclaw_last_error(void) { return g_err.c_str(); }
+
+int dclaw_version(void) { return 1; }
+
+int dclaw_strip_width(void) { return STRIP_W; }
+
+int dclaw_bc_fill(double* q, int mx, int my, const int* bc, void* stream)
+{
+    int rc = check_patch(mx, my, 1);
+    if (rc) return rc;
+    if (!q || !bc) return fail(DCLAW_EINVAL, "null pointer");
+    for (int k = 0; k < 4; k++)
+        if (bc[k] != DCLAW_BC_EXTRAP && bc[k] != DCLAW_BC_WALL)
+            return fail(DCLAW_EINVAL, "bc codes must be 1 (extrap) or 3 (wall)");
+    const long n = (long)(mx + 4) * (my + 4);
+    const int blocks = (int)std::min<long>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_bc_fill, dim3(blocks), dim3(256), 0, (hipStream_t)stream, q, mx, my,
+                       bc[0], bc[1], bc[2], bc[3]);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
+                           double* aux_out, int maux, int mx, int my, double dx, double dy,
+                           const double* dt_dev, const dclaw_params* params_dev,
+                           double* cfl_dev, hipStream_t stream)
+{
+    int rc = check_patch(mx, my, maux);
+    if (rc) return rc;
+    if (!q_in || !q_out || !aux || !dt_dev || !params_dev || !cfl_dev)
+        return fail(DCLAW_EINVAL, "null pointer");
+    if (q_in == q_out) return fail(DCLAW_EINVAL, "q_out must not alias q_in");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int nseg = (my + DEFAULT_SEG_ROWS - 1) / DEFAULT_SEG_ROWS;
+    if (nseg > 65535) return fail(DCLAW_EINVAL, "my too large for one launch");
+    hipLaunchKernelGGL(k_stepgrid, dim3(nstrips, nseg), dim3(64), 0, stream, q_in, q_out, aux,
+                       aux_out, maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev,
+                       DEFAULT_SEG_ROWS);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_stepgrid_dev(const double* q_in, double* q_out, const double* aux, int maux, int mx,
+                       int my, double dx, double dy, const double* dt_dev,
+                       const dclaw_params* params_dev, double* cfl_dev, void* stream)
+{
+    return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
+                           cfl_dev, (hipStream_t)stream);
+}
+
+int dclaw_worklist_size(const int* mx, const int* my, int npatch, int seg_rows)
+{
+    if (!mx || !my || npatch < 0 || seg_rows < 1) return fail(DCLAW_EINVAL, "bad work-list args");
+    long n = 0;
+    for (int p = 0; p < npatch; p++) {
+        if (mx[p] < 1 || my[p] < 1) return fail(DCLAW_EINVAL, "mx and my must be >= 1");
+        n += (long)((mx[p] + STRIP_W - 1) / STRIP_W) * ((my[p] + seg_rows - 1) / seg_rows);
+    }
+    if (n > 0x7fffffff) return fail(DCLAW_EINVAL, "work list too long");
+    return (int)n;
+}
+
+int dclaw_worklist_build(const int* mx, const int* my, int npatch, int seg_rows,
+                         int32_t* tiles)
+{
+    const int n = dclaw_worklist_size(mx, my, npatch, seg_rows);
+    if (n < 0) return n;
+    if (!tiles) return fail(DCLAW_EINVAL, "null pointer");
+    // Largest patches first so the long tiles start early; within a patch,
+    // segments of one strip are consecutive.
+    std::vector<int> order(npatch);
+    for (int p = 0; p < npatch; p++) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (long)mx[a] * my[a] > (long)mx[b] * my[b];
+    });
+    int t = 0;
+    for (int p : order) {
+        const int ns = (mx[p] + STRIP_W - 1) / STRIP_W;
+        const int ng = (my[p] + seg_rows - 1) / seg_rows;
+        for (int s = 0; s < ns; s++)
+            for (int g = 0; g < ng; g++) {
+                tiles[4 * t + 0] = p;
+                tiles[4 * t + 1] = s;
+                tiles[4 * t + 2] = g;
+                tiles[4 * t + 3] = 0;
+                t++;
+            }
+    }
+    return n;
+}
+
+int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
+                       int ntiles, int seg_rows, const double* dt_dev,
+                       const dclaw_params* params_dev, double* cfl_dev, void* stream)
+{
+    if (!patches_dev || !tiles_dev || !dt_dev || !params_dev || !cfl_dev)
+        return fail(DCLAW_EINVAL, "null pointer");
+    if (npatch < 1 || ntiles < 1 || seg_rows < 1) return fail(DCLAW_EINVAL, "empty batch");
+    hipLaunchKernelGGL(k_step_batched, dim3(ntiles), dim3(64), 0, (hipStream_t)stream,
+                       patches_dev, tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_dt_update(double* dt_dev, double* cfl_dev, int n, double cfl_desired, double dt_max,
+                    void* stream)
+{
+    if (!dt_dev || !cfl_dev || n < 1) return fail(DCLAW_EINVAL, "bad dt_update args");
+    hipLaunchKernelGGL(k_dt_update, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       dt_dev, cfl_dev, n, cfl_desired, dt_max);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_set_params(const dclaw_params* p)
+{
+    if (!p) return fail(DCLAW_EINVAL, "null params");
+    int rc = validate_params(*p);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_host.mu);
+    if (!g_host.d_params) {
+        HIP_TRY(hipMalloc(&g_host.d_params, sizeof(dclaw_params)));
+        HIP_TRY(hipMalloc(&g_host.d_scalars, 2 * sizeof(double)));
+    }
+    HIP_TRY(hipMemcpy(g_host.d_params, p, sizeof(dclaw_params), hipMemcpyHostToDevice));
+    g_host.params = *p;
+    g_host.have_params = true;
+    return DCLAW_OK;
+}
+
+static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int maux, double dx,
+                         double dy, double dt, double* cflgrid)
+{
+    if (mbc != DCLAW_MBC) return fail(DCLAW_EINVAL, "mbc must be 2");
+    int rc = check_patch(mx, my, maux);
+    if (rc) return rc;
+    if (maux < 6) return fail(DCLAW_EINVAL, "the host drop-in needs maux >= 6");
+    std::lock_guard<std::mutex> lk(g_host.mu);
+    if (!g_host.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
+    const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
+    const size_t na = (size_t)maux * (mx + 4) * (my + 4);
+    if ((rc = grow(&g_host.d_qa, &g_host.cap_q, nq))) return rc;
+    size_t capb = g_host.cap_q;
+    if (!g_host.d_qb || capb < nq) {
+        if (g_host.d_qb) (void)hipFree(g_host.d_qb);
+        HIP_TRY(hipMalloc(&g_host.d_qb, nq * sizeof(double)));
+    }
+    if ((rc = grow(&g_host.d_aux, &g_host.cap_aux, na))) return rc;
+    double scal[2] = {dt, 0.0};
+    HIP_TRY(hipMemcpy(g_host.d_qa, q, nq * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g_host.d_aux, aux, na * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
+    rc = launch_stepgrid(g_host.d_qa, g_host.d_qb, g_host.d_aux, g_host.d_aux, maux, mx, my, dx,
+                         dy, g_host.d_scalars, g_host.d_params, g_host.d_scalars + 1, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(q, g_host.d_qb, nq * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(aux, g_host.d_aux, na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(scal, g_host.d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    *cflgrid = scal[1];
+    return DCLAW_OK;
+}
+
+void stepgrid_dclaw_(double* q, double* aux, const int* mbc, const int* mx, const int* my,
+                     const int* maux, const double* dx, const double* dy, const double* dt,
+                     double* cflgrid)
+{
+    const int rc = stepgrid_host(q, aux, *mbc, *mx, *my, *maux, *dx, *dy, *dt, cflgrid);
+    if (rc != DCLAW_OK) {
+        // The Fortran caller has no status argument; like the reference's
+        // fatal paths (check4nans, set_dig) this stops the run.
+        std::fprintf(stderr, "stepgrid_dclaw: %s\n", g_err.c_str());
+        std::abort();
+    }
+}
+
+}  // extern "C"

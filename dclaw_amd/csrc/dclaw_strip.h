@@ -1,0 +1,530 @@
+// dclaw_strip.h -- fused stepgrid for one strip segment of one patch.
+//
+// One wavefront of 64 lanes owns a strip of 58 interior columns
+// [i0, i0+57] (lane l holds column c = i0 - 3 + l, so the strip carries a
+// 3-column halo on each side) and marches down the rows of a segment
+// [jA, jB], keeping a rolling window of 4 rows in registers.  At loop index
+// R (the row being loaded) it:
+//   S1  loads row R and applies b4step2's pointwise part (qfix, bdif >= 0,
+//       speed limit: b4step2.f90:56-100), then the qfix chain rpn2 applies
+//       to its slice (F1 = qfix(Q0), F2 = qfix(F1): rpn2_dclaw.f:164-168);
+//   S2  calc_taudir for row R-1 (digclaw_module.f90:500-719);
+//   S3  solves the y-edge between rows R-2 and R-1 (rpn2 with ixy = 2);
+//   S4  runs the whole x-sweep of row R-2 (flux2fw.f with ixy = 1: rpn2,
+//       limiter, second-order correction, transverse rpt2 split);
+//   S5  limits the y-edge between rows R-3 and R-2 and splits it
+//       transversally (flux2fw.f with ixy = 2);
+//   S6  assembles fm, fp, gm, gp for row R-3 in GeoClaw step2's summation
+//       order, updates it (stepgrid) and applies src2 (src2.f90:92-303).
+// Neighbouring columns are reached with lane shuffles; no LDS tile is held.
+// The code is a template on the lane policy so the same body runs on the GPU
+// (one wavefront) and in a host emulator (64 threads in lockstep).
+#pragma once
+
+#include "dclaw_physics.h"
+
+namespace dclaw {
+
+constexpr int STRIP_LANES = 64;
+constexpr int STRIP_HALO = 3;
+constexpr int STRIP_W = STRIP_LANES - 2 * STRIP_HALO;  // 58 interior columns
+
+struct StripArgs {
+    const double* q_in;
+    double* q_out;
+    const double* aux;
+    double* aux_out;       // if non-null: write aux(4..6) as calc_taudir does (maux >= 6)
+    int maux, mx, my;
+    double dx, dy, dt;
+    int i0, jA, jB;        // interior strip start column, segment rows [jA, jB]
+};
+
+// Row state of one column in the rolling window.
+struct RowCell {
+    double h, hu, hv, bdif, b;       // Q0 (b4step2 output); b = aux(1)
+    double hm0, p0, hchi0;           // Q0 remaining components
+    double hm1, p1, hchi1, m1, chi1; // F1 = qfix(Q0)
+    double u, v;                     // hu/h, hv/h (same for F1 and F2)
+    double hm2, p2, hchi2, m2, chi2; // F2 = qfix(F1)
+};
+
+DC_HD void zero_row(RowCell& r)
+{
+    r.h = r.hu = r.hv = r.bdif = r.b = 0.0;
+    r.hm0 = r.p0 = r.hchi0 = 0.0;
+    r.hm1 = r.p1 = r.hchi1 = r.m1 = r.chi1 = 0.0;
+    r.u = r.v = 0.0;
+    r.hm2 = r.p2 = r.hchi2 = r.m2 = r.chi2 = 0.0;
+}
+
+// b4step2 pointwise part on a loaded cell, then F1 and F2.
+DC_HD void prepare_cell(const dclaw_params& P, const double* q7, double b, RowCell& r)
+{
+    double h = q7[0], hu = q7[1], hv = q7[2], hm = q7[3], p = q7[4], hchi = q7[5];
+    double u, v, m, chi, rho;
+    const double gz = P.grav;
+    qfix(P, h, hu, hv, hm, p, hchi, u, v, m, chi, rho, gz);
+    r.bdif = dmax(q7[6], 0.0);
+    if (h > 0.0) {
+        const double s = sqrt((hu * hu + hv * hv)) / h;
+        if (s > P.speed_limit) {
+            const double sratio = P.speed_limit / s;
+            hu = hu * sratio;
+            hv = hv * sratio;
+        }
+    }
+    r.h = h; r.hu = hu; r.hv = hv; r.b = b;
+    r.hm0 = hm; r.p0 = p; r.hchi0 = hchi;
+    // F1
+    double h1 = h, hu1 = hu, hv1 = hv, hm1 = hm, p1 = p, hchi1 = hchi;
+    qfix(P, h1, hu1, hv1, hm1, p1, hchi1, u, v, m, chi, rho, gz);
+    r.hm1 = hm1; r.p1 = p1; r.hchi1 = hchi1; r.m1 = m; r.chi1 = chi; r.u = u; r.v = v;
+    // F2
+    double hm2 = hm1, p2 = p1, hchi2 = hchi1;
+    qfix(P, h1, hu1, hv1, hm2, p2, hchi2, u, v, m, chi, rho, gz);
+    r.hm2 = hm2; r.p2 = p2; r.hchi2 = hchi2; r.m2 = m; r.chi2 = chi;
+}
+
+// rpn2 side state from a row cell: F2 (left side, when the cell is not the
+// slice's first cell) or F1 (right side, or the slice's first cell).
+DC_HD Side side_of(const RowCell& r, bool second, double taudir, double fs)
+{
+    Side s;
+    s.h = r.h; s.hu = r.hu; s.hv = r.hv;
+    s.hm = second ? r.hm2 : r.hm1;
+    s.p = second ? r.p2 : r.p1;
+    s.hchi = second ? r.hchi2 : r.hchi1;
+    s.m = second ? r.m2 : r.m1;
+    s.chi = second ? r.chi2 : r.chi1;
+    s.u = r.u;
+    s.v = r.v;
+    s.b = r.b - r.bdif;
+    s.taudir = taudir;
+    s.fs = fs;
+    return s;
+}
+
+// Shuffle a whole wave record from lane `src`.
+template <class Lane>
+DC_HD EdgeWaves shfl_waves(Lane& L, const EdgeWaves& w, int src)
+{
+    EdgeWaves o;
+    for (int k = 0; k < 3; k++) o.s[k] = L.shfl(w.s[k], src);
+    for (int k = 0; k < 6; k++) o.w1[k] = L.shfl(w.w1[k], src);
+    for (int k = 0; k < 6; k++) o.w5[k] = L.shfl(w.w5[k], src);
+    o.w2h = L.shfl(w.w2h, src);
+    o.w2n = L.shfl(w.w2n, src);
+    return o;
+}
+
+// Limiter sums of inlinelimiter.f over the 7 components in q-index order
+// (structural zeros dropped).  Family 0 = wave 1, 1 = wave 2, 2 = wave 5.
+DC_HD void wave_dots(const EdgeWaves& a, const EdgeWaves& b, int ixy, double* dot3)
+{
+    double d = 0.0;
+    for (int k = 0; k < 6; k++) d = d + a.w1[k] * b.w1[k];
+    dot3[0] = d;
+    // wave 2 components in q-index order: h, then normal (hu for x, hv for y)
+    d = 0.0;
+    d = d + a.w2h * b.w2h;
+    d = d + a.w2n * b.w2n;
+    dot3[1] = d;
+    d = 0.0;
+    for (int k = 0; k < 6; k++) d = d + a.w5[k] * b.w5[k];
+    dot3[2] = d;
+    (void)ixy;
+}
+
+// Second-order correction and fluctuations of one edge (flux2fw.f:149-243):
+// amdq/apdq from the unlimited waves (rpn2_dclaw.f:347-379), limited waves
+// (inlinelimiter.f) with dotl/dotr against unlimited neighbours, cqxx with
+// the wet-dry gate, fadd = (0 +- a{m|p}dq) + cqxx/2, and the cqxx fold into
+// amdq/apdq when transverse = 2.  Vectors in q-index order (6 components).
+struct EdgeFlux {
+    double faddm[6], faddp[6], amdq[6], apdq[6];
+};
+
+DC_HD EdgeFlux edge_flux(const dclaw_params& P, int ixy, const EdgeWaves& w, const double* dotl,
+                         const double* dotr, bool gate_wet, double dtdx)
+{
+    EdgeFlux o;
+    const int nrm = (ixy == 1) ? 1 : 2;
+    double amdq[6], apdq[6];
+    for (int k = 0; k < 6; k++) { amdq[k] = 0.0; apdq[k] = 0.0; }
+    // rpn2 fluctuations, waves in order 1, 2 (3, 4 are zero), 5
+    if (w.s[0] < 0.0) { for (int k = 0; k < 6; k++) amdq[k] = amdq[k] + w.w1[k]; }
+    else if (w.s[0] > 0.0) { for (int k = 0; k < 6; k++) apdq[k] = apdq[k] + w.w1[k]; }
+    if (w.s[1] < 0.0) { amdq[0] = amdq[0] + w.w2h; amdq[nrm] = amdq[nrm] + w.w2n; }
+    else if (w.s[1] > 0.0) { apdq[0] = apdq[0] + w.w2h; apdq[nrm] = apdq[nrm] + w.w2n; }
+    if (w.s[2] < 0.0) { for (int k = 0; k < 6; k++) amdq[k] = amdq[k] + w.w5[k]; }
+    else if (w.s[2] > 0.0) { for (int k = 0; k < 6; k++) apdq[k] = apdq[k] + w.w5[k]; }
+
+    double cq[6];
+    for (int k = 0; k < 6; k++) cq[k] = 0.0;
+    if (P.order != 1) {
+        // limited waves
+        double phi[3] = {1.0, 1.0, 1.0};
+        const int fam_mw[3] = {0, 1, 4};
+        double wn2[3];
+        {
+            double d = 0.0;
+            for (int k = 0; k < 6; k++) d = d + w.w1[k] * w.w1[k];
+            wn2[0] = d;
+            d = 0.0;
+            d = d + w.w2h * w.w2h;
+            d = d + w.w2n * w.w2n;
+            wn2[1] = d;
+            d = 0.0;
+            for (int k = 0; k < 6; k++) d = d + w.w5[k] * w.w5[k];
+            wn2[2] = d;
+        }
+        for (int f = 0; f < 3; f++) {
+            const int lim = P.mthlim[fam_mw[f]];
+            if (lim == 0 || wn2[f] == 0.0) continue;
+            double r;
+            if (w.s[f] > 0.0) r = dotl[f] / wn2[f];
+            else r = dotr[f] / wn2[f];
+            phi[f] = limiter_phi(lim, r);
+        }
+        if (gate_wet) {
+            const double c0 = dsign(1.0, w.s[0]) * (1.0 - fabs(w.s[0]) * dtdx);
+            const double c1 = dsign(1.0, w.s[1]) * (1.0 - fabs(w.s[1]) * dtdx);
+            const double c2 = dsign(1.0, w.s[2]) * (1.0 - fabs(w.s[2]) * dtdx);
+            // waves limited only when the family's limiter applied
+            double l1[6], l5[6];
+            const bool a0 = (P.mthlim[0] != 0 && wn2[0] != 0.0);
+            const bool a1 = (P.mthlim[1] != 0 && wn2[1] != 0.0);
+            const bool a2 = (P.mthlim[4] != 0 && wn2[2] != 0.0);
+            for (int k = 0; k < 6; k++) {
+                l1[k] = a0 ? phi[0] * w.w1[k] : w.w1[k];
+                l5[k] = a2 ? phi[2] * w.w5[k] : w.w5[k];
+            }
+            const double l2h = a1 ? phi[1] * w.w2h : w.w2h;
+            const double l2n = a1 ? phi[1] * w.w2n : w.w2n;
+            for (int k = 0; k < 6; k++) {
+                double c = 0.0;
+                c = c + c0 * l1[k];
+                if (k == 0) c = c + c1 * l2h;
+                if (k == nrm) c = c + c1 * l2n;
+                c = c + c2 * l5[k];
+                cq[k] = c;
+            }
+        }
+    }
+    for (int k = 0; k < 6; k++) {
+        o.faddm[k] = (0.0 + amdq[k]) + 0.5 * cq[k];
+        o.faddp[k] = (0.0 - apdq[k]) + 0.5 * cq[k];
+    }
+    if (P.order > 1 && P.transverse == 2) {
+        for (int k = 0; k < 6; k++) {
+            o.amdq[k] = amdq[k] + cq[k];
+            o.apdq[k] = apdq[k] - cq[k];
+        }
+    } else {
+        for (int k = 0; k < 6; k++) { o.amdq[k] = amdq[k]; o.apdq[k] = apdq[k]; }
+    }
+    return o;
+}
+
+// G(k, side) of one cell: (0 - g_imp1) - g_imp2, g = (0.5*dtdx)*b{m|p}asdq.
+DC_HD void trans_combine(const double* bm1, const double* bp1, const double* bm2,
+                         const double* bp2, double dtdx, double* G1, double* G2)
+{
+    for (int k = 0; k < 6; k++) {
+        G1[k] = (0.0 - 0.5 * dtdx * bm1[k]) - 0.5 * dtdx * bm2[k];
+        G2[k] = (0.0 - 0.5 * dtdx * bp1[k]) - 0.5 * dtdx * bp2[k];
+    }
+}
+
+DC_HD size_t qidx(int m, int i, int j, int mx)
+{
+    // q(m, i, j) with m 0-based, i, j Fortran indices, mbc = 2
+    return (size_t)m + 7u * ((size_t)(i + 1) + (size_t)(mx + 4) * (size_t)(j + 1));
+}
+DC_HD size_t aidx(int i, int j, int mx, int maux)
+{
+    return (size_t)maux * ((size_t)(i + 1) + (size_t)(mx + 4) * (size_t)(j + 1));
+}
+
+// The strip body.  Returns this lane's CFL contribution (max |dt/dx * s|).
+template <class Lane>
+DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
+{
+    const int lane = L.id();
+    const int c = A.i0 - STRIP_HALO + lane;   // this lane's column
+    const int mx = A.mx, my = A.my;
+    const double dtdx = A.dt / A.dx, dtdy = A.dt / A.dy;
+    const bool col_in_array = (c >= -1 && c <= mx + 2);
+    const bool col_taudir = (c >= 0 && c <= mx + 1);
+    const int lane_prev = lane > 0 ? lane - 1 : 0;
+    const int lane_next = lane < STRIP_LANES - 1 ? lane + 1 : STRIP_LANES - 1;
+    // interior cells this lane writes
+    const bool col_out = (c >= A.i0 && c <= A.i0 + STRIP_W - 1 && c >= 1 && c <= mx);
+    // ghost columns written by this strip (first strip: -1, 0; last: mx+1, mx+2)
+    const bool col_ghost_w = (c >= -1 && c <= 0 && A.i0 == 1) ||
+                             (c >= mx + 1 && c <= mx + 2 && A.i0 + STRIP_W - 1 >= mx);
+
+    double cfl = 0.0;
+
+    RowCell q3, q2, q1, q0;   // rows R-3, R-2, R-1, R
+    zero_row(q3); zero_row(q2); zero_row(q1); zero_row(q0);
+    double h_r4 = 0.0;        // h of row R-4
+    Taudir ta2, ta1;          // rows R-2, R-1
+    ta2.fsphi = ta2.tx = ta2.ty = 0.0;
+    ta1 = ta2;
+    EdgeWaves yw_pend;        // y-edge R-2 (unlimited)
+    for (int k = 0; k < 3; k++) yw_pend.s[k] = 0.0;
+    for (int k = 0; k < 6; k++) { yw_pend.w1[k] = 0.0; yw_pend.w5[k] = 0.0; }
+    yw_pend.w2h = yw_pend.w2n = 0.0;
+    double ydotl[3] = {0.0, 0.0, 0.0};   // dot(w(R-3), w(R-2))
+    double gy_imp2_m[6], gy_imp2_p[6];    // imp2 of y-edge R-3 with cell (c, R-3)
+    double fxm_prev[6], fxp_prev[6];      // x faddm/faddp of row R-3
+    double gx_prev2[6];                   // Gx_{R-3}(c, 2)
+    double gp_prev[6];                    // gp(c, R-3)
+    for (int k = 0; k < 6; k++) {
+        gy_imp2_m[k] = gy_imp2_p[k] = 0.0;
+        fxm_prev[k] = fxp_prev[k] = 0.0;
+        gx_prev2[k] = gp_prev[k] = 0.0;
+    }
+
+    for (int R = A.jA - 3; R <= A.jB + 3; R++) {
+        // ---------------- S1: load row R, b4step2 pointwise, F1, F2
+        {
+            double q7[7] = {0, 0, 0, 0, 0, 0, 0};
+            double b = 0.0;
+            const bool row_in = (R >= -1 && R <= my + 2);
+            if (row_in && col_in_array) {
+                const size_t o = qidx(0, c, R, mx);
+                for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
+                b = A.aux[aidx(c, R, mx, A.maux)];
+            }
+            prepare_cell(P, q7, b, q0);
+            const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
+                                     (R >= my + 1 && R <= my + 2 && A.jB == my);
+            const bool row_interior = (R >= 1 && R <= my);
+            if (row_in && col_in_array &&
+                ((row_ghost_w && (col_out || col_ghost_w)) || (row_interior && col_ghost_w &&
+                                                              R >= A.jA && R <= A.jB))) {
+                const size_t o = qidx(0, c, R, mx);
+                A.q_out[o + 0] = q0.h; A.q_out[o + 1] = q0.hu; A.q_out[o + 2] = q0.hv;
+                A.q_out[o + 3] = q0.hm0; A.q_out[o + 4] = q0.p0; A.q_out[o + 5] = q0.hchi0;
+                A.q_out[o + 6] = q0.bdif;
+            }
+        }
+
+        // ---------------- S2: calc_taudir for row R-1
+        {
+            const int r = R - 1;
+            // neighbour columns of row R-1 (F1 states), all lanes participate
+            const double lh = L.shfl(q1.h, lane_prev), lhu = L.shfl(q1.hu, lane_prev);
+            const double lhv = L.shfl(q1.hv, lane_prev), lhm = L.shfl(q1.hm0, lane_prev);
+            const double lp = L.shfl(q1.p0, lane_prev), lhc = L.shfl(q1.hchi0, lane_prev);
+            const double lb = L.shfl(q1.b, lane_prev), lbd = L.shfl(q1.bdif, lane_prev);
+            const double rh = L.shfl(q1.h, lane_next), rhu = L.shfl(q1.hu, lane_next);
+            const double rhv = L.shfl(q1.hv, lane_next), rhm = L.shfl(q1.hm0, lane_next);
+            const double rp = L.shfl(q1.p0, lane_next), rhc = L.shfl(q1.hchi0, lane_next);
+            const double rb = L.shfl(q1.b, lane_next), rbd = L.shfl(q1.bdif, lane_next);
+            Taudir t;
+            t.fsphi = t.tx = t.ty = 0.0;
+            if (r >= 0 && r <= my + 1 && col_taudir && r >= A.jA - 2 && r <= A.jB + 2) {
+                Cell C = {q1.h, q1.hu, q1.hv, q1.hm0, q1.p0, q1.hchi0};
+                Cell Lc = {lh, lhu, lhv, lhm, lp, lhc};
+                Cell Rc = {rh, rhu, rhv, rhm, rp, rhc};
+                Cell B = {q2.h, q2.hu, q2.hv, q2.hm0, q2.p0, q2.hchi0};
+                Cell T = {q0.h, q0.hu, q0.hv, q0.hm0, q0.p0, q0.hchi0};
+                t = calc_taudir_cell(P, A.dx, A.dy, C, q1.b - q1.bdif, Lc, lb - lbd, Rc, rb - rbd,
+                                     B, q2.b - q2.bdif, T, q0.b - q0.bdif);
+            }
+            if (A.aux_out && r >= 0 && r <= my + 1 && col_taudir) {
+                const bool row_own = (r >= A.jA && r <= A.jB) || (r == 0 && A.jA == 1) ||
+                                     (r == my + 1 && A.jB == my);
+                const bool col_own = (c >= A.i0 && c <= A.i0 + STRIP_W - 1 && c >= 1 && c <= mx) ||
+                                     (c == 0 && A.i0 == 1) ||
+                                     (c == mx + 1 && A.i0 + STRIP_W - 1 >= mx);
+                if (row_own && col_own) {
+                    const size_t o = aidx(c, r, mx, A.maux);
+                    A.aux_out[o + 3] = t.fsphi;
+                    A.aux_out[o + 4] = t.tx;
+                    A.aux_out[o + 5] = t.ty;
+                }
+            }
+            ta2 = ta1;
+            ta1 = t;
+        }
+
+        // ---------------- S3: y-edge R-1 (cells (c, R-2), (c, R-1))
+        EdgeWaves yw_new;
+        double ydotr[3] = {0.0, 0.0, 0.0};
+        {
+            const int k = R - 1;
+            for (int z = 0; z < 3; z++) yw_new.s[z] = 0.0;
+            for (int z = 0; z < 6; z++) { yw_new.w1[z] = 0.0; yw_new.w5[z] = 0.0; }
+            yw_new.w2h = yw_new.w2n = 0.0;
+            if (k >= A.jA - 1 && k <= A.jB + 2 && col_taudir) {
+                const Side sL = side_of(q2, (k - 1) >= 0, ta2.ty, ta2.fsphi);
+                const Side sR = side_of(q1, false, ta1.ty, ta1.fsphi);
+                // y sweep: primitives u = hv/h (normal), v = hu/h (transverse)
+                Side l2 = sL, r2 = sR;
+                l2.u = q2.v; l2.v = q2.u;
+                r2.u = q1.v; r2.v = q1.u;
+                yw_new = rpn2_edge(P, 2, l2, r2);
+            }
+            wave_dots(yw_pend, yw_new, 2, ydotr);
+        }
+
+        // ---------------- S4: x-sweep of row R-2
+        double fxm[6], fxp[6], Gx1[6], Gx2[6];
+        for (int z = 0; z < 6; z++) { fxm[z] = fxp[z] = Gx1[z] = Gx2[z] = 0.0; }
+        {
+            const int r = R - 2;
+            const bool row_act = (r >= A.jA - 1 && r <= A.jB + 1);
+            // left neighbour's F2 (or F1 for the slice's first cell) state
+            const double lh = L.shfl(q2.h, lane_prev), lhu = L.shfl(q2.hu, lane_prev);
+            const double lhv = L.shfl(q2.hv, lane_prev);
+            const double lhm1 = L.shfl(q2.hm1, lane_prev), lp1 = L.shfl(q2.p1, lane_prev);
+            const double lhc1 = L.shfl(q2.hchi1, lane_prev), lm1 = L.shfl(q2.m1, lane_prev);
+            const double lc1 = L.shfl(q2.chi1, lane_prev);
+            const double lhm2 = L.shfl(q2.hm2, lane_prev), lp2 = L.shfl(q2.p2, lane_prev);
+            const double lhc2 = L.shfl(q2.hchi2, lane_prev), lm2 = L.shfl(q2.m2, lane_prev);
+            const double lc2 = L.shfl(q2.chi2, lane_prev);
+            const double lu = L.shfl(q2.u, lane_prev), lv = L.shfl(q2.v, lane_prev);
+            const double lb = L.shfl(q2.b, lane_prev), lbd = L.shfl(q2.bdif, lane_prev);
+            const double ltx = L.shfl(ta2.tx, lane_prev), lfs = L.shfl(ta2.fsphi, lane_prev);
+            EdgeWaves xw;
+            for (int z = 0; z < 3; z++) xw.s[z] = 0.0;
+            for (int z = 0; z < 6; z++) { xw.w1[z] = 0.0; xw.w5[z] = 0.0; }
+            xw.w2h = xw.w2n = 0.0;
+            const bool edge_ok = row_act && c >= 0 && c <= mx + 2 && lane >= 1;
+            if (edge_ok) {
+                Side sL;
+                const bool second = (c - 1) >= 0;
+                sL.h = lh; sL.hu = lhu; sL.hv = lhv;
+                sL.hm = second ? lhm2 : lhm1;
+                sL.p = second ? lp2 : lp1;
+                sL.hchi = second ? lhc2 : lhc1;
+                sL.m = second ? lm2 : lm1;
+                sL.chi = second ? lc2 : lc1;
+                sL.u = lu; sL.v = lv;
+                sL.b = lb - lbd;
+                sL.taudir = ltx; sL.fs = lfs;
+                const Side sR = side_of(q2, false, ta2.tx, ta2.fsphi);
+                xw = rpn2_edge(P, 1, sL, sR);
+            }
+            // limiter neighbours
+            const EdgeWaves xn = shfl_waves(L, xw, lane_next);
+            double dr[3];
+            wave_dots(xw, xn, 1, dr);
+            double dl[3];
+            for (int f = 0; f < 3; f++) dl[f] = L.shfl(dr[f], lane_prev);
+            // wet-dry gate: h(c-2) h(c-1) h(c) h(c+1) > 0, in flux2's product order
+            const double hm2c = L.shfl(q2.h, lane >= 2 ? lane - 2 : 0);
+            const double hm1c = lh;
+            const double hp1c = L.shfl(q2.h, lane_next);
+            const bool wet = (hm1c * q2.h * hm2c * hp1c > 0.0);
+            const EdgeFlux ef = edge_flux(P, 1, xw, dl, dr, wet, dtdx);
+            const bool lim_ok = row_act && c >= 1 && c <= mx + 1 && lane >= 2 && lane <= 62;
+            if (lim_ok) {
+                for (int z = 0; z < 6; z++) { fxm[z] = ef.faddm[z]; fxp[z] = ef.faddp[z]; }
+                for (int f = 0; f < 3; f++) {
+                    const double a = fabs(dtdx * xw.s[f]);
+                    cfl = dmax(cfl, a);
+                }
+            }
+            // transverse split in cell c: imp1 with amdq of edge c+1, imp2 with apdq of edge c
+            double amdq_next[6];
+            for (int z = 0; z < 6; z++) amdq_next[z] = L.shfl(ef.amdq[z], lane_next);
+            if (P.transverse != 0) {
+                const bool cell_ok = row_act && c >= 1 && c <= mx && lane >= 2 && lane <= 61;
+                double bm1[6], bp1[6], bm2[6], bp2[6];
+                rpt2_cell_split(P, 1, q2.h, q2.hu, q2.hv, q2.hm2, q2.hchi2, q2.b, q2.bdif, q3.b,
+                                q1.b, amdq_next, cell_ok, bm1, bp1);
+                rpt2_cell_split(P, 1, q2.h, q2.hu, q2.hv, q2.hm2, q2.hchi2, q2.b, q2.bdif, q3.b,
+                                q1.b, ef.apdq, cell_ok, bm2, bp2);
+                if (cell_ok) trans_combine(bm1, bp1, bm2, bp2, dtdx, Gx1, Gx2);
+            }
+        }
+
+        // ---------------- S5: y-edge R-2 limited and split
+        double fym[6], fyp[6], Gy1[6], Gy2[6];
+        for (int z = 0; z < 6; z++) { fym[z] = fyp[z] = Gy1[z] = Gy2[z] = 0.0; }
+        {
+            const int k = R - 2;
+            const bool act = (k >= A.jA && k <= A.jB + 1) && col_taudir;
+            // wet gate along the column: h(k-2) h(k-1) h(k) h(k+1) = rows R-4..R-1
+            const bool wet = (q3.h * q2.h * h_r4 * q1.h > 0.0);
+            const EdgeFlux ef = edge_flux(P, 2, yw_pend, ydotl, ydotr, wet, dtdy);
+            if (act) {
+                for (int z = 0; z < 6; z++) { fym[z] = ef.faddm[z]; fyp[z] = ef.faddp[z]; }
+                for (int f = 0; f < 3; f++) cfl = dmax(cfl, fabs(dtdy * yw_pend.s[f]));
+            }
+            // topo of the neighbouring columns for rows R-3 and R-2
+            const double b3l = L.shfl(q3.b, lane_prev), b3r = L.shfl(q3.b, lane_next);
+            const double b2l = L.shfl(q2.b, lane_prev), b2r = L.shfl(q2.b, lane_next);
+            if (P.transverse != 0) {
+                double bm1[6], bp1[6], bm2[6], bp2[6];
+                // imp1: amdq of edge R-2 split in cell (c, R-3)
+                rpt2_cell_split(P, 2, q3.h, q3.hu, q3.hv, q3.hm2, q3.hchi2, q3.b, q3.bdif, b3l,
+                                b3r, ef.amdq, act, bm1, bp1);
+                for (int z = 0; z < 6; z++) {
+                    Gy1[z] = (0.0 - 0.5 * dtdy * bm1[z]) - gy_imp2_m[z];
+                    Gy2[z] = (0.0 - 0.5 * dtdy * bp1[z]) - gy_imp2_p[z];
+                }
+                // imp2: apdq of edge R-2 split in cell (c, R-2), kept for the next row
+                rpt2_cell_split(P, 2, q2.h, q2.hu, q2.hv, q2.hm2, q2.hchi2, q2.b, q2.bdif, b2l,
+                                b2r, ef.apdq, act, bm2, bp2);
+                for (int z = 0; z < 6; z++) {
+                    gy_imp2_m[z] = 0.5 * dtdy * bm2[z];
+                    gy_imp2_p[z] = 0.5 * dtdy * bp2[z];
+                }
+            }
+        }
+
+        // ---------------- S6: assemble, update and src2 for row R-3
+        {
+            const int j = R - 3;
+            // fm(c, j) = ((0 + faddm_x(c, j)) + Gy_{c-1}(j, 2)) + Gy_c(j, 1); same for fp
+            double gyl2[6];
+            for (int z = 0; z < 6; z++) gyl2[z] = L.shfl(Gy2[z], lane_prev);
+            double fmc[6], fpc[6];
+            for (int z = 0; z < 6; z++) {
+                fmc[z] = ((0.0 + fxm_prev[z]) + gyl2[z]) + Gy1[z];
+                fpc[z] = ((0.0 + fxp_prev[z]) + gyl2[z]) + Gy1[z];
+            }
+            double fm_next[6];
+            for (int z = 0; z < 6; z++) fm_next[z] = L.shfl(fmc[z], lane_next);
+            // gm(c, j+1) = ((0 + Gx_j(c,2)) + Gx_{j+1}(c,1)) + faddm_y(c, j+1)
+            double gm_next[6], gp_next[6];
+            for (int z = 0; z < 6; z++) {
+                const double t = (0.0 + gx_prev2[z]) + Gx1[z];
+                gm_next[z] = t + fym[z];
+                gp_next[z] = t + fyp[z];
+            }
+            if (j >= A.jA && j <= A.jB && col_out) {
+                double qn[6] = {q3.h, q3.hu, q3.hv, q3.hm0, q3.p0, q3.hchi0};
+                for (int z = 0; z < 6; z++)
+                    qn[z] = qn[z] - dtdx * (fm_next[z] - fpc[z]) - dtdy * (gm_next[z] - gp_prev[z]);
+                src2_cell(P, A.dt, qn[0], qn[1], qn[2], qn[3], qn[4], qn[5]);
+                const size_t o = qidx(0, c, j, mx);
+                for (int z = 0; z < 6; z++) A.q_out[o + z] = qn[z];
+                A.q_out[o + 6] = q3.bdif;
+            }
+            for (int z = 0; z < 6; z++) {
+                gp_prev[z] = gp_next[z];
+                gx_prev2[z] = Gx2[z];
+                fxm_prev[z] = fxm[z];
+                fxp_prev[z] = fxp[z];
+            }
+        }
+
+        // ---------------- rotate the window
+        h_r4 = q3.h;
+        q3 = q2;
+        q2 = q1;
+        q1 = q0;
+        yw_pend = yw_new;
+        for (int f = 0; f < 3; f++) ydotl[f] = ydotr[f];
+    }
+    return cfl;
+}
+
+}  // namespace dclaw

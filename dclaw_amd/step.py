@@ -1,0 +1,186 @@
+"""Host-side driver of the gfx950 step: the calls GeoClaw's time loop makes
+for one patch (bound -> stepgrid -> dtnew), on device-resident buffers.
+
+torch tensors are used only as device buffers and for the stream; every
+computation happens in the HIP kernels behind the C-ABI (dclaw_amd/_lib.py).
+
+Reference call sequence mirrored here (Clawpack 5 GeoClaw, restated):
+``bound`` fills ghost cells (bc2amr), ``stepgrid`` runs b4step2
+(b4step2.f90:7), step2 (flux2fw.f:8, rpn2_dclaw.f:24, rpt2_dclaw.f:16), the
+conservative update and src2 (src2.f90:27), and returns cflgrid; the next
+dt is dt*cfl_desired/cflgrid capped by dt_max (tick.f:484-531).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .params import BC_EXTRAP, CParams, DigParams, to_c
+
+MEQN = 7
+MBC = 2
+
+
+def _ptr(t: torch.Tensor) -> int:
+    return int(t.data_ptr())
+
+
+def _stream() -> int:
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+def params_tensor(params: Sequence[DigParams], device="cuda") -> torch.Tensor:
+    """Device array of dclaw_params records (one per member)."""
+    recs = [to_c(p) for p in params]
+    raw = b"".join(bytes(r) for r in recs)
+    host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
+    return host.to(device)
+
+
+def to_device_q(q: np.ndarray, device="cuda") -> torch.Tensor:
+    """Fortran-ordered (7, mx+4, my+4) numpy array -> contiguous device tensor
+    of shape (my+4, mx+4, 7): the same bytes, component fastest."""
+    qf = np.asfortranarray(q, dtype=np.float64)
+    return torch.from_numpy(np.ascontiguousarray(qf.transpose(2, 1, 0))).to(device)
+
+
+def to_host_q(t: torch.Tensor) -> np.ndarray:
+    a = t.detach().cpu().numpy()
+    return np.asfortranarray(a.transpose(2, 1, 0))
+
+
+def bed_tensor(aux: np.ndarray, device="cuda") -> torch.Tensor:
+    """aux(1) (the bed) as a dense (my+4, mx+4) device array (maux = 1)."""
+    b = np.asfortranarray(aux)[0]
+    return torch.from_numpy(np.ascontiguousarray(b.T)).to(device)
+
+
+def bc_fill(q: torch.Tensor, mx: int, my: int, bc=(BC_EXTRAP,) * 4) -> None:
+    """Ghost-cell fill in place (bc2amr extrap / wall)."""
+    arr = (ctypes.c_int * 4)(*bc)
+    _lib.check(_lib.lib().dclaw_bc_fill(_ptr(q), mx, my, arr, _stream()))
+
+
+def stepgrid(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int, mx: int,
+             my: int, dx: float, dy: float, dt_dev: torch.Tensor, params_dev: torch.Tensor,
+             cfl_dev: torch.Tensor) -> None:
+    """One fused stepgrid on device buffers; cfl_dev receives max(cfl_dev, cflgrid)."""
+    _lib.check(_lib.lib().dclaw_stepgrid_dev(
+        _ptr(q_in), _ptr(q_out), _ptr(aux), maux, mx, my, dx, dy, _ptr(dt_dev),
+        _ptr(params_dev), _ptr(cfl_dev), _stream()))
+
+
+def dt_update(dt_dev: torch.Tensor, cfl_dev: torch.Tensor, cfl_desired: float,
+              dt_max: float) -> None:
+    _lib.check(_lib.lib().dclaw_dt_update(_ptr(dt_dev), _ptr(cfl_dev), dt_dev.numel(),
+                                          cfl_desired, dt_max, _stream()))
+
+
+class PatchRunner:
+    """Device-resident fixed-patch time loop (one member)."""
+
+    def __init__(self, q: np.ndarray, aux: np.ndarray, params: DigParams, dx: float, dy: float,
+                 bc=(BC_EXTRAP,) * 4, device="cuda"):
+        self.mx = q.shape[1] - 2 * MBC
+        self.my = q.shape[2] - 2 * MBC
+        self.dx, self.dy = float(dx), float(dy)
+        self.bc = tuple(bc)
+        self.qa = to_device_q(q, device)
+        self.qb = torch.empty_like(self.qa)
+        self.bed = bed_tensor(aux, device)
+        self.params = params_tensor([params], device)
+        self.dt = torch.zeros(1, dtype=torch.float64, device=device)
+        self.cfl = torch.zeros(1, dtype=torch.float64, device=device)
+
+    def set_dt(self, dt: float) -> None:
+        self.dt.fill_(float(dt))
+
+    def step(self) -> None:
+        """bound, stepgrid; the result is left in self.qa and cflgrid in self.cfl."""
+        bc_fill(self.qa, self.mx, self.my, self.bc)
+        stepgrid(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy, self.dt,
+                 self.params, self.cfl)
+        self.qa, self.qb = self.qb, self.qa
+
+    def run(self, nsteps: int, dt0: float, cfl_desired: float, dt_max: float,
+            cfl_hist: Optional[torch.Tensor] = None) -> None:
+        self.set_dt(dt0)
+        self.cfl.zero_()
+        for n in range(nsteps):
+            self.step()
+            if cfl_hist is not None:
+                cfl_hist[n].copy_(self.cfl[0])
+            dt_update(self.dt, self.cfl, cfl_desired, dt_max)
+
+    def q(self) -> np.ndarray:
+        return to_host_q(self.qa)
+
+
+class Ensemble:
+    """Many members of uneven patch sizes advanced by one launch per step over
+    a (patch, strip, segment) work list, each with its own parameters and dt."""
+
+    def __init__(self, members: List, bc=(BC_EXTRAP,) * 4, seg_rows: int = 32, device="cuda"):
+        self.n = len(members)
+        self.seg_rows = seg_rows
+        self.bc = tuple(bc)
+        self.mx = [m.patch.mx for m in members]
+        self.my = [m.patch.my for m in members]
+        self.qa = [to_device_q(m.patch.q, device) for m in members]
+        self.qb = [torch.empty_like(t) for t in self.qa]
+        self.bed = [bed_tensor(m.patch.aux, device) for m in members]
+        self.params = params_tensor([m.patch.params for m in members], device)
+        self.dt = torch.zeros(self.n, dtype=torch.float64, device=device)
+        self.cfl = torch.zeros(self.n, dtype=torch.float64, device=device)
+        L = _lib.lib()
+        mxa = (ctypes.c_int * self.n)(*self.mx)
+        mya = (ctypes.c_int * self.n)(*self.my)
+        nt = L.dclaw_worklist_size(mxa, mya, self.n, seg_rows)
+        if nt < 0:
+            _lib.check(nt)
+        tiles = (ctypes.c_int32 * (4 * nt))()
+        _lib.check(min(0, L.dclaw_worklist_build(mxa, mya, self.n, seg_rows, tiles)))
+        self.ntiles = nt
+        self.tiles = torch.tensor(list(tiles), dtype=torch.int32, device=device)
+        self.dx = [m.patch.dx for m in members]
+        self.dy = [m.patch.dy for m in members]
+        self._patches = [self._patch_table(False, device), self._patch_table(True, device)]
+        self.flip = False
+
+    def _patch_table(self, flipped: bool, device) -> torch.Tensor:
+        recs = []
+        for k in range(self.n):
+            qi, qo = (self.qb[k], self.qa[k]) if flipped else (self.qa[k], self.qb[k])
+            recs.append(_lib.CPatch(_ptr(qi), _ptr(qo), _ptr(self.bed[k]), self.mx[k],
+                                    self.my[k], 1, k, self.dx[k], self.dy[k]))
+        raw = b"".join(bytes(r) for r in recs)
+        return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+
+    def current(self, k: int) -> torch.Tensor:
+        return self.qb[k] if self.flip else self.qa[k]
+
+    def step(self) -> None:
+        L = _lib.lib()
+        for k in range(self.n):
+            bc_fill(self.current(k), self.mx[k], self.my[k], self.bc)
+        _lib.check(L.dclaw_step_batched(
+            _ptr(self._patches[1 if self.flip else 0]), self.n, _ptr(self.tiles), self.ntiles,
+            self.seg_rows, _ptr(self.dt), _ptr(self.params), _ptr(self.cfl), _stream()))
+        self.flip = not self.flip
+
+    def run(self, nsteps: int, dt0, cfl_desired: float, dt_max: float) -> None:
+        if np.isscalar(dt0):
+            self.dt.fill_(float(dt0))
+        else:
+            self.dt.copy_(torch.as_tensor(dt0, dtype=torch.float64))
+        self.cfl.zero_()
+        for _ in range(nsteps):
+            self.step()
+            dt_update(self.dt, self.cfl, cfl_desired, dt_max)
+
+    def q(self, k: int) -> np.ndarray:
+        return to_host_q(self.current(k))

@@ -1,0 +1,129 @@
+/*
+ * dclaw_hip.h -- C-ABI of the MI355X (gfx950) D-Claw per-patch step.
+ *
+ * Drop-in boundary for the path GeoClaw's stepgrid drives each step:
+ *   bound (ghost fill) -> b4step2 -> step2 (rpn2/rpt2/flux2 sweeps) ->
+ *   conservative update -> src2 -> dtnew = dt*cfl_desired/cflgrid.
+ *
+ * Layout is the Clawpack contract, unchanged: q(meqn=7, 1-mbc:mx+mbc,
+ * 1-mbc:my+mbc) and aux(maux, same), Fortran column-major, component
+ * fastest (rpn2_dclaw.f:67-74, src2.f90:47-49).  q components: 1 h, 2 hu,
+ * 3 hv, 4 hm, 5 pb, 6 hchi, 7 bdif (digclaw_module.f90:96-102).  aux(1) is
+ * the bed b; the device entry points read only aux(1) (stride maux), since
+ * phi comes from the parameter record and fsphi/taudir are recomputed inside
+ * the step exactly as b4step2 would (b4step2.f90:111).  mbc must be 2.
+ *
+ * All pointers passed to the device entry points are device (HBM) pointers;
+ * all take an explicit hipStream_t (passed as void*).  Every call returns
+ * 0 on success or a negative DCLAW_E* code; dclaw_last_error() gives text.
+ * Parameters are an explicit record (the reference's module globals,
+ * digclaw_module.f90:15-52, geoclaw_module grav/dry_tolerance/speed_limit,
+ * amr_module method/mthlim), not globals.
+ */
+#ifndef DCLAW_HIP_H
+#define DCLAW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DCLAW_OK 0
+#define DCLAW_EINVAL (-1)
+#define DCLAW_EHIP (-2)
+#define DCLAW_ENOMEM (-3)
+
+#define DCLAW_MEQN 7
+#define DCLAW_MBC 2
+
+/* Clawpack bc codes (bc2amr): */
+#define DCLAW_BC_EXTRAP 1
+#define DCLAW_BC_WALL 3
+
+typedef struct dclaw_params {
+    double grav;                /* geoclaw_module grav */
+    double dry_tolerance;       /* geoclaw_module dry_tolerance */
+    double speed_limit;         /* geoclaw_module speed_limit (b4step2.f90:78) */
+    double rho_s, rho_f, m_crit, mref, kref;
+    double phi;                 /* basal friction angle, radians (digclaw_module.f90:136) */
+    double delta, mu, alpha_c, c1, sigma_0;
+    double kappa;               /* earth-pressure coefficient, 1 (digclaw_module.f90:113) */
+    double beta_seg, delta_kr_order;
+    double manning_coefficient; /* manning_coefficient(1) (src2.f90:82) */
+    double friction_depth, manning_max;
+    int src2method;             /* -1, 0 or 1 */
+    int alphamethod, rarecorrectortest, segregation;
+    int friction_forcing, dd_manning;
+    int order;                  /* method(2): 1 or 2 */
+    int transverse;             /* method(3): 0, 1 or 2 */
+    int mthlim[5];              /* limiter per wave family (0 none, 1 minmod, 2 superbee,
+                                   3 van Leer, 4 MC, 5 Beam-Warming) */
+    int mwaves;                 /* 5 */
+} dclaw_params;
+
+/* One patch of a (possibly batched) step.  All pointers are device pointers. */
+typedef struct dclaw_patch {
+    const double* q_in;         /* q(7, mx+4, my+4): state at t_n, ghosts filled */
+    double* q_out;              /* q(7, mx+4, my+4): state at t_n+dt (may not alias q_in) */
+    const double* aux;          /* aux(maux, mx+4, my+4); only aux(1) = b is read */
+    int mx, my, maux, member;   /* member indexes params[], dt[], cfl[] */
+    double dx, dy;
+} dclaw_patch;
+
+const char* dclaw_last_error(void);
+int dclaw_version(void);
+
+/* Number of workgroups the fused step launches for one patch, and the strip
+ * width / segment height used (for tests and roofline accounting). */
+int dclaw_strip_width(void);
+
+/* Ghost-cell fill of q (bc2amr extrap / wall; bc[4] = xlo, xhi, ylo, yhi),
+ * on device memory.  Replaces AMRClaw's bound/bc2amr for one patch. */
+int dclaw_bc_fill(double* q, int mx, int my, const int* bc, void* stream);
+
+/* Fused stepgrid for ONE patch on device memory: b4step2 + step2 + update
+ * + src2 for interior cells of q_out (ghost cells of q_out receive the
+ * b4step2-adjusted values, as the reference leaves them in q).  dt_dev and
+ * cfl_dev are device scalars; cfl_dev must be zeroed by the caller (the
+ * kernel takes a max into it: cflgrid of stepgrid).  params_dev is a device
+ * dclaw_params.  Replaces GeoClaw stepgrid (b4step2.f90:7, flux2fw.f:8,
+ * rpn2_dclaw.f:24, rpt2_dclaw.f:16, src2.f90:27). */
+int dclaw_stepgrid_dev(const double* q_in, double* q_out, const double* aux, int maux,
+                       int mx, int my, double dx, double dy, const double* dt_dev,
+                       const dclaw_params* params_dev, double* cfl_dev, void* stream);
+
+/* Batched fused step over npatch patches (device array of dclaw_patch).
+ * One launch; the work list of (patch, strip, segment) tiles is built on
+ * the host by dclaw_worklist_size / dclaw_worklist_build into device memory.
+ * params_dev, dt_dev, cfl_dev are indexed by dclaw_patch.member. */
+int dclaw_worklist_size(const int* mx, const int* my, int npatch, int seg_rows);
+int dclaw_worklist_build(const int* mx, const int* my, int npatch, int seg_rows,
+                         int32_t* tiles_host /* 4*size ints */);
+int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
+                       int ntiles, int seg_rows, const double* dt_dev,
+                       const dclaw_params* params_dev, double* cfl_dev, void* stream);
+
+/* Device dt rule for n members: dt[k] <- min(dt[k]*cfl_desired/cfl[k], dt_max)
+ * when cfl[k] > 0 (else dt_max), then cfl[k] <- 0 (GeoClaw stepgrid dtnew
+ * and tick.f:499 dt_max cap). */
+int dclaw_dt_update(double* dt_dev, double* cfl_dev, int n, double cfl_desired,
+                    double dt_max, void* stream);
+
+/* Host-memory drop-ins with the Fortran calling convention (all arguments by
+ * reference, q/aux in the Clawpack layout), for AMRClaw executables that
+ * swap these objects in through Makefile SOURCES/EXCLUDE_SOURCES
+ * (Makefile_template:45-63).  Parameters come from dclaw_set_params()
+ * called once after set_dig.  stepgrid_dclaw_ performs b4step2 + step2 +
+ * update + src2 in place on q and returns cflgrid; aux(4..6) are written as
+ * b4step2/calc_taudir would (digclaw_module.f90:641-713) when maux >= 6. */
+int dclaw_set_params(const dclaw_params* params_host);
+void stepgrid_dclaw_(double* q, double* aux, const int* mbc, const int* mx, const int* my,
+                     const int* maux, const double* dx, const double* dy, const double* dt,
+                     double* cflgrid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

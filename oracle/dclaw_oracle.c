@@ -22,6 +22,27 @@
 #include <string.h>
 #include "dclaw_oracle.h"
 
+/* Elementary functions.  Default: the portable deterministic implementations
+ * of dcmath.h (the same source the GPU step uses), so that a CPU run and a
+ * GPU run of the scheme can agree bit for bit over thousands of steps.
+ * With -DORACLE_GLIBC: glibc libm, as a gfortran build of the reference
+ * would call; the tests use that build to bound the effect of the libm. */
+#ifdef ORACLE_GLIBC
+#define D_EXP exp
+#define D_ATAN atan
+#define D_TAN tan
+#define D_TANH tanh
+#define D_POW pow
+#else
+#define DCM_FN static inline
+#include "../dclaw_amd/csrc/dcmath.h"
+#define D_EXP dcm_exp
+#define D_ATAN dcm_atan
+#define D_TAN dcm_tan
+#define D_TANH dcm_tanh
+#define D_POW dcm_pow
+#endif
+
 #define MEQN 7
 #define I_H 1
 #define I_HU 2
@@ -100,8 +121,8 @@ static void setvars(const oracle_params* P, double h, double u, double v, double
 {
     *rho = m * (P->rho_s - P->rho_f) + P->rho_f;
     double kr_chi = 1.0;
-    if (P->segregation == 1) kr_chi = pow(10.0, P->delta_kr_order * 2.0 * (chi - 0.5));
-    *kperm = kr_chi * P->kref * exp(-((m - P->mref) / 0.04));
+    if (P->segregation == 1) kr_chi = D_POW(10.0, P->delta_kr_order * 2.0 * (chi - 0.5));
+    *kperm = kr_chi * P->kref * D_EXP(-((m - P->mref) / 0.04));
     double sig_eff = dmax(0.0, *rho * gz * h - p);
     double sig_0 = P->sigma_0;
     if (P->alphamethod == 1)
@@ -117,9 +138,9 @@ static void setvars(const oracle_params* P, double h, double u, double v, double
     if (Nnum <= 0.0) *m_eq = P->m_crit;
     else *m_eq = P->m_crit * (sqrt(Nden) / (sqrt(Nden) + sqrt(Nnum)));
     *tanpsi = P->c1 * (m - *m_eq);
-    double psi = atan(P->c1 * (m - *m_eq));
-    *tau = sig_eff * tan(P->phi + psi);
-    *tau = *tau * 0.5 * (1.0 + tanh(100.0 * (m - 0.1)));
+    double psi = D_ATAN(P->c1 * (m - *m_eq));
+    *tau = sig_eff * D_TAN(P->phi + psi);
+    *tau = *tau * 0.5 * (1.0 + D_TANH(100.0 * (m - 0.1)));
 }
 
 /* riemanntype: riemannsolvers_dclaw.f:526-637 */
@@ -1044,19 +1065,19 @@ static void mp_update_relax_Dclaw4(const oracle_params* P, double dt, double* h,
     double zeta = 3.0 * alphainv / (*h * 2.0) + (rho - rho_f) * rho_f * gz / (4.0 * rho);
     double krate = -(zeta * 2.0 * kperm / (*h * dmax(mu, 1e-16)));
     double p_eq = rho_f * *h * gz;
-    *p = p_eq + (*p - p_eq) * exp(krate * dt);
+    *p = p_eq + (*p - p_eq) * D_EXP(krate * dt);
 
     double p_exc = *p - rho_f * gz * *h;
     qfix_cmass(P, h, m, p, &rho, &p_exc, &hu, &hv, &hm, *u, *v, rhoh, gz);
     double D = -(2.0 * (kperm / (mu * *h)) * p_exc);
     krate = D * (rho - rho_f) / rhoh;
-    hm = hm * exp(-(dt * D * rho_f / (rhoh)));
+    hm = hm * D_EXP(-(dt * D * rho_f / (rhoh)));
 
     if (P->src2method == 0) {
         *h = *h + *h * krate * dt;
         hchi = *h * *chi;
-        hu = hu * exp(dt * krate);
-        hv = hv * exp(dt * krate);
+        hu = hu * D_EXP(dt * krate);
+        hv = hv * D_EXP(dt * krate);
         qfix(P, h, &hu, &hv, &hm, p, &hchi, u, v, m, chi, &rho, gz);
     } else if (P->src2method == 1) {
         *h = (rhoh - hm * (rho_s - rho_f)) / rho_f;
@@ -1097,7 +1118,7 @@ void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, doub
             double vnorm = hvnorm0 / h;
             if (hvnorm0 > 0.0) {
                 vnorm = dmax(0.0, vnorm - dt * tau / rhoh);
-                vnorm = vnorm * exp(-((1.0 - m) * 2.0 * mu * dt / (h * rhoh)));
+                vnorm = vnorm * D_EXP(-((1.0 - m) * 2.0 * mu * dt / (h * rhoh)));
                 double hvnorm = h * vnorm;
                 hu = hvnorm * hu / hvnorm0 + gx * h * dt;
                 hv = hvnorm * hv / hvnorm0;
@@ -1124,10 +1145,10 @@ void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, doub
                 if (h <= P->friction_depth) {
                     if (P->dd_manning)
                         coeffmanning = P->manning_max + (P->manning_coefficient - P->manning_max)
-                                       * 0.5 * (1.0 + tanh(300.0 * (h - 0.05)));
+                                       * 0.5 * (1.0 + D_TANH(300.0 * (h - 0.05)));
                     double beta = 1.0;
                     double gamma = beta * sqrt(hu * hu + hv * hv) * (gz * (coeffmanning * coeffmanning))
-                                   / pow(h, 7.0 / 3.0);
+                                   / D_POW(h, 7.0 / 3.0);
                     double dgamma = 1.0 + dt * gamma;
                     hu = hu / dgamma;
                     hv = hv / dgamma;

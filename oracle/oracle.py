@@ -14,6 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libdclaw_oracle.so")
+LIB_PATH_GLIBC = os.path.join(HERE, "_build", "libdclaw_oracle_glibc.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -62,15 +63,35 @@ def build() -> str:
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
+_variant = "dcmath"
+
+
+class libm:
+    """Context manager selecting the oracle build: "dcmath" (default; the
+    portable elementary functions shared with the GPU) or "glibc"."""
+
+    def __init__(self, variant: str):
+        assert variant in ("dcmath", "glibc")
+        self.variant = variant
+
+    def __enter__(self):
+        global _variant
+        self.prev = _variant
+        _variant = self.variant
+        return self
+
+    def __exit__(self, *exc):
+        global _variant
+        _variant = self.prev
 
 
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+    if _variant not in _libs:
+        path = LIB_PATH if _variant == "dcmath" else LIB_PATH_GLIBC
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         P = ctypes.POINTER(OracleParams)
         L.oracle_qfix.argtypes = [P, _dp, ctypes.c_double, _dp]
         L.oracle_setvars.argtypes = [P, _dp, _dp]
@@ -100,8 +121,8 @@ def lib():
                                  ctypes.c_double, ctypes.c_double, _ip, _dp, _dp, ctypes.c_int,
                                  _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
         L.oracle_run.restype = ctypes.c_int
-        _lib = L
-    return _lib
+        _libs[_variant] = L
+    return _libs[_variant]
 
 
 def params_from(dp) -> OracleParams:

@@ -1,0 +1,187 @@
+// strip_emu.cpp -- runs the product's strip kernel body (dclaw_strip.h) on
+// the host: 64 coroutines play the 64 lanes of one wavefront and meet at
+// every lane shuffle, so control flow and data movement are exactly the
+// GPU's.  With glibc libm this reproduces the CPU oracle's
+// arithmetic, so tests compare the two bit for bit.  Test tool only.
+#include <ucontext.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "dclaw_strip.h"
+
+using namespace dclaw;
+
+namespace {
+
+// Cooperative scheduler: the 64 lanes are coroutines run round-robin in one
+// thread.  Every lane yields once per shuffle, after publishing its value in
+// one of two buffers (alternating per shuffle), so when a lane resumes all
+// lanes have published and nobody has overwritten the buffer it reads.
+struct Sched {
+    ucontext_t main;
+    ucontext_t ctx[STRIP_LANES];
+    double buf[2][STRIP_LANES];
+    double cfl[STRIP_LANES];
+    bool done[STRIP_LANES];
+    const dclaw_params* P;
+    const StripArgs* A;
+    std::vector<char> stacks;
+};
+
+struct CoLane {
+    int lid;
+    int n;
+    Sched* s;
+    int id() const { return lid; }
+    double shfl(double v, int src)
+    {
+        const int b = n & 1;
+        n++;
+        s->buf[b][lid] = v;
+        swapcontext(&s->ctx[lid], &s->main);
+        return s->buf[b][src];
+    }
+};
+
+Sched* g_sched = nullptr;
+
+void lane_entry(int lid)
+{
+    Sched* s = g_sched;
+    CoLane L{lid, 0, s};
+    s->cfl[lid] = strip_step(L, *s->P, *s->A);
+    s->done[lid] = true;
+}
+
+void run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out)
+{
+    constexpr size_t STACK = 1 << 20;
+    Sched* s = new Sched();
+    s->P = &P;
+    s->A = &A;
+    s->stacks.resize(STACK * STRIP_LANES);
+    g_sched = s;
+    for (int l = 0; l < STRIP_LANES; l++) {
+        s->done[l] = false;
+        getcontext(&s->ctx[l]);
+        s->ctx[l].uc_stack.ss_sp = s->stacks.data() + STACK * l;
+        s->ctx[l].uc_stack.ss_size = STACK;
+        s->ctx[l].uc_link = &s->main;
+        makecontext(&s->ctx[l], (void (*)())lane_entry, 1, l);
+    }
+    bool all_done = false;
+    while (!all_done) {
+        all_done = true;
+        for (int l = 0; l < STRIP_LANES; l++) {
+            if (s->done[l]) continue;
+            swapcontext(&s->main, &s->ctx[l]);
+            if (!s->done[l]) all_done = false;
+        }
+    }
+    for (int l = 0; l < STRIP_LANES; l++) *cfl_out = dmax(*cfl_out, s->cfl[l]);
+    delete s;
+}
+
+}  // namespace
+
+extern "C" int emu_stepgrid(const dclaw_params* P, const double* q_in, double* q_out,
+                            double* aux, int maux, int write_aux, int mx, int my, double dx,
+                            double dy, double dt, int seg_rows, double* cflgrid)
+{
+    *cflgrid = 0.0;
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
+    for (int s = 0; s < nstrips; s++)
+        for (int g = 0; g < nseg; g++) {
+            StripArgs A;
+            A.q_in = q_in;
+            A.q_out = q_out;
+            A.aux = aux;
+            A.aux_out = write_aux ? aux : nullptr;
+            A.maux = maux;
+            A.mx = mx;
+            A.my = my;
+            A.dx = dx;
+            A.dy = dy;
+            A.dt = dt;
+            A.i0 = 1 + s * STRIP_W;
+            A.jA = 1 + g * seg_rows;
+            A.jB = std::min(my, A.jA + seg_rows - 1);
+            run_strip(*P, A, cflgrid);
+        }
+    return 0;
+}
+
+// rpn2 over one slice with the product's per-edge routine, in the
+// reference's array layout: q1d(7, 1-mbc:mx+mbc) (not modified: the qfix
+// chain is applied to copies), aux2(maux, ...), fwave(7, 5, ...), s(5, ...).
+extern "C" int emu_rpn2_slice(const dclaw_params* P, int ixy, int mx, int maux,
+                              const double* q1d, const double* aux2, double* fwave, double* s)
+{
+    const int n = mx + 4;
+    const int itx = (ixy == 1) ? 4 : 5;   // 0-based aux index of taudir_x / taudir_y
+    std::vector<RowCell> cells(n);
+    for (int k = 0; k < n; k++) {
+        double q7[7];
+        for (int m = 0; m < 7; m++) q7[m] = q1d[7 * k + m];
+        // the slice is already b4step2-processed: prepare_cell's qfix is then
+        // the identity on wet cells only up to rounding, so bypass it by
+        // building F1 / F2 exactly as rpn2 applies them.
+        RowCell r;
+        r.h = q7[0]; r.hu = q7[1]; r.hv = q7[2]; r.bdif = q7[6]; r.b = aux2[maux * k];
+        r.hm0 = q7[3]; r.p0 = q7[4]; r.hchi0 = q7[5];
+        double h = r.h, hu = r.hu, hv = r.hv, hm = r.hm0, p = r.p0, hchi = r.hchi0;
+        double u, v, m_, chi, rho;
+        qfix(*P, h, hu, hv, hm, p, hchi, u, v, m_, chi, rho, P->grav);
+        r.h = h; r.hu = hu; r.hv = hv;
+        r.hm1 = hm; r.p1 = p; r.hchi1 = hchi; r.m1 = m_; r.chi1 = chi; r.u = u; r.v = v;
+        qfix(*P, h, hu, hv, hm, p, hchi, u, v, m_, chi, rho, P->grav);
+        r.hm2 = hm; r.p2 = p; r.hchi2 = hchi; r.m2 = m_; r.chi2 = chi;
+        cells[k] = r;
+    }
+    for (int k = 0; k < 35 * n; k++) fwave[k] = 0.0;
+    for (int k = 0; k < 5 * n; k++) s[k] = 0.0;
+    for (int e = 1; e < n; e++) {          // edge between array cells e-1 and e
+        Side sL = side_of(cells[e - 1], e - 1 >= 1, aux2[maux * (e - 1) + itx],
+                          aux2[maux * (e - 1) + 3]);
+        Side sR = side_of(cells[e], false, aux2[maux * e + itx], aux2[maux * e + 3]);
+        if (ixy == 2) {
+            sL.u = cells[e - 1].v; sL.v = cells[e - 1].u;
+            sR.u = cells[e].v; sR.v = cells[e].u;
+        }
+        const EdgeWaves w = rpn2_edge(*P, ixy, sL, sR);
+        if (getenv("EMU_DBG") && e == atoi(getenv("EMU_DBG")))
+            fprintf(stderr, "L h%.17g hu%.17g hv%.17g hm%.17g p%.17g hc%.17g u%.17g v%.17g m%.17g chi%.17g b%.17g t%.17g fs%.17g\nR h%.17g hu%.17g hv%.17g hm%.17g p%.17g hc%.17g u%.17g v%.17g m%.17g chi%.17g b%.17g t%.17g fs%.17g\n",
+                sL.h, sL.hu, sL.hv, sL.hm, sL.p, sL.hchi, sL.u, sL.v, sL.m, sL.chi, sL.b, sL.taudir, sL.fs,
+                sR.h, sR.hu, sR.hv, sR.hm, sR.p, sR.hchi, sR.u, sR.v, sR.m, sR.chi, sR.b, sR.taudir, sR.fs);
+        double* sv = s + 5 * e;
+        sv[0] = w.s[0]; sv[1] = w.s[1]; sv[2] = w.s[1]; sv[3] = w.s[1]; sv[4] = w.s[2];
+        double* f = fwave + 35 * e;
+        const int nrm = (ixy == 1) ? 1 : 2;
+        for (int m = 0; m < 6; m++) { f[0 * 7 + m] = w.w1[m]; f[4 * 7 + m] = w.w5[m]; }
+        f[1 * 7 + 0] = w.w2h;
+        f[1 * 7 + nrm] = w.w2n;
+    }
+    return 0;
+}
+
+// One rpn2 edge from explicit side states (in[0..12] left, in[13..25] right:
+// h hu hv hm p hchi u v m chi b taudir fs), outputs s[3] and w1, w5 (q-index).
+extern "C" int emu_rpn2_edge(const dclaw_params* P, int ixy, const double* in, double* out)
+{
+    Side S[2];
+    for (int k = 0; k < 2; k++) {
+        const double* x = in + 13 * k;
+        S[k].h = x[0]; S[k].hu = x[1]; S[k].hv = x[2]; S[k].hm = x[3]; S[k].p = x[4];
+        S[k].hchi = x[5]; S[k].u = x[6]; S[k].v = x[7]; S[k].m = x[8]; S[k].chi = x[9];
+        S[k].b = x[10]; S[k].taudir = x[11]; S[k].fs = x[12];
+    }
+    const EdgeWaves w = rpn2_edge(*P, ixy, S[0], S[1]);
+    for (int k = 0; k < 3; k++) out[k] = w.s[k];
+    for (int k = 0; k < 6; k++) { out[3 + k] = w.w1[k]; out[9 + k] = w.w5[k]; }
+    out[15] = w.w2h; out[16] = w.w2n;
+    return 0;
+}
