@@ -1,0 +1,166 @@
+"""Benchmark of the fused gfx950 D-Claw step.
+
+Default workload: the plain drop-in GPU step at scale -- one 4096x4096 patch,
+dx = 2 m, seeded fractal bed (spectral slope -2, 200 m relief) on a 20 deg
+tilt, 64 seeded elliptical sources, D-Claw material parameters of the
+200x200 case; FP64.  A step is one device-resident pass of the hot path:
+ghost fill (bc2amr extrap), the fused stepgrid (b4step2, x/y sweeps with
+rpn2/rpt2 and MC limiter, update, src2) and the device dt rule.
+
+Prints one JSON line (rank 0) with the throughput in interior cell-updates
+per second, the roofline of the fused step kernel (HBM, algorithmic bytes
+120 B per cell-update: 7 q in + 1 bed in + 7 q out, doubles) timed with HIP
+events on the stream the kernels run on, and a CPU baseline: the oracle
+(scalar C port of the reference algorithm, one core) on a bounded sample of
+the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s fwd+adjoint FP64, batched vs plain GPU, at 1e-9 rel vs Fortran"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_FWD = 8 * (7 + 1 + 7)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=4096, help="patch side (cells)")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="steps of the CPU baseline sample (0 disables it)")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="after the timed steps, run one plain 1 GiB device copy (the HBM "
+                         "counter calibration dispatch for rocprofv3 --pmc runs)")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the path is sharded by "
+                                   "member (ensembles) or by strips with a 2-3 cell halo "
+                                   "exchange (single patch) on more GPUs -- not built here",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    _lib.lib()
+    n = args.n
+    t0 = time.time()
+    pt = W.fractal_sources(n, n, dx=2.0, seed=args.seed)
+    print(f"[bench] workload {n}x{n} built in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+
+    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+    r.set_dt(1e-3)
+    r.cfl.zero_()
+    cfl_desired, dt_max = 0.9, 1e9
+
+    def one_step(ev=None):
+        S.bc_fill(r.qa, r.mx, r.my, r.bc)
+        if ev is not None:
+            ev[0].record()
+        S.stepgrid(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.dt, r.params, r.cfl)
+        if ev is not None:
+            ev[1].record()
+        r.qa, r.qb = r.qb, r.qa
+        S.dt_update(r.dt, r.cfl, cfl_desired, dt_max)
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        one_step(evs[k])
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    kern_ms = [a.elapsed_time(b) for a, b in evs]
+    kern_avg_s = sum(kern_ms) / len(kern_ms) / 1e3
+
+    if args.copy_calib:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    cells = r.mx * r.my
+    value = cells * args.steps / t_total
+    q = r.q()
+    wet = int((q[0, 2:-2, 2:-2] > pt.params.dry_tolerance).sum())
+    achieved = BYTES_PER_CELL_FWD * cells / kern_avg_s / 1e9
+    print(f"[bench] {args.steps} steps {t_total:.3f}s, kernel avg {kern_avg_s * 1e3:.3f} ms, "
+          f"wet cells {wet} ({100.0 * wet / cells:.2f}%)", file=sys.stderr, flush=True)
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"k_stepgrid_fractal{n}_traffic.txt")
+    if os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith("traffic_bytes_per_launch"):
+                traffic = float(line.split("=")[1])
+
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        P = O.params_from(pt.params)
+        c0 = time.perf_counter()
+        O.run(P, pt.q, pt.aux, pt.dx, pt.dy, args.cpu_steps, 1e-3, cfl_desired, dt_max)
+        c_s = time.perf_counter() - c0
+        cpu = {"value": cells * args.cpu_steps / c_s, "unit": "cell-updates/s", "cores": 1,
+               "kind": "port",
+               "sample": f"first {args.cpu_steps} steps of the same {n}x{n} workload "
+                         f"({c_s:.1f}s on one host core, scalar C oracle, gcc -O2)"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain drop-in GPU step, single {n}x{n} patch, dx=2 m, "
+                               f"seeded fractal bed + 20 deg tilt, 64 elliptical sources, "
+                               f"forward only (adjoint not built)",
+                   "mx": n, "my": n, "seed": args.seed, "cfl_desired": cfl_desired,
+                   "wet_cells_end": wet},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_stepgrid", "kernel_avg_ms": kern_avg_s * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

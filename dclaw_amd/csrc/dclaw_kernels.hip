@@ -45,6 +45,7 @@ int hip_fail(hipError_t e, const char* what)
 struct GpuLane {
     __device__ __forceinline__ int id() const { return (int)threadIdx.x; }
     __device__ __forceinline__ double shfl(double v, int src) const { return __shfl(v, src, 64); }
+    __device__ __forceinline__ bool any(bool p) const { return __any((int)p) != 0; }
 };
 
 __device__ __forceinline__ void wave_max_to(double v, double* dst)
@@ -181,7 +182,7 @@ struct HostCtx {
     double* d_qa = nullptr;
     double* d_qb = nullptr;
     double* d_aux = nullptr;
-    size_t cap_q = 0, cap_aux = 0;
+    size_t cap_qa = 0, cap_qb = 0, cap_aux = 0;
 };
 HostCtx g_host;
 
@@ -354,12 +355,8 @@ static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int ma
     if (!g_host.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
     const size_t na = (size_t)maux * (mx + 4) * (my + 4);
-    if ((rc = grow(&g_host.d_qa, &g_host.cap_q, nq))) return rc;
-    size_t capb = g_host.cap_q;
-    if (!g_host.d_qb || capb < nq) {
-        if (g_host.d_qb) (void)hipFree(g_host.d_qb);
-        HIP_TRY(hipMalloc(&g_host.d_qb, nq * sizeof(double)));
-    }
+    if ((rc = grow(&g_host.d_qa, &g_host.cap_qa, nq))) return rc;
+    if ((rc = grow(&g_host.d_qb, &g_host.cap_qb, nq))) return rc;
     if ((rc = grow(&g_host.d_aux, &g_host.cap_aux, na))) return rc;
     double scal[2] = {dt, 0.0};
     HIP_TRY(hipMemcpy(g_host.d_qa, q, nq * sizeof(double), hipMemcpyHostToDevice));

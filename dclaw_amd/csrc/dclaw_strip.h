@@ -312,6 +312,53 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             }
         }
 
+        // A window whose rows R-4..R are dry in every lane of the strip has
+        // every flux, wave, transverse term and taudir exactly +0 (both-dry
+        // edges are skipped by rpn2, dry cells by rpt2 and src2, and the
+        // wet-dry gate zeroes the corrections), so the row reduces to
+        // writing the dry state; the decision is uniform across the wave.
+        const bool wet_here = (q0.h > 0.0) || (q1.h > 0.0) || (q2.h > 0.0) || (q3.h > 0.0) ||
+                              (h_r4 > 0.0);
+        if (!L.any(wet_here)) {
+            if (A.aux_out) {
+                const int r = R - 1;
+                if (r >= 0 && r <= my + 1 && col_taudir && r >= A.jA - 2 && r <= A.jB + 2) {
+                    const bool row_own = (r >= A.jA && r <= A.jB) || (r == 0 && A.jA == 1) ||
+                                         (r == my + 1 && A.jB == my);
+                    const bool col_own =
+                        (c >= A.i0 && c <= A.i0 + STRIP_W - 1 && c >= 1 && c <= mx) ||
+                        (c == 0 && A.i0 == 1) || (c == mx + 1 && A.i0 + STRIP_W - 1 >= mx);
+                    if (row_own && col_own) {
+                        const size_t o = aidx(c, r, mx, A.maux);
+                        A.aux_out[o + 3] = 0.0;
+                        A.aux_out[o + 4] = 0.0;
+                        A.aux_out[o + 5] = 0.0;
+                    }
+                }
+            }
+            ta2 = ta1;
+            ta1.fsphi = ta1.tx = ta1.ty = 0.0;
+            const int j = R - 3;
+            if (j >= A.jA && j <= A.jB && col_out) {
+                const size_t o = qidx(0, c, j, mx);
+                for (int z = 0; z < 6; z++) A.q_out[o + z] = 0.0;
+                A.q_out[o + 6] = q3.bdif;
+            }
+            for (int z = 0; z < 3; z++) { ydotl[z] = 0.0; yw_pend.s[z] = 0.0; }
+            for (int z = 0; z < 6; z++) {
+                yw_pend.w1[z] = 0.0;
+                yw_pend.w5[z] = 0.0;
+                gy_imp2_m[z] = gy_imp2_p[z] = 0.0;
+                fxm_prev[z] = fxp_prev[z] = 0.0;
+                gx_prev2[z] = gp_prev[z] = 0.0;
+            }
+            yw_pend.w2h = yw_pend.w2n = 0.0;
+            h_r4 = q3.h;
+            q3 = q2;
+            q2 = q1;
+            q1 = q0;
+            continue;
+        }
         // ---------------- S2: calc_taudir for row R-1
         {
             const int r = R - 1;

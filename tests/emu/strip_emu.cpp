@@ -44,6 +44,16 @@ struct CoLane {
         swapcontext(&s->ctx[lid], &s->main);
         return s->buf[b][src];
     }
+    bool any(bool p)
+    {
+        const int b = n & 1;
+        n++;
+        s->buf[b][lid] = p ? 1.0 : 0.0;
+        swapcontext(&s->ctx[lid], &s->main);
+        for (int l = 0; l < STRIP_LANES; l++)
+            if (s->buf[b][l] != 0.0) return true;
+        return false;
+    }
 };
 
 Sched* g_sched = nullptr;
