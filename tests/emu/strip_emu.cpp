@@ -195,3 +195,20 @@ extern "C" int emu_rpn2_edge(const dclaw_params* P, int ixy, const double* in, d
     out[15] = w.w2h; out[16] = w.w2n;
     return 0;
 }
+
+// Elementary functions of dcmath.h over an array (fn: 0 exp, 1 atan, 2 tan,
+// 3 tanh, 4 log, 5 pow(x, y2[k])) -- for the accuracy tests.
+extern "C" int emu_dcm(int fn, const double* x, const double* y2, double* out, int n)
+{
+    for (int k = 0; k < n; k++) {
+        switch (fn) {
+        case 0: out[k] = dcm_exp(x[k]); break;
+        case 1: out[k] = dcm_atan(x[k]); break;
+        case 2: out[k] = dcm_tan(x[k]); break;
+        case 3: out[k] = dcm_tanh(x[k]); break;
+        case 4: out[k] = dcm_log(x[k]); break;
+        default: out[k] = dcm_pow(x[k], y2[k]); break;
+        }
+    }
+    return 0;
+}

@@ -1,0 +1,64 @@
+"""The C-ABI library: it loads, exports every entry point include/dclaw_hip.h
+declares, and rejects bad arguments before touching a device (no GPU here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "dclaw_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*([a-z_][a-z0-9_]*)\s*\(",
+                       src, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from dclaw_amd import _lib
+    return _lib.lib()
+
+
+def test_every_declared_symbol_is_exported(L):
+    names = declared_functions()
+    assert "dclaw_stepgrid_dev" in names and "stepgrid_dclaw_" in names
+    from dclaw_amd import _lib
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.SIGNATURES, f"{n} has no ctypes signature"
+
+
+def test_version_and_width(L):
+    assert L.dclaw_version() == 1
+    assert L.dclaw_strip_width() == 58
+
+
+def test_argument_validation(L):
+    from dclaw_amd import _lib
+    rc = L.dclaw_stepgrid_dev(None, None, None, 1, 10, 10, 1.0, 1.0, None, None, None, None)
+    assert rc == -1 and b"null" in L.dclaw_last_error()
+    rc = L.dclaw_stepgrid_dev(1, 1, 2, 1, 10, 10, 1.0, 1.0, 3, 4, 5, None)
+    assert rc == -1 and b"alias" in L.dclaw_last_error()
+    rc = L.dclaw_stepgrid_dev(1, 2, 3, 1, 0, 10, 1.0, 1.0, 3, 4, 5, None)
+    assert rc == -1
+    bc = (ctypes.c_int * 4)(1, 1, 2, 1)
+    assert L.dclaw_bc_fill(1, 10, 10, bc, None) == -1       # periodic is not supported
+    mx = (ctypes.c_int * 3)(64, 200, 513)
+    my = (ctypes.c_int * 3)(64, 100, 65)
+    assert L.dclaw_worklist_size(mx, my, 3, 32) == 2 * 2 + 4 * 4 + 9 * 3
+    tiles = (ctypes.c_int32 * (4 * 47))()
+    assert L.dclaw_worklist_build(mx, my, 3, 32, tiles) == 47
+    assert tiles[0] == 2            # largest patch first
+    with pytest.raises(_lib.DClawError):
+        _lib.check(L.dclaw_worklist_size(mx, my, 3, 0))
+
+
+def test_params_struct_layout():
+    from dclaw_amd.params import CParams
+    from oracle.oracle import OracleParams
+    assert [f[0] for f in CParams._fields_] == [f[0] for f in OracleParams._fields_]
+    assert ctypes.sizeof(CParams) == 20 * 8 + 13 * 4 + 4

@@ -369,3 +369,19 @@ def test_dam_front_bound(O):
         front = x[wet].max()
         assert front <= 5.0 + 2 * math.sqrt(9.81 * 2.0) * t + 0.1
     assert front > 5.5          # and it does move
+
+
+def test_bc_fill_rules(O):
+    """Ghost cells: extrap copies the nearest interior cell; wall mirrors it
+    with the normal momentum negated (bc2amr)."""
+    rng = np.random.default_rng(4)
+    q = np.asfortranarray(rng.standard_normal((7, 9 + 4, 6 + 4)))
+    e = O.bc_fill(q, (1, 1, 1, 1))
+    I = slice(2, -2)
+    assert np.array_equal(e[:, 0, I], q[:, 2, I]) and np.array_equal(e[:, 1, I], q[:, 2, I])
+    assert np.array_equal(e[:, -1, I], q[:, -3, I]) and np.array_equal(e[:, I, 0], e[:, I, 2])
+    w = O.bc_fill(q, (3, 3, 3, 3))
+    assert np.array_equal(w[0, 1, I], q[0, 2, I]) and np.array_equal(w[0, 0, I], q[0, 3, I])
+    assert np.array_equal(w[1, 1, I], -q[1, 2, I]) and np.array_equal(w[2, 1, I], q[2, 2, I])
+    assert np.array_equal(w[2, I, -2], -w[2, I, -3]) and np.array_equal(w[1, I, -2], w[1, I, -3])
+    assert w[1, 0, 0] == q[1, 3, 3] * -1.0 and w[2, 0, 0] == -q[2, 3, 3]
