@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--workload", choices=["patch", "ensemble"], default="patch",
+                    help="patch: one 4096^2 patch (default); ensemble: 256 members with sides "
+                         "log-uniform in [64, 512], batched launch vs plain per-member loop")
+    ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy (the HBM "
                          "counter calibration dispatch for rocprofv3 --pmc runs)")
@@ -64,6 +68,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     _lib.lib()
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
     n = args.n
     t0 = time.time()
     pt = W.fractal_sources(n, n, dx=2.0, seed=args.seed)
@@ -158,6 +164,76 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_stepgrid", "kernel_avg_ms": kern_avg_s * 1e3},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_ensemble(args):
+    """Batched ensemble: all members advanced by one dclaw_step_batched launch
+    per step (boundary conditions on load), against the plain per-member loop
+    (one bound + stepgrid launch pair per member per step)."""
+    import torch
+
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+
+    t0 = time.time()
+    members = W.ensemble(args.members, 64, 512, seed=args.seed)
+    ens = S.Ensemble(members, seg_rows=32)
+    cells = sum(m.patch.mx * m.patch.my for m in members)
+    print(f"[bench] ensemble of {len(members)} members, {cells} cells, built in "
+          f"{time.time() - t0:.1f}s, {ens.ntiles} tiles", file=sys.stderr, flush=True)
+    ens.dt.fill_(1e-3)
+    ens.cfl.zero_()
+
+    def run(nsteps, plain, ev=None):
+        for k in range(nsteps):
+            if ev is not None:
+                ev[k][0].record()
+            if plain:
+                ens.step_plain()
+            else:
+                ens.step()
+            if ev is not None:
+                ev[k][1].record()
+            S.dt_update(ens.dt, ens.cfl, 0.9, 1e9)
+
+    run(args.warmup, False)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t_start = time.perf_counter()
+    run(args.steps, False, evs)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t_start
+    kern_avg_s = sum(a.elapsed_time(b) for a, b in evs) / len(evs) / 1e3
+    nplain = max(2, args.steps // 5)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    run(nplain, True)
+    torch.cuda.synchronize()
+    t_p = time.perf_counter() - t_start
+    value = cells * args.steps / t_b
+    plain = cells * nplain / t_p
+    achieved = BYTES_PER_CELL_FWD * cells / kern_avg_s / 1e9
+    print(f"[bench] batched {t_b / args.steps * 1e3:.3f} ms/step, plain "
+          f"{t_p / nplain * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_b / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batched ensemble forward, {len(members)} members, sides "
+                               f"log-uniform in [64,512], per-member phi/m0/kref, one launch per "
+                               f"step (adjoint not built)", "members": len(members),
+                   "cells": cells, "seed": args.seed},
+        "plain_loop": {"value": plain, "steps": nplain,
+                       "ms_per_step": t_p / nplain * 1e3},
+        "batched_over_plain": value / plain,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step_batched", "kernel_avg_ms": kern_avg_s * 1e3},
+        "cpu_baseline": None,
     }
     print(json.dumps(out))
 

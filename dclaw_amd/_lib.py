@@ -50,11 +50,14 @@ SIGNATURES = {
     "dclaw_stepgrid_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                           _vp, _vp, _vp, _vp]),
+    "dclaw_stepgrid_bc_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             _ip, _vp, _vp, _vp, _vp]),
     "dclaw_worklist_size": (ctypes.c_int, [_ip, _ip, ctypes.c_int, ctypes.c_int]),
     "dclaw_worklist_build": (ctypes.c_int, [_ip, _ip, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int32)]),
     "dclaw_step_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
-                                          _vp, _vp, _vp, _vp]),
+                                          _ip, _vp, _vp, _vp, _vp]),
     "dclaw_dt_update": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, _vp]),
     "dclaw_set_params": (ctypes.c_int, [ctypes.POINTER(CParams)]),

@@ -65,10 +65,15 @@ __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in
                                                  double dx, double dy,
                                                  const double* __restrict__ dt_dev,
                                                  const dclaw_params* __restrict__ params,
-                                                 double* cfl_dev, int seg_rows)
+                                                 double* cfl_dev, int seg_rows, int bx0, int bx1,
+                                                 int by0, int by1)
 {
     const dclaw_params P = *params;
     StripArgs A;
+    A.bc[0] = bx0;
+    A.bc[1] = bx1;
+    A.bc[2] = by0;
+    A.bc[3] = by1;
     A.q_in = q_in;
     A.q_out = q_out;
     A.aux = aux;
@@ -92,7 +97,8 @@ __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restri
                                                      int seg_rows,
                                                      const double* __restrict__ dt_dev,
                                                      const dclaw_params* __restrict__ params,
-                                                     double* cfl_dev)
+                                                     double* cfl_dev, int bx0, int bx1, int by0,
+                                                     int by1)
 {
     const int t = (int)blockIdx.x;
     const int pidx = tiles[4 * t + 0];
@@ -103,6 +109,10 @@ __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restri
     A.q_out = pt.q_out;
     A.aux = pt.aux;
     A.aux_out = nullptr;
+    A.bc[0] = bx0;
+    A.bc[1] = bx1;
+    A.bc[2] = by0;
+    A.bc[3] = by1;
     A.maux = pt.maux;
     A.mx = pt.mx;
     A.my = pt.my;
@@ -129,18 +139,10 @@ __global__ void k_bc_fill(double* q, int mx, int my, int bxl, int bxh, int byl, 
         const int ii = (int)(t % nx) - 1;   // Fortran i in [-1, mx+2]
         const int jj = (int)(t / nx) - 1;
         if (ii >= 1 && ii <= mx && jj >= 1 && jj <= my) continue;
-        int si = ii, sj = jj;
-        double fu = 1.0, fv = 1.0;
-        if (ii < 1) {
-            if (bxl == DCLAW_BC_WALL) { si = 1 - ii; fu = -1.0; } else si = 1;
-        } else if (ii > mx) {
-            if (bxh == DCLAW_BC_WALL) { si = 2 * mx + 1 - ii; fu = -1.0; } else si = mx;
-        }
-        if (jj < 1) {
-            if (byl == DCLAW_BC_WALL) { sj = 1 - jj; fv = -1.0; } else sj = 1;
-        } else if (jj > my) {
-            if (byh == DCLAW_BC_WALL) { sj = 2 * my + 1 - jj; fv = -1.0; } else sj = my;
-        }
+        const int bc[4] = {bxl, bxh, byl, byh};
+        int si, sj;
+        double fu, fv;
+        bc_source(bc, mx, my, ii, jj, si, sj, fu, fv);
         const size_t src = qidx(0, si, sj, mx), dst = qidx(0, ii, jj, mx);
         for (int m = 0; m < 7; m++) {
             double val = q[src + m];
@@ -235,11 +237,21 @@ int dclaw_bc_fill(double* q, int mx, int my, const int* bc, void* stream)
     return DCLAW_OK;
 }
 
+static int check_bc(const int* bc)
+{
+    if (!bc) return DCLAW_OK;
+    for (int k = 0; k < 4; k++)
+        if (bc[k] != DCLAW_BC_EXTRAP && bc[k] != DCLAW_BC_WALL)
+            return fail(DCLAW_EINVAL, "bc codes must be 1 (extrap) or 3 (wall)");
+    return DCLAW_OK;
+}
+
 static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
                            double* aux_out, int maux, int mx, int my, double dx, double dy,
                            const double* dt_dev, const dclaw_params* params_dev,
-                           double* cfl_dev, hipStream_t stream)
+                           double* cfl_dev, const int* bc, hipStream_t stream)
 {
+    if (int rcb = check_bc(bc)) return rcb;
     int rc = check_patch(mx, my, maux);
     if (rc) return rc;
     if (!q_in || !q_out || !aux || !dt_dev || !params_dev || !cfl_dev)
@@ -249,9 +261,10 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
     const int nseg = (my + DEFAULT_SEG_ROWS - 1) / DEFAULT_SEG_ROWS;
     if (nseg > 65535) return fail(DCLAW_EINVAL, "my too large for one launch");
+    const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
     hipLaunchKernelGGL(k_stepgrid, dim3(nstrips, nseg), dim3(64), 0, stream, q_in, q_out, aux,
                        aux_out, maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev,
-                       DEFAULT_SEG_ROWS);
+                       DEFAULT_SEG_ROWS, b0, b1, b2, b3);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }
@@ -261,7 +274,17 @@ int dclaw_stepgrid_dev(const double* q_in, double* q_out, const double* aux, int
                        const dclaw_params* params_dev, double* cfl_dev, void* stream)
 {
     return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
-                           cfl_dev, (hipStream_t)stream);
+                           cfl_dev, nullptr, (hipStream_t)stream);
+}
+
+int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, int maux,
+                          int mx, int my, double dx, double dy, const int* bc,
+                          const double* dt_dev, const dclaw_params* params_dev,
+                          double* cfl_dev, void* stream)
+{
+    if (!bc) return fail(DCLAW_EINVAL, "null bc");
+    return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
+                           cfl_dev, bc, (hipStream_t)stream);
 }
 
 int dclaw_worklist_size(const int* mx, const int* my, int npatch, int seg_rows)
@@ -306,14 +329,17 @@ int dclaw_worklist_build(const int* mx, const int* my, int npatch, int seg_rows,
 }
 
 int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
-                       int ntiles, int seg_rows, const double* dt_dev,
+                       int ntiles, int seg_rows, const int* bc, const double* dt_dev,
                        const dclaw_params* params_dev, double* cfl_dev, void* stream)
 {
     if (!patches_dev || !tiles_dev || !dt_dev || !params_dev || !cfl_dev)
         return fail(DCLAW_EINVAL, "null pointer");
     if (npatch < 1 || ntiles < 1 || seg_rows < 1) return fail(DCLAW_EINVAL, "empty batch");
+    if (int rcb = check_bc(bc)) return rcb;
+    const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
     hipLaunchKernelGGL(k_step_batched, dim3(ntiles), dim3(64), 0, (hipStream_t)stream,
-                       patches_dev, tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev);
+                       patches_dev, tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev, b0, b1, b2,
+                       b3);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }
@@ -363,7 +389,8 @@ static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int ma
     HIP_TRY(hipMemcpy(g_host.d_aux, aux, na * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
     rc = launch_stepgrid(g_host.d_qa, g_host.d_qb, g_host.d_aux, g_host.d_aux, maux, mx, my, dx,
-                         dy, g_host.d_scalars, g_host.d_params, g_host.d_scalars + 1, nullptr);
+                         dy, g_host.d_scalars, g_host.d_params, g_host.d_scalars + 1, nullptr,
+                         nullptr);
     if (rc) return rc;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(q, g_host.d_qb, nq * sizeof(double), hipMemcpyDeviceToHost));

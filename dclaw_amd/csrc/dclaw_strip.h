@@ -37,7 +37,28 @@ struct StripArgs {
     int maux, mx, my;
     double dx, dy, dt;
     int i0, jA, jB;        // interior strip start column, segment rows [jA, jB]
+    int bc[4];             // bc codes (xlo, xhi, ylo, yhi) applied on load; 0 = use stored ghosts
 };
+
+// Source cell and momentum signs of a ghost cell under the bc2amr rules
+// (extrap: nearest interior cell; wall: mirror, normal momentum negated).
+// Identical to what k_bc_fill writes, so loading through this map equals
+// filling the ghosts first.
+DC_HD void bc_source(const int* bc, int mx, int my, int i, int j, int& si, int& sj, double& fu,
+                     double& fv)
+{
+    si = i; sj = j; fu = 1.0; fv = 1.0;
+    if (i < 1) {
+        if (bc[0] == DCLAW_BC_WALL) { si = 1 - i; fu = -1.0; } else si = 1;
+    } else if (i > mx) {
+        if (bc[1] == DCLAW_BC_WALL) { si = 2 * mx + 1 - i; fu = -1.0; } else si = mx;
+    }
+    if (j < 1) {
+        if (bc[2] == DCLAW_BC_WALL) { sj = 1 - j; fv = -1.0; } else sj = 1;
+    } else if (j > my) {
+        if (bc[3] == DCLAW_BC_WALL) { sj = 2 * my + 1 - j; fv = -1.0; } else sj = my;
+    }
+}
 
 // Row state of one column in the rolling window.
 struct RowCell {
@@ -294,8 +315,19 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             double b = 0.0;
             const bool row_in = (R >= -1 && R <= my + 2);
             if (row_in && col_in_array) {
-                const size_t o = qidx(0, c, R, mx);
-                for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
+                const bool ghost = (c < 1 || c > mx || R < 1 || R > my);
+                if (ghost && A.bc[0] != 0) {
+                    int si, sj;
+                    double fu, fv;
+                    bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
+                    const size_t o = qidx(0, si, sj, mx);
+                    for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
+                    q7[1] = fu * q7[1];
+                    q7[2] = fv * q7[2];
+                } else {
+                    const size_t o = qidx(0, c, R, mx);
+                    for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
+                }
                 b = A.aux[aidx(c, R, mx, A.maux)];
             }
             prepare_cell(P, q7, b, q0);

@@ -74,6 +74,16 @@ def stepgrid(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: i
         _ptr(params_dev), _ptr(cfl_dev), _stream()))
 
 
+def stepgrid_bc(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int, mx: int,
+                my: int, dx: float, dy: float, bc, dt_dev: torch.Tensor,
+                params_dev: torch.Tensor, cfl_dev: torch.Tensor) -> None:
+    """stepgrid with the boundary conditions applied on load (no bound pass)."""
+    arr = (ctypes.c_int * 4)(*bc)
+    _lib.check(_lib.lib().dclaw_stepgrid_bc_dev(
+        _ptr(q_in), _ptr(q_out), _ptr(aux), maux, mx, my, dx, dy, arr, _ptr(dt_dev),
+        _ptr(params_dev), _ptr(cfl_dev), _stream()))
+
+
 def dt_update(dt_dev: torch.Tensor, cfl_dev: torch.Tensor, cfl_desired: float,
               dt_max: float) -> None:
     _lib.check(_lib.lib().dclaw_dt_update(_ptr(dt_dev), _ptr(cfl_dev), dt_dev.numel(),
@@ -99,19 +109,24 @@ class PatchRunner:
     def set_dt(self, dt: float) -> None:
         self.dt.fill_(float(dt))
 
-    def step(self) -> None:
-        """bound, stepgrid; the result is left in self.qa and cflgrid in self.cfl."""
-        bc_fill(self.qa, self.mx, self.my, self.bc)
-        stepgrid(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy, self.dt,
-                 self.params, self.cfl)
+    def step(self, fused_bc: bool = True) -> None:
+        """bound + stepgrid (boundary conditions applied on load when fused_bc);
+        the result is left in self.qa and cflgrid in self.cfl."""
+        if fused_bc:
+            stepgrid_bc(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy,
+                        self.bc, self.dt, self.params, self.cfl)
+        else:
+            bc_fill(self.qa, self.mx, self.my, self.bc)
+            stepgrid(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy, self.dt,
+                     self.params, self.cfl)
         self.qa, self.qb = self.qb, self.qa
 
     def run(self, nsteps: int, dt0: float, cfl_desired: float, dt_max: float,
-            cfl_hist: Optional[torch.Tensor] = None) -> None:
+            cfl_hist: Optional[torch.Tensor] = None, fused_bc: bool = True) -> None:
         self.set_dt(dt0)
         self.cfl.zero_()
         for n in range(nsteps):
-            self.step()
+            self.step(fused_bc)
             if cfl_hist is not None:
                 cfl_hist[n].copy_(self.cfl[0])
             dt_update(self.dt, self.cfl, cfl_desired, dt_max)
@@ -134,6 +149,7 @@ class Ensemble:
         self.qb = [torch.empty_like(t) for t in self.qa]
         self.bed = [bed_tensor(m.patch.aux, device) for m in members]
         self.params = params_tensor([m.patch.params for m in members], device)
+        self._prec = ctypes.sizeof(CParams)
         self.dt = torch.zeros(self.n, dtype=torch.float64, device=device)
         self.cfl = torch.zeros(self.n, dtype=torch.float64, device=device)
         L = _lib.lib()
@@ -164,22 +180,38 @@ class Ensemble:
         return self.qb[k] if self.flip else self.qa[k]
 
     def step(self) -> None:
+        """One launch for all members; boundary conditions applied on load."""
         L = _lib.lib()
-        for k in range(self.n):
-            bc_fill(self.current(k), self.mx[k], self.my[k], self.bc)
+        bc = (ctypes.c_int * 4)(*self.bc)
         _lib.check(L.dclaw_step_batched(
             _ptr(self._patches[1 if self.flip else 0]), self.n, _ptr(self.tiles), self.ntiles,
-            self.seg_rows, _ptr(self.dt), _ptr(self.params), _ptr(self.cfl), _stream()))
+            self.seg_rows, bc, _ptr(self.dt), _ptr(self.params), _ptr(self.cfl), _stream()))
         self.flip = not self.flip
 
-    def run(self, nsteps: int, dt0, cfl_desired: float, dt_max: float) -> None:
+    def step_plain(self) -> None:
+        """The plain per-member loop: one bound + stepgrid launch pair per member
+        (the reference's one-stepgrid-call-per-patch structure), for comparison."""
+        for k in range(self.n):
+            qi = self.current(k)
+            qo = self.qa[k] if self.flip else self.qb[k]
+            bc_fill(qi, self.mx[k], self.my[k], self.bc)
+            stepgrid(qi, qo, self.bed[k], 1, self.mx[k], self.my[k], self.dx[k], self.dy[k],
+                     self.dt[k:k + 1], self.params[k * self._prec:(k + 1) * self._prec],
+                     self.cfl[k:k + 1])
+        self.flip = not self.flip
+
+    def run(self, nsteps: int, dt0, cfl_desired: float, dt_max: float,
+            plain: bool = False) -> None:
         if np.isscalar(dt0):
             self.dt.fill_(float(dt0))
         else:
             self.dt.copy_(torch.as_tensor(dt0, dtype=torch.float64))
         self.cfl.zero_()
         for _ in range(nsteps):
-            self.step()
+            if plain:
+                self.step_plain()
+            else:
+                self.step()
             dt_update(self.dt, self.cfl, cfl_desired, dt_max)
 
     def q(self, k: int) -> np.ndarray:

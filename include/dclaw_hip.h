@@ -75,8 +75,8 @@ typedef struct dclaw_patch {
 const char* dclaw_last_error(void);
 int dclaw_version(void);
 
-/* Number of workgroups the fused step launches for one patch, and the strip
- * width / segment height used (for tests and roofline accounting). */
+/* Interior columns per strip of the fused step (one wavefront per strip
+ * and row segment). */
 int dclaw_strip_width(void);
 
 /* Ghost-cell fill of q (bc2amr extrap / wall; bc[4] = xlo, xhi, ylo, yhi),
@@ -94,15 +94,27 @@ int dclaw_stepgrid_dev(const double* q_in, double* q_out, const double* aux, int
                        int mx, int my, double dx, double dy, const double* dt_dev,
                        const dclaw_params* params_dev, double* cfl_dev, void* stream);
 
+/* The same step with the physical boundary conditions applied on load
+ * (bc[4] host array: xlo, xhi, ylo, yhi; 1 extrap, 3 wall): ghost cells of
+ * q_in are not read, so no separate bound/bc2amr pass is needed.  Result is
+ * identical to dclaw_bc_fill followed by dclaw_stepgrid_dev. */
+int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, int maux,
+                          int mx, int my, double dx, double dy, const int* bc,
+                          const double* dt_dev, const dclaw_params* params_dev,
+                          double* cfl_dev, void* stream);
+
 /* Batched fused step over npatch patches (device array of dclaw_patch).
  * One launch; the work list of (patch, strip, segment) tiles is built on
  * the host by dclaw_worklist_size / dclaw_worklist_build into device memory.
- * params_dev, dt_dev, cfl_dev are indexed by dclaw_patch.member. */
+ * params_dev, dt_dev, cfl_dev are indexed by dclaw_patch.member.  bc (host
+ * array of 4 codes, or NULL to read stored ghosts) is applied on load for
+ * every patch.  No reference counterpart: the reference advances patches
+ * one stepgrid call at a time (OpenMP over patches in GeoClaw advanc). */
 int dclaw_worklist_size(const int* mx, const int* my, int npatch, int seg_rows);
 int dclaw_worklist_build(const int* mx, const int* my, int npatch, int seg_rows,
                          int32_t* tiles_host /* 4*size ints */);
 int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
-                       int ntiles, int seg_rows, const double* dt_dev,
+                       int ntiles, int seg_rows, const int* bc, const double* dt_dev,
                        const dclaw_params* params_dev, double* cfl_dev, void* stream);
 
 /* Device dt rule for n members: dt[k] <- min(dt[k]*cfl_desired/cfl[k], dt_max)

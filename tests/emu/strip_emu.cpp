@@ -99,7 +99,7 @@ void run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out)
 
 extern "C" int emu_stepgrid(const dclaw_params* P, const double* q_in, double* q_out,
                             double* aux, int maux, int write_aux, int mx, int my, double dx,
-                            double dy, double dt, int seg_rows, double* cflgrid)
+                            double dy, double dt, int seg_rows, const int* bc, double* cflgrid)
 {
     *cflgrid = 0.0;
     const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
@@ -120,6 +120,7 @@ extern "C" int emu_stepgrid(const dclaw_params* P, const double* q_in, double* q
             A.i0 = 1 + s * STRIP_W;
             A.jA = 1 + g * seg_rows;
             A.jB = std::min(my, A.jA + seg_rows - 1);
+            for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
             run_strip(*P, A, cflgrid);
         }
     return 0;

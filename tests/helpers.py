@@ -61,9 +61,10 @@ class Emu:
         self.L = lib
         lib.emu_stepgrid.argtypes = [ctypes.POINTER(CParams), _dp, _dp, _dp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), _dp]
 
-    def stepgrid(self, params, q, aux, dx, dy, dt, seg_rows=32, write_aux=True):
+    def stepgrid(self, params, q, aux, dx, dy, dt, seg_rows=32, write_aux=True, bc=None):
         cp = to_c(params)
         q = np.array(q, dtype=np.float64, order="F", copy=True)
         qo = np.zeros_like(q, order="F")
@@ -72,7 +73,8 @@ class Emu:
         mx, my = q.shape[1] - 4, q.shape[2] - 4
         self.L.emu_stepgrid(ctypes.byref(cp), q.ctypes.data_as(_dp), qo.ctypes.data_as(_dp),
                             aux.ctypes.data_as(_dp), aux.shape[0], int(write_aux), mx, my, dx,
-                            dy, dt, seg_rows, ctypes.byref(cfl))
+                            dy, dt, seg_rows,
+                            None if bc is None else (ctypes.c_int * 4)(*bc), ctypes.byref(cfl))
         return qo, aux, cfl.value
 
 

@@ -82,3 +82,18 @@ def test_chained_steps_bitwise(oracle_mod, emu):
         assert c == cflo[n]
         dt = min(dt * 0.9 / c, 1e9)
     assert np.array_equal(q[:, 2:-2, 2:-2], qo[:, 2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("bc", [(1, 1, 1, 1), (3, 3, 3, 3), (3, 1, 1, 3)])
+def test_bc_on_load_equals_fill(oracle_mod, emu, bc):
+    """Boundary conditions applied while loading give the same step as filling
+    the ghost cells first (stale ghosts in the input must not matter)."""
+    O = oracle_mod
+    pt = slope_patch(61, 37, slope=0.7, phi_deg=24.0, seed=9)
+    q, aux, dt = advance_oracle(O, pt, 4, bc=bc)
+    stale = q.copy(order="F")
+    stale[:, :2, :] = 7.0
+    stale[:, :, -2:] = -3.0
+    qe, auxe, ce = emu.stepgrid(pt.params, stale, aux, 1.0, 1.0, dt, seg_rows=8, bc=bc)
+    qo, auxo, co = O.stepgrid(O.params_from(pt.params), q, aux, 1.0, 1.0, dt)
+    assert np.array_equal(qe, qo) and np.array_equal(auxe, auxo) and ce == co

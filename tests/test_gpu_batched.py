@@ -1,0 +1,106 @@
+"""GPU: batched ensembles (one launch per step over a work list), boundary
+conditions on load, and the Fortran-convention host drop-in, against the
+oracle and against the single-patch path."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import TOL_LONG, TOL_STEP, mask_equal, rel_linf_wet, slope_patch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dclaw_amd import _lib
+    _lib.lib()
+    return torch
+
+
+def _members(n=6, seed=5):
+    from dclaw_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        mx, my = int(rng.integers(10, 140)), int(rng.integers(10, 140))
+        pt = slope_patch(mx, my, slope=float(rng.uniform(0.3, 0.9)),
+                         phi_deg=float(rng.uniform(22, 35)), seed=k)
+        pt.params.kref = float(10 ** rng.uniform(-11, -9))
+        out.append(W.Member(pt, seed=k))
+    return out
+
+
+def test_batched_matches_oracle_per_member(dev, oracle_mod):
+    from dclaw_amd import step as S
+    O = oracle_mod
+    members = _members()
+    nsteps = 25
+    print("[batched] sizes", [(m.patch.mx, m.patch.my) for m in members], flush=True)
+    ens = S.Ensemble(members, seg_rows=16)
+    ens.run(nsteps, 1e-3, 0.9, 1e9)
+    dev.cuda.synchronize()
+    for k, m in enumerate(members):
+        pt = m.patch
+        qo, _, dto, _, _ = O.run(O.params_from(pt.params), pt.q, pt.aux, pt.dx, pt.dy, nsteps,
+                                 1e-3, 0.9, 1e9)
+        qg = ens.q(k)
+        I = (slice(None), slice(2, -2), slice(2, -2))
+        assert mask_equal(qg[I], qo[I], pt.params.dry_tolerance), k
+        assert rel_linf_wet(qg[I], qo[I], pt.params.dry_tolerance) <= TOL_LONG, k
+        assert ens.dt[k].item() == pytest.approx(dto, rel=1e-12)
+
+
+def test_batched_equals_plain_loop(dev):
+    from dclaw_amd import step as S
+    members = _members(seed=8)
+    a = S.Ensemble(members, seg_rows=32)
+    b = S.Ensemble(members, seg_rows=32)
+    a.run(10, 1e-3, 0.9, 1e9)
+    b.run(10, 1e-3, 0.9, 1e9, plain=True)
+    dev.cuda.synchronize()
+    for k in range(len(members)):
+        assert np.array_equal(a.q(k)[:, 2:-2, 2:-2], b.q(k)[:, 2:-2, 2:-2]), k
+    assert dev.equal(a.dt, b.dt)
+
+
+def test_fused_bc_equals_bound_then_step(dev):
+    from dclaw_amd import step as S
+    pt = slope_patch(90, 70, slope=0.8, phi_deg=25.0, seed=2)
+    for bc in [(1, 1, 1, 1), (3, 3, 3, 3)]:
+        r1 = S.PatchRunner(pt.q, pt.aux, pt.params, 1.0, 1.0, bc=bc)
+        r2 = S.PatchRunner(pt.q, pt.aux, pt.params, 1.0, 1.0, bc=bc)
+        r1.run(15, 1e-3, 0.9, 1e9, fused_bc=True)
+        r2.run(15, 1e-3, 0.9, 1e9, fused_bc=False)
+        dev.cuda.synchronize()
+        assert np.array_equal(r1.q()[:, 2:-2, 2:-2], r2.q()[:, 2:-2, 2:-2])
+
+
+def test_host_dropin_stepgrid(dev, oracle_mod):
+    """stepgrid_dclaw_ (Fortran convention, host arrays) = the oracle's
+    stepgrid on q and on aux(4..6)."""
+    from dclaw_amd import _lib
+    from dclaw_amd.params import to_c
+    from helpers import advance_oracle
+    O = oracle_mod
+    L = _lib.lib()
+    pt = slope_patch(75, 52, slope=0.7, phi_deg=26.0, seed=4)
+    q, aux, dt = advance_oracle(O, pt, 6)
+    cp = to_c(pt.params)
+    _lib.check(L.dclaw_set_params(ctypes.byref(cp)))
+    qh = np.array(q, order="F", copy=True)
+    ah = np.array(aux, order="F", copy=True)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    cfl = ctypes.c_double(0.0)
+    L.stepgrid_dclaw_(qh.ctypes.data_as(dp), ah.ctypes.data_as(dp), ci(2), ci(pt.mx), ci(pt.my),
+                      ci(ah.shape[0]), cd(1.0), cd(1.0), cd(dt), ctypes.byref(cfl))
+    qo, auxo, cflo = O.stepgrid(O.params_from(pt.params), q, aux, 1.0, 1.0, dt)
+    assert rel_linf_wet(qh, qo, pt.params.dry_tolerance) <= TOL_STEP
+    assert mask_equal(qh, qo, pt.params.dry_tolerance)
+    assert np.allclose(ah, auxo, rtol=1e-12, atol=1e-300)
+    assert abs(cfl.value - cflo) <= TOL_STEP * cflo
