@@ -44,6 +44,8 @@ def parse():
                     help="patch: one 4096^2 patch (default); ensemble: 256 members with sides "
                          "log-uniform in [64, 512], batched launch vs plain per-member loop")
     ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--seg-rows", type=int, default=0,
+                    help="rows per segment tile; 0 = pick with the autotuner (untimed)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy (the HBM "
                          "counter calibration dispatch for rocprofv3 --pmc runs)")
@@ -82,10 +84,10 @@ def main():
     cfl_desired, dt_max = 0.9, 1e9
 
     def one_step(ev=None):
-        S.bc_fill(r.qa, r.mx, r.my, r.bc)
         if ev is not None:
             ev[0].record()
-        S.stepgrid(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.dt, r.params, r.cfl)
+        S.stepgrid_bc(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.bc, r.dt, r.params,
+                      r.cfl)
         if ev is not None:
             ev[1].record()
         r.qa, r.qb = r.qb, r.qa
@@ -94,7 +96,15 @@ def main():
     for _ in range(args.warmup):
         one_step()
     torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    seg = args.seg_rows
+    tune_times = None
+    if seg <= 0:
+        from dclaw_amd import autotune
+        seg, tune_times = autotune.tune_patch(r)
+    else:
+        _lib.lib().dclaw_set_seg_rows(seg)
+    print(f"[bench] warmup done, segment rows {seg} (tuner: {tune_times})", file=sys.stderr,
+          flush=True)
 
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
@@ -159,7 +169,7 @@ def main():
                                f"seeded fractal bed + 20 deg tilt, 64 elliptical sources, "
                                f"forward only (adjoint not built)",
                    "mx": n, "my": n, "seed": args.seed, "cfl_desired": cfl_desired,
-                   "wet_cells_end": wet},
+                   "wet_cells_end": wet, "seg_rows": seg},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_stepgrid", "kernel_avg_ms": kern_avg_s * 1e3},
@@ -179,10 +189,16 @@ def bench_ensemble(args):
 
     t0 = time.time()
     members = W.ensemble(args.members, 64, 512, seed=args.seed)
-    ens = S.Ensemble(members, seg_rows=32)
+    seg = args.seg_rows
+    tune_times = None
+    if seg <= 0:
+        from dclaw_amd import autotune
+        seg, tune_times = autotune.tune_ensemble(members, lambda h: S.Ensemble(members, seg_rows=h))
+    ens = S.Ensemble(members, seg_rows=seg)
     cells = sum(m.patch.mx * m.patch.my for m in members)
     print(f"[bench] ensemble of {len(members)} members, {cells} cells, built in "
-          f"{time.time() - t0:.1f}s, {ens.ntiles} tiles", file=sys.stderr, flush=True)
+          f"{time.time() - t0:.1f}s, {ens.ntiles} tiles of {seg} rows (tuner: {tune_times})",
+          file=sys.stderr, flush=True)
     ens.dt.fill_(1e-3)
     ens.cfl.zero_()
 
@@ -226,7 +242,7 @@ def bench_ensemble(args):
         "config": {"workload": f"batched ensemble forward, {len(members)} members, sides "
                                f"log-uniform in [64,512], per-member phi/m0/kref, one launch per "
                                f"step (adjoint not built)", "members": len(members),
-                   "cells": cells, "seed": args.seed},
+                   "cells": cells, "seed": args.seed, "seg_rows": seg},
         "plain_loop": {"value": plain, "steps": nplain,
                        "ms_per_step": t_p / nplain * 1e3},
         "batched_over_plain": value / plain,

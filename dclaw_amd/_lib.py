@@ -19,7 +19,7 @@ except ImportError:  # pragma: no cover
 from .params import CParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libdclaw_hip.so")
+LIB_PATH = os.environ.get("DCLAW_HIP_LIB", os.path.join(HERE, "_build", "libdclaw_hip.so"))
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -46,6 +46,7 @@ SIGNATURES = {
     "dclaw_last_error": (ctypes.c_char_p, []),
     "dclaw_version": (ctypes.c_int, []),
     "dclaw_strip_width": (ctypes.c_int, []),
+    "dclaw_set_seg_rows": (ctypes.c_int, [ctypes.c_int]),
     "dclaw_bc_fill": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip, _vp]),
     "dclaw_stepgrid_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_double, ctypes.c_double,

@@ -1,0 +1,76 @@
+"""Tile-shape autotuner of the fused step.
+
+The only free tile parameter is the segment height (rows per wavefront
+tile; the strip width is fixed at 58 interior columns by the 64-lane
+wavefront).  Short segments shorten the critical path of wet regions and
+spread wet work over more waves; long segments amortise the 6 halo rows each
+segment recomputes.  Results are bit-identical for every height, so the
+tuner only times: each candidate runs a few real steps of the caller's
+workload on copies of its state, timed with HIP events on the current
+stream, and the fastest height is kept.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Callable, Dict, Iterable, Tuple
+
+import torch
+
+from . import _lib
+
+CANDIDATES = (4, 6, 8, 12, 16, 24, 32, 48)
+
+
+def _time(fn: Callable[[], None], reps: int) -> float:
+    fn()                                # warm (code objects, caches)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 3) -> Tuple[int, Dict]:
+    """Pick the segment height for a PatchRunner's single-patch launches.
+    The runner's state and dt are restored afterwards."""
+    L = _lib.lib()
+    qa, qb = runner.qa.clone(), runner.qb.clone()
+    dt, cfl = runner.dt.clone(), runner.cfl.clone()
+    times = {}
+    for h in candidates:
+        _lib.check(min(0, L.dclaw_set_seg_rows(h)))
+
+        def one():
+            runner.step()
+            runner.cfl.zero_()
+            runner.qa.copy_(qa)
+        times[h] = _time(one, reps)
+    best = min(times, key=times.get)
+    _lib.check(min(0, L.dclaw_set_seg_rows(best)))
+    runner.qa.copy_(qa)
+    runner.qb.copy_(qb)
+    runner.dt.copy_(dt)
+    runner.cfl.copy_(cfl)
+    return best, times
+
+
+def tune_ensemble(members, make, candidates: Iterable[int] = CANDIDATES,
+                  reps: int = 3) -> Tuple[int, Dict]:
+    """Pick the segment height of the batched launch: `make(seg_rows)` builds
+    an Ensemble from `members`; each candidate times a few steps."""
+    times = {}
+    for h in candidates:
+        ens = make(h)
+        ens.dt.fill_(1e-3)
+
+        def one():
+            ens.step()
+            ens.cfl.zero_()
+        times[h] = _time(one, reps)
+        del ens
+    best = min(times, key=times.get)
+    return best, times

@@ -42,9 +42,21 @@ int hip_fail(hipError_t e, const char* what)
         if (e_ != hipSuccess) return hip_fail(e_, #expr); \
     } while (0)
 
+// Lane policy of the strip body on a 64-lane wavefront.  prev/next are
+// whole-wave DPP shifts by one lane (wave_shr:1 = 0x138, wave_shl:1 =
+// 0x130); the end lane whose source is outside the wave keeps its own value.
 struct GpuLane {
     __device__ __forceinline__ int id() const { return (int)threadIdx.x; }
-    __device__ __forceinline__ double shfl(double v, int src) const { return __shfl(v, src, 64); }
+    template <int CTRL>
+    __device__ __forceinline__ static double dpp(double v)
+    {
+        const int lo = __double2loint(v), hi = __double2hiint(v);
+        const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+        const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+        return __hiloint2double(rhi, rlo);
+    }
+    __device__ __forceinline__ double prev(double v) const { return dpp<0x138>(v); }
+    __device__ __forceinline__ double next(double v) const { return dpp<0x130>(v); }
     __device__ __forceinline__ bool any(bool p) const { return __any((int)p) != 0; }
 };
 
@@ -56,7 +68,7 @@ __device__ __forceinline__ void wave_max_to(double v, double* dst)
                   (unsigned long long)__double_as_longlong(v));
 }
 
-constexpr int DEFAULT_SEG_ROWS = 32;
+int g_seg_rows = 32;   // rows per segment of the single-patch launch (tunable)
 
 __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in,
                                                  double* __restrict__ q_out,
@@ -221,6 +233,14 @@ int dclaw_version(void) { return 1; }
 
 int dclaw_strip_width(void) { return STRIP_W; }
 
+int dclaw_set_seg_rows(int seg_rows)
+{
+    if (seg_rows < 1 || seg_rows > 65536) return fail(DCLAW_EINVAL, "seg_rows must be in [1, 65536]");
+    const int prev = g_seg_rows;
+    g_seg_rows = seg_rows;
+    return prev;
+}
+
 int dclaw_bc_fill(double* q, int mx, int my, const int* bc, void* stream)
 {
     int rc = check_patch(mx, my, 1);
@@ -259,12 +279,13 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     if (q_in == q_out) return fail(DCLAW_EINVAL, "q_out must not alias q_in");
     if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
     const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
-    const int nseg = (my + DEFAULT_SEG_ROWS - 1) / DEFAULT_SEG_ROWS;
+    const int seg_rows = g_seg_rows;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
     if (nseg > 65535) return fail(DCLAW_EINVAL, "my too large for one launch");
     const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
     hipLaunchKernelGGL(k_stepgrid, dim3(nstrips, nseg), dim3(64), 0, stream, q_in, q_out, aux,
                        aux_out, maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev,
-                       DEFAULT_SEG_ROWS, b0, b1, b2, b3);
+                       seg_rows, b0, b1, b2, b3);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

@@ -16,7 +16,8 @@
 //       transversally (flux2fw.f with ixy = 2);
 //   S6  assembles fm, fp, gm, gp for row R-3 in GeoClaw step2's summation
 //       order, updates it (stepgrid) and applies src2 (src2.f90:92-303).
-// Neighbouring columns are reached with lane shuffles; no LDS tile is held.
+// Neighbouring columns are reached with lane shifts (prev: value of lane l-1,
+// next: of lane l+1; the end lanes keep their own); no LDS tile is held.
 // The code is a template on the lane policy so the same body runs on the GPU
 // (one wavefront) and in a host emulator (64 threads in lockstep).
 #pragma once
@@ -125,16 +126,16 @@ DC_HD Side side_of(const RowCell& r, bool second, double taudir, double fs)
     return s;
 }
 
-// Shuffle a whole wave record from lane `src`.
+// A whole wave record from the next lane.
 template <class Lane>
-DC_HD EdgeWaves shfl_waves(Lane& L, const EdgeWaves& w, int src)
+DC_HD EdgeWaves next_waves(Lane& L, const EdgeWaves& w)
 {
     EdgeWaves o;
-    for (int k = 0; k < 3; k++) o.s[k] = L.shfl(w.s[k], src);
-    for (int k = 0; k < 6; k++) o.w1[k] = L.shfl(w.w1[k], src);
-    for (int k = 0; k < 6; k++) o.w5[k] = L.shfl(w.w5[k], src);
-    o.w2h = L.shfl(w.w2h, src);
-    o.w2n = L.shfl(w.w2n, src);
+    for (int k = 0; k < 3; k++) o.s[k] = L.next(w.s[k]);
+    for (int k = 0; k < 6; k++) o.w1[k] = L.next(w.w1[k]);
+    for (int k = 0; k < 6; k++) o.w5[k] = L.next(w.w5[k]);
+    o.w2h = L.next(w.w2h);
+    o.w2n = L.next(w.w2n);
     return o;
 }
 
@@ -267,6 +268,30 @@ DC_HD size_t aidx(int i, int j, int mx, int maux)
     return (size_t)maux * ((size_t)(i + 1) + (size_t)(mx + 4) * (size_t)(j + 1));
 }
 
+// Raw q and bed of column c, row R (zeros outside the array; boundary
+// conditions applied through bc_source when A.bc is set).
+DC_HD void load_row(const StripArgs& A, int c, int R, bool col_in_array, double* q7, double& b)
+{
+    for (int m = 0; m < 7; m++) q7[m] = 0.0;
+    b = 0.0;
+    const int mx = A.mx, my = A.my;
+    if (!(R >= -1 && R <= my + 2) || !col_in_array) return;
+    const bool ghost = (c < 1 || c > mx || R < 1 || R > my);
+    if (ghost && A.bc[0] != 0) {
+        int si, sj;
+        double fu, fv;
+        bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
+        const size_t o = qidx(0, si, sj, mx);
+        for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
+        q7[1] = fu * q7[1];
+        q7[2] = fv * q7[2];
+    } else {
+        const size_t o = qidx(0, c, R, mx);
+        for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
+    }
+    b = A.aux[aidx(c, R, mx, A.maux)];
+}
+
 // The strip body.  Returns this lane's CFL contribution (max |dt/dx * s|).
 template <class Lane>
 DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
@@ -308,28 +333,96 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
         gx_prev2[k] = gp_prev[k] = 0.0;
     }
 
-    for (int R = A.jA - 3; R <= A.jB + 3; R++) {
-        // ---------------- S1: load row R, b4step2 pointwise, F1, F2
-        {
-            double q7[7] = {0, 0, 0, 0, 0, 0, 0};
-            double b = 0.0;
-            const bool row_in = (R >= -1 && R <= my + 2);
-            if (row_in && col_in_array) {
-                const bool ghost = (c < 1 || c > mx || R < 1 || R > my);
-                if (ghost && A.bc[0] != 0) {
-                    int si, sj;
-                    double fu, fv;
-                    bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
-                    const size_t o = qidx(0, si, sj, mx);
-                    for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
-                    q7[1] = fu * q7[1];
-                    q7[2] = fv * q7[2];
-                } else {
-                    const size_t o = qidx(0, c, R, mx);
-                    for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
+    // Whole-segment dry check: if no cell the segment reads is wet (h >
+    // dry_tolerance before qfix), every output is the dry state and the
+    // marching loop below would only copy it, so write it directly with all
+    // loads in flight at once.  Stored ghosts are skipped when the boundary
+    // conditions are applied on load (their sources are interior cells of
+    // the same window).
+    {
+        bool wet = false;
+        const bool bcl = (A.bc[0] != 0);
+        const int clo = bcl ? 1 : -1, chi = bcl ? mx : mx + 2;
+        const int rlo = bcl ? 1 : -1, rhi = bcl ? my : my + 2;
+        const bool col_chk = (c >= clo && c <= chi);
+        const int r0 = A.jA - 3 > rlo ? A.jA - 3 : rlo;
+        const int r1 = A.jB + 3 < rhi ? A.jB + 3 : rhi;
+        if (col_chk) {
+            const double* __restrict__ qi = A.q_in;
+            bool w = false;
+#pragma unroll 8
+            for (int r = r0; r <= r1; r++) w |= (qi[qidx(0, c, r, mx)] > P.dry_tolerance);
+            wet = w;
+        }
+        if (!L.any(wet)) {
+            // rows of this segment in chunks of 8: all bdif loads first, then stores
+            const double* __restrict__ qi = A.q_in;
+            double* __restrict__ qo = A.q_out;
+            for (int R0 = A.jA - 3; R0 <= A.jB + 3; R0 += 8) {
+                double bd[8];
+                bool wr[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int R = R0 + k;
+                    const bool row_in = (R >= -1 && R <= my + 2) && R <= A.jB + 3;
+                    const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
+                                             (R >= my + 1 && R <= my + 2 && A.jB == my);
+                    const bool row_interior = (R >= 1 && R <= my) && R >= A.jA && R <= A.jB;
+                    wr[k] = row_in && col_in_array &&
+                            ((row_ghost_w && (col_out || col_ghost_w)) ||
+                             (row_interior && (col_out || col_ghost_w)));
+                    bd[k] = 0.0;
+                    if (wr[k]) {
+                        if ((c < 1 || c > mx || R < 1 || R > my) && bcl) {
+                            int si, sj;
+                            double fu, fv;
+                            bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
+                            bd[k] = qi[qidx(6, si, sj, mx)];
+                        } else {
+                            bd[k] = qi[qidx(6, c, R, mx)];
+                        }
+                    }
                 }
-                b = A.aux[aidx(c, R, mx, A.maux)];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    if (!wr[k]) continue;
+                    const size_t o = qidx(0, c, R0 + k, mx);
+                    for (int z = 0; z < 6; z++) qo[o + z] = 0.0;
+                    qo[o + 6] = dmax(bd[k], 0.0);
+                }
             }
+            if (A.aux_out) {
+                for (int r = A.jA - 2; r <= A.jB + 2; r++) {
+                    if (r < 0 || r > my + 1 || !col_taudir) continue;
+                    const bool row_own = (r >= A.jA && r <= A.jB) || (r == 0 && A.jA == 1) ||
+                                         (r == my + 1 && A.jB == my);
+                    const bool col_own =
+                        (c >= A.i0 && c <= A.i0 + STRIP_W - 1 && c >= 1 && c <= mx) ||
+                        (c == 0 && A.i0 == 1) || (c == mx + 1 && A.i0 + STRIP_W - 1 >= mx);
+                    if (row_own && col_own) {
+                        const size_t o = aidx(c, r, mx, A.maux);
+                        A.aux_out[o + 3] = 0.0;
+                        A.aux_out[o + 4] = 0.0;
+                        A.aux_out[o + 5] = 0.0;
+                    }
+                }
+            }
+            return 0.0;
+        }
+    }
+
+    // Row loads run one iteration ahead of their use (software prefetch):
+    // the raw values of row R+1 are requested before row R is processed.
+    double nq7[7], nb;
+    load_row(A, c, A.jA - 3, col_in_array, nq7, nb);
+    for (int R = A.jA - 3; R <= A.jB + 3; R++) {
+        // ---------------- S1: row R (prefetched), b4step2 pointwise, F1, F2
+        {
+            double q7[7];
+            for (int m = 0; m < 7; m++) q7[m] = nq7[m];
+            const double b = nb;
+            if (R + 1 <= A.jB + 3) load_row(A, c, R + 1, col_in_array, nq7, nb);
+            const bool row_in = (R >= -1 && R <= my + 2);
             prepare_cell(P, q7, b, q0);
             const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
                                      (R >= my + 1 && R <= my + 2 && A.jB == my);
@@ -395,14 +488,14 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
         {
             const int r = R - 1;
             // neighbour columns of row R-1 (F1 states), all lanes participate
-            const double lh = L.shfl(q1.h, lane_prev), lhu = L.shfl(q1.hu, lane_prev);
-            const double lhv = L.shfl(q1.hv, lane_prev), lhm = L.shfl(q1.hm0, lane_prev);
-            const double lp = L.shfl(q1.p0, lane_prev), lhc = L.shfl(q1.hchi0, lane_prev);
-            const double lb = L.shfl(q1.b, lane_prev), lbd = L.shfl(q1.bdif, lane_prev);
-            const double rh = L.shfl(q1.h, lane_next), rhu = L.shfl(q1.hu, lane_next);
-            const double rhv = L.shfl(q1.hv, lane_next), rhm = L.shfl(q1.hm0, lane_next);
-            const double rp = L.shfl(q1.p0, lane_next), rhc = L.shfl(q1.hchi0, lane_next);
-            const double rb = L.shfl(q1.b, lane_next), rbd = L.shfl(q1.bdif, lane_next);
+            const double lh = L.prev(q1.h), lhu = L.prev(q1.hu);
+            const double lhv = L.prev(q1.hv), lhm = L.prev(q1.hm0);
+            const double lp = L.prev(q1.p0), lhc = L.prev(q1.hchi0);
+            const double lb = L.prev(q1.b), lbd = L.prev(q1.bdif);
+            const double rh = L.next(q1.h), rhu = L.next(q1.hu);
+            const double rhv = L.next(q1.hv), rhm = L.next(q1.hm0);
+            const double rp = L.next(q1.p0), rhc = L.next(q1.hchi0);
+            const double rb = L.next(q1.b), rbd = L.next(q1.bdif);
             Taudir t;
             t.fsphi = t.tx = t.ty = 0.0;
             if (r >= 0 && r <= my + 1 && col_taudir && r >= A.jA - 2 && r <= A.jB + 2) {
@@ -458,17 +551,17 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             const int r = R - 2;
             const bool row_act = (r >= A.jA - 1 && r <= A.jB + 1);
             // left neighbour's F2 (or F1 for the slice's first cell) state
-            const double lh = L.shfl(q2.h, lane_prev), lhu = L.shfl(q2.hu, lane_prev);
-            const double lhv = L.shfl(q2.hv, lane_prev);
-            const double lhm1 = L.shfl(q2.hm1, lane_prev), lp1 = L.shfl(q2.p1, lane_prev);
-            const double lhc1 = L.shfl(q2.hchi1, lane_prev), lm1 = L.shfl(q2.m1, lane_prev);
-            const double lc1 = L.shfl(q2.chi1, lane_prev);
-            const double lhm2 = L.shfl(q2.hm2, lane_prev), lp2 = L.shfl(q2.p2, lane_prev);
-            const double lhc2 = L.shfl(q2.hchi2, lane_prev), lm2 = L.shfl(q2.m2, lane_prev);
-            const double lc2 = L.shfl(q2.chi2, lane_prev);
-            const double lu = L.shfl(q2.u, lane_prev), lv = L.shfl(q2.v, lane_prev);
-            const double lb = L.shfl(q2.b, lane_prev), lbd = L.shfl(q2.bdif, lane_prev);
-            const double ltx = L.shfl(ta2.tx, lane_prev), lfs = L.shfl(ta2.fsphi, lane_prev);
+            const double lh = L.prev(q2.h), lhu = L.prev(q2.hu);
+            const double lhv = L.prev(q2.hv);
+            const double lhm1 = L.prev(q2.hm1), lp1 = L.prev(q2.p1);
+            const double lhc1 = L.prev(q2.hchi1), lm1 = L.prev(q2.m1);
+            const double lc1 = L.prev(q2.chi1);
+            const double lhm2 = L.prev(q2.hm2), lp2 = L.prev(q2.p2);
+            const double lhc2 = L.prev(q2.hchi2), lm2 = L.prev(q2.m2);
+            const double lc2 = L.prev(q2.chi2);
+            const double lu = L.prev(q2.u), lv = L.prev(q2.v);
+            const double lb = L.prev(q2.b), lbd = L.prev(q2.bdif);
+            const double ltx = L.prev(ta2.tx), lfs = L.prev(ta2.fsphi);
             EdgeWaves xw;
             for (int z = 0; z < 3; z++) xw.s[z] = 0.0;
             for (int z = 0; z < 6; z++) { xw.w1[z] = 0.0; xw.w5[z] = 0.0; }
@@ -490,15 +583,15 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                 xw = rpn2_edge(P, 1, sL, sR);
             }
             // limiter neighbours
-            const EdgeWaves xn = shfl_waves(L, xw, lane_next);
+            const EdgeWaves xn = next_waves(L, xw);
             double dr[3];
             wave_dots(xw, xn, 1, dr);
             double dl[3];
-            for (int f = 0; f < 3; f++) dl[f] = L.shfl(dr[f], lane_prev);
+            for (int f = 0; f < 3; f++) dl[f] = L.prev(dr[f]);
             // wet-dry gate: h(c-2) h(c-1) h(c) h(c+1) > 0, in flux2's product order
-            const double hm2c = L.shfl(q2.h, lane >= 2 ? lane - 2 : 0);
+            const double hm2c = L.prev(lh);
             const double hm1c = lh;
-            const double hp1c = L.shfl(q2.h, lane_next);
+            const double hp1c = L.next(q2.h);
             const bool wet = (hm1c * q2.h * hm2c * hp1c > 0.0);
             const EdgeFlux ef = edge_flux(P, 1, xw, dl, dr, wet, dtdx);
             const bool lim_ok = row_act && c >= 1 && c <= mx + 1 && lane >= 2 && lane <= 62;
@@ -511,7 +604,7 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             }
             // transverse split in cell c: imp1 with amdq of edge c+1, imp2 with apdq of edge c
             double amdq_next[6];
-            for (int z = 0; z < 6; z++) amdq_next[z] = L.shfl(ef.amdq[z], lane_next);
+            for (int z = 0; z < 6; z++) amdq_next[z] = L.next(ef.amdq[z]);
             if (P.transverse != 0) {
                 const bool cell_ok = row_act && c >= 1 && c <= mx && lane >= 2 && lane <= 61;
                 double bm1[6], bp1[6], bm2[6], bp2[6];
@@ -537,8 +630,8 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                 for (int f = 0; f < 3; f++) cfl = dmax(cfl, fabs(dtdy * yw_pend.s[f]));
             }
             // topo of the neighbouring columns for rows R-3 and R-2
-            const double b3l = L.shfl(q3.b, lane_prev), b3r = L.shfl(q3.b, lane_next);
-            const double b2l = L.shfl(q2.b, lane_prev), b2r = L.shfl(q2.b, lane_next);
+            const double b3l = L.prev(q3.b), b3r = L.next(q3.b);
+            const double b2l = L.prev(q2.b), b2r = L.next(q2.b);
             if (P.transverse != 0) {
                 double bm1[6], bp1[6], bm2[6], bp2[6];
                 // imp1: amdq of edge R-2 split in cell (c, R-3)
@@ -563,14 +656,14 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             const int j = R - 3;
             // fm(c, j) = ((0 + faddm_x(c, j)) + Gy_{c-1}(j, 2)) + Gy_c(j, 1); same for fp
             double gyl2[6];
-            for (int z = 0; z < 6; z++) gyl2[z] = L.shfl(Gy2[z], lane_prev);
+            for (int z = 0; z < 6; z++) gyl2[z] = L.prev(Gy2[z]);
             double fmc[6], fpc[6];
             for (int z = 0; z < 6; z++) {
                 fmc[z] = ((0.0 + fxm_prev[z]) + gyl2[z]) + Gy1[z];
                 fpc[z] = ((0.0 + fxp_prev[z]) + gyl2[z]) + Gy1[z];
             }
             double fm_next[6];
-            for (int z = 0; z < 6; z++) fm_next[z] = L.shfl(fmc[z], lane_next);
+            for (int z = 0; z < 6; z++) fm_next[z] = L.next(fmc[z]);
             // gm(c, j+1) = ((0 + Gx_j(c,2)) + Gx_{j+1}(c,1)) + faddm_y(c, j+1)
             double gm_next[6], gp_next[6];
             for (int z = 0; z < 6; z++) {

@@ -79,6 +79,12 @@ int dclaw_version(void);
  * and row segment). */
 int dclaw_strip_width(void);
 
+/* Rows per segment (the tile height) of the single-patch launches; returns
+ * the previous value or a negative error.  Results do not depend on it; the
+ * autotuner (dclaw_amd/autotune.py) picks it per workload.  No reference
+ * counterpart (tiling of the GPU launch). */
+int dclaw_set_seg_rows(int seg_rows);
+
 /* Ghost-cell fill of q (bc2amr extrap / wall; bc[4] = xlo, xhi, ylo, yhi),
  * on device memory.  Replaces AMRClaw's bound/bc2amr for one patch. */
 int dclaw_bc_fill(double* q, int mx, int my, const int* bc, void* stream);

@@ -44,6 +44,8 @@ struct CoLane {
         swapcontext(&s->ctx[lid], &s->main);
         return s->buf[b][src];
     }
+    double prev(double v) { return shfl(v, lid > 0 ? lid - 1 : lid); }
+    double next(double v) { return shfl(v, lid < STRIP_LANES - 1 ? lid + 1 : lid); }
     bool any(bool p)
     {
         const int b = n & 1;
