@@ -208,7 +208,27 @@ struct Side {
     double u, v, m, chi;             // qfix primitives, u = normal / h
     double b;                        // aux(1) - bdif
     double taudir, fs;               // aux(i_taudir_{x|y}), aux(i_fsphi)
+    double rho, tau;                 // setvars(h, u, v, m, p, chi) of this state; read
+                                     // only when h > dry_tolerance
 };
+
+// The two setvars outputs rpn2 and calc_taudir read.  setvars depends on the
+// velocity only through u*u + v*v, so the value is the same for the x and
+// y sweeps and for a wall-mirrored copy of the state (u -> -u): one
+// evaluation per cell state serves every edge that reads it.
+struct RhoTau {
+    double rho, tau;
+};
+
+DC_HD RhoTau rho_tau(const dclaw_params& P, double h, double u, double v, double m, double p,
+                     double chi)
+{
+    const Vars s = setvars(P, h, u, v, m, p, chi, P.grav);
+    RhoTau o;
+    o.rho = s.rho;
+    o.tau = s.tau;
+    return o;
+}
 
 // riemann_dig2_aug_sswave_ez (riemannsolvers_dclaw.f:3-521) fused with the
 // rpn2 per-edge body (rpn2_dclaw.f:125-315): dry skip, wall/inundation tests,
@@ -237,36 +257,55 @@ DC_HD EdgeWaves rpn2_edge(const dclaw_params& P, int ixy, const Side& SL, const 
     const double taudirL = SL.taudir, taudirR = SR.taudir;
     const double fsL = SL.fs, fsR = SR.fs;
 
+    // setvars values of the two states as the solver will see them
+    double svrL = SL.rho, svtL = SL.tau, svrR = SR.rho, svtR = SR.tau;
     double wall0 = 1.0, wall1 = 1.0, wall2 = 1.0;
     bool wallprob = false;
+    bool resetL = false, resetR = false;   // a dry side reset in place (not mirrored)
     if (hR <= drytol) {
         hR = 0.0;
         pR = 0.0;
         mR = mL;
         chiR = chiL;
+        resetR = true;
         const RType rt = riemanntype(hL, hL, uL, -uL, drytol, grav);
         const double hstartest = dmax(hL, rt.hm);
         if (hstartest + bL < bR) {
             wall1 = 0.0;
             wall2 = 0.0;
             wallprob = true;
+            resetR = false;
             hR = hL; huR = -huL; hvR = hvL; hmR = hmL; bR = bL;
             uR = -uL; vR = vL; mR = mL; pR = pL; chiR = chiL;
+            svrR = svrL; svtR = svtL;
         }
     } else if (hL <= drytol) {
         hL = 0.0;
         pL = 0.0;
         mL = mR;
         chiL = chiR;
+        resetL = true;
         const RType rt = riemanntype(hR, hR, -uR, uR, drytol, grav);
         const double hstartest = dmax(hR, rt.hm);
         if (hstartest + bR < bL) {
             wall0 = 0.0;
             wall1 = 0.0;
             wallprob = true;
+            resetL = false;
             hL = hR; huL = -huR; hvL = hvR; hmL = hmR; mL = mR; bL = bR;
             uL = -uR; vL = vR; pL = pR; chiL = chiR;
+            svrL = svrR; svtL = svtR;
         }
+    }
+    // A reset dry side (h = 0) reaches setvars below only when dry_tolerance
+    // <= 0; evaluate it on its reset state then.
+    if (drytol <= 0.0 && resetR) {
+        const RhoTau t = rho_tau(P, hR, uR, vR, mR, pR, chiR);
+        svrR = t.rho; svtR = t.tau;
+    }
+    if (drytol <= 0.0 && resetL) {
+        const RhoTau t = rho_tau(P, hL, uL, vL, mL, pL, chiL);
+        svrL = t.rho; svtL = t.tau;
     }
 
     // ---- riemann_dig2_aug_sswave_ez (thetaL = thetaR = 0) ----
@@ -278,20 +317,20 @@ DC_HD EdgeWaves rpn2_edge(const dclaw_params& P, int ixy, const Side& SL, const 
     const double gz = grav * cos(0.5 * (0.0 + 0.0));
     const double alpha_seg = 1.0 - P.beta_seg;
 
+    // setvars of each wet side (riemannsolvers_dclaw.f:64-80), evaluated once
+    // per cell state by the caller (gzL = gzR = grav: theta = 0)
+    (void)gzL;
+    (void)gzR;
     double rhoL, rhoR, tauL, tauR, h;
     if (hL >= drytol && hR >= drytol) {
-        const Vars a = setvars(P, hL, uL, vL, mL, pL, chiL, gzL);
-        const Vars b = setvars(P, hR, uR, vR, mR, pR, chiR, gzR);
-        rhoL = a.rho; tauL = a.tau; rhoR = b.rho; tauR = b.tau;
+        rhoL = svrL; tauL = svtL; rhoR = svrR; tauR = svtR;
         h = 0.5 * (hL + hR);
     } else if (hL >= drytol) {
-        const Vars a = setvars(P, hL, uL, vL, mL, pL, chiL, gzL);
-        rhoL = a.rho; tauL = a.tau; rhoR = a.rho;
+        rhoL = svrL; tauL = svtL; rhoR = svrL;
         tauR = 0.5 * tauL;
         h = 0.5 * hL;
     } else {
-        const Vars b = setvars(P, hR, uR, vR, mR, pR, chiR, gzR);
-        rhoL = b.rho; rhoR = b.rho; tauR = b.tau;
+        rhoL = svrR; rhoR = svrR; tauR = svtR;
         tauL = 0.5 * tauR;
         h = 0.5 * hR;
     }
@@ -673,7 +712,8 @@ struct Taudir {
 
 DC_HD Taudir calc_taudir_cell(const dclaw_params& P, double dx, double dy, const Cell& C,
                               double bC, const Cell& L, double bL_, const Cell& R, double bR_,
-                              const Cell& B, double bB_, const Cell& T, double bT_)
+                              const Cell& B, double bB_, const Cell& T, double bT_,
+                              const RhoTau& svC)
 {
     const double gz = P.grav, drytol = P.dry_tolerance;
     double u, v, m, chi, rho, tu, tv, tm, tchi, trho;
@@ -710,8 +750,11 @@ DC_HD Taudir calc_taudir_cell(const dclaw_params& P, double dx, double dy, const
         o.tx = 0.0; o.ty = 0.0; o.fsphi = 0.0;
         return o;
     }
-    const Vars sv = setvars(P, h, u, v, m, p, chi, gz);
-    rho = sv.rho;
+    // setvars of the centre's qfix'ed state (= F1 of the cell), read only
+    // where hvnorm > 0, i.e. on a wet centre
+    (void)m;
+    (void)chi;
+    rho = svC.rho;
     const double sth = sin(0.0);
     const double FxL = -(gz * 0.5 * (h + hL) * (eta - etaL) / dx) + gz * 0.5 * (h + hL) * sth;
     const double FyL = -(gz * 0.5 * (h + hB) * (eta - etaB) / dy);
@@ -730,7 +773,7 @@ DC_HD Taudir calc_taudir_cell(const dclaw_params& P, double dx, double dy, const
         const double dot = dmin(dmax(0.0, Fx * hu), dmax(0.0, Fy * hv));
         if (dot > 0.0) {
             const double Fproj = dot / hvnorm;
-            o.fsphi = dmin(1.0, Fproj * rho / dmax(sv.tau, 1e-16));
+            o.fsphi = dmin(1.0, Fproj * rho / dmax(svC.tau, 1e-16));
         } else {
             o.fsphi = 0.0;
         }

@@ -68,6 +68,7 @@ struct RowCell {
     double hm1, p1, hchi1, m1, chi1; // F1 = qfix(Q0)
     double u, v;                     // hu/h, hv/h (same for F1 and F2)
     double hm2, p2, hchi2, m2, chi2; // F2 = qfix(F1)
+    double rho1, tau1;               // setvars of F1 (wet cells; set when the row is R-1)
 };
 
 DC_HD void zero_row(RowCell& r)
@@ -77,6 +78,7 @@ DC_HD void zero_row(RowCell& r)
     r.hm1 = r.p1 = r.hchi1 = r.m1 = r.chi1 = 0.0;
     r.u = r.v = 0.0;
     r.hm2 = r.p2 = r.hchi2 = r.m2 = r.chi2 = 0.0;
+    r.rho1 = r.tau1 = 0.0;
 }
 
 // b4step2 pointwise part on a loaded cell, then F1 and F2.
@@ -108,8 +110,9 @@ DC_HD void prepare_cell(const dclaw_params& P, const double* q7, double b, RowCe
 }
 
 // rpn2 side state from a row cell: F2 (left side, when the cell is not the
-// slice's first cell) or F1 (right side, or the slice's first cell).
-DC_HD Side side_of(const RowCell& r, bool second, double taudir, double fs)
+// slice's first cell) or F1 (right side, or the slice's first cell); sv is
+// setvars of that state.
+DC_HD Side side_of(const RowCell& r, bool second, double taudir, double fs, const RhoTau& sv)
 {
     Side s;
     s.h = r.h; s.hu = r.hu; s.hv = r.hv;
@@ -123,6 +126,8 @@ DC_HD Side side_of(const RowCell& r, bool second, double taudir, double fs)
     s.b = r.b - r.bdif;
     s.taudir = taudir;
     s.fs = fs;
+    s.rho = sv.rho;
+    s.tau = sv.tau;
     return s;
 }
 
@@ -311,6 +316,12 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                              (c >= mx + 1 && c <= mx + 2 && A.i0 + STRIP_W - 1 >= mx);
 
     double cfl = 0.0;
+    // calc_taudir of a cell whose whole neighbourhood is dry: 0 when
+    // dry_tolerance > 0 (early return), else the h = 0 fall-through values
+    const bool drytol_pos = P.dry_tolerance > 0.0;
+    const double dry_fs = drytol_pos ? 0.0 : 1.0;
+    const double dry_tx = drytol_pos ? 0.0 : A.dx * 1.0;
+    const double dry_ty = drytol_pos ? 0.0 : A.dy * 1.0;
 
     RowCell q3, q2, q1, q0;   // rows R-3, R-2, R-1, R
     zero_row(q3); zero_row(q2); zero_row(q1); zero_row(q0);
@@ -401,9 +412,9 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                         (c == 0 && A.i0 == 1) || (c == mx + 1 && A.i0 + STRIP_W - 1 >= mx);
                     if (row_own && col_own) {
                         const size_t o = aidx(c, r, mx, A.maux);
-                        A.aux_out[o + 3] = 0.0;
-                        A.aux_out[o + 4] = 0.0;
-                        A.aux_out[o + 5] = 0.0;
+                        A.aux_out[o + 3] = dry_fs;
+                        A.aux_out[o + 4] = dry_tx;
+                        A.aux_out[o + 5] = dry_ty;
                     }
                 }
             }
@@ -455,9 +466,9 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                         (c == 0 && A.i0 == 1) || (c == mx + 1 && A.i0 + STRIP_W - 1 >= mx);
                     if (row_own && col_own) {
                         const size_t o = aidx(c, r, mx, A.maux);
-                        A.aux_out[o + 3] = 0.0;
-                        A.aux_out[o + 4] = 0.0;
-                        A.aux_out[o + 5] = 0.0;
+                        A.aux_out[o + 3] = dry_fs;
+                        A.aux_out[o + 4] = dry_tx;
+                        A.aux_out[o + 5] = dry_ty;
                     }
                 }
             }
@@ -484,9 +495,21 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             q1 = q0;
             continue;
         }
-        // ---------------- S2: calc_taudir for row R-1
+        // ---------------- S2: setvars of F1 and calc_taudir for row R-1
         {
             const int r = R - 1;
+            // setvars(F1) of a wet cell serves calc_taudir here and the right
+            // side of both sweeps' edges (S3 now, S4 when the row is R-2)
+            q1.rho1 = 0.0;
+            q1.tau1 = 0.0;
+            if (r >= A.jA - 2 && r <= A.jB + 2 && q1.h > P.dry_tolerance) {
+                const RhoTau v1 = rho_tau(P, q1.h, q1.u, q1.v, q1.m1, q1.p1, q1.chi1);
+                q1.rho1 = v1.rho;
+                q1.tau1 = v1.tau;
+            }
+            RhoTau sv1;
+            sv1.rho = q1.rho1;
+            sv1.tau = q1.tau1;
             // neighbour columns of row R-1 (F1 states), all lanes participate
             const double lh = L.prev(q1.h), lhu = L.prev(q1.hu);
             const double lhv = L.prev(q1.hv), lhm = L.prev(q1.hm0);
@@ -505,7 +528,7 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                 Cell B = {q2.h, q2.hu, q2.hv, q2.hm0, q2.p0, q2.hchi0};
                 Cell T = {q0.h, q0.hu, q0.hv, q0.hm0, q0.p0, q0.hchi0};
                 t = calc_taudir_cell(P, A.dx, A.dy, C, q1.b - q1.bdif, Lc, lb - lbd, Rc, rb - rbd,
-                                     B, q2.b - q2.bdif, T, q0.b - q0.bdif);
+                                     B, q2.b - q2.bdif, T, q0.b - q0.bdif, sv1);
             }
             if (A.aux_out && r >= 0 && r <= my + 1 && col_taudir) {
                 const bool row_own = (r >= A.jA && r <= A.jB) || (r == 0 && A.jA == 1) ||
@@ -524,6 +547,19 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             ta1 = t;
         }
 
+        // setvars of F2 for row R-2: the left side of the y-edge R-1 and of
+        // the x-edges of row R-2
+        RhoTau v2;
+        v2.rho = 0.0;
+        v2.tau = 0.0;
+        if (R - 2 >= A.jA - 2 && R - 2 <= A.jB + 1 && q2.h > P.dry_tolerance)
+            v2 = rho_tau(P, q2.h, q2.u, q2.v, q2.m2, q2.p2, q2.chi2);
+        RhoTau v1_q2, v1_q1;
+        v1_q2.rho = q2.rho1;
+        v1_q2.tau = q2.tau1;
+        v1_q1.rho = q1.rho1;
+        v1_q1.tau = q1.tau1;
+
         // ---------------- S3: y-edge R-1 (cells (c, R-2), (c, R-1))
         EdgeWaves yw_new;
         double ydotr[3] = {0.0, 0.0, 0.0};
@@ -533,8 +569,9 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             for (int z = 0; z < 6; z++) { yw_new.w1[z] = 0.0; yw_new.w5[z] = 0.0; }
             yw_new.w2h = yw_new.w2n = 0.0;
             if (k >= A.jA - 1 && k <= A.jB + 2 && col_taudir) {
-                const Side sL = side_of(q2, (k - 1) >= 0, ta2.ty, ta2.fsphi);
-                const Side sR = side_of(q1, false, ta1.ty, ta1.fsphi);
+                const bool second = (k - 1) >= 0;
+                const Side sL = side_of(q2, second, ta2.ty, ta2.fsphi, second ? v2 : v1_q2);
+                const Side sR = side_of(q1, false, ta1.ty, ta1.fsphi, v1_q1);
                 // y sweep: primitives u = hv/h (normal), v = hu/h (transverse)
                 Side l2 = sL, r2 = sR;
                 l2.u = q2.v; l2.v = q2.u;
@@ -550,15 +587,19 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
         {
             const int r = R - 2;
             const bool row_act = (r >= A.jA - 1 && r <= A.jB + 1);
-            // left neighbour's F2 (or F1 for the slice's first cell) state
+            // left neighbour's F2 (or F1 for the slice's first cell, column
+            // -1) state: each lane selects its own left-role state, then the
+            // lane to its right reads it
+            const bool sec_me = (c >= 0);
             const double lh = L.prev(q2.h), lhu = L.prev(q2.hu);
             const double lhv = L.prev(q2.hv);
-            const double lhm1 = L.prev(q2.hm1), lp1 = L.prev(q2.p1);
-            const double lhc1 = L.prev(q2.hchi1), lm1 = L.prev(q2.m1);
-            const double lc1 = L.prev(q2.chi1);
-            const double lhm2 = L.prev(q2.hm2), lp2 = L.prev(q2.p2);
-            const double lhc2 = L.prev(q2.hchi2), lm2 = L.prev(q2.m2);
-            const double lc2 = L.prev(q2.chi2);
+            const double lhm = L.prev(sec_me ? q2.hm2 : q2.hm1);
+            const double lp = L.prev(sec_me ? q2.p2 : q2.p1);
+            const double lhc = L.prev(sec_me ? q2.hchi2 : q2.hchi1);
+            const double lm = L.prev(sec_me ? q2.m2 : q2.m1);
+            const double lc = L.prev(sec_me ? q2.chi2 : q2.chi1);
+            const double lrho = L.prev(sec_me ? v2.rho : q2.rho1);
+            const double ltau = L.prev(sec_me ? v2.tau : q2.tau1);
             const double lu = L.prev(q2.u), lv = L.prev(q2.v);
             const double lb = L.prev(q2.b), lbd = L.prev(q2.bdif);
             const double ltx = L.prev(ta2.tx), lfs = L.prev(ta2.fsphi);
@@ -569,17 +610,17 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             const bool edge_ok = row_act && c >= 0 && c <= mx + 2 && lane >= 1;
             if (edge_ok) {
                 Side sL;
-                const bool second = (c - 1) >= 0;
                 sL.h = lh; sL.hu = lhu; sL.hv = lhv;
-                sL.hm = second ? lhm2 : lhm1;
-                sL.p = second ? lp2 : lp1;
-                sL.hchi = second ? lhc2 : lhc1;
-                sL.m = second ? lm2 : lm1;
-                sL.chi = second ? lc2 : lc1;
+                sL.hm = lhm;
+                sL.p = lp;
+                sL.hchi = lhc;
+                sL.m = lm;
+                sL.chi = lc;
                 sL.u = lu; sL.v = lv;
                 sL.b = lb - lbd;
                 sL.taudir = ltx; sL.fs = lfs;
-                const Side sR = side_of(q2, false, ta2.tx, ta2.fsphi);
+                sL.rho = lrho; sL.tau = ltau;
+                const Side sR = side_of(q2, false, ta2.tx, ta2.fsphi, v1_q2);
                 xw = rpn2_edge(P, 1, sL, sR);
             }
             // limiter neighbours

@@ -158,9 +158,16 @@ extern "C" int emu_rpn2_slice(const dclaw_params* P, int ixy, int mx, int maux,
     for (int k = 0; k < 35 * n; k++) fwave[k] = 0.0;
     for (int k = 0; k < 5 * n; k++) s[k] = 0.0;
     for (int e = 1; e < n; e++) {          // edge between array cells e-1 and e
-        Side sL = side_of(cells[e - 1], e - 1 >= 1, aux2[maux * (e - 1) + itx],
-                          aux2[maux * (e - 1) + 3]);
-        Side sR = side_of(cells[e], false, aux2[maux * e + itx], aux2[maux * e + 3]);
+        const RowCell& cl = cells[e - 1];
+        const RowCell& cr = cells[e];
+        const bool second = e - 1 >= 1;
+        RhoTau vl = {0.0, 0.0}, vr = {0.0, 0.0};
+        if (cl.h > P->dry_tolerance)
+            vl = second ? rho_tau(*P, cl.h, cl.u, cl.v, cl.m2, cl.p2, cl.chi2)
+                        : rho_tau(*P, cl.h, cl.u, cl.v, cl.m1, cl.p1, cl.chi1);
+        if (cr.h > P->dry_tolerance) vr = rho_tau(*P, cr.h, cr.u, cr.v, cr.m1, cr.p1, cr.chi1);
+        Side sL = side_of(cl, second, aux2[maux * (e - 1) + itx], aux2[maux * (e - 1) + 3], vl);
+        Side sR = side_of(cr, false, aux2[maux * e + itx], aux2[maux * e + 3], vr);
         if (ixy == 2) {
             sL.u = cells[e - 1].v; sL.v = cells[e - 1].u;
             sR.u = cells[e].v; sR.v = cells[e].u;
@@ -191,6 +198,12 @@ extern "C" int emu_rpn2_edge(const dclaw_params* P, int ixy, const double* in, d
         S[k].h = x[0]; S[k].hu = x[1]; S[k].hv = x[2]; S[k].hm = x[3]; S[k].p = x[4];
         S[k].hchi = x[5]; S[k].u = x[6]; S[k].v = x[7]; S[k].m = x[8]; S[k].chi = x[9];
         S[k].b = x[10]; S[k].taudir = x[11]; S[k].fs = x[12];
+        S[k].rho = S[k].tau = 0.0;
+        if (S[k].h > P->dry_tolerance) {
+            const RhoTau t = rho_tau(*P, S[k].h, S[k].u, S[k].v, S[k].m, S[k].p, S[k].chi);
+            S[k].rho = t.rho;
+            S[k].tau = t.tau;
+        }
     }
     const EdgeWaves w = rpn2_edge(*P, ixy, S[0], S[1]);
     for (int k = 0; k < 3; k++) out[k] = w.s[k];

@@ -97,3 +97,16 @@ def test_bc_on_load_equals_fill(oracle_mod, emu, bc):
     qe, auxe, ce = emu.stepgrid(pt.params, stale, aux, 1.0, 1.0, dt, seg_rows=8, bc=bc)
     qo, auxo, co = O.stepgrid(O.params_from(pt.params), q, aux, 1.0, 1.0, dt)
     assert np.array_equal(qe, qo) and np.array_equal(auxe, auxo) and ce == co
+
+
+def test_zero_dry_tolerance(oracle_mod, emu):
+    """dry_tolerance = 0: cells with h = 0 exactly are the only dry ones, and
+    rpn2 evaluates setvars on a dry side after resetting it in place."""
+    pt = slope_patch(64, 36, slope=0.9, phi_deg=22.0, seed=5)
+    thin = pt.q[0] < 0.05
+    for m in range(6):
+        pt.q[m][thin] = 0.0
+    pt.params.dry_tolerance = 0.0
+    q, aux, dt = advance_oracle(oracle_mod, pt, 4)
+    assert (q[0][2:-2, 2:-2] == 0.0).any() and (q[0] > 0.0).any()
+    _check(oracle_mod, emu, pt, q, aux, dt, seg_rows=8)
