@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in
     A.jA = 1 + (int)blockIdx.y * seg_rows;
     A.jB = min(my, A.jA + seg_rows - 1);
     GpuLane L;
-    const double cfl = strip_step(L, P, A);
+    const double cfl = strip_step_dispatch(L, P, A);
     wave_max_to(cfl, cfl_dev);
 }
 
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restri
     A.jA = 1 + tiles[4 * t + 2] * seg_rows;
     A.jB = min(pt.my, A.jA + seg_rows - 1);
     GpuLane L;
-    const double cfl = strip_step(L, P, A);
+    const double cfl = strip_step_dispatch(L, P, A);
     wave_max_to(cfl, cfl_dev + pt.member);
 }
 

@@ -740,4 +740,37 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
     return cfl;
 }
 
+// The default method switches of a D-Claw run (second order, transverse
+// = 2, MC limiter on every wave family, src2method 0, alphamethod 0, no
+// rare-corrector test, segregation and Manning friction off).
+DC_HD bool scheme_is_std(const dclaw_params& P)
+{
+    return P.order == 2 && P.transverse == 2 && P.mthlim[0] == 4 && P.mthlim[1] == 4 &&
+           P.mthlim[2] == 4 && P.mthlim[3] == 4 && P.mthlim[4] == 4 && P.src2method == 0 &&
+           P.alphamethod == 0 && P.rarecorrectortest == 0 && P.segregation == 0 &&
+           P.friction_forcing == 0;
+}
+
+// The strip body with the method switches folded to constants when they are
+// the defaults (a copy of the record whose switches are literal constants, so
+// the inlined body drops the branches of the other methods), or read from
+// the record otherwise.  Both copies compute the same thing.
+template <class Lane>
+DC_HD double strip_step_dispatch(Lane& L, const dclaw_params& P, const StripArgs& A)
+{
+    if (scheme_is_std(P)) {
+        dclaw_params Q = P;
+        Q.order = 2;
+        Q.transverse = 2;
+        for (int k = 0; k < 5; k++) Q.mthlim[k] = 4;
+        Q.src2method = 0;
+        Q.alphamethod = 0;
+        Q.rarecorrectortest = 0;
+        Q.segregation = 0;
+        Q.friction_forcing = 0;
+        return strip_step(L, Q, A);
+    }
+    return strip_step(L, P, A);
+}
+
 }  // namespace dclaw

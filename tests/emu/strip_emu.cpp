@@ -64,7 +64,7 @@ void lane_entry(int lid)
 {
     Sched* s = g_sched;
     CoLane L{lid, 0, s};
-    s->cfl[lid] = strip_step(L, *s->P, *s->A);
+    s->cfl[lid] = strip_step_dispatch(L, *s->P, *s->A);
     s->done[lid] = true;
 }
 
