@@ -20,6 +20,12 @@
 // next: of lane l+1; the end lanes keep their own); no LDS tile is held.
 // The code is a template on the lane policy so the same body runs on the GPU
 // (one wavefront) and in a host emulator (64 threads in lockstep).
+//
+// With FLUX = true the same body is GeoClaw's step2 instead of stepgrid: the
+// input is already b4step2's output (no qfix / speed limit on load), fsphi
+// and taudir are read from aux(4..6) instead of recomputed, and S6 stores
+// fm, fp, gm, gp (every entry the reference's sweeps accumulate, including
+// the border rows and columns no update reads) instead of updating q.
 #pragma once
 
 #include "dclaw_physics.h"
@@ -39,6 +45,7 @@ struct StripArgs {
     double dx, dy, dt;
     int i0, jA, jB;        // interior strip start column, segment rows [jA, jB]
     int bc[4];             // bc codes (xlo, xhi, ylo, yhi) applied on load; 0 = use stored ghosts
+    double *fm, *fp, *gm, *gp;   // step2 outputs (FLUX mode; same shape as q, zero-filled)
 };
 
 // Source cell and momentum signs of a ghost cell under the bc2amr rules
@@ -104,6 +111,25 @@ DC_HD void prepare_cell(const dclaw_params& P, const double* q7, double b, RowCe
     qfix(P, h1, hu1, hv1, hm1, p1, hchi1, u, v, m, chi, rho, gz);
     r.hm1 = hm1; r.p1 = p1; r.hchi1 = hchi1; r.m1 = m; r.chi1 = chi; r.u = u; r.v = v;
     // F2
+    double hm2 = hm1, p2 = p1, hchi2 = hchi1;
+    qfix(P, h1, hu1, hv1, hm2, p2, hchi2, u, v, m, chi, rho, gz);
+    r.hm2 = hm2; r.p2 = p2; r.hchi2 = hchi2; r.m2 = m; r.chi2 = chi;
+}
+
+// step2 input: the cell is already b4step2's output (Q0 as stored), then F1
+// and F2 as rpn2 applies them.
+DC_HD void prepare_cell_q0(const dclaw_params& P, const double* q7, double b, RowCell& r)
+{
+    double u, v, m, chi, rho;
+    const double gz = P.grav;
+    r.h = q7[0]; r.hu = q7[1]; r.hv = q7[2]; r.b = b;
+    r.hm0 = q7[3]; r.p0 = q7[4]; r.hchi0 = q7[5];
+    r.bdif = q7[6];
+    double h1 = q7[0], hu1 = q7[1], hv1 = q7[2], hm1 = q7[3], p1 = q7[4], hchi1 = q7[5];
+    qfix(P, h1, hu1, hv1, hm1, p1, hchi1, u, v, m, chi, rho, gz);
+    // rpn2 reads h, hu, hv after its qfix (a dry cell comes back zeroed)
+    r.h = h1; r.hu = hu1; r.hv = hv1;
+    r.hm1 = hm1; r.p1 = p1; r.hchi1 = hchi1; r.m1 = m; r.chi1 = chi; r.u = u; r.v = v;
     double hm2 = hm1, p2 = p1, hchi2 = hchi1;
     qfix(P, h1, hu1, hv1, hm2, p2, hchi2, u, v, m, chi, rho, gz);
     r.hm2 = hm2; r.p2 = p2; r.hchi2 = hchi2; r.m2 = m; r.chi2 = chi;
@@ -298,7 +324,7 @@ DC_HD void load_row(const StripArgs& A, int c, int R, bool col_in_array, double*
 }
 
 // The strip body.  Returns this lane's CFL contribution (max |dt/dx * s|).
-template <class Lane>
+template <bool FLUX, class Lane>
 DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
 {
     const int lane = L.id();
@@ -365,6 +391,7 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             for (int r = r0; r <= r1; r++) w |= (qi[qidx(0, c, r, mx)] > P.dry_tolerance);
             wet = w;
         }
+        if (!L.any(wet) && FLUX) return 0.0;   // fluxes are zero-filled by the caller
         if (!L.any(wet)) {
             // rows of this segment in chunks of 8: all bdif loads first, then stores
             const double* __restrict__ qi = A.q_in;
@@ -424,21 +451,25 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
 
     // Row loads run one iteration ahead of their use (software prefetch):
     // the raw values of row R+1 are requested before row R is processed.
+    // step2's top segment runs one more row to assemble the fluxes of row
+    // my+1 (and gm/gp of row my+2)
+    const int R_end = A.jB + 3 + ((FLUX && A.jB == my) ? 1 : 0);
     double nq7[7], nb;
     load_row(A, c, A.jA - 3, col_in_array, nq7, nb);
-    for (int R = A.jA - 3; R <= A.jB + 3; R++) {
+    for (int R = A.jA - 3; R <= R_end; R++) {
         // ---------------- S1: row R (prefetched), b4step2 pointwise, F1, F2
         {
             double q7[7];
             for (int m = 0; m < 7; m++) q7[m] = nq7[m];
             const double b = nb;
-            if (R + 1 <= A.jB + 3) load_row(A, c, R + 1, col_in_array, nq7, nb);
+            if (R + 1 <= R_end) load_row(A, c, R + 1, col_in_array, nq7, nb);
             const bool row_in = (R >= -1 && R <= my + 2);
-            prepare_cell(P, q7, b, q0);
+            if (FLUX) prepare_cell_q0(P, q7, b, q0);
+            else prepare_cell(P, q7, b, q0);
             const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
                                      (R >= my + 1 && R <= my + 2 && A.jB == my);
             const bool row_interior = (R >= 1 && R <= my);
-            if (row_in && col_in_array &&
+            if (!FLUX && row_in && col_in_array &&
                 ((row_ghost_w && (col_out || col_ghost_w)) || (row_interior && col_ghost_w &&
                                                               R >= A.jA && R <= A.jB))) {
                 const size_t o = qidx(0, c, R, mx);
@@ -475,7 +506,7 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             ta2 = ta1;
             ta1.fsphi = ta1.tx = ta1.ty = 0.0;
             const int j = R - 3;
-            if (j >= A.jA && j <= A.jB && col_out) {
+            if (!FLUX && j >= A.jA && j <= A.jB && col_out) {
                 const size_t o = qidx(0, c, j, mx);
                 for (int z = 0; z < 6; z++) A.q_out[o + z] = 0.0;
                 A.q_out[o + 6] = q3.bdif;
@@ -521,7 +552,15 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             const double rb = L.next(q1.b), rbd = L.next(q1.bdif);
             Taudir t;
             t.fsphi = t.tx = t.ty = 0.0;
-            if (r >= 0 && r <= my + 1 && col_taudir && r >= A.jA - 2 && r <= A.jB + 2) {
+            if (FLUX) {
+                // step2 reads fsphi and taudir as b4step2 left them in aux
+                if (r >= -1 && r <= my + 2 && col_in_array && r >= A.jA - 2 && r <= A.jB + 2) {
+                    const size_t o = aidx(c, r, mx, A.maux);
+                    t.fsphi = A.aux[o + 3];
+                    t.tx = A.aux[o + 4];
+                    t.ty = A.aux[o + 5];
+                }
+            } else if (r >= 0 && r <= my + 1 && col_taudir && r >= A.jA - 2 && r <= A.jB + 2) {
                 Cell C = {q1.h, q1.hu, q1.hv, q1.hm0, q1.p0, q1.hchi0};
                 Cell Lc = {lh, lhu, lhv, lhm, lp, lhc};
                 Cell Rc = {rh, rhu, rhv, rhm, rp, rhc};
@@ -530,7 +569,7 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                 t = calc_taudir_cell(P, A.dx, A.dy, C, q1.b - q1.bdif, Lc, lb - lbd, Rc, rb - rbd,
                                      B, q2.b - q2.bdif, T, q0.b - q0.bdif, sv1);
             }
-            if (A.aux_out && r >= 0 && r <= my + 1 && col_taudir) {
+            if (!FLUX && A.aux_out && r >= 0 && r <= my + 1 && col_taudir) {
                 const bool row_own = (r >= A.jA && r <= A.jB) || (r == 0 && A.jA == 1) ||
                                      (r == my + 1 && A.jB == my);
                 const bool col_own = (c >= A.i0 && c <= A.i0 + STRIP_W - 1 && c >= 1 && c <= mx) ||
@@ -647,13 +686,18 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             double amdq_next[6];
             for (int z = 0; z < 6; z++) amdq_next[z] = L.next(ef.amdq[z]);
             if (P.transverse != 0) {
-                const bool cell_ok = row_act && c >= 1 && c <= mx && lane >= 2 && lane <= 61;
+                // imp1 (amdq of edge c+1) reaches cells 0..mx, imp2 (apdq of edge c)
+                // cells 1..mx+1 (flux2's loops over edges 1..mx+1); the update
+                // reads cells 1..mx, step2's gm/gp also cell mx+1
+                const bool cell_ok1 = row_act && c >= 1 && c <= mx && lane >= 2 && lane <= 61;
+                const bool cell_ok2 = row_act && c >= 1 && c <= (FLUX ? mx + 1 : mx) &&
+                                      lane >= 2 && lane <= 61;
                 double bm1[6], bp1[6], bm2[6], bp2[6];
                 rpt2_cell_split(P, 1, q2.h, q2.hu, q2.hv, q2.hm2, q2.hchi2, q2.b, q2.bdif, q3.b,
-                                q1.b, amdq_next, cell_ok, bm1, bp1);
+                                q1.b, amdq_next, cell_ok1, bm1, bp1);
                 rpt2_cell_split(P, 1, q2.h, q2.hu, q2.hv, q2.hm2, q2.hchi2, q2.b, q2.bdif, q3.b,
-                                q1.b, ef.apdq, cell_ok, bm2, bp2);
-                if (cell_ok) trans_combine(bm1, bp1, bm2, bp2, dtdx, Gx1, Gx2);
+                                q1.b, ef.apdq, cell_ok2, bm2, bp2);
+                if (cell_ok1 || cell_ok2) trans_combine(bm1, bp1, bm2, bp2, dtdx, Gx1, Gx2);
             }
         }
 
@@ -712,7 +756,37 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                 gm_next[z] = t + fym[z];
                 gp_next[z] = t + fyp[z];
             }
-            if (j >= A.jA && j <= A.jB && col_out) {
+            if (FLUX) {
+                // fm/fp(c, j): rows 1..my (+ 0 and my+1 at the patch edges), columns
+                // 1..mx (+ 0, mx+1, mx+2); gm/gp(c, j+1): rows 1..my (+ 0, my+1,
+                // my+2), columns 1..mx (+ 0, mx+1).  Row 0 of fm/fp has no y-sweep
+                // part (step2 accumulates the y-sweeps' transverse terms for rows
+                // 1..my+1 only).
+                const bool first_s = (A.i0 == 1), last_s = (A.i0 + STRIP_W - 1 >= mx);
+                const bool in_s = (c >= A.i0 && c <= A.i0 + STRIP_W - 1 && c >= 1 && c <= mx);
+                const bool col_f = in_s || (first_s && c == 0) ||
+                                   (last_s && (c == mx + 1 || c == mx + 2));
+                const bool col_g = in_s || (first_s && c == 0) || (last_s && c == mx + 1);
+                const bool row_f = (j >= A.jA && j <= A.jB) || (j == 0 && A.jA == 1) ||
+                                   (j == my + 1 && A.jB == my);
+                const int jg = j + 1;
+                const bool row_g = (jg >= A.jA && jg <= A.jB) || (jg == 0 && A.jA == 1) ||
+                                   ((jg == my + 1 || jg == my + 2) && A.jB == my);
+                if (row_f && col_f) {
+                    const size_t o = qidx(0, c, j, mx);
+                    for (int z = 0; z < 6; z++) {
+                        A.fm[o + z] = (j == 0) ? 0.0 + fxm_prev[z] : fmc[z];
+                        A.fp[o + z] = (j == 0) ? 0.0 + fxp_prev[z] : fpc[z];
+                    }
+                }
+                if (row_g && col_g) {
+                    const size_t o = qidx(0, c, jg, mx);
+                    for (int z = 0; z < 6; z++) {
+                        A.gm[o + z] = gm_next[z];
+                        A.gp[o + z] = gp_next[z];
+                    }
+                }
+            } else if (j >= A.jA && j <= A.jB && col_out) {
                 double qn[6] = {q3.h, q3.hu, q3.hv, q3.hm0, q3.p0, q3.hchi0};
                 for (int z = 0; z < 6; z++)
                     qn[z] = qn[z] - dtdx * (fm_next[z] - fpc[z]) - dtdy * (gm_next[z] - gp_prev[z]);
@@ -755,7 +829,7 @@ DC_HD bool scheme_is_std(const dclaw_params& P)
 // the defaults (a copy of the record whose switches are literal constants, so
 // the inlined body drops the branches of the other methods), or read from
 // the record otherwise.  Both copies compute the same thing.
-template <class Lane>
+template <bool FLUX, class Lane>
 DC_HD double strip_step_dispatch(Lane& L, const dclaw_params& P, const StripArgs& A)
 {
     if (scheme_is_std(P)) {
@@ -768,9 +842,9 @@ DC_HD double strip_step_dispatch(Lane& L, const dclaw_params& P, const StripArgs
         Q.rarecorrectortest = 0;
         Q.segregation = 0;
         Q.friction_forcing = 0;
-        return strip_step(L, Q, A);
+        return strip_step<FLUX>(L, Q, A);
     }
-    return strip_step(L, P, A);
+    return strip_step<FLUX>(L, P, A);
 }
 
 }  // namespace dclaw

@@ -84,6 +84,56 @@ def stepgrid_bc(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux
         _ptr(params_dev), _ptr(cfl_dev), _stream()))
 
 
+def aux_tensor(aux: np.ndarray, device="cuda") -> torch.Tensor:
+    """The whole aux(maux, mx+4, my+4) array as a device tensor of shape
+    (my+4, mx+4, maux) (the same bytes, component fastest)."""
+    af = np.asfortranarray(aux, dtype=np.float64)
+    return torch.from_numpy(np.ascontiguousarray(af.transpose(2, 1, 0))).to(device)
+
+
+def to_host_aux(t: torch.Tensor) -> np.ndarray:
+    return np.asfortranarray(t.detach().cpu().numpy().transpose(2, 1, 0))
+
+
+NO_NAN = 0x7FFFFFFF
+
+
+def nan_flag(device="cuda") -> torch.Tensor:
+    """A device int32 for check4nans results, initialised to 'none'."""
+    return torch.full((1,), NO_NAN, dtype=torch.int32, device=device)
+
+
+def check4nans(q: torch.Tensor, mx: int, my: int, flag: torch.Tensor) -> None:
+    _lib.check(_lib.lib().dclaw_check4nans_dev(_ptr(q), mx, my, _ptr(flag), _stream()))
+
+
+def b4step2(q: torch.Tensor, aux: torch.Tensor, mx: int, my: int, dx: float, dy: float,
+            params_dev: torch.Tensor, flag: Optional[torch.Tensor] = None) -> None:
+    """b4step2 in place on q and aux(4..6) (aux: full (my+4, mx+4, maux) tensor)."""
+    _lib.check(_lib.lib().dclaw_b4step2_dev(
+        _ptr(q), _ptr(aux), aux.shape[2], mx, my, dx, dy, _ptr(params_dev),
+        None if flag is None else _ptr(flag), _stream()))
+
+
+def step2(q: torch.Tensor, aux: torch.Tensor, mx: int, my: int, dx: float, dy: float,
+          dt_dev: torch.Tensor, params_dev: torch.Tensor, cfl_dev: torch.Tensor):
+    """GeoClaw step2 on device buffers: returns (fm, fp, gm, gp) tensors shaped
+    like q; cfl_dev receives max(cfl_dev, cflgrid)."""
+    outs = [torch.empty_like(q) for _ in range(4)]
+    _lib.check(_lib.lib().dclaw_step2_dev(
+        _ptr(q), _ptr(aux), aux.shape[2], mx, my, dx, dy, _ptr(dt_dev), _ptr(params_dev),
+        *[_ptr(o) for o in outs], _ptr(cfl_dev), _stream()))
+    return outs
+
+
+def src2(q: torch.Tensor, aux: torch.Tensor, mx: int, my: int, dt_dev: torch.Tensor,
+         params_dev: torch.Tensor, flag: Optional[torch.Tensor] = None) -> None:
+    """src2 in place on the interior cells of q."""
+    _lib.check(_lib.lib().dclaw_src2_dev(
+        _ptr(q), _ptr(aux), aux.shape[2], mx, my, _ptr(dt_dev), _ptr(params_dev),
+        None if flag is None else _ptr(flag), _stream()))
+
+
 def dt_update(dt_dev: torch.Tensor, cfl_dev: torch.Tensor, cfl_desired: float,
               dt_max: float) -> None:
     _lib.check(_lib.lib().dclaw_dt_update(_ptr(dt_dev), _ptr(cfl_dev), dt_dev.numel(),

@@ -129,6 +129,38 @@ int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t
 int dclaw_dt_update(double* dt_dev, double* cfl_dev, int n, double cfl_desired,
                     double dt_max, void* stream);
 
+/* ---- Per-stage device entry points (GeoClaw stepgrid's stages one at a
+ * time, on device memory; the fused dclaw_stepgrid_dev equals b4step2, step2,
+ * the conservative update and src2 in sequence). ---- */
+
+/* check4nans (AMRClaw; called at b4step2.f90:49 and src2.f90:73): the
+ * smallest flat index (component fastest, q(m,i,j) -> m-1 + 7*((i+1) +
+ * (mx+4)*(j+1))) of a NaN anywhere in q, ghosts included, min-reduced into
+ * *first_nan_dev; initialise it to 0x7fffffff (none). */
+int dclaw_check4nans_dev(const double* q, int mx, int my, int* first_nan_dev, void* stream);
+
+/* b4step2 (b4step2.f90:7-113) in place: qfix and bdif >= 0 on every cell
+ * (:56-64), the speed limit (:74-100), then calc_taudir
+ * (digclaw_module.f90:500-719) into aux(4..6) on cells 0..mx+1 x 0..my+1
+ * (maux >= 6).  The check4nans of :49 runs first when first_nan_dev is not
+ * NULL. */
+int dclaw_b4step2_dev(double* q, double* aux, int maux, int mx, int my, double dx, double dy,
+                      const dclaw_params* params_dev, int* first_nan_dev, void* stream);
+
+/* step2 (GeoClaw step2.f90 with flux2fw.f:8-286, rpn2_dclaw.f:24, rpt2_dclaw.f:16)
+ * on the b4step2 output q and aux (maux >= 6; aux(4..6) read as fsphi and
+ * taudir): fm, fp, gm, gp (same shape as q; every entry overwritten, in the
+ * reference's summation order) and the max into *cfl_dev (cflgrid). */
+int dclaw_step2_dev(const double* q, const double* aux, int maux, int mx, int my, double dx,
+                    double dy, const double* dt_dev, const dclaw_params* params_dev, double* fm,
+                    double* fp, double* gm, double* gp, double* cfl_dev, void* stream);
+
+/* src2 (src2.f90:27-306) in place on the interior cells; check4nans (:73)
+ * first when first_nan_dev is not NULL.  aux is not read on this path
+ * (bed_normal, curvature and entrainment off). */
+int dclaw_src2_dev(double* q, const double* aux, int maux, int mx, int my, const double* dt_dev,
+                   const dclaw_params* params_dev, int* first_nan_dev, void* stream);
+
 /* Host-memory drop-ins with the Fortran calling convention (all arguments by
  * reference, q/aux in the Clawpack layout), for AMRClaw executables that
  * swap these objects in through Makefile SOURCES/EXCLUDE_SOURCES
@@ -140,6 +172,27 @@ int dclaw_set_params(const dclaw_params* params_host);
 void stepgrid_dclaw_(double* q, double* aux, const int* mbc, const int* mx, const int* my,
                      const int* maux, const double* dx, const double* dy, const double* dt,
                      double* cflgrid);
+
+/* Per-stage Fortran drop-ins with the reference's own argument lists, for
+ * executables that keep GeoClaw's stepgrid and exclude the D-Claw stage
+ * sources instead (Makefile.dclaw:155-173).  Host memory; each call copies
+ * the patch to and from the device.  A NaN in q stops the run with
+ * check4nans' message, like the reference.
+ *   b4step2_: b4step2.f90:7 (setaux is not re-run: aux must be final)
+ *   step2_:   GeoClaw step2 (Clawpack 5 form); rpn2/rpt2 are ignored (the
+ *             D-Claw solvers are built in); maxm is not needed
+ *   src2_:    src2.f90:27 */
+void b4step2_(const int* mbc, const int* mx, const int* my, const int* meqn, double* q,
+              const double* xlower, const double* ylower, const double* dx, const double* dy,
+              const double* t, const double* dt, const int* maux, double* aux,
+              const int* actualstep);
+void step2_(const int* maxm, const int* meqn, const int* maux, const int* mbc, const int* mx,
+            const int* my, const double* qold, const double* aux, const double* dx,
+            const double* dy, const double* dt, double* cflgrid, double* fm, double* fp,
+            double* gm, double* gp, void* rpn2, void* rpt2);
+void src2_(const int* meqn, const int* mbc, const int* mx, const int* my, const double* xlower,
+           const double* ylower, const double* dx, const double* dy, double* q, const int* maux,
+           const double* aux, const double* t, const double* dt);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,7 @@ struct Sched {
     bool done[STRIP_LANES];
     const dclaw_params* P;
     const StripArgs* A;
+    bool flux;
     std::vector<char> stacks;
 };
 
@@ -64,16 +65,18 @@ void lane_entry(int lid)
 {
     Sched* s = g_sched;
     CoLane L{lid, 0, s};
-    s->cfl[lid] = strip_step_dispatch(L, *s->P, *s->A);
+    s->cfl[lid] = s->flux ? strip_step_dispatch<true>(L, *s->P, *s->A)
+                          : strip_step_dispatch<false>(L, *s->P, *s->A);
     s->done[lid] = true;
 }
 
-void run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out)
+void run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out, bool flux = false)
 {
     constexpr size_t STACK = 1 << 20;
     Sched* s = new Sched();
     s->P = &P;
     s->A = &A;
+    s->flux = flux;
     s->stacks.resize(STACK * STRIP_LANES);
     g_sched = s;
     for (int l = 0; l < STRIP_LANES; l++) {
@@ -123,7 +126,45 @@ extern "C" int emu_stepgrid(const dclaw_params* P, const double* q_in, double* q
             A.jA = 1 + g * seg_rows;
             A.jB = std::min(my, A.jA + seg_rows - 1);
             for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
+            A.fm = A.fp = A.gm = A.gp = nullptr;
             run_strip(*P, A, cflgrid);
+        }
+    return 0;
+}
+
+// step2 (flux mode of the strip body) on the b4step2 output: fm, fp, gm, gp
+// are zero-filled here, then the strips store their entries.
+extern "C" int emu_step2(const dclaw_params* P, const double* q, const double* aux, int maux,
+                         int mx, int my, double dx, double dy, double dt, int seg_rows,
+                         double* fm, double* fp, double* gm, double* gp, double* cflgrid)
+{
+    *cflgrid = 0.0;
+    const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
+    for (size_t k = 0; k < nq; k++) fm[k] = fp[k] = gm[k] = gp[k] = 0.0;
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
+    for (int s = 0; s < nstrips; s++)
+        for (int g = 0; g < nseg; g++) {
+            StripArgs A;
+            A.q_in = q;
+            A.q_out = nullptr;
+            A.aux = aux;
+            A.aux_out = nullptr;
+            A.maux = maux;
+            A.mx = mx;
+            A.my = my;
+            A.dx = dx;
+            A.dy = dy;
+            A.dt = dt;
+            A.i0 = 1 + s * STRIP_W;
+            A.jA = 1 + g * seg_rows;
+            A.jB = std::min(my, A.jA + seg_rows - 1);
+            for (int k = 0; k < 4; k++) A.bc[k] = 0;
+            A.fm = fm;
+            A.fp = fp;
+            A.gm = gm;
+            A.gp = gp;
+            run_strip(*P, A, cflgrid, true);
         }
     return 0;
 }

@@ -62,3 +62,15 @@ def test_params_struct_layout():
     from oracle.oracle import OracleParams
     assert [f[0] for f in CParams._fields_] == [f[0] for f in OracleParams._fields_]
     assert ctypes.sizeof(CParams) == 20 * 8 + 13 * 4 + 4
+
+
+def test_stage_argument_validation(L):
+    """The per-stage entry points reject bad arguments before touching the GPU."""
+    assert L.dclaw_b4step2_dev(1, 2, 1, 10, 10, 1.0, 1.0, 3, None, None) == -1
+    assert b"maux" in L.dclaw_last_error()
+    assert L.dclaw_step2_dev(1, 2, 6, 10, 10, 1.0, 1.0, 3, 4, None, 5, 6, 7, 8, None) == -1
+    assert b"null" in L.dclaw_last_error()
+    assert L.dclaw_step2_dev(1, 2, 6, 10, 10, 0.0, 1.0, 3, 4, 5, 5, 6, 7, 8, None) == -1
+    assert L.dclaw_src2_dev(None, 2, 6, 10, 10, 3, 4, None, None) == -1
+    assert L.dclaw_check4nans_dev(1, 0, 10, 2, None) == -1
+    assert L.dclaw_check4nans_dev(1, 100000, 100000, 2, None) == -1   # flat index overflows

@@ -110,3 +110,36 @@ def test_zero_dry_tolerance(oracle_mod, emu):
     q, aux, dt = advance_oracle(oracle_mod, pt, 4)
     assert (q[0][2:-2, 2:-2] == 0.0).any() and (q[0] > 0.0).any()
     _check(oracle_mod, emu, pt, q, aux, dt, seg_rows=8)
+
+
+def _emu_step2(lib, params, q, aux, dx, dy, dt, seg):
+    from dclaw_amd.params import to_c
+    dp = ctypes.POINTER(ctypes.c_double)
+    q = np.asfortranarray(q)
+    aux = np.asfortranarray(aux)
+    f = [np.zeros_like(q, order="F") for _ in range(4)]
+    cfl = ctypes.c_double(0.0)
+    cp = to_c(params)
+    lib.emu_step2(ctypes.byref(cp), q.ctypes.data_as(dp), aux.ctypes.data_as(dp), aux.shape[0],
+                  q.shape[1] - 4, q.shape[2] - 4, ctypes.c_double(dx), ctypes.c_double(dy),
+                  ctypes.c_double(dt), seg, *[a.ctypes.data_as(dp) for a in f], ctypes.byref(cfl))
+    return cfl.value, f
+
+
+@pytest.mark.parametrize("mx,my,seg", [(70, 40, 16), (130, 33, 8), (59, 61, 4), (7, 9, 32),
+                                       (1, 1, 32)])
+def test_step2_flux_mode_bitwise(oracle_mod, emu_lib, mx, my, seg):
+    """The strip body in step2 mode: every entry of fm, fp, gm, gp (border rows
+    and columns included) and cflgrid equal the oracle's step2 bit for bit."""
+    O = oracle_mod
+    pt = slope_patch(mx, my, slope=0.9, phi_deg=22.0, seed=mx)
+    q, aux, dt = advance_oracle(O, pt, 5)
+    P = O.params_from(pt.params)
+    qb, auxb = O.b4step2(P, q, aux, pt.dx, pt.dy)
+    cflo, *fo = O.step2(P, qb, auxb, pt.dx, pt.dy, dt)
+    cfle, fe = _emu_step2(emu_lib, pt.params, qb, auxb, pt.dx, pt.dy, dt, seg)
+    assert cfle == cflo
+    for name, a, b in zip(("fm", "fp", "gm", "gp"), fe, fo):
+        assert np.array_equal(a, b), f"{name}: {(a != b).sum()} entries differ"
+    if mx * my > 1:      # a single cell with extrapolated ghosts has no flux
+        assert np.count_nonzero(fo[0]) > 0

@@ -1,0 +1,169 @@
+"""GPU: the per-stage entry points (b4step2, step2, src2, check4nans) against
+the oracle's stages, their composition against the fused step, and the
+Fortran-convention per-stage host drop-ins.  The stages share the fused
+kernel's arithmetic, so the comparisons are bit for bit."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import advance_oracle, slope_patch
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(70, 40, 16), (130, 33, 8), (59, 61, 4), (7, 9, 32)]
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dclaw_amd import _lib
+    _lib.lib()
+    return torch
+
+
+def _state(O, mx, my, seed):
+    pt = slope_patch(mx, my, slope=0.9, phi_deg=22.0, seed=seed)
+    q, aux, dt = advance_oracle(O, pt, 5)
+    return pt, q, aux, dt
+
+
+def _scalar(dev, v):
+    return dev.tensor([v], dtype=dev.float64, device="cuda")
+
+
+@pytest.mark.parametrize("mx,my,seg", SIZES)
+def test_b4step2_equals_oracle(dev, oracle_mod, mx, my, seg):
+    from dclaw_amd import step as S
+    O = oracle_mod
+    pt, q, aux, _ = _state(O, mx, my, mx)
+    print(f"[b4step2] {mx}x{my}", flush=True)
+    qd, ad = S.to_device_q(q), S.aux_tensor(aux)
+    flag = S.nan_flag()
+    S.b4step2(qd, ad, mx, my, pt.dx, pt.dy, S.params_tensor([pt.params]), flag)
+    qo, auxo = O.b4step2(O.params_from(pt.params), q, aux, pt.dx, pt.dy)
+    assert np.array_equal(S.to_host_q(qd), qo)
+    assert np.array_equal(S.to_host_aux(ad), auxo)
+    assert flag.item() == S.NO_NAN
+
+
+@pytest.mark.parametrize("mx,my,seg", SIZES)
+def test_step2_equals_oracle(dev, oracle_mod, mx, my, seg):
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    O = oracle_mod
+    pt, q, aux, dt = _state(O, mx, my, mx + 1)
+    P = O.params_from(pt.params)
+    qb, auxb = O.b4step2(P, q, aux, pt.dx, pt.dy)
+    print(f"[step2] {mx}x{my} seg {seg}", flush=True)
+    prev = _lib.lib().dclaw_set_seg_rows(seg)
+    try:
+        cfl = _scalar(dev, 0.0)
+        outs = S.step2(S.to_device_q(qb), S.aux_tensor(auxb), mx, my, pt.dx, pt.dy,
+                       _scalar(dev, dt), S.params_tensor([pt.params]), cfl)
+    finally:
+        _lib.lib().dclaw_set_seg_rows(prev)
+    cflo, *fo = O.step2(P, qb, auxb, pt.dx, pt.dy, dt)
+    for name, a, b in zip(("fm", "fp", "gm", "gp"), outs, fo):
+        a = S.to_host_q(a)
+        assert np.array_equal(a, b), f"{name}: {(a != b).sum()} entries differ"
+    assert cfl.item() == cflo
+
+
+@pytest.mark.parametrize("mx,my,seg", SIZES[:2])
+def test_src2_equals_oracle(dev, oracle_mod, mx, my, seg):
+    from dclaw_amd import step as S
+    O = oracle_mod
+    pt, q, aux, dt = _state(O, mx, my, mx + 2)
+    print(f"[src2] {mx}x{my}", flush=True)
+    qd = S.to_device_q(q)
+    S.src2(qd, S.aux_tensor(aux), mx, my, _scalar(dev, dt), S.params_tensor([pt.params]))
+    qo = O.src2(O.params_from(pt.params), q, aux, dt)
+    assert np.array_equal(S.to_host_q(qd), qo)
+
+
+def test_stages_compose_to_fused_step(dev, oracle_mod):
+    """b4step2 -> step2 -> q -= dtdx (fm(i+1)-fp(i)) + dtdy (gm(j+1)-gp(j)) ->
+    src2 gives the fused step's result bit for bit."""
+    from dclaw_amd import step as S
+    O = oracle_mod
+    mx, my = 97, 64
+    pt, q, aux, dt = _state(O, mx, my, 3)
+    pdev = S.params_tensor([pt.params])
+    dtd = _scalar(dev, dt)
+    print(f"[compose] {mx}x{my}", flush=True)
+    # fused
+    qa, qb_ = S.to_device_q(q), S.to_device_q(q)
+    cfl_f = _scalar(dev, 0.0)
+    S.stepgrid(qa, qb_, S.bed_tensor(aux), 1, mx, my, pt.dx, pt.dy, dtd, pdev, cfl_f)
+    fused = S.to_host_q(qb_)
+    # staged
+    qd, ad = S.to_device_q(q), S.aux_tensor(aux)
+    S.b4step2(qd, ad, mx, my, pt.dx, pt.dy, pdev)
+    cfl_s = _scalar(dev, 0.0)
+    fm, fp, gm, gp = (S.to_host_q(t) for t in S.step2(qd, ad, mx, my, pt.dx, pt.dy, dtd, pdev,
+                                                       cfl_s))
+    qh = S.to_host_q(qd)
+    dtdx, dtdy = dt / pt.dx, dt / pt.dy
+    I, J = slice(2, mx + 2), slice(2, my + 2)
+    for m in range(6):
+        qh[m, I, J] = (qh[m, I, J] - dtdx * (fm[m, 3:mx + 3, J] - fp[m, I, J])
+                       - dtdy * (gm[m, I, 3:my + 3] - gp[m, I, J]))
+    qd2 = S.to_device_q(qh)
+    S.src2(qd2, ad, mx, my, dtd, pdev)
+    staged = S.to_host_q(qd2)
+    assert np.array_equal(staged, fused)
+    assert cfl_s.item() == cfl_f.item()
+
+
+def test_check4nans_reports_first(dev):
+    from dclaw_amd import step as S
+    mx, my = 30, 20
+    q = np.zeros((7, mx + 4, my + 4), order="F")
+    flag = S.nan_flag()
+    qd = S.to_device_q(q)
+    S.check4nans(qd, mx, my, flag)
+    assert flag.item() == S.NO_NAN
+    q[4, 10, 7] = np.nan
+    q[2, 20, 15] = np.nan
+    S.check4nans(S.to_device_q(q), mx, my, flag)
+    assert flag.item() == 4 + 7 * (10 + (mx + 4) * 7)
+
+
+def test_host_stage_dropins(dev, oracle_mod):
+    """b4step2_, step2_ and src2_ with the reference's Fortran argument lists."""
+    from dclaw_amd import _lib
+    from dclaw_amd.params import to_c
+    O = oracle_mod
+    L = _lib.lib()
+    mx, my = 66, 47
+    pt, q, aux, dt = _state(O, mx, my, 9)
+    P = O.params_from(pt.params)
+    cp = to_c(pt.params)
+    _lib.check(L.dclaw_set_params(ctypes.byref(cp)))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    maux = aux.shape[0]
+    print(f"[host stages] {mx}x{my}", flush=True)
+    qh = np.array(q, order="F", copy=True)
+    ah = np.array(aux, order="F", copy=True)
+    L.b4step2_(ci(2), ci(mx), ci(my), ci(7), qh.ctypes.data_as(dp), cd(0.0), cd(0.0), cd(pt.dx),
+               cd(pt.dy), cd(0.0), cd(dt), ci(maux), ah.ctypes.data_as(dp), ci(1))
+    qo, auxo = O.b4step2(P, q, aux, pt.dx, pt.dy)
+    assert np.array_equal(qh, qo) and np.array_equal(ah, auxo)
+    f = [np.zeros_like(qh, order="F") for _ in range(4)]
+    cfl = ctypes.c_double(0.0)
+    L.step2_(ci(max(mx, my)), ci(7), ci(maux), ci(2), ci(mx), ci(my), qh.ctypes.data_as(dp),
+             ah.ctypes.data_as(dp), cd(pt.dx), cd(pt.dy), cd(dt), ctypes.byref(cfl),
+             *[a.ctypes.data_as(dp) for a in f], None, None)
+    cflo, *fo = O.step2(P, qo, auxo, pt.dx, pt.dy, dt)
+    assert cfl.value == cflo
+    for a, b in zip(f, fo):
+        assert np.array_equal(a, b)
+    q2 = np.array(qo, order="F", copy=True)
+    L.src2_(ci(7), ci(2), ci(mx), ci(my), cd(0.0), cd(0.0), cd(pt.dx), cd(pt.dy),
+            q2.ctypes.data_as(dp), ci(maux), ah.ctypes.data_as(dp), cd(0.0), cd(dt))
+    assert np.array_equal(q2, O.src2(P, qo, auxo, dt))
