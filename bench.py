@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--seg-rows", type=int, default=0,
                     help="rows per segment tile; 0 = pick with the autotuner (untimed)")
+    ap.add_argument("--fused", action="store_true",
+                    help="run src2 inside the strip kernel (one launch per step) instead of "
+                         "the default split step (strip kernel + src2 pass over wet tiles)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy (the HBM "
                          "counter calibration dispatch for rocprofv3 --pmc runs)")
@@ -78,7 +81,7 @@ def main():
     print(f"[bench] workload {n}x{n} built in {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
 
-    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_src2=not args.fused)
     r.set_dt(1e-3)
     r.cfl.zero_()
     cfl_desired, dt_max = 0.9, 1e9
@@ -86,8 +89,12 @@ def main():
     def one_step(ev=None):
         if ev is not None:
             ev[0].record()
-        S.stepgrid_bc(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.bc, r.dt, r.params,
-                      r.cfl)
+        if r.split_src2:
+            S.stepgrid_split(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.bc, r.dt, r.params,
+                             r.cfl, r.flags)
+        else:
+            S.stepgrid_bc(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.bc, r.dt, r.params,
+                          r.cfl)
         if ev is not None:
             ev[1].record()
         r.qa, r.qb = r.qb, r.qa
@@ -169,10 +176,13 @@ def main():
                                f"seeded fractal bed + 20 deg tilt, 64 elliptical sources, "
                                f"forward only (adjoint not built)",
                    "mx": n, "my": n, "seed": args.seed, "cfl_desired": cfl_desired,
-                   "wet_cells_end": wet, "seg_rows": seg},
+                   "wet_cells_end": wet, "seg_rows": seg,
+                   "src2": "fused" if args.fused else "split"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stepgrid", "kernel_avg_ms": kern_avg_s * 1e3},
+                     "kernel": ("k_stepgrid<0>" if args.fused
+                                else "k_stepgrid<1> + k_src2_tiles (one step)"),
+                     "kernel_avg_ms": kern_avg_s * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -193,8 +203,9 @@ def bench_ensemble(args):
     tune_times = None
     if seg <= 0:
         from dclaw_amd import autotune
-        seg, tune_times = autotune.tune_ensemble(members, lambda h: S.Ensemble(members, seg_rows=h))
-    ens = S.Ensemble(members, seg_rows=seg)
+        seg, tune_times = autotune.tune_ensemble(
+            members, lambda h: S.Ensemble(members, seg_rows=h, split_src2=not args.fused))
+    ens = S.Ensemble(members, seg_rows=seg, split_src2=not args.fused)
     cells = sum(m.patch.mx * m.patch.my for m in members)
     print(f"[bench] ensemble of {len(members)} members, {cells} cells, built in "
           f"{time.time() - t0:.1f}s, {ens.ntiles} tiles of {seg} rows (tuner: {tune_times})",
@@ -240,15 +251,19 @@ def bench_ensemble(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"batched ensemble forward, {len(members)} members, sides "
-                               f"log-uniform in [64,512], per-member phi/m0/kref, one launch per "
-                               f"step (adjoint not built)", "members": len(members),
-                   "cells": cells, "seed": args.seed, "seg_rows": seg},
+                               f"log-uniform in [64,512], per-member phi/m0/kref, one batched "
+                               f"launch per step (plus the src2 pass over wet tiles unless "
+                               f"fused; adjoint not built)", "members": len(members),
+                   "cells": cells, "seed": args.seed, "seg_rows": seg,
+                   "src2": "fused" if args.fused else "split"},
         "plain_loop": {"value": plain, "steps": nplain,
                        "ms_per_step": t_p / nplain * 1e3},
         "batched_over_plain": value / plain,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step_batched", "kernel_avg_ms": kern_avg_s * 1e3},
+                     "kernel": ("k_step_batched<0>" if args.fused
+                                else "k_step_batched<1> + k_src2_batched (one step)"),
+                     "kernel_avg_ms": kern_avg_s * 1e3},
         "cpu_baseline": None,
     }
     print(json.dumps(out))

@@ -58,7 +58,11 @@ SIGNATURES = {
     "dclaw_worklist_build": (ctypes.c_int, [_ip, _ip, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int32)]),
     "dclaw_step_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
-                                          _ip, _vp, _vp, _vp, _vp]),
+                                          _ip, _vp, _vp, _vp, _vp, _vp]),
+    "dclaw_tile_count": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "dclaw_stepgrid_split_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                _ip, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "dclaw_dt_update": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, _vp]),
     "dclaw_set_params": (ctypes.c_int, [ctypes.POINTER(CParams)]),

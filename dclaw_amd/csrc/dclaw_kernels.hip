@@ -76,6 +76,7 @@ __device__ __forceinline__ void wave_max_to(double v, double* dst)
 
 int g_seg_rows = 32;   // rows per segment of the single-patch launch (tunable)
 
+template <int MODE>
 __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in,
                                                  double* __restrict__ q_out,
                                                  const double* __restrict__ aux,
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in
                                                  const double* __restrict__ dt_dev,
                                                  const dclaw_params* __restrict__ params,
                                                  double* cfl_dev, int seg_rows, int bx0, int bx1,
-                                                 int by0, int by1)
+                                                 int by0, int by1, int* tile_flags)
 {
     const dclaw_params P = *params;
     StripArgs A;
@@ -105,18 +106,21 @@ __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in
     A.i0 = 1 + (int)blockIdx.x * STRIP_W;
     A.jA = 1 + (int)blockIdx.y * seg_rows;
     A.jB = min(my, A.jA + seg_rows - 1);
+    A.fm = A.fp = A.gm = A.gp = nullptr;
+    A.tile_flag = tile_flags ? tile_flags + (blockIdx.y * gridDim.x + blockIdx.x) : nullptr;
     GpuLane L;
-    const double cfl = strip_step_dispatch<false>(L, P, A);
+    const double cfl = strip_step_dispatch<MODE>(L, P, A);
     wave_max_to(cfl, cfl_dev);
 }
 
+template <int MODE>
 __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restrict__ patches,
                                                      const int32_t* __restrict__ tiles,
                                                      int seg_rows,
                                                      const double* __restrict__ dt_dev,
                                                      const dclaw_params* __restrict__ params,
                                                      double* cfl_dev, int bx0, int bx1, int by0,
-                                                     int by1)
+                                                     int by1, int* tile_flags)
 {
     const int t = (int)blockIdx.x;
     const int pidx = tiles[4 * t + 0];
@@ -140,9 +144,60 @@ __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restri
     A.i0 = 1 + tiles[4 * t + 1] * STRIP_W;
     A.jA = 1 + tiles[4 * t + 2] * seg_rows;
     A.jB = min(pt.my, A.jA + seg_rows - 1);
+    A.fm = A.fp = A.gm = A.gp = nullptr;
+    A.tile_flag = tile_flags ? tile_flags + t : nullptr;
     GpuLane L;
-    const double cfl = strip_step_dispatch<false>(L, P, A);
+    const double cfl = strip_step_dispatch<MODE>(L, P, A);
     wave_max_to(cfl, cfl_dev + pt.member);
+}
+
+// src2 over the interior cells of the tiles the no-src2 step flagged (one
+// patch; tile t = segment * nstrips + strip).  One wavefront per task, a task
+// being a 64-cell chunk of a tile, so the wet tiles spread over the whole chip;
+// grid-stride over the tasks.
+__global__ __launch_bounds__(64) void k_src2_tiles(double* q, int mx, int my, int seg_rows,
+                                                   int nstrips, int ntiles,
+                                                   const int* __restrict__ tile_flags,
+                                                   const double* __restrict__ dt_dev,
+                                                   const dclaw_params* __restrict__ params)
+{
+    const dclaw_params P = *params;
+    const double dt = *dt_dev;
+    const int nchunk = (STRIP_W * seg_rows + 63) / 64;
+    const long ntask = (long)ntiles * nchunk;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int t = (int)(task / nchunk);
+        if (!tile_flags[t]) continue;
+        const int i0 = 1 + (t % nstrips) * STRIP_W;
+        const int jA = 1 + (t / nstrips) * seg_rows;
+        const int jB = min(my, jA + seg_rows - 1);
+        const int k = (int)(task % nchunk) * 64 + (int)threadIdx.x;
+        if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, q, mx, i0, jA, jB, k);
+    }
+}
+
+// The same over a batched work list (each tile's patch, parameters and dt).
+__global__ __launch_bounds__(64) void k_src2_batched(const dclaw_patch* __restrict__ patches,
+                                                     const int32_t* __restrict__ tiles,
+                                                     int ntiles, int seg_rows,
+                                                     const int* __restrict__ tile_flags,
+                                                     const double* __restrict__ dt_dev,
+                                                     const dclaw_params* __restrict__ params)
+{
+    const int nchunk = (STRIP_W * seg_rows + 63) / 64;
+    const long ntask = (long)ntiles * nchunk;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int t = (int)(task / nchunk);
+        if (!tile_flags[t]) continue;
+        const dclaw_patch pt = patches[tiles[4 * t + 0]];
+        const dclaw_params P = params[pt.member];
+        const double dt = dt_dev[pt.member];
+        const int i0 = 1 + tiles[4 * t + 1] * STRIP_W;
+        const int jA = 1 + tiles[4 * t + 2] * seg_rows;
+        const int jB = min(pt.my, jA + seg_rows - 1);
+        const int k = (int)(task % nchunk) * 64 + (int)threadIdx.x;
+        if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, pt.q_out, pt.mx, i0, jA, jB, k);
+    }
 }
 
 __global__ __launch_bounds__(64) void k_step2(const double* __restrict__ q,
@@ -173,8 +228,9 @@ __global__ __launch_bounds__(64) void k_step2(const double* __restrict__ q,
     A.fp = fp;
     A.gm = gm;
     A.gp = gp;
+    A.tile_flag = nullptr;
     GpuLane L;
-    const double cfl = strip_step_dispatch<true>(L, P, A);
+    const double cfl = strip_step_dispatch<MODE_FLUX>(L, P, A);
     wave_max_to(cfl, cfl_dev);
 }
 
@@ -414,7 +470,8 @@ static int check_bc(const int* bc)
 static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
                            double* aux_out, int maux, int mx, int my, double dx, double dy,
                            const double* dt_dev, const dclaw_params* params_dev,
-                           double* cfl_dev, const int* bc, hipStream_t stream)
+                           double* cfl_dev, const int* bc, hipStream_t stream,
+                           int* tile_flags = nullptr, int n_flags = 0)
 {
     if (int rcb = check_bc(bc)) return rcb;
     int rc = check_patch(mx, my, maux);
@@ -428,9 +485,23 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     const int nseg = (my + seg_rows - 1) / seg_rows;
     if (nseg > 65535) return fail(DCLAW_EINVAL, "my too large for one launch");
     const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
-    hipLaunchKernelGGL(k_stepgrid, dim3(nstrips, nseg), dim3(64), 0, stream, q_in, q_out, aux,
-                       aux_out, maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev,
-                       seg_rows, b0, b1, b2, b3);
+    if (!tile_flags) {
+        hipLaunchKernelGGL(k_stepgrid<MODE_STEP>, dim3(nstrips, nseg), dim3(64), 0, stream, q_in,
+                           q_out, aux, aux_out, maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev,
+                           seg_rows, b0, b1, b2, b3, nullptr);
+        HIP_TRY(hipGetLastError());
+        return DCLAW_OK;
+    }
+    const int ntiles = nstrips * nseg;
+    if (n_flags < ntiles) return fail(DCLAW_EINVAL, "tile_flags holds fewer than dclaw_tile_count() ints");
+    if (aux_out) return fail(DCLAW_EINVAL, "the split step does not write aux");
+    hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips, nseg), dim3(64), 0, stream,
+                       q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
+                       cfl_dev, seg_rows, b0, b1, b2, b3, tile_flags);
+    HIP_TRY(hipGetLastError());
+    const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
+    hipLaunchKernelGGL(k_src2_tiles, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, stream,
+                       q_out, mx, my, seg_rows, nstrips, ntiles, tile_flags, dt_dev, params_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }
@@ -451,6 +522,24 @@ int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, 
     if (!bc) return fail(DCLAW_EINVAL, "null bc");
     return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
                            cfl_dev, bc, (hipStream_t)stream);
+}
+
+int dclaw_tile_count(int mx, int my)
+{
+    if (mx < 1 || my < 1) return fail(DCLAW_EINVAL, "mx and my must be >= 1");
+    const long n = (long)((mx + STRIP_W - 1) / STRIP_W) * ((my + g_seg_rows - 1) / g_seg_rows);
+    if (n > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
+    return (int)n;
+}
+
+int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* aux, int maux,
+                             int mx, int my, double dx, double dy, const int* bc,
+                             const double* dt_dev, const dclaw_params* params_dev,
+                             double* cfl_dev, int* tile_flags_dev, int n_flags, void* stream)
+{
+    if (!tile_flags_dev) return fail(DCLAW_EINVAL, "null tile_flags");
+    return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
+                           cfl_dev, bc, (hipStream_t)stream, tile_flags_dev, n_flags);
 }
 
 int dclaw_worklist_size(const int* mx, const int* my, int npatch, int seg_rows)
@@ -496,16 +585,30 @@ int dclaw_worklist_build(const int* mx, const int* my, int npatch, int seg_rows,
 
 int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
                        int ntiles, int seg_rows, const int* bc, const double* dt_dev,
-                       const dclaw_params* params_dev, double* cfl_dev, void* stream)
+                       const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,
+                       void* stream)
 {
     if (!patches_dev || !tiles_dev || !dt_dev || !params_dev || !cfl_dev)
         return fail(DCLAW_EINVAL, "null pointer");
     if (npatch < 1 || ntiles < 1 || seg_rows < 1) return fail(DCLAW_EINVAL, "empty batch");
     if (int rcb = check_bc(bc)) return rcb;
     const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
-    hipLaunchKernelGGL(k_step_batched, dim3(ntiles), dim3(64), 0, (hipStream_t)stream,
+    const hipStream_t st = (hipStream_t)stream;
+    if (!tile_flags_dev) {
+        hipLaunchKernelGGL(k_step_batched<MODE_STEP>, dim3(ntiles), dim3(64), 0, st, patches_dev,
+                           tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev, b0, b1, b2, b3,
+                           nullptr);
+        HIP_TRY(hipGetLastError());
+        return DCLAW_OK;
+    }
+    hipLaunchKernelGGL(k_step_batched<MODE_STEP_NOSRC2>, dim3(ntiles), dim3(64), 0, st,
                        patches_dev, tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev, b0, b1, b2,
-                       b3);
+                       b3, tile_flags_dev);
+    HIP_TRY(hipGetLastError());
+    const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
+    hipLaunchKernelGGL(k_src2_batched, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, st,
+                       patches_dev, tiles_dev, ntiles, seg_rows, tile_flags_dev, dt_dev,
+                       params_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

@@ -32,6 +32,12 @@
 
 namespace dclaw {
 
+// Modes of the strip body: the fused step (stepgrid), the same without src2
+// (a separate pass applies it to the tiles flagged wet), and step2 alone.
+constexpr int MODE_STEP = 0;
+constexpr int MODE_STEP_NOSRC2 = 1;
+constexpr int MODE_FLUX = 2;
+
 constexpr int STRIP_LANES = 64;
 constexpr int STRIP_HALO = 3;
 constexpr int STRIP_W = STRIP_LANES - 2 * STRIP_HALO;  // 58 interior columns
@@ -46,6 +52,7 @@ struct StripArgs {
     int i0, jA, jB;        // interior strip start column, segment rows [jA, jB]
     int bc[4];             // bc codes (xlo, xhi, ylo, yhi) applied on load; 0 = use stored ghosts
     double *fm, *fp, *gm, *gp;   // step2 outputs (FLUX mode; same shape as q, zero-filled)
+    int* tile_flag;              // MODE_STEP_NOSRC2: set to 1 if src2 has wet cells here, else 0
 };
 
 // Source cell and momentum signs of a ghost cell under the bc2amr rules
@@ -324,9 +331,11 @@ DC_HD void load_row(const StripArgs& A, int c, int R, bool col_in_array, double*
 }
 
 // The strip body.  Returns this lane's CFL contribution (max |dt/dx * s|).
-template <bool FLUX, class Lane>
+template <int MODE, class Lane>
 DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
 {
+    constexpr bool FLUX = (MODE == MODE_FLUX);
+    constexpr bool NOSRC2 = (MODE == MODE_STEP_NOSRC2);
     const int lane = L.id();
     const int c = A.i0 - STRIP_HALO + lane;   // this lane's column
     const int mx = A.mx, my = A.my;
@@ -342,6 +351,7 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                              (c >= mx + 1 && c <= mx + 2 && A.i0 + STRIP_W - 1 >= mx);
 
     double cfl = 0.0;
+    bool src2_wet = false;   // MODE_STEP_NOSRC2: a cell of this tile needs src2
     // calc_taudir of a cell whose whole neighbourhood is dry: 0 when
     // dry_tolerance > 0 (early return), else the h = 0 fall-through values
     const bool drytol_pos = P.dry_tolerance > 0.0;
@@ -391,8 +401,10 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             for (int r = r0; r <= r1; r++) w |= (qi[qidx(0, c, r, mx)] > P.dry_tolerance);
             wet = w;
         }
-        if (!L.any(wet) && FLUX) return 0.0;   // fluxes are zero-filled by the caller
-        if (!L.any(wet)) {
+        const bool any_wet = L.any(wet);
+        if (!any_wet && FLUX) return 0.0;   // fluxes are zero-filled by the caller
+        if (!any_wet) {
+            if (NOSRC2 && L.id() == 0) *A.tile_flag = 0;
             // rows of this segment in chunks of 8: all bdif loads first, then stores
             const double* __restrict__ qi = A.q_in;
             double* __restrict__ qo = A.q_out;
@@ -790,7 +802,17 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                 double qn[6] = {q3.h, q3.hu, q3.hv, q3.hm0, q3.p0, q3.hchi0};
                 for (int z = 0; z < 6; z++)
                     qn[z] = qn[z] - dtdx * (fm_next[z] - fpc[z]) - dtdy * (gm_next[z] - gp_prev[z]);
-                src2_cell(P, A.dt, qn[0], qn[1], qn[2], qn[3], qn[4], qn[5]);
+                if (NOSRC2) {
+                    // src2 zeroes a cell with h <= dry_tolerance (src2.f90:103-111);
+                    // the others are left for the src2 pass over flagged tiles
+                    if (qn[0] <= P.dry_tolerance) {
+                        for (int z = 0; z < 6; z++) qn[z] = 0.0;
+                    } else {
+                        src2_wet = true;
+                    }
+                } else {
+                    src2_cell(P, A.dt, qn[0], qn[1], qn[2], qn[3], qn[4], qn[5]);
+                }
                 const size_t o = qidx(0, c, j, mx);
                 for (int z = 0; z < 6; z++) A.q_out[o + z] = qn[z];
                 A.q_out[o + 6] = q3.bdif;
@@ -811,7 +833,24 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
         yw_pend = yw_new;
         for (int f = 0; f < 3; f++) ydotl[f] = ydotr[f];
     }
+    if (NOSRC2) {
+        const bool w = L.any(src2_wet);
+        if (L.id() == 0) *A.tile_flag = w ? 1 : 0;
+    }
     return cfl;
+}
+
+// src2 on the interior cells of one (strip, segment) tile: cell k of the
+// tile's 58 x (jB - jA + 1) block, k = column offset + 58 * row offset.
+DC_HD void src2_tile_cell(const dclaw_params& P, double dt, double* q, int mx, int i0, int jA,
+                          int jB, int k)
+{
+    const int i = i0 + k % STRIP_W, j = jA + k / STRIP_W;
+    if (i > mx || j > jB) return;
+    double* c = q + qidx(0, i, j, mx);
+    double h = c[0], hu = c[1], hv = c[2], hm = c[3], p = c[4], hchi = c[5];
+    src2_cell(P, dt, h, hu, hv, hm, p, hchi);
+    c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi;
 }
 
 // The default method switches of a D-Claw run (second order, transverse
@@ -829,7 +868,7 @@ DC_HD bool scheme_is_std(const dclaw_params& P)
 // the defaults (a copy of the record whose switches are literal constants, so
 // the inlined body drops the branches of the other methods), or read from
 // the record otherwise.  Both copies compute the same thing.
-template <bool FLUX, class Lane>
+template <int MODE, class Lane>
 DC_HD double strip_step_dispatch(Lane& L, const dclaw_params& P, const StripArgs& A)
 {
     if (scheme_is_std(P)) {
@@ -842,9 +881,9 @@ DC_HD double strip_step_dispatch(Lane& L, const dclaw_params& P, const StripArgs
         Q.rarecorrectortest = 0;
         Q.segregation = 0;
         Q.friction_forcing = 0;
-        return strip_step<FLUX>(L, Q, A);
+        return strip_step<MODE>(L, Q, A);
     }
-    return strip_step<FLUX>(L, P, A);
+    return strip_step<MODE>(L, P, A);
 }
 
 }  // namespace dclaw

@@ -134,6 +134,17 @@ def src2(q: torch.Tensor, aux: torch.Tensor, mx: int, my: int, dt_dev: torch.Ten
         None if flag is None else _ptr(flag), _stream()))
 
 
+def stepgrid_split(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int, mx: int,
+                   my: int, dx: float, dy: float, bc, dt_dev: torch.Tensor,
+                   params_dev: torch.Tensor, cfl_dev: torch.Tensor, flags: torch.Tensor) -> None:
+    """The step with src2 in a second pass over the wet tiles (bc None: stored
+    ghosts); flags is an int32 workspace of at least dclaw_tile_count ints."""
+    arr = None if bc is None else (ctypes.c_int * 4)(*bc)
+    _lib.check(_lib.lib().dclaw_stepgrid_split_dev(
+        _ptr(q_in), _ptr(q_out), _ptr(aux), maux, mx, my, dx, dy, arr, _ptr(dt_dev),
+        _ptr(params_dev), _ptr(cfl_dev), _ptr(flags), flags.numel(), _stream()))
+
+
 def dt_update(dt_dev: torch.Tensor, cfl_dev: torch.Tensor, cfl_desired: float,
               dt_max: float) -> None:
     _lib.check(_lib.lib().dclaw_dt_update(_ptr(dt_dev), _ptr(cfl_dev), dt_dev.numel(),
@@ -144,7 +155,8 @@ class PatchRunner:
     """Device-resident fixed-patch time loop (one member)."""
 
     def __init__(self, q: np.ndarray, aux: np.ndarray, params: DigParams, dx: float, dy: float,
-                 bc=(BC_EXTRAP,) * 4, device="cuda"):
+                 bc=(BC_EXTRAP,) * 4, device="cuda", split_src2: bool = True):
+        self.split_src2 = split_src2
         self.mx = q.shape[1] - 2 * MBC
         self.my = q.shape[2] - 2 * MBC
         self.dx, self.dy = float(dx), float(dy)
@@ -155,6 +167,9 @@ class PatchRunner:
         self.params = params_tensor([params], device)
         self.dt = torch.zeros(1, dtype=torch.float64, device=device)
         self.cfl = torch.zeros(1, dtype=torch.float64, device=device)
+        # tile flags for any segment height (one row per tile at most)
+        nstrips = -(-self.mx // _lib.lib().dclaw_strip_width())
+        self.flags = torch.zeros(nstrips * self.my, dtype=torch.int32, device=device)
 
     def set_dt(self, dt: float) -> None:
         self.dt.fill_(float(dt))
@@ -162,7 +177,10 @@ class PatchRunner:
     def step(self, fused_bc: bool = True) -> None:
         """bound + stepgrid (boundary conditions applied on load when fused_bc);
         the result is left in self.qa and cflgrid in self.cfl."""
-        if fused_bc:
+        if fused_bc and self.split_src2:
+            stepgrid_split(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy,
+                           self.bc, self.dt, self.params, self.cfl, self.flags)
+        elif fused_bc:
             stepgrid_bc(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy,
                         self.bc, self.dt, self.params, self.cfl)
         else:
@@ -189,7 +207,9 @@ class Ensemble:
     """Many members of uneven patch sizes advanced by one launch per step over
     a (patch, strip, segment) work list, each with its own parameters and dt."""
 
-    def __init__(self, members: List, bc=(BC_EXTRAP,) * 4, seg_rows: int = 32, device="cuda"):
+    def __init__(self, members: List, bc=(BC_EXTRAP,) * 4, seg_rows: int = 32, device="cuda",
+                 split_src2: bool = True):
+        self.split_src2 = split_src2
         self.n = len(members)
         self.seg_rows = seg_rows
         self.bc = tuple(bc)
@@ -212,6 +232,7 @@ class Ensemble:
         _lib.check(min(0, L.dclaw_worklist_build(mxa, mya, self.n, seg_rows, tiles)))
         self.ntiles = nt
         self.tiles = torch.tensor(list(tiles), dtype=torch.int32, device=device)
+        self.flags = torch.zeros(nt, dtype=torch.int32, device=device)
         self.dx = [m.patch.dx for m in members]
         self.dy = [m.patch.dy for m in members]
         self._patches = [self._patch_table(False, device), self._patch_table(True, device)]
@@ -235,7 +256,8 @@ class Ensemble:
         bc = (ctypes.c_int * 4)(*self.bc)
         _lib.check(L.dclaw_step_batched(
             _ptr(self._patches[1 if self.flip else 0]), self.n, _ptr(self.tiles), self.ntiles,
-            self.seg_rows, bc, _ptr(self.dt), _ptr(self.params), _ptr(self.cfl), _stream()))
+            self.seg_rows, bc, _ptr(self.dt), _ptr(self.params), _ptr(self.cfl),
+            _ptr(self.flags) if self.split_src2 else None, _stream()))
         self.flip = not self.flip
 
     def step_plain(self) -> None:

@@ -109,19 +109,41 @@ int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, 
                           const double* dt_dev, const dclaw_params* params_dev,
                           double* cfl_dev, void* stream);
 
+/* Tiles (58-column strip x segment) of one patch at the current segment
+ * height: the size of the tile_flags workspace of the split step below. */
+int dclaw_tile_count(int mx, int my);
+
+/* The step of dclaw_stepgrid_bc_dev (bc may be NULL: stored ghosts) with src2
+ * split into a second pass: the strip kernel writes the updated state,
+ * zeroes the cells src2 would zero (h <= dry_tolerance, src2.f90:103-111)
+ * and flags its tile when a cell needs src2; then src2 runs, at full
+ * occupancy, over the interior cells of the flagged tiles only.  Same
+ * arithmetic, same result as the fused step.  tile_flags_dev: device
+ * workspace of n_flags >= dclaw_tile_count(mx, my) ints, written every call
+ * (no initialisation needed).  No reference counterpart (a GPU scheduling
+ * choice). */
+int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* aux, int maux,
+                             int mx, int my, double dx, double dy, const int* bc,
+                             const double* dt_dev, const dclaw_params* params_dev,
+                             double* cfl_dev, int* tile_flags_dev, int n_flags, void* stream);
+
 /* Batched fused step over npatch patches (device array of dclaw_patch).
  * One launch; the work list of (patch, strip, segment) tiles is built on
  * the host by dclaw_worklist_size / dclaw_worklist_build into device memory.
  * params_dev, dt_dev, cfl_dev are indexed by dclaw_patch.member.  bc (host
  * array of 4 codes, or NULL to read stored ghosts) is applied on load for
- * every patch.  No reference counterpart: the reference advances patches
- * one stepgrid call at a time (OpenMP over patches in GeoClaw advanc). */
+ * every patch.  tile_flags_dev (NULL: fused step) is a workspace of ntiles
+ * ints that selects the split step (src2 in a second pass over the flagged
+ * tiles, as in dclaw_stepgrid_split_dev).  No reference counterpart: the
+ * reference advances patches one stepgrid call at a time (OpenMP over
+ * patches in GeoClaw advanc). */
 int dclaw_worklist_size(const int* mx, const int* my, int npatch, int seg_rows);
 int dclaw_worklist_build(const int* mx, const int* my, int npatch, int seg_rows,
                          int32_t* tiles_host /* 4*size ints */);
 int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
                        int ntiles, int seg_rows, const int* bc, const double* dt_dev,
-                       const dclaw_params* params_dev, double* cfl_dev, void* stream);
+                       const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,
+                       void* stream);
 
 /* Device dt rule for n members: dt[k] <- min(dt[k]*cfl_desired/cfl[k], dt_max)
  * when cfl[k] > 0 (else dt_max), then cfl[k] <- 0 (GeoClaw stepgrid dtnew

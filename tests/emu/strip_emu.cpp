@@ -28,7 +28,7 @@ struct Sched {
     bool done[STRIP_LANES];
     const dclaw_params* P;
     const StripArgs* A;
-    bool flux;
+    int mode;
     std::vector<char> stacks;
 };
 
@@ -65,18 +65,20 @@ void lane_entry(int lid)
 {
     Sched* s = g_sched;
     CoLane L{lid, 0, s};
-    s->cfl[lid] = s->flux ? strip_step_dispatch<true>(L, *s->P, *s->A)
-                          : strip_step_dispatch<false>(L, *s->P, *s->A);
+    s->cfl[lid] = s->mode == MODE_FLUX ? strip_step_dispatch<MODE_FLUX>(L, *s->P, *s->A)
+                  : s->mode == MODE_STEP_NOSRC2
+                      ? strip_step_dispatch<MODE_STEP_NOSRC2>(L, *s->P, *s->A)
+                      : strip_step_dispatch<MODE_STEP>(L, *s->P, *s->A);
     s->done[lid] = true;
 }
 
-void run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out, bool flux = false)
+void run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out, int mode = MODE_STEP)
 {
     constexpr size_t STACK = 1 << 20;
     Sched* s = new Sched();
     s->P = &P;
     s->A = &A;
-    s->flux = flux;
+    s->mode = mode;
     s->stacks.resize(STACK * STRIP_LANES);
     g_sched = s;
     for (int l = 0; l < STRIP_LANES; l++) {
@@ -127,7 +129,55 @@ extern "C" int emu_stepgrid(const dclaw_params* P, const double* q_in, double* q
             A.jB = std::min(my, A.jA + seg_rows - 1);
             for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
             A.fm = A.fp = A.gm = A.gp = nullptr;
+            A.tile_flag = nullptr;
             run_strip(*P, A, cflgrid);
+        }
+    return 0;
+}
+
+// The step with src2 split out: the strip body without src2 flags its tiles,
+// then src2 runs over the interior cells of the flagged tiles.
+extern "C" int emu_stepgrid_split(const dclaw_params* P, const double* q_in, double* q_out,
+                                  const double* aux, int maux, int mx, int my, double dx,
+                                  double dy, double dt, int seg_rows, const int* bc,
+                                  double* cflgrid, int* nflagged)
+{
+    *cflgrid = 0.0;
+    *nflagged = 0;
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
+    std::vector<int> flags((size_t)nstrips * nseg, -1);
+    for (int s = 0; s < nstrips; s++)
+        for (int g = 0; g < nseg; g++) {
+            StripArgs A;
+            A.q_in = q_in;
+            A.q_out = q_out;
+            A.aux = aux;
+            A.aux_out = nullptr;
+            A.maux = maux;
+            A.mx = mx;
+            A.my = my;
+            A.dx = dx;
+            A.dy = dy;
+            A.dt = dt;
+            A.i0 = 1 + s * STRIP_W;
+            A.jA = 1 + g * seg_rows;
+            A.jB = std::min(my, A.jA + seg_rows - 1);
+            for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
+            A.fm = A.fp = A.gm = A.gp = nullptr;
+            A.tile_flag = &flags[(size_t)g * nstrips + s];
+            run_strip(*P, A, cflgrid, MODE_STEP_NOSRC2);
+        }
+    for (int s = 0; s < nstrips; s++)
+        for (int g = 0; g < nseg; g++) {
+            const int f = flags[(size_t)g * nstrips + s];
+            if (f < 0) return -1;          // a tile did not report
+            if (!f) continue;
+            (*nflagged)++;
+            const int i0 = 1 + s * STRIP_W, jA = 1 + g * seg_rows;
+            const int jB = std::min(my, jA + seg_rows - 1);
+            for (int k = 0; k < STRIP_W * (jB - jA + 1); k++)
+                src2_tile_cell(*P, dt, q_out, mx, i0, jA, jB, k);
         }
     return 0;
 }
@@ -164,7 +214,8 @@ extern "C" int emu_step2(const dclaw_params* P, const double* q, const double* a
             A.fp = fp;
             A.gm = gm;
             A.gp = gp;
-            run_strip(*P, A, cflgrid, true);
+            A.tile_flag = nullptr;
+            run_strip(*P, A, cflgrid, MODE_FLUX);
         }
     return 0;
 }

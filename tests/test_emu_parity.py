@@ -143,3 +143,33 @@ def test_step2_flux_mode_bitwise(oracle_mod, emu_lib, mx, my, seg):
         assert np.array_equal(a, b), f"{name}: {(a != b).sum()} entries differ"
     if mx * my > 1:      # a single cell with extrapolated ghosts has no flux
         assert np.count_nonzero(fo[0]) > 0
+
+
+@pytest.mark.parametrize("mx,my,seg,bc", [(130, 70, 8, (3, 3, 3, 3)), (59, 33, 4, (1, 3, 3, 1)),
+                                          (116, 40, 16, (1, 1, 1, 1))])
+def test_split_src2_bitwise(oracle_mod, emu_lib, mx, my, seg, bc):
+    """The strip body without src2 plus the src2 pass over the flagged tiles
+    equals the oracle's step (q and cflgrid) bit for bit."""
+    from dclaw_amd.params import to_c
+    O = oracle_mod
+    dp = ctypes.POINTER(ctypes.c_double)
+    pt = slope_patch(mx, my, slope=0.8, phi_deg=25.0, seed=mx + my)
+    q, aux, dt = advance_oracle(O, pt, 6, bc=bc)
+    qo, _, cflo = O.stepgrid(O.params_from(pt.params), q, aux, pt.dx, pt.dy, dt)
+    qi = np.array(q, order="F", copy=True)
+    qe = np.zeros_like(qi, order="F")
+    aux = np.asfortranarray(aux)
+    cp = to_c(pt.params)
+    cfl = ctypes.c_double(0.0)
+    nfl = ctypes.c_int(0)
+    rc = emu_lib.emu_stepgrid_split(ctypes.byref(cp), qi.ctypes.data_as(dp), qe.ctypes.data_as(dp),
+                                    aux.ctypes.data_as(dp), aux.shape[0], mx, my,
+                                    ctypes.c_double(pt.dx), ctypes.c_double(pt.dy),
+                                    ctypes.c_double(dt), seg, (ctypes.c_int * 4)(*bc),
+                                    ctypes.byref(cfl), ctypes.byref(nfl))
+    assert rc == 0
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    assert np.array_equal(qe[I], qo[I]), f"{(qe[I] != qo[I]).sum()} values differ"
+    assert cfl.value == cflo
+    ntiles = -(-mx // 58) * -(-my // seg)
+    assert 0 < nfl.value <= ntiles
