@@ -381,67 +381,70 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
     }
 
     // Whole-segment dry check: if no cell the segment reads is wet (h >
-    // dry_tolerance before qfix), every output is the dry state and the
-    // marching loop below would only copy it, so write it directly with all
-    // loads in flight at once.  Stored ghosts are skipped when the boundary
-    // conditions are applied on load (their sources are interior cells of
-    // the same window).
+    // dry_tolerance before qfix), every output is the dry state (zeros and
+    // bdif >= 0) and the marching loop below would only copy it.  Rows are
+    // scanned in chunks of 8 with h and bdif loaded together (one pass over
+    // the cache lines), and the dry state of each chunk's own rows is written
+    // as soon as the chunk is known dry; if a later chunk is wet, the marching
+    // loop rewrites every row this segment owns, so those early writes are
+    // harmless.  Stored ghosts are not checked when the boundary conditions
+    // are applied on load (their sources are interior cells of the same
+    // window).
     {
-        bool wet = false;
         const bool bcl = (A.bc[0] != 0);
         const int clo = bcl ? 1 : -1, chi = bcl ? mx : mx + 2;
         const int rlo = bcl ? 1 : -1, rhi = bcl ? my : my + 2;
         const bool col_chk = (c >= clo && c <= chi);
         const int r0 = A.jA - 3 > rlo ? A.jA - 3 : rlo;
         const int r1 = A.jB + 3 < rhi ? A.jB + 3 : rhi;
-        if (col_chk) {
-            const double* __restrict__ qi = A.q_in;
-            bool w = false;
-#pragma unroll 8
-            for (int r = r0; r <= r1; r++) w |= (qi[qidx(0, c, r, mx)] > P.dry_tolerance);
-            wet = w;
-        }
-        const bool any_wet = L.any(wet);
-        if (!any_wet && FLUX) return 0.0;   // fluxes are zero-filled by the caller
-        if (!any_wet) {
-            if (NOSRC2 && L.id() == 0) *A.tile_flag = 0;
-            // rows of this segment in chunks of 8: all bdif loads first, then stores
-            const double* __restrict__ qi = A.q_in;
-            double* __restrict__ qo = A.q_out;
-            for (int R0 = A.jA - 3; R0 <= A.jB + 3; R0 += 8) {
-                double bd[8];
-                bool wr[8];
+        const double* __restrict__ qi = A.q_in;
+        double* __restrict__ qo = A.q_out;
+        bool seg_wet = false;
+        for (int R0 = A.jA - 3; R0 <= A.jB + 3; R0 += 8) {
+            double hh[8], bd[8];
+            bool wr[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int R = R0 + k;
-                    const bool row_in = (R >= -1 && R <= my + 2) && R <= A.jB + 3;
-                    const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
-                                             (R >= my + 1 && R <= my + 2 && A.jB == my);
-                    const bool row_interior = (R >= 1 && R <= my) && R >= A.jA && R <= A.jB;
-                    wr[k] = row_in && col_in_array &&
-                            ((row_ghost_w && (col_out || col_ghost_w)) ||
-                             (row_interior && (col_out || col_ghost_w)));
-                    bd[k] = 0.0;
-                    if (wr[k]) {
-                        if ((c < 1 || c > mx || R < 1 || R > my) && bcl) {
-                            int si, sj;
-                            double fu, fv;
-                            bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
-                            bd[k] = qi[qidx(6, si, sj, mx)];
-                        } else {
-                            bd[k] = qi[qidx(6, c, R, mx)];
-                        }
+            for (int k = 0; k < 8; k++) {
+                const int R = R0 + k;
+                hh[k] = 0.0;
+                if (col_chk && R >= r0 && R <= r1) hh[k] = qi[qidx(0, c, R, mx)];
+                const bool row_in = (R >= -1 && R <= my + 2) && R <= A.jB + 3;
+                const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
+                                         (R >= my + 1 && R <= my + 2 && A.jB == my);
+                const bool row_interior = (R >= 1 && R <= my) && R >= A.jA && R <= A.jB;
+                wr[k] = !FLUX && row_in && col_in_array &&
+                        ((row_ghost_w && (col_out || col_ghost_w)) ||
+                         (row_interior && (col_out || col_ghost_w)));
+                bd[k] = 0.0;
+                if (wr[k]) {
+                    if ((c < 1 || c > mx || R < 1 || R > my) && bcl) {
+                        int si, sj;
+                        double fu, fv;
+                        bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
+                        bd[k] = qi[qidx(6, si, sj, mx)];
+                    } else {
+                        bd[k] = qi[qidx(6, c, R, mx)];
                     }
                 }
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    if (!wr[k]) continue;
-                    const size_t o = qidx(0, c, R0 + k, mx);
-                    for (int z = 0; z < 6; z++) qo[o + z] = 0.0;
-                    qo[o + 6] = dmax(bd[k], 0.0);
-                }
             }
-            if (A.aux_out) {
+            bool w = false;
+#pragma unroll
+            for (int k = 0; k < 8; k++) w |= (hh[k] > P.dry_tolerance);
+            if (L.any(w)) {
+                seg_wet = true;
+                break;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (!wr[k]) continue;
+                const size_t o = qidx(0, c, R0 + k, mx);
+                for (int z = 0; z < 6; z++) qo[o + z] = 0.0;
+                qo[o + 6] = dmax(bd[k], 0.0);
+            }
+        }
+        if (!seg_wet) {
+            if (NOSRC2 && L.id() == 0) *A.tile_flag = 0;
+            if (!FLUX && A.aux_out) {
                 for (int r = A.jA - 2; r <= A.jB + 2; r++) {
                     if (r < 0 || r > my + 1 || !col_taudir) continue;
                     const bool row_own = (r >= A.jA && r <= A.jB) || (r == 0 && A.jA == 1) ||
