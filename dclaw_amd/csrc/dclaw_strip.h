@@ -39,6 +39,7 @@ constexpr int MODE_STEP_NOSRC2 = 1;
 constexpr int MODE_FLUX = 2;
 
 constexpr int STRIP_LANES = 64;
+constexpr int DRY_CHUNK = 8;       // rows per load batch of the whole-segment dry check
 constexpr int STRIP_HALO = 3;
 constexpr int STRIP_W = STRIP_LANES - 2 * STRIP_HALO;  // 58 interior columns
 
@@ -380,35 +381,45 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
         gx_prev2[k] = gp_prev[k] = 0.0;
     }
 
-    // Whole-segment dry check: if no cell the segment reads is wet (h >
-    // dry_tolerance before qfix), every output is the dry state (zeros and
-    // bdif >= 0) and the marching loop below would only copy it.  Rows are
-    // scanned in chunks of 8 with h and bdif loaded together (one pass over
+    // Whole-segment dry check.  An owned cell's output differs from the dry
+    // state (zeros and bdif >= 0) only if a cell of its 3x3 neighbourhood is
+    // wet (h > dry_tolerance before qfix): a wave needs a wet side, the
+    // second-order correction needs four wet cells in a row (flux2's gate),
+    // and the transverse terms reaching the cell come from edges touching its
+    // neighbours; an owned ghost cell only needs itself dry (b4step2 is
+    // pointwise).  So if rows jA-1..jB+1 and columns i0-1..i0+58 (plus the
+    // owned ghost rows and columns) are dry, every output of the segment is
+    // the dry state and the marching loop would only copy it.  Rows are
+    // scanned in chunks of DRY_CHUNK with h and bdif loaded together (one pass over
     // the cache lines), and the dry state of each chunk's own rows is written
     // as soon as the chunk is known dry; if a later chunk is wet, the marching
     // loop rewrites every row this segment owns, so those early writes are
-    // harmless.  Stored ghosts are not checked when the boundary conditions
-    // are applied on load (their sources are interior cells of the same
-    // window).
+    // harmless.  With the boundary conditions applied on load only interior
+    // cells are checked (the ghost sources are interior cells in range).
     {
         const bool bcl = (A.bc[0] != 0);
+        const bool first_s = (A.i0 == 1), last_s = (A.i0 + STRIP_W - 1 >= mx);
         const int clo = bcl ? 1 : -1, chi = bcl ? mx : mx + 2;
         const int rlo = bcl ? 1 : -1, rhi = bcl ? my : my + 2;
-        const bool col_chk = (c >= clo && c <= chi);
-        const int r0 = A.jA - 3 > rlo ? A.jA - 3 : rlo;
-        const int r1 = A.jB + 3 < rhi ? A.jB + 3 : rhi;
+        const int ca = first_s ? -1 : A.i0 - 1;
+        const int cb = last_s ? mx + 2 : A.i0 + STRIP_W;
+        const bool col_chk = (c >= ca && c <= cb && c >= clo && c <= chi);
+        const int lo = (A.jA == 1) ? -1 : A.jA - 1;
+        const int hi = (A.jB == my) ? my + 2 : A.jB + 1;
+        const int r0 = lo > rlo ? lo : rlo;
+        const int r1 = hi < rhi ? hi : rhi;
         const double* __restrict__ qi = A.q_in;
         double* __restrict__ qo = A.q_out;
         bool seg_wet = false;
-        for (int R0 = A.jA - 3; R0 <= A.jB + 3; R0 += 8) {
-            double hh[8], bd[8];
-            bool wr[8];
+        for (int R0 = lo; R0 <= hi; R0 += DRY_CHUNK) {
+            double hh[DRY_CHUNK], bd[DRY_CHUNK];
+            bool wr[DRY_CHUNK];
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
+            for (int k = 0; k < DRY_CHUNK; k++) {
                 const int R = R0 + k;
                 hh[k] = 0.0;
                 if (col_chk && R >= r0 && R <= r1) hh[k] = qi[qidx(0, c, R, mx)];
-                const bool row_in = (R >= -1 && R <= my + 2) && R <= A.jB + 3;
+                const bool row_in = (R >= -1 && R <= my + 2) && R <= hi;
                 const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
                                          (R >= my + 1 && R <= my + 2 && A.jB == my);
                 const bool row_interior = (R >= 1 && R <= my) && R >= A.jA && R <= A.jB;
@@ -429,13 +440,13 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
             }
             bool w = false;
 #pragma unroll
-            for (int k = 0; k < 8; k++) w |= (hh[k] > P.dry_tolerance);
+            for (int k = 0; k < DRY_CHUNK; k++) w |= (hh[k] > P.dry_tolerance);
             if (L.any(w)) {
                 seg_wet = true;
                 break;
             }
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
+            for (int k = 0; k < DRY_CHUNK; k++) {
                 if (!wr[k]) continue;
                 const size_t o = qidx(0, c, R0 + k, mx);
                 for (int z = 0; z < 6; z++) qo[o + z] = 0.0;

@@ -173,3 +173,30 @@ def test_split_src2_bitwise(oracle_mod, emu_lib, mx, my, seg, bc):
     assert cfl.value == cflo
     ntiles = -(-mx // 58) * -(-my // seg)
     assert 0 < nfl.value <= ntiles
+
+
+@pytest.mark.parametrize("bc", [None, (1, 1, 1, 1), (3, 3, 3, 3)])
+def test_dry_tiles_next_to_wet_cells(oracle_mod, emu, bc):
+    """Wet cells two rows below a segment and two columns left of a strip:
+    those tiles take the whole-segment dry path (their 3x3 neighbourhoods are
+    dry) and must still equal the oracle, over several steps."""
+    O = oracle_mod
+    mx, my, seg = 130, 40, 8
+    pt = slope_patch(mx, my, slope=0.6, phi_deg=25.0, seed=1, h0=0.0)
+    # block of wet cells: rows 18..19 (segment [9,16] ends at 16), columns
+    # 50..57 (the second strip starts at column 59); 0-based + 2 ghosts
+    for (i0, i1, j0, j1) in [(50, 57, 18, 19), (1, 3, 1, 2), (128, 130, 39, 40)]:
+        sl = (slice(i0 + 1, i1 + 2), slice(j0 + 1, j1 + 2))
+        pt.q[0][sl] = 2.0
+        pt.q[3][sl] = 1.2
+        pt.q[4][sl] = 0.8 * 1000 * 9.81 * 2.0
+    q = O.bc_fill(pt.q, bc or (1, 1, 1, 1))
+    aux = pt.aux
+    dt = 0.01
+    for step in range(4):
+        qe, auxe, cfle = emu.stepgrid(pt.params, q, aux, pt.dx, pt.dy, dt, seg_rows=seg, bc=bc)
+        qo, auxo, cflo = O.stepgrid(O.params_from(pt.params), q, aux, pt.dx, pt.dy, dt)
+        assert np.array_equal(qe, qo), f"step {step}: {(qe != qo).sum()} values differ"
+        assert np.array_equal(auxe, auxo), f"step {step}: aux differs"
+        assert cfle == cflo
+        q, aux = O.bc_fill(qo, bc or (1, 1, 1, 1)), auxo
