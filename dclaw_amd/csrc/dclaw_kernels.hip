@@ -375,6 +375,78 @@ __global__ void k_dt_update(double* dt, double* cfl, int n, double cfl_desired, 
     cfl[k] = 0.0;
 }
 
+// dt rule with the output-time hit of tick.f:135-144: the member has just
+// taken a step of dt[k] at time t[k]; t advances, the next dt is the CFL
+// rule capped by dt_max, shortened to land on tout[k] when
+// t + 1.001 dt >= tout[k] (hit[k] = 1).
+__global__ void k_dt_update_out(double* dt, double* cfl, double* t, const double* tout, int* hit,
+                                int n, double cfl_desired, double dt_max)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double tn = t[k] + dt[k];
+    const double c = cfl[k];
+    double dtn;
+    if (c > 0.0) dtn = dt[k] * cfl_desired / c;
+    else dtn = dt_max;
+    dtn = dmin(dtn, dt_max);
+    int h = 0;
+    const double to = tout[k];
+    if (tn < to && tn + 1.001 * dtn >= to) {
+        dtn = to - tn;
+        h = 1;
+    }
+    t[k] = tn;
+    dt[k] = dtn;
+    cfl[k] = 0.0;
+    hit[k] = h;
+}
+
+// Gauges: q components and eta = h + (aux(1) - bdif), bilinear in the four
+// cells whose centres surround the gauge when all four are wet, else the
+// containing cell's values (the flume port's convention,
+// gradient_particle_filter/flume_solver.py:91-99, in two dimensions).  A
+// gauge whose cells are not all interior gets NaN.  xy: (x, y) per gauge.
+__global__ void k_gauges(const double* __restrict__ q, const double* __restrict__ aux, int maux,
+                         int mx, int my, double xlower, double ylower, double dx, double dy,
+                         double drytol, const double* __restrict__ xy, int ng, double* out)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const double fx = (xy[2 * g] - xlower) / dx, fy = (xy[2 * g + 1] - ylower) / dy;
+    const int ic = (int)floor(fx) + 1, jc = (int)floor(fy) + 1;
+    const int i0 = (int)floor(fx + 0.5), j0 = (int)floor(fy + 0.5);
+    const double xo = fx + 0.5 - (double)i0, yo = fy + 0.5 - (double)j0;
+    double* o = out + 8 * g;
+    if (!(fx >= 0.5 && fy >= 0.5 && i0 >= 1 && i0 + 1 <= mx && j0 >= 1 && j0 + 1 <= my)) {
+        for (int m = 0; m < 8; m++) o[m] = __longlong_as_double(0x7ff8000000000000LL);
+        return;
+    }
+    bool wet = true;
+    for (int dj = 0; dj < 2; dj++)
+        for (int di = 0; di < 2; di++)
+            if (!(q[qidx(0, i0 + di, j0 + dj, mx)] > drytol)) wet = false;
+    if (wet) {
+        const double w00 = (1.0 - xo) * (1.0 - yo), w10 = xo * (1.0 - yo);
+        const double w01 = (1.0 - xo) * yo, w11 = xo * yo;
+        for (int m = 0; m < 7; m++)
+            o[m] = ((w00 * q[qidx(m, i0, j0, mx)] + w10 * q[qidx(m, i0 + 1, j0, mx)]) +
+                    w01 * q[qidx(m, i0, j0 + 1, mx)]) +
+                   w11 * q[qidx(m, i0 + 1, j0 + 1, mx)];
+        double topo = 0.0;
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++) {
+                const double w = (di ? xo : 1.0 - xo) * (dj ? yo : 1.0 - yo);
+                topo = topo + w * (aux[aidx(i0 + di, j0 + dj, mx, maux)] -
+                                   q[qidx(6, i0 + di, j0 + dj, mx)]);
+            }
+        o[7] = o[0] + topo;
+    } else {
+        for (int m = 0; m < 7; m++) o[m] = q[qidx(m, ic, jc, mx)];
+        o[7] = o[0] + (aux[aidx(ic, jc, mx, maux)] - q[qidx(6, ic, jc, mx)]);
+    }
+}
+
 int check_patch(int mx, int my, int maux)
 {
     if (mx < 1 || my < 1) return fail(DCLAW_EINVAL, "mx and my must be >= 1");
@@ -699,6 +771,31 @@ int dclaw_src2_dev(double* q, const double* aux, int maux, int mx, int my, const
     const long n = (long)mx * my;
     hipLaunchKernelGGL(k_src2, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, q, mx, my,
                        dt_dev, params_dev);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_dt_update_out(double* dt_dev, double* cfl_dev, double* t_dev, const double* tout_dev,
+                        int* hit_dev, int n, double cfl_desired, double dt_max, void* stream)
+{
+    if (!dt_dev || !cfl_dev || !t_dev || !tout_dev || !hit_dev || n < 1)
+        return fail(DCLAW_EINVAL, "bad dt_update_out args");
+    hipLaunchKernelGGL(k_dt_update_out, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       dt_dev, cfl_dev, t_dev, tout_dev, hit_dev, n, cfl_desired, dt_max);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_gauges_dev(const double* q, const double* aux, int maux, int mx, int my, double xlower,
+                     double ylower, double dx, double dy, double dry_tolerance,
+                     const double* xy_dev, int ngauges, double* out_dev, void* stream)
+{
+    if (int rc = check_patch(mx, my, maux)) return rc;
+    if (!q || !aux || !xy_dev || !out_dev || ngauges < 1) return fail(DCLAW_EINVAL, "bad gauge args");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+    hipLaunchKernelGGL(k_gauges, dim3((ngauges + 63) / 64), dim3(64), 0, (hipStream_t)stream, q,
+                       aux, maux, mx, my, xlower, ylower, dx, dy, dry_tolerance, xy_dev, ngauges,
+                       out_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

@@ -151,6 +151,38 @@ def dt_update(dt_dev: torch.Tensor, cfl_dev: torch.Tensor, cfl_desired: float,
                                           cfl_desired, dt_max, _stream()))
 
 
+def dt_update_out(dt_dev: torch.Tensor, cfl_dev: torch.Tensor, t_dev: torch.Tensor,
+                  tout_dev: torch.Tensor, hit_dev: torch.Tensor, cfl_desired: float,
+                  dt_max: float) -> None:
+    """The dt rule with the output-time hit (tick.f:135-144); t advances by the
+    dt just used."""
+    _lib.check(_lib.lib().dclaw_dt_update_out(_ptr(dt_dev), _ptr(cfl_dev), _ptr(t_dev),
+                                              _ptr(tout_dev), _ptr(hit_dev), dt_dev.numel(),
+                                              cfl_desired, dt_max, _stream()))
+
+
+def gauges(q: torch.Tensor, aux: torch.Tensor, maux: int, mx: int, my: int, xlower: float,
+           ylower: float, dx: float, dy: float, dry_tolerance: float, xy_dev: torch.Tensor,
+           out: torch.Tensor) -> None:
+    """Gauge values (ngauges, 8) into `out` (a device view)."""
+    _lib.check(_lib.lib().dclaw_gauges_dev(_ptr(q), _ptr(aux), maux, mx, my, xlower, ylower, dx,
+                                           dy, dry_tolerance, _ptr(xy_dev), xy_dev.shape[0],
+                                           _ptr(out), _stream()))
+
+
+class RunResult:
+    """What PatchRunner.run_until returns: final time, step count, the output
+    times reached, the time of every step and the gauge records (nsteps,
+    ngauges, 8: h hu hv hm pb hchi bdif eta after each step)."""
+
+    def __init__(self, t, nsteps, outputs, times, gauge_values):
+        self.t = t
+        self.nsteps = nsteps
+        self.outputs = outputs
+        self.times = times
+        self.gauge_values = gauge_values
+
+
 class PatchRunner:
     """Device-resident fixed-patch time loop (one member)."""
 
@@ -165,6 +197,7 @@ class PatchRunner:
         self.qb = torch.empty_like(self.qa)
         self.bed = bed_tensor(aux, device)
         self.params = params_tensor([params], device)
+        self.params_host = params
         self.dt = torch.zeros(1, dtype=torch.float64, device=device)
         self.cfl = torch.zeros(1, dtype=torch.float64, device=device)
         # tile flags for any segment height (one row per tile at most)
@@ -198,6 +231,88 @@ class PatchRunner:
             if cfl_hist is not None:
                 cfl_hist[n].copy_(self.cfl[0])
             dt_update(self.dt, self.cfl, cfl_desired, dt_max)
+
+    def run_until(self, t_outputs: Sequence[float], dt0: float, cfl_desired: float,
+                  dt_max: float, gauges_xy: Optional[Sequence] = None, xlower: float = 0.0,
+                  ylower: float = 0.0, check_nans: bool = False, on_output=None,
+                  max_steps: int = 10 ** 7) -> RunResult:
+        """Device-resident loop to the output times (the last one is the end
+        time), as AMRClaw's tick does: bound + stepgrid + the dt rule with
+        dt_max and the output-time hit; after every step the gauges are
+        sampled on the device; at each output time on_output(self, t) is
+        called.  check_nans runs check4nans on the state before every step and
+        raises DClawError (with the first NaN's q(m,i,j)) at the next output."""
+        outs = sorted(float(x) for x in t_outputs)
+        if not outs or outs[0] <= 0.0:
+            raise ValueError("output times must be > 0")
+        dev = self.qa.device
+        t = torch.zeros(1, dtype=torch.float64, device=dev)
+        tout = torch.tensor([outs[0]], dtype=torch.float64, device=dev)
+        hit = torch.zeros(1, dtype=torch.int32, device=dev)
+        flag = nan_flag(dev) if check_nans else None
+        ng = 0
+        xy_dev = None
+        if gauges_xy is not None:
+            xy = np.asarray(gauges_xy, dtype=np.float64).reshape(-1, 2)
+            for gx, gy in xy:
+                if not (xlower + 0.5 * self.dx <= gx <= xlower + (self.mx - 0.5) * self.dx and
+                        ylower + 0.5 * self.dy <= gy <= ylower + (self.my - 0.5) * self.dy):
+                    raise ValueError(f"gauge ({gx}, {gy}) is not half a cell inside the patch")
+            ng = xy.shape[0]
+            xy_dev = torch.from_numpy(np.ascontiguousarray(xy)).to(dev)
+        cap = 1024
+        gbuf = torch.zeros((cap, max(ng, 1), 8), dtype=torch.float64, device=dev)
+        tbuf = torch.zeros(cap, dtype=torch.float64, device=dev)
+        g_host, t_host = [], []
+        self.cfl.zero_()
+        # tick.f:135-144 before the first step: shorten dt0 to land on outs[0]
+        target, landing = 0, None
+        if 1.001 * dt0 >= outs[0]:
+            dt0, landing, target = outs[0], 0, 1
+        self.set_dt(dt0)
+        tout.fill_(outs[target] if target < len(outs) else float("inf"))
+        reached, n = [], 0
+        while n < max_steps:
+            if flag is not None:
+                check4nans(self.qa, self.mx, self.my, flag)
+            self.step()
+            dt_update_out(self.dt, self.cfl, t, tout, hit, cfl_desired, dt_max)
+            slot = n % cap
+            tbuf[slot].copy_(t[0])
+            if ng:
+                gauges(self.qa, self.bed, 1, self.mx, self.my, xlower, ylower, self.dx, self.dy,
+                       self.params_host.dry_tolerance, xy_dev, gbuf[slot])
+            n += 1
+            if slot == cap - 1:
+                t_host.append(tbuf.cpu().numpy().copy())
+                g_host.append(gbuf.cpu().numpy().copy())
+            if landing is not None:
+                # the step just taken was shortened to land on outs[landing]
+                if flag is not None and int(flag.item()) != NO_NAN:
+                    raise _lib.DClawError(self._nan_message(int(flag.item())))
+                reached.append(float(t.item()))
+                if on_output is not None:
+                    on_output(self, outs[landing])
+                if landing == len(outs) - 1:
+                    break
+                landing = None
+            if bool(hit.item()):
+                # the next step lands on outs[target]; later checks use the one after
+                landing, target = target, target + 1
+                tout.fill_(outs[target] if target < len(outs) else float("inf"))
+        rem = n % cap
+        if rem:
+            t_host.append(tbuf[:rem].cpu().numpy().copy())
+            g_host.append(gbuf[:rem].cpu().numpy().copy())
+        times = np.concatenate(t_host) if t_host else np.zeros(0)
+        gv = np.concatenate(g_host)[:, :ng] if g_host else np.zeros((0, ng, 8))
+        return RunResult(float(t.item()), n, reached, times, gv)
+
+    def _nan_message(self, first: int) -> str:
+        m, cell = first % 7, first // 7
+        i, j = cell % (self.mx + 4) - 1, cell // (self.mx + 4) - 1
+        return (f"SOLUTION ERROR --- ABORTING CALCULATION: NaN in q(m,i,j) with m,i,j = "
+                f"{m + 1} {i} {j} (check4nans)")
 
     def q(self) -> np.ndarray:
         return to_host_q(self.qa)

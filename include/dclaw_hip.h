@@ -151,6 +151,26 @@ int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t
 int dclaw_dt_update(double* dt_dev, double* cfl_dev, int n, double cfl_desired,
                     double dt_max, void* stream);
 
+/* dt rule with output times for n members (device arrays): the member has
+ * just taken a step of dt[k] at time t[k]; then t[k] += dt[k], dt[k] <-
+ * min(dt[k]*cfl_desired/cfl[k] (dt_max if cfl[k] = 0), dt_max), cfl[k] <- 0,
+ * and if t[k] < tout[k] <= t[k] + 1.001*dt[k] the next step is shortened to
+ * land on tout[k] (dt[k] = tout[k] - t[k], hit[k] = 1, else 0): AMRClaw
+ * tick.f:135-144 (output times; tout = +inf for none) and :499 (dt_max). */
+int dclaw_dt_update_out(double* dt_dev, double* cfl_dev, double* t_dev, const double* tout_dev,
+                        int* hit_dev, int n, double cfl_desired, double dt_max, void* stream);
+
+/* Gauges on device memory: for each gauge (xy_dev: x, y pairs), out_dev gets
+ * 8 doubles: the 7 q components and eta = h + (aux(1) - bdif), bilinear in
+ * the four cells whose centres surround the gauge when all four are wet
+ * (h > dry_tolerance), else the containing cell's values (the convention of
+ * gradient_particle_filter/flume_solver.py:91-99 in two dimensions; GeoClaw's
+ * gauges_module, which the reference links, is not in its tree).  Gauges must
+ * lie at least half a cell inside the patch; others get NaN. */
+int dclaw_gauges_dev(const double* q, const double* aux, int maux, int mx, int my, double xlower,
+                     double ylower, double dx, double dy, double dry_tolerance,
+                     const double* xy_dev, int ngauges, double* out_dev, void* stream);
+
 /* ---- Per-stage device entry points (GeoClaw stepgrid's stages one at a
  * time, on device memory; the fused dclaw_stepgrid_dev equals b4step2, step2,
  * the conservative update and src2 in sequence). ---- */

@@ -1238,6 +1238,58 @@ int oracle_run(oracle_params* P, int mx, int my, int mbc, int maux, double dx, d
     return n;
 }
 
+void oracle_dt_rule(double dt_used, double cfl, double t_before, double cfl_desired,
+                    double dt_max, double tout, double* t_after, double* dt_next, int* hit)
+{
+    const double t = t_before + dt_used;
+    double dtn = (cfl > 0.0) ? dt_used * cfl_desired / cfl : INFINITY;
+    dtn = dmin(dtn, dt_max);
+    *hit = 0;
+    if (t < tout && t + 1.001 * dtn >= tout) {   /* tick.f:135-144 */
+        dtn = tout - t;
+        *hit = 1;
+    }
+    *t_after = t;
+    *dt_next = dtn;
+}
+
+void oracle_gauges(int mx, int my, int mbc, int maux, const double* q, const double* aux,
+                   double xlower, double ylower, double dx, double dy, double dry_tolerance,
+                   const double* xg, const double* yg, int ngauges, double* out)
+{
+    (void)my;
+    for (int g = 0; g < ngauges; g++) {
+        const double fx = (xg[g] - xlower) / dx, fy = (yg[g] - ylower) / dy;
+        const int ic = (int)floor(fx) + 1, jc = (int)floor(fy) + 1;   /* containing cell */
+        const int i0 = (int)floor(fx + 0.5), j0 = (int)floor(fy + 0.5);  /* lower-left centre */
+        const double xo = fx + 0.5 - (double)i0, yo = fy + 0.5 - (double)j0;
+        int wet = 1;
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++)
+                if (!(P2(q, MEQN, 1, i0 + di, j0 + dj) > dry_tolerance)) wet = 0;
+        double* o = out + 8 * g;
+        if (wet) {
+            const double w00 = (1.0 - xo) * (1.0 - yo), w10 = xo * (1.0 - yo);
+            const double w01 = (1.0 - xo) * yo, w11 = xo * yo;
+            for (int m = 1; m <= MEQN; m++)
+                o[m - 1] = ((w00 * P2(q, MEQN, m, i0, j0) + w10 * P2(q, MEQN, m, i0 + 1, j0)) +
+                            w01 * P2(q, MEQN, m, i0, j0 + 1)) +
+                           w11 * P2(q, MEQN, m, i0 + 1, j0 + 1);
+            double topo = 0.0;
+            for (int dj = 0; dj < 2; dj++)
+                for (int di = 0; di < 2; di++) {
+                    const double w = (di ? xo : 1.0 - xo) * (dj ? yo : 1.0 - yo);
+                    topo = topo + w * (P2(aux, maux, 1, i0 + di, j0 + dj) -
+                                       P2(q, MEQN, MEQN, i0 + di, j0 + dj));
+                }
+            o[7] = o[0] + topo;
+        } else {
+            for (int m = 1; m <= MEQN; m++) o[m - 1] = P2(q, MEQN, m, ic, jc);
+            o[7] = o[0] + (P2(aux, maux, 1, ic, jc) - P2(q, MEQN, MEQN, ic, jc));
+        }
+    }
+}
+
 /* ---- point-routine entry points for tests ---- */
 void oracle_qfix(const oracle_params* P, double* q6, double gz, double* prim5)
 {

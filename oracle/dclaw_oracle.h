@@ -90,6 +90,27 @@ int oracle_run(oracle_params* p, int mx, int my, int mbc, int maux, double dx,
                double* dt_inout, double cfl_desired, double dt_max,
                double* cfl_hist, double* dt_hist, int nthreads);
 
+/* Time loop pieces (device-resident loop of the build, restated):
+ * oracle_dt_rule: the dt rule after a step of length dt_used taken at time
+ * t_before with CFL number cfl: t_after = t_before + dt_used;
+ * dt_next = min(dt_used*cfl_desired/cfl (infinity if cfl = 0), dt_max)
+ * (GeoClaw stepgrid dtnew, tick.f:499); then the output-time hit of
+ * tick.f:135-144: if t_after < tout and t_after + 1.001*dt_next >= tout,
+ * dt_next = tout - t_after and *hit = 1 (else 0). */
+void oracle_dt_rule(double dt_used, double cfl, double t_before, double cfl_desired,
+                    double dt_max, double tout, double* t_after, double* dt_next, int* hit);
+
+/* Gauges: for each (xg, yg), the 7 q components and eta = h + (aux(1) - bdif),
+ * bilinear in the four cells whose centres surround the gauge when all four
+ * are wet (h > dry_tolerance), else the values of the cell containing the
+ * gauge (the convention of gradient_particle_filter/flume_solver.py:91-99,
+ * two cells in 1-D; GeoClaw's gauges_module is not in the reference tree).
+ * out: 8 doubles per gauge.  Gauges must lie at least half a cell inside
+ * the patch (all four cells interior). */
+void oracle_gauges(int mx, int my, int mbc, int maux, const double* q, const double* aux,
+                   double xlower, double ylower, double dx, double dy, double dry_tolerance,
+                   const double* xg, const double* yg, int ngauges, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,6 +121,9 @@ def lib():
                                  ctypes.c_double, ctypes.c_double, _ip, _dp, _dp, ctypes.c_int,
                                  _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
         L.oracle_run.restype = ctypes.c_int
+        L.oracle_dt_rule.argtypes = [ctypes.c_double] * 6 + [_dp, _dp, _ip]
+        L.oracle_gauges.argtypes = [ctypes.c_int] * 4 + [_dp, _dp] + [ctypes.c_double] * 5 + [
+            _dp, _dp, ctypes.c_int, _dp]
         _libs[_variant] = L
     return _libs[_variant]
 
@@ -297,3 +300,24 @@ def run(params, q, aux, dx, dy, nsteps, dt0, cfl_desired, dt_max, bc=(1, 1, 1, 1
                      ctypes.cast(bcs, _ip), _ptr(q), _ptr(aux), nsteps, ctypes.byref(dt),
                      cfl_desired, dt_max, _ptr(cfl_hist), _ptr(dt_hist), 1)
     return q, aux, dt.value, cfl_hist[:nsteps], dt_hist[:nsteps]
+
+
+def dt_rule(dt_used, cfl, t_before, cfl_desired, dt_max, tout=float("inf")):
+    """(t_after, dt_next, hit) of the dt rule with the output-time hit."""
+    ta, dn, hit = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    lib().oracle_dt_rule(dt_used, cfl, t_before, cfl_desired, dt_max, tout, ctypes.byref(ta),
+                         ctypes.byref(dn), ctypes.byref(hit))
+    return ta.value, dn.value, hit.value
+
+
+def gauges(q, aux, xlower, ylower, dx, dy, dry_tolerance, xg, yg, mbc=2):
+    """(ngauges, 8): h hu hv hm pb hchi bdif eta at each gauge."""
+    q = np.asfortranarray(q, dtype=np.float64)
+    aux = np.asfortranarray(aux, dtype=np.float64)
+    mx, my = _dims(q, mbc)
+    xg = np.ascontiguousarray(xg, dtype=np.float64)
+    yg = np.ascontiguousarray(yg, dtype=np.float64)
+    out = np.zeros((len(xg), 8))
+    lib().oracle_gauges(mx, my, mbc, aux.shape[0], _ptr(q), _ptr(aux), xlower, ylower, dx, dy,
+                        dry_tolerance, _ptr(xg), _ptr(yg), len(xg), _ptr(out))
+    return out
