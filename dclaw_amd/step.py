@@ -196,6 +196,7 @@ class PatchRunner:
         self.qa = to_device_q(q, device)
         self.qb = torch.empty_like(self.qa)
         self.bed = bed_tensor(aux, device)
+        self.bed_host = np.array(np.asfortranarray(aux)[0:1], order="F", copy=True)
         self.params = params_tensor([params], device)
         self.params_host = params
         self.dt = torch.zeros(1, dtype=torch.float64, device=device)
@@ -255,8 +256,10 @@ class PatchRunner:
         if gauges_xy is not None:
             xy = np.asarray(gauges_xy, dtype=np.float64).reshape(-1, 2)
             for gx, gy in xy:
-                if not (xlower + 0.5 * self.dx <= gx <= xlower + (self.mx - 0.5) * self.dx and
-                        ylower + 0.5 * self.dy <= gy <= ylower + (self.my - 0.5) * self.dy):
+                # the four cells around the gauge must be interior:
+                # x in [xlower + dx/2, xlower + (mx - 1/2) dx), same in y
+                if not (xlower + 0.5 * self.dx <= gx < xlower + (self.mx - 0.5) * self.dx and
+                        ylower + 0.5 * self.dy <= gy < ylower + (self.my - 0.5) * self.dy):
                     raise ValueError(f"gauge ({gx}, {gy}) is not half a cell inside the patch")
             ng = xy.shape[0]
             xy_dev = torch.from_numpy(np.ascontiguousarray(xy)).to(dev)
@@ -307,6 +310,19 @@ class PatchRunner:
         times = np.concatenate(t_host) if t_host else np.zeros(0)
         gv = np.concatenate(g_host)[:, :ng] if g_host else np.zeros((0, ng, 8))
         return RunResult(float(t.item()), n, reached, times, gv)
+
+    def write_frame(self, outdir: str, frame: int, time: float,
+                    output_format: str = "binary64", xlower: float = 0.0,
+                    ylower: float = 0.0):
+        """valout's frame of the current state (fort.q/fort.t, fort.b for the
+        binary formats); ghost cells are filled on the device first, as valout
+        calls bound() (valout.f90:254-257)."""
+        from . import io as dio
+        q = self.qa.clone()
+        bc_fill(q, self.mx, self.my, self.bc)
+        grid = dio.Grid(q=to_host_q(q), aux=self.bed_host, xlow=xlower, ylow=ylower, dx=self.dx,
+                        dy=self.dy)
+        return dio.write_frame(outdir, frame, time, [grid], output_format)
 
     def _nan_message(self, first: int) -> str:
         m, cell = first % 7, first // 7

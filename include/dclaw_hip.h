@@ -165,8 +165,9 @@ int dclaw_dt_update_out(double* dt_dev, double* cfl_dev, double* t_dev, const do
  * the four cells whose centres surround the gauge when all four are wet
  * (h > dry_tolerance), else the containing cell's values (the convention of
  * gradient_particle_filter/flume_solver.py:91-99 in two dimensions; GeoClaw's
- * gauges_module, which the reference links, is not in its tree).  Gauges must
- * lie at least half a cell inside the patch; others get NaN. */
+ * gauges_module, which the reference links, is not in its tree).  The four
+ * cells must be interior: x in [xlower + dx/2, xlower + (mx - 1/2) dx), same
+ * in y; other gauges get NaN. */
 int dclaw_gauges_dev(const double* q, const double* aux, int maux, int mx, int my, double xlower,
                      double ylower, double dx, double dy, double dry_tolerance,
                      const double* xy_dev, int ngauges, double* out_dev, void* stream);

@@ -105,8 +105,8 @@ void oracle_dt_rule(double dt_used, double cfl, double t_before, double cfl_desi
  * are wet (h > dry_tolerance), else the values of the cell containing the
  * gauge (the convention of gradient_particle_filter/flume_solver.py:91-99,
  * two cells in 1-D; GeoClaw's gauges_module is not in the reference tree).
- * out: 8 doubles per gauge.  Gauges must lie at least half a cell inside
- * the patch (all four cells interior). */
+ * out: 8 doubles per gauge.  The four cells must be interior: x in
+ * [xlower + dx/2, xlower + (mx - 1/2) dx), same in y. */
 void oracle_gauges(int mx, int my, int mbc, int maux, const double* q, const double* aux,
                    double xlower, double ylower, double dx, double dy, double dry_tolerance,
                    const double* xg, const double* yg, int ngauges, double* out);

@@ -52,7 +52,7 @@ def test_gauges_equal_oracle(dev, oracle_mod):
     q, aux, _ = advance_oracle(O, pt, 8)
     rng = np.random.default_rng(1)
     xy = np.stack([rng.uniform(0.5, 79.5, 40), rng.uniform(0.5, 49.5, 40)], axis=1)
-    xy[:5] = [[10.0, 10.0], [10.5, 10.5], [30.0, 20.5], [0.5, 0.5], [79.5, 49.5]]
+    xy[:5] = [[10.0, 10.0], [10.5, 10.5], [30.0, 20.5], [0.5, 0.5], [79.49, 49.49]]
     qd = S.to_device_q(q)
     out = dev.zeros((40, 8), dtype=dev.float64, device="cuda")
     S.gauges(qd, S.bed_tensor(aux), 1, 80, 50, 0.0, 0.0, 1.0, 1.0, pt.params.dry_tolerance,
@@ -123,3 +123,29 @@ def test_run_until_reports_nan(dev):
     r = S.PatchRunner(q, pt.aux, pt.params, pt.dx, pt.dy)
     with pytest.raises(_lib.DClawError, match="check4nans"):
         r.run_until([0.01], 1e-3, 0.9, 1e9, check_nans=True)
+
+
+def test_runner_frames_round_trip(dev, oracle_mod, tmp_path):
+    """Frames written from the device state at the output times read back as
+    the oracle's state (ghosts filled as bound() fills them) and eta."""
+    from dclaw_amd import io as DIO
+    from dclaw_amd import step as S
+    O = oracle_mod
+    pt = slope_patch(40, 30, slope=0.9, phi_deg=22.0, seed=8)
+    outs = [0.01, 0.03]
+    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+    frames = []
+
+    def out(run, tout):
+        k = len(frames) + 1
+        run.write_frame(str(tmp_path), k, tout, "binary64")
+        frames.append(k)
+
+    r.run_until(outs, 1e-3, 0.9, 1e9, on_output=out)
+    assert frames == [1, 2]
+    t, grids = DIO.read_frame(str(tmp_path), 2)
+    assert t == 0.03
+    qh = O.bc_fill(r.q(), (1, 1, 1, 1))
+    eta = qh[0] + pt.aux[0] - qh[6]
+    assert np.array_equal(grids[0]["qeta"][:7], qh)
+    assert np.array_equal(grids[0]["qeta"][7], eta)
