@@ -155,7 +155,8 @@ __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restri
 // patch; tile t = segment * nstrips + strip).  One wavefront per task, a task
 // being a 64-cell chunk of a tile, so the wet tiles spread over the whole chip;
 // grid-stride over the tasks.
-__global__ __launch_bounds__(64) void k_src2_tiles(double* q, int mx, int my, int seg_rows,
+__global__ __launch_bounds__(64) void k_src2_tiles(double* q, const double* __restrict__ aux,
+                                                   int maux, int mx, int my, int seg_rows,
                                                    int nstrips, int ntiles,
                                                    const int* __restrict__ tile_flags,
                                                    const double* __restrict__ dt_dev,
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(64) void k_src2_tiles(double* q, int mx, int my, in
         const int jA = 1 + (t / nstrips) * seg_rows;
         const int jB = min(my, jA + seg_rows - 1);
         const int k = (int)(task % nchunk) * 64 + (int)threadIdx.x;
-        if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, q, mx, i0, jA, jB, k);
+        if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, q, aux, maux, mx, i0, jA, jB, k);
     }
 }
 
@@ -196,7 +197,8 @@ __global__ __launch_bounds__(64) void k_src2_batched(const dclaw_patch* __restri
         const int jA = 1 + tiles[4 * t + 2] * seg_rows;
         const int jB = min(pt.my, jA + seg_rows - 1);
         const int k = (int)(task % nchunk) * 64 + (int)threadIdx.x;
-        if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, pt.q_out, pt.mx, i0, jA, jB, k);
+        if (k < STRIP_W * (jB - jA + 1))
+            src2_tile_cell(P, dt, pt.q_out, pt.aux, pt.maux, pt.mx, i0, jA, jB, k);
     }
 }
 
@@ -296,7 +298,8 @@ __global__ void k_calc_taudir(const double* __restrict__ q, double* aux, int mau
 }
 
 // src2.f90:92-300 on the interior cells, in place.
-__global__ void k_src2(double* q, int mx, int my, const double* __restrict__ dt_dev,
+__global__ void k_src2(double* q, const double* __restrict__ aux, int maux, int mx, int my,
+                       const double* __restrict__ dt_dev,
                        const dclaw_params* __restrict__ params)
 {
     const dclaw_params P = *params;
@@ -306,9 +309,10 @@ __global__ void k_src2(double* q, int mx, int my, const double* __restrict__ dt_
          t += (long)gridDim.x * blockDim.x) {
         const int i = 1 + (int)(t % mx), j = 1 + (int)(t / mx);
         double* c = q + qidx(0, i, j, mx);
-        double h = c[0], hu = c[1], hv = c[2], hm = c[3], p = c[4], hchi = c[5];
-        src2_cell(P, dt, h, hu, hv, hm, p, hchi);
-        c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi;
+        double h = c[0], hu = c[1], hv = c[2], hm = c[3], p = c[4], hchi = c[5], bdif = c[6];
+        const double h_e = P.entrainment ? aux[aidx(i, j, mx, maux) + 6] : 0.0;
+        src2_cell(P, dt, h, hu, hv, hm, p, hchi, bdif, h_e);
+        c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi; c[6] = bdif;
     }
 }
 
@@ -491,6 +495,9 @@ int validate_params(const dclaw_params& P)
     if (P.transverse < 0 || P.transverse > 2) return fail(DCLAW_EINVAL, "transverse must be 0..2");
     if (P.src2method < -1 || P.src2method > 1)
         return fail(DCLAW_EINVAL, "src2method must be -1, 0 or 1 on this path");
+    if (P.entrainment < 0 || P.entrainment > 1 || P.entrainment_method < 0 ||
+        P.entrainment_method > 1)
+        return fail(DCLAW_EINVAL, "entrainment and entrainment_method must be 0 or 1");
     if (!(P.dry_tolerance >= 0.0) || !(P.grav > 0.0))
         return fail(DCLAW_EINVAL, "grav must be > 0 and dry_tolerance >= 0");
     return DCLAW_OK;
@@ -573,7 +580,8 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     HIP_TRY(hipGetLastError());
     const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
     hipLaunchKernelGGL(k_src2_tiles, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, stream,
-                       q_out, mx, my, seg_rows, nstrips, ntiles, tile_flags, dt_dev, params_dev);
+                       q_out, aux, maux, mx, my, seg_rows, nstrips, ntiles, tile_flags, dt_dev,
+                       params_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }
@@ -769,8 +777,8 @@ int dclaw_src2_dev(double* q, const double* aux, int maux, int mx, int my, const
     if (first_nan_dev)
         if (int rc = dclaw_check4nans_dev(q, mx, my, first_nan_dev, stream)) return rc;
     const long n = (long)mx * my;
-    hipLaunchKernelGGL(k_src2, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, q, mx, my,
-                       dt_dev, params_dev);
+    hipLaunchKernelGGL(k_src2, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, q, aux, maux,
+                       mx, my, dt_dev, params_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

@@ -645,10 +645,47 @@ DC_HD void mp_update_relax(const dclaw_params& P, double dt, double& h, double& 
     }
 }
 
-// src2 body for one interior cell (src2.f90:100-300; bed_normal, curvature
-// and entrainment off).  Updates the six state components in place.
+// ent_dclaw4 (entrainment.f90:27-100): erosion of the bed layer into the
+// flow at one cell.  Its b_x, b_y arguments are not used by the reference, so
+// they are not passed here.
+DC_HD void ent_dclaw4(const dclaw_params& P, double dt, double& h, double u, double v, double& m,
+                      double& p, double& rho, double& hchi, double gz, double tau,
+                      double& b_eroded, double b_remaining)
+{
+    double coeff;
+    if (P.friction_forcing) coeff = P.manning_coefficient;
+    else coeff = 0.0;
+    const double vnorm = sqrt(u * u + v * v);
+    const double vlow = 0.1;
+    if (vnorm > vlow) {
+        if (b_remaining > 0.0) {
+            const double rhoe = P.me * P.rho_s + (1.0 - P.me) * P.rho_f;
+            const double beta = dmax(1.0 - m, 0.1);
+            const double visc = beta * vnorm * 2.0 * P.mu * (1.0 - m) / (dcm_tanh(h + 1.e-2));
+            const double gamma = rho * beta * (vnorm * vnorm) * (beta * gz * (coeff * coeff)) /
+                                 (dcm_pow(dcm_tanh(h + 1.e-2), 1.0 / 3.0));
+            const double t1bot = visc + gamma + tau;
+            const double t2top = dmin(t1bot, (1.0 - beta * P.entrainment_rate) * tau);
+            const double prat = p / (rho * h);
+            double dh = P.entrainment_rate * dt * (t1bot - t2top) / (rhoe * beta * vnorm);
+            dh = dmin(dh, b_remaining);
+            h = h + dh;
+            const double hm = h * m + dh * P.me;
+            m = hm / h;
+            hchi = hchi + dh * P.chie;
+            const double hrho = hm * P.rho_s + (h - hm) * P.rho_f;
+            rho = hrho / h;
+            b_eroded = b_eroded + dh;
+            p = prat * hrho;
+        }
+    }
+}
+
+// src2 body for one interior cell (src2.f90:100-300; bed_normal and curvature
+// off).  Updates the six state components and bdif in place; h_e is the
+// erodible depth aux(i_ent) (read only with entrainment on).
 DC_HD void src2_cell(const dclaw_params& P, double dt, double& qh, double& qhu, double& qhv,
-                     double& qhm, double& qp, double& qhchi)
+                     double& qhm, double& qp, double& qhchi, double& qbdif, double h_e)
 {
     const double drytol = P.dry_tolerance;
     double h = qh;
@@ -684,6 +721,19 @@ DC_HD void src2_cell(const dclaw_params& P, double dt, double& qh, double& qhu, 
     if (h <= drytol) {
         qh = 0.0; qhu = 0.0; qhv = 0.0; qhm = 0.0; qp = 0.0; qhchi = 0.0;
         return;
+    }
+    if (P.entrainment == 1) {
+        // src2.f90:250-263: setvars after the source integration feeds
+        // ent_dclaw4; h, m, p, hchi and bdif change there, while hu, hv and
+        // hm keep their values from before (they are not recomputed)
+        const Vars s3 = setvars(P, h, u, v, m, p, chi, gz);
+        rho = s3.rho;
+        double b_eroded = qbdif;
+        const double b_remaining = h_e - b_eroded;
+        if (P.entrainment_method == 0) {
+            ent_dclaw4(P, dt, h, u, v, m, p, rho, hchi, gz, s3.tau, b_eroded, b_remaining);
+            qbdif = b_eroded;
+        }
     }
     if (P.friction_forcing && P.manning_coefficient > 0.0) {
         if (h <= P.friction_depth) {

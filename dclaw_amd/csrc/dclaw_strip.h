@@ -814,6 +814,7 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                 }
             } else if (j >= A.jA && j <= A.jB && col_out) {
                 double qn[6] = {q3.h, q3.hu, q3.hv, q3.hm0, q3.p0, q3.hchi0};
+                double bdif_out = q3.bdif;
                 for (int z = 0; z < 6; z++)
                     qn[z] = qn[z] - dtdx * (fm_next[z] - fpc[z]) - dtdy * (gm_next[z] - gp_prev[z]);
                 if (NOSRC2) {
@@ -825,11 +826,13 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                         src2_wet = true;
                     }
                 } else {
-                    src2_cell(P, A.dt, qn[0], qn[1], qn[2], qn[3], qn[4], qn[5]);
+                    // aux(i_ent) = aux(7): the erodible depth (maux >= 7 with entrainment on)
+                    const double h_e = P.entrainment ? A.aux[aidx(c, j, mx, A.maux) + 6] : 0.0;
+                    src2_cell(P, A.dt, qn[0], qn[1], qn[2], qn[3], qn[4], qn[5], bdif_out, h_e);
                 }
                 const size_t o = qidx(0, c, j, mx);
                 for (int z = 0; z < 6; z++) A.q_out[o + z] = qn[z];
-                A.q_out[o + 6] = q3.bdif;
+                A.q_out[o + 6] = bdif_out;
             }
             for (int z = 0; z < 6; z++) {
                 gp_prev[z] = gp_next[z];
@@ -856,15 +859,16 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
 
 // src2 on the interior cells of one (strip, segment) tile: cell k of the
 // tile's 58 x (jB - jA + 1) block, k = column offset + 58 * row offset.
-DC_HD void src2_tile_cell(const dclaw_params& P, double dt, double* q, int mx, int i0, int jA,
-                          int jB, int k)
+DC_HD void src2_tile_cell(const dclaw_params& P, double dt, double* q, const double* aux,
+                          int maux, int mx, int i0, int jA, int jB, int k)
 {
     const int i = i0 + k % STRIP_W, j = jA + k / STRIP_W;
     if (i > mx || j > jB) return;
     double* c = q + qidx(0, i, j, mx);
-    double h = c[0], hu = c[1], hv = c[2], hm = c[3], p = c[4], hchi = c[5];
-    src2_cell(P, dt, h, hu, hv, hm, p, hchi);
-    c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi;
+    double h = c[0], hu = c[1], hv = c[2], hm = c[3], p = c[4], hchi = c[5], bdif = c[6];
+    const double h_e = P.entrainment ? aux[aidx(i, j, mx, maux) + 6] : 0.0;
+    src2_cell(P, dt, h, hu, hv, hm, p, hchi, bdif, h_e);
+    c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi; c[6] = bdif;
 }
 
 // The default method switches of a D-Claw run (second order, transverse
@@ -875,7 +879,7 @@ DC_HD bool scheme_is_std(const dclaw_params& P)
     return P.order == 2 && P.transverse == 2 && P.mthlim[0] == 4 && P.mthlim[1] == 4 &&
            P.mthlim[2] == 4 && P.mthlim[3] == 4 && P.mthlim[4] == 4 && P.src2method == 0 &&
            P.alphamethod == 0 && P.rarecorrectortest == 0 && P.segregation == 0 &&
-           P.friction_forcing == 0;
+           P.friction_forcing == 0 && P.entrainment == 0;
 }
 
 // The strip body with the method switches folded to constants when they are
@@ -895,6 +899,7 @@ DC_HD double strip_step_dispatch(Lane& L, const dclaw_params& P, const StripArgs
         Q.rarecorrectortest = 0;
         Q.segregation = 0;
         Q.friction_forcing = 0;
+        Q.entrainment = 0;
         return strip_step<MODE>(L, Q, A);
     }
     return strip_step<MODE>(L, P, A);

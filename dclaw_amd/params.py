@@ -56,6 +56,10 @@ class DigParams:
     segregation: int = 0
     bed_normal: int = 0
     entrainment: int = 0
+    entrainment_method: int = 1      # data.py:339 default (1: no update yet, src2.f90:258)
+    entrainment_rate: float = 0.2    # data.py:340
+    me: float = 0.65                 # solid fraction of the erodible layer
+    chie: float = 0.5                # data.py:346
     curvature: int = 0
     # amr_module / clawdata numerics
     order: int = 2
@@ -79,8 +83,11 @@ class DigParams:
             raise ValueError("src2method == 2 requires alphamethod = 1 (data.py:362-365)")
         if self.src2method not in (-1, 0, 1):
             raise ValueError("src2method must be -1, 0 or 1 on this path")
-        if self.entrainment != 0 or self.curvature != 0:
-            raise ValueError("entrainment and curvature are not implemented on this path")
+        if self.curvature != 0:
+            raise ValueError("curvature is not implemented on this path (the reference "
+                             "accumulates gz over the cells of src2's loop, src2.f90:87,156)")
+        if self.entrainment not in (0, 1) or self.entrainment_method not in (0, 1):
+            raise ValueError("entrainment and entrainment_method must be 0 or 1")
         if self.mwaves != 5:
             raise ValueError("D-Claw's rpn2 produces exactly 5 waves")
         if self.order not in (1, 2) or self.transverse not in (0, 1, 2):
@@ -123,6 +130,11 @@ class CParams(ctypes.Structure):
         ("transverse", ctypes.c_int),
         ("mthlim", ctypes.c_int * 5),
         ("mwaves", ctypes.c_int),
+        ("entrainment_rate", ctypes.c_double),
+        ("me", ctypes.c_double),
+        ("chie", ctypes.c_double),
+        ("entrainment", ctypes.c_int),
+        ("entrainment_method", ctypes.c_int),
     ]
 
 

@@ -59,6 +59,17 @@ def bed_tensor(aux: np.ndarray, device="cuda") -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(b.T)).to(device)
 
 
+def aux_planes(aux: np.ndarray, params, device="cuda"):
+    """(device aux, maux) the step reads: the bed alone (maux = 1), or the
+    whole aux array when entrainment is on (aux(7) holds the erodible depth
+    h_e, digclaw_module.f90:110)."""
+    if getattr(params, "entrainment", 0):
+        if aux.shape[0] < 7:
+            raise ValueError("entrainment needs maux >= 7 (aux(7) = h_e)")
+        return aux_tensor(aux, device), aux.shape[0]
+    return bed_tensor(aux, device), 1
+
+
 def bc_fill(q: torch.Tensor, mx: int, my: int, bc=(BC_EXTRAP,) * 4) -> None:
     """Ghost-cell fill in place (bc2amr extrap / wall)."""
     arr = (ctypes.c_int * 4)(*bc)
@@ -195,7 +206,7 @@ class PatchRunner:
         self.bc = tuple(bc)
         self.qa = to_device_q(q, device)
         self.qb = torch.empty_like(self.qa)
-        self.bed = bed_tensor(aux, device)
+        self.bed, self.maux = aux_planes(aux, params, device)
         self.bed_host = np.array(np.asfortranarray(aux)[0:1], order="F", copy=True)
         self.params = params_tensor([params], device)
         self.params_host = params
@@ -212,14 +223,15 @@ class PatchRunner:
         """bound + stepgrid (boundary conditions applied on load when fused_bc);
         the result is left in self.qa and cflgrid in self.cfl."""
         if fused_bc and self.split_src2:
-            stepgrid_split(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy,
+            stepgrid_split(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
                            self.bc, self.dt, self.params, self.cfl, self.flags)
         elif fused_bc:
-            stepgrid_bc(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy,
+            stepgrid_bc(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
                         self.bc, self.dt, self.params, self.cfl)
         else:
             bc_fill(self.qa, self.mx, self.my, self.bc)
-            stepgrid(self.qa, self.qb, self.bed, 1, self.mx, self.my, self.dx, self.dy, self.dt,
+            stepgrid(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
+                     self.dt,
                      self.params, self.cfl)
         self.qa, self.qb = self.qb, self.qa
 
@@ -283,7 +295,8 @@ class PatchRunner:
             slot = n % cap
             tbuf[slot].copy_(t[0])
             if ng:
-                gauges(self.qa, self.bed, 1, self.mx, self.my, xlower, ylower, self.dx, self.dy,
+                gauges(self.qa, self.bed, self.maux, self.mx, self.my, xlower, ylower, self.dx,
+                       self.dy,
                        self.params_host.dry_tolerance, xy_dev, gbuf[slot])
             n += 1
             if slot == cap - 1:
@@ -348,7 +361,9 @@ class Ensemble:
         self.my = [m.patch.my for m in members]
         self.qa = [to_device_q(m.patch.q, device) for m in members]
         self.qb = [torch.empty_like(t) for t in self.qa]
-        self.bed = [bed_tensor(m.patch.aux, device) for m in members]
+        planes = [aux_planes(m.patch.aux, m.patch.params, device) for m in members]
+        self.bed = [pl[0] for pl in planes]
+        self.maux = [pl[1] for pl in planes]
         self.params = params_tensor([m.patch.params for m in members], device)
         self._prec = ctypes.sizeof(CParams)
         self.dt = torch.zeros(self.n, dtype=torch.float64, device=device)
@@ -374,7 +389,7 @@ class Ensemble:
         for k in range(self.n):
             qi, qo = (self.qb[k], self.qa[k]) if flipped else (self.qa[k], self.qb[k])
             recs.append(_lib.CPatch(_ptr(qi), _ptr(qo), _ptr(self.bed[k]), self.mx[k],
-                                    self.my[k], 1, k, self.dx[k], self.dy[k]))
+                                    self.my[k], self.maux[k], k, self.dx[k], self.dy[k]))
         raw = b"".join(bytes(r) for r in recs)
         return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
 
@@ -398,7 +413,8 @@ class Ensemble:
             qi = self.current(k)
             qo = self.qa[k] if self.flip else self.qb[k]
             bc_fill(qi, self.mx[k], self.my[k], self.bc)
-            stepgrid(qi, qo, self.bed[k], 1, self.mx[k], self.my[k], self.dx[k], self.dy[k],
+            stepgrid(qi, qo, self.bed[k], self.maux[k], self.mx[k], self.my[k], self.dx[k],
+                     self.dy[k],
                      self.dt[k:k + 1], self.params[k * self._prec:(k + 1) * self._prec],
                      self.cfl[k:k + 1])
         self.flip = not self.flip

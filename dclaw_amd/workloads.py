@@ -53,7 +53,8 @@ def _clamp_ghosts(a2: np.ndarray, mbc: int) -> np.ndarray:
 
 
 def make_patch(b_int: np.ndarray, h_int: np.ndarray, m_int, pratio_int, params: DigParams,
-               dx: float, dy: float, mbc: int = MBC, maux: int = MAUX) -> Patch:
+               dx: float, dy: float, mbc: int = MBC, maux: int = MAUX,
+               h_e=None) -> Patch:
     """Build (q, aux) from interior fields of shape (mx, my) (x index first)."""
     mx, my = h_int.shape
     nx, ny = mx + 2 * mbc, my + 2 * mbc
@@ -82,6 +83,11 @@ def make_patch(b_int: np.ndarray, h_int: np.ndarray, m_int, pratio_int, params: 
     aux[0] = _clamp_ghosts(b_int, mbc)
     aux[1] = params.phi
     aux[2] = 0.0
+    if h_e is not None:
+        # aux(7) = h_e, the erodible layer depth (digclaw_module.f90:110, i_ent)
+        if maux < 7:
+            raise ValueError("h_e needs maux >= 7")
+        aux[6] = _clamp_ghosts(np.broadcast_to(h_e, (mx, my)).astype(np.float64), mbc)
     return Patch(q=q, aux=aux, params=params, dx=dx, dy=dy, mbc=mbc)
 
 

@@ -61,6 +61,10 @@ typedef struct dclaw_params {
     int mthlim[5];              /* limiter per wave family (0 none, 1 minmod, 2 superbee,
                                    3 van Leer, 4 MC, 5 Beam-Warming) */
     int mwaves;                 /* 5 */
+    /* entrainment (src2.f90:253-263, entrainment.f90:27-100) */
+    double entrainment_rate, me, chie;
+    int entrainment;            /* 0 or 1 */
+    int entrainment_method;     /* 0: ent_dclaw4; 1: no update (the reference's default) */
 } dclaw_params;
 
 /* One patch of a (possibly batched) step.  All pointers are device pointers. */

@@ -1090,10 +1090,44 @@ static void mp_update_relax_Dclaw4(const oracle_params* P, double dt, double* h,
 
 /* src2: src2.f90:27-306 (bed_normal = curvature = entrainment = 0;
  * src2method -1, 0, 1) */
+/* ent_dclaw4: entrainment.f90:27-100.  b_x and b_y are arguments of the
+ * reference routine that it never reads; they are left out. */
+static void ent_dclaw4(const oracle_params* P, double dt, double* h, double u, double v,
+                       double* m, double* p, double* rho, double* hchi, double gz, double tau,
+                       double* b_eroded, double b_remaining)
+{
+    double coeff, vlow, vnorm, rhoe, beta, visc, gamma, t1bot, t2top, prat, dh, hm, hrho;
+    if (P->friction_forcing) coeff = P->manning_coefficient;      /* :44-48 */
+    else coeff = 0.0;
+    vnorm = sqrt(u * u + v * v);
+    vlow = 0.1;
+    if (vnorm > vlow) {
+        if (b_remaining > 0.0) {
+            rhoe = P->me * P->rho_s + (1.0 - P->me) * P->rho_f;   /* :62 */
+            beta = dmax(1.0 - *m, 0.1);                            /* :66 */
+            visc = beta * vnorm * 2.0 * P->mu * (1.0 - *m) / (D_TANH(*h + 1.e-2));
+            gamma = *rho * beta * (vnorm * vnorm) * (beta * gz * (coeff * coeff)) /
+                    (D_POW(D_TANH(*h + 1.e-2), 1.0 / 3.0));        /* :69 */
+            t1bot = visc + gamma + tau;
+            t2top = dmin(t1bot, (1.0 - beta * P->entrainment_rate) * tau);
+            prat = *p / (*rho * *h);                               /* :75 */
+            dh = P->entrainment_rate * dt * (t1bot - t2top) / (rhoe * beta * vnorm);
+            dh = dmin(dh, b_remaining);                            /* :80 */
+            *h = *h + dh;
+            hm = *h * *m + dh * P->me;
+            *m = hm / *h;
+            *hchi = *hchi + dh * P->chie;                          /* :88 */
+            hrho = hm * P->rho_s + (*h - hm) * P->rho_f;
+            *rho = hrho / *h;
+            *b_eroded = *b_eroded + dh;                            /* :93 */
+            *p = prat * hrho;
+        }
+    }
+}
+
 void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, double* q,
                  const double* aux, double dt)
 {
-    (void)aux; (void)maux;
     const double drytol = P->dry_tolerance, rho_s = P->rho_s, rho_f = P->rho_f, mu = P->mu;
     double coeffmanning = P->friction_forcing ? P->manning_coefficient : 0.0;
     const double gz = P->grav, gx = 0.0;
@@ -1140,6 +1174,17 @@ void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, doub
                 continue;
             }
             setvars(P, h, u, v, m, p, chi, gz, &rho, &kperm, &alphainv, &m_eq, &tanpsi, &tau);
+
+            /* mass entrainment, src2.f90:253-263 (aux(i_ent) = aux(7)) */
+            if (P->entrainment == 1) {
+                double b_eroded = P2(q, MEQN, I_BDIF, i, j);
+                double b_remaining = P2(aux, maux, 7, i, j) - b_eroded;
+                if (P->entrainment_method == 0) {
+                    ent_dclaw4(P, dt, &h, u, v, &m, &p, &rho, &hchi, gz, tau, &b_eroded,
+                               b_remaining);
+                    P2(q, MEQN, I_BDIF, i, j) = b_eroded;
+                }
+            }
 
             if (P->friction_forcing && coeffmanning > 0.0) {
                 if (h <= P->friction_depth) {

@@ -40,6 +40,10 @@ typedef struct oracle_params {
     int transverse;             /* method(3): 0, 1, 2 */
     int mthlim[5];              /* limiter per wave family */
     int mwaves;                 /* 5 */
+    /* entrainment (src2.f90:253-263, entrainment.f90:27-100) */
+    double entrainment_rate, me, chie;
+    int entrainment;            /* 0 or 1 */
+    int entrainment_method;     /* 0: ent_dclaw4; 1: no update (the reference's default) */
 } oracle_params;
 
 /* ---- point routines (for unit / known-answer tests) ---- */

@@ -54,6 +54,11 @@ class OracleParams(ctypes.Structure):
         ("transverse", ctypes.c_int),
         ("mthlim", ctypes.c_int * 5),
         ("mwaves", ctypes.c_int),
+        ("entrainment_rate", ctypes.c_double),
+        ("me", ctypes.c_double),
+        ("chie", ctypes.c_double),
+        ("entrainment", ctypes.c_int),
+        ("entrainment_method", ctypes.c_int),
     ]
 
 

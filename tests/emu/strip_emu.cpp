@@ -177,7 +177,7 @@ extern "C" int emu_stepgrid_split(const dclaw_params* P, const double* q_in, dou
             const int i0 = 1 + s * STRIP_W, jA = 1 + g * seg_rows;
             const int jB = std::min(my, jA + seg_rows - 1);
             for (int k = 0; k < STRIP_W * (jB - jA + 1); k++)
-                src2_tile_cell(*P, dt, q_out, mx, i0, jA, jB, k);
+                src2_tile_cell(*P, dt, q_out, aux, maux, mx, i0, jA, jB, k);
         }
     return 0;
 }

@@ -200,3 +200,43 @@ def test_dry_tiles_next_to_wet_cells(oracle_mod, emu, bc):
         assert np.array_equal(auxe, auxo), f"step {step}: aux differs"
         assert cfle == cflo
         q, aux = O.bc_fill(qo, bc or (1, 1, 1, 1)), auxo
+
+
+def _entrainment_patch(O):
+    from dclaw_amd import workloads as W
+    pt = slope_patch(90, 50, slope=0.9, phi_deg=22.0, seed=12)
+    rng = np.random.default_rng(3)
+    h_e = rng.uniform(0.0, 0.6, (90, 50))
+    pe = W.make_patch(np.zeros((90, 50)), np.zeros((90, 50)), 0.6, 0.8, pt.params, 1.0, 1.0,
+                      maux=7, h_e=h_e)
+    pt.aux = np.concatenate([pt.aux, pe.aux[6:7]], axis=0)
+    pt.params.entrainment, pt.params.entrainment_method = 1, 0
+    pt.params.entrainment_rate = 0.3
+    return pt
+
+
+def test_entrainment_fused_and_split_bitwise(oracle_mod, emu, emu_lib):
+    """Entrainment (method 0) through the fused strip body and the split
+    step, against the oracle, over several steps."""
+    from dclaw_amd.params import to_c
+    O = oracle_mod
+    pt = _entrainment_patch(O)
+    q, aux, dt = advance_oracle(O, pt, 6)
+    assert (q[6] > 0).any()                       # the bed has been eroded
+    _check(O, emu, pt, q, aux, dt, seg_rows=8)
+    # split
+    qo, _, cflo = O.stepgrid(O.params_from(pt.params), q, aux, pt.dx, pt.dy, dt)
+    dp = ctypes.POINTER(ctypes.c_double)
+    qi = np.array(q, order="F", copy=True)
+    qe = np.zeros_like(qi, order="F")
+    auxf = np.asfortranarray(aux)
+    cp = to_c(pt.params)
+    cfl, nfl = ctypes.c_double(0.0), ctypes.c_int(0)
+    assert emu_lib.emu_stepgrid_split(ctypes.byref(cp), qi.ctypes.data_as(dp),
+                                      qe.ctypes.data_as(dp), auxf.ctypes.data_as(dp),
+                                      auxf.shape[0], 90, 50, ctypes.c_double(pt.dx),
+                                      ctypes.c_double(pt.dy), ctypes.c_double(dt), 16, None,
+                                      ctypes.byref(cfl), ctypes.byref(nfl)) == 0
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    assert np.array_equal(qe[I], qo[I])
+    assert cfl.value == cflo

@@ -88,3 +88,53 @@ def test_split_batched_equals_fused(dev):
     for k in range(len(members)):
         assert np.array_equal(a.q(k), b.q(k)), k
     assert dev.equal(a.dt, b.dt)
+
+
+def _ent_patch():
+    from dclaw_amd import workloads as W
+    pt = slope_patch(90, 50, slope=0.9, phi_deg=22.0, seed=12)
+    rng = np.random.default_rng(3)
+    pe = W.make_patch(np.zeros((90, 50)), np.zeros((90, 50)), 0.6, 0.8, pt.params, 1.0, 1.0,
+                      maux=7, h_e=rng.uniform(0.0, 0.6, (90, 50)))
+    pt.aux = np.concatenate([pt.aux, pe.aux[6:7]], axis=0)
+    pt.params.entrainment, pt.params.entrainment_method = 1, 0
+    pt.params.entrainment_rate = 0.3
+    return pt
+
+
+def test_entrainment_runner_matches_oracle(dev, oracle_mod):
+    """Entrainment (erodible layer in aux(7)) through the device loop, split
+    and fused, against the oracle over 60 steps."""
+    from dclaw_amd import step as S
+    from helpers import TOL_LONG, mask_equal, rel_linf_wet
+    O = oracle_mod
+    pt = _ent_patch()
+    n = 60
+    qo, _, dto, _, _ = O.run(O.params_from(pt.params), pt.q, pt.aux, pt.dx, pt.dy, n, 1e-3, 0.9,
+                             1e9)
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    for split in (True, False):
+        r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_src2=split)
+        assert r.maux == 7
+        r.run(n, 1e-3, 0.9, 1e9)
+        qg = r.q()
+        assert mask_equal(qg[I], qo[I], pt.params.dry_tolerance)
+        assert rel_linf_wet(qg[I], qo[I], pt.params.dry_tolerance) <= TOL_LONG
+        assert np.array_equal(qg[6][2:-2, 2:-2], qo[6][2:-2, 2:-2])   # eroded depth
+    assert (qo[6] > 0).any()
+
+
+def test_entrainment_in_a_batch(dev, oracle_mod):
+    """A batch mixing an entrainment member with plain members."""
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    O = oracle_mod
+    members = [W.Member(_ent_patch(), 0), W.Member(slope_patch(40, 60, seed=4), 1)]
+    ens = S.Ensemble(members, seg_rows=16)
+    ens.run(20, 1e-3, 0.9, 1e9)
+    for k, m in enumerate(members):
+        pt = m.patch
+        qo, _, _, _, _ = O.run(O.params_from(pt.params), pt.q, pt.aux, pt.dx, pt.dy, 20, 1e-3,
+                               0.9, 1e9)
+        I = (slice(None), slice(2, -2), slice(2, -2))
+        assert np.array_equal(ens.q(k)[I], qo[I]), k
