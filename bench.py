@@ -1,18 +1,22 @@
-"""Benchmark of the fused gfx950 D-Claw step.
+"""Benchmark of the gfx950 D-Claw step.
 
 Default workload: the plain drop-in GPU step at scale -- one 4096x4096 patch,
 dx = 2 m, seeded fractal bed (spectral slope -2, 200 m relief) on a 20 deg
 tilt, 64 seeded elliptical sources, D-Claw material parameters of the
 200x200 case; FP64.  A step is one device-resident pass of the hot path:
-ghost fill (bc2amr extrap), the fused stepgrid (b4step2, x/y sweeps with
-rpn2/rpt2 and MC limiter, update, src2) and the device dt rule.
+the physical boundary conditions applied on load (bc2amr extrap), stepgrid
+(b4step2, x/y sweeps with rpn2/rpt2 and MC limiter, update, src2; src2 as a
+second pass over the wet tiles unless --fused, optionally after a separate
+dry pass) and the device dt rule.  Segment height and dry pass are picked by
+the autotuner (untimed) unless given.
 
 Prints one JSON line (rank 0) with the throughput in interior cell-updates
-per second, the roofline of the fused step kernel (HBM, algorithmic bytes
-120 B per cell-update: 7 q in + 1 bed in + 7 q out, doubles) timed with HIP
-events on the stream the kernels run on, and a CPU baseline: the oracle
-(scalar C port of the reference algorithm, one core) on a bounded sample of
-the same workload.
+per second, the roofline of the step's kernels (HBM, algorithmic bytes 120 B
+per cell-update: 7 q in + 1 bed in + 7 q out, doubles) timed with HIP events
+on the stream the kernels run on, the measured HBM traffic of one step from
+the committed counter summary, and a CPU baseline: the oracle (scalar C port
+of the reference algorithm, one core) on a bounded sample of the same
+workload.  --workload ensemble runs the batched 256-member step instead.
 """
 from __future__ import annotations
 
@@ -37,7 +41,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=4096, help="patch side (cells)")
-    ap.add_argument("--cpu-steps", type=int, default=1,
+    ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--workload", choices=["patch", "ensemble"], default="patch",
@@ -49,6 +53,10 @@ def parse():
     ap.add_argument("--fused", action="store_true",
                     help="run src2 inside the strip kernel (one launch per step) instead of "
                          "the default split step (strip kernel + src2 pass over wet tiles)")
+    ap.add_argument("--no-dry-pass", action="store_true",
+                    help="split step as one strip launch over every tile (no separate dry pass)")
+    ap.add_argument("--dry-rows", type=int, default=0,
+                    help="rows per wavefront of the dry pass (0 = library default)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy (the HBM "
                          "counter calibration dispatch for rocprofv3 --pmc runs)")
@@ -81,7 +89,10 @@ def main():
     print(f"[bench] workload {n}x{n} built in {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
 
-    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_src2=not args.fused)
+    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_src2=not args.fused,
+                      dry_pass=not args.no_dry_pass)
+    if args.dry_rows > 0:
+        _lib.check(min(0, _lib.lib().dclaw_set_dry_rows(args.dry_rows)))
     r.set_dt(1e-3)
     r.cfl.zero_()
     cfl_desired, dt_max = 0.9, 1e9
@@ -91,7 +102,7 @@ def main():
             ev[0].record()
         if r.split_src2:
             S.stepgrid_split(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.bc, r.dt, r.params,
-                             r.cfl, r.flags)
+                             r.cfl, r.flags, r.dry_pass)
         else:
             S.stepgrid_bc(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.bc, r.dt, r.params,
                           r.cfl)
@@ -107,10 +118,13 @@ def main():
     tune_times = None
     if seg <= 0:
         from dclaw_amd import autotune
-        seg, tune_times = autotune.tune_patch(r)
+        seg, tune_times = autotune.tune_patch(
+            r, dry_pass_modes=(False,) if args.no_dry_pass else (True, False))
+        tune_times = {f"{h}{'/dry' if d else ''}": round(v, 4) for (h, d), v in tune_times.items()}
     else:
         _lib.lib().dclaw_set_seg_rows(seg)
-    print(f"[bench] warmup done, segment rows {seg} (tuner: {tune_times})", file=sys.stderr,
+    print(f"[bench] warmup done, segment rows {seg}, dry pass {r.dry_pass} (tuner: {tune_times})",
+          file=sys.stderr,
           flush=True)
 
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -140,11 +154,14 @@ def main():
     print(f"[bench] {args.steps} steps {t_total:.3f}s, kernel avg {kern_avg_s * 1e3:.3f} ms, "
           f"wet cells {wet} ({100.0 * wet / cells:.2f}%)", file=sys.stderr, flush=True)
 
+    # HBM bytes of one step from the committed counter summary of this mode
+    # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a 1 GiB copy)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"k_stepgrid_fractal{n}_traffic.txt")
+    mode = "fused" if args.fused else ("dry_pass" if r.dry_pass else "split_step")
+    tpath = os.path.join(ROOT, "profiles", f"{mode}_fractal{n}_traffic.txt")
     if os.path.exists(tpath):
         for line in open(tpath):
-            if line.startswith("traffic_bytes_per_launch"):
+            if line.startswith("traffic_bytes_per_step"):
                 traffic = float(line.split("=")[1])
 
     cpu = None
@@ -177,11 +194,14 @@ def main():
                                f"forward only (adjoint not built)",
                    "mx": n, "my": n, "seed": args.seed, "cfl_desired": cfl_desired,
                    "wet_cells_end": wet, "seg_rows": seg,
-                   "src2": "fused" if args.fused else "split"},
+                   "src2": "fused" if args.fused else "split",
+                   "dry_pass": bool(r.split_src2 and r.dry_pass)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("k_stepgrid<0>" if args.fused
-                                else "k_stepgrid<1> + k_src2_tiles (one step)"),
+                                else "k_stepgrid<1> + k_src2_tiles (one step)" if not r.dry_pass
+                                else "k_zero_ints + k_classify + k_stepgrid_list<1> + "
+                                     "k_src2_list (one step)"),
                      "kernel_avg_ms": kern_avg_s * 1e3},
         "cpu_baseline": cpu,
     }

@@ -60,6 +60,8 @@ SIGNATURES = {
     "dclaw_step_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                           _ip, _vp, _vp, _vp, _vp, _vp]),
     "dclaw_tile_count": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "dclaw_split_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "dclaw_set_dry_rows": (ctypes.c_int, [ctypes.c_int]),
     "dclaw_stepgrid_split_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 _ip, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),

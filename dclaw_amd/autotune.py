@@ -34,22 +34,30 @@ def _time(fn: Callable[[], None], reps: int) -> float:
     return a.elapsed_time(b) / reps
 
 
-def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 3) -> Tuple[int, Dict]:
-    """Pick the segment height for a PatchRunner's single-patch launches.
-    The runner's state and dt are restored afterwards."""
+def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 5,
+               dry_pass_modes: Iterable[bool] = (True, False)) -> Tuple[int, Dict]:
+    """Pick the segment height for a PatchRunner's single-patch launches and,
+    for the split step, whether the separate dry pass runs (it pays when most
+    tiles are dry).  Every candidate starts from the same state (restored
+    outside the timed region) and times `reps` consecutive steps at a fixed
+    dt; the runner's state and dt are restored afterwards.  Returns the
+    height and the times in ms keyed by (height, dry_pass)."""
     L = _lib.lib()
     qa, qb = runner.qa.clone(), runner.qb.clone()
     dt, cfl = runner.dt.clone(), runner.cfl.clone()
+    modes = tuple(dry_pass_modes) if runner.split_src2 else (runner.dry_pass,)
     times = {}
-    for h in candidates:
-        _lib.check(min(0, L.dclaw_set_seg_rows(h)))
-
-        def one():
-            runner.step()
-            runner.cfl.zero_()
+    for dry in modes:
+        runner.dry_pass = dry
+        for h in candidates:
+            _lib.check(min(0, L.dclaw_set_seg_rows(h)))
             runner.qa.copy_(qa)
-        times[h] = _time(one, reps)
-    best = min(times, key=times.get)
+
+            def one():
+                runner.step()
+                runner.cfl.zero_()
+            times[(h, dry)] = _time(one, reps)
+    best, runner.dry_pass = min(times, key=times.get)
     _lib.check(min(0, L.dclaw_set_seg_rows(best)))
     runner.qa.copy_(qa)
     runner.qb.copy_(qb)

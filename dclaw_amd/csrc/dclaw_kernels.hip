@@ -75,6 +75,23 @@ __device__ __forceinline__ void wave_max_to(double v, double* dst)
 }
 
 int g_seg_rows = 32;   // rows per segment of the single-patch launch (tunable)
+int g_dry_rows = 48;   // rows per wavefront of the split step's dry pass (whole segments)
+
+// Compute units of the current device (cached per device).
+int cu_count()
+{
+    static int cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cache[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n < 1)
+            n = 256;
+        cache[dev] = n;
+    }
+    return cache[dev];
+}
 
 template <int MODE>
 __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in,
@@ -84,9 +101,14 @@ __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in
                                                  double dx, double dy,
                                                  const double* __restrict__ dt_dev,
                                                  const dclaw_params* __restrict__ params,
-                                                 double* cfl_dev, int seg_rows, int bx0, int bx1,
-                                                 int by0, int by1, int* tile_flags)
+                                                 double* cfl_dev, int seg_rows, int nstrips,
+                                                 int nseg, int bx0, int bx1, int by0, int by1,
+                                                 int* tile_flags)
 {
+    // tile order: the dispatcher deals consecutive workgroups to the XCDs
+    // round-robin, which spreads a cluster of wet tiles over the whole chip
+    // (contiguous runs per XCD concentrate it and measured slower)
+    const int strip = (int)blockIdx.x % nstrips, seg = (int)blockIdx.x / nstrips;
     const dclaw_params P = *params;
     StripArgs A;
     A.bc[0] = bx0;
@@ -103,11 +125,11 @@ __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in
     A.dx = dx;
     A.dy = dy;
     A.dt = *dt_dev;
-    A.i0 = 1 + (int)blockIdx.x * STRIP_W;
-    A.jA = 1 + (int)blockIdx.y * seg_rows;
+    A.i0 = 1 + strip * STRIP_W;
+    A.jA = 1 + seg * seg_rows;
     A.jB = min(my, A.jA + seg_rows - 1);
     A.fm = A.fp = A.gm = A.gp = nullptr;
-    A.tile_flag = tile_flags ? tile_flags + (blockIdx.y * gridDim.x + blockIdx.x) : nullptr;
+    A.tile_flag = tile_flags ? tile_flags + (seg * nstrips + strip) : nullptr;
     GpuLane L;
     const double cfl = strip_step_dispatch<MODE>(L, P, A);
     wave_max_to(cfl, cfl_dev);
@@ -175,6 +197,137 @@ __global__ __launch_bounds__(64) void k_src2_tiles(double* q, const double* __re
         const int k = (int)(task % nchunk) * 64 + (int)threadIdx.x;
         if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, q, aux, maux, mx, i0, jA, jB, k);
     }
+}
+
+// Dry pass of the split step (one patch): one wavefront per group of `group`
+// consecutive segments of a strip.  It writes the dry state of every cell the
+// group owns and src2 flag 0 for its dry segments, and appends the others
+// (tile index segment * nstrips + strip) to the wet list, counters[0] being
+// the list length (zero before the launch).
+__global__ __launch_bounds__(64) void k_classify(const double* __restrict__ q_in,
+                                                 double* __restrict__ q_out, int mx, int my,
+                                                 const dclaw_params* __restrict__ params,
+                                                 int seg_rows, int group, int nseg,
+                                                 int nstrips, int bx0, int bx1,
+                                                 int by0, int by1, int* tile_flags,
+                                                 int* wet_list, int* counters)
+{
+    const int strip = (int)blockIdx.x % nstrips, gi = (int)blockIdx.x / nstrips;
+    StripArgs A;
+    A.bc[0] = bx0;
+    A.bc[1] = bx1;
+    A.bc[2] = by0;
+    A.bc[3] = by1;
+    A.q_in = q_in;
+    A.q_out = q_out;
+    A.aux = nullptr;
+    A.aux_out = nullptr;
+    A.maux = 1;
+    A.mx = mx;
+    A.my = my;
+    A.dx = A.dy = A.dt = 0.0;
+    A.i0 = 1 + strip * STRIP_W;
+    const int g0 = gi * group;
+    A.jA = 1 + g0 * seg_rows;
+    A.jB = min(my, A.jA + group * seg_rows - 1);
+    A.fm = A.fp = A.gm = A.gp = nullptr;
+    A.tile_flag = nullptr;
+    GpuLane L;
+    const unsigned wet = strip_classify(L, params->dry_tolerance, A, seg_rows);
+    if (threadIdx.x == 0) {
+        const int ng = min(group, nseg - g0);
+        const int nw = __popc(wet);
+        int at = nw ? atomicAdd(counters, nw) : 0;
+        for (int s = 0; s < ng; s++) {
+            const int t = (g0 + s) * nstrips + strip;
+            if ((wet >> s) & 1u) wet_list[at++] = t;
+            else tile_flags[t] = 0;
+        }
+    }
+}
+
+// The strip body over the wet list: persistent wavefronts take list entries
+// through the work cursor counters[1] (zero before the launch) until the
+// list (counters[0] entries) is exhausted.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_stepgrid_list(const double* __restrict__ q_in,
+                                                      double* __restrict__ q_out,
+                                                      const double* __restrict__ aux, int maux,
+                                                      int mx, int my, double dx, double dy,
+                                                      const double* __restrict__ dt_dev,
+                                                      const dclaw_params* __restrict__ params,
+                                                      double* cfl_dev, int seg_rows, int nstrips,
+                                                      int bx0, int bx1, int by0, int by1,
+                                                      int* tile_flags,
+                                                      const int* __restrict__ wet_list,
+                                                      int* counters)
+{
+    const dclaw_params P = *params;
+    StripArgs A;
+    A.bc[0] = bx0;
+    A.bc[1] = bx1;
+    A.bc[2] = by0;
+    A.bc[3] = by1;
+    A.q_in = q_in;
+    A.q_out = q_out;
+    A.aux = aux;
+    A.aux_out = nullptr;
+    A.maux = maux;
+    A.mx = mx;
+    A.my = my;
+    A.dx = dx;
+    A.dy = dy;
+    A.dt = *dt_dev;
+    A.fm = A.fp = A.gm = A.gp = nullptr;
+    A.known_wet = true;
+    const int n = counters[0];
+    GpuLane L;
+    double cfl = 0.0;
+    for (;;) {
+        int w = 0;
+        if (threadIdx.x == 0) w = atomicAdd(counters + 1, 1);
+        w = __builtin_amdgcn_readfirstlane(w);
+        if (w >= n) break;
+        const int t = wet_list[w];
+        A.i0 = 1 + (t % nstrips) * STRIP_W;
+        A.jA = 1 + (t / nstrips) * seg_rows;
+        A.jB = min(my, A.jA + seg_rows - 1);
+        A.tile_flag = tile_flags + t;
+        cfl = dmax(cfl, strip_step_dispatch<MODE>(L, P, A));
+    }
+    wave_max_to(cfl, cfl_dev);
+}
+
+// src2 over the interior cells of the wet-list tiles the strip body flagged;
+// tasks are 64-cell chunks of those tiles, grid-strided.
+__global__ __launch_bounds__(64) void k_src2_list(double* q, const double* __restrict__ aux,
+                                                  int maux, int mx, int my, int seg_rows,
+                                                  int nstrips, const int* __restrict__ tile_flags,
+                                                  const int* __restrict__ wet_list,
+                                                  const int* __restrict__ counters,
+                                                  const double* __restrict__ dt_dev,
+                                                  const dclaw_params* __restrict__ params)
+{
+    const int nchunk = (STRIP_W * seg_rows + 63) / 64;
+    const long ntask = (long)counters[0] * nchunk;
+    if ((long)blockIdx.x >= ntask) return;
+    const dclaw_params P = *params;
+    const double dt = *dt_dev;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int t = wet_list[task / nchunk];
+        if (!tile_flags[t]) continue;
+        const int i0 = 1 + (t % nstrips) * STRIP_W;
+        const int jA = 1 + (t / nstrips) * seg_rows;
+        const int jB = min(my, jA + seg_rows - 1);
+        const int k = (int)(task % nchunk) * 64 + (int)threadIdx.x;
+        if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, q, aux, maux, mx, i0, jA, jB, k);
+    }
+}
+
+__global__ void k_zero_ints(int* a, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) a[i] = 0;
 }
 
 // The same over a batched work list (each tile's patch, parameters and dt).
@@ -562,21 +715,43 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
     const int seg_rows = g_seg_rows;
     const int nseg = (my + seg_rows - 1) / seg_rows;
-    if (nseg > 65535) return fail(DCLAW_EINVAL, "my too large for one launch");
+    if ((long)nstrips * nseg > 0x7fffffffL) return fail(DCLAW_EINVAL, "patch too large for one launch");
     const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
     if (!tile_flags) {
-        hipLaunchKernelGGL(k_stepgrid<MODE_STEP>, dim3(nstrips, nseg), dim3(64), 0, stream, q_in,
-                           q_out, aux, aux_out, maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev,
-                           seg_rows, b0, b1, b2, b3, nullptr);
+        hipLaunchKernelGGL(k_stepgrid<MODE_STEP>, dim3(nstrips * nseg), dim3(64), 0, stream,
+                           q_in, q_out, aux, aux_out, maux, mx, my, dx, dy, dt_dev, params_dev,
+                           cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, nullptr);
         HIP_TRY(hipGetLastError());
         return DCLAW_OK;
     }
     const int ntiles = nstrips * nseg;
     if (n_flags < ntiles) return fail(DCLAW_EINVAL, "tile_flags holds fewer than dclaw_tile_count() ints");
     if (aux_out) return fail(DCLAW_EINVAL, "the split step does not write aux");
-    hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips, nseg), dim3(64), 0, stream,
+    if ((long)n_flags >= 2L * ntiles + 2) {
+        // dry pass, strip body over the wet list, src2 over the wet list
+        int* wet_list = tile_flags + ntiles;
+        int* counters = tile_flags + 2 * ntiles;
+        const int group = std::max(1, std::min(32, g_dry_rows / seg_rows));
+        const int ngroups = (nseg + group - 1) / group;
+        const int ncu = cu_count();
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, stream, counters, 2);
+        hipLaunchKernelGGL(k_classify, dim3(nstrips * ngroups), dim3(64), 0, stream, q_in, q_out,
+                           mx, my, params_dev, seg_rows, group, nseg, nstrips, b0, b1, b2, b3,
+                           tile_flags, wet_list, counters);
+        hipLaunchKernelGGL(k_stepgrid_list<MODE_STEP_NOSRC2>, dim3(std::min(ntiles, 4 * ncu)),
+                           dim3(64), 0, stream, q_in, q_out, aux, maux, mx, my, dx, dy, dt_dev,
+                           params_dev, cfl_dev, seg_rows, nstrips, b0, b1, b2, b3, tile_flags,
+                           wet_list, counters);
+        const long nt = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
+        hipLaunchKernelGGL(k_src2_list, dim3((int)std::min<long>(nt, 16L * ncu)), dim3(64), 0,
+                           stream, q_out, aux, maux, mx, my, seg_rows, nstrips, tile_flags,
+                           wet_list, counters, dt_dev, params_dev);
+        HIP_TRY(hipGetLastError());
+        return DCLAW_OK;
+    }
+    hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips * nseg), dim3(64), 0, stream,
                        q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
-                       cfl_dev, seg_rows, b0, b1, b2, b3, tile_flags);
+                       cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, tile_flags);
     HIP_TRY(hipGetLastError());
     const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
     hipLaunchKernelGGL(k_src2_tiles, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, stream,
@@ -610,6 +785,22 @@ int dclaw_tile_count(int mx, int my)
     const long n = (long)((mx + STRIP_W - 1) / STRIP_W) * ((my + g_seg_rows - 1) / g_seg_rows);
     if (n > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
     return (int)n;
+}
+
+int dclaw_split_workspace_size(int mx, int my)
+{
+    const int n = dclaw_tile_count(mx, my);
+    if (n < 0) return n;
+    if (2L * n + 2 > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
+    return 2 * n + 2;
+}
+
+int dclaw_set_dry_rows(int rows)
+{
+    if (rows < 1 || rows > 65536) return fail(DCLAW_EINVAL, "dry rows must be in [1, 65536]");
+    const int prev = g_dry_rows;
+    g_dry_rows = rows;
+    return prev;
 }
 
 int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* aux, int maux,

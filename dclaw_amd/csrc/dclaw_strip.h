@@ -40,6 +40,7 @@ constexpr int MODE_FLUX = 2;
 
 constexpr int STRIP_LANES = 64;
 constexpr int DRY_CHUNK = 8;       // rows per load batch of the whole-segment dry check
+constexpr int CLASSIFY_CHUNK = 4;  // rows per load batch of the split step's dry pass
 constexpr int STRIP_HALO = 3;
 constexpr int STRIP_W = STRIP_LANES - 2 * STRIP_HALO;  // 58 interior columns
 
@@ -54,6 +55,8 @@ struct StripArgs {
     int bc[4];             // bc codes (xlo, xhi, ylo, yhi) applied on load; 0 = use stored ghosts
     double *fm, *fp, *gm, *gp;   // step2 outputs (FLUX mode; same shape as q, zero-filled)
     int* tile_flag;              // MODE_STEP_NOSRC2: set to 1 if src2 has wet cells here, else 0
+    bool known_wet = false;      // skip the whole-segment dry check (strip_classify found the
+                                 // segment not dry and already wrote the dry state)
 };
 
 // Source cell and momentum signs of a ghost cell under the bc2amr rules
@@ -331,6 +334,197 @@ DC_HD void load_row(const StripArgs& A, int c, int R, bool col_in_array, double*
     b = A.aux[aidx(c, R, mx, A.maux)];
 }
 
+// The whole-segment dry check of rows [jA, jB] of a strip.  An owned cell's
+// output differs from the dry state (zeros and bdif >= 0) only if a cell of
+// its 3x3 neighbourhood is wet (h > dry_tolerance before qfix): a wave needs
+// a wet side, the second-order correction needs four wet cells in a row
+// (flux2's gate), and the transverse terms reaching the cell come from edges
+// touching its neighbours; an owned ghost cell only needs itself dry
+// (b4step2 is pointwise).  So if rows jA-1..jB+1 and columns i0-1..i0+58
+// (plus the owned ghost rows and columns) are dry, every output of the
+// segment is the dry state.  With the boundary conditions applied on load
+// only interior cells are checked (the ghost sources are interior cells in
+// range).
+//
+// The strip body scans one cell per lane (dry_scan_cell).  The split step's
+// dry pass reads each checked row as one contiguous run of doubles (the cells
+// [kl, kh] of row R are q words [7(kl+1+(mx+4)(R+1)), 7(kh+2+(mx+4)(R+1))))
+// with 16-byte accesses, and writes the dry state of the owned cells [wl, wh]
+// from the same run: word m of a cell becomes 0 for m < 6 and max(bdif, 0)
+// for m = 6, an elementwise map of the run.  Owned ghost cells whose values
+// come through the boundary map are written one by one.
+struct DryScan {
+    bool own;       // this lane's column is written by the strip (interior or owned ghost)
+    bool bcl;       // boundary conditions applied on load
+    bool vec_out;   // q_out has q_in's 16-byte alignment (16-byte stores)
+    int lo, hi;     // rows scanned: jA-1..jB+1, or the owned ghost rows at the patch edges
+    int r0, r1;     // rows read as runs (their h checked)
+    int kl, kh;     // columns of the run: the checked columns
+    int wl, wh;     // owned columns written from the run
+};
+
+DC_HD DryScan dry_scan(const StripArgs& A, bool own)
+{
+    DryScan D;
+    const int mx = A.mx, my = A.my;
+    D.bcl = (A.bc[0] != 0);
+    const bool first_s = (A.i0 == 1), last_s = (A.i0 + STRIP_W - 1 >= mx);
+    const int clo = D.bcl ? 1 : -1, chi = D.bcl ? mx : mx + 2;
+    const int rlo = D.bcl ? 1 : -1, rhi = D.bcl ? my : my + 2;
+    const int ca = first_s ? -1 : A.i0 - 1;
+    const int cb = last_s ? mx + 2 : A.i0 + STRIP_W;
+    const int ol = first_s ? -1 : A.i0;
+    const int oh = last_s ? mx + 2 : A.i0 + STRIP_W - 1;
+    D.kl = ca > clo ? ca : clo;
+    D.kh = cb < chi ? cb : chi;
+    D.wl = ol > clo ? ol : clo;
+    D.wh = oh < chi ? oh : chi;
+    D.own = own;
+    D.vec_out = A.q_out != nullptr &&
+                ((((uintptr_t)A.q_in) ^ ((uintptr_t)A.q_out)) & (uintptr_t)15) == 0;
+    D.lo = (A.jA == 1) ? -1 : A.jA - 1;
+    D.hi = (A.jB == my) ? my + 2 : A.jB + 1;
+    D.r0 = D.lo > rlo ? D.lo : rlo;
+    D.r1 = D.hi < rhi ? D.hi : rhi;
+    return D;
+}
+
+// Row R of the scan on this lane's own column, one cell at a time (the strip
+// body's check): h if the cell is checked (else 0), whether the lane writes
+// the cell (wr) and then its raw bdif (through the boundary map for a ghost
+// cell when the boundary conditions are applied on load).
+DC_HD void dry_scan_cell(const StripArgs& A, const DryScan& D, int c, int R, double& h, bool& wr,
+                         double& bd)
+{
+    const int mx = A.mx, my = A.my;
+    const double* __restrict__ qi = A.q_in;
+    h = 0.0;
+    if (c >= D.kl && c <= D.kh && R >= D.r0 && R <= D.r1) h = qi[qidx(0, c, R, mx)];
+    const bool row_in = (R >= -1 && R <= my + 2) && R <= D.hi;
+    const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
+                             (R >= my + 1 && R <= my + 2 && A.jB == my);
+    const bool row_interior = (R >= 1 && R <= my) && R >= A.jA && R <= A.jB;
+    wr = D.own && row_in && (row_ghost_w || row_interior);
+    bd = 0.0;
+    if (wr) {
+        if ((c < 1 || c > mx || R < 1 || R > my) && D.bcl) {
+            int si, sj;
+            double fu, fv;
+            bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
+            bd = qi[qidx(6, si, sj, mx)];
+        } else {
+            bd = qi[qidx(6, c, R, mx)];
+        }
+    }
+}
+
+// One row run as this lane holds it: the 16-byte chunks lane, lane+64,
+// lane+128, lane+192 of the run aligned down to a 16-byte boundary of q_in
+// (62 cells at most: 218 chunks), each as its two doubles (0 outside the run).
+constexpr int VROW_CHUNKS = 4;
+struct VRow {
+    double x[VROW_CHUNKS], y[VROW_CHUNKS];
+};
+
+// Word positions of row R's run: rb = first word of the row, [wb, we) the
+// run, a0 its 16-byte aligned start.
+struct VRun {
+    long rb, wb, we, a0;
+};
+
+DC_HD VRun vrun(const StripArgs& A, const DryScan& D, int R)
+{
+    VRun r;
+    r.rb = 7L * (long)(A.mx + 4) * (long)(R + 1);
+    r.wb = r.rb + 7L * (D.kl + 1);
+    r.we = r.rb + 7L * (D.kh + 2);
+    r.a0 = r.wb - (long)((((uintptr_t)A.q_in >> 3) + (uintptr_t)r.wb) & (uintptr_t)1);
+    return r;
+}
+
+DC_HD void vrow_load(const double* __restrict__ q, const VRun& r, int lane, VRow& v)
+{
+#pragma unroll
+    for (int u = 0; u < VROW_CHUNKS; u++) {
+        const long w = r.a0 + 2L * (lane + STRIP_LANES * u);
+        if (w >= r.wb && w + 1 < r.we) {
+            const double2 t = *reinterpret_cast<const double2*>(q + w);
+            v.x[u] = t.x;
+            v.y[u] = t.y;
+        } else {
+            v.x[u] = (w >= r.wb && w < r.we) ? q[w] : 0.0;
+            v.y[u] = (w + 1 >= r.wb && w + 1 < r.we) ? q[w + 1] : 0.0;
+        }
+    }
+}
+
+// Any h > dry_tolerance in this lane's part of the run (the run holds only
+// checked cells).
+DC_HD bool vrow_wet(const VRun& r, const VRow& v, int lane, double dry_tolerance)
+{
+    bool wet = false;
+#pragma unroll
+    for (int u = 0; u < VROW_CHUNKS; u++) {
+        const long w = r.a0 + 2L * (lane + STRIP_LANES * u);
+        const int m0 = ((int)(w - r.rb) + 7) % 7;   // component of the first double
+        if (m0 == 0 && w >= r.wb && w < r.we && v.x[u] > dry_tolerance) wet = true;
+        if (m0 == 6 && w + 1 >= r.wb && w + 1 < r.we && v.y[u] > dry_tolerance) wet = true;
+    }
+    return wet;
+}
+
+// The dry state of cell (c, R): qfix zeroes the dry cell and bdif is clipped
+// at 0 (b4step2.f90:56-63); a cell with a dry neighbourhood keeps that state.
+DC_HD void dry_write(const StripArgs& A, int c, int R, double bd)
+{
+    double* __restrict__ qo = A.q_out;
+    const size_t o = qidx(0, c, R, A.mx);
+    for (int z = 0; z < 6; z++) qo[o + z] = 0.0;
+    qo[o + 6] = dmax(bd, 0.0);
+}
+
+// Dry state of the cells row R of the segment owns: the run's owned columns
+// from the run (v, when R was read as a run), and on this lane's own column
+// an owned ghost cell whose bdif comes through the boundary map.
+DC_HD void dry_row_write(const StripArgs& A, const DryScan& D, int c, int lane, int R,
+                         const VRow& v)
+{
+    const int mx = A.mx, my = A.my;
+    if (R > D.hi) return;
+    const bool row_own = (R >= -1 && R <= 0 && A.jA == 1) ||
+                         (R >= my + 1 && R <= my + 2 && A.jB == my) ||
+                         (R >= 1 && R <= my && R >= A.jA && R <= A.jB);
+    if (!row_own) return;
+    if (R >= D.r0 && R <= D.r1) {
+        const VRun r = vrun(A, D, R);
+        const long ob = r.rb + 7L * (D.wl + 1), oe = r.rb + 7L * (D.wh + 2);
+        double* __restrict__ qo = A.q_out;
+#pragma unroll
+        for (int u = 0; u < VROW_CHUNKS; u++) {
+            const long w = r.a0 + 2L * (lane + STRIP_LANES * u);
+            const int m0 = ((int)(w - r.rb) + 7) % 7;
+            const double ox = (m0 == 6) ? dmax(v.x[u], 0.0) : 0.0;
+            const double oy = (m0 == 5) ? dmax(v.y[u], 0.0) : 0.0;
+            const bool in0 = (w >= ob && w < oe), in1 = (w + 1 >= ob && w + 1 < oe);
+            if (in0 && in1 && D.vec_out) {
+                double2 t;
+                t.x = ox;
+                t.y = oy;
+                *reinterpret_cast<double2*>(qo + w) = t;
+            } else {
+                if (in0) qo[w] = ox;
+                if (in1) qo[w + 1] = oy;
+            }
+        }
+    }
+    if (D.bcl && D.own && (c < 1 || c > mx || R < 1 || R > my)) {
+        int si, sj;
+        double fu, fv;
+        bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
+        dry_write(A, c, R, A.q_in[qidx(6, si, sj, mx)]);
+    }
+}
+
 // The strip body.  Returns this lane's CFL contribution (max |dt/dx * s|).
 template <int MODE, class Lane>
 DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
@@ -381,63 +575,21 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
         gx_prev2[k] = gp_prev[k] = 0.0;
     }
 
-    // Whole-segment dry check.  An owned cell's output differs from the dry
-    // state (zeros and bdif >= 0) only if a cell of its 3x3 neighbourhood is
-    // wet (h > dry_tolerance before qfix): a wave needs a wet side, the
-    // second-order correction needs four wet cells in a row (flux2's gate),
-    // and the transverse terms reaching the cell come from edges touching its
-    // neighbours; an owned ghost cell only needs itself dry (b4step2 is
-    // pointwise).  So if rows jA-1..jB+1 and columns i0-1..i0+58 (plus the
-    // owned ghost rows and columns) are dry, every output of the segment is
-    // the dry state and the marching loop would only copy it.  Rows are
-    // scanned in chunks of DRY_CHUNK with h and bdif loaded together (one pass over
-    // the cache lines), and the dry state of each chunk's own rows is written
-    // as soon as the chunk is known dry; if a later chunk is wet, the marching
-    // loop rewrites every row this segment owns, so those early writes are
-    // harmless.  With the boundary conditions applied on load only interior
-    // cells are checked (the ghost sources are interior cells in range).
-    {
-        const bool bcl = (A.bc[0] != 0);
-        const bool first_s = (A.i0 == 1), last_s = (A.i0 + STRIP_W - 1 >= mx);
-        const int clo = bcl ? 1 : -1, chi = bcl ? mx : mx + 2;
-        const int rlo = bcl ? 1 : -1, rhi = bcl ? my : my + 2;
-        const int ca = first_s ? -1 : A.i0 - 1;
-        const int cb = last_s ? mx + 2 : A.i0 + STRIP_W;
-        const bool col_chk = (c >= ca && c <= cb && c >= clo && c <= chi);
-        const int lo = (A.jA == 1) ? -1 : A.jA - 1;
-        const int hi = (A.jB == my) ? my + 2 : A.jB + 1;
-        const int r0 = lo > rlo ? lo : rlo;
-        const int r1 = hi < rhi ? hi : rhi;
-        const double* __restrict__ qi = A.q_in;
-        double* __restrict__ qo = A.q_out;
+    // Whole-segment dry check (DryScan).  Rows are scanned in chunks of
+    // DRY_CHUNK with h and bdif loaded together (one pass over the cache
+    // lines), and the dry state of each chunk's own rows is written as soon as
+    // the chunk is known dry; if a later chunk is wet, the marching loop
+    // rewrites every row this segment owns, so those early writes are
+    // harmless.  Skipped when the split step's dry pass has already found the
+    // segment not dry and written its dry state.
+    if (!A.known_wet) {
+        const DryScan D = dry_scan(A, !FLUX && col_in_array && (col_out || col_ghost_w));
         bool seg_wet = false;
-        for (int R0 = lo; R0 <= hi; R0 += DRY_CHUNK) {
+        for (int R0 = D.lo; R0 <= D.hi; R0 += DRY_CHUNK) {
             double hh[DRY_CHUNK], bd[DRY_CHUNK];
             bool wr[DRY_CHUNK];
 #pragma unroll
-            for (int k = 0; k < DRY_CHUNK; k++) {
-                const int R = R0 + k;
-                hh[k] = 0.0;
-                if (col_chk && R >= r0 && R <= r1) hh[k] = qi[qidx(0, c, R, mx)];
-                const bool row_in = (R >= -1 && R <= my + 2) && R <= hi;
-                const bool row_ghost_w = (R >= -1 && R <= 0 && A.jA == 1) ||
-                                         (R >= my + 1 && R <= my + 2 && A.jB == my);
-                const bool row_interior = (R >= 1 && R <= my) && R >= A.jA && R <= A.jB;
-                wr[k] = !FLUX && row_in && col_in_array &&
-                        ((row_ghost_w && (col_out || col_ghost_w)) ||
-                         (row_interior && (col_out || col_ghost_w)));
-                bd[k] = 0.0;
-                if (wr[k]) {
-                    if ((c < 1 || c > mx || R < 1 || R > my) && bcl) {
-                        int si, sj;
-                        double fu, fv;
-                        bc_source(A.bc, mx, my, c, R, si, sj, fu, fv);
-                        bd[k] = qi[qidx(6, si, sj, mx)];
-                    } else {
-                        bd[k] = qi[qidx(6, c, R, mx)];
-                    }
-                }
-            }
+            for (int k = 0; k < DRY_CHUNK; k++) dry_scan_cell(A, D, c, R0 + k, hh[k], wr[k], bd[k]);
             bool w = false;
 #pragma unroll
             for (int k = 0; k < DRY_CHUNK; k++) w |= (hh[k] > P.dry_tolerance);
@@ -446,12 +598,8 @@ DC_HD double strip_step(Lane& L, const dclaw_params& P, const StripArgs& A)
                 break;
             }
 #pragma unroll
-            for (int k = 0; k < DRY_CHUNK; k++) {
-                if (!wr[k]) continue;
-                const size_t o = qidx(0, c, R0 + k, mx);
-                for (int z = 0; z < 6; z++) qo[o + z] = 0.0;
-                qo[o + 6] = dmax(bd[k], 0.0);
-            }
+            for (int k = 0; k < DRY_CHUNK; k++)
+                if (wr[k]) dry_write(A, c, R0 + k, bd[k]);
         }
         if (!seg_wet) {
             if (NOSRC2 && L.id() == 0) *A.tile_flag = 0;
@@ -869,6 +1017,59 @@ DC_HD void src2_tile_cell(const dclaw_params& P, double dt, double* q, const dou
     const double h_e = P.entrainment ? aux[aidx(i, j, mx, maux) + 6] : 0.0;
     src2_cell(P, dt, h, hu, hv, hm, p, hchi, bdif, h_e);
     c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi; c[6] = bdif;
+}
+
+// Dry pass of the split step over a group of consecutive segments of one
+// strip: A.jA..A.jB span the group, cut into segments of seg_rows rows from
+// A.jA (the last possibly shorter).  One scan of rows A.jA-1..A.jB+1 answers
+// the whole-segment dry check of strip_step for every segment of the group
+// (row by row, each wet row marking the segments whose scan range holds it)
+// and writes the dry state to every cell the group owns.  Returns the mask of
+// the segments that are not dry (bit s: segment s; at most 32 segments);
+// strip_step with known_wet must then run on each of them, and it rewrites
+// every cell its segment owns.  Only the scan and the stores touch memory, so
+// this runs at far higher occupancy than the strip body.
+template <class Lane>
+DC_HD unsigned strip_classify(Lane& L, double dry_tolerance, const StripArgs& A, int seg_rows)
+{
+    const int c = A.i0 - STRIP_HALO + L.id();
+    const int mx = A.mx, my = A.my;
+    const bool col_in_array = (c >= -1 && c <= mx + 2);
+    const bool col_out = (c >= A.i0 && c <= A.i0 + STRIP_W - 1 && c >= 1 && c <= mx);
+    const bool col_ghost_w = (c >= -1 && c <= 0 && A.i0 == 1) ||
+                             (c >= mx + 1 && c <= mx + 2 && A.i0 + STRIP_W - 1 >= mx);
+    const int lane = L.id();
+    const DryScan D = dry_scan(A, col_in_array && (col_out || col_ghost_w));
+    const int nseg = (A.jB - A.jA) / seg_rows + 1;
+    unsigned wet = 0u;
+    for (int R0 = D.lo; R0 <= D.hi; R0 += CLASSIFY_CHUNK) {
+        VRow vr[CLASSIFY_CHUNK];
+#pragma unroll
+        for (int k = 0; k < CLASSIFY_CHUNK; k++)
+            if (R0 + k >= D.r0 && R0 + k <= D.r1)
+                vrow_load(A.q_in, vrun(A, D, R0 + k), lane, vr[k]);
+#pragma unroll
+        for (int k = 0; k < CLASSIFY_CHUNK; k++) {
+            const bool rw = R0 + k >= D.r0 && R0 + k <= D.r1 &&
+                            vrow_wet(vrun(A, D, R0 + k), vr[k], lane, dry_tolerance);
+            if (!L.any(rw)) continue;
+            // segments s whose scan range [jA_s - 1, jB_s + 1] (extended to
+            // the ghost rows at the patch edges) holds row R
+            const int R = R0 + k;
+            const int sr = R >= A.jA ? (R - A.jA) / seg_rows : -1;
+            for (int s = sr - 1; s <= sr + 1; s++) {
+                if (s < 0 || s >= nseg) continue;
+                const int sa = A.jA + s * seg_rows;
+                const int sb = sa + seg_rows - 1 < A.jB ? sa + seg_rows - 1 : A.jB;
+                const int slo = (sa == 1) ? -1 : sa - 1;
+                const int shi = (sb == my) ? my + 2 : sb + 1;
+                if (R >= slo && R <= shi) wet |= 1u << s;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CLASSIFY_CHUNK; k++) dry_row_write(A, D, c, lane, R0 + k, vr[k]);
+    }
+    return wet;
 }
 
 // The default method switches of a D-Claw run (second order, transverse

@@ -147,13 +147,24 @@ def src2(q: torch.Tensor, aux: torch.Tensor, mx: int, my: int, dt_dev: torch.Ten
 
 def stepgrid_split(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int, mx: int,
                    my: int, dx: float, dy: float, bc, dt_dev: torch.Tensor,
-                   params_dev: torch.Tensor, cfl_dev: torch.Tensor, flags: torch.Tensor) -> None:
+                   params_dev: torch.Tensor, cfl_dev: torch.Tensor, flags: torch.Tensor,
+                   dry_pass: bool = True) -> None:
     """The step with src2 in a second pass over the wet tiles (bc None: stored
-    ghosts); flags is an int32 workspace of at least dclaw_tile_count ints."""
+    ghosts); flags is an int32 workspace of at least dclaw_tile_count ints.
+    With dry_pass and at least dclaw_split_workspace_size ints the dry tiles
+    are written by a separate light pass and the strip kernel runs over the
+    others only; dry_pass=False passes only dclaw_tile_count ints (the single
+    strip launch).  Both give the same result."""
     arr = None if bc is None else (ctypes.c_int * 4)(*bc)
-    _lib.check(_lib.lib().dclaw_stepgrid_split_dev(
+    L = _lib.lib()
+    n = flags.numel()
+    if not dry_pass:
+        tc = L.dclaw_tile_count(mx, my)
+        _lib.check(min(tc, 0))
+        n = min(n, tc)
+    _lib.check(L.dclaw_stepgrid_split_dev(
         _ptr(q_in), _ptr(q_out), _ptr(aux), maux, mx, my, dx, dy, arr, _ptr(dt_dev),
-        _ptr(params_dev), _ptr(cfl_dev), _ptr(flags), flags.numel(), _stream()))
+        _ptr(params_dev), _ptr(cfl_dev), _ptr(flags), n, _stream()))
 
 
 def dt_update(dt_dev: torch.Tensor, cfl_dev: torch.Tensor, cfl_desired: float,
@@ -198,8 +209,10 @@ class PatchRunner:
     """Device-resident fixed-patch time loop (one member)."""
 
     def __init__(self, q: np.ndarray, aux: np.ndarray, params: DigParams, dx: float, dy: float,
-                 bc=(BC_EXTRAP,) * 4, device="cuda", split_src2: bool = True):
+                 bc=(BC_EXTRAP,) * 4, device="cuda", split_src2: bool = True,
+                 dry_pass: bool = True):
         self.split_src2 = split_src2
+        self.dry_pass = dry_pass
         self.mx = q.shape[1] - 2 * MBC
         self.my = q.shape[2] - 2 * MBC
         self.dx, self.dy = float(dx), float(dy)
@@ -212,9 +225,10 @@ class PatchRunner:
         self.params_host = params
         self.dt = torch.zeros(1, dtype=torch.float64, device=device)
         self.cfl = torch.zeros(1, dtype=torch.float64, device=device)
-        # tile flags for any segment height (one row per tile at most)
+        # split-step workspace for any segment height (one row per tile at
+        # most): tile flags, the wet list and its two counters
         nstrips = -(-self.mx // _lib.lib().dclaw_strip_width())
-        self.flags = torch.zeros(nstrips * self.my, dtype=torch.int32, device=device)
+        self.flags = torch.zeros(2 * nstrips * self.my + 2, dtype=torch.int32, device=device)
 
     def set_dt(self, dt: float) -> None:
         self.dt.fill_(float(dt))
@@ -224,7 +238,7 @@ class PatchRunner:
         the result is left in self.qa and cflgrid in self.cfl."""
         if fused_bc and self.split_src2:
             stepgrid_split(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
-                           self.bc, self.dt, self.params, self.cfl, self.flags)
+                           self.bc, self.dt, self.params, self.cfl, self.flags, self.dry_pass)
         elif fused_bc:
             stepgrid_bc(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
                         self.bc, self.dt, self.params, self.cfl)

@@ -114,8 +114,17 @@ int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, 
                           double* cfl_dev, void* stream);
 
 /* Tiles (58-column strip x segment) of one patch at the current segment
- * height: the size of the tile_flags workspace of the split step below. */
+ * height: the smallest tile_flags workspace of the split step below. */
 int dclaw_tile_count(int mx, int my);
+
+/* Workspace ints (2 * dclaw_tile_count(mx, my) + 2) with which the split step
+ * below also runs its dry pass.  No reference counterpart. */
+int dclaw_split_workspace_size(int mx, int my);
+
+/* Rows per wavefront of the split step's dry pass (rounded down to whole
+ * segments, at least one, at most 32); returns the previous value or a
+ * negative error.  Results do not depend on it.  No reference counterpart. */
+int dclaw_set_dry_rows(int rows);
 
 /* The step of dclaw_stepgrid_bc_dev (bc may be NULL: stored ghosts) with src2
  * split into a second pass: the strip kernel writes the updated state,
@@ -124,8 +133,13 @@ int dclaw_tile_count(int mx, int my);
  * occupancy, over the interior cells of the flagged tiles only.  Same
  * arithmetic, same result as the fused step.  tile_flags_dev: device
  * workspace of n_flags >= dclaw_tile_count(mx, my) ints, written every call
- * (no initialisation needed).  No reference counterpart (a GPU scheduling
- * choice). */
+ * (no initialisation needed); its first dclaw_tile_count ints are the src2
+ * flags of the tiles (segment * strips + strip).  With n_flags >=
+ * dclaw_split_workspace_size(mx, my) a dry pass runs first: a light kernel
+ * writes the dry state of every tile whose 1-cell neighbourhood is dry and
+ * lists the others, and the strip kernel and src2 run over that list only
+ * (the rest of the workspace holds the list and its counters).  Same result
+ * either way.  No reference counterpart (a GPU scheduling choice). */
 int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* aux, int maux,
                              int mx, int my, double dx, double dy, const int* bc,
                              const double* dt_dev, const dclaw_params* params_dev,

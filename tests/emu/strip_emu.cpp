@@ -29,8 +29,12 @@ struct Sched {
     const dclaw_params* P;
     const StripArgs* A;
     int mode;
+    int seg_rows;                 // MODE_CLASSIFY
+    unsigned mask[STRIP_LANES];   // MODE_CLASSIFY result per lane
     std::vector<char> stacks;
 };
+
+constexpr int MODE_CLASSIFY = 100;   // strip_classify instead of the strip body
 
 struct CoLane {
     int lid;
@@ -65,6 +69,12 @@ void lane_entry(int lid)
 {
     Sched* s = g_sched;
     CoLane L{lid, 0, s};
+    if (s->mode == MODE_CLASSIFY) {
+        s->mask[lid] = strip_classify(L, s->P->dry_tolerance, *s->A, s->seg_rows);
+        s->cfl[lid] = 0.0;
+        s->done[lid] = true;
+        return;
+    }
     s->cfl[lid] = s->mode == MODE_FLUX ? strip_step_dispatch<MODE_FLUX>(L, *s->P, *s->A)
                   : s->mode == MODE_STEP_NOSRC2
                       ? strip_step_dispatch<MODE_STEP_NOSRC2>(L, *s->P, *s->A)
@@ -72,13 +82,17 @@ void lane_entry(int lid)
     s->done[lid] = true;
 }
 
-void run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out, int mode = MODE_STEP)
+// Runs one wavefront; returns the classify mask (MODE_CLASSIFY; -1 if the
+// lanes disagree), else 0.
+long run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out, int mode = MODE_STEP,
+               int seg_rows = 0)
 {
     constexpr size_t STACK = 1 << 20;
     Sched* s = new Sched();
     s->P = &P;
     s->A = &A;
     s->mode = mode;
+    s->seg_rows = seg_rows;
     s->stacks.resize(STACK * STRIP_LANES);
     g_sched = s;
     for (int l = 0; l < STRIP_LANES; l++) {
@@ -99,7 +113,14 @@ void run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out, int m
         }
     }
     for (int l = 0; l < STRIP_LANES; l++) *cfl_out = dmax(*cfl_out, s->cfl[l]);
+    long mask = 0;
+    if (mode == MODE_CLASSIFY) {
+        mask = (long)s->mask[0];
+        for (int l = 1; l < STRIP_LANES; l++)
+            if (s->mask[l] != s->mask[0]) mask = -1;
+    }
     delete s;
+    return mask;
 }
 
 }  // namespace
@@ -179,6 +200,73 @@ extern "C" int emu_stepgrid_split(const dclaw_params* P, const double* q_in, dou
             for (int k = 0; k < STRIP_W * (jB - jA + 1); k++)
                 src2_tile_cell(*P, dt, q_out, aux, maux, mx, i0, jA, jB, k);
         }
+    return 0;
+}
+
+// The split step with its dry pass, as the GPU runs it: strip_classify over
+// groups of `group` segments writes the dry state and lists the other
+// segments; the strip body (known_wet, no src2) runs on the listed segments
+// in reverse list order (the GPU's order is arbitrary), then src2 on the
+// flagged ones.  nwet: listed segments.
+extern "C" int emu_stepgrid_classified(const dclaw_params* P, const double* q_in, double* q_out,
+                                       const double* aux, int maux, int mx, int my, double dx,
+                                       double dy, double dt, int seg_rows, int group,
+                                       const int* bc, double* cflgrid, int* nwet)
+{
+    *cflgrid = 0.0;
+    *nwet = 0;
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
+    std::vector<int> flags((size_t)nstrips * nseg, -1);
+    std::vector<int> list;
+    StripArgs A;
+    A.q_in = q_in;
+    A.q_out = q_out;
+    A.aux = aux;
+    A.aux_out = nullptr;
+    A.maux = maux;
+    A.mx = mx;
+    A.my = my;
+    A.dx = dx;
+    A.dy = dy;
+    A.dt = dt;
+    for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
+    A.fm = A.fp = A.gm = A.gp = nullptr;
+    A.tile_flag = nullptr;
+    for (int s = 0; s < nstrips; s++)
+        for (int g0 = 0; g0 < nseg; g0 += group) {
+            A.i0 = 1 + s * STRIP_W;
+            A.jA = 1 + g0 * seg_rows;
+            A.jB = std::min(my, A.jA + group * seg_rows - 1);
+            A.known_wet = false;
+            const long m = run_strip(*P, A, cflgrid, MODE_CLASSIFY, seg_rows);
+            if (m < 0) return -2;          // lanes disagree on the mask
+            const int ng = std::min(group, nseg - g0);
+            if (m >> ng) return -3;        // a bit beyond the group
+            for (int k = 0; k < ng; k++) {
+                const int t = (g0 + k) * nstrips + s;
+                if ((m >> k) & 1) list.push_back(t);
+                else flags[t] = 0;
+            }
+        }
+    *nwet = (int)list.size();
+    for (auto it = list.rbegin(); it != list.rend(); ++it) {
+        const int t = *it;
+        A.i0 = 1 + (t % nstrips) * STRIP_W;
+        A.jA = 1 + (t / nstrips) * seg_rows;
+        A.jB = std::min(my, A.jA + seg_rows - 1);
+        A.known_wet = true;
+        A.tile_flag = &flags[t];
+        run_strip(*P, A, cflgrid, MODE_STEP_NOSRC2);
+    }
+    for (int t = 0; t < nstrips * nseg; t++) {
+        if (flags[t] < 0) return -1;       // a tile did not report
+        if (!flags[t]) continue;
+        const int i0 = 1 + (t % nstrips) * STRIP_W, jA = 1 + (t / nstrips) * seg_rows;
+        const int jB = std::min(my, jA + seg_rows - 1);
+        for (int k = 0; k < STRIP_W * (jB - jA + 1); k++)
+            src2_tile_cell(*P, dt, q_out, aux, maux, mx, i0, jA, jB, k);
+    }
     return 0;
 }
 

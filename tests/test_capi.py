@@ -74,3 +74,25 @@ def test_stage_argument_validation(L):
     assert L.dclaw_src2_dev(None, 2, 6, 10, 10, 3, 4, None, None) == -1
     assert L.dclaw_check4nans_dev(1, 0, 10, 2, None) == -1
     assert L.dclaw_check4nans_dev(1, 100000, 100000, 2, None) == -1   # flat index overflows
+
+
+def test_split_workspace_and_dry_rows(L):
+    """Host-side sizing of the split step's workspace and the dry-pass
+    setting: 2 ints per tile plus the two list counters; bad values rejected
+    without changing the setting."""
+    prev = L.dclaw_set_seg_rows(12)
+    try:
+        nt = L.dclaw_tile_count(4096, 4096)
+        assert nt == 71 * 342
+        assert L.dclaw_split_workspace_size(4096, 4096) == 2 * nt + 2
+        assert L.dclaw_split_workspace_size(1, 1) == 4
+        assert L.dclaw_split_workspace_size(0, 5) == -1
+    finally:
+        L.dclaw_set_seg_rows(prev)
+    p = L.dclaw_set_dry_rows(96)
+    assert p >= 1
+    assert L.dclaw_set_dry_rows(0) == -1 and b"dry rows" in L.dclaw_last_error()
+    assert L.dclaw_set_dry_rows(p) == 96
+    # the split step rejects a workspace smaller than one int per tile
+    rc = L.dclaw_stepgrid_split_dev(1, 2, 3, 1, 100, 100, 1.0, 1.0, None, 4, 5, 6, 7, 1, None)
+    assert rc == -1 and b"tile_flags" in L.dclaw_last_error()

@@ -240,3 +240,142 @@ def test_entrainment_fused_and_split_bitwise(oracle_mod, emu, emu_lib):
     I = (slice(None), slice(2, -2), slice(2, -2))
     assert np.array_equal(qe[I], qo[I])
     assert cfl.value == cflo
+
+
+def _emu_split(lib, params, q, aux, dx, dy, dt, seg, bc, group=None):
+    """The split step by the emulator: single strip launch (group None) or
+    with the dry pass over groups of `group` segments.  q_out starts as NaN,
+    so a cell the step does not write shows."""
+    from dclaw_amd.params import to_c
+    dp = ctypes.POINTER(ctypes.c_double)
+    qi = np.array(q, order="F", copy=True)
+    qe = np.full_like(qi, np.nan, order="F")
+    auxf = np.array(aux, order="F", copy=True)
+    mx, my = qi.shape[1] - 4, qi.shape[2] - 4
+    cp = to_c(params)
+    cfl, n = ctypes.c_double(0.0), ctypes.c_int(0)
+    bca = None if bc is None else (ctypes.c_int * 4)(*bc)
+    args = (ctypes.byref(cp), qi.ctypes.data_as(dp), qe.ctypes.data_as(dp),
+            auxf.ctypes.data_as(dp), auxf.shape[0], mx, my, ctypes.c_double(dx),
+            ctypes.c_double(dy), ctypes.c_double(dt), seg)
+    if group is None:
+        rc = lib.emu_stepgrid_split(*args, bca, ctypes.byref(cfl), ctypes.byref(n))
+    else:
+        rc = lib.emu_stepgrid_classified(*args, group, bca, ctypes.byref(cfl), ctypes.byref(n))
+    assert rc == 0, f"emulator returned {rc}"
+    return qe, cfl.value, n.value
+
+
+@pytest.mark.parametrize("mx,my,seg,group,bc", [
+    (130, 70, 8, 4, (3, 3, 3, 3)),
+    (59, 33, 4, 5, (1, 3, 3, 1)),
+    (116, 40, 16, 3, (1, 1, 1, 1)),
+    (7, 90, 3, 11, (3, 1, 1, 3)),
+    (1, 1, 32, 1, (1, 1, 1, 1)),
+    (60, 200, 6, 8, None),
+    (180, 64, 1, 32, (1, 1, 3, 3)),
+])
+def test_dry_pass_bitwise(oracle_mod, emu_lib, mx, my, seg, group, bc):
+    """The split step with the dry pass (classify, strip body over the listed
+    segments, src2 over the flagged ones) equals the single-launch split step
+    on every cell, ghosts included, and the oracle's step on the interior."""
+    O = oracle_mod
+    pt = slope_patch(mx, my, slope=0.8, phi_deg=25.0, seed=mx + 3 * my)
+    q, aux, dt = advance_oracle(O, pt, 6, bc=bc or (1, 1, 1, 1))
+    qs, cfls, _ = _emu_split(emu_lib, pt.params, q, aux, pt.dx, pt.dy, dt, seg, bc)
+    qc, cflc, nwet = _emu_split(emu_lib, pt.params, q, aux, pt.dx, pt.dy, dt, seg, bc, group)
+    assert not np.isnan(qc).any(), f"{np.isnan(qc).sum()} values not written"
+    assert np.array_equal(qc, qs), f"{(qc != qs).sum()} values differ from the split step"
+    assert cflc == cfls
+    qo, _, cflo = O.stepgrid(O.params_from(pt.params), q, aux, pt.dx, pt.dy, dt)
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    assert np.array_equal(qc[I], qo[I])
+    assert cflc == cflo
+    ntiles = -(-mx // 58) * -(-my // seg)
+    assert 0 < nwet <= ntiles
+    if mx * my >= 60 * 200:
+        assert nwet < ntiles          # some segments take the dry pass
+
+
+@pytest.mark.parametrize("bc,group", [(None, 1), ((1, 1, 1, 1), 2), ((3, 3, 3, 3), 5),
+                                      ((1, 3, 1, 3), 3)])
+def test_dry_pass_edges_of_wet_blocks(oracle_mod, emu_lib, bc, group):
+    """Wet blocks two rows below a segment, two columns left of a strip and in
+    the patch corners, over several steps: the dry pass must list exactly the
+    segments whose 1-cell neighbourhood holds a wet cell (a missed one shows
+    as a difference, a spurious one costs only time)."""
+    O = oracle_mod
+    mx, my, seg = 130, 40, 8
+    pt = slope_patch(mx, my, slope=0.6, phi_deg=25.0, seed=1, h0=0.0)
+    for (i0, i1, j0, j1) in [(50, 57, 18, 19), (1, 3, 1, 2), (128, 130, 39, 40),
+                             (100, 101, 25, 25)]:
+        sl = (slice(i0 + 1, i1 + 2), slice(j0 + 1, j1 + 2))
+        pt.q[0][sl] = 2.0
+        pt.q[3][sl] = 1.2
+        pt.q[4][sl] = 0.8 * 1000 * 9.81 * 2.0
+    q = O.bc_fill(pt.q, bc or (1, 1, 1, 1))
+    aux = pt.aux
+    dt = 0.01
+    for step in range(4):
+        qs, cfls, _ = _emu_split(emu_lib, pt.params, q, aux, pt.dx, pt.dy, dt, seg, bc)
+        qc, cflc, nwet = _emu_split(emu_lib, pt.params, q, aux, pt.dx, pt.dy, dt, seg, bc, group)
+        assert not np.isnan(qc).any()
+        assert np.array_equal(qc, qs), f"step {step}: {(qc != qs).sum()} values differ"
+        assert cflc == cfls
+        assert 0 < nwet < 3 * 5
+        q = O.bc_fill(qc, bc or (1, 1, 1, 1))
+
+
+def _offset_like(a, off):
+    """A Fortran-ordered copy of `a` whose data starts `off` doubles into a
+    16-byte aligned buffer (off = 1: 8-byte aligned only)."""
+    buf = np.zeros(a.size + 2, dtype=np.float64)
+    shift = (buf.ctypes.data // 8) % 2          # make buf + shift 16-byte aligned
+    v = buf[shift + off:shift + off + a.size].reshape(a.shape, order="F")
+    v[...] = a
+    assert (v.ctypes.data % 16 == 0) == (off == 0)
+    return v
+
+
+@pytest.mark.parametrize("in_off,out_off", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_dry_scan_alignment(oracle_mod, emu_lib, in_off, out_off):
+    """The dry paths read and write rows with 16-byte accesses aligned to
+    q_in; q_in or q_out only 8-byte aligned (and of different parity) must
+    give the same result, fused and with the dry pass."""
+    from dclaw_amd.params import to_c
+    O = oracle_mod
+    dp = ctypes.POINTER(ctypes.c_double)
+    mx, my, seg = 130, 40, 8
+    pt = slope_patch(mx, my, slope=0.6, phi_deg=25.0, seed=1, h0=0.0)
+    for (i0, i1, j0, j1) in [(50, 57, 18, 19), (1, 3, 1, 2), (128, 130, 39, 40)]:
+        sl = (slice(i0 + 1, i1 + 2), slice(j0 + 1, j1 + 2))
+        pt.q[0][sl] = 2.0
+        pt.q[3][sl] = 1.2
+        pt.q[4][sl] = 0.8 * 1000 * 9.81 * 2.0
+    pt.q[6] = np.random.default_rng(5).uniform(-0.5, 0.5, pt.q[6].shape)   # bdif of both signs
+    q = O.bc_fill(pt.q, (1, 1, 1, 1))
+    aux = np.asfortranarray(pt.aux)
+    dt = 0.01
+    cp = to_c(pt.params)
+    for bc in (None, (3, 1, 1, 3)):
+        bca = None if bc is None else (ctypes.c_int * 4)(*bc)
+        ref = None
+        for group in (None, 3):
+            qi = _offset_like(np.asfortranarray(q), in_off)
+            qe = _offset_like(np.full(q.shape, np.nan, order="F"), out_off)
+            cfl, n = ctypes.c_double(0.0), ctypes.c_int(0)
+            args = (ctypes.byref(cp), qi.ctypes.data_as(dp), qe.ctypes.data_as(dp),
+                    aux.ctypes.data_as(dp), aux.shape[0], mx, my, ctypes.c_double(pt.dx),
+                    ctypes.c_double(pt.dy), ctypes.c_double(dt), seg)
+            if group is None:
+                rc = emu_lib.emu_stepgrid_split(*args, bca, ctypes.byref(cfl), ctypes.byref(n))
+            else:
+                rc = emu_lib.emu_stepgrid_classified(*args, group, bca, ctypes.byref(cfl),
+                                                     ctypes.byref(n))
+            assert rc == 0
+            assert not np.isnan(qe).any()
+            qa, _, _ = _emu_split(emu_lib, pt.params, q, aux, pt.dx, pt.dy, dt, seg, bc, group)
+            assert np.array_equal(np.asarray(qe), qa), f"bc {bc} group {group}"
+            if ref is None:
+                ref = qa
+            assert np.array_equal(qa, ref)

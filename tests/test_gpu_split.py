@@ -138,3 +138,110 @@ def test_entrainment_in_a_batch(dev, oracle_mod):
                                0.9, 1e9)
         I = (slice(None), slice(2, -2), slice(2, -2))
         assert np.array_equal(ens.q(k)[I], qo[I]), k
+
+
+@pytest.mark.parametrize("mx,my,seg,dry_rows,bc", [
+    (130, 70, 8, 32, (3, 3, 3, 3)), (59, 33, 4, 20, (1, 3, 3, 1)),
+    (200, 150, 16, 48, (1, 1, 1, 1)), (7, 90, 32, 48, None),
+    (600, 500, 6, 48, (1, 1, 1, 1)), (300, 300, 1, 200, (1, 3, 1, 3)),
+])
+def test_dry_pass_equals_fused(dev, oracle_mod, mx, my, seg, dry_rows, bc):
+    """The split step with its dry pass (classify kernel, strip kernel over the
+    wet list, src2 over the list) equals the fused step on every cell, ghosts
+    included; q_out starts as NaN so an unwritten cell shows."""
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    O = oracle_mod
+    pt = slope_patch(mx, my, slope=0.8, phi_deg=25.0, seed=mx + my)
+    q, aux, dt = advance_oracle(O, pt, 6, bc=bc or (1, 1, 1, 1))
+    print(f"[dry pass] {mx}x{my} seg {seg} dry rows {dry_rows} bc {bc}", flush=True)
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(seg)
+    prev_dry = L.dclaw_set_dry_rows(dry_rows)
+    try:
+        ntiles = L.dclaw_tile_count(mx, my)
+        nws = L.dclaw_split_workspace_size(mx, my)
+        assert nws == 2 * ntiles + 2
+        pdev = S.params_tensor([pt.params])
+        dtd = dev.tensor([dt], dtype=dev.float64, device="cuda")
+        bed = S.bed_tensor(aux)
+        qi = S.to_device_q(q)
+        qf = dev.zeros_like(qi)
+        qs = dev.full_like(qi, float("nan"))
+        cf = dev.zeros(1, dtype=dev.float64, device="cuda")
+        cs = dev.zeros(1, dtype=dev.float64, device="cuda")
+        if bc is None:
+            S.stepgrid(qi, qf, bed, 1, mx, my, pt.dx, pt.dy, dtd, pdev, cf)
+        else:
+            S.stepgrid_bc(qi, qf, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cf)
+        ws = dev.full((nws,), -1, dtype=dev.int32, device="cuda")
+        S.stepgrid_split(qi, qs, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cs, ws)
+        dev.cuda.synchronize()
+    finally:
+        L.dclaw_set_seg_rows(prev)
+        L.dclaw_set_dry_rows(prev_dry)
+    a, b = S.to_host_q(qs), S.to_host_q(qf)
+    assert not np.isnan(a).any(), f"{np.isnan(a).sum()} values not written"
+    assert np.array_equal(a, b), f"{(a != b).sum()} values differ"
+    assert cs.item() == cf.item()
+    w = ws.cpu().numpy()
+    nwet = int(w[2 * ntiles])
+    assert 0 < nwet <= ntiles
+    assert w[2 * ntiles + 1] >= nwet                      # the work cursor passed the list
+    assert set(w[:ntiles].tolist()) <= {0, 1}
+    listed = w[ntiles:ntiles + nwet]
+    assert len(set(listed.tolist())) == nwet and listed.min() >= 0 and listed.max() < ntiles
+    # unlisted tiles are dry: flag 0
+    unl = np.setdiff1d(np.arange(ntiles), listed)
+    assert (w[unl] == 0).all()
+    if mx * my >= 600 * 500:
+        assert nwet < ntiles          # the mound's far tail leaves some tiles dry
+
+
+def test_dry_pass_runner_long(dev):
+    """150 steps of the runner on a 700 x 520 fractal patch with 64 sources,
+    with and without the dry pass: identical states, dt and wet lists that
+    grow as the flow spreads."""
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    pt = W.fractal_sources(700, 520, dx=2.0, seed=3)
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(8)
+    try:
+        ra = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, dry_pass=True)
+        rb = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, dry_pass=False)
+        ra.run(150, 1e-3, 0.9, 1e9)
+        rb.run(150, 1e-3, 0.9, 1e9)
+        dev.cuda.synchronize()
+        ntiles = L.dclaw_tile_count(700, 520)
+        nwet = int(ra.flags[2 * ntiles].item())
+    finally:
+        L.dclaw_set_seg_rows(prev)
+    qa, qb = ra.q(), rb.q()
+    assert np.array_equal(qa, qb), f"{(qa != qb).sum()} values differ"
+    assert ra.dt.item() == rb.dt.item()
+    assert 0 < nwet < ntiles
+
+
+def test_tuner_restores_state(dev):
+    """The autotuner times (height, dry pass) candidates on copies of the
+    runner's state and leaves state, dt and the library's height as chosen."""
+    from dclaw_amd import _lib, autotune
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    pt = W.fractal_sources(300, 200, dx=2.0, seed=4, nsrc=8)
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(32)
+    try:
+        r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+        r.set_dt(1e-3)
+        r.step()
+        q0, dt0 = r.q(), r.dt.item()
+        h, times = autotune.tune_patch(r, candidates=(4, 8), reps=2)
+        assert set(times) == {(4, True), (8, True), (4, False), (8, False)}
+        assert (h, r.dry_pass) == min(times, key=times.get)
+        assert L.dclaw_set_seg_rows(h) == h
+        assert np.array_equal(r.q(), q0) and r.dt.item() == dt0
+    finally:
+        L.dclaw_set_seg_rows(prev)
