@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s fwd+adjoint FP64, batched vs plain GPU, at 1e-9 rel vs Fortran"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_CELL_FWD = 8 * (7 + 1 + 7)
 
 
@@ -44,9 +45,11 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--workload", choices=["patch", "ensemble"], default="patch",
+    ap.add_argument("--workload", choices=["patch", "ensemble", "wet"], default="patch",
                     help="patch: one 4096^2 patch (default); ensemble: 256 members with sides "
-                         "log-uniform in [64, 512], batched launch vs plain per-member loop")
+                         "log-uniform in [64, 512], batched launch vs plain per-member loop; "
+                         "wet: a fully wet --wet-n^2 inclined plane (the FP64-bound regime)")
+    ap.add_argument("--wet-n", type=int, default=2048, help="side of the wet workload")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--seg-rows", type=int, default=0,
                     help="rows per segment tile; 0 = pick with the autotuner (untimed)")
@@ -83,9 +86,10 @@ def main():
     _lib.lib()
     if args.workload == "ensemble":
         return bench_ensemble(args)
-    n = args.n
+    wet_run = args.workload == "wet"
+    n = args.wet_n if wet_run else args.n
     t0 = time.time()
-    pt = W.fractal_sources(n, n, dx=2.0, seed=args.seed)
+    pt = W.wet_plane(n) if wet_run else W.fractal_sources(n, n, dx=2.0, seed=args.seed)
     print(f"[bench] workload {n}x{n} built in {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
 
@@ -158,11 +162,29 @@ def main():
     # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a 1 GiB copy)
     traffic = None
     mode = "fused" if args.fused else ("dry_pass" if r.dry_pass else "split_step")
-    tpath = os.path.join(ROOT, "profiles", f"{mode}_fractal{n}_traffic.txt")
+    tpath = os.path.join(ROOT, "profiles",
+                         f"{mode}_{'wet' if wet_run else 'fractal'}{n}_traffic.txt")
     if os.path.exists(tpath):
         for line in open(tpath):
             if line.startswith("traffic_bytes_per_step"):
                 traffic = float(line.split("=")[1])
+
+    # FP64 VALU work of one step from the committed counter summary
+    # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, FMA counted twice)
+    fp64 = None
+    fpath = os.path.join(ROOT, "profiles", f"{mode}_{'wet' if wet_run else 'fractal'}{n}_fp64.txt")
+    if os.path.exists(fpath):
+        for line in open(fpath):
+            if line.startswith("fp64_ops_per_step"):
+                ops = float(line.split("=")[1])
+                rate = ops / kern_avg_s / 1e12
+                fp64 = {"ops_per_step": ops, "achieved": rate, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": rate / FP64_PEAK_TFLOPS}
+
+    # cells whose state can change this step: within 3 cells of a wet cell
+    hw = torch.from_numpy(np.ascontiguousarray(q[0, 2:-2, 2:-2] > pt.params.dry_tolerance))
+    act = torch.nn.functional.max_pool2d(hw.double()[None, None], 7, stride=1, padding=3)
+    active = int((act > 0).sum())
 
     cpu = None
     if args.cpu_steps > 0:
@@ -189,11 +211,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"plain drop-in GPU step, single {n}x{n} patch, dx=2 m, "
-                               f"seeded fractal bed + 20 deg tilt, 64 elliptical sources, "
-                               f"forward only (adjoint not built)",
+        "config": {"workload": (f"plain drop-in GPU step, fully wet {n}x{n} plane, dx=2 m, "
+                                f"20 deg tilt, uniform 2 m layer, forward only" if wet_run else
+                                f"plain drop-in GPU step, single {n}x{n} patch, dx=2 m, "
+                                f"seeded fractal bed + 20 deg tilt, 64 elliptical sources, "
+                                f"forward only (adjoint not built)"),
                    "mx": n, "my": n, "seed": args.seed, "cfl_desired": cfl_desired,
-                   "wet_cells_end": wet, "seg_rows": seg,
+                   "wet_cells_end": wet, "active_cells_end": active, "seg_rows": seg,
                    "src2": "fused" if args.fused else "split",
                    "dry_pass": bool(r.split_src2 and r.dry_pass)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -203,6 +227,7 @@ def main():
                                 else "k_zero_ints + k_classify + k_stepgrid_list<1> + "
                                      "k_src2_list (one step)"),
                      "kernel_avg_ms": kern_avg_s * 1e3},
+        "fp64": fp64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -24,6 +24,8 @@ LIB_PATH = os.environ.get("DCLAW_HIP_LIB", os.path.join(HERE, "_build", "libdcla
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
 
 
 class DClawError(RuntimeError):
@@ -62,6 +64,10 @@ SIGNATURES = {
     "dclaw_tile_count": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dclaw_split_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dclaw_set_dry_rows": (ctypes.c_int, [ctypes.c_int]),
+    "dclaw_amr_average_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i,
+                                             _i, _i, _i, _i, _i, _d, _vp]),
+    "dclaw_amr_interp_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _i,
+                                            _i, _d, _d, _d, _d, _vp]),
     "dclaw_stepgrid_split_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 _ip, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "dclaw_hip.h"
+#include "dclaw_amr.h"
 #include "dclaw_strip.h"
 
 using namespace dclaw;
@@ -564,6 +565,40 @@ __global__ void k_dt_update_out(double* dt, double* cfl, double* t, const double
 // containing cell's values (the flume port's convention,
 // gradient_particle_filter/flume_solver.py:91-99, in two dimensions).  A
 // gauge whose cells are not all interior gets NaN.  xy: (x, y) per gauge.
+// Fine -> coarse average (update.f90:112-203): one thread per coarse cell of
+// the ni x nj block from (ci0, cj0).
+__global__ void k_amr_average(double* qc, const double* __restrict__ auxc, int mauxc, int mitc,
+                              const double* __restrict__ qf, const double* __restrict__ auxf,
+                              int mauxf, int mitf, int ci0, int cj0, int ni, int nj, int fi0,
+                              int fj0, int rx, int ry, double dry_tolerance)
+{
+    const long n = (long)ni * nj;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int a = (int)(t % ni), b = (int)(t / ni);
+        amr_average_cell(qc, auxc, mauxc, mitc, qf, auxf, mauxf, mitf, ci0 + a, cj0 + b,
+                         fi0 + a * rx, fj0 + b * ry, rx, ry, dry_tolerance);
+    }
+}
+
+// Coarse -> fine interpolation (filval.f90:190-381): one thread per interior
+// cell of the enlarged coarse patch.
+__global__ void k_amr_interp(const double* __restrict__ valc, const double* __restrict__ auxc,
+                             int mauxc, int mic, int mjc, double* val,
+                             const double* __restrict__ auxf, int mauxf, int mitot,
+                             const unsigned char* __restrict__ needs_set, int rx, int ry,
+                             double dry_tolerance, double sea_level, double rho_f, double grav)
+{
+    const int ni = mic - 2;
+    const long n = (long)ni * (mjc - 2);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(t % ni), j = 1 + (int)(t / ni);
+        amr_interp_cell(valc, auxc, mauxc, mic, val, auxf, mauxf, mitot, needs_set, rx, ry, i, j,
+                        dry_tolerance, sea_level, rho_f, grav);
+    }
+}
+
 __global__ void k_gauges(const double* __restrict__ q, const double* __restrict__ aux, int maux,
                          int mx, int my, double xlower, double ylower, double dx, double dy,
                          double drytol, const double* __restrict__ xy, int ng, double* out)
@@ -995,6 +1030,48 @@ int dclaw_gauges_dev(const double* q, const double* aux, int maux, int mx, int m
     hipLaunchKernelGGL(k_gauges, dim3((ngauges + 63) / 64), dim3(64), 0, (hipStream_t)stream, q,
                        aux, maux, mx, my, xlower, ylower, dx, dy, dry_tolerance, xy_dev, ngauges,
                        out_dev);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_amr_average_dev(double* qc, const double* auxc, int mauxc, int mitc, int mjtc,
+                          const double* qf, const double* auxf, int mauxf, int mitf, int mjtf,
+                          int ci0, int cj0, int ni, int nj, int fi0, int fj0, int rx, int ry,
+                          double dry_tolerance, void* stream)
+{
+    if (!qc || !auxc || !qf || !auxf) return fail(DCLAW_EINVAL, "null pointer");
+    if (mauxc < 1 || mauxf < 1 || rx < 1 || ry < 1 || ni < 0 || nj < 0)
+        return fail(DCLAW_EINVAL, "bad AMR average sizes");
+    if (ci0 < 0 || cj0 < 0 || (long)ci0 + ni > mitc || (long)cj0 + nj > mjtc ||
+        fi0 < 0 || fj0 < 0 || (long)fi0 + (long)ni * rx > mitf || (long)fj0 + (long)nj * ry > mjtf)
+        return fail(DCLAW_EINVAL, "AMR average block outside the patch arrays");
+    if (!(dry_tolerance >= 0.0)) return fail(DCLAW_EINVAL, "dry_tolerance must be >= 0");
+    const long n = (long)ni * nj;
+    if (n == 0) return DCLAW_OK;
+    const int blocks = (int)std::min<long>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_amr_average, dim3(blocks), dim3(256), 0, (hipStream_t)stream, qc, auxc,
+                       mauxc, mitc, qf, auxf, mauxf, mitf, ci0, cj0, ni, nj, fi0, fj0, rx, ry,
+                       dry_tolerance);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_amr_interp_dev(const double* valc, const double* auxc, int mauxc, int mic, int mjc,
+                         double* val, const double* auxf, int mauxf, int mitot, int mjtot,
+                         const unsigned char* needs_set, int rx, int ry, double dry_tolerance,
+                         double sea_level, double rho_f, double grav, void* stream)
+{
+    if (!valc || !auxc || !val || !auxf || !needs_set) return fail(DCLAW_EINVAL, "null pointer");
+    if (mauxc < 1 || mauxf < 1 || rx < 1 || ry < 1 || mic < 3 || mjc < 3)
+        return fail(DCLAW_EINVAL, "bad AMR interpolation sizes");
+    if ((long)(mic - 2) * rx + 4 > mitot || (long)(mjc - 2) * ry + 4 > mjtot)
+        return fail(DCLAW_EINVAL, "fine patch smaller than the coarse patch's interior blocks");
+    if (!(dry_tolerance >= 0.0)) return fail(DCLAW_EINVAL, "dry_tolerance must be >= 0");
+    const long n = (long)(mic - 2) * (mjc - 2);
+    const int blocks = (int)std::min<long>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_amr_interp, dim3(blocks), dim3(256), 0, (hipStream_t)stream, valc, auxc,
+                       mauxc, mic, mjc, val, auxf, mauxf, mitot, needs_set, rx, ry, dry_tolerance,
+                       sea_level, rho_f, grav);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

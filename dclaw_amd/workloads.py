@@ -105,6 +105,17 @@ def inclined_plane_mound(n: int = 200, dx: float = 1.0) -> Patch:
     return make_patch(b, h, 0.62, 0.8, p, dx, dx)
 
 
+def wet_plane(n: int = 2048, dx: float = 2.0, slope_deg: float = 20.0, h: float = 2.0) -> Patch:
+    """Fully wet n x n patch: a plane inclined slope_deg in x under a uniform
+    layer of depth h (m = 0.6, pb at 0.8 of hydrostatic), starting at rest.
+    Every cell is wet, so every edge runs the full Riemann solver: the
+    FP64-bound regime of the step."""
+    p = config0_params()
+    x = (np.arange(n) + 0.5) * dx
+    b = -math.tan(math.radians(slope_deg)) * x[:, None] + np.zeros((n, n))
+    return make_patch(b, np.full((n, n), h), 0.6, 0.8, p, dx, dx)
+
+
 def fractal_surface(mx: int, my: int, rng: np.random.Generator, relief: float = 200.0,
                     slope: float = -2.0) -> np.ndarray:
     """Isotropic random surface with power spectrum ~ k^slope, rescaled to the

@@ -190,6 +190,41 @@ int dclaw_gauges_dev(const double* q, const double* aux, int maux, int mx, int m
                      double ylower, double dx, double dy, double dry_tolerance,
                      const double* xy_dev, int ngauges, double* out_dev, void* stream);
 
+/* AMR transfers between refinement levels, on device memory (the numerical
+ * part of the AMR driver's coarse/fine coupling; coordinate_system 1).
+ * Arrays: q(7, nx, ny), aux(maux, nx, ny) (aux component 1 = bed), indices
+ * 0-based over the whole array including ghost cells.
+ *
+ * Fine -> coarse (replaces the averaging loop of update.f90:112-203): coarse
+ * cells (ci0 + a, cj0 + b), 0 <= a < ni, 0 <= b < nj, of qc (mitc x mjtc) are
+ * overwritten by the average of the rx x ry fine cells of qf (mitf x mjtf)
+ * from (fi0 + a*rx, fj0 + b*ry): depth from the mean surface of the wet fine
+ * cells over the coarse bed, capped by their mean depth; the other
+ * components from the wet fine cells, scaled by the new depth over the total
+ * fine depth.  The flux-register correction (upbnd) is not part of it. */
+int dclaw_amr_average_dev(double* qc, const double* auxc, int mauxc, int mitc, int mjtc,
+                          const double* qf, const double* auxf, int mauxf, int mitf, int mjtf,
+                          int ci0, int cj0, int ni, int nj, int fi0, int fj0, int rx, int ry,
+                          double dry_tolerance, void* stream);
+
+/* Coarse -> fine (replaces the interpolation loop of filval.f90:190-381):
+ * valc/auxc is the coarse patch enlarged by one coarse cell all round
+ * (mic x mjc, already filled, boundary conditions applied); each interior
+ * coarse cell (1..mic-2, 1..mjc-2) fills its rx x ry block of val
+ * (mitot x mjtot with 2 ghost cells: block of coarse cell (i, j) starts at
+ * ((i-1)*rx + 2, (j-1)*ry + 2)).  needs_set (mitot x mjtot bytes, nonzero =
+ * the reference's NEEDS_TO_BE_SET) selects the fine cells whose depth is
+ * interpolated (surface by minmod-limited slopes, sea_level for dry coarse
+ * cells, minus the fine bed aux(1) of auxf, clipped at 0); the other
+ * components are then set over the whole block, as the reference does:
+ * limited-linear if no fine value / depth leaves the range of the coarse cell
+ * and its wet neighbours, else spread in proportion to the fine depth.
+ * use_force_dry and variable_eta_init are not supported (their defaults). */
+int dclaw_amr_interp_dev(const double* valc, const double* auxc, int mauxc, int mic, int mjc,
+                         double* val, const double* auxf, int mauxf, int mitot, int mjtot,
+                         const unsigned char* needs_set, int rx, int ry, double dry_tolerance,
+                         double sea_level, double rho_f, double grav, void* stream);
+
 /* ---- Per-stage device entry points (GeoClaw stepgrid's stages one at a
  * time, on device memory; the fused dclaw_stepgrid_dev equals b4step2, step2,
  * the conservative update and src2 in sequence). ---- */

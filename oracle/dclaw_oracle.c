@@ -1367,3 +1367,204 @@ void oracle_riemann_solver(const oracle_params* P, int ixy, const double* x, int
                                x[18], x[19], sw3, fw18, wallprob, x[20], x[21], x[22], x[23],
                                x[24], x[25]);
 }
+
+/* ------------------------------------------------------------------------
+ * AMR transfers between a coarse patch and a fine patch (coordinate_system 1,
+ * mcapa = 0: capa = capac = 1).  Arrays are (7, ni, nj) / (maux, ni, nj),
+ * component fastest, indices 0-based over the whole array including ghosts.
+ * ------------------------------------------------------------------------ */
+
+/* update.f90:112-203 (fine -> coarse): coarse cells (ci0 + a, cj0 + b),
+ * a < ni, b < nj, each the average of the rx x ry fine cells from
+ * (fi0 + a rx, fj0 + b ry): depth from the mean surface of the wet fine
+ * cells (nwet of them) capped by their mean depth, momenta (components 2..7)
+ * of the wet fine cells scaled by hc / hsum, where hsum sums the depth of
+ * every fine cell, dry ones included (update.f90:160-164). */
+void oracle_amr_average(double* qc, const double* auxc, int mauxc, int mitc, const double* qf,
+                        const double* auxf, int mauxf, int mitf, int ci0, int cj0, int ni, int nj,
+                        int fi0, int fj0, int rx, int ry, double dry_tolerance)
+{
+    for (int a = 0; a < ni; a++) {          /* update.f90:124-125: i outer, j inner */
+        for (int b = 0; b < nj; b++) {
+            const int ic = ci0 + a, jc = cj0 + b;
+            const int if0 = fi0 + a * rx, jf0 = fj0 + b * ry;
+            const double capac = 1.0;
+            const double bc = auxc[(size_t)mauxc * ((size_t)ic + (size_t)mitc * jc)];
+            double etasum = 0.0, hsum = 0.0, husum[7], huc[7];
+            for (int m = 1; m < 7; m++) husum[m] = 0.0;
+            int nwet = 0;
+            for (int jco = 0; jco < ry; jco++) {
+                for (int ico = 0; ico < rx; ico++) {
+                    const size_t cf = (size_t)(if0 + ico) + (size_t)mitf * (jf0 + jco);
+                    const double* f = qf + 7 * cf;
+                    const double capa = 1.0;
+                    const double hf = f[0] * capa;
+                    const double bf = auxf[(size_t)mauxf * cf] * capa;
+                    double huf[7], etaf;
+                    for (int m = 1; m < 7; m++) huf[m] = f[m] * capa;
+                    if (f[0] > dry_tolerance) {
+                        etaf = hf + bf;
+                        nwet = nwet + 1;
+                    } else {
+                        etaf = 0.0;
+                        for (int m = 1; m < 7; m++) huf[m] = 0.0;
+                    }
+                    hsum = hsum + hf;
+                    for (int m = 1; m < 7; m++) husum[m] = husum[m] + huf[m];
+                    etasum = etasum + etaf;
+                }
+            }
+            double hc;
+            if (nwet > 0) {
+                const double etaav = etasum / (double)nwet;
+                const double hav = hsum / (double)nwet;
+                hc = dmin(hav, dmax(etaav - bc * capac, 0.0));
+                const double fac = dmin(hav, hc) / hsum;
+                for (int m = 1; m < 7; m++) huc[m] = fac * husum[m];
+            } else {
+                hc = 0.0;
+                for (int m = 1; m < 7; m++) huc[m] = 0.0;
+            }
+            double* c = qc + 7 * ((size_t)ic + (size_t)mitc * jc);
+            c[0] = hc / capac;
+            for (int m = 1; m < 7; m++) c[m] = huc[m] / capac;
+        }
+    }
+}
+
+/* filval.f90:190-381 (coarse -> fine, the interpolation loop): valc/auxc is
+ * the coarse patch enlarged by one coarse cell all round (mic x mjc, filled
+ * beforehand); coarse cell (i, j), 1 <= i <= mic-2, 1 <= j <= mjc-2, fills
+ * the fine cells ((i-1) rx + nghost + ico, (j-1) ry + nghost + jco),
+ * 0 <= ico < rx, 0 <= jco < ry, of val (mitot x mjtot, nghost = 2).  Depth:
+ * the surface (h + b, or sea_level for a dry coarse cell) limited-linear
+ * (minmod) interpolated, minus the fine bed, clipped at 0, only in fine cells
+ * with needs_set != 0 (the reference's NEEDS_TO_BE_SET flag).  Components
+ * 2..7: limited-linear interpolation kept when every fine "velocity"
+ * hvf / h stays within the range of the coarse cell and its wet neighbours,
+ * else (or if a set fine cell came out dry) the coarse value spread in
+ * proportion to the fine depth, clamped to that range for components 4..7
+ * (filval.f90:289-381); this part runs over every fine cell of the block
+ * whether or not it was set.  use_force_dry and variable_eta_init are off. */
+void oracle_amr_interp(const double* valc, const double* auxc, int mauxc, int mic, int mjc,
+                       double* val, const double* auxf, int mauxf, int mitot,
+                       const unsigned char* needs_set, int rx, int ry, double dry_tolerance,
+                       double sea_level, double rho_f, double grav)
+{
+    const int nghost = 2;
+#define VC(m, i, j) valc[(m) + 7 * ((size_t)(i) + (size_t)mic * (j))]
+#define BCX(i, j) auxc[(size_t)mauxc * ((size_t)(i) + (size_t)mic * (j))]
+#define VF(m, i, j) val[(m) + 7 * ((size_t)(i) + (size_t)mitot * (j))]
+    for (int j = 1; j <= mjc - 2; j++) {        /* filval.f90:190-191: j outer, i inner */
+        for (int i = 1; i <= mic - 2; i++) {
+            int fineflag1 = 0;
+            double cv[3];
+            for (int ii = -1; ii <= 1; ii++) {
+                cv[1 + ii] = VC(0, i + ii, j) + BCX(i + ii, j);
+                if (VC(0, i + ii, j) <= dry_tolerance) cv[1 + ii] = sea_level;
+            }
+            double s1p = cv[2] - cv[1], s1m = cv[1] - cv[0];
+            double slopex = dmin(fabs(s1p), fabs(s1m)) * dsign(1.0, cv[2] - cv[0]);
+            if (s1m * s1p <= 0.0) slopex = 0.0;
+            for (int jj = -1; jj <= 1; jj++) {
+                cv[1 + jj] = VC(0, i, j + jj) + BCX(i, j + jj);
+                if (VC(0, i, j + jj) <= dry_tolerance) cv[1 + jj] = sea_level;
+            }
+            s1p = cv[2] - cv[1];
+            s1m = cv[1] - cv[0];
+            double slopey = dmin(fabs(s1p), fabs(s1m)) * dsign(1.0, cv[2] - cv[0]);
+            if (s1m * s1p <= 0.0) slopey = 0.0;
+            double finemass = 0.0;
+            for (int jco = 1; jco <= ry; jco++) {
+                for (int ico = 1; ico <= rx; ico++) {
+                    const double yoff = ((double)jco - 0.5) / (double)ry - 0.5;
+                    const double xoff = ((double)ico - 0.5) / (double)rx - 0.5;
+                    const int jf = (j - 1) * ry + nghost + jco - 1;
+                    const int ifn = (i - 1) * rx + nghost + ico - 1;
+                    if (!needs_set[(size_t)ifn + (size_t)mitot * jf]) continue;
+                    double v = (cv[1] + xoff * slopex) + yoff * slopey;
+                    v = dmax(0.0, v - auxf[(size_t)mauxf * ((size_t)ifn + (size_t)mitot * jf)]);
+                    VF(0, ifn, jf) = v;
+                    finemass = finemass + v;
+                    if (v <= dry_tolerance) {
+                        fineflag1 = 1;
+                        for (int m = 1; m < 7; m++) VF(m, ifn, jf) = 0.0;
+                    }
+                }
+            }
+            if (!(finemass >= dry_tolerance)) continue;
+            for (int iv = 1; iv < 7; iv++) {       /* Fortran ivar = iv + 1 */
+                int flag = 0;
+                s1p = VC(iv, i + 1, j) - VC(iv, i, j);
+                s1m = VC(iv, i, j) - VC(iv, i - 1, j);
+                slopex = dmin(fabs(s1p), fabs(s1m)) * dsign(1.0, VC(iv, i + 1, j) - VC(iv, i - 1, j));
+                if (s1m * s1p <= 0.0) slopex = 0.0;
+                s1p = VC(iv, i, j + 1) - VC(iv, i, j);
+                s1m = VC(iv, i, j) - VC(iv, i, j - 1);
+                slopey = dmin(fabs(s1p), fabs(s1m)) * dsign(1.0, VC(iv, i, j + 1) - VC(iv, i, j - 1));
+                if (s1m * s1p <= 0.0) slopey = 0.0;
+                double velmax, velmin;
+                if (VC(0, i, j) > dry_tolerance) {
+                    velmax = VC(iv, i, j) / VC(0, i, j);
+                    velmin = VC(iv, i, j) / VC(0, i, j);
+                } else {
+                    velmax = 0.0;
+                    velmin = 0.0;
+                    if (iv + 1 == 4) {
+                        velmax = 0.0;
+                        velmin = 0.0;
+                    } else if (iv + 1 == 5) {
+                        velmax = rho_f * grav;
+                        velmin = rho_f * grav;
+                    }
+                }
+                for (int ii = -1; ii <= 1; ii += 2) {
+                    if (VC(0, i + ii, j) > dry_tolerance) {
+                        const double vel = VC(iv, i + ii, j) / VC(0, i + ii, j);
+                        velmax = dmax(vel, velmax);
+                        velmin = dmin(vel, velmin);
+                    }
+                    if (VC(0, i, j + ii) > dry_tolerance) {
+                        const double vel = VC(iv, i, j + ii) / VC(0, i, j + ii);
+                        velmax = dmax(vel, velmax);
+                        velmin = dmin(vel, velmin);
+                    }
+                }
+                for (int ico = 1; ico <= rx; ico++) {      /* ico outer, jco inner (:330-331) */
+                    if (fineflag1 || flag) break;
+                    for (int jco = 1; jco <= ry; jco++) {
+                        const int jf = (j - 1) * ry + nghost + jco - 1;
+                        const int ifn = (i - 1) * rx + nghost + ico - 1;
+                        const double yoff = ((double)jco - 0.5) / (double)ry - 0.5;
+                        const double xoff = ((double)ico - 0.5) / (double)rx - 0.5;
+                        const double hvf = (VC(iv, i, j) + xoff * slopex) + yoff * slopey;
+                        const double vf = hvf / VF(0, ifn, jf);
+                        if (vf > velmax || vf < velmin) {
+                            flag = 1;
+                            break;
+                        }
+                        VF(iv, ifn, jf) = hvf;
+                    }
+                }
+                if (fineflag1 || flag) {
+                    const double area = (double)(rx * ry);
+                    const double dividemass = dmax(finemass, VC(0, i, j));
+                    double vnew = area * VC(iv, i, j) / dividemass;
+                    if (iv + 1 > 3) {
+                        vnew = dmax(velmin, vnew);
+                        vnew = dmin(velmax, vnew);
+                    }
+                    for (int jco = 1; jco <= ry; jco++)
+                        for (int ico = 1; ico <= rx; ico++) {
+                            const int jf = (j - 1) * ry + nghost + jco - 1;
+                            const int ifn = (i - 1) * rx + nghost + ico - 1;
+                            VF(iv, ifn, jf) = vnew * VF(0, ifn, jf);
+                        }
+                }
+            }
+        }
+    }
+#undef VC
+#undef BCX
+#undef VF
+}

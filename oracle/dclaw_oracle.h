@@ -115,6 +115,22 @@ void oracle_gauges(int mx, int my, int mbc, int maux, const double* q, const dou
                    double xlower, double ylower, double dx, double dy, double dry_tolerance,
                    const double* xg, const double* yg, int ngauges, double* out);
 
+/* AMR transfers (coordinate_system 1, mcapa = 0), arrays 0-based including
+ * ghosts, component fastest.  update.f90:112-203, fine -> coarse: coarse cells
+ * (ci0 + a, cj0 + b), a < ni, b < nj, from the rx x ry fine blocks at
+ * (fi0 + a rx, fj0 + b ry). */
+void oracle_amr_average(double* qc, const double* auxc, int mauxc, int mitc, const double* qf,
+                        const double* auxf, int mauxf, int mitf, int ci0, int cj0, int ni, int nj,
+                        int fi0, int fj0, int rx, int ry, double dry_tolerance);
+
+/* filval.f90:190-381, coarse -> fine: the interpolation loop over the coarse
+ * patch enlarged by one cell (mic x mjc) into the fine patch (mitot columns,
+ * nghost = 2); needs_set[fine cell] != 0 marks the cells NEEDS_TO_BE_SET. */
+void oracle_amr_interp(const double* valc, const double* auxc, int mauxc, int mic, int mjc,
+                       double* val, const double* auxf, int mauxf, int mitot,
+                       const unsigned char* needs_set, int rx, int ry, double dry_tolerance,
+                       double sea_level, double rho_f, double grav);
+
 #ifdef __cplusplus
 }
 #endif

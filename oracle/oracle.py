@@ -129,6 +129,11 @@ def lib():
         L.oracle_dt_rule.argtypes = [ctypes.c_double] * 6 + [_dp, _dp, _ip]
         L.oracle_gauges.argtypes = [ctypes.c_int] * 4 + [_dp, _dp] + [ctypes.c_double] * 5 + [
             _dp, _dp, ctypes.c_int, _dp]
+        _ci = ctypes.c_int
+        L.oracle_amr_average.argtypes = [_dp, _dp, _ci, _ci, _dp, _dp, _ci, _ci] + [_ci] * 8 + [
+            ctypes.c_double]
+        L.oracle_amr_interp.argtypes = [_dp, _dp, _ci, _ci, _ci, _dp, _dp, _ci, _ci,
+                                        ctypes.c_void_p, _ci, _ci] + [ctypes.c_double] * 4
         _libs[_variant] = L
     return _libs[_variant]
 
@@ -326,3 +331,38 @@ def gauges(q, aux, xlower, ylower, dx, dy, dry_tolerance, xg, yg, mbc=2):
     lib().oracle_gauges(mx, my, mbc, aux.shape[0], _ptr(q), _ptr(aux), xlower, ylower, dx, dy,
                         dry_tolerance, _ptr(xg), _ptr(yg), len(xg), _ptr(out))
     return out
+
+
+def amr_average(qc, auxc, qf, auxf, ci0, cj0, ni, nj, fi0, fj0, rx, ry, dry_tolerance):
+    """update.f90's fine -> coarse average (a copy of qc with the covered
+    cells replaced).  qc, qf: (7, nx, ny); auxc, auxf: (maux, nx, ny), all
+    indices 0-based including ghosts."""
+    qc = np.array(qc, dtype=np.float64, order="F", copy=True)
+    qf = np.asfortranarray(qf, dtype=np.float64)
+    auxc = np.asfortranarray(auxc, dtype=np.float64)
+    auxf = np.asfortranarray(auxf, dtype=np.float64)
+    assert ci0 >= 0 and cj0 >= 0 and ci0 + ni <= qc.shape[1] and cj0 + nj <= qc.shape[2]
+    assert fi0 >= 0 and fj0 >= 0 and fi0 + ni * rx <= qf.shape[1] and fj0 + nj * ry <= qf.shape[2]
+    lib().oracle_amr_average(_ptr(qc), _ptr(auxc), auxc.shape[0], qc.shape[1], _ptr(qf),
+                             _ptr(auxf), auxf.shape[0], qf.shape[1], ci0, cj0, ni, nj, fi0, fj0,
+                             rx, ry, dry_tolerance)
+    return qc
+
+
+def amr_interp(valc, auxc, val, auxf, needs_set, rx, ry, dry_tolerance, sea_level, rho_f, grav):
+    """filval.f90's coarse -> fine interpolation loop (a copy of val with the
+    fine cells of the coarse interior blocks updated).  valc: (7, mic, mjc)
+    enlarged coarse patch; val: (7, mitot, mjtot) with 2 ghost cells;
+    needs_set: (mitot, mjtot) bool."""
+    valc = np.asfortranarray(valc, dtype=np.float64)
+    auxc = np.asfortranarray(auxc, dtype=np.float64)
+    val = np.array(val, dtype=np.float64, order="F", copy=True)
+    auxf = np.asfortranarray(auxf, dtype=np.float64)
+    ns = np.asfortranarray(needs_set, dtype=np.uint8)
+    mic, mjc = valc.shape[1], valc.shape[2]
+    assert (mic - 2) * rx + 4 <= val.shape[1] and (mjc - 2) * ry + 4 <= val.shape[2]
+    assert ns.shape == val.shape[1:]
+    lib().oracle_amr_interp(_ptr(valc), _ptr(auxc), auxc.shape[0], mic, mjc, _ptr(val), _ptr(auxf),
+                            auxf.shape[0], val.shape[1], ns.ctypes.data, rx, ry, dry_tolerance,
+                            sea_level, rho_f, grav)
+    return val

@@ -21,7 +21,7 @@ def pytest_configure(config):
 def build_emulator() -> str:
     """Compile the host lane emulator (product kernel body on the CPU)."""
     srcs = [EMU_SRC] + [os.path.join(ROOT, "dclaw_amd", "csrc", f)
-                        for f in ("dclaw_strip.h", "dclaw_physics.h")]
+                        for f in ("dclaw_strip.h", "dclaw_physics.h", "dclaw_amr.h")]
     if os.path.exists(EMU_LIB) and all(os.path.getmtime(EMU_LIB) >= os.path.getmtime(s)
                                        for s in srcs):
         return EMU_LIB

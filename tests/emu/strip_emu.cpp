@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "dclaw_amr.h"
 #include "dclaw_strip.h"
 
 using namespace dclaw;
@@ -407,4 +408,27 @@ extern "C" int emu_dcm(int fn, const double* x, const double* y2, double* out, i
         }
     }
     return 0;
+}
+
+// The AMR transfer bodies (dclaw_amr.h) run cell by cell on the host.
+extern "C" void emu_amr_average(double* qc, const double* auxc, int mauxc, int mitc,
+                                const double* qf, const double* auxf, int mauxf, int mitf, int ci0,
+                                int cj0, int ni, int nj, int fi0, int fj0, int rx, int ry,
+                                double dry_tolerance)
+{
+    for (int b = 0; b < nj; b++)
+        for (int a = 0; a < ni; a++)
+            amr_average_cell(qc, auxc, mauxc, mitc, qf, auxf, mauxf, mitf, ci0 + a, cj0 + b,
+                             fi0 + a * rx, fj0 + b * ry, rx, ry, dry_tolerance);
+}
+
+extern "C" void emu_amr_interp(const double* valc, const double* auxc, int mauxc, int mic,
+                               int mjc, double* val, const double* auxf, int mauxf, int mitot,
+                               const unsigned char* needs_set, int rx, int ry,
+                               double dry_tolerance, double sea_level, double rho_f, double grav)
+{
+    for (int j = 1; j <= mjc - 2; j++)
+        for (int i = 1; i <= mic - 2; i++)
+            amr_interp_cell(valc, auxc, mauxc, mic, val, auxf, mauxf, mitot, needs_set, rx, ry, i,
+                            j, dry_tolerance, sea_level, rho_f, grav);
 }
