@@ -60,6 +60,9 @@ def parse():
                     help="split step as one strip launch over every tile (no separate dry pass)")
     ap.add_argument("--dry-rows", type=int, default=0,
                     help="rows per wavefront of the dry pass (0 = library default)")
+    ap.add_argument("--tangent-steps", type=int, default=20,
+                    help="timed steps of the tangent-linear step (forward + d/dphi) on the same "
+                         "workload, reported beside the forward figures (0 disables)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy (the HBM "
                          "counter calibration dispatch for rocprofv3 --pmc runs)")
@@ -169,6 +172,30 @@ def main():
             if line.startswith("traffic_bytes_per_step"):
                 traffic = float(line.split("=")[1])
 
+    # tangent-linear step (values + d/dphi) on the same workload, from the
+    # initial state, after the forward timing
+    tangent = None
+    if args.tangent_steps > 0 and not args.fused:
+        from dclaw_amd import tangent as T
+        tr = T.TangentRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy,
+                             dparams=T.param_direction(phi=1.0))
+        tr.set_dt(1e-3)
+        for _ in range(2):
+            tr.step()
+        torch.cuda.synchronize()
+        ta, tb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ta.record()
+        for _ in range(args.tangent_steps):
+            tr.step()
+        tb.record()
+        torch.cuda.synchronize()
+        tms = ta.elapsed_time(tb) / args.tangent_steps
+        tangent = {"ms_per_step": tms, "value": cells / (tms * 1e-3), "unit": "cell-updates/s",
+                   "steps": args.tangent_steps, "direction": "d/dphi",
+                   "kernel": "k_stepgrid_tl (fused step with its tangent, forward mode)"}
+        del tr
+        print(f"[bench] tangent-linear step {tms:.3f} ms", file=sys.stderr, flush=True)
+
     # FP64 VALU work of one step from the committed counter summary
     # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, FMA counted twice)
     fp64 = None
@@ -214,8 +241,10 @@ def main():
         "config": {"workload": (f"plain drop-in GPU step, fully wet {n}x{n} plane, dx=2 m, "
                                 f"20 deg tilt, uniform 2 m layer, forward only" if wet_run else
                                 f"plain drop-in GPU step, single {n}x{n} patch, dx=2 m, "
-                                f"seeded fractal bed + 20 deg tilt, 64 elliptical sources, "
-                                f"forward only (adjoint not built)"),
+                                f"seeded fractal bed + 20 deg tilt, 64 elliptical sources; "
+                                f"value = forward step; the tangent-linear step (forward-mode "
+                                f"derivative) is timed separately under 'tangent'; a "
+                                f"reverse-mode adjoint is not built"),
                    "mx": n, "my": n, "seed": args.seed, "cfl_desired": cfl_desired,
                    "wet_cells_end": wet, "active_cells_end": active, "seg_rows": seg,
                    "src2": "fused" if args.fused else "split",
@@ -228,6 +257,7 @@ def main():
                                      "k_src2_list (one step)"),
                      "kernel_avg_ms": kern_avg_s * 1e3},
         "fp64": fp64,
+        "tangent": tangent,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

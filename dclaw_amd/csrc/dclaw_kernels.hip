@@ -25,6 +25,7 @@
 #include "dclaw_hip.h"
 #include "dclaw_amr.h"
 #include "dclaw_strip.h"
+#include "dclaw_tangent.h"
 
 using namespace dclaw;
 
@@ -64,6 +65,15 @@ struct GpuLane {
     }
     __device__ __forceinline__ double prev(double v) const { return dpp<0x138>(v); }
     __device__ __forceinline__ double next(double v) const { return dpp<0x130>(v); }
+    // value and tangent shift together (tangent-linear step)
+    __device__ __forceinline__ dclaw_tl::Dual prev(const dclaw_tl::Dual& v) const
+    {
+        return dclaw_tl::Dual(prev(v.v), prev(v.d));
+    }
+    __device__ __forceinline__ dclaw_tl::Dual next(const dclaw_tl::Dual& v) const
+    {
+        return dclaw_tl::Dual(next(v.v), next(v.d));
+    }
     __device__ __forceinline__ bool any(bool p) const { return __any((int)p) != 0; }
 };
 
@@ -172,6 +182,50 @@ __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restri
     GpuLane L;
     const double cfl = strip_step_dispatch<MODE>(L, P, A);
     wave_max_to(cfl, cfl_dev + pt.member);
+}
+
+// Tangent-linear fused step (one patch, MODE_STEP): q_out and cflgrid equal
+// k_stepgrid<MODE_STEP>'s bit for bit, dq_out = J_q dq_in + J_P dP at the
+// launch's dt.
+__global__ __launch_bounds__(64) void k_stepgrid_tl(const double* __restrict__ q_in,
+                                                    const double* __restrict__ dq_in,
+                                                    double* __restrict__ q_out,
+                                                    double* __restrict__ dq_out,
+                                                    const double* __restrict__ aux, int maux,
+                                                    int mx, int my, double dx, double dy,
+                                                    const double* __restrict__ dt_dev,
+                                                    const dclaw_params* __restrict__ params,
+                                                    const dclaw_params* __restrict__ dparams,
+                                                    double* cfl_dev, int seg_rows, int nstrips,
+                                                    int bx0, int bx1, int by0, int by1)
+{
+    const int strip = (int)blockIdx.x % nstrips, seg = (int)blockIdx.x / nstrips;
+    const dclaw_tl::Params P = dclaw_tl::make_params(*params, *dparams);
+    dclaw_tl::StripArgs A;
+    A.q_in.v = q_in;
+    A.q_in.d = dq_in;
+    A.q_out.v = q_out;
+    A.q_out.d = dq_out;
+    A.aux = aux;
+    A.aux_out = nullptr;
+    A.maux = maux;
+    A.mx = mx;
+    A.my = my;
+    A.dx = dx;
+    A.dy = dy;
+    A.dt = *dt_dev;
+    A.i0 = 1 + strip * STRIP_W;
+    A.jA = 1 + seg * seg_rows;
+    A.jB = min(my, A.jA + seg_rows - 1);
+    A.bc[0] = bx0;
+    A.bc[1] = bx1;
+    A.bc[2] = by0;
+    A.bc[3] = by1;
+    A.fm = A.fp = A.gm = A.gp = nullptr;
+    A.tile_flag = nullptr;
+    GpuLane L;
+    const dclaw_tl::Dual cfl = dclaw_tl::strip_step<MODE_STEP>(L, P, A);
+    wave_max_to(cfl.v, cfl_dev);
 }
 
 // src2 over the interior cells of the tiles the no-src2 step flagged (one
@@ -812,6 +866,31 @@ int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, 
     if (!bc) return fail(DCLAW_EINVAL, "null bc");
     return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
                            cfl_dev, bc, (hipStream_t)stream);
+}
+
+int dclaw_stepgrid_tl_dev(const double* q_in, const double* dq_in, double* q_out, double* dq_out,
+                          const double* aux, int maux, int mx, int my, double dx, double dy,
+                          const int* bc, const double* dt_dev, const dclaw_params* params_dev,
+                          const dclaw_params* dparams_dev, double* cfl_dev, void* stream)
+{
+    if (int rcb = check_bc(bc)) return rcb;
+    if (int rc = check_patch(mx, my, maux)) return rc;
+    if (!q_in || !dq_in || !q_out || !dq_out || !aux || !dt_dev || !params_dev || !dparams_dev ||
+        !cfl_dev)
+        return fail(DCLAW_EINVAL, "null pointer");
+    if (q_in == q_out || dq_in == dq_out || q_out == dq_out || q_in == dq_out || dq_in == q_out)
+        return fail(DCLAW_EINVAL, "outputs must not alias inputs or each other");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int seg_rows = g_seg_rows;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
+    if ((long)nstrips * nseg > 0x7fffffffL) return fail(DCLAW_EINVAL, "patch too large for one launch");
+    const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
+    hipLaunchKernelGGL(k_stepgrid_tl, dim3(nstrips * nseg), dim3(64), 0, (hipStream_t)stream, q_in,
+                       dq_in, q_out, dq_out, aux, maux, mx, my, dx, dy, dt_dev, params_dev,
+                       dparams_dev, cfl_dev, seg_rows, nstrips, b0, b1, b2, b3);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
 }
 
 int dclaw_tile_count(int mx, int my)

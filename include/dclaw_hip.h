@@ -113,6 +113,23 @@ int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, 
                           const double* dt_dev, const dclaw_params* params_dev,
                           double* cfl_dev, void* stream);
 
+/* Tangent-linear (forward-mode) fused step: the step of dclaw_stepgrid_bc_dev
+ * (bc may be NULL: stored ghosts) on q_in together with the tangent dq_in
+ * (same layout), with the parameters' tangent record dparams_dev (a device
+ * dclaw_params whose floating fields hold the derivative of each parameter
+ * along the direction; integer fields ignored).  q_out and cfl_dev equal
+ * dclaw_stepgrid_bc_dev's bit for bit; dq_out = (dq_out/dq_in) dq_in +
+ * (dq_out/dparams) dparams at dt_dev (dt is not differentiated).  At a branch
+ * of the scheme the tangent is the derivative of the branch the values take.
+ * aux (the bed, h_e) carries no tangent; aux(4..6) is not written.  No
+ * reference counterpart: the reference has no derivative of the step (the
+ * gradient-based workflows need it; this is the forward-mode building block,
+ * a reverse-mode adjoint is not provided). */
+int dclaw_stepgrid_tl_dev(const double* q_in, const double* dq_in, double* q_out, double* dq_out,
+                          const double* aux, int maux, int mx, int my, double dx, double dy,
+                          const int* bc, const double* dt_dev, const dclaw_params* params_dev,
+                          const dclaw_params* dparams_dev, double* cfl_dev, void* stream);
+
 /* Tiles (58-column strip x segment) of one patch at the current segment
  * height: the smallest tile_flags workspace of the split step below. */
 int dclaw_tile_count(int mx, int my);

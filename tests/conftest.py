@@ -19,9 +19,13 @@ def pytest_configure(config):
 
 
 def build_emulator() -> str:
-    """Compile the host lane emulator (product kernel body on the CPU)."""
+    """Compile the host lane emulator (product kernel body on the CPU).  It
+    is host code only and links no HIP runtime (--as-needed drops it), so a
+    GPU test process can load it beside torch's runtime."""
     srcs = [EMU_SRC] + [os.path.join(ROOT, "dclaw_amd", "csrc", f)
-                        for f in ("dclaw_strip.h", "dclaw_physics.h", "dclaw_amr.h")]
+                        for f in ("dclaw_strip.h", "dclaw_physics.h", "dclaw_amr.h",
+                                  "dclaw_tangent.h", "dclaw_strip_body.inc",
+                                  "dclaw_physics_body.inc", "dcmath.h")]
     if os.path.exists(EMU_LIB) and all(os.path.getmtime(EMU_LIB) >= os.path.getmtime(s)
                                        for s in srcs):
         return EMU_LIB
@@ -29,7 +33,7 @@ def build_emulator() -> str:
     cmd = ["/opt/rocm/bin/hipcc", "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17",
            "-fPIC", "-ffp-contract=off", "-fno-fast-math",
            "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "dclaw_amd", "csrc"),
-           "-shared", "-o", EMU_LIB, EMU_SRC]
+           "-Wl,--as-needed", "-shared", "-o", EMU_LIB, EMU_SRC]
     subprocess.run(cmd, check=True)
     return EMU_LIB
 

@@ -12,6 +12,7 @@
 
 #include "dclaw_amr.h"
 #include "dclaw_strip.h"
+#include "dclaw_tangent.h"
 
 using namespace dclaw;
 
@@ -32,10 +33,13 @@ struct Sched {
     int mode;
     int seg_rows;                 // MODE_CLASSIFY
     unsigned mask[STRIP_LANES];   // MODE_CLASSIFY result per lane
+    const dclaw_tl::Params* Ptl;  // MODE_TANGENT
+    const dclaw_tl::StripArgs* Atl;
     std::vector<char> stacks;
 };
 
 constexpr int MODE_CLASSIFY = 100;   // strip_classify instead of the strip body
+constexpr int MODE_TANGENT = 101;    // the tangent-linear strip body (fused step)
 
 struct CoLane {
     int lid;
@@ -52,6 +56,16 @@ struct CoLane {
     }
     double prev(double v) { return shfl(v, lid > 0 ? lid - 1 : lid); }
     double next(double v) { return shfl(v, lid < STRIP_LANES - 1 ? lid + 1 : lid); }
+    dclaw_tl::Dual prev(const dclaw_tl::Dual& v)
+    {
+        const double a = prev(v.v);
+        return dclaw_tl::Dual(a, prev(v.d));
+    }
+    dclaw_tl::Dual next(const dclaw_tl::Dual& v)
+    {
+        const double a = next(v.v);
+        return dclaw_tl::Dual(a, next(v.d));
+    }
     bool any(bool p)
     {
         const int b = n & 1;
@@ -70,6 +84,11 @@ void lane_entry(int lid)
 {
     Sched* s = g_sched;
     CoLane L{lid, 0, s};
+    if (s->mode == MODE_TANGENT) {
+        s->cfl[lid] = dclaw_tl::strip_step<MODE_STEP>(L, *s->Ptl, *s->Atl).v;
+        s->done[lid] = true;
+        return;
+    }
     if (s->mode == MODE_CLASSIFY) {
         s->mask[lid] = strip_classify(L, s->P->dry_tolerance, *s->A, s->seg_rows);
         s->cfl[lid] = 0.0;
@@ -86,12 +105,15 @@ void lane_entry(int lid)
 // Runs one wavefront; returns the classify mask (MODE_CLASSIFY; -1 if the
 // lanes disagree), else 0.
 long run_strip(const dclaw_params& P, const StripArgs& A, double* cfl_out, int mode = MODE_STEP,
-               int seg_rows = 0)
+               int seg_rows = 0, const dclaw_tl::Params* Ptl = nullptr,
+               const dclaw_tl::StripArgs* Atl = nullptr)
 {
     constexpr size_t STACK = 1 << 20;
     Sched* s = new Sched();
     s->P = &P;
     s->A = &A;
+    s->Ptl = Ptl;
+    s->Atl = Atl;
     s->mode = mode;
     s->seg_rows = seg_rows;
     s->stacks.resize(STACK * STRIP_LANES);
@@ -431,4 +453,42 @@ extern "C" void emu_amr_interp(const double* valc, const double* auxc, int mauxc
         for (int i = 1; i <= mic - 2; i++)
             amr_interp_cell(valc, auxc, mauxc, mic, val, auxf, mauxf, mitot, needs_set, rx, ry, i,
                             j, dry_tolerance, sea_level, rho_f, grav);
+}
+
+// The tangent-linear fused step: q_out, cflgrid as emu_stepgrid (aux not
+// written); dq_out = J_q dq_in + J_P dP at dt.
+extern "C" int emu_stepgrid_tl(const dclaw_params* P, const dclaw_params* dP, const double* q_in,
+                               const double* dq_in, double* q_out, double* dq_out,
+                               const double* aux, int maux, int mx, int my, double dx, double dy,
+                               double dt, int seg_rows, const int* bc, double* cflgrid)
+{
+    *cflgrid = 0.0;
+    const dclaw_tl::Params Ptl = dclaw_tl::make_params(*P, *dP);
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
+    StripArgs dummy{};
+    for (int s = 0; s < nstrips; s++)
+        for (int g = 0; g < nseg; g++) {
+            dclaw_tl::StripArgs A;
+            A.q_in.v = q_in;
+            A.q_in.d = dq_in;
+            A.q_out.v = q_out;
+            A.q_out.d = dq_out;
+            A.aux = aux;
+            A.aux_out = nullptr;
+            A.maux = maux;
+            A.mx = mx;
+            A.my = my;
+            A.dx = dx;
+            A.dy = dy;
+            A.dt = dt;
+            A.i0 = 1 + s * STRIP_W;
+            A.jA = 1 + g * seg_rows;
+            A.jB = std::min(my, A.jA + seg_rows - 1);
+            for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
+            A.fm = A.fp = A.gm = A.gp = nullptr;
+            A.tile_flag = nullptr;
+            run_strip(*P, dummy, cflgrid, MODE_TANGENT, 0, &Ptl, &A);
+        }
+    return 0;
 }

@@ -1,0 +1,110 @@
+"""GPU: the tangent-linear step (dclaw_stepgrid_tl_dev) against the host
+emulator of the same body, bit for bit in values and tangents; its values
+against the step; and a multi-step tangent run against central differences
+of the GPU step on the same time grid."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from dclaw_amd.params import CParams, to_c
+from helpers import advance_oracle, slope_patch
+from test_emu_tangent import _tl
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dclaw_amd import _lib
+    _lib.lib()
+    return torch
+
+
+@pytest.mark.parametrize("bc,seed", [((1, 1, 1, 1), 0), ((3, 3, 3, 3), 1), (None, 2)])
+def test_tangent_step_bitwise_emulator(dev, oracle_mod, emu_lib, bc, seed):
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    from dclaw_amd import tangent as T
+    torch = dev
+    pt = slope_patch(130, 70, slope=0.9, phi_deg=22.0, seed=11 + seed)
+    q, aux, dt = advance_oracle(oracle_mod, pt, 8, bc=bc or (1, 1, 1, 1))
+    q, aux = np.asfortranarray(q), np.asfortranarray(aux)
+    rng = np.random.default_rng(seed)
+    dq = np.asfortranarray(rng.normal(size=q.shape) * np.abs(q))
+    dpar = T.param_direction(phi=1.0, kref=1e-10)
+    print(f"[tangent] 130x70 bc {bc} seed {seed}", flush=True)
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(8)
+    try:
+        qe, dqe, cfle = _tl(emu_lib, to_c(pt.params), dpar, q, dq, aux, pt.dx, pt.dy, dt, 8, bc)
+        qi, dqi = S.to_device_q(q), S.to_device_q(dq)
+        qo, dqo = torch.zeros_like(qi), torch.zeros_like(qi)
+        bed = S.bed_tensor(aux)
+        cfl = torch.zeros(1, dtype=torch.float64, device="cuda")
+        dtd = torch.tensor([dt], dtype=torch.float64, device="cuda")
+        T.stepgrid_tl(qi, dqi, qo, dqo, bed, 1, pt.mx, pt.my, pt.dx, pt.dy, bc, dtd,
+                      S.params_tensor([pt.params]), T._record_tensor(dpar), cfl)
+        # the step itself on the same input
+        qf = torch.zeros_like(qi)
+        cf = torch.zeros(1, dtype=torch.float64, device="cuda")
+        if bc is None:
+            S.stepgrid(qi, qf, bed, 1, pt.mx, pt.my, pt.dx, pt.dy, dtd, S.params_tensor([pt.params]),
+                       cf)
+        else:
+            S.stepgrid_bc(qi, qf, bed, 1, pt.mx, pt.my, pt.dx, pt.dy, bc, dtd,
+                          S.params_tensor([pt.params]), cf)
+        torch.cuda.synchronize()
+    finally:
+        L.dclaw_set_seg_rows(prev)
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    a, b = S.to_host_q(qo), S.to_host_q(dqo)
+    assert np.array_equal(a[I], qe[I]) and cfl.item() == cfle
+    assert np.array_equal(b[I], dqe[I]), f"{(b[I] != dqe[I]).sum()} tangents differ"
+    assert np.array_equal(a, S.to_host_q(qf)) and cfl.item() == cf.item()
+    assert np.abs(b[I]).max() > 0.0
+
+
+def test_tangent_run_matches_finite_differences(dev):
+    """30 steps of the runner (the values' own dt rule, recorded) on a
+    fractal-bed patch with sources, d/dkref and d/dmu of the final state
+    against central differences of the forward step on the same dt sequence."""
+    from dclaw_amd import step as S
+    from dclaw_amd import tangent as T
+    from dclaw_amd import workloads as W
+    torch = dev
+    pt = W.fractal_sources(300, 200, dx=2.0, seed=6, nsrc=6)
+    nsteps = 30
+    base = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+    base.run(nsteps, 1e-3, 0.9, 1e9)
+    for name in ("kref", "mu"):
+        r = T.TangentRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy,
+                            dparams=T.param_direction(**{name: 1.0}))
+        r.run(nsteps, 1e-3, 0.9, 1e9, record_dt=True)
+        qv, dq = r.q(), r.dq()
+        assert np.array_equal(qv, base.q())          # values = the step's trajectory
+        c0 = getattr(to_c(pt.params), name)
+        eps = 1e-6 * c0
+
+        def fwd(sign):
+            p = to_c(pt.params)
+            setattr(p, name, c0 + sign * eps)
+            rr = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+            rr.params = torch.frombuffer(bytearray(bytes(p)), dtype=torch.uint8).to("cuda")
+            for d in r.dts:
+                rr.set_dt(d)
+                rr.cfl.zero_()
+                rr.step()
+            return rr.q()
+        fd = (fwd(+1) - fwd(-1)) / (2 * eps)
+        I = (slice(None), slice(2, -2), slice(2, -2))
+        for m in range(5):
+            scale = np.abs(fd[I][m]).max()
+            assert scale > 0.0
+            floor = 8.0 * np.abs(qv[I][m]).max() * np.finfo(float).eps / eps
+            err = np.abs(dq[I][m] - fd[I][m])
+            nbad = int((err > 1e-3 * scale + floor).sum())
+            assert nbad <= 0.001 * err.size, f"d/d{name} component {m}: {nbad} cells"
