@@ -45,10 +45,13 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--workload", choices=["patch", "ensemble", "wet"], default="patch",
+    ap.add_argument("--workload", choices=["patch", "ensemble", "wet", "flume"], default="patch",
                     help="patch: one 4096^2 patch (default); ensemble: 256 members with sides "
                          "log-uniform in [64, 512], batched launch vs plain per-member loop; "
-                         "wet: a fully wet --wet-n^2 inclined plane (the FP64-bound regime)")
+                         "wet: a fully wet --wet-n^2 inclined plane (the FP64-bound regime); "
+                         "flume: 2000 x 40 release, gradient of a gauge misfit w.r.t. phi, kref, "
+                         "m0 by tangent-linear runs")
+    ap.add_argument("--flume-steps", type=int, default=2000)
     ap.add_argument("--wet-n", type=int, default=2048, help="side of the wet workload")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--seg-rows", type=int, default=0,
@@ -89,6 +92,8 @@ def main():
     _lib.lib()
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "flume":
+        return bench_flume(args)
     wet_run = args.workload == "wet"
     n = args.wet_n if wet_run else args.n
     t0 = time.time()
@@ -259,6 +264,81 @@ def main():
         "fp64": fp64,
         "tangent": tangent,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_flume(args):
+    """Gradient workload: a 2000 x 40 flume release (dx = 0.05 m, 31 then 2
+    deg), 8 seeded gauges, loss = sum over steps and gauges of (h - target)^2
+    with a seeded synthetic target; the forward run timed, then the gradient
+    with respect to phi, kref and m0 by three tangent-linear runs (forward
+    mode; the reverse-mode adjoint is not built)."""
+    import numpy as np
+    import torch
+
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    from dclaw_amd import tangent as T
+    from dclaw_amd import workloads as W
+
+    pt, gxy = W.flume_release(seed=args.seed)
+    nsteps = args.flume_steps
+    rng = np.random.default_rng(args.seed)
+    centres = rng.uniform(0.1 * nsteps, 0.9 * nsteps, len(gxy))
+    n = np.arange(1, nsteps + 1)[:, None]
+    targets = 0.3 * np.exp(-((n - centres[None, :]) / (0.1 * nsteps)) ** 2)
+    seg = args.seg_rows if args.seg_rows > 0 else 4
+    _lib.check(min(0, _lib.lib().dclaw_set_seg_rows(seg)))
+    cells = pt.mx * pt.my
+
+    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+    r.run(5, 1e-3, 0.9, 1e9)
+    torch.cuda.synchronize()
+    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+    t0 = time.perf_counter()
+    r.run(nsteps, 1e-3, 0.9, 1e9)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+
+    dq_m0 = np.zeros_like(pt.q)
+    dq_m0[3] = pt.q[0]
+    dirs = {"phi": (T.param_direction(phi=1.0), None),
+            "kref": (T.param_direction(kref=1.0), None), "m0": (None, dq_m0)}
+    T.gauge_loss_gradient(pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy, targets[:5],
+                          {"phi": dirs["phi"]})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss, grads, _ = T.gauge_loss_gradient(pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy,
+                                           targets, dirs)
+    t_grad = time.perf_counter() - t0
+    print(f"[bench] flume forward {nsteps} steps {t_fwd:.3f}s, gradient (3 tangent runs) "
+          f"{t_grad:.3f}s, loss {loss:.6e}, grads {grads}", file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC,
+        "value": 3 * nsteps * cells / t_grad,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": 5,
+        "ms_per_step": t_grad / (3 * nsteps) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"gradient of a gauge misfit over a {pt.mx}x{pt.my} flume release "
+                               f"(dx=0.05 m, 31 then 2 deg, 0.75 m block, 8 seeded gauges, "
+                               f"seeded synthetic targets) w.r.t. phi, kref, m0: value = "
+                               f"tangent-linear step-updates/s over the 3 forward-mode runs "
+                               f"(reverse-mode adjoint not built)",
+                   "mx": pt.mx, "my": pt.my, "seg_rows": seg, "seed": args.seed},
+        "forward": {"seconds": t_fwd, "value": nsteps * cells / t_fwd,
+                    "ms_per_step": t_fwd / nsteps * 1e3},
+        "gradient": {"seconds": t_grad, "loss": loss, "dloss": grads,
+                     "units": {"phi": "per radian", "kref": "per m^2", "m0": "per unit m0"}},
+        "roofline": None,
+        "cpu_baseline": None,
     }
     print(json.dumps(out))
 

@@ -64,6 +64,8 @@ SIGNATURES = {
     "dclaw_tile_count": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dclaw_split_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dclaw_set_dry_rows": (ctypes.c_int, [ctypes.c_int]),
+    "dclaw_gauges_tl_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _vp, _i,
+                                           _vp, _vp, _vp]),
     "dclaw_stepgrid_tl_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _ip, _vp,
                                              _vp, _vp, _vp, _vp]),
     "dclaw_amr_average_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i,

@@ -653,9 +653,12 @@ __global__ void k_amr_interp(const double* __restrict__ valc, const double* __re
     }
 }
 
+// With dq (the tangent of q) the tangent of each gauge value goes to dout,
+// through the same branch and weights as the value (the bed has no tangent).
 __global__ void k_gauges(const double* __restrict__ q, const double* __restrict__ aux, int maux,
                          int mx, int my, double xlower, double ylower, double dx, double dy,
-                         double drytol, const double* __restrict__ xy, int ng, double* out)
+                         double drytol, const double* __restrict__ xy, int ng, double* out,
+                         const double* __restrict__ dq, double* dout)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
@@ -666,6 +669,8 @@ __global__ void k_gauges(const double* __restrict__ q, const double* __restrict_
     double* o = out + 8 * g;
     if (!(fx >= 0.5 && fy >= 0.5 && i0 >= 1 && i0 + 1 <= mx && j0 >= 1 && j0 + 1 <= my)) {
         for (int m = 0; m < 8; m++) o[m] = __longlong_as_double(0x7ff8000000000000LL);
+        if (dq)
+            for (int m = 0; m < 8; m++) dout[8 * g + m] = __longlong_as_double(0x7ff8000000000000LL);
         return;
     }
     bool wet = true;
@@ -687,9 +692,22 @@ __global__ void k_gauges(const double* __restrict__ q, const double* __restrict_
                                    q[qidx(6, i0 + di, j0 + dj, mx)]);
             }
         o[7] = o[0] + topo;
+        if (dq) {
+            double* d = dout + 8 * g;
+            for (int m = 0; m < 7; m++)
+                d[m] = ((w00 * dq[qidx(m, i0, j0, mx)] + w10 * dq[qidx(m, i0 + 1, j0, mx)]) +
+                        w01 * dq[qidx(m, i0, j0 + 1, mx)]) +
+                       w11 * dq[qidx(m, i0 + 1, j0 + 1, mx)];
+            d[7] = d[0] - d[6];
+        }
     } else {
         for (int m = 0; m < 7; m++) o[m] = q[qidx(m, ic, jc, mx)];
         o[7] = o[0] + (aux[aidx(ic, jc, mx, maux)] - q[qidx(6, ic, jc, mx)]);
+        if (dq) {
+            double* d = dout + 8 * g;
+            for (int m = 0; m < 7; m++) d[m] = dq[qidx(m, ic, jc, mx)];
+            d[7] = d[0] - d[6];
+        }
     }
 }
 
@@ -1108,7 +1126,23 @@ int dclaw_gauges_dev(const double* q, const double* aux, int maux, int mx, int m
     if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
     hipLaunchKernelGGL(k_gauges, dim3((ngauges + 63) / 64), dim3(64), 0, (hipStream_t)stream, q,
                        aux, maux, mx, my, xlower, ylower, dx, dy, dry_tolerance, xy_dev, ngauges,
-                       out_dev);
+                       out_dev, nullptr, nullptr);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_gauges_tl_dev(const double* q, const double* dq, const double* aux, int maux, int mx,
+                        int my, double xlower, double ylower, double dx, double dy,
+                        double dry_tolerance, const double* xy_dev, int ngauges, double* out_dev,
+                        double* dout_dev, void* stream)
+{
+    if (int rc = check_patch(mx, my, maux)) return rc;
+    if (!q || !dq || !aux || !xy_dev || !out_dev || !dout_dev || ngauges < 1)
+        return fail(DCLAW_EINVAL, "bad gauge args");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+    hipLaunchKernelGGL(k_gauges, dim3((ngauges + 63) / 64), dim3(64), 0, (hipStream_t)stream, q,
+                       aux, maux, mx, my, xlower, ylower, dx, dy, dry_tolerance, xy_dev, ngauges,
+                       out_dev, dq, dout_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

@@ -123,3 +123,53 @@ def directional_derivative(q: np.ndarray, aux: np.ndarray, params: DigParams, dx
     r = TangentRunner(q, aux, params, dx, dy, dparams=dparams, dq0=dq0, bc=bc)
     r.run_dts(dts)
     return r.q(), r.dq()
+
+
+def gauges_tl(q: torch.Tensor, dq: torch.Tensor, aux: torch.Tensor, maux: int, mx: int, my: int,
+              xlower: float, ylower: float, dx: float, dy: float, dry_tolerance: float,
+              xy_dev: torch.Tensor, out: torch.Tensor, dout: torch.Tensor) -> None:
+    """Gauge values (ngauges, 8) into out and their tangents into dout."""
+    _lib.check(_lib.lib().dclaw_gauges_tl_dev(
+        _ptr(q), _ptr(dq), _ptr(aux), maux, mx, my, xlower, ylower, dx, dy, dry_tolerance,
+        _ptr(xy_dev), xy_dev.shape[0], _ptr(out), _ptr(dout), _stream()))
+
+
+def gauge_loss_gradient(q: np.ndarray, aux: np.ndarray, params: DigParams, dx: float, dy: float,
+                        xlower: float, ylower: float, gauges_xy, targets: np.ndarray,
+                        directions: dict, dt0: float = 1e-3, cfl_desired: float = 0.9,
+                        dt_max: float = 1e9, component: int = 0, bc=(BC_EXTRAP,) * 4):
+    """Loss = sum over steps n = 1..N and gauges g of (gauge_g(q_n)[component]
+    - targets[n-1, g])^2, N = len(targets), with the device dt rule, and its
+    derivative along each direction by one tangent-linear run:
+    ``directions = {name: (dparams or None, dq0 or None)}``.  Returns
+    (loss, {name: dloss/dname}, dts) -- dts is the time grid the values chose
+    (the same in every run)."""
+    targets = np.asarray(targets, dtype=np.float64)
+    nsteps, ng = targets.shape
+    xy = torch.tensor(np.asarray(gauges_xy, dtype=np.float64).reshape(-1, 2), device="cuda")
+    if xy.shape[0] != ng:
+        raise ValueError("targets must have one column per gauge")
+    tgt = torch.tensor(targets, device="cuda")
+    out = torch.zeros((ng, 8), dtype=torch.float64, device="cuda")
+    dout = torch.zeros_like(out)
+    dts = torch.zeros(nsteps, dtype=torch.float64, device="cuda")
+    loss, grads = None, {}
+    for name, (dparams, dq0) in directions.items():
+        r = TangentRunner(q, aux, params, dx, dy, dparams=dparams, dq0=dq0, bc=bc)
+        acc = torch.zeros(2, dtype=torch.float64, device="cuda")
+        r.set_dt(dt0)
+        r.cfl.zero_()
+        for n in range(nsteps):
+            dts[n].copy_(r.dt[0])
+            r.step()
+            dt_update(r.dt, r.cfl, cfl_desired, dt_max)
+            gauges_tl(r.qa, r.dqa, r.aux, r.maux, r.mx, r.my, xlower, ylower, r.dx, r.dy,
+                      params.dry_tolerance, xy, out, dout)
+            res = out[:, component] - tgt[n]
+            acc[0] += (res * res).sum()
+            acc[1] += (2.0 * res * dout[:, component]).sum()
+        torch.cuda.synchronize()
+        if loss is None:
+            loss = float(acc[0].item())
+        grads[name] = float(acc[1].item())
+    return loss, grads, dts.cpu().numpy()

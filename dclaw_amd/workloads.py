@@ -116,6 +116,31 @@ def wet_plane(n: int = 2048, dx: float = 2.0, slope_deg: float = 20.0, h: float 
     return make_patch(b, np.full((n, n), h), 0.6, 0.8, p, dx, dx)
 
 
+def flume_release(mx: int = 2000, my: int = 40, dx: float = 0.05, seed: int = 3,
+                  ngauges: int = 8):
+    """Gradient workload: a 2000 x 40 flume of dx = 0.05 m, bed slope 31 deg
+    over the first 1600 cells then 2 deg (continuous), a block of 6 m^3 in
+    cells 0-79 x 0-39 (h = 6 / (80 * 40 * dx^2) = 0.75 m, m0 = 0.6, pb at 0.8
+    of hydrostatic; the pore-pressure ratio is this workload's choice)
+    released at t = 0.  Other sizes keep the proportions (break at 80 % of
+    the length, block over the first 4 %, depth 0.75 m).  Returns (patch,
+    gauge (x, y) positions seeded along the centre line over 30-95 % of the
+    length)."""
+    rng = np.random.default_rng(seed)
+    p = config0_params()
+    x = (np.arange(mx) + 0.5) * dx
+    xb = int(0.8 * mx) * dx
+    b1 = math.tan(math.radians(2.0)) * (mx * dx - x) + np.where(
+        x < xb, math.tan(math.radians(31.0)) * (xb - x), 0.0)
+    b = b1[:, None] + np.zeros((mx, my))
+    h = np.zeros((mx, my))
+    h[:max(1, mx // 25), :] = 0.75
+    pt = make_patch(b, h, 0.6, 0.8, p, dx, dx)
+    gx = np.sort(rng.uniform(0.3 * mx * dx, 0.95 * mx * dx, ngauges))
+    gy = np.full(ngauges, 0.5 * my * dx)
+    return pt, np.stack([gx, gy], axis=1)
+
+
 def fractal_surface(mx: int, my: int, rng: np.random.Generator, relief: float = 200.0,
                     slope: float = -2.0) -> np.ndarray:
     """Isotropic random surface with power spectrum ~ k^slope, rescaled to the

@@ -207,6 +207,16 @@ int dclaw_gauges_dev(const double* q, const double* aux, int maux, int mx, int m
                      double ylower, double dx, double dy, double dry_tolerance,
                      const double* xy_dev, int ngauges, double* out_dev, void* stream);
 
+/* The same gauges with their tangents: dq is the tangent of q (as from
+ * dclaw_stepgrid_tl_dev); dout_dev (8 doubles per gauge) receives the tangent
+ * of each gauge value through the branch and weights the values take (bed
+ * without tangent).  No reference counterpart (derivatives of gauge
+ * records for gradient-based fitting). */
+int dclaw_gauges_tl_dev(const double* q, const double* dq, const double* aux, int maux, int mx,
+                        int my, double xlower, double ylower, double dx, double dy,
+                        double dry_tolerance, const double* xy_dev, int ngauges, double* out_dev,
+                        double* dout_dev, void* stream);
+
 /* AMR transfers between refinement levels, on device memory (the numerical
  * part of the AMR driver's coarse/fine coupling; coordinate_system 1).
  * Arrays: q(7, nx, ny), aux(maux, nx, ny) (aux component 1 = bed), indices

@@ -108,3 +108,54 @@ def test_tangent_run_matches_finite_differences(dev):
             err = np.abs(dq[I][m] - fd[I][m])
             nbad = int((err > 1e-3 * scale + floor).sum())
             assert nbad <= 0.001 * err.size, f"d/d{name} component {m}: {nbad} cells"
+
+
+def test_gauge_loss_gradient_matches_finite_differences(dev):
+    """Gradient of a gauge misfit (sum over steps and gauges of h^2) with
+    respect to phi, kref (parameters) and m0 (initial-state direction: hm =
+    m0 h) by tangent-linear runs, against central differences of the forward
+    runs on the same time grid."""
+    from dclaw_amd import step as S
+    from dclaw_amd import tangent as T
+    from dclaw_amd import workloads as W
+    from dclaw_amd.params import to_c
+    torch = dev
+    pt, _ = W.flume_release(mx=400, my=12, dx=0.05)
+    gxy = np.array([[0.9, 0.3], [1.3, 0.3], [1.7, 0.25], [2.2, 0.35]])
+    nsteps = 120
+    targets = np.zeros((nsteps, len(gxy)))
+    dq_m0 = np.zeros_like(pt.q)
+    dq_m0[3] = pt.q[0]
+    dirs = {"phi": (T.param_direction(phi=1.0), None), "kref": (T.param_direction(kref=1.0), None),
+            "m0": (None, dq_m0)}
+    loss, grads, dts = T.gauge_loss_gradient(pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy,
+                                             targets, dirs)
+    assert loss > 0.0
+
+    def fwd_loss(c=None, q0=None):
+        r = S.PatchRunner(pt.q if q0 is None else q0, pt.aux, pt.params, pt.dx, pt.dy)
+        if c is not None:
+            r.params = torch.frombuffer(bytearray(bytes(c)), dtype=torch.uint8).to("cuda")
+        xy = torch.tensor(gxy, device="cuda")
+        out = torch.zeros((len(gxy), 8), dtype=torch.float64, device="cuda")
+        tot = 0.0
+        for d in dts:
+            r.set_dt(float(d))
+            r.cfl.zero_()
+            r.step()
+            S.gauges(r.qa, r.bed, r.maux, r.mx, r.my, 0.0, 0.0, r.dx, r.dy,
+                     pt.params.dry_tolerance, xy, out)
+            tot += float((out[:, 0] ** 2).sum().item())
+        return tot
+    assert fwd_loss() == pytest.approx(loss, rel=1e-14)
+    for name in ("phi", "kref"):
+        c0 = getattr(to_c(pt.params), name)
+        eps = 1e-6 * c0
+        cp, cm = to_c(pt.params), to_c(pt.params)
+        setattr(cp, name, c0 + eps)
+        setattr(cm, name, c0 - eps)
+        fd = (fwd_loss(c=cp) - fwd_loss(c=cm)) / (2 * eps)
+        assert grads[name] == pytest.approx(fd, rel=1e-4), name
+    eps = 1e-6
+    fd = (fwd_loss(q0=pt.q + eps * dq_m0) - fwd_loss(q0=pt.q - eps * dq_m0)) / (2 * eps)
+    assert grads["m0"] == pytest.approx(fd, rel=1e-4)
