@@ -400,6 +400,32 @@ def bench_ensemble(args):
     achieved = BYTES_PER_CELL_FWD * cells / kern_avg_s / 1e9
     print(f"[bench] batched {t_b / args.steps * 1e3:.3f} ms/step, plain "
           f"{t_p / nplain * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # the tangent-linear step (values + d/dphi for every member): batched
+    # launch against the per-member loop of single-patch launches
+    tangent = None
+    if args.tangent_steps > 0:
+        from dclaw_amd import tangent as T
+        dirs = [T.param_direction(phi=1.0) for _ in members]
+        te = T.TangentEnsemble(members, directions=dirs, seg_rows=seg)
+        te.run(2, 1e-3, 0.9, 1e9)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        te.run(args.tangent_steps, 1e-3, 0.9, 1e9)
+        torch.cuda.synchronize()
+        t_tb = (time.perf_counter() - t0) / args.tangent_steps
+        np_tl = max(2, args.tangent_steps // 5)
+        t0 = time.perf_counter()
+        te.run(np_tl, 1e-3, 0.9, 1e9, plain=True)
+        torch.cuda.synchronize()
+        t_tp = (time.perf_counter() - t0) / np_tl
+        tangent = {"ms_per_step": t_tb * 1e3, "value": cells / t_tb, "unit": "cell-updates/s",
+                   "steps": args.tangent_steps, "direction": "d/dphi per member",
+                   "plain_loop_ms_per_step": t_tp * 1e3, "batched_over_plain": t_tp / t_tb,
+                   "kernel": "k_step_batched_tl (forward mode, src2 fused)"}
+        del te
+        print(f"[bench] tangent-linear batched {t_tb * 1e3:.3f} ms/step, plain "
+              f"{t_tp * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_b / args.steps * 1e3,
@@ -408,7 +434,9 @@ def bench_ensemble(args):
         "config": {"workload": f"batched ensemble forward, {len(members)} members, sides "
                                f"log-uniform in [64,512], per-member phi/m0/kref, one batched "
                                f"launch per step (plus the src2 pass over wet tiles unless "
-                               f"fused; adjoint not built)", "members": len(members),
+                               f"fused); the tangent-linear batched step is timed under "
+                               f"'tangent'; reverse-mode adjoint not built",
+                   "members": len(members),
                    "cells": cells, "seed": args.seed, "seg_rows": seg,
                    "src2": "fused" if args.fused else "split"},
         "plain_loop": {"value": plain, "steps": nplain,
@@ -419,6 +447,7 @@ def bench_ensemble(args):
                      "kernel": ("k_step_batched<0>" if args.fused
                                 else "k_step_batched<1> + k_src2_batched (one step)"),
                      "kernel_avg_ms": kern_avg_s * 1e3},
+        "tangent": tangent,
         "cpu_baseline": None,
     }
     print(json.dumps(out))

@@ -66,6 +66,8 @@ SIGNATURES = {
     "dclaw_set_dry_rows": (ctypes.c_int, [ctypes.c_int]),
     "dclaw_gauges_tl_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _vp, _i,
                                            _vp, _vp, _vp]),
+    "dclaw_step_batched_tl": (ctypes.c_int, [_vp, _vp, _i, _vp, _i, _i, _ip, _vp, _vp, _vp, _vp,
+                                             _vp]),
     "dclaw_stepgrid_tl_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _ip, _vp,
                                              _vp, _vp, _vp, _vp]),
     "dclaw_amr_average_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i,

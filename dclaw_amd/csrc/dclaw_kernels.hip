@@ -228,6 +228,51 @@ __global__ __launch_bounds__(64) void k_stepgrid_tl(const double* __restrict__ q
     wave_max_to(cfl.v, cfl_dev);
 }
 
+// Tangent-linear fused step over a batched work list: tangents[p] holds the
+// tangent arrays of patch p (its q_in / q_out fields), dparams[member] the
+// member's direction.
+__global__ __launch_bounds__(64) void k_step_batched_tl(const dclaw_patch* __restrict__ patches,
+                                                        const dclaw_patch* __restrict__ tangents,
+                                                        const int32_t* __restrict__ tiles,
+                                                        int seg_rows,
+                                                        const double* __restrict__ dt_dev,
+                                                        const dclaw_params* __restrict__ params,
+                                                        const dclaw_params* __restrict__ dparams,
+                                                        double* cfl_dev, int bx0, int bx1,
+                                                        int by0, int by1)
+{
+    const int t = (int)blockIdx.x;
+    const int pidx = tiles[4 * t + 0];
+    const dclaw_patch pt = patches[pidx];
+    const dclaw_patch tp = tangents[pidx];
+    const dclaw_tl::Params P = dclaw_tl::make_params(params[pt.member], dparams[pt.member]);
+    dclaw_tl::StripArgs A;
+    A.q_in.v = pt.q_in;
+    A.q_in.d = tp.q_in;
+    A.q_out.v = pt.q_out;
+    A.q_out.d = tp.q_out;
+    A.aux = pt.aux;
+    A.aux_out = nullptr;
+    A.bc[0] = bx0;
+    A.bc[1] = bx1;
+    A.bc[2] = by0;
+    A.bc[3] = by1;
+    A.maux = pt.maux;
+    A.mx = pt.mx;
+    A.my = pt.my;
+    A.dx = pt.dx;
+    A.dy = pt.dy;
+    A.dt = dt_dev[pt.member];
+    A.i0 = 1 + tiles[4 * t + 1] * STRIP_W;
+    A.jA = 1 + tiles[4 * t + 2] * seg_rows;
+    A.jB = min(pt.my, A.jA + seg_rows - 1);
+    A.fm = A.fp = A.gm = A.gp = nullptr;
+    A.tile_flag = nullptr;
+    GpuLane L;
+    const dclaw_tl::Dual cfl = dclaw_tl::strip_step<MODE_STEP>(L, P, A);
+    wave_max_to(cfl.v, cfl_dev + pt.member);
+}
+
 // src2 over the interior cells of the tiles the no-src2 step flagged (one
 // patch; tile t = segment * nstrips + strip).  One wavefront per task, a task
 // being a 64-cell chunk of a tile, so the wet tiles spread over the whole chip;
@@ -1012,6 +1057,24 @@ int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t
     hipLaunchKernelGGL(k_src2_batched, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, st,
                        patches_dev, tiles_dev, ntiles, seg_rows, tile_flags_dev, dt_dev,
                        params_dev);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_step_batched_tl(const dclaw_patch* patches_dev, const dclaw_patch* tangents_dev,
+                          int npatch, const int32_t* tiles_dev, int ntiles, int seg_rows,
+                          const int* bc, const double* dt_dev, const dclaw_params* params_dev,
+                          const dclaw_params* dparams_dev, double* cfl_dev, void* stream)
+{
+    if (!patches_dev || !tangents_dev || !tiles_dev || !dt_dev || !params_dev || !dparams_dev ||
+        !cfl_dev)
+        return fail(DCLAW_EINVAL, "null pointer");
+    if (npatch < 1 || ntiles < 1 || seg_rows < 1) return fail(DCLAW_EINVAL, "empty batch");
+    if (int rcb = check_bc(bc)) return rcb;
+    const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
+    hipLaunchKernelGGL(k_step_batched_tl, dim3(ntiles), dim3(64), 0, (hipStream_t)stream,
+                       patches_dev, tangents_dev, tiles_dev, seg_rows, dt_dev, params_dev,
+                       dparams_dev, cfl_dev, b0, b1, b2, b3);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

@@ -173,3 +173,94 @@ def gauge_loss_gradient(q: np.ndarray, aux: np.ndarray, params: DigParams, dx: f
             loss = float(acc[0].item())
         grads[name] = float(acc[1].item())
     return loss, grads, dts.cpu().numpy()
+
+
+class TangentEnsemble:
+    """Members of uneven patch sizes advanced with their tangents by one
+    launch per step (``dclaw_step_batched_tl``): member k carries the
+    direction directions[k] (a ``param_direction`` record; default 0) and the
+    initial tangent dq0[k] (default 0).  Values equal the forward ensemble's
+    (src2 fused) bit for bit."""
+
+    def __init__(self, members: List, directions: Optional[Sequence[CParams]] = None,
+                 dq0: Optional[Sequence[Optional[np.ndarray]]] = None, bc=(BC_EXTRAP,) * 4,
+                 seg_rows: int = 16, device="cuda"):
+        from .step import Ensemble
+        self.ens = Ensemble(members, bc=bc, seg_rows=seg_rows, device=device, split_src2=False)
+        e = self.ens
+        self.n = e.n
+        self.dqa = [to_device_q(dq0[k], device) if dq0 is not None and dq0[k] is not None
+                    else torch.zeros_like(e.qa[k]) for k in range(e.n)]
+        self.dqb = [torch.empty_like(t) for t in self.dqa]
+        recs = list(directions) if directions is not None else [CParams() for _ in range(e.n)]
+        if len(recs) != e.n:
+            raise ValueError("one direction record per member")
+        self.dparams = torch.frombuffer(bytearray(b"".join(bytes(r) for r in recs)),
+                                        dtype=torch.uint8).to(device)
+        self._tpatches = [self._tangent_table(False, device), self._tangent_table(True, device)]
+
+    def _tangent_table(self, flipped: bool, device) -> torch.Tensor:
+        e = self.ens
+        recs = []
+        for k in range(e.n):
+            di, do = (self.dqb[k], self.dqa[k]) if flipped else (self.dqa[k], self.dqb[k])
+            recs.append(_lib.CPatch(_ptr(di), _ptr(do), _ptr(e.bed[k]), e.mx[k], e.my[k],
+                                    e.maux[k], k, e.dx[k], e.dy[k]))
+        raw = b"".join(bytes(r) for r in recs)
+        return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+
+    @property
+    def dt(self) -> torch.Tensor:
+        return self.ens.dt
+
+    @property
+    def cfl(self) -> torch.Tensor:
+        return self.ens.cfl
+
+    def step(self) -> None:
+        """One launch for all members and their tangents."""
+        e = self.ens
+        bc = (ctypes.c_int * 4)(*e.bc)
+        f = 1 if e.flip else 0
+        _lib.check(_lib.lib().dclaw_step_batched_tl(
+            _ptr(e._patches[f]), _ptr(self._tpatches[f]), e.n, _ptr(e.tiles), e.ntiles,
+            e.seg_rows, bc, _ptr(e.dt), _ptr(e.params), _ptr(self.dparams), _ptr(e.cfl),
+            _stream()))
+        e.flip = not e.flip
+
+    def step_plain(self) -> None:
+        """The per-member loop of single-patch tangent-linear launches."""
+        e = self.ens
+        rec = ctypes.sizeof(CParams)
+        for k in range(e.n):
+            qi = e.current(k)
+            qo = e.qa[k] if e.flip else e.qb[k]
+            di = self.dqb[k] if e.flip else self.dqa[k]
+            do = self.dqa[k] if e.flip else self.dqb[k]
+            stepgrid_tl(qi, di, qo, do, e.bed[k], e.maux[k], e.mx[k], e.my[k], e.dx[k], e.dy[k],
+                        e.bc, e.dt[k:k + 1], e.params[k * rec:(k + 1) * rec],
+                        self.dparams[k * rec:(k + 1) * rec], e.cfl[k:k + 1])
+        e.flip = not e.flip
+
+    def run(self, nsteps: int, dt0, cfl_desired: float, dt_max: float,
+            plain: bool = False) -> None:
+        e = self.ens
+        if np.isscalar(dt0):
+            e.dt.fill_(float(dt0))
+        else:
+            e.dt.copy_(torch.as_tensor(dt0, dtype=torch.float64))
+        e.cfl.zero_()
+        for _ in range(nsteps):
+            if plain:
+                self.step_plain()
+            else:
+                self.step()
+            dt_update(e.dt, e.cfl, cfl_desired, dt_max)
+
+    def q(self, k: int) -> np.ndarray:
+        torch.cuda.synchronize()
+        return self.ens.q(k)
+
+    def dq(self, k: int) -> np.ndarray:
+        torch.cuda.synchronize()
+        return to_host_q(self.dqb[k] if self.ens.flip else self.dqa[k])

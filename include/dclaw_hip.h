@@ -180,6 +180,16 @@ int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t
                        const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,
                        void* stream);
 
+/* The tangent-linear step (as dclaw_stepgrid_tl_dev) over a batched work
+ * list (as dclaw_step_batched, src2 fused): tangents_dev[p] gives patch p's
+ * tangent arrays in its q_in (dq at t_n) and q_out (dq at t_n+dt) fields, the
+ * other fields ignored; dparams_dev[member] is the member's direction.  No
+ * reference counterpart. */
+int dclaw_step_batched_tl(const dclaw_patch* patches_dev, const dclaw_patch* tangents_dev,
+                          int npatch, const int32_t* tiles_dev, int ntiles, int seg_rows,
+                          const int* bc, const double* dt_dev, const dclaw_params* params_dev,
+                          const dclaw_params* dparams_dev, double* cfl_dev, void* stream);
+
 /* Device dt rule for n members: dt[k] <- min(dt[k]*cfl_desired/cfl[k], dt_max)
  * when cfl[k] > 0 (else dt_max), then cfl[k] <- 0 (GeoClaw stepgrid dtnew
  * and tick.f:499 dt_max cap). */

@@ -159,3 +159,69 @@ def test_gauge_loss_gradient_matches_finite_differences(dev):
     eps = 1e-6
     fd = (fwd_loss(q0=pt.q + eps * dq_m0) - fwd_loss(q0=pt.q - eps * dq_m0)) / (2 * eps)
     assert grads["m0"] == pytest.approx(fd, rel=1e-4)
+
+
+def test_flume_tangent_is_the_derivative_of_the_discrete_map(dev):
+    """On the 2000 x 40 flume release the tangent after 150 steps equals
+    central differences taken at a tiny step (1e-8 relative in phi): the
+    map's linear regime is that narrow there (larger steps cross the
+    limiter's and the friction law's kinks), and the tangent is its
+    derivative."""
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    from dclaw_amd import tangent as T
+    from dclaw_amd import workloads as W
+    torch = dev
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(4)
+    try:
+        pt, _ = W.flume_release()
+        r = T.TangentRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy,
+                            dparams=T.param_direction(phi=1.0))
+        r.run(150, 1e-3, 0.9, 1e9, record_dt=True)
+        dq = r.dq()
+        c0 = to_c(pt.params).phi
+        eps = 1e-8 * c0
+
+        def fwd(sign):
+            p = to_c(pt.params)
+            p.phi = c0 + sign * eps
+            rr = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+            rr.params = torch.frombuffer(bytearray(bytes(p)), dtype=torch.uint8).to("cuda")
+            for d in r.dts:
+                rr.set_dt(d)
+                rr.cfl.zero_()
+                rr.step()
+            return rr.q()
+        fd = (fwd(1) - fwd(-1)) / (2 * eps)
+    finally:
+        L.dclaw_set_seg_rows(prev)
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    for m in (0, 1, 3, 4):
+        scale = np.abs(dq[I][m]).max()
+        assert scale > 0.0
+        assert np.abs(fd[I][m] - dq[I][m]).max() <= 1e-4 * scale, m
+
+
+def test_batched_tangent_equals_per_member_launches(dev):
+    """Five members of uneven sizes with different directions: the batched
+    tangent-linear launch equals the per-member launches bit for bit (values
+    and tangents), and its values equal the forward ensemble's."""
+    from dclaw_amd import step as S
+    from dclaw_amd import tangent as T
+    from dclaw_amd import workloads as W
+    torch = dev
+    members = W.ensemble(5, 60, 200, seed=4)
+    dirs = [T.param_direction(phi=1.0), T.param_direction(kref=1e-10), T.param_direction(mu=1.0),
+            T.param_direction(phi=0.5, delta=1.0), T.param_direction()]
+    a = T.TangentEnsemble(members, directions=dirs, seg_rows=8)
+    b = T.TangentEnsemble(members, directions=dirs, seg_rows=8)
+    f = S.Ensemble(members, seg_rows=8, split_src2=False)
+    a.run(12, 1e-3, 0.9, 1e9)
+    b.run(12, 1e-3, 0.9, 1e9, plain=True)
+    f.run(12, 1e-3, 0.9, 1e9)
+    torch.cuda.synchronize()
+    for k in range(5):
+        assert np.array_equal(a.q(k), b.q(k)) and np.array_equal(a.dq(k), b.dq(k)), k
+        assert np.array_equal(a.q(k), f.q(k)), k
+    assert np.abs(a.dq(0)).max() > 0.0 and not a.dq(4).any()
