@@ -53,14 +53,19 @@ def parse():
                          "m0 by tangent-linear runs")
     ap.add_argument("--flume-steps", type=int, default=2000)
     ap.add_argument("--wet-n", type=int, default=2048, help="side of the wet workload")
+    ap.add_argument("--ens-list-order", action="store_true",
+                    help="ensemble: dispatch the work list in its own order instead of the "
+                         "previous step's wet tiles first")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--seg-rows", type=int, default=0,
                     help="rows per segment tile; 0 = pick with the autotuner (untimed)")
     ap.add_argument("--fused", action="store_true",
                     help="run src2 inside the strip kernel (one launch per step) instead of "
                          "the default split step (strip kernel + src2 pass over wet tiles)")
-    ap.add_argument("--no-dry-pass", action="store_true",
-                    help="split step as one strip launch over every tile (no separate dry pass)")
+    ap.add_argument("--split-mode", choices=["auto", "single", "ordered", "dry"], default="auto",
+                    help="split step: one strip launch in tile order (single), with the previous "
+                         "step's wet tiles first (ordered), or after a separate dry pass (dry); "
+                         "auto = the autotuner's pick")
     ap.add_argument("--dry-rows", type=int, default=0,
                     help="rows per wavefront of the dry pass (0 = library default)")
     ap.add_argument("--tangent-steps", type=int, default=20,
@@ -102,7 +107,7 @@ def main():
           flush=True)
 
     r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_src2=not args.fused,
-                      dry_pass=not args.no_dry_pass)
+                      split_mode="ordered" if args.split_mode == "auto" else args.split_mode)
     if args.dry_rows > 0:
         _lib.check(min(0, _lib.lib().dclaw_set_dry_rows(args.dry_rows)))
     r.set_dt(1e-3)
@@ -114,7 +119,7 @@ def main():
             ev[0].record()
         if r.split_src2:
             S.stepgrid_split(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.bc, r.dt, r.params,
-                             r.cfl, r.flags, r.dry_pass)
+                             r.cfl, r.flags, r.split_mode)
         else:
             S.stepgrid_bc(r.qa, r.qb, r.bed, 1, r.mx, r.my, r.dx, r.dy, r.bc, r.dt, r.params,
                           r.cfl)
@@ -131,11 +136,13 @@ def main():
     if seg <= 0:
         from dclaw_amd import autotune
         seg, tune_times = autotune.tune_patch(
-            r, dry_pass_modes=(False,) if args.no_dry_pass else (True, False))
-        tune_times = {f"{h}{'/dry' if d else ''}": round(v, 4) for (h, d), v in tune_times.items()}
+            r, split_modes=(("single", "ordered", "dry") if args.split_mode == "auto"
+                            else (args.split_mode,)))
+        tune_times = {f"{h}/{m}": round(v, 4) for (h, m), v in tune_times.items()}
     else:
         _lib.lib().dclaw_set_seg_rows(seg)
-    print(f"[bench] warmup done, segment rows {seg}, dry pass {r.dry_pass} (tuner: {tune_times})",
+    print(f"[bench] warmup done, segment rows {seg}, split mode {r.split_mode} "
+          f"(tuner: {tune_times})",
           file=sys.stderr,
           flush=True)
 
@@ -169,13 +176,26 @@ def main():
     # HBM bytes of one step from the committed counter summary of this mode
     # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a 1 GiB copy)
     traffic = None
-    mode = "fused" if args.fused else ("dry_pass" if r.dry_pass else "split_step")
-    tpath = os.path.join(ROOT, "profiles",
-                         f"{mode}_{'wet' if wet_run else 'fractal'}{n}_traffic.txt")
-    if os.path.exists(tpath):
-        for line in open(tpath):
-            if line.startswith("traffic_bytes_per_step"):
-                traffic = float(line.split("=")[1])
+    mode = "fused" if args.fused else {"single": "split_step", "ordered": "ordered",
+                                       "dry": "dry_pass"}[r.split_mode]
+    run_cfg = f"--seg-rows {seg} --split-mode {r.split_mode}"
+
+    def summary_value(path, key):
+        """key's value from a committed counter summary, if that summary was
+        taken on this run's segment height and split mode."""
+        if not os.path.exists(path):
+            return None
+        text = open(path).read()
+        if run_cfg not in text:
+            return None
+        for line in text.splitlines():
+            if line.startswith(key):
+                return float(line.split("=")[1])
+        return None
+
+    wl = "wet" if wet_run else "fractal"
+    traffic = summary_value(os.path.join(ROOT, "profiles", f"{mode}_{wl}{n}_traffic.txt"),
+                            "traffic_bytes_per_step")
 
     # tangent-linear step (values + d/dphi) on the same workload, from the
     # initial state, after the forward timing
@@ -204,14 +224,12 @@ def main():
     # FP64 VALU work of one step from the committed counter summary
     # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, FMA counted twice)
     fp64 = None
-    fpath = os.path.join(ROOT, "profiles", f"{mode}_{'wet' if wet_run else 'fractal'}{n}_fp64.txt")
-    if os.path.exists(fpath):
-        for line in open(fpath):
-            if line.startswith("fp64_ops_per_step"):
-                ops = float(line.split("=")[1])
-                rate = ops / kern_avg_s / 1e12
-                fp64 = {"ops_per_step": ops, "achieved": rate, "peak": FP64_PEAK_TFLOPS,
-                        "unit": "TFLOP/s", "frac": rate / FP64_PEAK_TFLOPS}
+    ops = summary_value(os.path.join(ROOT, "profiles", f"{mode}_{wl}{n}_fp64.txt"),
+                        "fp64_ops_per_step")
+    if ops is not None:
+        rate = ops / kern_avg_s / 1e12
+        fp64 = {"ops_per_step": ops, "achieved": rate, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": rate / FP64_PEAK_TFLOPS}
 
     # cells whose state can change this step: within 3 cells of a wet cell
     hw = torch.from_numpy(np.ascontiguousarray(q[0, 2:-2, 2:-2] > pt.params.dry_tolerance))
@@ -253,11 +271,14 @@ def main():
                    "mx": n, "my": n, "seed": args.seed, "cfl_desired": cfl_desired,
                    "wet_cells_end": wet, "active_cells_end": active, "seg_rows": seg,
                    "src2": "fused" if args.fused else "split",
-                   "dry_pass": bool(r.split_src2 and r.dry_pass)},
+                   "split_mode": None if args.fused else r.split_mode},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("k_stepgrid<0>" if args.fused
-                                else "k_stepgrid<1> + k_src2_tiles (one step)" if not r.dry_pass
+                                else "k_stepgrid<1> + k_src2_tiles (one step)"
+                                if r.split_mode == "single"
+                                else "k_zero_ints + k_order_tiles + k_stepgrid<1> + "
+                                     "k_src2_tiles (one step)" if r.split_mode == "ordered"
                                 else "k_zero_ints + k_classify + k_stepgrid_list<1> + "
                                      "k_src2_list (one step)"),
                      "kernel_avg_ms": kern_avg_s * 1e3},
@@ -359,8 +380,10 @@ def bench_ensemble(args):
     if seg <= 0:
         from dclaw_amd import autotune
         seg, tune_times = autotune.tune_ensemble(
-            members, lambda h: S.Ensemble(members, seg_rows=h, split_src2=not args.fused))
-    ens = S.Ensemble(members, seg_rows=seg, split_src2=not args.fused)
+            members, lambda h: S.Ensemble(members, seg_rows=h, split_src2=not args.fused,
+                                          ordered=not args.ens_list_order))
+    ens = S.Ensemble(members, seg_rows=seg, split_src2=not args.fused,
+                     ordered=not args.ens_list_order)
     cells = sum(m.patch.mx * m.patch.my for m in members)
     print(f"[bench] ensemble of {len(members)} members, {cells} cells, built in "
           f"{time.time() - t0:.1f}s, {ens.ntiles} tiles of {seg} rows (tuner: {tune_times})",
@@ -438,7 +461,8 @@ def bench_ensemble(args):
                                f"'tangent'; reverse-mode adjoint not built",
                    "members": len(members),
                    "cells": cells, "seed": args.seed, "seg_rows": seg,
-                   "src2": "fused" if args.fused else "split"},
+                   "src2": "fused" if args.fused else "split",
+                   "order": "list" if args.ens_list_order else "wet-first"},
         "plain_loop": {"value": plain, "steps": nplain,
                        "ms_per_step": t_p / nplain * 1e3},
         "batched_over_plain": value / plain,

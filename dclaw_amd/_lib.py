@@ -61,9 +61,12 @@ SIGNATURES = {
                                             ctypes.POINTER(ctypes.c_int32)]),
     "dclaw_step_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                           _ip, _vp, _vp, _vp, _vp, _vp]),
+    "dclaw_step_batched_ws": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+                                          _ip, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "dclaw_tile_count": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dclaw_split_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dclaw_set_dry_rows": (ctypes.c_int, [ctypes.c_int]),
+    "dclaw_set_split_mode": (ctypes.c_int, [ctypes.c_int]),
     "dclaw_gauges_tl_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _vp, _i,
                                            _vp, _vp, _vp]),
     "dclaw_step_batched_tl": (ctypes.c_int, [_vp, _vp, _i, _vp, _i, _i, _ip, _vp, _vp, _vp, _vp,

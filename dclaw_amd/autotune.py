@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 
-CANDIDATES = (4, 6, 8, 12, 16, 24, 32, 48)
+CANDIDATES = (4, 6, 8, 12, 16, 20, 24, 28, 32, 48)
 
 
 def _time(fn: Callable[[], None], reps: int) -> float:
@@ -35,20 +35,21 @@ def _time(fn: Callable[[], None], reps: int) -> float:
 
 
 def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 5,
-               dry_pass_modes: Iterable[bool] = (True, False)) -> Tuple[int, Dict]:
+               split_modes: Iterable[str] = ("single", "ordered", "dry")) -> Tuple[int, Dict]:
     """Pick the segment height for a PatchRunner's single-patch launches and,
-    for the split step, whether the separate dry pass runs (it pays when most
-    tiles are dry).  Every candidate starts from the same state (restored
-    outside the timed region) and times `reps` consecutive steps at a fixed
-    dt; the runner's state and dt are restored afterwards.  Returns the
-    height and the times in ms keyed by (height, dry_pass)."""
+    for the split step, its mode (one strip launch in tile order, with the
+    previous step's wet tiles first, or after a separate dry pass -- which
+    wins depends on the wet fraction).  Every candidate starts from the same
+    state (restored outside the timed region) and times `reps` consecutive
+    steps at a fixed dt; the runner's state and dt are restored afterwards.
+    Returns the height and the times in ms keyed by (height, mode)."""
     L = _lib.lib()
     qa, qb = runner.qa.clone(), runner.qb.clone()
     dt, cfl = runner.dt.clone(), runner.cfl.clone()
-    modes = tuple(dry_pass_modes) if runner.split_src2 else (runner.dry_pass,)
+    modes = tuple(split_modes) if runner.split_src2 else (runner.split_mode,)
     times = {}
-    for dry in modes:
-        runner.dry_pass = dry
+    for mode in modes:
+        runner.split_mode = mode
         for h in candidates:
             _lib.check(min(0, L.dclaw_set_seg_rows(h)))
             runner.qa.copy_(qa)
@@ -56,8 +57,8 @@ def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 5,
             def one():
                 runner.step()
                 runner.cfl.zero_()
-            times[(h, dry)] = _time(one, reps)
-    best, runner.dry_pass = min(times, key=times.get)
+            times[(h, mode)] = _time(one, reps)
+    best, runner.split_mode = min(times, key=times.get)
     _lib.check(min(0, L.dclaw_set_seg_rows(best)))
     runner.qa.copy_(qa)
     runner.qb.copy_(qb)

@@ -87,6 +87,7 @@ __device__ __forceinline__ void wave_max_to(double v, double* dst)
 
 int g_seg_rows = 32;   // rows per segment of the single-patch launch (tunable)
 int g_dry_rows = 48;   // rows per wavefront of the split step's dry pass (whole segments)
+int g_split_mode = 1;  // split step with the full workspace: 1 wet-first order, 2 dry pass
 
 // Compute units of the current device (cached per device).
 int cu_count()
@@ -114,12 +115,15 @@ __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in
                                                  const dclaw_params* __restrict__ params,
                                                  double* cfl_dev, int seg_rows, int nstrips,
                                                  int nseg, int bx0, int bx1, int by0, int by1,
-                                                 int* tile_flags)
+                                                 int* tile_flags,
+                                                 const int* __restrict__ order)
 {
-    // tile order: the dispatcher deals consecutive workgroups to the XCDs
-    // round-robin, which spreads a cluster of wet tiles over the whole chip
-    // (contiguous runs per XCD concentrate it and measured slower)
-    const int strip = (int)blockIdx.x % nstrips, seg = (int)blockIdx.x / nstrips;
+    // tile order: row-major, or the permutation `order` (wet tiles of the
+    // previous step first, so the long tiles start first); the dispatcher
+    // deals consecutive workgroups to the XCDs round-robin, which spreads a
+    // cluster of wet tiles over the whole chip
+    const int T = order ? order[blockIdx.x] : (int)blockIdx.x;
+    const int strip = T % nstrips, seg = T / nstrips;
     const dclaw_params P = *params;
     StripArgs A;
     A.bc[0] = bx0;
@@ -153,9 +157,10 @@ __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restri
                                                      const double* __restrict__ dt_dev,
                                                      const dclaw_params* __restrict__ params,
                                                      double* cfl_dev, int bx0, int bx1, int by0,
-                                                     int by1, int* tile_flags)
+                                                     int by1, int* tile_flags,
+                                                     const int* __restrict__ order)
 {
-    const int t = (int)blockIdx.x;
+    const int t = order ? order[blockIdx.x] : (int)blockIdx.x;
     const int pidx = tiles[4 * t + 0];
     const dclaw_patch pt = patches[pidx];
     const dclaw_params P = params[pt.member];
@@ -422,6 +427,28 @@ __global__ __launch_bounds__(64) void k_src2_list(double* q, const double* __res
         const int k = (int)(task % nchunk) * 64 + (int)threadIdx.x;
         if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, q, aux, maux, mx, i0, jA, jB, k);
     }
+}
+
+// Permutation of the n tiles with the flagged ones first (flags from the
+// previous step's strip kernel): counters[0] counts flagged tiles from the
+// front, counters[1] the others from the back (both zero before the launch).
+__global__ void k_order_tiles(const int* __restrict__ flags, int n, int* order, int* counters)
+{
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool in = t < n;
+    const bool wet = in && flags[t] != 0;
+    const unsigned long long bw = __ballot(wet), bd = __ballot(in && !wet);
+    const int lane = (int)(threadIdx.x & 63u);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int base_w = 0, base_d = 0;
+    if (lane == 0) {
+        if (bw) base_w = atomicAdd(counters, __popcll(bw));
+        if (bd) base_d = atomicAdd(counters + 1, __popcll(bd));
+    }
+    base_w = __shfl(base_w, 0, 64);
+    base_d = __shfl(base_d, 0, 64);
+    if (wet) order[base_w + __popcll(bw & below)] = t;
+    else if (in) order[n - 1 - (base_d + __popcll(bd & below))] = t;
 }
 
 __global__ void k_zero_ints(int* a, int n)
@@ -872,14 +899,33 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     if (!tile_flags) {
         hipLaunchKernelGGL(k_stepgrid<MODE_STEP>, dim3(nstrips * nseg), dim3(64), 0, stream,
                            q_in, q_out, aux, aux_out, maux, mx, my, dx, dy, dt_dev, params_dev,
-                           cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, nullptr);
+                           cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, nullptr, nullptr);
         HIP_TRY(hipGetLastError());
         return DCLAW_OK;
     }
     const int ntiles = nstrips * nseg;
     if (n_flags < ntiles) return fail(DCLAW_EINVAL, "tile_flags holds fewer than dclaw_tile_count() ints");
     if (aux_out) return fail(DCLAW_EINVAL, "the split step does not write aux");
-    if ((long)n_flags >= 2L * ntiles + 2) {
+    if ((long)n_flags >= 2L * ntiles + 4 && g_split_mode == 1) {
+        // one strip launch with the previous step's wet tiles first
+        int* order = tile_flags + ntiles;
+        int* counters = tile_flags + 2 * ntiles + 2;
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, stream, counters, 2);
+        hipLaunchKernelGGL(k_order_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, stream,
+                           tile_flags, ntiles, order, counters);
+        hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips * nseg), dim3(64), 0,
+                           stream, q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev,
+                           params_dev, cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3,
+                           tile_flags, order);
+        HIP_TRY(hipGetLastError());
+        const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
+        hipLaunchKernelGGL(k_src2_tiles, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0,
+                           stream, q_out, aux, maux, mx, my, seg_rows, nstrips, ntiles,
+                           tile_flags, dt_dev, params_dev);
+        HIP_TRY(hipGetLastError());
+        return DCLAW_OK;
+    }
+    if ((long)n_flags >= 2L * ntiles + 2 && g_split_mode == 2) {
         // dry pass, strip body over the wet list, src2 over the wet list
         int* wet_list = tile_flags + ntiles;
         int* counters = tile_flags + 2 * ntiles;
@@ -903,7 +949,7 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     }
     hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips * nseg), dim3(64), 0, stream,
                        q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
-                       cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, tile_flags);
+                       cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, tile_flags, nullptr);
     HIP_TRY(hipGetLastError());
     const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
     hipLaunchKernelGGL(k_src2_tiles, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, stream,
@@ -968,8 +1014,16 @@ int dclaw_split_workspace_size(int mx, int my)
 {
     const int n = dclaw_tile_count(mx, my);
     if (n < 0) return n;
-    if (2L * n + 2 > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
-    return 2 * n + 2;
+    if (2L * n + 4 > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
+    return 2 * n + 4;
+}
+
+int dclaw_set_split_mode(int mode)
+{
+    if (mode < 0 || mode > 2) return fail(DCLAW_EINVAL, "split mode must be 0, 1 or 2");
+    const int prev = g_split_mode;
+    g_split_mode = mode;
+    return prev;
 }
 
 int dclaw_set_dry_rows(int rows)
@@ -1031,27 +1085,39 @@ int dclaw_worklist_build(const int* mx, const int* my, int npatch, int seg_rows,
     return n;
 }
 
-int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
-                       int ntiles, int seg_rows, const int* bc, const double* dt_dev,
-                       const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,
-                       void* stream)
+int dclaw_step_batched_ws(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
+                          int ntiles, int seg_rows, const int* bc, const double* dt_dev,
+                          const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,
+                          int n_ws, void* stream)
 {
     if (!patches_dev || !tiles_dev || !dt_dev || !params_dev || !cfl_dev)
         return fail(DCLAW_EINVAL, "null pointer");
     if (npatch < 1 || ntiles < 1 || seg_rows < 1) return fail(DCLAW_EINVAL, "empty batch");
+    if (tile_flags_dev && n_ws < ntiles)
+        return fail(DCLAW_EINVAL, "the workspace holds fewer ints than the work list has tiles");
     if (int rcb = check_bc(bc)) return rcb;
     const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
     const hipStream_t st = (hipStream_t)stream;
     if (!tile_flags_dev) {
         hipLaunchKernelGGL(k_step_batched<MODE_STEP>, dim3(ntiles), dim3(64), 0, st, patches_dev,
                            tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev, b0, b1, b2, b3,
-                           nullptr);
+                           nullptr, nullptr);
         HIP_TRY(hipGetLastError());
         return DCLAW_OK;
     }
+    const int* order = nullptr;
+    if ((long)n_ws >= 2L * ntiles + 4) {
+        // the tiles the previous call flagged first
+        int* ord = tile_flags_dev + ntiles;
+        int* counters = tile_flags_dev + 2 * ntiles + 2;
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, counters, 2);
+        hipLaunchKernelGGL(k_order_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, st,
+                           tile_flags_dev, ntiles, ord, counters);
+        order = ord;
+    }
     hipLaunchKernelGGL(k_step_batched<MODE_STEP_NOSRC2>, dim3(ntiles), dim3(64), 0, st,
                        patches_dev, tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev, b0, b1, b2,
-                       b3, tile_flags_dev);
+                       b3, tile_flags_dev, order);
     HIP_TRY(hipGetLastError());
     const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
     hipLaunchKernelGGL(k_src2_batched, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, st,
@@ -1059,6 +1125,15 @@ int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t
                        params_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
+}
+
+int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
+                       int ntiles, int seg_rows, const int* bc, const double* dt_dev,
+                       const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,
+                       void* stream)
+{
+    return dclaw_step_batched_ws(patches_dev, npatch, tiles_dev, ntiles, seg_rows, bc, dt_dev,
+                                 params_dev, cfl_dev, tile_flags_dev, ntiles, stream);
 }
 
 int dclaw_step_batched_tl(const dclaw_patch* patches_dev, const dclaw_patch* tangents_dev,

@@ -145,23 +145,31 @@ def src2(q: torch.Tensor, aux: torch.Tensor, mx: int, my: int, dt_dev: torch.Ten
         None if flag is None else _ptr(flag), _stream()))
 
 
+SPLIT_MODES = ("single", "ordered", "dry")
+
+
 def stepgrid_split(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int, mx: int,
                    my: int, dx: float, dy: float, bc, dt_dev: torch.Tensor,
                    params_dev: torch.Tensor, cfl_dev: torch.Tensor, flags: torch.Tensor,
-                   dry_pass: bool = True) -> None:
+                   mode: str = "ordered") -> None:
     """The step with src2 in a second pass over the wet tiles (bc None: stored
-    ghosts); flags is an int32 workspace of at least dclaw_tile_count ints.
-    With dry_pass and at least dclaw_split_workspace_size ints the dry tiles
-    are written by a separate light pass and the strip kernel runs over the
-    others only; dry_pass=False passes only dclaw_tile_count ints (the single
-    strip launch).  Both give the same result."""
+    ghosts); flags is an int32 workspace (kept between steps for "ordered").
+    mode "single": one strip launch in tile order (dclaw_tile_count ints
+    suffice); "ordered": one strip launch with the tiles the previous step
+    flagged first; "dry": a light dry pass writes the dry tiles and the strip
+    kernel runs over the others (both need dclaw_split_workspace_size ints).
+    All give the same result."""
+    if mode not in SPLIT_MODES:
+        raise ValueError(f"mode must be one of {SPLIT_MODES}")
     arr = None if bc is None else (ctypes.c_int * 4)(*bc)
     L = _lib.lib()
     n = flags.numel()
-    if not dry_pass:
+    if mode == "single":
         tc = L.dclaw_tile_count(mx, my)
         _lib.check(min(tc, 0))
         n = min(n, tc)
+    else:
+        _lib.check(min(0, L.dclaw_set_split_mode(1 if mode == "ordered" else 2)))
     _lib.check(L.dclaw_stepgrid_split_dev(
         _ptr(q_in), _ptr(q_out), _ptr(aux), maux, mx, my, dx, dy, arr, _ptr(dt_dev),
         _ptr(params_dev), _ptr(cfl_dev), _ptr(flags), n, _stream()))
@@ -210,9 +218,11 @@ class PatchRunner:
 
     def __init__(self, q: np.ndarray, aux: np.ndarray, params: DigParams, dx: float, dy: float,
                  bc=(BC_EXTRAP,) * 4, device="cuda", split_src2: bool = True,
-                 dry_pass: bool = True):
+                 split_mode: str = "ordered"):
         self.split_src2 = split_src2
-        self.dry_pass = dry_pass
+        if split_mode not in SPLIT_MODES:
+            raise ValueError(f"split_mode must be one of {SPLIT_MODES}")
+        self.split_mode = split_mode
         self.mx = q.shape[1] - 2 * MBC
         self.my = q.shape[2] - 2 * MBC
         self.dx, self.dy = float(dx), float(dy)
@@ -226,9 +236,9 @@ class PatchRunner:
         self.dt = torch.zeros(1, dtype=torch.float64, device=device)
         self.cfl = torch.zeros(1, dtype=torch.float64, device=device)
         # split-step workspace for any segment height (one row per tile at
-        # most): tile flags, the wet list and its two counters
+        # most): tile flags, the wet list or tile order, and counters
         nstrips = -(-self.mx // _lib.lib().dclaw_strip_width())
-        self.flags = torch.zeros(2 * nstrips * self.my + 2, dtype=torch.int32, device=device)
+        self.flags = torch.zeros(2 * nstrips * self.my + 4, dtype=torch.int32, device=device)
 
     def set_dt(self, dt: float) -> None:
         self.dt.fill_(float(dt))
@@ -238,7 +248,7 @@ class PatchRunner:
         the result is left in self.qa and cflgrid in self.cfl."""
         if fused_bc and self.split_src2:
             stepgrid_split(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
-                           self.bc, self.dt, self.params, self.cfl, self.flags, self.dry_pass)
+                           self.bc, self.dt, self.params, self.cfl, self.flags, self.split_mode)
         elif fused_bc:
             stepgrid_bc(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
                         self.bc, self.dt, self.params, self.cfl)
@@ -366,8 +376,9 @@ class Ensemble:
     a (patch, strip, segment) work list, each with its own parameters and dt."""
 
     def __init__(self, members: List, bc=(BC_EXTRAP,) * 4, seg_rows: int = 32, device="cuda",
-                 split_src2: bool = True):
+                 split_src2: bool = True, ordered: bool = True):
         self.split_src2 = split_src2
+        self.ordered = ordered
         self.n = len(members)
         self.seg_rows = seg_rows
         self.bc = tuple(bc)
@@ -392,7 +403,9 @@ class Ensemble:
         _lib.check(min(0, L.dclaw_worklist_build(mxa, mya, self.n, seg_rows, tiles)))
         self.ntiles = nt
         self.tiles = torch.tensor(list(tiles), dtype=torch.int32, device=device)
-        self.flags = torch.zeros(nt, dtype=torch.int32, device=device)
+        # src2 flags, plus (ordered) the tile order built from the previous
+        # step's flags and its counters
+        self.flags = torch.zeros(2 * nt + 4 if ordered else nt, dtype=torch.int32, device=device)
         self.dx = [m.patch.dx for m in members]
         self.dy = [m.patch.dy for m in members]
         self._patches = [self._patch_table(False, device), self._patch_table(True, device)]
@@ -414,10 +427,10 @@ class Ensemble:
         """One launch for all members; boundary conditions applied on load."""
         L = _lib.lib()
         bc = (ctypes.c_int * 4)(*self.bc)
-        _lib.check(L.dclaw_step_batched(
+        _lib.check(L.dclaw_step_batched_ws(
             _ptr(self._patches[1 if self.flip else 0]), self.n, _ptr(self.tiles), self.ntiles,
             self.seg_rows, bc, _ptr(self.dt), _ptr(self.params), _ptr(self.cfl),
-            _ptr(self.flags) if self.split_src2 else None, _stream()))
+            _ptr(self.flags) if self.split_src2 else None, self.flags.numel(), _stream()))
         self.flip = not self.flip
 
     def step_plain(self) -> None:

@@ -134,9 +134,19 @@ int dclaw_stepgrid_tl_dev(const double* q_in, const double* dq_in, double* q_out
  * height: the smallest tile_flags workspace of the split step below. */
 int dclaw_tile_count(int mx, int my);
 
-/* Workspace ints (2 * dclaw_tile_count(mx, my) + 2) with which the split step
- * below also runs its dry pass.  No reference counterpart. */
+/* Workspace ints (2 * dclaw_tile_count(mx, my) + 4) with which the split step
+ * below runs in the mode set by dclaw_set_split_mode.  No reference
+ * counterpart. */
 int dclaw_split_workspace_size(int mx, int my);
+
+/* Mode of the split step when its workspace holds dclaw_split_workspace_size
+ * ints: 1 = one strip launch with the tiles the previous call flagged first
+ * (the workspace carries the flags between calls; any contents give a valid
+ * order; the default), 2 = a separate dry pass first; with fewer ints (at
+ * least dclaw_tile_count) or mode 0: one strip launch in tile order.  Results
+ * do not depend on it; a library-wide setting like the segment height.
+ * Returns the previous mode or a negative error.  No reference counterpart. */
+int dclaw_set_split_mode(int mode);
 
 /* Rows per wavefront of the split step's dry pass (rounded down to whole
  * segments, at least one, at most 32); returns the previous value or a
@@ -152,11 +162,13 @@ int dclaw_set_dry_rows(int rows);
  * workspace of n_flags >= dclaw_tile_count(mx, my) ints, written every call
  * (no initialisation needed); its first dclaw_tile_count ints are the src2
  * flags of the tiles (segment * strips + strip).  With n_flags >=
- * dclaw_split_workspace_size(mx, my) a dry pass runs first: a light kernel
- * writes the dry state of every tile whose 1-cell neighbourhood is dry and
- * lists the others, and the strip kernel and src2 run over that list only
- * (the rest of the workspace holds the list and its counters).  Same result
- * either way.  No reference counterpart (a GPU scheduling choice). */
+ * dclaw_split_workspace_size(mx, my) the mode set by dclaw_set_split_mode
+ * applies: the dry pass (a light kernel writes the dry state of every tile
+ * whose 1-cell neighbourhood is dry and lists the others, and the strip
+ * kernel and src2 run over that list only) or the wet-first tile order (the
+ * previous call's flagged tiles dispatched first).  The rest of the
+ * workspace holds the list or order and counters.  Same result in every
+ * mode.  No reference counterpart (a GPU scheduling choice). */
 int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* aux, int maux,
                              int mx, int my, double dx, double dy, const int* bc,
                              const double* dt_dev, const dclaw_params* params_dev,
@@ -179,6 +191,16 @@ int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t
                        int ntiles, int seg_rows, const int* bc, const double* dt_dev,
                        const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,
                        void* stream);
+
+/* dclaw_step_batched with the workspace size given: with n_ws >= 2 * ntiles +
+ * 4 ints in tile_flags_dev the tiles the previous call flagged are
+ * dispatched first (keep the workspace between steps; any contents give a
+ * valid order), else as dclaw_step_batched.  Same result.  No reference
+ * counterpart (a GPU scheduling choice). */
+int dclaw_step_batched_ws(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
+                          int ntiles, int seg_rows, const int* bc, const double* dt_dev,
+                          const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,
+                          int n_ws, void* stream);
 
 /* The tangent-linear step (as dclaw_stepgrid_tl_dev) over a batched work
  * list (as dclaw_step_batched, src2 fused): tangents_dev[p] gives patch p's

@@ -81,13 +81,19 @@ def test_split_batched_equals_fused(dev):
         mx, my = int(rng.integers(10, 150)), int(rng.integers(10, 150))
         members.append(W.Member(slope_patch(mx, my, slope=0.7, phi_deg=25.0, seed=k), seed=k))
     a = S.Ensemble(members, seg_rows=16, split_src2=False)
-    b = S.Ensemble(members, seg_rows=16, split_src2=True)
+    b = S.Ensemble(members, seg_rows=16, split_src2=True)                 # wet-first order
+    c = S.Ensemble(members, seg_rows=16, split_src2=True, ordered=False)  # work-list order
     a.run(20, 1e-3, 0.9, 1e9)
     b.run(20, 1e-3, 0.9, 1e9)
+    c.run(20, 1e-3, 0.9, 1e9)
     dev.cuda.synchronize()
     for k in range(len(members)):
         assert np.array_equal(a.q(k), b.q(k)), k
-    assert dev.equal(a.dt, b.dt)
+        assert np.array_equal(a.q(k), c.q(k)), k
+    assert dev.equal(a.dt, b.dt) and dev.equal(a.dt, c.dt)
+    nt = b.ntiles
+    order = b.flags[nt:2 * nt].cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(nt))
 
 
 def _ent_patch():
@@ -161,7 +167,7 @@ def test_dry_pass_equals_fused(dev, oracle_mod, mx, my, seg, dry_rows, bc):
     try:
         ntiles = L.dclaw_tile_count(mx, my)
         nws = L.dclaw_split_workspace_size(mx, my)
-        assert nws == 2 * ntiles + 2
+        assert nws == 2 * ntiles + 4
         pdev = S.params_tensor([pt.params])
         dtd = dev.tensor([dt], dtype=dev.float64, device="cuda")
         bed = S.bed_tensor(aux)
@@ -175,7 +181,7 @@ def test_dry_pass_equals_fused(dev, oracle_mod, mx, my, seg, dry_rows, bc):
         else:
             S.stepgrid_bc(qi, qf, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cf)
         ws = dev.full((nws,), -1, dtype=dev.int32, device="cuda")
-        S.stepgrid_split(qi, qs, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cs, ws)
+        S.stepgrid_split(qi, qs, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cs, ws, mode="dry")
         dev.cuda.synchronize()
     finally:
         L.dclaw_set_seg_rows(prev)
@@ -199,9 +205,9 @@ def test_dry_pass_equals_fused(dev, oracle_mod, mx, my, seg, dry_rows, bc):
 
 
 def test_dry_pass_runner_long(dev):
-    """150 steps of the runner on a 700 x 520 fractal patch with 64 sources,
-    with and without the dry pass: identical states, dt and wet lists that
-    grow as the flow spreads."""
+    """150 steps of the runner on a 700 x 520 fractal patch with 64 sources
+    in each split mode (dry pass, one launch in tile order, one launch with
+    the previous step's wet tiles first): identical states and dt."""
     from dclaw_amd import _lib
     from dclaw_amd import step as S
     from dclaw_amd import workloads as W
@@ -209,19 +215,27 @@ def test_dry_pass_runner_long(dev):
     L = _lib.lib()
     prev = L.dclaw_set_seg_rows(8)
     try:
-        ra = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, dry_pass=True)
-        rb = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, dry_pass=False)
+        ra = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_mode="dry")
+        rb = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_mode="single")
+        rc = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_mode="ordered")
         ra.run(150, 1e-3, 0.9, 1e9)
         rb.run(150, 1e-3, 0.9, 1e9)
+        rc.run(150, 1e-3, 0.9, 1e9)
         dev.cuda.synchronize()
         ntiles = L.dclaw_tile_count(700, 520)
         nwet = int(ra.flags[2 * ntiles].item())
+        order = rc.flags[ntiles:2 * ntiles].cpu().numpy()
+        nfirst = int(rc.flags[2 * ntiles + 2].item())
     finally:
         L.dclaw_set_seg_rows(prev)
-    qa, qb = ra.q(), rb.q()
+    qa, qb, qc = ra.q(), rb.q(), rc.q()
     assert np.array_equal(qa, qb), f"{(qa != qb).sum()} values differ"
-    assert ra.dt.item() == rb.dt.item()
+    assert np.array_equal(qa, qc), f"{(qa != qc).sum()} values differ"
+    assert ra.dt.item() == rb.dt.item() == rc.dt.item()
     assert 0 < nwet < ntiles
+    # the last step's order: a permutation of the tiles, flagged ones first
+    assert np.array_equal(np.sort(order), np.arange(ntiles))
+    assert 0 < nfirst < ntiles
 
 
 def test_tuner_restores_state(dev):
@@ -239,9 +253,60 @@ def test_tuner_restores_state(dev):
         r.step()
         q0, dt0 = r.q(), r.dt.item()
         h, times = autotune.tune_patch(r, candidates=(4, 8), reps=2)
-        assert set(times) == {(4, True), (8, True), (4, False), (8, False)}
-        assert (h, r.dry_pass) == min(times, key=times.get)
+        assert set(times) == {(h_, m) for h_ in (4, 8) for m in ("single", "ordered", "dry")}
+        assert (h, r.split_mode) == min(times, key=times.get)
         assert L.dclaw_set_seg_rows(h) == h
         assert np.array_equal(r.q(), q0) and r.dt.item() == dt0
     finally:
         L.dclaw_set_seg_rows(prev)
+
+
+@pytest.mark.parametrize("mx,my,seg,bc", [(130, 70, 8, (3, 3, 3, 3)), (59, 33, 4, (1, 3, 3, 1)),
+                                          (600, 500, 6, (1, 1, 1, 1)), (7, 90, 32, None)])
+def test_ordered_equals_fused(dev, oracle_mod, mx, my, seg, bc):
+    """The split step with the wet-first tile order, starting from arbitrary
+    flags in the workspace, equals the fused step on every cell; the order it
+    used is a permutation of the tiles with the previously flagged first."""
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    O = oracle_mod
+    pt = slope_patch(mx, my, slope=0.8, phi_deg=25.0, seed=mx + my)
+    q, aux, dt = advance_oracle(O, pt, 6, bc=bc or (1, 1, 1, 1))
+    print(f"[ordered] {mx}x{my} seg {seg} bc {bc}", flush=True)
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(seg)
+    try:
+        ntiles = L.dclaw_tile_count(mx, my)
+        nws = L.dclaw_split_workspace_size(mx, my)
+        rng = np.random.default_rng(mx)
+        prev_flags = (rng.random(ntiles) < 0.3).astype(np.int32)
+        ws = dev.full((nws,), -7, dtype=dev.int32, device="cuda")
+        ws[:ntiles] = dev.from_numpy(prev_flags).to("cuda")
+        pdev = S.params_tensor([pt.params])
+        dtd = dev.tensor([dt], dtype=dev.float64, device="cuda")
+        bed = S.bed_tensor(aux)
+        qi = S.to_device_q(q)
+        qf = dev.zeros_like(qi)
+        qs = dev.full_like(qi, float("nan"))
+        cf = dev.zeros(1, dtype=dev.float64, device="cuda")
+        cs = dev.zeros(1, dtype=dev.float64, device="cuda")
+        if bc is None:
+            S.stepgrid(qi, qf, bed, 1, mx, my, pt.dx, pt.dy, dtd, pdev, cf)
+        else:
+            S.stepgrid_bc(qi, qf, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cf)
+        S.stepgrid_split(qi, qs, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cs, ws,
+                         mode="ordered")
+        dev.cuda.synchronize()
+    finally:
+        L.dclaw_set_seg_rows(prev)
+    a, b = S.to_host_q(qs), S.to_host_q(qf)
+    assert not np.isnan(a).any()
+    assert np.array_equal(a, b), f"{(a != b).sum()} values differ"
+    assert cs.item() == cf.item()
+    w = ws.cpu().numpy()
+    order = w[ntiles:2 * ntiles]
+    nfirst = int(w[2 * ntiles + 2])
+    assert np.array_equal(np.sort(order), np.arange(ntiles))
+    assert nfirst == int(prev_flags.sum())
+    assert set(order[:nfirst].tolist()) == set(np.flatnonzero(prev_flags).tolist())
+    assert set(w[:ntiles].tolist()) <= {0, 1}
