@@ -71,6 +71,10 @@ def parse():
     ap.add_argument("--tangent-steps", type=int, default=20,
                     help="timed steps of the tangent-linear step (forward + d/dphi) on the same "
                          "workload, reported beside the forward figures (0 disables)")
+    ap.add_argument("--adjoint-steps", type=int, default=3,
+                    help="timed adjoint (reverse-mode) steps on the same workload, gradient "
+                         "with respect to the state and (phi, kref), reported beside the "
+                         "forward figures (0 disables)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy (the HBM "
                          "counter calibration dispatch for rocprofv3 --pmc runs)")
@@ -221,6 +225,12 @@ def main():
         del tr
         print(f"[bench] tangent-linear step {tms:.3f} ms", file=sys.stderr, flush=True)
 
+    # adjoint step (record pass + reverse pass: lambda_n and d/d(phi, kref))
+    # on the same workload, from the initial state
+    adjoint = None
+    if args.adjoint_steps > 0 and not args.fused:
+        adjoint = time_adjoint(pt, args.adjoint_steps, cells, kern_avg_s)
+
     # FP64 VALU work of one step from the committed counter summary
     # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, FMA counted twice)
     fp64 = None
@@ -266,8 +276,8 @@ def main():
                                 f"plain drop-in GPU step, single {n}x{n} patch, dx=2 m, "
                                 f"seeded fractal bed + 20 deg tilt, 64 elliptical sources; "
                                 f"value = forward step; the tangent-linear step (forward-mode "
-                                f"derivative) is timed separately under 'tangent'; a "
-                                f"reverse-mode adjoint is not built"),
+                                f"derivative) is timed separately under 'tangent', the "
+                                f"reverse-mode adjoint step under 'adjoint'"),
                    "mx": n, "my": n, "seed": args.seed, "cfl_desired": cfl_desired,
                    "wet_cells_end": wet, "active_cells_end": active, "seg_rows": seg,
                    "src2": "fused" if args.fused else "split",
@@ -284,21 +294,71 @@ def main():
                      "kernel_avg_ms": kern_avg_s * 1e3},
         "fp64": fp64,
         "tangent": tangent,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
+def time_adjoint(pt, steps, cells, fwd_s):
+    """HIP-event time of the adjoint step of one patch (lambda = a seeded
+    random field, directions phi and kref), after one untimed call."""
+    import numpy as np
+    import torch
+
+    from dclaw_amd import adjoint as AD
+    from dclaw_amd import step as S
+    from dclaw_amd import tangent as T
+
+    mx, my = pt.q.shape[1] - 4, pt.q.shape[2] - 4
+    qi = S.to_device_q(pt.q)
+    lam = torch.randn(qi.shape, dtype=torch.float64, device="cuda",
+                      generator=torch.Generator(device="cuda").manual_seed(3))
+    lo = torch.empty_like(qi)
+    bed, maux = S.aux_planes(pt.aux, pt.params)
+    work = AD.new_workspace(mx, my)
+    dt = torch.full((1,), 1e-3, dtype=torch.float64, device="cuda")
+    par = S.params_tensor([pt.params])
+    dpar = torch.cat([T._record_tensor(T.param_direction(phi=1.0)),
+                      T._record_tensor(T.param_direction(kref=1.0))])
+    grad = torch.zeros(2, dtype=torch.float64, device="cuda")
+
+    def one():
+        AD.adjoint_step(qi, bed, maux, mx, my, pt.dx, pt.dy, (S.BC_EXTRAP,) * 4, dt, par, dpar, 2,
+                        lam, lo, None, grad, work)
+    one()
+    torch.cuda.synchronize()
+    ta, tb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ta.record()
+    for _ in range(steps):
+        one()
+    tb.record()
+    torch.cuda.synchronize()
+    ams = ta.elapsed_time(tb) / steps
+    pair_s = fwd_s + ams * 1e-3
+    del work
+    torch.cuda.empty_cache()
+    print(f"[bench] adjoint step {ams:.3f} ms", file=sys.stderr, flush=True)
+    return {"ms_per_step": ams, "value": cells / (ams * 1e-3), "unit": "cell-updates/s",
+            "fwd_plus_adjoint_value": cells / pair_s, "steps": steps,
+            "directions": "state, phi, kref",
+            "workspace_bytes": AD.workspace_doubles(mx, my) * 8,
+            "kernel": "k_adjoint<0..12> (record pass + reverse pass) + k_adjoint_gp"}
+
+
 def bench_flume(args):
     """Gradient workload: a 2000 x 40 flume release (dx = 0.05 m, 31 then 2
     deg), 8 seeded gauges, loss = sum over steps and gauges of (h - target)^2
-    with a seeded synthetic target; the forward run timed, then the gradient
-    with respect to phi, kref and m0 by three tangent-linear runs (forward
-    mode; the reverse-mode adjoint is not built)."""
+    with a seeded synthetic target.  Timed: the forward run; the gradient with
+    respect to phi, kref and the whole initial state (so m0 and the initial
+    depth field) by one checkpointed reverse sweep (checkpoint every 100
+    steps); and the same phi, kref, m0 derivatives by three tangent-linear
+    runs, for comparison."""
     import numpy as np
     import torch
 
     from dclaw_amd import _lib
+    from dclaw_amd import adjoint as AD
     from dclaw_amd import step as S
     from dclaw_amd import tangent as T
     from dclaw_amd import workloads as W
@@ -322,6 +382,17 @@ def bench_flume(args):
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
 
+    # reverse sweep: loss, d/dphi, d/dkref and dloss/dq0 in one pass
+    AD.gauge_loss_gradient_adjoint(pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy,
+                                   targets[:5], parameters=("phi", "kref"))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss_a, grads_a, lam0, _ = AD.gauge_loss_gradient_adjoint(
+        pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy, targets, parameters=("phi", "kref"),
+        checkpoint_every=100)
+    t_adj = time.perf_counter() - t0
+    grads_a["m0"] = float((lam0[3] * pt.q[0]).sum())
+
     dq_m0 = np.zeros_like(pt.q)
     dq_m0[3] = pt.q[0]
     dirs = {"phi": (T.param_direction(phi=1.0), None),
@@ -333,16 +404,18 @@ def bench_flume(args):
     loss, grads, _ = T.gauge_loss_gradient(pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy,
                                            targets, dirs)
     t_grad = time.perf_counter() - t0
-    print(f"[bench] flume forward {nsteps} steps {t_fwd:.3f}s, gradient (3 tangent runs) "
-          f"{t_grad:.3f}s, loss {loss:.6e}, grads {grads}", file=sys.stderr, flush=True)
+    rel = {k: abs(grads_a[k] - grads[k]) / max(abs(grads[k]), 1e-300) for k in grads}
+    print(f"[bench] flume forward {nsteps} steps {t_fwd:.3f}s, reverse gradient {t_adj:.3f}s, "
+          f"tangent gradient (3 runs) {t_grad:.3f}s, loss {loss:.6e}, grads {grads_a}, "
+          f"adjoint vs tangent rel {rel}", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
-        "value": 3 * nsteps * cells / t_grad,
+        "value": nsteps * cells / t_adj,
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": nsteps,
         "warmup": 5,
-        "ms_per_step": t_grad / (3 * nsteps) * 1e3,
+        "ms_per_step": t_adj / nsteps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -350,14 +423,18 @@ def bench_flume(args):
         "data": "synthetic",
         "config": {"workload": f"gradient of a gauge misfit over a {pt.mx}x{pt.my} flume release "
                                f"(dx=0.05 m, 31 then 2 deg, 0.75 m block, 8 seeded gauges, "
-                               f"seeded synthetic targets) w.r.t. phi, kref, m0: value = "
-                               f"tangent-linear step-updates/s over the 3 forward-mode runs "
-                               f"(reverse-mode adjoint not built)",
-                   "mx": pt.mx, "my": pt.my, "seg_rows": seg, "seed": args.seed},
+                               f"seeded synthetic targets) w.r.t. phi, kref and the initial "
+                               f"state: value = cell-updates/s of the whole fwd+adjoint "
+                               f"gradient (forward run, checkpoint recompute, reverse sweep)",
+                   "mx": pt.mx, "my": pt.my, "seg_rows": seg, "seed": args.seed,
+                   "checkpoint_every": 100},
         "forward": {"seconds": t_fwd, "value": nsteps * cells / t_fwd,
                     "ms_per_step": t_fwd / nsteps * 1e3},
-        "gradient": {"seconds": t_grad, "loss": loss, "dloss": grads,
+        "gradient": {"seconds": t_adj, "loss": loss_a, "dloss": grads_a,
+                     "method": "reverse sweep (one forward run, recompute, adjoint steps)",
                      "units": {"phi": "per radian", "kref": "per m^2", "m0": "per unit m0"}},
+        "tangent_gradient": {"seconds": t_grad, "loss": loss, "dloss": grads,
+                             "rel_diff_vs_adjoint": rel},
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -458,7 +535,8 @@ def bench_ensemble(args):
                                f"log-uniform in [64,512], per-member phi/m0/kref, one batched "
                                f"launch per step (plus the src2 pass over wet tiles unless "
                                f"fused); the tangent-linear batched step is timed under "
-                               f"'tangent'; reverse-mode adjoint not built",
+                               f"'tangent'; the adjoint is per patch (timed on the patch and "
+                               f"flume workloads), not batched",
                    "members": len(members),
                    "cells": cells, "seed": args.seed, "seg_rows": seg,
                    "src2": "fused" if args.fused else "split",

@@ -73,6 +73,11 @@ SIGNATURES = {
                                              _vp]),
     "dclaw_stepgrid_tl_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _ip, _vp,
                                              _vp, _vp, _vp, _vp]),
+    "dclaw_adjoint_workspace_doubles": (ctypes.c_size_t, [_i, _i]),
+    "dclaw_adjoint_step_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _d, _d, _ip, _vp, _vp, _vp, _i,
+                                              _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "dclaw_gauges_adj_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _vp, _i, _vp,
+                                            _vp, _vp]),
     "dclaw_amr_average_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i,
                                              _i, _i, _i, _i, _i, _d, _vp]),
     "dclaw_amr_interp_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _i,

@@ -1,0 +1,275 @@
+"""Reverse-mode (adjoint) derivatives of the fused step on the GPU.
+
+``dclaw_adjoint_step_dev`` maps lambda_{n+1} (the gradient of a loss with
+respect to q_{n+1}) to lambda_n = J_q^T lambda_{n+1} and gives the gradient
+contribution of up to four parameter directions, where J is the Jacobian of
+the fused step (b4step2, step2 with rpn2/rpt2/flux2, update, src2) at q_n for
+the step's dt (the dt sequence is the one the values choose and is not
+differentiated, as for a fixed time grid).  J is the tangent-linear step's
+(``tangent.py``): the adjoint is its transpose, and one reverse sweep gives
+the gradient of a scalar loss with respect to the whole initial state and
+the parameters together.
+
+A run of N steps is reversed with checkpoints: the forward pass keeps the
+state every ``checkpoint_every`` steps; the reverse pass recomputes each
+block's states from its checkpoint (bit for bit: the step is deterministic)
+and runs the adjoint steps back through it.
+
+``rollout`` is the ``torch.autograd.Function`` form: gauge records and the
+final state of an N-step run as functions of the initial state and of named
+parameters, differentiable by ``torch.autograd``."""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .params import CParams, DigParams, to_c
+from .step import (BC_EXTRAP, _ptr, _stream, aux_planes, dt_update, gauges, stepgrid_bc,
+                   to_device_q, to_host_q)
+from .tangent import PARAM_FIELDS, _record_tensor, param_direction
+
+MAX_DIRECTIONS = 4
+
+
+def workspace_doubles(mx: int, my: int) -> int:
+    """Doubles of device scratch one adjoint step of an mx x my patch needs."""
+    return int(_lib.lib().dclaw_adjoint_workspace_doubles(mx, my))
+
+
+def new_workspace(mx: int, my: int, device="cuda") -> torch.Tensor:
+    """A zero-filled adjoint workspace for mx x my patches."""
+    return torch.zeros(workspace_doubles(mx, my), dtype=torch.float64, device=device)
+
+
+def adjoint_step(q_in: torch.Tensor, aux: torch.Tensor, maux: int, mx: int, my: int, dx: float,
+                 dy: float, bc, dt_dev: torch.Tensor, params_dev: torch.Tensor,
+                 dparams_dev: Optional[torch.Tensor], np_dirs: int, lam_out: torch.Tensor,
+                 lam_in: torch.Tensor, q_out: Optional[torch.Tensor],
+                 grad_dev: Optional[torch.Tensor], work: torch.Tensor) -> None:
+    """One adjoint step on device buffers (see include/dclaw_hip.h).  work
+    must be zero-filled before its first use (``new_workspace``) and passed
+    unchanged between calls for the same mx, my."""
+    arr = None if bc is None else (ctypes.c_int * 4)(*bc)
+    _lib.check(_lib.lib().dclaw_adjoint_step_dev(
+        _ptr(q_in), _ptr(aux), maux, mx, my, dx, dy, arr, _ptr(dt_dev), _ptr(params_dev),
+        None if dparams_dev is None else _ptr(dparams_dev), np_dirs, _ptr(lam_out), _ptr(lam_in),
+        None if q_out is None else _ptr(q_out), None if grad_dev is None else _ptr(grad_dev),
+        _ptr(work), work.numel(),
+        _stream()))
+
+
+def gauges_adj(q: torch.Tensor, aux: torch.Tensor, maux: int, mx: int, my: int, xlower: float,
+               ylower: float, dx: float, dy: float, dry_tolerance: float, xy_dev: torch.Tensor,
+               lam_gauges: torch.Tensor, lam_q: torch.Tensor) -> None:
+    """lam_q += (d gauges / dq)^T lam_gauges (ngauges x 8)."""
+    _lib.check(_lib.lib().dclaw_gauges_adj_dev(
+        _ptr(q), _ptr(aux), maux, mx, my, xlower, ylower, dx, dy, dry_tolerance, _ptr(xy_dev),
+        xy_dev.shape[0], _ptr(lam_gauges), _ptr(lam_q), _stream()))
+
+
+def _directions(names: Sequence[str]) -> List[CParams]:
+    if len(names) > MAX_DIRECTIONS:
+        raise ValueError(f"at most {MAX_DIRECTIONS} parameters per run")
+    for n in names:
+        if n not in PARAM_FIELDS:
+            raise ValueError(f"{n} is not a floating field of dclaw_params")
+    return [param_direction(**{n: 1.0}) for n in names]
+
+
+@dataclass
+class RolloutSpec:
+    """A fixed-patch run: the bed (aux), grid, parameter record, boundary
+    conditions, dt rule (dt0, cfl_desired, dt_max), step count and gauges
+    (x, y pairs; none for a run without gauge records).  ``parameters`` names
+    the dclaw_params fields the run is differentiated with respect to (their
+    values come from ``theta`` in ``rollout``)."""
+    aux: np.ndarray
+    params: DigParams
+    dx: float
+    dy: float
+    nsteps: int
+    dt0: float = 1e-3
+    cfl_desired: float = 0.9
+    dt_max: float = 1e9
+    bc: Tuple[int, int, int, int] = (BC_EXTRAP,) * 4
+    gauges_xy: Optional[np.ndarray] = None
+    xlower: float = 0.0
+    ylower: float = 0.0
+    parameters: Tuple[str, ...] = ()
+    checkpoint_every: int = 100
+    device: str = "cuda"
+    _dev: dict = field(default_factory=dict, repr=False)
+
+
+class Rollout:
+    """N steps of the fused step with gauge records, and their reverse pass.
+
+    forward(q0, record) runs the device loop (boundary conditions on load,
+    device dt rule), keeping a checkpoint every ``checkpoint_every`` steps and
+    the dt of every step; it returns the gauge records (nsteps, ngauges, 8)
+    and the final state.  backward(g_series, g_final) runs the adjoint back
+    through the run: returns lambda_0 (the gradient with respect to q0, layout
+    of q; ghost entries 0) and the gradient with respect to each named
+    parameter."""
+
+    def __init__(self, spec: RolloutSpec):
+        if spec.nsteps < 1:
+            raise ValueError("nsteps must be >= 1")
+        if spec.checkpoint_every < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.spec = spec
+        dev = spec.device
+        aux = np.asfortranarray(spec.aux)
+        self.mx, self.my = aux.shape[1] - 4, aux.shape[2] - 4
+        self.aux, self.maux = aux_planes(aux, spec.params, dev)
+        self.dirs = _directions(spec.parameters)
+        self.np = len(self.dirs)
+        self.dparams = (torch.cat([_record_tensor(d, dev) for d in self.dirs])
+                        if self.dirs else None)
+        # zero-filled before the first call; the library keeps it so
+        self.work = new_workspace(self.mx, self.my, dev)
+        self.xy = None
+        self.ng = 0
+        if spec.gauges_xy is not None:
+            xy = np.asarray(spec.gauges_xy, dtype=np.float64).reshape(-1, 2)
+            self.xy = torch.from_numpy(np.ascontiguousarray(xy)).to(dev)
+            self.ng = xy.shape[0]
+
+    # parameter record with the named fields set to theta
+    def record(self, theta: Optional[Sequence[float]] = None) -> torch.Tensor:
+        c = to_c(self.spec.params)
+        if theta is not None:
+            if len(theta) != self.np:
+                raise ValueError("one value per named parameter")
+            for n, v in zip(self.spec.parameters, theta):
+                setattr(c, n, float(v))
+        return _record_tensor(c, self.spec.device)
+
+    def forward(self, q0: torch.Tensor, params_dev: torch.Tensor):
+        s = self.spec
+        self.params_dev = params_dev
+        dev = q0.device
+        qa = q0.detach().clone()
+        qb = torch.empty_like(qa)
+        dt = torch.full((1,), float(s.dt0), dtype=torch.float64, device=dev)
+        cfl = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.dts = torch.zeros(s.nsteps, dtype=torch.float64, device=dev)
+        self.ckpt: Dict[int, torch.Tensor] = {0: qa.clone()}
+        series = torch.zeros((s.nsteps, max(self.ng, 1), 8), dtype=torch.float64, device=dev)
+        for n in range(s.nsteps):
+            self.dts[n].copy_(dt[0])
+            stepgrid_bc(qa, qb, self.aux, self.maux, self.mx, self.my, s.dx, s.dy, s.bc, dt,
+                        params_dev, cfl)
+            qa, qb = qb, qa
+            dt_update(dt, cfl, s.cfl_desired, s.dt_max)
+            if self.ng:
+                gauges(qa, self.aux, self.maux, self.mx, self.my, s.xlower, s.ylower, s.dx, s.dy,
+                       s.params.dry_tolerance, self.xy, series[n])
+            if (n + 1) % s.checkpoint_every == 0 and n + 1 < s.nsteps:
+                self.ckpt[n + 1] = qa.clone()
+        return series[:, :self.ng], qa
+
+    def backward(self, g_series: Optional[torch.Tensor], g_final: Optional[torch.Tensor]):
+        s = self.spec
+        N = s.nsteps
+        dev = self.dts.device
+        q0 = self.ckpt[0]
+        lam = (g_final.detach().clone().contiguous() if g_final is not None
+               else torch.zeros_like(q0))
+        lam_b = torch.empty_like(lam)
+        grad = torch.zeros(max(self.np, 1), dtype=torch.float64, device=dev)
+        gstep = torch.zeros_like(grad)
+        cfl = torch.zeros(1, dtype=torch.float64, device=dev)
+        dt = torch.zeros(1, dtype=torch.float64, device=dev)
+        starts = sorted(self.ckpt)
+        for b0 in reversed(starts):
+            b1 = min(b0 + s.checkpoint_every, N)
+            # states q_{b0} .. q_{b1} of the block, recomputed from its checkpoint
+            states = [self.ckpt[b0]]
+            for n in range(b0, b1):
+                dt.copy_(self.dts[n:n + 1])
+                x = torch.empty_like(q0)
+                stepgrid_bc(states[-1], x, self.aux, self.maux, self.mx, self.my, s.dx, s.dy,
+                            s.bc, dt, self.params_dev, cfl)
+                states.append(x)
+            for n in range(b1 - 1, b0 - 1, -1):
+                # step n+1 maps q_n -> q_{n+1}; its gauge record's adjoint joins first
+                if self.ng and g_series is not None:
+                    gauges_adj(states[n + 1 - b0], self.aux, self.maux, self.mx, self.my,
+                               s.xlower, s.ylower, s.dx, s.dy, s.params.dry_tolerance, self.xy,
+                               g_series[n].detach().contiguous(), lam)
+                dt.copy_(self.dts[n:n + 1])
+                adjoint_step(states[n - b0], self.aux, self.maux, self.mx, self.my, s.dx, s.dy,
+                             s.bc, dt, self.params_dev, self.dparams, self.np, lam, lam_b,
+                             None, gstep if self.np else None, self.work)
+                if self.np:
+                    grad += gstep
+                lam, lam_b = lam_b, lam
+            del states
+        return lam, grad[:self.np]
+
+
+class _RolloutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, q0, theta, spec):
+        r = Rollout(spec)
+        params_dev = r.record(theta.detach().cpu().tolist() if theta is not None else None)
+        series, qf = r.forward(q0, params_dev)
+        ctx.r = r
+        ctx.theta_device = theta.device if theta is not None else None
+        return series, qf
+
+    @staticmethod
+    def backward(ctx, g_series, g_final):
+        lam0, grad = ctx.r.backward(g_series, g_final)
+        ctx.r = None
+        gtheta = grad.to(ctx.theta_device) if ctx.theta_device is not None else None
+        return lam0, gtheta, None
+
+
+def rollout(q0: torch.Tensor, theta: Optional[torch.Tensor], spec: RolloutSpec):
+    """Differentiable N-step run: (gauge records (nsteps, ngauges, 8), final
+    state) as functions of q0 (device tensor in the layout of
+    ``step.to_device_q``) and theta (float64 tensor, one value per name in
+    ``spec.parameters``, in the record's units: phi in radians, kref in m^2).
+    The backward pass is the checkpointed adjoint."""
+    if theta is not None and theta.numel() != len(spec.parameters):
+        raise ValueError("theta needs one value per named parameter")
+    return _RolloutFn.apply(q0, theta, spec)
+
+
+def gauge_loss_gradient_adjoint(q: np.ndarray, aux: np.ndarray, params: DigParams, dx: float,
+                                dy: float, xlower: float, ylower: float, gauges_xy,
+                                targets: np.ndarray, parameters: Sequence[str] = ("phi", "kref"),
+                                dt0: float = 1e-3, cfl_desired: float = 0.9, dt_max: float = 1e9,
+                                component: int = 0, bc=(BC_EXTRAP,) * 4,
+                                checkpoint_every: int = 100):
+    """The gauge misfit of ``tangent.gauge_loss_gradient`` (sum over steps and
+    gauges of (gauge value[component] - target)^2) and its gradient by one
+    reverse sweep: returns (loss, {parameter: dloss/dparameter}, dloss/dq0 as
+    a host array in q's layout (ghost entries 0), dts)."""
+    targets = np.asarray(targets, dtype=np.float64)
+    nsteps, ng = targets.shape
+    spec = RolloutSpec(aux=aux, params=params, dx=dx, dy=dy, nsteps=nsteps, dt0=dt0,
+                       cfl_desired=cfl_desired, dt_max=dt_max, bc=tuple(bc),
+                       gauges_xy=np.asarray(gauges_xy, dtype=np.float64).reshape(-1, 2),
+                       xlower=xlower, ylower=ylower, parameters=tuple(parameters),
+                       checkpoint_every=checkpoint_every)
+    if spec.gauges_xy.shape[0] != ng:
+        raise ValueError("targets must have one column per gauge")
+    r = Rollout(spec)
+    series, _ = r.forward(to_device_q(q), r.record())
+    tgt = torch.tensor(targets, device=series.device)
+    res = series[:, :, component] - tgt
+    loss = float((res * res).sum().item())
+    g = torch.zeros_like(series)
+    g[:, :, component] = 2.0 * res
+    lam0, grad = r.backward(g, None)
+    torch.cuda.synchronize()
+    grads = {n: float(v) for n, v in zip(parameters, grad.cpu().tolist())}
+    return loss, grads, to_host_q(lam0), r.dts.cpu().numpy()

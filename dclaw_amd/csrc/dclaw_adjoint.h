@@ -1,0 +1,960 @@
+// dclaw_adjoint.h -- the reverse-mode (adjoint) step.
+//
+// Given lambda_{n+1} (the gradient of a loss with respect to q_{n+1}, same
+// layout as q), the adjoint step returns lambda_n = J_q^T lambda_{n+1} and,
+// for each parameter direction dP_k, g_k = lambda_{n+1} . J_P dP_k, where J is
+// the Jacobian of the fused step (b4step2, step2, update, src2) at q_n with
+// dt fixed.  J is exactly the tangent-linear step's (dclaw_tangent.h): at a
+// branch of the scheme the adjoint is the transpose of the active branch's
+// derivative, and the dot-product identity
+//     <lambda_{n+1}, J dq> + sum_k g_k a_k = <lambda_n, dq>  (dP = sum a_k dP_k)
+// holds to rounding.
+//
+// The step is laid out over the ghosted grid as stages, each a function of a
+// few values of its own cell or edge (the order and index ranges of the
+// strip body, dclaw_strip_body.inc):
+//   F1 prepare   cell  q (boundary map on load) -> b4step2 state Q0, the two
+//                      qfix passes of rpn2 (F1, F2) and setvars of both
+//   F2 taudir    cell  calc_taudir from the prepared states of the cell and
+//                      its 4 neighbours
+//   F3 waves     edge  rpn2 (x-edges between (i-1,j) and (i,j), y-edges
+//                      between (i,k-1) and (i,k))
+//   F4 flux      edge  limiter, second-order correction, fluctuations
+//                      (flux2's edge body, neighbours' waves through the
+//                      limiter's dot products)
+//   F5 split     cell  transverse rpt2 splits of the neighbouring edges'
+//                      fluctuations
+//   F6 update    cell  assembly of fm, fp, gm, gp in step2's order, the
+//                      conservative update and src2
+// The record pass runs F1..F6 and keeps every stage's inputs in a
+// workspace; its q_out is the step's result bit for bit.  The reverse pass
+// runs the stages backwards (B1 src2, B2 splits, B3 flux, B4 waves, B5
+// taudir, B6 prepare, B7 the boundary map), each computing the
+// vector-Jacobian product of its stage from the stage's local Jacobian:
+// one tangent-linear evaluation of the stage per input, seeded with that
+// input (and one per parameter direction).  Every stage output has exactly
+// one consumer slot, so each reverse kernel gathers from its neighbours'
+// slots and writes only its own cell: no atomics, and the result does not
+// depend on scheduling.
+#pragma once
+
+#include "dclaw_tangent.h"
+
+namespace dclaw_adj {
+
+using dclaw_tl::Dual;
+using TP = dclaw_tl::Params;
+using dclaw::aidx;
+using dclaw::bc_source;
+using dclaw::qidx;
+
+// prepared-cell fields
+enum {
+    PC_H, PC_HU, PC_HV, PC_BDIF, PC_HM0, PC_P0, PC_HCHI0,
+    PC_HM1, PC_P1, PC_HCHI1, PC_M1, PC_CHI1, PC_U, PC_V,
+    PC_HM2, PC_P2, PC_HCHI2, PC_M2, PC_CHI2,
+    PC_RHO1, PC_TAU1, PC_RHO2, PC_TAU2,
+    NPC
+};
+// rpn2 side inputs (side's own orientation: u normal)
+enum {
+    SD_H, SD_HU, SD_HV, SD_HM, SD_P, SD_HCHI, SD_U, SD_V, SD_M, SD_CHI, SD_BDIF, SD_TAUDIR,
+    SD_FS, SD_RHO, SD_TAU,
+    NSD
+};
+constexpr int NW = 17;      // wave record: s[3], w1[6], w5[6], w2h, w2n
+constexpr int NEF = 24;     // faddm[6], faddp[6], amdq[6], apdq[6]
+constexpr int NEI = 23;     // flux inputs: waves, dotl[3], dotr[3]
+constexpr int NTI = 37;     // taudir inputs: 5 x (Q0 state 6 + bdif), rho1, tau1 of the centre
+constexpr int NSI = 12;     // split inputs: h, hu, hv, hm2, hchi2, bdif, asdq[6]
+constexpr int MAXP = 4;     // parameter directions per call
+
+// Workspace fields (each a plane of N = (mx+4)(my+4) doubles, cell (i, j)
+// at (i+1) + (mx+4)(j+1); an edge's values sit at its right/top cell).
+enum {
+    W_PC = 0,                 // prepared cell (NPC)
+    W_TD = W_PC + NPC,        // fsphi, taudir_x, taudir_y (3)
+    W_XW = W_TD + 3,          // x-edge waves (NW)
+    W_YW = W_XW + NW,         // y-edge waves (NW)
+    W_XE = W_YW + NW,         // x-edge flux outputs (NEF)
+    W_YE = W_XE + NEF,        // y-edge flux outputs (NEF)
+    W_G = W_YE + NEF,         // Gx1, Gx2, Gy1, Gy2 (24)
+    W_QN = W_G + 24,          // updated state before src2 (7)
+    W_LQN = W_QN + 7,         // adjoint of that state (7)
+    W_LS = W_LQN + 7,         // adjoint of the split inputs of the cell state (6)
+    W_LXA = W_LS + 6,         // adjoint of amdq of x-edge i+1, apdq of x-edge i (12)
+    W_LYA = W_LXA + 12,       // same for y-edges j+1, j (12)
+    W_LXE = W_LYA + 12,       // adjoint of the x-edge flux inputs (NEI)
+    W_LYE = W_LXE + NEI,      // y-edges (NEI)
+    W_LXS = W_LYE + NEI,      // adjoint of the x-edge rpn2 sides, left then right (2 NSD)
+    W_LYS = W_LXS + 2 * NSD,  // y-edges (2 NSD)
+    W_LTD = W_LYS + 2 * NSD,  // adjoint of the taudir inputs (NTI)
+    W_LQ7 = W_LTD + NTI,      // adjoint of the loaded q of the cell (7)
+    W_GP = W_LQ7 + 7,         // parameter gradient per cell (MAXP)
+    W_FIELDS = W_GP + MAXP
+};
+
+struct AdjArgs {
+    const double* q_in;      // q_n (ghosts read only when bc[0] == 0)
+    const double* aux;
+    int maux, mx, my;
+    double dx, dy, dt;
+    int bc[4];               // boundary conditions applied on load (0: stored ghosts)
+    dclaw_params P;
+    const dclaw_params* dP;  // np parameter directions
+    int np;
+    const double* lam_out;   // lambda_{n+1}, layout of q
+    double* lam_in;          // lambda_n, layout of q
+    double* q_out;           // q_{n+1} (the record pass; optional)
+    double* W;               // workspace, W_FIELDS planes
+    size_t N;
+    const dclaw_params* Pd;  // device copies of P and dt read by each GPU thread (or null)
+    const double* dt_dev;
+};
+
+DC_HD size_t cix(const AdjArgs& A, int i, int j)
+{
+    return (size_t)(i + 1) + (size_t)(A.mx + 4) * (size_t)(j + 1);
+}
+DC_HD double* plane(const AdjArgs& A, int f) { return A.W + (size_t)f * A.N; }
+DC_HD double wv(const AdjArgs& A, int f, size_t n) { return A.W[(size_t)f * A.N + n]; }
+DC_HD void ws(const AdjArgs& A, int f, size_t n, double v) { A.W[(size_t)f * A.N + n] = v; }
+DC_HD bool in_grid(const AdjArgs& A, int i, int j)
+{
+    return i >= -1 && i <= A.mx + 2 && j >= -1 && j <= A.my + 2;
+}
+DC_HD bool interior(const AdjArgs& A, int i, int j)
+{
+    return i >= 1 && i <= A.mx && j >= 1 && j <= A.my;
+}
+DC_HD double bed(const AdjArgs& A, int i, int j) { return A.aux[aidx(i, j, A.mx, A.maux)]; }
+
+DC_HD TP zero_params(const dclaw_params& P)
+{
+    dclaw_params z = {};
+    return dclaw_tl::make_params(P, z);
+}
+
+// ---------------------------------------------------------------- stages
+
+// F1: q of the cell (after the boundary map) -> prepared fields.
+struct PrepF {
+    double bed;
+    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    {
+        dclaw_tl::RowCell r;
+        dclaw_tl::prepare_cell(P, x, Dual(bed), r);
+        y[PC_H] = r.h; y[PC_HU] = r.hu; y[PC_HV] = r.hv; y[PC_BDIF] = r.bdif;
+        y[PC_HM0] = r.hm0; y[PC_P0] = r.p0; y[PC_HCHI0] = r.hchi0;
+        y[PC_HM1] = r.hm1; y[PC_P1] = r.p1; y[PC_HCHI1] = r.hchi1; y[PC_M1] = r.m1;
+        y[PC_CHI1] = r.chi1; y[PC_U] = r.u; y[PC_V] = r.v;
+        y[PC_HM2] = r.hm2; y[PC_P2] = r.p2; y[PC_HCHI2] = r.hchi2; y[PC_M2] = r.m2;
+        y[PC_CHI2] = r.chi2;
+        y[PC_RHO1] = y[PC_TAU1] = y[PC_RHO2] = y[PC_TAU2] = Dual(0.0);
+        if (r.h > P.dry_tolerance) {
+            const dclaw_tl::RhoTau v1 = dclaw_tl::rho_tau(P, r.h, r.u, r.v, r.m1, r.p1, r.chi1);
+            const dclaw_tl::RhoTau v2 = dclaw_tl::rho_tau(P, r.h, r.u, r.v, r.m2, r.p2, r.chi2);
+            y[PC_RHO1] = v1.rho; y[PC_TAU1] = v1.tau;
+            y[PC_RHO2] = v2.rho; y[PC_TAU2] = v2.tau;
+        }
+    }
+};
+
+// F2: calc_taudir of the centre from 5 Q0 states (each 6 values and bdif) and
+// setvars(F1) of the centre.  Slot s (0 centre, 1 left, 2 right, 3 bottom,
+// 4 top) holds inputs 7s..7s+6; 35, 36 are rho1, tau1.
+struct TaudirF {
+    double dx, dy;
+    double b[5];
+    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    {
+        dclaw_tl::Cell c[5];
+        Dual bb[5];
+        for (int s = 0; s < 5; s++) {
+            c[s].h = x[7 * s + 0]; c[s].hu = x[7 * s + 1]; c[s].hv = x[7 * s + 2];
+            c[s].hm = x[7 * s + 3]; c[s].p = x[7 * s + 4]; c[s].hchi = x[7 * s + 5];
+            bb[s] = Dual(b[s]) - x[7 * s + 6];
+        }
+        dclaw_tl::RhoTau sv;
+        sv.rho = x[35];
+        sv.tau = x[36];
+        const dclaw_tl::Taudir t = dclaw_tl::calc_taudir_cell(P, Dual(dx), Dual(dy), c[0], bb[0],
+                                                              c[1], bb[1], c[2], bb[2], c[3],
+                                                              bb[3], c[4], bb[4], sv);
+        y[0] = t.fsphi;
+        y[1] = t.tx;
+        y[2] = t.ty;
+    }
+};
+
+DC_HD void side_from(const Dual* x, double bedv, dclaw_tl::Side& s)
+{
+    s.h = x[SD_H]; s.hu = x[SD_HU]; s.hv = x[SD_HV]; s.hm = x[SD_HM]; s.p = x[SD_P];
+    s.hchi = x[SD_HCHI]; s.u = x[SD_U]; s.v = x[SD_V]; s.m = x[SD_M]; s.chi = x[SD_CHI];
+    s.b = Dual(bedv) - x[SD_BDIF];
+    s.taudir = x[SD_TAUDIR]; s.fs = x[SD_FS]; s.rho = x[SD_RHO]; s.tau = x[SD_TAU];
+}
+
+// F3: rpn2 of one edge from its two sides (inputs 0..14 left, 15..29 right).
+struct Rpn2F {
+    int ixy;
+    double bedL, bedR;
+    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    {
+        dclaw_tl::Side L, R;
+        side_from(x, bedL, L);
+        side_from(x + NSD, bedR, R);
+        const dclaw_tl::EdgeWaves w = dclaw_tl::rpn2_edge(P, ixy, L, R);
+        for (int k = 0; k < 3; k++) y[k] = w.s[k];
+        for (int k = 0; k < 6; k++) y[3 + k] = w.w1[k];
+        for (int k = 0; k < 6; k++) y[9 + k] = w.w5[k];
+        y[15] = w.w2h;
+        y[16] = w.w2n;
+    }
+};
+
+template <class WavesT, class R>
+DC_HD void waves_from(const R* x, WavesT& w)
+{
+    for (int k = 0; k < 3; k++) w.s[k] = x[k];
+    for (int k = 0; k < 6; k++) w.w1[k] = x[3 + k];
+    for (int k = 0; k < 6; k++) w.w5[k] = x[9 + k];
+    w.w2h = x[15];
+    w.w2n = x[16];
+}
+
+// F4: flux2's edge body from the edge's waves and the limiter's dot products.
+struct FluxF {
+    int ixy;
+    bool gate;
+    double dtdx;
+    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    {
+        dclaw_tl::EdgeWaves w;
+        waves_from(x, w);
+        const dclaw_tl::EdgeFlux e = dclaw_tl::edge_flux(P, ixy, w, x + NW, x + NW + 3, gate,
+                                                         Dual(dtdx));
+        for (int k = 0; k < 6; k++) {
+            y[k] = e.faddm[k];
+            y[6 + k] = e.faddp[k];
+            y[12 + k] = e.amdq[k];
+            y[18 + k] = e.apdq[k];
+        }
+    }
+};
+
+// F5: one rpt2 split of a fluctuation in a cell.
+struct SplitF {
+    int ixy;
+    double bedc, topo1, topo3;
+    bool active;
+    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    {
+        dclaw_tl::rpt2_cell_split(P, ixy, x[0], x[1], x[2], x[3], x[4], Dual(bedc), x[5],
+                                  Dual(topo1), Dual(topo3), x + 6, active, y, y + 6);
+    }
+};
+
+// F6: src2 of the updated state (6 values and bdif).
+struct Src2F {
+    double dt, h_e;
+    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    {
+        for (int k = 0; k < 7; k++) y[k] = x[k];
+        dclaw_tl::src2_cell(P, Dual(dt), y[0], y[1], y[2], y[3], y[4], y[5], y[6], Dual(h_e));
+    }
+};
+
+// Values of a stage (tangents zero).
+template <int NI, int NO, class F>
+DC_HD void eval(const F& f, const TP& Pz, const double* x, double* y)
+{
+    Dual X[NI], Y[NO];
+    for (int a = 0; a < NI; a++) X[a] = Dual(x[a], 0.0);
+    f(Pz, X, Y);
+    for (int o = 0; o < NO; o++) y[o] = Y[o].v;
+}
+
+template <int N>
+DC_HD bool all_zero(const double* v)
+{
+    for (int k = 0; k < N; k++)
+        if (v[k] != 0.0) return false;
+    return true;
+}
+
+// lx = J^T ly for the stage's inputs (one seeded evaluation per input) and
+// gp[k] += ly . dY/dP_k for the parameter directions.  Nothing is evaluated
+// when ly is zero.
+template <int NI, int NO, class F>
+DC_HD void vjp(const AdjArgs& A, const F& f, const TP& Pz, const double* x, const double* ly,
+               double* lx, double* gp)
+{
+    for (int k = 0; k < NI; k++) lx[k] = 0.0;
+    if (all_zero<NO>(ly)) return;
+#pragma unroll 1
+    for (int k = 0; k < NI; k++) {
+        Dual X[NI], Y[NO];
+#pragma unroll
+        for (int a = 0; a < NI; a++) X[a] = Dual(x[a], a == k ? 1.0 : 0.0);
+        f(Pz, X, Y);
+        double s = 0.0;
+        for (int o = 0; o < NO; o++) s = s + ly[o] * Y[o].d;
+        lx[k] = s;
+    }
+#pragma unroll 1
+    for (int p = 0; p < A.np; p++) {
+        const TP Pk = dclaw_tl::make_params(A.P, A.dP[p]);
+        Dual X[NI], Y[NO];
+        for (int a = 0; a < NI; a++) X[a] = Dual(x[a], 0.0);
+        f(Pk, X, Y);
+        double s = 0.0;
+        for (int o = 0; o < NO; o++) s = s + ly[o] * Y[o].d;
+        gp[p] += s;
+    }
+}
+
+// ------------------------------------------------------- input gathering
+
+// q of cell (i, j) as the step loads it (boundary map for a ghost cell when
+// bc[0] != 0) and the momentum signs of the map.
+DC_HD void load_q(const AdjArgs& A, int i, int j, double* q7, double& fu, double& fv)
+{
+    int si = i, sj = j;
+    fu = 1.0;
+    fv = 1.0;
+    if (!interior(A, i, j) && A.bc[0] != 0) bc_source(A.bc, A.mx, A.my, i, j, si, sj, fu, fv);
+    const size_t o = qidx(0, si, sj, A.mx);
+    for (int m = 0; m < 7; m++) q7[m] = A.q_in[o + m];
+    q7[1] = fu * q7[1];
+    q7[2] = fv * q7[2];
+}
+
+DC_HD void taudir_inputs(const AdjArgs& A, int i, int j, double* x, TaudirF& f)
+{
+    const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
+    for (int s = 0; s < 5; s++) {
+        const size_t n = cix(A, i + di[s], j + dj[s]);
+        x[7 * s + 0] = wv(A, W_PC + PC_H, n);
+        x[7 * s + 1] = wv(A, W_PC + PC_HU, n);
+        x[7 * s + 2] = wv(A, W_PC + PC_HV, n);
+        x[7 * s + 3] = wv(A, W_PC + PC_HM0, n);
+        x[7 * s + 4] = wv(A, W_PC + PC_P0, n);
+        x[7 * s + 5] = wv(A, W_PC + PC_HCHI0, n);
+        x[7 * s + 6] = wv(A, W_PC + PC_BDIF, n);
+        f.b[s] = bed(A, i + di[s], j + dj[s]);
+    }
+    const size_t n = cix(A, i, j);
+    x[35] = wv(A, W_PC + PC_RHO1, n);
+    x[36] = wv(A, W_PC + PC_TAU1, n);
+    f.dx = A.dx;
+    f.dy = A.dy;
+}
+
+// rpn2 side of cell n: F2 state ("second") or F1; ixy = 2 swaps u and v.
+DC_HD void side_inputs(const AdjArgs& A, size_t n, bool second, int ixy, double* s)
+{
+    s[SD_H] = wv(A, W_PC + PC_H, n);
+    s[SD_HU] = wv(A, W_PC + PC_HU, n);
+    s[SD_HV] = wv(A, W_PC + PC_HV, n);
+    s[SD_HM] = wv(A, W_PC + (second ? PC_HM2 : PC_HM1), n);
+    s[SD_P] = wv(A, W_PC + (second ? PC_P2 : PC_P1), n);
+    s[SD_HCHI] = wv(A, W_PC + (second ? PC_HCHI2 : PC_HCHI1), n);
+    s[SD_U] = wv(A, W_PC + (ixy == 1 ? PC_U : PC_V), n);
+    s[SD_V] = wv(A, W_PC + (ixy == 1 ? PC_V : PC_U), n);
+    s[SD_M] = wv(A, W_PC + (second ? PC_M2 : PC_M1), n);
+    s[SD_CHI] = wv(A, W_PC + (second ? PC_CHI2 : PC_CHI1), n);
+    s[SD_BDIF] = wv(A, W_PC + PC_BDIF, n);
+    s[SD_TAUDIR] = wv(A, W_TD + (ixy == 1 ? 1 : 2), n);
+    s[SD_FS] = wv(A, W_TD + 0, n);
+    s[SD_RHO] = wv(A, W_PC + (second ? PC_RHO2 : PC_RHO1), n);
+    s[SD_TAU] = wv(A, W_PC + (second ? PC_TAU2 : PC_TAU1), n);
+}
+
+// Edge ranges: x-edge (i, j) exists for i in [0, mx+2], j in [0, my+1]; its
+// flux is used for i in [1, mx+1].  y-edge (i, k): k in [0, my+2], i in
+// [0, mx+1]; flux for k in [1, my+1].
+DC_HD bool xedge(const AdjArgs& A, int i, int j)
+{
+    return i >= 0 && i <= A.mx + 2 && j >= 0 && j <= A.my + 1;
+}
+DC_HD bool yedge(const AdjArgs& A, int i, int k)
+{
+    return k >= 0 && k <= A.my + 2 && i >= 0 && i <= A.mx + 1;
+}
+DC_HD bool xflux(const AdjArgs& A, int i, int j)
+{
+    return i >= 1 && i <= A.mx + 1 && j >= 0 && j <= A.my + 1;
+}
+DC_HD bool yflux(const AdjArgs& A, int i, int k)
+{
+    return k >= 1 && k <= A.my + 1 && i >= 0 && i <= A.mx + 1;
+}
+
+DC_HD void rpn2_inputs(const AdjArgs& A, int ixy, int i, int j, double* x, Rpn2F& f)
+{
+    // left/bottom cell, right/top cell
+    const int li = ixy == 1 ? i - 1 : i, lj = ixy == 1 ? j : j - 1;
+    const bool second = (ixy == 1 ? li : lj) >= 0;
+    side_inputs(A, cix(A, li, lj), second, ixy, x);
+    side_inputs(A, cix(A, i, j), false, ixy, x + NSD);
+    f.ixy = ixy;
+    f.bedL = bed(A, li, lj);
+    f.bedR = bed(A, i, j);
+}
+
+DC_HD void wave_dots3(const double* a, const double* b, double* d3)
+{
+    double d = 0.0;
+    for (int k = 0; k < 6; k++) d = d + a[3 + k] * b[3 + k];
+    d3[0] = d;
+    d = 0.0;
+    d = d + a[15] * b[15];
+    d = d + a[16] * b[16];
+    d3[1] = d;
+    d = 0.0;
+    for (int k = 0; k < 6; k++) d = d + a[9 + k] * b[9 + k];
+    d3[2] = d;
+}
+
+DC_HD void load_waves(const AdjArgs& A, int f0, size_t n, double* w)
+{
+    for (int k = 0; k < NW; k++) w[k] = wv(A, f0 + k, n);
+}
+
+DC_HD void flux_inputs(const AdjArgs& A, int ixy, int i, int j, double* x, FluxF& f)
+{
+    const int f0 = ixy == 1 ? W_XW : W_YW;
+    const int di = ixy == 1 ? 1 : 0, dj = ixy == 1 ? 0 : 1;
+    double wl[NW], wr[NW];
+    load_waves(A, f0, cix(A, i - di, j - dj), wl);
+    load_waves(A, f0, cix(A, i, j), x);
+    load_waves(A, f0, cix(A, i + di, j + dj), wr);
+    wave_dots3(wl, x, x + NW);
+    wave_dots3(x, wr, x + NW + 3);
+    // wet-dry gate h(-1) h(0) h(-2) h(+1) in flux2's product order
+    const double hm1 = wv(A, W_PC + PC_H, cix(A, i - di, j - dj));
+    const double h0 = wv(A, W_PC + PC_H, cix(A, i, j));
+    const double hm2 = wv(A, W_PC + PC_H, cix(A, i - 2 * di, j - 2 * dj));
+    const double hp1 = wv(A, W_PC + PC_H, cix(A, i + di, j + dj));
+    f.ixy = ixy;
+    f.gate = (hm1 * h0 * hm2 * hp1 > 0.0);
+    f.dtdx = ixy == 1 ? A.dt / A.dx : A.dt / A.dy;
+}
+
+DC_HD void split_state(const AdjArgs& A, size_t n, double* x)
+{
+    x[0] = wv(A, W_PC + PC_H, n);
+    x[1] = wv(A, W_PC + PC_HU, n);
+    x[2] = wv(A, W_PC + PC_HV, n);
+    x[3] = wv(A, W_PC + PC_HM2, n);
+    x[4] = wv(A, W_PC + PC_HCHI2, n);
+    x[5] = wv(A, W_PC + PC_BDIF, n);
+}
+
+// The four splits of cell (i, j): s = 0 x imp1 (amdq of x-edge i+1), 1 x imp2
+// (apdq of x-edge i), 2 y imp1 (amdq of y-edge j+1), 3 y imp2 (apdq of y-edge
+// j).  Returns false if the split does not take place.
+DC_HD bool split_inputs(const AdjArgs& A, int s, int i, int j, double* x, SplitF& f)
+{
+    if (A.P.transverse == 0) return false;
+    const size_t n = cix(A, i, j);
+    split_state(A, n, x);
+    f.bedc = bed(A, i, j);
+    if (s < 2) {
+        if (!(i >= 1 && i <= A.mx && j >= 0 && j <= A.my + 1)) return false;
+        f.ixy = 1;
+        f.topo1 = bed(A, i, j - 1);
+        f.topo3 = bed(A, i, j + 1);
+        f.active = true;
+        const size_t e = s == 0 ? cix(A, i + 1, j) : n;
+        const int o = s == 0 ? 12 : 18;
+        for (int k = 0; k < 6; k++) x[6 + k] = wv(A, W_XE + o + k, e);
+    } else {
+        if (!(i >= 0 && i <= A.mx + 1 && j >= -1 && j <= A.my + 1)) return false;
+        f.ixy = 2;
+        f.topo1 = bed(A, i - 1, j);
+        f.topo3 = bed(A, i + 1, j);
+        const int k = s == 2 ? j + 1 : j;
+        f.active = (k >= 1 && k <= A.my + 1);
+        const size_t e = cix(A, i, k);
+        const int o = s == 2 ? 12 : 18;
+        for (int z = 0; z < 6; z++) x[6 + z] = wv(A, W_YE + o + z, e);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------- record pass
+
+DC_HD void rec_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    double q7[7], fu, fv, y[NPC];
+    load_q(A, i, j, q7, fu, fv);
+    PrepF f{bed(A, i, j)};
+    eval<7, NPC>(f, Pz, q7, y);
+    const size_t n = cix(A, i, j);
+    for (int k = 0; k < NPC; k++) ws(A, W_PC + k, n, y[k]);
+}
+
+DC_HD void rec_taudir(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double y[3] = {0.0, 0.0, 0.0};
+    if (i >= 0 && i <= A.mx + 1 && j >= 0 && j <= A.my + 1) {
+        double x[NTI];
+        TaudirF f;
+        taudir_inputs(A, i, j, x, f);
+        eval<NTI, 3>(f, Pz, x, y);
+    }
+    for (int k = 0; k < 3; k++) ws(A, W_TD + k, n, y[k]);
+}
+
+DC_HD void rec_waves(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    for (int ixy = 1; ixy <= 2; ixy++) {
+        double y[NW];
+        for (int k = 0; k < NW; k++) y[k] = 0.0;
+        if (ixy == 1 ? xedge(A, i, j) : yedge(A, i, j)) {
+            double x[2 * NSD];
+            Rpn2F f;
+            rpn2_inputs(A, ixy, i, j, x, f);
+            eval<2 * NSD, NW>(f, Pz, x, y);
+        }
+        const int f0 = ixy == 1 ? W_XW : W_YW;
+        for (int k = 0; k < NW; k++) ws(A, f0 + k, n, y[k]);
+    }
+}
+
+DC_HD void rec_flux(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    for (int ixy = 1; ixy <= 2; ixy++) {
+        double y[NEF];
+        for (int k = 0; k < NEF; k++) y[k] = 0.0;
+        if (ixy == 1 ? xflux(A, i, j) : yflux(A, i, j)) {
+            double x[NEI];
+            FluxF f;
+            flux_inputs(A, ixy, i, j, x, f);
+            eval<NEI, NEF>(f, Pz, x, y);
+        }
+        const int f0 = ixy == 1 ? W_XE : W_YE;
+        for (int k = 0; k < NEF; k++) ws(A, f0 + k, n, y[k]);
+    }
+}
+
+DC_HD void rec_split(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double b[4][12];
+    for (int s = 0; s < 4; s++) {
+        for (int k = 0; k < 12; k++) b[s][k] = 0.0;
+        double x[NSI];
+        SplitF f;
+        if (split_inputs(A, s, i, j, x, f)) eval<NSI, 12>(f, Pz, x, b[s]);
+    }
+    // G = (0 - (dt/2dx) b_imp1) - (dt/2dx) b_imp2 (step2's transverse terms)
+    const double dtdx = A.dt / A.dx, dtdy = A.dt / A.dy;
+    for (int k = 0; k < 6; k++) {
+        ws(A, W_G + k, n, (0.0 - 0.5 * dtdx * b[0][k]) - 0.5 * dtdx * b[1][k]);
+        ws(A, W_G + 6 + k, n, (0.0 - 0.5 * dtdx * b[0][6 + k]) - 0.5 * dtdx * b[1][6 + k]);
+        ws(A, W_G + 12 + k, n, (0.0 - 0.5 * dtdy * b[2][k]) - 0.5 * dtdy * b[3][k]);
+        ws(A, W_G + 18 + k, n, (0.0 - 0.5 * dtdy * b[2][6 + k]) - 0.5 * dtdy * b[3][6 + k]);
+    }
+}
+
+DC_HD void rec_update(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double q[7];
+    q[0] = wv(A, W_PC + PC_H, n); q[1] = wv(A, W_PC + PC_HU, n); q[2] = wv(A, W_PC + PC_HV, n);
+    q[3] = wv(A, W_PC + PC_HM0, n); q[4] = wv(A, W_PC + PC_P0, n);
+    q[5] = wv(A, W_PC + PC_HCHI0, n); q[6] = wv(A, W_PC + PC_BDIF, n);
+    double* o = A.q_out ? A.q_out + qidx(0, i, j, A.mx) : nullptr;
+    if (!interior(A, i, j)) {
+        if (o)
+            for (int m = 0; m < 7; m++) o[m] = q[m];
+        return;
+    }
+    const double dtdx = A.dt / A.dx, dtdy = A.dt / A.dy;
+    const size_t nl = cix(A, i - 1, j), nr = cix(A, i + 1, j);
+    const size_t nb = cix(A, i, j - 1), nt = cix(A, i, j + 1);
+    for (int z = 0; z < 6; z++) {
+        const double fm_next = ((0.0 + wv(A, W_XE + z, nr)) + wv(A, W_G + 18 + z, n)) +
+                               wv(A, W_G + 12 + z, nr);
+        const double fpc = ((0.0 + wv(A, W_XE + 6 + z, n)) + wv(A, W_G + 18 + z, nl)) +
+                           wv(A, W_G + 12 + z, n);
+        const double gm_next = ((0.0 + wv(A, W_G + 6 + z, n)) + wv(A, W_G + z, nt)) +
+                               wv(A, W_YE + z, nt);
+        const double gp_prev = ((0.0 + wv(A, W_G + 6 + z, nb)) + wv(A, W_G + z, n)) +
+                               wv(A, W_YE + 6 + z, n);
+        q[z] = q[z] - dtdx * (fm_next - fpc) - dtdy * (gm_next - gp_prev);
+    }
+    for (int m = 0; m < 7; m++) ws(A, W_QN + m, n, q[m]);
+    const double h_e = A.P.entrainment ? A.aux[aidx(i, j, A.mx, A.maux) + 6] : 0.0;
+    Src2F f{A.dt, h_e};
+    double y[7];
+    eval<7, 7>(f, Pz, q, y);
+    if (o)
+        for (int m = 0; m < 7; m++) o[m] = y[m];
+}
+
+// --------------------------------------------------------- reverse pass
+
+// adjoint of fm(i, j) + fp(i, j) and of gm(i, j) + gp(i, j) (each flux
+// entry feeds one update: fm(i, j) cell i-1, fp(i, j) cell i, gm(i, j) cell
+// j-1, gp(i, j) cell j)
+DC_HD double lam_qn(const AdjArgs& A, int i, int j, int z)
+{
+    return interior(A, i, j) ? wv(A, W_LQN + z, cix(A, i, j)) : 0.0;
+}
+DC_HD double lam_fm(const AdjArgs& A, int i, int j, int z)
+{
+    return -(A.dt / A.dx) * lam_qn(A, i - 1, j, z);
+}
+DC_HD double lam_fp(const AdjArgs& A, int i, int j, int z)
+{
+    return (A.dt / A.dx) * lam_qn(A, i, j, z);
+}
+DC_HD double lam_gm(const AdjArgs& A, int i, int j, int z)
+{
+    return -(A.dt / A.dy) * lam_qn(A, i, j - 1, z);
+}
+DC_HD double lam_gp(const AdjArgs& A, int i, int j, int z)
+{
+    return (A.dt / A.dy) * lam_qn(A, i, j, z);
+}
+
+DC_HD void gp_zero(const AdjArgs& A, size_t n)
+{
+    for (int p = 0; p < MAXP; p++) ws(A, W_GP + p, n, 0.0);
+}
+DC_HD void gp_add(const AdjArgs& A, size_t n, const double* g)
+{
+    for (int p = 0; p < A.np; p++) ws(A, W_GP + p, n, wv(A, W_GP + p, n) + g[p]);
+}
+
+// B1: src2 (and the parameter gradient is started here).
+DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    gp_zero(A, n);
+    double lx[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (interior(A, i, j)) {
+        double x[7], ly[7], g[MAXP] = {0, 0, 0, 0};
+        for (int m = 0; m < 7; m++) x[m] = wv(A, W_QN + m, n);
+        for (int m = 0; m < 7; m++) ly[m] = A.lam_out[qidx(m, i, j, A.mx)];
+        if (x[0] <= A.P.dry_tolerance) {
+            // src2 zeroes a dry cell and leaves bdif (src2.f90:103-111)
+            lx[6] = ly[6];
+        } else {
+            const double h_e = A.P.entrainment ? A.aux[aidx(i, j, A.mx, A.maux) + 6] : 0.0;
+            Src2F f{A.dt, h_e};
+            vjp<7, 7>(A, f, Pz, x, ly, lx, g);
+            gp_add(A, n, g);
+        }
+    }
+    for (int m = 0; m < 7; m++) ws(A, W_LQN + m, n, lx[m]);
+}
+
+// B2: the four transverse splits of the cell.
+DC_HD void rev_split(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double ls[6] = {0, 0, 0, 0, 0, 0};
+    double la[4][6];
+    double g[MAXP] = {0, 0, 0, 0};
+    const double hx = 0.5 * (A.dt / A.dx), hy = 0.5 * (A.dt / A.dy);
+    for (int s = 0; s < 4; s++) {
+        for (int k = 0; k < 6; k++) la[s][k] = 0.0;
+        double x[NSI];
+        SplitF f;
+        if (!split_inputs(A, s, i, j, x, f)) continue;
+        double ly[12];
+        for (int z = 0; z < 6; z++) {
+            if (s < 2) {
+                // Gx1(i, j) feeds gm(i, j) and gp(i, j); Gx2(i, j) feeds gm, gp(i, j+1)
+                ly[z] = -hx * (lam_gm(A, i, j, z) + lam_gp(A, i, j, z));
+                ly[6 + z] = -hx * (lam_gm(A, i, j + 1, z) + lam_gp(A, i, j + 1, z));
+            } else {
+                // Gy1(i, j) feeds fm(i, j) and fp(i, j); Gy2(i, j) feeds fm, fp(i+1, j)
+                ly[z] = -hy * (lam_fm(A, i, j, z) + lam_fp(A, i, j, z));
+                ly[6 + z] = -hy * (lam_fm(A, i + 1, j, z) + lam_fp(A, i + 1, j, z));
+            }
+        }
+        // rpt2 splits nothing in an inactive or dry cell (zero Jacobian)
+        if (!f.active || !(x[0] > A.P.dry_tolerance)) continue;
+        double lx[NSI];
+        vjp<NSI, 12>(A, f, Pz, x, ly, lx, g);
+        for (int k = 0; k < 6; k++) ls[k] += lx[k];
+        for (int k = 0; k < 6; k++) la[s][k] = lx[6 + k];
+    }
+    for (int k = 0; k < 6; k++) ws(A, W_LS + k, n, ls[k]);
+    for (int k = 0; k < 6; k++) {
+        ws(A, W_LXA + k, n, la[0][k]);
+        ws(A, W_LXA + 6 + k, n, la[1][k]);
+        ws(A, W_LYA + k, n, la[2][k]);
+        ws(A, W_LYA + 6 + k, n, la[3][k]);
+    }
+    gp_add(A, n, g);
+}
+
+// B3: the flux body of the x- and y-edge at (i, j).
+DC_HD void rev_flux(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double g[MAXP] = {0, 0, 0, 0};
+    for (int ixy = 1; ixy <= 2; ixy++) {
+        double lx[NEI];
+        for (int k = 0; k < NEI; k++) lx[k] = 0.0;
+        if (ixy == 1 ? xflux(A, i, j) : yflux(A, i, j)) {
+            double x[NEI], ly[NEF];
+            FluxF f;
+            flux_inputs(A, ixy, i, j, x, f);
+            for (int z = 0; z < 6; z++) {
+                if (ixy == 1) {
+                    ly[z] = lam_fm(A, i, j, z);
+                    ly[6 + z] = lam_fp(A, i, j, z);
+                    ly[12 + z] = wv(A, W_LXA + z, cix(A, i - 1, j));   // amdq: cell i-1's split
+                    ly[18 + z] = wv(A, W_LXA + 6 + z, n);              // apdq: cell i's split
+                } else {
+                    ly[z] = lam_gm(A, i, j, z);
+                    ly[6 + z] = lam_gp(A, i, j, z);
+                    ly[12 + z] = wv(A, W_LYA + z, cix(A, i, j - 1));
+                    ly[18 + z] = wv(A, W_LYA + 6 + z, n);
+                }
+            }
+            // with the gate dry and every speed 0 the edge body adds nothing
+            // (waves with s = 0 are dropped, no correction): zero Jacobian
+            if (f.gate || x[0] != 0.0 || x[1] != 0.0 || x[2] != 0.0)
+                vjp<NEI, NEF>(A, f, Pz, x, ly, lx, g);
+        }
+        const int f0 = ixy == 1 ? W_LXE : W_LYE;
+        for (int k = 0; k < NEI; k++) ws(A, f0 + k, n, lx[k]);
+    }
+    gp_add(A, n, g);
+}
+
+// Adjoint of the waves of an edge: its own flux body's, plus the limiter dot
+// products of its flux body and of the two neighbouring edges'.
+DC_HD void lam_waves(const AdjArgs& A, int ixy, int i, int j, double* lw)
+{
+    const int fw = ixy == 1 ? W_XW : W_YW, fl = ixy == 1 ? W_LXE : W_LYE;
+    const int di = ixy == 1 ? 1 : 0, dj = ixy == 1 ? 0 : 1;
+    const size_t n = cix(A, i, j);
+    for (int k = 0; k < NW; k++) lw[k] = wv(A, fl + k, n);
+    // dotl(e) = dots(w(e-1), w(e)) and dotr(e) = dots(w(e), w(e+1)); w(e-1)
+    // multiplies the own dotl's and the left neighbour's dotr's adjoints,
+    // w(e+1) the own dotr's and the right neighbour's dotl's
+    double wl[NW], wr[NW];
+    const bool has_l = in_grid(A, i - di, j - dj), has_r = in_grid(A, i + di, j + dj);
+    for (int k = 0; k < NW; k++) wl[k] = wr[k] = 0.0;
+    if (has_l) load_waves(A, fw, cix(A, i - di, j - dj), wl);
+    if (has_r) load_waves(A, fw, cix(A, i + di, j + dj), wr);
+    double cl[3], cr[3];
+    for (int f = 0; f < 3; f++) {
+        cl[f] = wv(A, fl + NW + f, n) +
+                (has_l ? wv(A, fl + NW + 3 + f, cix(A, i - di, j - dj)) : 0.0);
+        cr[f] = wv(A, fl + NW + 3 + f, n) +
+                (has_r ? wv(A, fl + NW + f, cix(A, i + di, j + dj)) : 0.0);
+    }
+    for (int k = 0; k < 6; k++) lw[3 + k] += cl[0] * wl[3 + k] + cr[0] * wr[3 + k];
+    lw[15] += cl[1] * wl[15] + cr[1] * wr[15];
+    lw[16] += cl[1] * wl[16] + cr[1] * wr[16];
+    for (int k = 0; k < 6; k++) lw[9 + k] += cl[2] * wl[9 + k] + cr[2] * wr[9 + k];
+}
+
+// B4: rpn2 of the x- and y-edge at (i, j).
+DC_HD void rev_waves(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double g[MAXP] = {0, 0, 0, 0};
+    for (int ixy = 1; ixy <= 2; ixy++) {
+        double lx[2 * NSD];
+        for (int k = 0; k < 2 * NSD; k++) lx[k] = 0.0;
+        if (ixy == 1 ? xedge(A, i, j) : yedge(A, i, j)) {
+            double x[2 * NSD], ly[NW];
+            Rpn2F f;
+            rpn2_inputs(A, ixy, i, j, x, f);
+            lam_waves(A, ixy, i, j, ly);
+            const double drytol = A.P.dry_tolerance;
+            // both sides dry: rpn2 returns constant zero waves
+            if (x[SD_H] > drytol || x[NSD + SD_H] > drytol) vjp<2 * NSD, NW>(A, f, Pz, x, ly, lx, g);
+        }
+        const int f0 = ixy == 1 ? W_LXS : W_LYS;
+        for (int k = 0; k < 2 * NSD; k++) ws(A, f0 + k, n, lx[k]);
+    }
+    gp_add(A, n, g);
+}
+
+DC_HD double lxs(const AdjArgs& A, int f0, int i, int j, int k)
+{
+    return in_grid(A, i, j) ? wv(A, f0 + k, cix(A, i, j)) : 0.0;
+}
+
+// B5: calc_taudir of the cell.
+DC_HD void rev_taudir(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double lx[NTI];
+    for (int k = 0; k < NTI; k++) lx[k] = 0.0;
+    if (i >= 0 && i <= A.mx + 1 && j >= 0 && j <= A.my + 1) {
+        double ly[3];
+        // the cell is the right side of x-edge i, the left of x-edge i+1, the
+        // right (top) side of y-edge j, the left of y-edge j+1
+        ly[0] = lxs(A, W_LXS, i, j, NSD + SD_FS) + lxs(A, W_LXS, i + 1, j, SD_FS) +
+                lxs(A, W_LYS, i, j, NSD + SD_FS) + lxs(A, W_LYS, i, j + 1, SD_FS);
+        ly[1] = lxs(A, W_LXS, i, j, NSD + SD_TAUDIR) + lxs(A, W_LXS, i + 1, j, SD_TAUDIR);
+        ly[2] = lxs(A, W_LYS, i, j, NSD + SD_TAUDIR) + lxs(A, W_LYS, i, j + 1, SD_TAUDIR);
+        double x[NTI], g[MAXP] = {0, 0, 0, 0};
+        TaudirF f;
+        taudir_inputs(A, i, j, x, f);
+        // five dry states: calc_taudir returns zeros (its h-sum test, with
+        // dry_tolerance > 0)
+        const double dtol = A.P.dry_tolerance;
+        const bool dry5 = dtol > 0.0 && x[0] <= dtol && x[7] <= dtol && x[14] <= dtol &&
+                          x[21] <= dtol && x[28] <= dtol;
+        if (!dry5) {
+            vjp<NTI, 3>(A, f, Pz, x, ly, lx, g);
+            gp_add(A, n, g);
+        }
+    }
+    for (int k = 0; k < NTI; k++) ws(A, W_LTD + k, n, lx[k]);
+}
+
+// Add a side's adjoint to the prepared fields it was read from.
+DC_HD void side_to_pc(const double* ls, bool second, int ixy, double* lp)
+{
+    lp[PC_H] += ls[SD_H];
+    lp[PC_HU] += ls[SD_HU];
+    lp[PC_HV] += ls[SD_HV];
+    lp[second ? PC_HM2 : PC_HM1] += ls[SD_HM];
+    lp[second ? PC_P2 : PC_P1] += ls[SD_P];
+    lp[second ? PC_HCHI2 : PC_HCHI1] += ls[SD_HCHI];
+    lp[ixy == 1 ? PC_U : PC_V] += ls[SD_U];
+    lp[ixy == 1 ? PC_V : PC_U] += ls[SD_V];
+    lp[second ? PC_M2 : PC_M1] += ls[SD_M];
+    lp[second ? PC_CHI2 : PC_CHI1] += ls[SD_CHI];
+    lp[PC_BDIF] += ls[SD_BDIF];
+    lp[second ? PC_RHO2 : PC_RHO1] += ls[SD_RHO];
+    lp[second ? PC_TAU2 : PC_TAU1] += ls[SD_TAU];
+}
+
+// B6: gather the adjoint of the prepared fields, then b4step2's pointwise part.
+DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double lp[NPC];
+    for (int k = 0; k < NPC; k++) lp[k] = 0.0;
+    // Q0 feeds the update (interior) or is the output (ghost cells)
+    {
+        double l7[7];
+        if (interior(A, i, j)) {
+            for (int m = 0; m < 7; m++) l7[m] = wv(A, W_LQN + m, n);
+        } else {
+            for (int m = 0; m < 7; m++) l7[m] = A.lam_out[qidx(m, i, j, A.mx)];
+        }
+        lp[PC_H] += l7[0]; lp[PC_HU] += l7[1]; lp[PC_HV] += l7[2]; lp[PC_HM0] += l7[3];
+        lp[PC_P0] += l7[4]; lp[PC_HCHI0] += l7[5]; lp[PC_BDIF] += l7[6];
+    }
+    // transverse splits in this cell
+    lp[PC_H] += wv(A, W_LS + 0, n); lp[PC_HU] += wv(A, W_LS + 1, n);
+    lp[PC_HV] += wv(A, W_LS + 2, n); lp[PC_HM2] += wv(A, W_LS + 3, n);
+    lp[PC_HCHI2] += wv(A, W_LS + 4, n); lp[PC_BDIF] += wv(A, W_LS + 5, n);
+    // rpn2 sides: right side of x-edge i and y-edge j (F1), left side of
+    // x-edge i+1 and y-edge j+1 (F2 unless the cell is the slice's first)
+    {
+        double ls[NSD];
+        if (xedge(A, i, j)) {
+            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LXS + NSD + k, n);
+            side_to_pc(ls, false, 1, lp);
+        }
+        if (xedge(A, i + 1, j)) {
+            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LXS + k, cix(A, i + 1, j));
+            side_to_pc(ls, i >= 0, 1, lp);
+        }
+        if (yedge(A, i, j)) {
+            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LYS + NSD + k, n);
+            side_to_pc(ls, false, 2, lp);
+        }
+        if (yedge(A, i, j + 1)) {
+            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LYS + k, cix(A, i, j + 1));
+            side_to_pc(ls, j >= 0, 2, lp);
+        }
+    }
+    // calc_taudir of this cell (slot 0) and of its neighbours: the cell is the
+    // left neighbour (slot 1) of (i+1, j), the right (2) of (i-1, j), the
+    // bottom (3) of (i, j+1), the top (4) of (i, j-1)
+    {
+        const int ci[5] = {i, i + 1, i - 1, i, i}, cj[5] = {j, j, j, j + 1, j - 1};
+        for (int s = 0; s < 5; s++) {
+            if (!in_grid(A, ci[s], cj[s])) continue;
+            const size_t c = cix(A, ci[s], cj[s]);
+            const int o = 7 * s;
+            lp[PC_H] += wv(A, W_LTD + o + 0, c);
+            lp[PC_HU] += wv(A, W_LTD + o + 1, c);
+            lp[PC_HV] += wv(A, W_LTD + o + 2, c);
+            lp[PC_HM0] += wv(A, W_LTD + o + 3, c);
+            lp[PC_P0] += wv(A, W_LTD + o + 4, c);
+            lp[PC_HCHI0] += wv(A, W_LTD + o + 5, c);
+            lp[PC_BDIF] += wv(A, W_LTD + o + 6, c);
+        }
+        lp[PC_RHO1] += wv(A, W_LTD + 35, n);
+        lp[PC_TAU1] += wv(A, W_LTD + 36, n);
+    }
+    double q7[7], fu, fv, lx[7], g[MAXP] = {0, 0, 0, 0};
+    load_q(A, i, j, q7, fu, fv);
+    if (q7[0] <= A.P.dry_tolerance) {
+        // qfix zeroes a dry cell; only bdif = max(bdif, 0) remains
+        for (int m = 0; m < 6; m++) lx[m] = 0.0;
+        lx[6] = q7[6] > 0.0 ? lp[PC_BDIF] : 0.0;
+    } else {
+        PrepF f{bed(A, i, j)};
+        vjp<7, NPC>(A, f, Pz, q7, lp, lx, g);
+        gp_add(A, n, g);
+    }
+    for (int m = 0; m < 7; m++) ws(A, W_LQ7 + m, n, lx[m]);
+}
+
+// B7: lambda_n.  With the boundary map on load a ghost cell's q is its source
+// cell's (normal momentum negated at a wall), so its adjoint goes to the
+// source; the stored ghosts of q_n are not read and get 0.
+DC_HD void rev_bc(const AdjArgs& A, int i, int j)
+{
+    const size_t n = cix(A, i, j);
+    double l[7];
+    for (int m = 0; m < 7; m++) l[m] = wv(A, W_LQ7 + m, n);
+    if (A.bc[0] != 0) {
+        if (!interior(A, i, j)) {
+            for (int m = 0; m < 7; m++) l[m] = 0.0;
+        } else if (i <= 2 || i >= A.mx - 1 || j <= 2 || j >= A.my - 1) {
+            // ghost cells whose source is (i, j): columns -1, 0 / mx+1, mx+2
+            // and rows -1, 0 / my+1, my+2 around the cell's own
+            int gi[5], gj[5], ni = 0, nj = 0;
+            gi[ni++] = i;
+            if (i <= 2) { gi[ni++] = -1; gi[ni++] = 0; }
+            if (i >= A.mx - 1) { gi[ni++] = A.mx + 1; gi[ni++] = A.mx + 2; }
+            gj[nj++] = j;
+            if (j <= 2) { gj[nj++] = -1; gj[nj++] = 0; }
+            if (j >= A.my - 1) { gj[nj++] = A.my + 1; gj[nj++] = A.my + 2; }
+            for (int b = 0; b < nj; b++)
+                for (int a = 0; a < ni; a++) {
+                    const int ci = gi[a], cj = gj[b];
+                    if (interior(A, ci, cj)) continue;
+                    int si, sj;
+                    double fu, fv;
+                    bc_source(A.bc, A.mx, A.my, ci, cj, si, sj, fu, fv);
+                    if (si != i || sj != j) continue;
+                    const size_t c = cix(A, ci, cj);
+                    for (int m = 0; m < 7; m++) {
+                        const double s = m == 1 ? fu : (m == 2 ? fv : 1.0);
+                        l[m] += s * wv(A, W_LQ7 + m, c);
+                    }
+                }
+        }
+    }
+    double* o = A.lam_in + qidx(0, i, j, A.mx);
+    for (int m = 0; m < 7; m++) o[m] = l[m];
+}
+
+}  // namespace dclaw_adj

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "dclaw_hip.h"
+#include "dclaw_adjoint.h"
 #include "dclaw_amr.h"
 #include "dclaw_strip.h"
 #include "dclaw_tangent.h"
@@ -783,6 +784,210 @@ __global__ void k_gauges(const double* __restrict__ q, const double* __restrict_
     }
 }
 
+
+// ---- adjoint step (dclaw_adjoint.h), tile-sparse: the stages run over the
+// 8 x 8 cell tiles within ADJ_HALO cells of a wet cell (and the tiles of the
+// border band, where the boundary map gathers ghost adjoints), one wavefront
+// per tile, tiles taken from a compacted list by a persistent grid.  Outside
+// those tiles the step maps a cell to its dry state and every intermediate of
+// the stages is 0, so the workspace planes of inactive tiles read as 0 (they
+// are cleared after each call), bdif's plane is written for every cell, and
+// one dense pass writes the adjoint of the dry map (only bdif passes).
+enum AdjStage {
+    ADJ_REC_PREPARE, ADJ_REC_TAUDIR, ADJ_REC_WAVES, ADJ_REC_FLUX, ADJ_REC_SPLIT, ADJ_REC_UPDATE,
+    ADJ_REV_SRC2, ADJ_REV_SPLIT, ADJ_REV_FLUX, ADJ_REV_WAVES, ADJ_REV_TAUDIR, ADJ_REV_PREPARE,
+    ADJ_REV_BC, ADJ_CLEAR, ADJ_NSTAGES
+};
+constexpr int ADJ_TILE = 8;       // tile side (64 cells: one wavefront)
+constexpr int ADJ_HALO = 5;       // cells around a wet cell whose tiles are active
+constexpr int ADJ_BLOCK = 256;    // dense passes
+constexpr int ADJ_GRID = 8192;    // persistent grid of the tile passes
+
+struct AdjTiles {
+    int ntx, nty;
+    int* flags;       // ntx * nty, 0 between calls
+    int* list;        // active tiles
+    int* count;       // [0]: number of active tiles
+};
+
+template <int STAGE>
+__global__ __launch_bounds__(64) void k_adjoint(dclaw_adj::AdjArgs A, AdjTiles T)
+{
+    A.P = *A.Pd;
+    A.dt = *A.dt_dev;
+    const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
+    const int nact = *T.count;
+    for (int k = blockIdx.x; k < nact; k += gridDim.x) {
+        const int t = T.list[k];
+        const int i = -1 + (t % T.ntx) * ADJ_TILE + (int)(threadIdx.x % ADJ_TILE);
+        const int j = -1 + (t / T.ntx) * ADJ_TILE + (int)(threadIdx.x / ADJ_TILE);
+        if (i > A.mx + 2 || j > A.my + 2) continue;
+        if constexpr (STAGE == ADJ_REC_PREPARE) dclaw_adj::rec_prepare(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REC_TAUDIR) dclaw_adj::rec_taudir(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REC_WAVES) dclaw_adj::rec_waves(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REC_FLUX) dclaw_adj::rec_flux(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REC_SPLIT) dclaw_adj::rec_split(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REC_UPDATE) dclaw_adj::rec_update(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REV_SRC2) dclaw_adj::rev_src2(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REV_SPLIT) dclaw_adj::rev_split(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REV_FLUX) dclaw_adj::rev_flux(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REV_WAVES) dclaw_adj::rev_waves(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REV_TAUDIR) dclaw_adj::rev_taudir(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REV_PREPARE) dclaw_adj::rev_prepare(A, Pz, i, j);
+        else if constexpr (STAGE == ADJ_REV_BC) dclaw_adj::rev_bc(A, i, j);
+        else {
+            // clear the tile's planes (bdif's is rewritten densely each call)
+            const size_t n = dclaw_adj::cix(A, i, j);
+            for (int f = 0; f < dclaw_adj::W_FIELDS; f++)
+                if (f != dclaw_adj::W_PC + dclaw_adj::PC_BDIF) A.W[(size_t)f * A.N + n] = 0.0;
+        }
+    }
+}
+
+// Dense pass over every cell: bdif's plane (max(bdif, 0) of the loaded
+// state), the dry state as q_out (if wanted) and the adjoint of the dry map
+// as lam_in (the tile passes overwrite both in active tiles); marks the tiles
+// within ADJ_HALO of a wet cell and the tiles of the border band (i <= 2,
+// i >= mx-1, j <= 2, j >= my-1), or every tile when dry_tolerance <= 0.
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint_dense(dclaw_adj::AdjArgs A, AdjTiles T)
+{
+    const size_t n = (size_t)blockIdx.x * ADJ_BLOCK + threadIdx.x;
+    if (n >= A.N) return;
+    const int w = A.mx + 4;
+    const int i = (int)(n % (size_t)w) - 1, j = (int)(n / (size_t)w) - 1;
+    const double drytol = A.Pd->dry_tolerance;
+    double q7[7], fu, fv;
+    dclaw_adj::load_q(A, i, j, q7, fu, fv);
+    const double bd = q7[6] > 0.0 ? q7[6] : 0.0;
+    A.W[(size_t)(dclaw_adj::W_PC + dclaw_adj::PC_BDIF) * A.N + n] = bd;
+    if (A.q_out) {
+        double* o = A.q_out + qidx(0, i, j, A.mx);
+        for (int m = 0; m < 6; m++) o[m] = 0.0;
+        o[6] = bd;
+    }
+    {
+        const double* lo = A.lam_out + qidx(0, i, j, A.mx);
+        const double l6 = q7[6] > 0.0 ? lo[6] : 0.0;
+        double* li = A.lam_in + qidx(0, i, j, A.mx);
+        for (int m = 0; m < 6; m++) li[m] = 0.0;
+        li[6] = l6;
+    }
+    const bool border = i <= 2 || i >= A.mx - 1 || j <= 2 || j >= A.my - 1;
+    const bool wet = !(drytol > 0.0) || q7[0] > drytol;
+    if (border && !wet) {
+        T.flags[((j + 1) / ADJ_TILE) * T.ntx + (i + 1) / ADJ_TILE] = 1;
+    } else if (wet) {
+        const int ia = max(i - ADJ_HALO, -1), ib = min(i + ADJ_HALO, A.mx + 2);
+        const int ja = max(j - ADJ_HALO, -1), jb = min(j + ADJ_HALO, A.my + 2);
+        for (int ty = (ja + 1) / ADJ_TILE; ty <= (jb + 1) / ADJ_TILE; ty++)
+            for (int tx = (ia + 1) / ADJ_TILE; tx <= (ib + 1) / ADJ_TILE; tx++)
+                T.flags[ty * T.ntx + tx] = 1;
+    }
+}
+
+// Compact the marked tiles into the list (ballot within the wavefront, one
+// counter atomic per wavefront) and reset the marks.  The list's order does
+// not matter: every tile pass writes only its own cells.
+__global__ __launch_bounds__(64) void k_adjoint_list(AdjTiles T, int ntiles)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const bool f = t < ntiles && T.flags[t] != 0;
+    if (f) T.flags[t] = 0;
+    const unsigned long long b = __ballot(f);
+    if (b == 0ull) return;
+    int base = 0;
+    if (threadIdx.x == 0) base = atomicAdd(T.count, (int)__popcll(b));
+    base = __shfl(base, 0);
+    if (f) {
+        const unsigned long long below = b & ((1ull << threadIdx.x) - 1ull);
+        T.list[base + (int)__popcll(below)] = t;
+    }
+}
+
+__global__ void k_adjoint_count_zero(int* count)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) count[0] = 0;
+}
+
+// Sum of the per-cell parameter gradients, in a fixed order: block (b, p)
+// sums chunk b of direction p's plane (strided partial sums, then a tree in
+// LDS) into part[p][b]; k_adjoint_gp_final adds the ADJ_GP_PARTS partials of
+// each direction the same way.
+constexpr int ADJ_GP_PARTS = 256;
+__global__ __launch_bounds__(256) void k_adjoint_gp(const double* __restrict__ W, size_t N,
+                                                    double* __restrict__ part)
+{
+    __shared__ double sh[256];
+    const double* g = W + (size_t)(dclaw_adj::W_GP + blockIdx.y) * N;
+    const size_t chunk = (N + ADJ_GP_PARTS - 1) / ADJ_GP_PARTS;
+    const size_t lo = (size_t)blockIdx.x * chunk;
+    const size_t hi = lo + chunk < N ? lo + chunk : N;
+    double s = 0.0;
+    for (size_t n = lo + threadIdx.x; n < hi; n += 256) s += g[n];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.y * ADJ_GP_PARTS + blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(ADJ_GP_PARTS) void k_adjoint_gp_final(const double* __restrict__ part,
+                                                                   double* __restrict__ out)
+{
+    __shared__ double sh[ADJ_GP_PARTS];
+    sh[threadIdx.x] = part[blockIdx.x * ADJ_GP_PARTS + threadIdx.x];
+    __syncthreads();
+    for (int h = ADJ_GP_PARTS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// Adjoint of the gauge samples (k_gauges): lam_q += d(out)/dq^T lam_out for
+// every gauge, gauges in order (one thread, so gauges sharing a cell add in a
+// fixed order).  Gauges outside the patch have no derivative.
+__global__ void k_gauges_adj(const double* __restrict__ q, const double* __restrict__ aux,
+                             int maux, int mx, int my, double xlower, double ylower, double dx,
+                             double dy, double drytol, const double* __restrict__ xy, int ng,
+                             const double* __restrict__ lam_g, double* lam_q)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int g = 0; g < ng; g++) {
+        const double fx = (xy[2 * g] - xlower) / dx, fy = (xy[2 * g + 1] - ylower) / dy;
+        const int ic = (int)floor(fx) + 1, jc = (int)floor(fy) + 1;
+        const int i0 = (int)floor(fx + 0.5), j0 = (int)floor(fy + 0.5);
+        const double xo = fx + 0.5 - (double)i0, yo = fy + 0.5 - (double)j0;
+        if (!(fx >= 0.5 && fy >= 0.5 && i0 >= 1 && i0 + 1 <= mx && j0 >= 1 && j0 + 1 <= my))
+            continue;
+        const double* l = lam_g + 8 * g;
+        bool wet = true;
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++)
+                if (!(q[qidx(0, i0 + di, j0 + dj, mx)] > drytol)) wet = false;
+        if (wet) {
+            // out[m] = sum_c w_c q(m, c) (m < 7); out[7] = out[0] + sum_c w_c (b_c - q(6, c))
+            for (int dj = 0; dj < 2; dj++)
+                for (int di = 0; di < 2; di++) {
+                    const double wgt = (di ? xo : 1.0 - xo) * (dj ? yo : 1.0 - yo);
+                    double* o = lam_q + qidx(0, i0 + di, j0 + dj, mx);
+                    for (int m = 0; m < 7; m++) o[m] += wgt * l[m];
+                    o[0] += wgt * l[7];
+                    o[6] -= wgt * l[7];
+                }
+        } else {
+            double* o = lam_q + qidx(0, ic, jc, mx);
+            for (int m = 0; m < 7; m++) o[m] += l[m];
+            o[0] += l[7];
+            o[6] -= l[7];
+        }
+    }
+    (void)aux;
+    (void)maux;
+}
+
 int check_patch(int mx, int my, int maux)
 {
     if (mx < 1 || my < 1) return fail(DCLAW_EINVAL, "mx and my must be >= 1");
@@ -998,6 +1203,114 @@ int dclaw_stepgrid_tl_dev(const double* q_in, const double* dq_in, double* q_out
     hipLaunchKernelGGL(k_stepgrid_tl, dim3(nstrips * nseg), dim3(64), 0, (hipStream_t)stream, q_in,
                        dq_in, q_out, dq_out, aux, maux, mx, my, dx, dy, dt_dev, params_dev,
                        dparams_dev, cfl_dev, seg_rows, nstrips, b0, b1, b2, b3);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+size_t dclaw_adjoint_workspace_doubles(int mx, int my)
+{
+    if (mx < 1 || my < 1) return 0;
+    // the stage planes, the parameter-gradient partial sums, then the tile
+    // marks, the active-tile list and its counter (ints)
+    const size_t ntiles = (size_t)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) *
+                          (size_t)((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+    return (size_t)dclaw_adj::W_FIELDS * (size_t)(mx + 4) * (size_t)(my + 4) +
+           (size_t)dclaw_adj::MAXP * ADJ_GP_PARTS + (2 * ntiles + 2 + 1) / 2;
+}
+
+int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int mx, int my,
+                           double dx, double dy, const int* bc, const double* dt_dev,
+                           const dclaw_params* params_dev, const dclaw_params* dparams_dev, int np,
+                           const double* lam_out, double* lam_in, double* q_out, double* grad_dev,
+                           double* work, size_t work_doubles, void* stream)
+{
+    if (int rcb = check_bc(bc)) return rcb;
+    if (int rc = check_patch(mx, my, maux)) return rc;
+    if (mx < 2 || my < 2) return fail(DCLAW_EINVAL, "the adjoint step needs mx, my >= 2");
+    if (!q_in || !aux || !dt_dev || !params_dev || !lam_out || !lam_in || !work)
+        return fail(DCLAW_EINVAL, "null pointer");
+    if (np < 0 || np > dclaw_adj::MAXP) return fail(DCLAW_EINVAL, "np must be in [0, 4]");
+    if (np > 0 && (!dparams_dev || !grad_dev))
+        return fail(DCLAW_EINVAL, "parameter directions need dparams and grad");
+    if ((q_out && (q_out == q_in || q_out == lam_out || q_out == lam_in)) || lam_in == q_in ||
+        lam_in == lam_out)
+        return fail(DCLAW_EINVAL, "q_out and lam_in must not alias q_in, lam_out or each other");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+    if (work_doubles < dclaw_adjoint_workspace_doubles(mx, my))
+        return fail(DCLAW_EINVAL, "adjoint workspace too small");
+    dclaw_adj::AdjArgs A;
+    A.q_in = q_in;
+    A.aux = aux;
+    A.maux = maux;
+    A.mx = mx;
+    A.my = my;
+    A.dx = dx;
+    A.dy = dy;
+    A.dt = 0.0;
+    for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
+    std::memset(&A.P, 0, sizeof(A.P));
+    A.dP = dparams_dev;
+    A.np = np;
+    A.lam_out = lam_out;
+    A.lam_in = lam_in;
+    A.q_out = q_out;
+    A.W = work;
+    A.N = (size_t)(mx + 4) * (size_t)(my + 4);
+    A.Pd = params_dev;
+    A.dt_dev = dt_dev;
+    const size_t nb = (A.N + ADJ_BLOCK - 1) / ADJ_BLOCK;
+    if (nb > 0x7fffffffUL) return fail(DCLAW_EINVAL, "patch too large for one launch");
+    AdjTiles T;
+    T.ntx = (mx + 4 + ADJ_TILE - 1) / ADJ_TILE;
+    T.nty = (my + 4 + ADJ_TILE - 1) / ADJ_TILE;
+    const int ntiles = T.ntx * T.nty;
+    int* ints = (int*)(work + (size_t)dclaw_adj::W_FIELDS * A.N +
+                       (size_t)dclaw_adj::MAXP * ADJ_GP_PARTS);
+    T.flags = ints;
+    T.list = ints + ntiles;
+    T.count = ints + 2 * (size_t)ntiles;
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 tg((unsigned)std::min(ntiles, ADJ_GRID)), tb(64);
+    hipLaunchKernelGGL(k_adjoint_count_zero, dim3(1), dim3(1), 0, st, T.count);
+    hipLaunchKernelGGL(k_adjoint_dense, dim3((unsigned)nb), dim3(ADJ_BLOCK), 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint_list, dim3((ntiles + 63) / 64), dim3(64), 0, st, T, ntiles);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REC_PREPARE>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REC_TAUDIR>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REC_WAVES>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REC_FLUX>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REC_SPLIT>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REC_UPDATE>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REV_SRC2>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REV_SPLIT>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REV_FLUX>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REV_WAVES>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REV_TAUDIR>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REV_PREPARE>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REV_BC>, tg, tb, 0, st, A, T);
+    if (np > 0) {
+        double* part = work + (size_t)dclaw_adj::W_FIELDS * A.N;
+        hipLaunchKernelGGL(k_adjoint_gp, dim3(ADJ_GP_PARTS, np), dim3(256), 0, st, work, A.N,
+                           part);
+        hipLaunchKernelGGL(k_adjoint_gp_final, dim3(np), dim3(ADJ_GP_PARTS), 0, st, part,
+                           grad_dev);
+    }
+    hipLaunchKernelGGL(k_adjoint<ADJ_CLEAR>, tg, tb, 0, st, A, T);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_gauges_adj_dev(const double* q, const double* aux, int maux, int mx, int my,
+                         double xlower, double ylower, double dx, double dy, double dry_tolerance,
+                         const double* xy_dev, int ngauges, const double* lam_gauges_dev,
+                         double* lam_q_dev, void* stream)
+{
+    if (int rc = check_patch(mx, my, maux)) return rc;
+    if (!q || !aux || !xy_dev || !lam_gauges_dev || !lam_q_dev || ngauges < 1)
+        return fail(DCLAW_EINVAL, "bad gauge args");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+    hipLaunchKernelGGL(k_gauges_adj, dim3(1), dim3(64), 0, (hipStream_t)stream, q, aux, maux, mx,
+                       my, xlower, ylower, dx, dy, dry_tolerance, xy_dev, ngauges, lam_gauges_dev,
+                       lam_q_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

@@ -249,6 +249,43 @@ int dclaw_gauges_tl_dev(const double* q, const double* dq, const double* aux, in
                         double dry_tolerance, const double* xy_dev, int ngauges, double* out_dev,
                         double* dout_dev, void* stream);
 
+/* The adjoint (reverse-mode) fused step: the transpose of the tangent-linear
+ * step's Jacobian (dclaw_stepgrid_tl_dev) at q_in, dt fixed.  Given lam_out
+ * (the gradient of a loss with respect to q_{n+1}, layout of q, ghosts
+ * included), writes lam_in = J_q^T lam_out (with bc set, the ghosts of q_in
+ * are not read and get 0: their adjoint goes to the interior cells the
+ * boundary map loads them from) and, for the np <= 4 parameter directions
+ * dparams_dev[k], grad_dev[k] = lam_out . J_P dparams_dev[k] (device
+ * doubles).  If q_out is not null the step's own result q_{n+1} is written
+ * there, bit for bit as dclaw_stepgrid_bc_dev (the record pass).  work:
+ * caller-owned device scratch of at least dclaw_adjoint_workspace_doubles(mx,
+ * my) doubles, zero-filled before its first use; each call leaves it as the
+ * next call with the same mx, my needs it (do not modify it in between).
+ * q_out and lam_in must not alias q_in, lam_out or each other.  mx, my >= 2.
+ * The work is confined to the 8 x 8 tiles within 5 cells of a wet cell (and
+ * the border band); elsewhere the step is the dry map, whose adjoint passes
+ * only bdif.  The reference has no adjoint; this is the "adjoint step"
+ * call of the device API a gradient-based caller (parameter fitting, data
+ * assimilation) binds, with rpn2/rpt2/flux2/src2 (rpn2_dclaw.f,
+ * rpt2_dclaw.f, flux2fw.f, src2.f90) differentiated as the step computes
+ * them. */
+size_t dclaw_adjoint_workspace_doubles(int mx, int my);
+int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int mx, int my,
+                           double dx, double dy, const int* bc, const double* dt_dev,
+                           const dclaw_params* params_dev, const dclaw_params* dparams_dev, int np,
+                           const double* lam_out, double* lam_in, double* q_out, double* grad_dev,
+                           double* work, size_t work_doubles, void* stream);
+
+/* Adjoint of dclaw_gauges_dev: lam_q_dev (layout of q) += the transpose of
+ * the gauge map's derivative at q applied to lam_gauges_dev (8 doubles per
+ * gauge, the adjoint of each gauge value), through the branch and weights
+ * the values take.  Gauges outside the patch contribute nothing.  No
+ * reference counterpart. */
+int dclaw_gauges_adj_dev(const double* q, const double* aux, int maux, int mx, int my,
+                         double xlower, double ylower, double dx, double dy, double dry_tolerance,
+                         const double* xy_dev, int ngauges, const double* lam_gauges_dev,
+                         double* lam_q_dev, void* stream);
+
 /* AMR transfers between refinement levels, on device memory (the numerical
  * part of the AMR driver's coarse/fine coupling; coordinate_system 1).
  * Arrays: q(7, nx, ny), aux(maux, nx, ny) (aux component 1 = bed), indices

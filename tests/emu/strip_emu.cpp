@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "dclaw_adjoint.h"
 #include "dclaw_amr.h"
 #include "dclaw_strip.h"
 #include "dclaw_tangent.h"
@@ -490,5 +491,63 @@ extern "C" int emu_stepgrid_tl(const dclaw_params* P, const dclaw_params* dP, co
             A.tile_flag = nullptr;
             run_strip(*P, dummy, cflgrid, MODE_TANGENT, 0, &Ptl, &A);
         }
+    return 0;
+}
+
+// The adjoint step (dclaw_adjoint.h) with its stages run over the grid in
+// order on the host: q_out is the record pass's step, lam_in = J_q^T lam_out
+// and gp[k] = lam_out . J_P dP[k] summed over the cells in index order.
+extern "C" int emu_adjoint_step(const dclaw_params* P, const dclaw_params* dP, int np,
+                                const double* q_in, const double* aux, int maux, int mx, int my,
+                                double dx, double dy, double dt, const int* bc,
+                                const double* lam_out, double* lam_in, double* q_out, double* gp)
+{
+    namespace D = dclaw_adj;
+    if (np < 0 || np > D::MAXP) return 1;
+    D::AdjArgs A;
+    A.q_in = q_in;
+    A.aux = aux;
+    A.maux = maux;
+    A.mx = mx;
+    A.my = my;
+    A.dx = dx;
+    A.dy = dy;
+    A.dt = dt;
+    for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
+    A.P = *P;
+    A.dP = dP;
+    A.np = np;
+    A.lam_out = lam_out;
+    A.lam_in = lam_in;
+    A.q_out = q_out;
+    A.N = (size_t)(mx + 4) * (size_t)(my + 4);
+    A.Pd = nullptr;
+    A.dt_dev = nullptr;
+    std::vector<double> W((size_t)D::W_FIELDS * A.N, 0.0);
+    A.W = W.data();
+    const D::TP Pz = D::zero_params(*P);
+    auto all = [&](auto fn) {
+        for (int j = -1; j <= my + 2; j++)
+            for (int i = -1; i <= mx + 2; i++) fn(A, Pz, i, j);
+    };
+    all(D::rec_prepare);
+    all(D::rec_taudir);
+    all(D::rec_waves);
+    all(D::rec_flux);
+    all(D::rec_split);
+    all(D::rec_update);
+    all(D::rev_src2);
+    all(D::rev_split);
+    all(D::rev_flux);
+    all(D::rev_waves);
+    all(D::rev_taudir);
+    all(D::rev_prepare);
+    for (int j = -1; j <= my + 2; j++)
+        for (int i = -1; i <= mx + 2; i++) D::rev_bc(A, i, j);
+    for (int k = 0; k < np; k++) {
+        double s = 0.0;
+        for (size_t n = 0; n < A.N; n++) s += W[(size_t)(D::W_GP + k) * A.N + n];
+        gp[k] = s;
+    }
     return 0;
 }

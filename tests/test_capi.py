@@ -100,3 +100,23 @@ def test_split_workspace_and_dry_rows(L):
     # the split step rejects a workspace smaller than one int per tile
     rc = L.dclaw_stepgrid_split_dev(1, 2, 3, 1, 100, 100, 1.0, 1.0, None, 4, 5, 6, 7, 1, None)
     assert rc == -1 and b"tile_flags" in L.dclaw_last_error()
+
+
+def test_adjoint_workspace_and_validation(L):
+    """The adjoint step's workspace size and its argument checks (host side,
+    before any device work)."""
+    from dclaw_amd import adjoint as AD
+    n = L.dclaw_adjoint_workspace_doubles(10, 6)
+    assert n == AD.workspace_doubles(10, 6) and n > 100 * 14 * 10
+    assert L.dclaw_adjoint_workspace_doubles(0, 6) == 0
+    bc = (ctypes.c_int * 4)(1, 1, 1, 1)
+    args = lambda np_, work, nwork, mx=10: (1, 2, 1, mx, 6, 1.0, 1.0, bc, 3, 4, 5, np_, 6, 7, 8,
+                                            9, work, nwork, None)
+    assert L.dclaw_adjoint_step_dev(*args(5, 10, n)) == -1
+    assert b"np" in L.dclaw_last_error()
+    assert L.dclaw_adjoint_step_dev(*args(1, 10, n - 1)) == -1
+    assert b"workspace" in L.dclaw_last_error()
+    assert L.dclaw_adjoint_step_dev(*args(1, None, n)) == -1
+    assert L.dclaw_adjoint_step_dev(*args(1, 10, n, mx=1)) == -1
+    assert b"mx, my >= 2" in L.dclaw_last_error()
+    assert L.dclaw_gauges_adj_dev(1, 2, 1, 10, 6, 0.0, 0.0, 1.0, 1.0, 1e-3, 3, 0, 4, 5, None) == -1

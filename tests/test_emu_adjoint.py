@@ -1,0 +1,175 @@
+"""The adjoint step (dclaw_adjoint.h), run on the CPU by the host emulator of
+its stages:
+
+* its record pass is the step, bit for bit (the stages, laid out over the
+  grid, reproduce the strip body's arithmetic exactly);
+* it is the transpose of the tangent-linear step (pinned against finite
+  differences in test_emu_tangent.py): for random lambda, dq and parameter
+  weights a, <lambda, TL(dq, sum a_k dP_k)> = <J^T lambda, dq> + sum a_k g_k
+  to rounding, for every boundary condition and for the other branches of
+  the scheme (first order, transverse 0/1, segregation, the other source
+  closures, Manning friction, entrainment);
+* a gradient it gives agrees with central differences of the step directly.
+
+lambda and dq are scaled per component (pore pressure is ~1e4 times the
+other components), so every component takes part in the identity."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from dclaw_amd.params import CParams, to_c
+from helpers import advance_oracle, slope_patch
+from test_emu_tangent import _fwd, _tl
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _adj(lib, c, dPs, q, aux, dx, dy, dt, bc, lam):
+    mx, my = q.shape[1] - 4, q.shape[2] - 4
+    lin = np.zeros_like(q, order="F")
+    qo = np.zeros_like(q, order="F")
+    gp = np.zeros(4)
+    arr = (CParams * max(1, len(dPs)))(*dPs)
+    rc = lib.emu_adjoint_step(ctypes.byref(c), arr, len(dPs), q.ctypes.data_as(_dp),
+                              aux.ctypes.data_as(_dp), aux.shape[0], mx, my, ctypes.c_double(dx),
+                              ctypes.c_double(dy), ctypes.c_double(dt),
+                              None if bc is None else (ctypes.c_int * 4)(*bc),
+                              lam.ctypes.data_as(_dp), lin.ctypes.data_as(_dp),
+                              qo.ctypes.data_as(_dp), gp.ctypes.data_as(_dp))
+    assert rc == 0
+    return qo, lin, gp[:len(dPs)]
+
+
+def _scaled(rng, q):
+    """Random array with component m scaled by max|q_m| (+1 for zero components)."""
+    x = rng.normal(size=q.shape)
+    for m in range(7):
+        x[m] *= np.abs(q[m]).max() + 1.0
+    return np.asfortranarray(x)
+
+
+def _dot_identity(lib, pt, q, aux, dt, bc, names, seed):
+    c = to_c(pt.params)
+    rng = np.random.default_rng(seed)
+    lam = np.asfortranarray(rng.normal(size=q.shape))
+    for m in range(7):
+        lam[m] /= np.abs(q[m]).max() + 1.0
+    scales = [abs(getattr(c, n)) or 1.0 for n in names]
+    dPs = [CParams(**{n: s}) for n, s in zip(names, scales)]
+    qo, lin, gp = _adj(lib, c, dPs, q, aux, pt.dx, pt.dy, dt, bc, lam)
+    qf, _ = _fwd(lib, c, q, aux, pt.dx, pt.dy, dt, 8, bc)
+    assert np.array_equal(qo, qf), "record pass differs from the step"
+    dq = _scaled(rng, q)
+    a = rng.normal(size=len(names))
+    dP = CParams(**{n: ak * s for n, ak, s in zip(names, a, scales)})
+    _, dqo, _ = _tl(lib, c, dP, q, dq, aux, pt.dx, pt.dy, dt, 8, bc)
+    lhs = float((lam * dqo).sum())
+    rhs = float((lin * dq).sum()) + float(gp @ a)
+    scale = float(np.abs(lam * dqo).sum())
+    assert scale > 0.0
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
+    # parameters alone, one at a time
+    for k, (n, s) in enumerate(zip(names, scales)):
+        _, dqp, _ = _tl(lib, c, CParams(**{n: s}), q, np.zeros_like(q, order="F"), aux, pt.dx,
+                        pt.dy, dt, 8, bc)
+        l2 = float((lam * dqp).sum())
+        sc = float(np.abs(lam * dqp).sum())
+        assert abs(l2 - gp[k]) <= 1e-12 * sc + 1e-300, (n, l2, gp[k])
+    return lin
+
+
+@pytest.fixture(scope="module")
+def moving(oracle_mod):
+    pt = slope_patch(70, 40, slope=0.9, phi_deg=22.0, seed=11)
+    q, aux, dt = advance_oracle(oracle_mod, pt, 8)
+    return pt, np.asfortranarray(q), np.asfortranarray(aux), dt
+
+
+@pytest.mark.parametrize("bc", [None, (1, 1, 1, 1), (3, 3, 3, 3), (1, 3, 3, 1)])
+def test_record_is_step_and_adjoint_is_transpose(emu_lib, moving, bc):
+    pt, q, aux, dt = moving
+    lin = _dot_identity(emu_lib, pt, q, aux, dt, bc, ["phi", "kref", "mu", "m_crit"], seed=1)
+    if bc is not None:
+        # ghost entries of q_n are not read with the boundary map on load
+        assert not lin[:, :2].any() and not lin[:, -2:].any()
+        assert not lin[:, :, :2].any() and not lin[:, :, -2:].any()
+
+
+@pytest.mark.parametrize("variant", [
+    dict(order=1), dict(transverse=0), dict(transverse=1),
+    dict(segregation=1, delta_kr_order=1.5), dict(src2method=1), dict(alphamethod=1),
+    dict(friction_forcing=1, manning_coefficient=0.06), dict(src2method=-1),
+    dict(friction_forcing=1, manning_coefficient=0.06, dd_manning=1),
+])
+def test_adjoint_through_other_branches(emu_lib, oracle_mod, variant):
+    pt = slope_patch(60, 36, slope=0.9, phi_deg=22.0, seed=5)
+    for k, v in variant.items():
+        setattr(pt.params, k, v)
+    if variant.get("segregation"):
+        pt.q[5] = 0.4 * pt.q[0]
+    q, aux, dt = advance_oracle(oracle_mod, pt, 6)
+    q, aux = np.asfortranarray(q), np.asfortranarray(aux)
+    names = ["phi", "kref"] + (["manning_coefficient"] if variant.get("friction_forcing") else [])
+    _dot_identity(emu_lib, pt, q, aux, dt, (1, 1, 3, 3), names, seed=2)
+
+
+def test_adjoint_with_entrainment(emu_lib, oracle_mod):
+    pt = slope_patch(48, 30, slope=0.9, phi_deg=22.0, seed=9)
+    pt.params.entrainment = 1
+    pt.params.entrainment_rate = 0.2
+    aux = np.zeros((7,) + pt.q.shape[1:], order="F")
+    aux[0] = pt.aux[0]
+    aux[6] = 0.3
+    pt.aux = aux
+    q, aux, dt = advance_oracle(oracle_mod, pt, 5)
+    q, aux = np.asfortranarray(q), np.asfortranarray(aux)
+    _dot_identity(emu_lib, pt, q, aux, dt, (1, 1, 1, 1), ["phi", "entrainment_rate"], seed=3)
+
+
+def test_adjoint_on_a_dry_patch_is_the_bdif_clip(emu_lib, oracle_mod):
+    """All dry: the step maps q to the dry state (zeros, bdif clipped at 0),
+    so lambda_n is lambda_{n+1}'s bdif entry where bdif > 0 and 0 elsewhere."""
+    pt = slope_patch(30, 20, slope=0.9, phi_deg=22.0, seed=1)
+    q = np.zeros_like(pt.q, order="F")
+    rng = np.random.default_rng(0)
+    q[6] = rng.normal(size=q[6].shape)
+    lam = np.asfortranarray(rng.normal(size=q.shape))
+    _, lin, gp = _adj(emu_lib, to_c(pt.params), [CParams(phi=1.0)], q, np.asfortranarray(pt.aux),
+                      pt.dx, pt.dy, 0.01, None, lam)
+    want = np.zeros_like(q)
+    want[6] = np.where(q[6] > 0.0, lam[6], 0.0)
+    assert np.array_equal(lin, want) and gp[0] == 0.0
+
+
+def test_adjoint_gradient_matches_finite_differences(emu_lib, moving):
+    """d/dh(cell) and d/dphi of a weighted sum of the step's depth and x
+    momentum (one step, fixed dt) against central differences of the step."""
+    pt, q, aux, dt = moving
+    c = to_c(pt.params)
+    rng = np.random.default_rng(7)
+    lam = np.zeros_like(q, order="F")
+    for m in (0, 1):   # depth and x momentum (friction acts on the momentum)
+        lam[m, 2:-2, 2:-2] = rng.normal(size=lam[m, 2:-2, 2:-2].shape)
+    _, lin, gp = _adj(emu_lib, c, [CParams(phi=1.0)], q, aux, pt.dx, pt.dy, dt, (1, 1, 1, 1), lam)
+
+    def loss(qq, cc):
+        return float((lam * _fwd(emu_lib, cc, np.asfortranarray(qq), aux, pt.dx, pt.dy, dt, 8,
+                                 (1, 1, 1, 1))[0]).sum())
+    wet = np.argwhere(q[0, 2:-2, 2:-2] > 0.5)
+    checked = 0
+    for i, j in wet[rng.choice(len(wet), size=min(6, len(wet)), replace=False)]:
+        eps = 1e-6 * q[0, i + 2, j + 2]
+        qp, qm = q.copy(order="F"), q.copy(order="F")
+        qp[0, i + 2, j + 2] += eps
+        qm[0, i + 2, j + 2] -= eps
+        fd = (loss(qp, c) - loss(qm, c)) / (2 * eps)
+        assert lin[0, i + 2, j + 2] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+        checked += 1
+    assert checked > 0
+    eps = 1e-6 * c.phi
+    cp, cm = to_c(pt.params), to_c(pt.params)
+    cp.phi += eps
+    cm.phi -= eps
+    fd = (loss(q, cp) - loss(q, cm)) / (2 * eps)
+    assert gp[0] == pytest.approx(fd, rel=1e-5)
