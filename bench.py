@@ -75,6 +75,9 @@ def parse():
                     help="timed adjoint (reverse-mode) steps on the same workload, gradient "
                          "with respect to the state and (phi, kref), reported beside the "
                          "forward figures (0 disables)")
+    ap.add_argument("--graph-steps", type=int, default=10,
+                    help="ensemble / flume: also time the loop captured as a HIP graph of this "
+                         "many steps (0 disables)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy (the HBM "
                          "counter calibration dispatch for rocprofv3 --pmc runs)")
@@ -381,6 +384,23 @@ def bench_flume(args):
     r.run(nsteps, 1e-3, 0.9, 1e9)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
+    q_eager = r.q()
+    # the same run as replays of a captured graph of 100 steps
+    t_fwd_graph = None
+    if args.graph_steps > 0 and nsteps % 100 == 0:
+        rg = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
+        rg.set_dt(1e-3)
+        rg.cfl.zero_()
+        cap = S.CapturedSteps(rg, 100, 0.9, 1e9)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps // 100):
+            cap.replay()
+        torch.cuda.synchronize()
+        t_fwd_graph = time.perf_counter() - t0
+        if not np.array_equal(rg.q(), q_eager):
+            raise RuntimeError("graph-captured run differs from the eager run")
+        del cap, rg
 
     # reverse sweep: loss, d/dphi, d/dkref and dloss/dq0 in one pass
     AD.gauge_loss_gradient_adjoint(pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy,
@@ -429,7 +449,10 @@ def bench_flume(args):
                    "mx": pt.mx, "my": pt.my, "seg_rows": seg, "seed": args.seed,
                    "checkpoint_every": 100},
         "forward": {"seconds": t_fwd, "value": nsteps * cells / t_fwd,
-                    "ms_per_step": t_fwd / nsteps * 1e3},
+                    "ms_per_step": t_fwd / nsteps * 1e3,
+                    "graph_seconds": t_fwd_graph,
+                    "graph_ms_per_step": None if t_fwd_graph is None
+                    else t_fwd_graph / nsteps * 1e3},
         "gradient": {"seconds": t_adj, "loss": loss_a, "dloss": grads_a,
                      "method": "reverse sweep (one forward run, recompute, adjoint steps)",
                      "units": {"phi": "per radian", "kref": "per m^2", "m0": "per unit m0"}},
@@ -497,6 +520,33 @@ def bench_ensemble(args):
     t_p = time.perf_counter() - t_start
     value = cells * args.steps / t_b
     plain = cells * nplain / t_p
+    # the same batched loop captured as a HIP graph (steps + dt rule), on a
+    # second ensemble from the same initial state: the warmup steps eager,
+    # then the timed steps as graph replays (the same step range as the eager
+    # timing)
+    graph = None
+    if args.graph_steps > 0:
+        G = args.graph_steps + (args.graph_steps % 2)
+        eg = S.Ensemble(members, seg_rows=seg, split_src2=not args.fused,
+                        ordered=not args.ens_list_order)
+        eg.dt.fill_(1e-3)
+        eg.cfl.zero_()
+        for _ in range(args.warmup):
+            eg.step()
+            S.dt_update(eg.dt, eg.cfl, 0.9, 1e9)
+        cap = S.CapturedSteps(eg, G, 0.9, 1e9)
+        torch.cuda.synchronize()
+        nrep = max(1, args.steps // G)
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            cap.replay()
+        torch.cuda.synchronize()
+        t_g = (time.perf_counter() - t0) / (nrep * G)
+        graph = {"steps_per_graph": G, "steps": nrep * G, "ms_per_step": t_g * 1e3,
+                 "value": cells / t_g, "over_plain": (cells / t_g) / plain}
+        del cap, eg
+        print(f"[bench] graph-captured batched {t_g * 1e3:.3f} ms/step", file=sys.stderr,
+              flush=True)
     achieved = BYTES_PER_CELL_FWD * cells / kern_avg_s / 1e9
     print(f"[bench] batched {t_b / args.steps * 1e3:.3f} ms/step, plain "
           f"{t_p / nplain * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
@@ -549,6 +599,7 @@ def bench_ensemble(args):
                      "kernel": ("k_step_batched<0>" if args.fused
                                 else "k_step_batched<1> + k_src2_batched (one step)"),
                      "kernel_avg_ms": kern_avg_s * 1e3},
+        "graph": graph,
         "tangent": tangent,
         "cpu_baseline": None,
     }

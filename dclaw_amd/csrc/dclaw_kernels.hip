@@ -796,7 +796,7 @@ __global__ void k_gauges(const double* __restrict__ q, const double* __restrict_
 enum AdjStage {
     ADJ_REC_PREPARE, ADJ_REC_TAUDIR, ADJ_REC_WAVES, ADJ_REC_FLUX, ADJ_REC_SPLIT, ADJ_REC_UPDATE,
     ADJ_REV_SRC2, ADJ_REV_SPLIT, ADJ_REV_FLUX, ADJ_REV_WAVES, ADJ_REV_TAUDIR, ADJ_REV_PREPARE,
-    ADJ_REV_BC, ADJ_CLEAR, ADJ_NSTAGES
+    ADJ_REV_BC, ADJ_NSTAGES
 };
 constexpr int ADJ_TILE = 8;       // tile side (64 cells: one wavefront)
 constexpr int ADJ_HALO = 5;       // cells around a wet cell whose tiles are active
@@ -834,13 +834,26 @@ __global__ __launch_bounds__(64) void k_adjoint(dclaw_adj::AdjArgs A, AdjTiles T
         else if constexpr (STAGE == ADJ_REV_WAVES) dclaw_adj::rev_waves(A, Pz, i, j);
         else if constexpr (STAGE == ADJ_REV_TAUDIR) dclaw_adj::rev_taudir(A, Pz, i, j);
         else if constexpr (STAGE == ADJ_REV_PREPARE) dclaw_adj::rev_prepare(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REV_BC) dclaw_adj::rev_bc(A, i, j);
-        else {
-            // clear the tile's planes (bdif's is rewritten densely each call)
-            const size_t n = dclaw_adj::cix(A, i, j);
-            for (int f = 0; f < dclaw_adj::W_FIELDS; f++)
-                if (f != dclaw_adj::W_PC + dclaw_adj::PC_BDIF) A.W[(size_t)f * A.N + n] = 0.0;
-        }
+        else dclaw_adj::rev_bc(A, i, j);
+    }
+}
+
+// Clear the planes of the tiles of the previous call's list that are not
+// marked for this call (bdif's plane is rewritten densely each call; every
+// tile pass rewrites all planes of its own cells, so the tiles that stay
+// active need no clearing).
+__global__ __launch_bounds__(64) void k_adjoint_clear_stale(dclaw_adj::AdjArgs A, AdjTiles T)
+{
+    const int nold = *T.count;
+    for (int k = blockIdx.x; k < nold; k += gridDim.x) {
+        const int t = T.list[k];
+        if (T.flags[t] != 0) continue;
+        const int i = -1 + (t % T.ntx) * ADJ_TILE + (int)(threadIdx.x % ADJ_TILE);
+        const int j = -1 + (t / T.ntx) * ADJ_TILE + (int)(threadIdx.x / ADJ_TILE);
+        if (i > A.mx + 2 || j > A.my + 2) continue;
+        const size_t n = dclaw_adj::cix(A, i, j);
+        for (int f = 0; f < dclaw_adj::W_FIELDS; f++)
+            if (f != dclaw_adj::W_PC + dclaw_adj::PC_BDIF) A.W[(size_t)f * A.N + n] = 0.0;
     }
 }
 
@@ -1271,8 +1284,9 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
     T.count = ints + 2 * (size_t)ntiles;
     hipStream_t st = (hipStream_t)stream;
     const dim3 tg((unsigned)std::min(ntiles, ADJ_GRID)), tb(64);
-    hipLaunchKernelGGL(k_adjoint_count_zero, dim3(1), dim3(1), 0, st, T.count);
     hipLaunchKernelGGL(k_adjoint_dense, dim3((unsigned)nb), dim3(ADJ_BLOCK), 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint_clear_stale, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint_count_zero, dim3(1), dim3(1), 0, st, T.count);
     hipLaunchKernelGGL(k_adjoint_list, dim3((ntiles + 63) / 64), dim3(64), 0, st, T, ntiles);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_PREPARE>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_TAUDIR>, tg, tb, 0, st, A, T);
@@ -1294,7 +1308,6 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
         hipLaunchKernelGGL(k_adjoint_gp_final, dim3(np), dim3(ADJ_GP_PARTS), 0, st, part,
                            grad_dev);
     }
-    hipLaunchKernelGGL(k_adjoint<ADJ_CLEAR>, tg, tb, 0, st, A, T);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

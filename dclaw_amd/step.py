@@ -462,3 +462,28 @@ class Ensemble:
 
     def q(self, k: int) -> np.ndarray:
         return to_host_q(self.current(k))
+
+
+class CapturedSteps:
+    """k steps of a runner (PatchRunner or Ensemble: its step() followed by
+    the device dt rule), captured once as a HIP graph; replay() advances the
+    runner by k steps with one graph launch.  Everything a step touches is
+    preallocated device memory read through pointers (state buffers, dt, cfl,
+    tile flags, work lists) and no step records a memset, so a replay is the
+    same work as k eager steps, bit for bit.  k must be even: the runners
+    alternate between two state buffers, and an even count leaves them where
+    the capture found them."""
+
+    def __init__(self, runner, k: int, cfl_desired: float, dt_max: float):
+        if k < 2 or k % 2:
+            raise ValueError("k must be an even number >= 2")
+        self.runner = runner
+        self.k = k
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for _ in range(k):
+                runner.step()
+                dt_update(runner.dt, runner.cfl, cfl_desired, dt_max)
+
+    def replay(self) -> None:
+        self.graph.replay()
