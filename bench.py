@@ -551,6 +551,44 @@ def bench_ensemble(args):
     print(f"[bench] batched {t_b / args.steps * 1e3:.3f} ms/step, plain "
           f"{t_p / nplain * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
 
+    # the adjoint step of every member at the current states (lambda a seeded
+    # random field, directions phi and kref): one launch per pass for the
+    # ensemble against the per-member loop of single-patch adjoint steps
+    adjoint = None
+    if args.adjoint_steps > 0:
+        from dclaw_amd import adjoint as AD
+        adj = AD.EnsembleAdjoint(ens, parameters=("phi", "kref"))
+        qs = [ens.current(k) for k in range(ens.n)]
+        gen = torch.Generator(device="cuda").manual_seed(3)
+        lam = [torch.randn(x.shape, dtype=torch.float64, device="cuda", generator=gen) for x in qs]
+        lin = [torch.empty_like(x) for x in qs]
+        adj.step(qs, lam, lin)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.adjoint_steps):
+            adj.step(qs, lam, lin)
+        torch.cuda.synchronize()
+        t_ab = (time.perf_counter() - t0) / args.adjoint_steps
+        adj.step_plain(qs, lam, lin)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.adjoint_steps):
+            adj.step_plain(qs, lam, lin)
+        torch.cuda.synchronize()
+        t_ap = (time.perf_counter() - t0) / args.adjoint_steps
+        t_fwd = t_b / args.steps
+        adjoint = {"ms_per_step": t_ab * 1e3, "value": cells / t_ab, "unit": "cell-updates/s",
+                   "plain_ms_per_step": t_ap * 1e3, "batched_over_plain": t_ap / t_ab,
+                   "fwd_plus_adjoint_value": cells / (t_fwd + t_ab),
+                   "fwd_plus_adjoint_plain_value": cells / (t_p / nplain + t_ap),
+                   "steps": args.adjoint_steps, "directions": "state, phi, kref per member",
+                   "kernel": "dclaw_adjoint_batched: k_adjb_* (record + reverse passes over "
+                             "every member's active tiles)"}
+        print(f"[bench] adjoint batched {t_ab * 1e3:.3f} ms/step, plain {t_ap * 1e3:.3f} ms/step",
+              file=sys.stderr, flush=True)
+        del adj, lam, lin
+        torch.cuda.empty_cache()
+
     # the tangent-linear step (values + d/dphi for every member): batched
     # launch against the per-member loop of single-patch launches
     tangent = None
@@ -599,6 +637,7 @@ def bench_ensemble(args):
                      "kernel": ("k_step_batched<0>" if args.fused
                                 else "k_step_batched<1> + k_src2_batched (one step)"),
                      "kernel_avg_ms": kern_avg_s * 1e3},
+        "adjoint": adjoint,
         "graph": graph,
         "tangent": tangent,
         "cpu_baseline": None,

@@ -28,6 +28,17 @@ _i = ctypes.c_int
 _d = ctypes.c_double
 
 
+class CAdjPatch(ctypes.Structure):
+    """Mirror of ``dclaw_adj_patch`` in include/dclaw_hip.h."""
+
+    _fields_ = [
+        ("q_in", _vp), ("aux", _vp), ("lam_out", _vp), ("lam_in", _vp), ("q_out", _vp),
+        ("work", _vp), ("grad", _vp),
+        ("mx", ctypes.c_int), ("my", ctypes.c_int), ("maux", ctypes.c_int),
+        ("member", ctypes.c_int), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+    ]
+
+
 class DClawError(RuntimeError):
     pass
 
@@ -76,6 +87,9 @@ SIGNATURES = {
     "dclaw_adjoint_workspace_doubles": (ctypes.c_size_t, [_i, _i]),
     "dclaw_adjoint_step_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _d, _d, _ip, _vp, _vp, _vp, _i,
                                               _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "dclaw_adjoint_batch_doubles": (ctypes.c_size_t, [_vp, _i]),
+    "dclaw_adjoint_batched": (ctypes.c_int, [_vp, _vp, _i, _ip, _vp, _vp, _vp, _i, _vp,
+                                             ctypes.c_size_t, _vp]),
     "dclaw_gauges_adj_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _vp, _i, _vp,
                                             _vp, _vp]),
     "dclaw_amr_average_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i,

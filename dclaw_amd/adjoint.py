@@ -214,6 +214,76 @@ class Rollout:
         return lam, grad[:self.np]
 
 
+def _adj_records(rows) -> tuple:
+    """(host array, device copy) of dclaw_adj_patch records."""
+    arr = (_lib.CAdjPatch * len(rows))(*rows)
+    dev = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to("cuda")
+    return arr, dev
+
+
+class EnsembleAdjoint:
+    """The adjoint step of every member of an ``step.Ensemble`` in one set of
+    launches (``dclaw_adjoint_batched``): member k's lambda_{n+1} at its
+    state q_n -> lambda_n, and d/d(parameters) for the named parameters (the
+    same names for every member, each member's own values), with the members'
+    own dt.  Each member keeps its own workspace.  ``step_plain`` is the
+    per-member loop of single-patch adjoint steps, for comparison."""
+
+    def __init__(self, ens, parameters: Sequence[str] = ("phi", "kref")):
+        self.ens = ens
+        self.dirs = _directions(parameters)
+        self.np = len(self.dirs)
+        n = ens.n
+        dev = ens.dt.device
+        self.work = [new_workspace(ens.mx[k], ens.my[k], dev) for k in range(n)]
+        self.work_plain = None
+        self.grad = torch.zeros((n, max(self.np, 1)), dtype=torch.float64, device=dev)
+        # member k's directions at dparams[k * np .. k * np + np - 1]
+        self.dparams = (torch.cat([_record_tensor(d, dev) for _ in range(n) for d in self.dirs])
+                        if self.np else None)
+        self._rec = ctypes.sizeof(CParams)
+        self.batch = None
+
+    def step(self, q: Sequence[torch.Tensor], lam_out: Sequence[torch.Tensor],
+             lam_in: Sequence[torch.Tensor]) -> torch.Tensor:
+        """lam_in[k] = J_k^T lam_out[k] at q[k] for every member (one launch per
+        pass); returns the (n, np) parameter gradients (device)."""
+        e = self.ens
+        rows = []
+        for k in range(e.n):
+            rows.append(_lib.CAdjPatch(
+                _ptr(q[k]), _ptr(e.bed[k]), _ptr(lam_out[k]), _ptr(lam_in[k]), None,
+                _ptr(self.work[k]), _ptr(self.grad[k]) if self.np else None, e.mx[k], e.my[k],
+                e.maux[k], k, e.dx[k], e.dy[k]))
+        host, dev = _adj_records(rows)
+        L = _lib.lib()
+        nb = int(L.dclaw_adjoint_batch_doubles(host, e.n))
+        if self.batch is None or self.batch.numel() < nb:
+            self.batch = torch.empty(nb, dtype=torch.float64, device=e.dt.device)
+        bc = (ctypes.c_int * 4)(*e.bc)
+        _lib.check(L.dclaw_adjoint_batched(
+            _ptr(dev), host, e.n, bc, _ptr(e.dt), _ptr(e.params),
+            None if self.dparams is None else _ptr(self.dparams), self.np, _ptr(self.batch),
+            self.batch.numel(), _stream()))
+        self._keep = dev     # the record copy stays alive until the launches are queued
+        return self.grad[:, :self.np]
+
+    def step_plain(self, q: Sequence[torch.Tensor], lam_out: Sequence[torch.Tensor],
+                   lam_in: Sequence[torch.Tensor]) -> torch.Tensor:
+        """The same by one single-patch adjoint step per member."""
+        e = self.ens
+        if self.work_plain is None:
+            self.work_plain = [new_workspace(e.mx[k], e.my[k], e.dt.device) for k in range(e.n)]
+        for k in range(e.n):
+            rec = e.params[k * self._rec:(k + 1) * self._rec]
+            dpar = (self.dparams[k * self.np * self._rec:(k + 1) * self.np * self._rec]
+                    if self.np else None)
+            adjoint_step(q[k], e.bed[k], e.maux[k], e.mx[k], e.my[k], e.dx[k], e.dy[k], e.bc,
+                         e.dt[k:k + 1], rec, dpar, self.np, lam_out[k], lam_in[k], None,
+                         self.grad[k] if self.np else None, self.work_plain[k])
+        return self.grad[:, :self.np]
+
+
 class _RolloutFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, q0, theta, spec):

@@ -70,7 +70,15 @@ constexpr int NSI = 12;     // split inputs: h, hu, hv, hm2, hchi2, bdif, asdq[6
 constexpr int MAXP = 4;     // parameter directions per call
 
 // Workspace fields (each a plane of N = (mx+4)(my+4) doubles, cell (i, j)
-// at (i+1) + (mx+4)(j+1); an edge's values sit at its right/top cell).
+// at (i+1) + (mx+4)(j+1); an edge's values sit at its right/top cell).  A
+// reverse-pass field shares planes with record fields that are dead by the
+// time it is written (no stage reads a plane it or a neighbour writes in the
+// same pass):
+//   TD, XW, YW    read through the waves' reverse pass  -> then LTD
+//   XE, YE        read through the splits' reverse pass -> then LXE, LYE
+//   G             read by the update                    -> then LXA, LYA,
+//                 and after the flux body's reverse pass   LQ7
+//   QN            read by src2's reverse pass           -> then LS
 enum {
     W_PC = 0,                 // prepared cell (NPC)
     W_TD = W_PC + NPC,        // fsphi, taudir_x, taudir_y (3)
@@ -81,18 +89,21 @@ enum {
     W_G = W_YE + NEF,         // Gx1, Gx2, Gy1, Gy2 (24)
     W_QN = W_G + 24,          // updated state before src2 (7)
     W_LQN = W_QN + 7,         // adjoint of that state (7)
-    W_LS = W_LQN + 7,         // adjoint of the split inputs of the cell state (6)
-    W_LXA = W_LS + 6,         // adjoint of amdq of x-edge i+1, apdq of x-edge i (12)
-    W_LYA = W_LXA + 12,       // same for y-edges j+1, j (12)
-    W_LXE = W_LYA + 12,       // adjoint of the x-edge flux inputs (NEI)
-    W_LYE = W_LXE + NEI,      // y-edges (NEI)
-    W_LXS = W_LYE + NEI,      // adjoint of the x-edge rpn2 sides, left then right (2 NSD)
+    W_LXS = W_LQN + 7,        // adjoint of the x-edge rpn2 sides, left then right (2 NSD)
     W_LYS = W_LXS + 2 * NSD,  // y-edges (2 NSD)
-    W_LTD = W_LYS + 2 * NSD,  // adjoint of the taudir inputs (NTI)
-    W_LQ7 = W_LTD + NTI,      // adjoint of the loaded q of the cell (7)
-    W_GP = W_LQ7 + 7,         // parameter gradient per cell (MAXP)
-    W_FIELDS = W_GP + MAXP
+    W_GP = W_LYS + 2 * NSD,   // parameter gradient per cell (MAXP)
+    W_FIELDS = W_GP + MAXP,
+    // aliases
+    W_LTD = W_TD,             // adjoint of the taudir inputs (NTI = 3 + 2 NW planes)
+    W_LXE = W_XE,             // adjoint of the x-edge flux inputs (NEI)
+    W_LYE = W_XE + NEI,       // y-edges (NEI; 2 NEI <= 2 NEF)
+    W_LXA = W_G,              // adjoint of amdq of x-edge i+1, apdq of x-edge i (12)
+    W_LYA = W_G + 12,         // same for y-edges j+1, j (12)
+    W_LQ7 = W_G,              // adjoint of the loaded q of the cell (7)
+    W_LS = W_QN               // adjoint of the split inputs of the cell state (6)
 };
+static_assert(NTI == 3 + 2 * NW, "taudir adjoint must fit the taudir and wave planes");
+static_assert(2 * NEI <= 2 * NEF, "flux adjoint must fit the flux planes");
 
 struct AdjArgs {
     const double* q_in;      // q_n (ghosts read only when bc[0] == 0)

@@ -790,9 +790,11 @@ __global__ void k_gauges(const double* __restrict__ q, const double* __restrict_
 // border band, where the boundary map gathers ghost adjoints), one wavefront
 // per tile, tiles taken from a compacted list by a persistent grid.  Outside
 // those tiles the step maps a cell to its dry state and every intermediate of
-// the stages is 0, so the workspace planes of inactive tiles read as 0 (they
-// are cleared after each call), bdif's plane is written for every cell, and
-// one dense pass writes the adjoint of the dry map (only bdif passes).
+// the stages is 0, so the workspace planes of inactive tiles read as 0 (a
+// tile leaving the active set is cleared at the next call), bdif's plane is
+// written for every cell, and one dense pass writes the adjoint of the dry
+// map (only bdif passes).  The batched form runs the same passes over every
+// member's tiles at once (one launch per pass for the whole ensemble).
 enum AdjStage {
     ADJ_REC_PREPARE, ADJ_REC_TAUDIR, ADJ_REC_WAVES, ADJ_REC_FLUX, ADJ_REC_SPLIT, ADJ_REC_UPDATE,
     ADJ_REV_SRC2, ADJ_REV_SPLIT, ADJ_REV_FLUX, ADJ_REV_WAVES, ADJ_REV_TAUDIR, ADJ_REV_PREPARE,
@@ -802,73 +804,78 @@ constexpr int ADJ_TILE = 8;       // tile side (64 cells: one wavefront)
 constexpr int ADJ_HALO = 5;       // cells around a wet cell whose tiles are active
 constexpr int ADJ_BLOCK = 256;    // dense passes
 constexpr int ADJ_GRID = 8192;    // persistent grid of the tile passes
+constexpr int ADJ_GP_PARTS = 256; // partial sums per parameter direction
 
 struct AdjTiles {
     int ntx, nty;
     int* flags;       // ntx * nty, 0 between calls
-    int* list;        // active tiles
-    int* count;       // [0]: number of active tiles
+    int* list;        // active tiles of the last call
+    int* count;       // [0]: their number
 };
 
+// A patch's tile bookkeeping at the end of its workspace: after the stage
+// planes and the parameter-gradient partial sums come the tile marks, the
+// active list and its counter (ints).
+__host__ __device__ inline AdjTiles adj_tiles(double* work, int mx, int my, size_t N)
+{
+    AdjTiles T;
+    T.ntx = (mx + 4 + ADJ_TILE - 1) / ADJ_TILE;
+    T.nty = (my + 4 + ADJ_TILE - 1) / ADJ_TILE;
+    const size_t ntiles = (size_t)T.ntx * (size_t)T.nty;
+    int* ints = (int*)(work + (size_t)dclaw_adj::W_FIELDS * N +
+                       (size_t)dclaw_adj::MAXP * ADJ_GP_PARTS);
+    T.flags = ints;
+    T.list = ints + ntiles;
+    T.count = ints + 2 * ntiles;
+    return T;
+}
+
 template <int STAGE>
-__global__ __launch_bounds__(64) void k_adjoint(dclaw_adj::AdjArgs A, AdjTiles T)
+__device__ __forceinline__ void adj_stage(const dclaw_adj::AdjArgs& A, const dclaw_adj::TP& Pz,
+                                          int i, int j)
 {
-    A.P = *A.Pd;
-    A.dt = *A.dt_dev;
-    const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
-    const int nact = *T.count;
-    for (int k = blockIdx.x; k < nact; k += gridDim.x) {
-        const int t = T.list[k];
-        const int i = -1 + (t % T.ntx) * ADJ_TILE + (int)(threadIdx.x % ADJ_TILE);
-        const int j = -1 + (t / T.ntx) * ADJ_TILE + (int)(threadIdx.x / ADJ_TILE);
-        if (i > A.mx + 2 || j > A.my + 2) continue;
-        if constexpr (STAGE == ADJ_REC_PREPARE) dclaw_adj::rec_prepare(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REC_TAUDIR) dclaw_adj::rec_taudir(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REC_WAVES) dclaw_adj::rec_waves(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REC_FLUX) dclaw_adj::rec_flux(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REC_SPLIT) dclaw_adj::rec_split(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REC_UPDATE) dclaw_adj::rec_update(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REV_SRC2) dclaw_adj::rev_src2(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REV_SPLIT) dclaw_adj::rev_split(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REV_FLUX) dclaw_adj::rev_flux(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REV_WAVES) dclaw_adj::rev_waves(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REV_TAUDIR) dclaw_adj::rev_taudir(A, Pz, i, j);
-        else if constexpr (STAGE == ADJ_REV_PREPARE) dclaw_adj::rev_prepare(A, Pz, i, j);
-        else dclaw_adj::rev_bc(A, i, j);
-    }
+    if constexpr (STAGE == ADJ_REC_PREPARE) dclaw_adj::rec_prepare(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REC_TAUDIR) dclaw_adj::rec_taudir(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REC_WAVES) dclaw_adj::rec_waves(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REC_FLUX) dclaw_adj::rec_flux(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REC_SPLIT) dclaw_adj::rec_split(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REC_UPDATE) dclaw_adj::rec_update(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REV_SRC2) dclaw_adj::rev_src2(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REV_SPLIT) dclaw_adj::rev_split(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REV_FLUX) dclaw_adj::rev_flux(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REV_WAVES) dclaw_adj::rev_waves(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REV_TAUDIR) dclaw_adj::rev_taudir(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REV_PREPARE) dclaw_adj::rev_prepare(A, Pz, i, j);
+    else dclaw_adj::rev_bc(A, i, j);
 }
 
-// Clear the planes of the tiles of the previous call's list that are not
-// marked for this call (bdif's plane is rewritten densely each call; every
-// tile pass rewrites all planes of its own cells, so the tiles that stay
-// active need no clearing).
-__global__ __launch_bounds__(64) void k_adjoint_clear_stale(dclaw_adj::AdjArgs A, AdjTiles T)
+// Cell (i, j) of tile t for this lane; false outside the grid.
+__device__ __forceinline__ bool adj_tile_cell(const dclaw_adj::AdjArgs& A, const AdjTiles& T,
+                                              int t, int& i, int& j)
 {
-    const int nold = *T.count;
-    for (int k = blockIdx.x; k < nold; k += gridDim.x) {
-        const int t = T.list[k];
-        if (T.flags[t] != 0) continue;
-        const int i = -1 + (t % T.ntx) * ADJ_TILE + (int)(threadIdx.x % ADJ_TILE);
-        const int j = -1 + (t / T.ntx) * ADJ_TILE + (int)(threadIdx.x / ADJ_TILE);
-        if (i > A.mx + 2 || j > A.my + 2) continue;
-        const size_t n = dclaw_adj::cix(A, i, j);
-        for (int f = 0; f < dclaw_adj::W_FIELDS; f++)
-            if (f != dclaw_adj::W_PC + dclaw_adj::PC_BDIF) A.W[(size_t)f * A.N + n] = 0.0;
-    }
+    i = -1 + (t % T.ntx) * ADJ_TILE + (int)(threadIdx.x % ADJ_TILE);
+    j = -1 + (t / T.ntx) * ADJ_TILE + (int)(threadIdx.x / ADJ_TILE);
+    return i <= A.mx + 2 && j <= A.my + 2;
 }
 
-// Dense pass over every cell: bdif's plane (max(bdif, 0) of the loaded
-// state), the dry state as q_out (if wanted) and the adjoint of the dry map
-// as lam_in (the tile passes overwrite both in active tiles); marks the tiles
-// within ADJ_HALO of a wet cell and the tiles of the border band (i <= 2,
-// i >= mx-1, j <= 2, j >= my-1), or every tile when dry_tolerance <= 0.
-__global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint_dense(dclaw_adj::AdjArgs A, AdjTiles T)
+// Clear every plane of cell n but bdif's (rewritten densely each call).
+__device__ __forceinline__ void adj_clear_cell(const dclaw_adj::AdjArgs& A, size_t n)
 {
-    const size_t n = (size_t)blockIdx.x * ADJ_BLOCK + threadIdx.x;
-    if (n >= A.N) return;
+    for (int f = 0; f < dclaw_adj::W_FIELDS; f++)
+        if (f != dclaw_adj::W_PC + dclaw_adj::PC_BDIF) A.W[(size_t)f * A.N + n] = 0.0;
+}
+
+// Dense work of cell n: bdif's plane (max(bdif, 0) of the loaded state), the
+// dry state as q_out (if wanted) and the adjoint of the dry map as lam_in
+// (the tile passes overwrite both in active tiles); marks the tiles within
+// ADJ_HALO of a wet cell and the tiles of the border band (i <= 2, i >= mx-1,
+// j <= 2, j >= my-1), or every tile when dry_tolerance <= 0.
+__device__ __forceinline__ void adj_dense_cell(const dclaw_adj::AdjArgs& A, const AdjTiles& T,
+                                               size_t n)
+{
     const int w = A.mx + 4;
     const int i = (int)(n % (size_t)w) - 1, j = (int)(n / (size_t)w) - 1;
-    const double drytol = A.Pd->dry_tolerance;
+    const double drytol = A.P.dry_tolerance;
     double q7[7], fu, fv;
     dclaw_adj::load_q(A, i, j, q7, fu, fv);
     const double bd = q7[6] > 0.0 ? q7[6] : 0.0;
@@ -898,6 +905,43 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint_dense(dclaw_adj::AdjArgs 
     }
 }
 
+// ---- one patch
+
+template <int STAGE>
+__global__ __launch_bounds__(64) void k_adjoint(dclaw_adj::AdjArgs A, AdjTiles T)
+{
+    A.P = *A.Pd;
+    A.dt = *A.dt_dev;
+    const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
+    const int nact = *T.count;
+    for (int k = blockIdx.x; k < nact; k += gridDim.x) {
+        int i, j;
+        if (adj_tile_cell(A, T, T.list[k], i, j)) adj_stage<STAGE>(A, Pz, i, j);
+    }
+}
+
+// Clear the tiles of the previous call's list that are not marked for this
+// call (every tile pass rewrites all planes of its own cells, so the tiles
+// that stay active need no clearing).
+__global__ __launch_bounds__(64) void k_adjoint_clear_stale(dclaw_adj::AdjArgs A, AdjTiles T)
+{
+    const int nold = *T.count;
+    for (int k = blockIdx.x; k < nold; k += gridDim.x) {
+        const int t = T.list[k];
+        int i, j;
+        if (T.flags[t] == 0 && adj_tile_cell(A, T, t, i, j))
+            adj_clear_cell(A, dclaw_adj::cix(A, i, j));
+    }
+}
+
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint_dense(dclaw_adj::AdjArgs A, AdjTiles T)
+{
+    const size_t n = (size_t)blockIdx.x * ADJ_BLOCK + threadIdx.x;
+    if (n >= A.N) return;
+    A.P = *A.Pd;
+    adj_dense_cell(A, T, n);
+}
+
 // Compact the marked tiles into the list (ballot within the wavefront, one
 // counter atomic per wavefront) and reset the marks.  The list's order does
 // not matter: every tile pass writes only its own cells.
@@ -924,16 +968,15 @@ __global__ void k_adjoint_count_zero(int* count)
 
 // Sum of the per-cell parameter gradients, in a fixed order: block (b, p)
 // sums chunk b of direction p's plane (strided partial sums, then a tree in
-// LDS) into part[p][b]; k_adjoint_gp_final adds the ADJ_GP_PARTS partials of
-// each direction the same way.
-constexpr int ADJ_GP_PARTS = 256;
-__global__ __launch_bounds__(256) void k_adjoint_gp(const double* __restrict__ W, size_t N,
-                                                    double* __restrict__ part)
+// LDS) into part[p][b]; adj_gp_final adds the ADJ_GP_PARTS partials of each
+// direction the same way.
+__device__ __forceinline__ void adj_gp_part(const double* __restrict__ W, size_t N, int p, int b,
+                                            double* __restrict__ part)
 {
     __shared__ double sh[256];
-    const double* g = W + (size_t)(dclaw_adj::W_GP + blockIdx.y) * N;
+    const double* g = W + (size_t)(dclaw_adj::W_GP + p) * N;
     const size_t chunk = (N + ADJ_GP_PARTS - 1) / ADJ_GP_PARTS;
-    const size_t lo = (size_t)blockIdx.x * chunk;
+    const size_t lo = (size_t)b * chunk;
     const size_t hi = lo + chunk < N ? lo + chunk : N;
     double s = 0.0;
     for (size_t n = lo + threadIdx.x; n < hi; n += 256) s += g[n];
@@ -943,20 +986,179 @@ __global__ __launch_bounds__(256) void k_adjoint_gp(const double* __restrict__ W
         if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.y * ADJ_GP_PARTS + blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) part[p * ADJ_GP_PARTS + b] = sh[0];
 }
 
-__global__ __launch_bounds__(ADJ_GP_PARTS) void k_adjoint_gp_final(const double* __restrict__ part,
-                                                                   double* __restrict__ out)
+__device__ __forceinline__ void adj_gp_final(const double* __restrict__ part, int p,
+                                             double* __restrict__ out)
 {
     __shared__ double sh[ADJ_GP_PARTS];
-    sh[threadIdx.x] = part[blockIdx.x * ADJ_GP_PARTS + threadIdx.x];
+    sh[threadIdx.x] = part[p * ADJ_GP_PARTS + threadIdx.x];
     __syncthreads();
     for (int h = ADJ_GP_PARTS / 2; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) out[p] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_adjoint_gp(const double* __restrict__ W, size_t N,
+                                                    double* __restrict__ part)
+{
+    adj_gp_part(W, N, blockIdx.y, blockIdx.x, part);
+}
+
+__global__ __launch_bounds__(ADJ_GP_PARTS) void k_adjoint_gp_final(const double* __restrict__ part,
+                                                                   double* __restrict__ out)
+{
+    adj_gp_final(part, blockIdx.x, out);
+}
+
+// ---- many patches (one launch per pass for the whole ensemble)
+
+struct AdjBatch {
+    const dclaw_adj_patch* pt;
+    int npatch;
+    int bc[4];
+    const dclaw_params* params;    // per member
+    const dclaw_params* dparams;   // np per member
+    int np;
+    const double* dt;              // per member
+    long long* cell_prefix;        // npatch + 1
+    long long* tile_prefix;        // npatch + 1
+    int* glist;                    // active (patch, tile) pairs as global tile ids
+    int* gcount;
+};
+
+// Largest m with prefix[m] <= g.
+__device__ __forceinline__ int adj_find(const long long* prefix, int npatch, long long g)
+{
+    int lo = 0, hi = npatch - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ dclaw_adj::AdjArgs adj_member(const AdjBatch& B, int m, AdjTiles& T)
+{
+    const dclaw_adj_patch& p = B.pt[m];
+    dclaw_adj::AdjArgs A;
+    A.q_in = p.q_in;
+    A.aux = p.aux;
+    A.maux = p.maux;
+    A.mx = p.mx;
+    A.my = p.my;
+    A.dx = p.dx;
+    A.dy = p.dy;
+    for (int k = 0; k < 4; k++) A.bc[k] = B.bc[k];
+    A.Pd = B.params + p.member;
+    A.dt_dev = B.dt + p.member;
+    A.P = *A.Pd;
+    A.dt = *A.dt_dev;
+    A.dP = B.dparams ? B.dparams + (size_t)p.member * (size_t)B.np : nullptr;
+    A.np = B.np;
+    A.lam_out = p.lam_out;
+    A.lam_in = p.lam_in;
+    A.q_out = p.q_out;
+    A.W = p.work;
+    A.N = (size_t)(p.mx + 4) * (size_t)(p.my + 4);
+    T = adj_tiles(p.work, p.mx, p.my, A.N);
+    return A;
+}
+
+__global__ void k_adjb_prefix(AdjBatch B)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    long long c = 0, t = 0;
+    for (int m = 0; m < B.npatch; m++) {
+        B.cell_prefix[m] = c;
+        B.tile_prefix[m] = t;
+        const int mx = B.pt[m].mx, my = B.pt[m].my;
+        c += (long long)(mx + 4) * (my + 4);
+        t += (long long)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) * ((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+    }
+    B.cell_prefix[B.npatch] = c;
+    B.tile_prefix[B.npatch] = t;
+}
+
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adjb_dense(AdjBatch B, long long ncells)
+{
+    const long long g = (long long)blockIdx.x * ADJ_BLOCK + threadIdx.x;
+    if (g >= ncells) return;
+    const int m = adj_find(B.cell_prefix, B.npatch, g);
+    AdjTiles T;
+    const dclaw_adj::AdjArgs A = adj_member(B, m, T);
+    adj_dense_cell(A, T, (size_t)(g - B.cell_prefix[m]));
+}
+
+__global__ __launch_bounds__(64) void k_adjb_clear_stale(AdjBatch B)
+{
+    AdjTiles T;
+    const dclaw_adj::AdjArgs A = adj_member(B, blockIdx.y, T);
+    const int nold = *T.count;
+    for (int k = blockIdx.x; k < nold; k += gridDim.x) {
+        const int t = T.list[k];
+        int i, j;
+        if (T.flags[t] == 0 && adj_tile_cell(A, T, t, i, j))
+            adj_clear_cell(A, dclaw_adj::cix(A, i, j));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adjb_count_zero(AdjBatch B)
+{
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m == 0) *B.gcount = 0;
+    if (m >= B.npatch) return;
+    const dclaw_adj_patch& p = B.pt[m];
+    adj_tiles(p.work, p.mx, p.my, (size_t)(p.mx + 4) * (size_t)(p.my + 4)).count[0] = 0;
+}
+
+// Compaction over every patch's marks: each marked tile goes to its patch's
+// own list (for the next call's clearing) and to the global list.
+__global__ __launch_bounds__(64) void k_adjb_list(AdjBatch B, long long ntiles)
+{
+    const long long g = (long long)blockIdx.x * 64 + threadIdx.x;
+    if (g >= ntiles) return;
+    const int m = adj_find(B.tile_prefix, B.npatch, g);
+    const dclaw_adj_patch& p = B.pt[m];
+    const AdjTiles T = adj_tiles(p.work, p.mx, p.my, (size_t)(p.mx + 4) * (size_t)(p.my + 4));
+    const int t = (int)(g - B.tile_prefix[m]);
+    if (T.flags[t] == 0) return;
+    T.flags[t] = 0;
+    T.list[atomicAdd(T.count, 1)] = t;
+    B.glist[atomicAdd(B.gcount, 1)] = (int)g;
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(64) void k_adjb(AdjBatch B)
+{
+    const int nact = *B.gcount;
+    for (int k = blockIdx.x; k < nact; k += gridDim.x) {
+        const long long g = B.glist[k];
+        const int m = adj_find(B.tile_prefix, B.npatch, g);
+        AdjTiles T;
+        const dclaw_adj::AdjArgs A = adj_member(B, m, T);
+        const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
+        int i, j;
+        if (adj_tile_cell(A, T, (int)(g - B.tile_prefix[m]), i, j)) adj_stage<STAGE>(A, Pz, i, j);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adjb_gp(AdjBatch B)
+{
+    const dclaw_adj_patch& p = B.pt[blockIdx.z];
+    const size_t N = (size_t)(p.mx + 4) * (size_t)(p.my + 4);
+    adj_gp_part(p.work, N, blockIdx.y, blockIdx.x, p.work + (size_t)dclaw_adj::W_FIELDS * N);
+}
+
+__global__ __launch_bounds__(ADJ_GP_PARTS) void k_adjb_gp_final(AdjBatch B)
+{
+    const dclaw_adj_patch& p = B.pt[blockIdx.y];
+    const size_t N = (size_t)(p.mx + 4) * (size_t)(p.my + 4);
+    adj_gp_final(p.work + (size_t)dclaw_adj::W_FIELDS * N, blockIdx.x, p.grad);
 }
 
 // Adjoint of the gauge samples (k_gauges): lam_q += d(out)/dq^T lam_out for
@@ -1273,15 +1475,8 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
     A.dt_dev = dt_dev;
     const size_t nb = (A.N + ADJ_BLOCK - 1) / ADJ_BLOCK;
     if (nb > 0x7fffffffUL) return fail(DCLAW_EINVAL, "patch too large for one launch");
-    AdjTiles T;
-    T.ntx = (mx + 4 + ADJ_TILE - 1) / ADJ_TILE;
-    T.nty = (my + 4 + ADJ_TILE - 1) / ADJ_TILE;
+    const AdjTiles T = adj_tiles(work, mx, my, A.N);
     const int ntiles = T.ntx * T.nty;
-    int* ints = (int*)(work + (size_t)dclaw_adj::W_FIELDS * A.N +
-                       (size_t)dclaw_adj::MAXP * ADJ_GP_PARTS);
-    T.flags = ints;
-    T.list = ints + ntiles;
-    T.count = ints + 2 * (size_t)ntiles;
     hipStream_t st = (hipStream_t)stream;
     const dim3 tg((unsigned)std::min(ntiles, ADJ_GRID)), tb(64);
     hipLaunchKernelGGL(k_adjoint_dense, dim3((unsigned)nb), dim3(ADJ_BLOCK), 0, st, A, T);
@@ -1307,6 +1502,96 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
                            part);
         hipLaunchKernelGGL(k_adjoint_gp_final, dim3(np), dim3(ADJ_GP_PARTS), 0, st, part,
                            grad_dev);
+    }
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+static long long adj_batch_tiles(const dclaw_adj_patch* ph, int npatch, long long* cells)
+{
+    long long t = 0, c = 0;
+    for (int k = 0; k < npatch; k++) {
+        const int mx = ph[k].mx, my = ph[k].my;
+        c += (long long)(mx + 4) * (my + 4);
+        t += (long long)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) * ((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+    }
+    if (cells) *cells = c;
+    return t;
+}
+
+size_t dclaw_adjoint_batch_doubles(const dclaw_adj_patch* patches_host, int npatch)
+{
+    if (!patches_host || npatch < 1) return 0;
+    const long long t = adj_batch_tiles(patches_host, npatch, nullptr);
+    // cell and tile prefixes, the global counter, then the global list (ints)
+    return (size_t)(2 * (npatch + 1) + 1) + (size_t)((t + 1) / 2);
+}
+
+int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_patch* patches_host,
+                          int npatch, const int* bc, const double* dt_dev,
+                          const dclaw_params* params_dev, const dclaw_params* dparams_dev, int np,
+                          double* batch_work, size_t batch_doubles, void* stream)
+{
+    if (int rcb = check_bc(bc)) return rcb;
+    if (!patches_dev || !patches_host || !dt_dev || !params_dev || !batch_work)
+        return fail(DCLAW_EINVAL, "null pointer");
+    if (npatch < 1 || npatch > 65535) return fail(DCLAW_EINVAL, "npatch must be in [1, 65535]");
+    if (np < 0 || np > dclaw_adj::MAXP) return fail(DCLAW_EINVAL, "np must be in [0, 4]");
+    if (np > 0 && !dparams_dev) return fail(DCLAW_EINVAL, "parameter directions need dparams");
+    for (int k = 0; k < npatch; k++) {
+        const dclaw_adj_patch& p = patches_host[k];
+        if (int rc = check_patch(p.mx, p.my, p.maux)) return rc;
+        if (p.mx < 2 || p.my < 2) return fail(DCLAW_EINVAL, "the adjoint step needs mx, my >= 2");
+        if (!p.q_in || !p.aux || !p.lam_out || !p.lam_in || !p.work || (np > 0 && !p.grad))
+            return fail(DCLAW_EINVAL, "null pointer in patch record");
+        if ((p.q_out && (p.q_out == p.q_in || p.q_out == p.lam_out || p.q_out == p.lam_in)) ||
+            p.lam_in == p.q_in || p.lam_in == p.lam_out)
+            return fail(DCLAW_EINVAL, "q_out and lam_in must not alias q_in, lam_out or each other");
+        if (!(p.dx > 0.0) || !(p.dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+        if (p.member < 0) return fail(DCLAW_EINVAL, "negative member index");
+    }
+    if (batch_doubles < dclaw_adjoint_batch_doubles(patches_host, npatch))
+        return fail(DCLAW_EINVAL, "adjoint batch workspace too small");
+    long long ncells = 0;
+    const long long ntiles = adj_batch_tiles(patches_host, npatch, &ncells);
+    if (ntiles > 0x7fffffffLL || (ncells + ADJ_BLOCK - 1) / ADJ_BLOCK > 0x7fffffffLL)
+        return fail(DCLAW_EINVAL, "batch too large for one launch");
+    AdjBatch B;
+    B.pt = patches_dev;
+    B.npatch = npatch;
+    for (int k = 0; k < 4; k++) B.bc[k] = bc ? bc[k] : 0;
+    B.params = params_dev;
+    B.dparams = np > 0 ? dparams_dev : nullptr;
+    B.np = np;
+    B.dt = dt_dev;
+    B.cell_prefix = (long long*)batch_work;
+    B.tile_prefix = B.cell_prefix + (npatch + 1);
+    B.gcount = (int*)(B.tile_prefix + (npatch + 1));
+    B.glist = (int*)(batch_work + 2 * (npatch + 1) + 1);
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 tg((unsigned)std::min<long long>(ntiles, ADJ_GRID)), tb(64);
+    hipLaunchKernelGGL(k_adjb_prefix, dim3(1), dim3(1), 0, st, B);
+    hipLaunchKernelGGL(k_adjb_dense, dim3((unsigned)((ncells + ADJ_BLOCK - 1) / ADJ_BLOCK)),
+                       dim3(ADJ_BLOCK), 0, st, B, ncells);
+    hipLaunchKernelGGL(k_adjb_clear_stale, dim3(32, npatch), tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb_count_zero, dim3((npatch + 255) / 256), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(k_adjb_list, dim3((unsigned)((ntiles + 63) / 64)), tb, 0, st, B, ntiles);
+    hipLaunchKernelGGL(k_adjb<ADJ_REC_PREPARE>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REC_TAUDIR>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REC_WAVES>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REC_FLUX>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REC_SPLIT>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REC_UPDATE>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REV_SRC2>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REV_SPLIT>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REV_FLUX>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REV_WAVES>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REV_TAUDIR>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REV_PREPARE>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REV_BC>, tg, tb, 0, st, B);
+    if (np > 0) {
+        hipLaunchKernelGGL(k_adjb_gp, dim3(ADJ_GP_PARTS, np, npatch), dim3(256), 0, st, B);
+        hipLaunchKernelGGL(k_adjb_gp_final, dim3(np, npatch), dim3(ADJ_GP_PARTS), 0, st, B);
     }
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;

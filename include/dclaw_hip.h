@@ -76,6 +76,19 @@ typedef struct dclaw_patch {
     double dx, dy;
 } dclaw_patch;
 
+/* One patch of a batched adjoint step (dclaw_adjoint_batched). */
+typedef struct dclaw_adj_patch {
+    const double* q_in;         /* q_n, layout of q */
+    const double* aux;          /* aux(maux, mx+4, my+4) */
+    const double* lam_out;      /* lambda_{n+1} */
+    double* lam_in;             /* lambda_n (written) */
+    double* q_out;              /* q_{n+1} (written if not null) */
+    double* work;               /* dclaw_adjoint_workspace_doubles(mx, my), zero-filled once */
+    double* grad;               /* np doubles (written if np > 0) */
+    int mx, my, maux, member;   /* member indexes params[], dt[] and the np dparams */
+    double dx, dy;
+} dclaw_adj_patch;
+
 const char* dclaw_last_error(void);
 int dclaw_version(void);
 
@@ -124,7 +137,7 @@ int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, 
  * aux (the bed, h_e) carries no tangent; aux(4..6) is not written.  No
  * reference counterpart: the reference has no derivative of the step (the
  * gradient-based workflows need it; this is the forward-mode building block,
- * a reverse-mode adjoint is not provided). */
+ * dclaw_adjoint_step_dev the reverse-mode one). */
 int dclaw_stepgrid_tl_dev(const double* q_in, const double* dq_in, double* q_out, double* dq_out,
                           const double* aux, int maux, int mx, int my, double dx, double dy,
                           const int* bc, const double* dt_dev, const dclaw_params* params_dev,
@@ -275,6 +288,22 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
                            const dclaw_params* params_dev, const dclaw_params* dparams_dev, int np,
                            const double* lam_out, double* lam_in, double* q_out, double* grad_dev,
                            double* work, size_t work_doubles, void* stream);
+
+/* The adjoint step of many patches of uneven size at once: every pass of
+ * dclaw_adjoint_step_dev is one launch for the whole set, over the active
+ * tiles of all patches (patches_dev: device array of npatch records,
+ * patches_host: the same records in host memory, read for the launch sizes).
+ * Patch p uses params_dev[member], dt_dev[member] and the np directions
+ * dparams_dev[member * np .. member * np + np - 1]; each patch has its own
+ * workspace (as for dclaw_adjoint_step_dev).  batch_work: device scratch of
+ * dclaw_adjoint_batch_doubles(patches_host, npatch) doubles (no contents
+ * kept between calls).  Results equal dclaw_adjoint_step_dev on each patch
+ * bit for bit.  No reference counterpart (ensemble gradients). */
+size_t dclaw_adjoint_batch_doubles(const dclaw_adj_patch* patches_host, int npatch);
+int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_patch* patches_host,
+                          int npatch, const int* bc, const double* dt_dev,
+                          const dclaw_params* params_dev, const dclaw_params* dparams_dev, int np,
+                          double* batch_work, size_t batch_doubles, void* stream);
 
 /* Adjoint of dclaw_gauges_dev: lam_q_dev (layout of q) += the transpose of
  * the gauge map's derivative at q applied to lam_gauges_dev (8 doubles per
