@@ -62,7 +62,8 @@ def parse():
     ap.add_argument("--fused", action="store_true",
                     help="run src2 inside the strip kernel (one launch per step) instead of "
                          "the default split step (strip kernel + src2 pass over wet tiles)")
-    ap.add_argument("--split-mode", choices=["auto", "single", "ordered", "dry"], default="auto",
+    ap.add_argument("--split-mode", choices=["auto", "single", "ordered", "dry", "hybrid"],
+                    default="auto",
                     help="split step: one strip launch in tile order (single), with the previous "
                          "step's wet tiles first (ordered), or after a separate dry pass (dry); "
                          "auto = the autotuner's pick")
@@ -143,7 +144,7 @@ def main():
     if seg <= 0:
         from dclaw_amd import autotune
         seg, tune_times = autotune.tune_patch(
-            r, split_modes=(("single", "ordered", "dry") if args.split_mode == "auto"
+            r, split_modes=(("single", "ordered", "dry", "hybrid") if args.split_mode == "auto"
                             else (args.split_mode,)))
         tune_times = {f"{h}/{m}": round(v, 4) for (h, m), v in tune_times.items()}
     else:
@@ -184,7 +185,7 @@ def main():
     # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a 1 GiB copy)
     traffic = None
     mode = "fused" if args.fused else {"single": "split_step", "ordered": "ordered",
-                                       "dry": "dry_pass"}[r.split_mode]
+                                       "dry": "dry_pass", "hybrid": "hybrid"}[r.split_mode]
     run_cfg = f"--seg-rows {seg} --split-mode {r.split_mode}"
 
     def summary_value(path, key):
@@ -292,6 +293,8 @@ def main():
                                 if r.split_mode == "single"
                                 else "k_zero_ints + k_order_tiles + k_stepgrid<1> + "
                                      "k_src2_tiles (one step)" if r.split_mode == "ordered"
+                                else "k_zero_ints + k_order_hybrid + k_stepgrid_hybrid + "
+                                     "k_src2_tiles (one step)" if r.split_mode == "hybrid"
                                 else "k_zero_ints + k_classify + k_stepgrid_list<1> + "
                                      "k_src2_list (one step)"),
                      "kernel_avg_ms": kern_avg_s * 1e3},

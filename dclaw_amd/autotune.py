@@ -35,10 +35,12 @@ def _time(fn: Callable[[], None], reps: int) -> float:
 
 
 def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 5,
-               split_modes: Iterable[str] = ("single", "ordered", "dry")) -> Tuple[int, Dict]:
+               split_modes: Iterable[str] = ("single", "ordered", "dry", "hybrid")
+               ) -> Tuple[int, Dict]:
     """Pick the segment height for a PatchRunner's single-patch launches and,
     for the split step, its mode (one strip launch in tile order, with the
-    previous step's wet tiles first, or after a separate dry pass -- which
+    previous step's wet tiles first, after a separate dry pass, or with the
+    wet segments single and first and the dry ones in tall groups -- which
     wins depends on the wet fraction).  Every candidate starts from the same
     state (restored outside the timed region) and times `reps` consecutive
     steps at a fixed dt; the runner's state and dt are restored afterwards.

@@ -88,7 +88,8 @@ __device__ __forceinline__ void wave_max_to(double v, double* dst)
 
 int g_seg_rows = 32;   // rows per segment of the single-patch launch (tunable)
 int g_dry_rows = 48;   // rows per wavefront of the split step's dry pass (whole segments)
-int g_split_mode = 1;  // split step with the full workspace: 1 wet-first order, 2 dry pass
+int g_split_mode = 1;  // split step with the full workspace: 1 wet-first order, 2 dry pass,
+                       // 3 wet-first short segments + tall dry groups
 
 // Compute units of the current device (cached per device).
 int cu_count()
@@ -450,6 +451,96 @@ __global__ void k_order_tiles(const int* __restrict__ flags, int n, int* order, 
     base_d = __shfl(base_d, 0, 64);
     if (wet) order[base_w + __popcll(bw & below)] = t;
     else if (in) order[n - 1 - (base_d + __popcll(bd & below))] = t;
+}
+
+// Work list of the hybrid split step: each group of HYB_GROUP consecutive
+// segments of a strip goes to the wet list as its single segments (tile index
+// segment * nstrips + strip) if the previous step flagged any of them, else
+// to the dry list as one entry (group index gi * nstrips + strip).
+// counters[0], counters[1]: the two lengths (zero before the launch).
+constexpr int HYB_GROUP = 4;
+__global__ void k_order_hybrid(const int* __restrict__ flags, int nstrips, int nseg,
+                               int* wet_list, int* dry_list, int* counters)
+{
+    const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int ngi = (nseg + HYB_GROUP - 1) / HYB_GROUP;
+    if (g >= nstrips * ngi) return;
+    const int strip = g % nstrips, gi = g / nstrips;
+    const int s0 = gi * HYB_GROUP, s1 = min(nseg, s0 + HYB_GROUP);
+    bool wet = false;
+    for (int s = s0; s < s1; s++) wet |= flags[s * nstrips + strip] != 0;
+    if (wet) {
+        const int base = atomicAdd(counters, s1 - s0);
+        for (int s = s0; s < s1; s++) wet_list[base + s - s0] = s * nstrips + strip;
+    } else {
+        dry_list[atomicAdd(counters + 1, 1)] = g;
+    }
+}
+
+// The no-src2 strip kernel over the hybrid work list: workgroups [0, nw) take
+// the wet list's single segments (the long tiles start first, and short ones
+// keep the longest tile's march short), [nw, nw + nd) the dry groups as one
+// tall segment each (one pass over their rows and halo); the rest exit.  A
+// group's src2 flag is copied to each of its segments (a segment without a wet
+// cell flagged with its group only costs src2 a no-op pass).
+__global__ __launch_bounds__(64) void k_stepgrid_hybrid(const double* __restrict__ q_in,
+                                                        double* __restrict__ q_out,
+                                                        const double* __restrict__ aux, int maux,
+                                                        int mx, int my, double dx, double dy,
+                                                        const double* __restrict__ dt_dev,
+                                                        const dclaw_params* __restrict__ params,
+                                                        double* cfl_dev, int seg_rows, int nstrips,
+                                                        int nseg, int bx0, int bx1, int by0,
+                                                        int by1, int* tile_flags,
+                                                        const int* __restrict__ wet_list,
+                                                        const int* __restrict__ dry_list,
+                                                        const int* __restrict__ counters)
+{
+    const int nw = counters[0], nd = counters[1];
+    const int b = (int)blockIdx.x;
+    int strip, s0, nsub;
+    if (b < nw) {
+        const int t = wet_list[b];
+        strip = t % nstrips;
+        s0 = t / nstrips;
+        nsub = 1;
+    } else if (b < nw + nd) {
+        const int g = dry_list[b - nw];
+        strip = g % nstrips;
+        s0 = (g / nstrips) * HYB_GROUP;
+        nsub = min(nseg, s0 + HYB_GROUP) - s0;
+    } else {
+        return;
+    }
+    const dclaw_params P = *params;
+    StripArgs A;
+    A.bc[0] = bx0;
+    A.bc[1] = bx1;
+    A.bc[2] = by0;
+    A.bc[3] = by1;
+    A.q_in = q_in;
+    A.q_out = q_out;
+    A.aux = aux;
+    A.aux_out = nullptr;
+    A.maux = maux;
+    A.mx = mx;
+    A.my = my;
+    A.dx = dx;
+    A.dy = dy;
+    A.dt = *dt_dev;
+    A.i0 = 1 + strip * STRIP_W;
+    A.jA = 1 + s0 * seg_rows;
+    A.jB = min(my, (s0 + nsub) * seg_rows);
+    A.fm = A.fp = A.gm = A.gp = nullptr;
+    int* flag0 = tile_flags + (s0 * nstrips + strip);
+    A.tile_flag = flag0;
+    GpuLane L;
+    const double cfl = strip_step_dispatch<MODE_STEP_NOSRC2>(L, P, A);
+    wave_max_to(cfl, cfl_dev);
+    if (nsub > 1 && threadIdx.x == 0) {
+        const int v = *flag0;
+        for (int k = 1; k < nsub; k++) flag0[k * nstrips] = v;
+    }
 }
 
 __global__ void k_zero_ints(int* a, int n)
@@ -1326,6 +1417,27 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     const int ntiles = nstrips * nseg;
     if (n_flags < ntiles) return fail(DCLAW_EINVAL, "tile_flags holds fewer than dclaw_tile_count() ints");
     if (aux_out) return fail(DCLAW_EINVAL, "the split step does not write aux");
+    if ((long)n_flags >= 3L * ntiles + 4 && g_split_mode == 3) {
+        // one strip launch: the previous step's wet segments singly and first,
+        // the other segments in tall dry groups
+        int* wet_list = tile_flags + ntiles;
+        int* dry_list = tile_flags + 2 * ntiles;
+        int* counters = tile_flags + 3 * ntiles;
+        const int ngroups = nstrips * ((nseg + HYB_GROUP - 1) / HYB_GROUP);
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, stream, counters, 2);
+        hipLaunchKernelGGL(k_order_hybrid, dim3((ngroups + 255) / 256), dim3(256), 0, stream,
+                           tile_flags, nstrips, nseg, wet_list, dry_list, counters);
+        hipLaunchKernelGGL(k_stepgrid_hybrid, dim3(ntiles), dim3(64), 0, stream, q_in, q_out, aux,
+                           maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev, seg_rows, nstrips,
+                           nseg, b0, b1, b2, b3, tile_flags, wet_list, dry_list, counters);
+        HIP_TRY(hipGetLastError());
+        const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
+        hipLaunchKernelGGL(k_src2_tiles, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0,
+                           stream, q_out, aux, maux, mx, my, seg_rows, nstrips, ntiles,
+                           tile_flags, dt_dev, params_dev);
+        HIP_TRY(hipGetLastError());
+        return DCLAW_OK;
+    }
     if ((long)n_flags >= 2L * ntiles + 4 && g_split_mode == 1) {
         // one strip launch with the previous step's wet tiles first
         int* order = tile_flags + ntiles;
@@ -1625,13 +1737,13 @@ int dclaw_split_workspace_size(int mx, int my)
 {
     const int n = dclaw_tile_count(mx, my);
     if (n < 0) return n;
-    if (2L * n + 4 > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
-    return 2 * n + 4;
+    if (3L * n + 4 > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
+    return 3 * n + 4;
 }
 
 int dclaw_set_split_mode(int mode)
 {
-    if (mode < 0 || mode > 2) return fail(DCLAW_EINVAL, "split mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(DCLAW_EINVAL, "split mode must be 0, 1, 2 or 3");
     const int prev = g_split_mode;
     g_split_mode = mode;
     return prev;

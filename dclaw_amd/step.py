@@ -145,7 +145,7 @@ def src2(q: torch.Tensor, aux: torch.Tensor, mx: int, my: int, dt_dev: torch.Ten
         None if flag is None else _ptr(flag), _stream()))
 
 
-SPLIT_MODES = ("single", "ordered", "dry")
+SPLIT_MODES = ("single", "ordered", "dry", "hybrid")
 
 
 def stepgrid_split(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int, mx: int,
@@ -157,8 +157,9 @@ def stepgrid_split(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, m
     mode "single": one strip launch in tile order (dclaw_tile_count ints
     suffice); "ordered": one strip launch with the tiles the previous step
     flagged first; "dry": a light dry pass writes the dry tiles and the strip
-    kernel runs over the others (both need dclaw_split_workspace_size ints).
-    All give the same result."""
+    kernel runs over the others; "hybrid": the previous step's wet segments
+    singly and first, the others in tall groups of 4 segments (these three
+    need dclaw_split_workspace_size ints).  All give the same result."""
     if mode not in SPLIT_MODES:
         raise ValueError(f"mode must be one of {SPLIT_MODES}")
     arr = None if bc is None else (ctypes.c_int * 4)(*bc)
@@ -169,7 +170,7 @@ def stepgrid_split(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, m
         _lib.check(min(tc, 0))
         n = min(n, tc)
     else:
-        _lib.check(min(0, L.dclaw_set_split_mode(1 if mode == "ordered" else 2)))
+        _lib.check(min(0, L.dclaw_set_split_mode({"ordered": 1, "dry": 2, "hybrid": 3}[mode])))
     _lib.check(L.dclaw_stepgrid_split_dev(
         _ptr(q_in), _ptr(q_out), _ptr(aux), maux, mx, my, dx, dy, arr, _ptr(dt_dev),
         _ptr(params_dev), _ptr(cfl_dev), _ptr(flags), n, _stream()))
@@ -238,7 +239,7 @@ class PatchRunner:
         # split-step workspace for any segment height (one row per tile at
         # most): tile flags, the wet list or tile order, and counters
         nstrips = -(-self.mx // _lib.lib().dclaw_strip_width())
-        self.flags = torch.zeros(2 * nstrips * self.my + 4, dtype=torch.int32, device=device)
+        self.flags = torch.zeros(3 * nstrips * self.my + 4, dtype=torch.int32, device=device)
 
     def set_dt(self, dt: float) -> None:
         self.dt.fill_(float(dt))

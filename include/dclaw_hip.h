@@ -147,7 +147,7 @@ int dclaw_stepgrid_tl_dev(const double* q_in, const double* dq_in, double* q_out
  * height: the smallest tile_flags workspace of the split step below. */
 int dclaw_tile_count(int mx, int my);
 
-/* Workspace ints (2 * dclaw_tile_count(mx, my) + 4) with which the split step
+/* Workspace ints (3 * dclaw_tile_count(mx, my) + 4) with which the split step
  * below runs in the mode set by dclaw_set_split_mode.  No reference
  * counterpart. */
 int dclaw_split_workspace_size(int mx, int my);
@@ -155,10 +155,13 @@ int dclaw_split_workspace_size(int mx, int my);
 /* Mode of the split step when its workspace holds dclaw_split_workspace_size
  * ints: 1 = one strip launch with the tiles the previous call flagged first
  * (the workspace carries the flags between calls; any contents give a valid
- * order; the default), 2 = a separate dry pass first; with fewer ints (at
- * least dclaw_tile_count) or mode 0: one strip launch in tile order.  Results
- * do not depend on it; a library-wide setting like the segment height.
- * Returns the previous mode or a negative error.  No reference counterpart. */
+ * order; the default), 2 = a separate dry pass first, 3 = one strip launch
+ * in which the segments of 4-segment groups the previous call flagged run
+ * singly and first and the other groups as one tall segment each; with fewer
+ * ints (at least dclaw_tile_count) or mode 0: one strip launch in tile order.
+ * Results do not depend on it; a library-wide setting like the segment
+ * height.  Returns the previous mode or a negative error.  No reference
+ * counterpart. */
 int dclaw_set_split_mode(int mode);
 
 /* Rows per wavefront of the split step's dry pass (rounded down to whole

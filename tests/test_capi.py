@@ -84,14 +84,14 @@ def test_split_workspace_and_dry_rows(L):
     try:
         nt = L.dclaw_tile_count(4096, 4096)
         assert nt == 71 * 342
-        assert L.dclaw_split_workspace_size(4096, 4096) == 2 * nt + 4
-        assert L.dclaw_split_workspace_size(1, 1) == 6
+        assert L.dclaw_split_workspace_size(4096, 4096) == 3 * nt + 4
+        assert L.dclaw_split_workspace_size(1, 1) == 7
         assert L.dclaw_split_workspace_size(0, 5) == -1
     finally:
         L.dclaw_set_seg_rows(prev)
     m = L.dclaw_set_split_mode(1)
     assert m in (0, 1, 2)
-    assert L.dclaw_set_split_mode(3) == -1 and b"split mode" in L.dclaw_last_error()
+    assert L.dclaw_set_split_mode(4) == -1 and b"split mode" in L.dclaw_last_error()
     assert L.dclaw_set_split_mode(m) == 1
     p = L.dclaw_set_dry_rows(96)
     assert p >= 1

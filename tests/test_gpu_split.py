@@ -167,7 +167,7 @@ def test_dry_pass_equals_fused(dev, oracle_mod, mx, my, seg, dry_rows, bc):
     try:
         ntiles = L.dclaw_tile_count(mx, my)
         nws = L.dclaw_split_workspace_size(mx, my)
-        assert nws == 2 * ntiles + 4
+        assert nws == 3 * ntiles + 4
         pdev = S.params_tensor([pt.params])
         dtd = dev.tensor([dt], dtype=dev.float64, device="cuda")
         bed = S.bed_tensor(aux)
@@ -207,7 +207,8 @@ def test_dry_pass_equals_fused(dev, oracle_mod, mx, my, seg, dry_rows, bc):
 def test_dry_pass_runner_long(dev):
     """150 steps of the runner on a 700 x 520 fractal patch with 64 sources
     in each split mode (dry pass, one launch in tile order, one launch with
-    the previous step's wet tiles first): identical states and dt."""
+    the previous step's wet tiles first, the same with the dry segments in
+    tall groups): identical states and dt."""
     from dclaw_amd import _lib
     from dclaw_amd import step as S
     from dclaw_amd import workloads as W
@@ -218,9 +219,11 @@ def test_dry_pass_runner_long(dev):
         ra = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_mode="dry")
         rb = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_mode="single")
         rc = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_mode="ordered")
+        rd = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_mode="hybrid")
         ra.run(150, 1e-3, 0.9, 1e9)
         rb.run(150, 1e-3, 0.9, 1e9)
         rc.run(150, 1e-3, 0.9, 1e9)
+        rd.run(150, 1e-3, 0.9, 1e9)
         dev.cuda.synchronize()
         ntiles = L.dclaw_tile_count(700, 520)
         nwet = int(ra.flags[2 * ntiles].item())
@@ -228,10 +231,11 @@ def test_dry_pass_runner_long(dev):
         nfirst = int(rc.flags[2 * ntiles + 2].item())
     finally:
         L.dclaw_set_seg_rows(prev)
-    qa, qb, qc = ra.q(), rb.q(), rc.q()
+    qa, qb, qc, qd = ra.q(), rb.q(), rc.q(), rd.q()
     assert np.array_equal(qa, qb), f"{(qa != qb).sum()} values differ"
     assert np.array_equal(qa, qc), f"{(qa != qc).sum()} values differ"
-    assert ra.dt.item() == rb.dt.item() == rc.dt.item()
+    assert np.array_equal(qa, qd), f"{(qa != qd).sum()} values differ"
+    assert ra.dt.item() == rb.dt.item() == rc.dt.item() == rd.dt.item()
     assert 0 < nwet < ntiles
     # the last step's order: a permutation of the tiles, flagged ones first
     assert np.array_equal(np.sort(order), np.arange(ntiles))
@@ -253,7 +257,8 @@ def test_tuner_restores_state(dev):
         r.step()
         q0, dt0 = r.q(), r.dt.item()
         h, times = autotune.tune_patch(r, candidates=(4, 8), reps=2)
-        assert set(times) == {(h_, m) for h_ in (4, 8) for m in ("single", "ordered", "dry")}
+        assert set(times) == {(h_, m) for h_ in (4, 8)
+                              for m in ("single", "ordered", "dry", "hybrid")}
         assert (h, r.split_mode) == min(times, key=times.get)
         assert L.dclaw_set_seg_rows(h) == h
         assert np.array_equal(r.q(), q0) and r.dt.item() == dt0
@@ -309,4 +314,67 @@ def test_ordered_equals_fused(dev, oracle_mod, mx, my, seg, bc):
     assert np.array_equal(np.sort(order), np.arange(ntiles))
     assert nfirst == int(prev_flags.sum())
     assert set(order[:nfirst].tolist()) == set(np.flatnonzero(prev_flags).tolist())
+    assert set(w[:ntiles].tolist()) <= {0, 1}
+
+
+@pytest.mark.parametrize("mx,my,seg,bc", [(130, 70, 8, (3, 3, 3, 3)), (59, 33, 4, (1, 3, 3, 1)),
+                                          (600, 500, 6, (1, 1, 1, 1)), (7, 90, 5, None)])
+def test_hybrid_equals_fused(dev, oracle_mod, mx, my, seg, bc):
+    """The hybrid split step (the previous step's flagged 4-segment groups
+    as single segments first, the others as one tall segment each), from
+    arbitrary flags, equals the fused step on every cell; its work list holds
+    every segment of the flagged groups and every other group once; a group's
+    new src2 flag is on each of its segments."""
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    O = oracle_mod
+    pt = slope_patch(mx, my, slope=0.8, phi_deg=25.0, seed=mx + my + 1)
+    q, aux, dt = advance_oracle(O, pt, 6, bc=bc or (1, 1, 1, 1))
+    print(f"[hybrid] {mx}x{my} seg {seg} bc {bc}", flush=True)
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(seg)
+    try:
+        ntiles = L.dclaw_tile_count(mx, my)
+        nws = L.dclaw_split_workspace_size(mx, my)
+        nstrips = -(-mx // 58)
+        nseg = ntiles // nstrips
+        rng = np.random.default_rng(mx + 1)
+        prev_flags = (rng.random(ntiles) < 0.2).astype(np.int32)
+        ws = dev.full((nws,), -7, dtype=dev.int32, device="cuda")
+        ws[:ntiles] = dev.from_numpy(prev_flags).to("cuda")
+        pdev = S.params_tensor([pt.params])
+        dtd = dev.tensor([dt], dtype=dev.float64, device="cuda")
+        bed = S.bed_tensor(aux)
+        qi = S.to_device_q(q)
+        qf = dev.zeros_like(qi)
+        qs = dev.full_like(qi, float("nan"))
+        cf = dev.zeros(1, dtype=dev.float64, device="cuda")
+        cs = dev.zeros(1, dtype=dev.float64, device="cuda")
+        if bc is None:
+            S.stepgrid(qi, qf, bed, 1, mx, my, pt.dx, pt.dy, dtd, pdev, cf)
+        else:
+            S.stepgrid_bc(qi, qf, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cf)
+        S.stepgrid_split(qi, qs, bed, 1, mx, my, pt.dx, pt.dy, bc, dtd, pdev, cs, ws,
+                         mode="hybrid")
+        dev.cuda.synchronize()
+    finally:
+        L.dclaw_set_seg_rows(prev)
+    a, b = S.to_host_q(qs), S.to_host_q(qf)
+    assert not np.isnan(a).any()
+    assert np.array_equal(a, b), f"{(a != b).sum()} values differ"
+    assert cs.item() == cf.item()
+    w = ws.cpu().numpy()
+    nw, nd = int(w[3 * ntiles]), int(w[3 * ntiles + 1])
+    wet_list = w[ntiles:ntiles + nw]
+    dry_list = w[2 * ntiles:2 * ntiles + nd]
+    ngi = -(-nseg // 4)
+    flagged = set()
+    for g in range(nstrips * ngi):
+        st, gi = g % nstrips, g // nstrips
+        segs = [s_ * nstrips + st for s_ in range(gi * 4, min(nseg, gi * 4 + 4))]
+        if prev_flags[segs].any():
+            flagged.update(segs)
+    assert set(wet_list.tolist()) == flagged and len(wet_list) == len(flagged)
+    assert len(dry_list) + len({t // nstrips // 4 * nstrips + t % nstrips for t in flagged}) \
+        == nstrips * ngi
     assert set(w[:ntiles].tolist()) <= {0, 1}
