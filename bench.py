@@ -57,6 +57,10 @@ def parse():
                     help="ensemble: dispatch the work list in its own order instead of the "
                          "previous step's wet tiles first")
     ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--ens-lo", type=int, default=64, help="ensemble: smallest patch side")
+    ap.add_argument("--ens-hi", type=int, default=512, help="ensemble: largest patch side "
+                    "(sides log-uniform in [lo, hi]; lo = hi = 1024 with 64 members is the "
+                    "batched-adjoint shape)")
     ap.add_argument("--seg-rows", type=int, default=0,
                     help="rows per segment tile; 0 = pick with the autotuner (untimed)")
     ap.add_argument("--fused", action="store_true",
@@ -477,7 +481,7 @@ def bench_ensemble(args):
     from dclaw_amd import workloads as W
 
     t0 = time.time()
-    members = W.ensemble(args.members, 64, 512, seed=args.seed)
+    members = W.ensemble(args.members, args.ens_lo, args.ens_hi, seed=args.seed)
     seg = args.seg_rows
     tune_times = None
     if seg <= 0:
@@ -623,7 +627,8 @@ def bench_ensemble(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"batched ensemble forward, {len(members)} members, sides "
-                               f"log-uniform in [64,512], per-member phi/m0/kref, one batched "
+                               f"log-uniform in [{args.ens_lo},{args.ens_hi}], per-member "
+                               f"phi/m0/kref, one batched "
                                f"launch per step (plus the src2 pass over wet tiles unless "
                                f"fused); the tangent-linear batched step is timed under "
                                f"'tangent'; the adjoint is per patch (timed on the patch and "

@@ -227,7 +227,8 @@ class EnsembleAdjoint:
     state q_n -> lambda_n, and d/d(parameters) for the named parameters (the
     same names for every member, each member's own values), with the members'
     own dt.  Each member keeps its own workspace.  ``step_plain`` is the
-    per-member loop of single-patch adjoint steps, for comparison."""
+    per-member loop of single-patch adjoint steps, for comparison (on the
+    same workspaces)."""
 
     def __init__(self, ens, parameters: Sequence[str] = ("phi", "kref")):
         self.ens = ens
@@ -235,8 +236,9 @@ class EnsembleAdjoint:
         self.np = len(self.dirs)
         n = ens.n
         dev = ens.dt.device
+        # one workspace per member, valid for the batched and the single-patch
+        # call alike (same layout and contents between calls)
         self.work = [new_workspace(ens.mx[k], ens.my[k], dev) for k in range(n)]
-        self.work_plain = None
         self.grad = torch.zeros((n, max(self.np, 1)), dtype=torch.float64, device=dev)
         # member k's directions at dparams[k * np .. k * np + np - 1]
         self.dparams = (torch.cat([_record_tensor(d, dev) for _ in range(n) for d in self.dirs])
@@ -272,15 +274,13 @@ class EnsembleAdjoint:
                    lam_in: Sequence[torch.Tensor]) -> torch.Tensor:
         """The same by one single-patch adjoint step per member."""
         e = self.ens
-        if self.work_plain is None:
-            self.work_plain = [new_workspace(e.mx[k], e.my[k], e.dt.device) for k in range(e.n)]
         for k in range(e.n):
             rec = e.params[k * self._rec:(k + 1) * self._rec]
             dpar = (self.dparams[k * self.np * self._rec:(k + 1) * self.np * self._rec]
                     if self.np else None)
             adjoint_step(q[k], e.bed[k], e.maux[k], e.mx[k], e.my[k], e.dx[k], e.dy[k], e.bc,
                          e.dt[k:k + 1], rec, dpar, self.np, lam_out[k], lam_in[k], None,
-                         self.grad[k] if self.np else None, self.work_plain[k])
+                         self.grad[k] if self.np else None, self.work[k])
         return self.grad[:, :self.np]
 
 
