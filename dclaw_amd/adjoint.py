@@ -21,7 +21,7 @@ parameters, differentiable by ``torch.autograd``."""
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -103,7 +103,6 @@ class RolloutSpec:
     parameters: Tuple[str, ...] = ()
     checkpoint_every: int = 100
     device: str = "cuda"
-    _dev: dict = field(default_factory=dict, repr=False)
 
 
 class Rollout:
