@@ -668,6 +668,112 @@ DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
     for (int m = 0; m < 7; m++) ws(A, W_LQN + m, n, lx[m]);
 }
 
+// Vector-Jacobian product of one rpt2 split (SplitF, rpt2_cell_split)
+// written out: given the adjoints ly of bm[6], bp[6], the adjoints lx of
+// (h, hu, hv, hm, hchi, bdif, asdq[6]) and, in gp[k], the parameter
+// directions' contributions (the split reads grav, rho_s, rho_f).  The same
+// branches as the split (dry cell, eta below both neighbours' bed, the sign
+// of each speed and the eta test of its side); equals vjp<NSI, 12> of SplitF
+// up to rounding.
+DC_HD void split_vjp(const AdjArgs& A, const SplitF& f, const double* x, const double* ly,
+                     double* lx, double* gp)
+{
+    const dclaw_params& P = A.P;
+    for (int k = 0; k < NSI; k++) lx[k] = 0.0;
+    const double h = x[0];
+    if (!f.active || !(h > P.dry_tolerance)) return;
+    const bool xd = (f.ixy == 1);
+    const int mu = xd ? 1 : 2, mv = xd ? 2 : 1;
+    const double nu = xd ? x[1] : x[2], nv = xd ? x[2] : x[1];
+    const double u = nu / h, v = nv / h, m = x[3] / h, chi = x[4] / h;
+    const double eta = h + f.bedc - x[5];
+    const double g = P.grav;
+    const double rho = m * P.rho_s + (1.0 - m) * P.rho_f;
+    double gamma;
+    if (P.src2method == -1) gamma = P.rho_f / rho;
+    else gamma = 0.25 * (P.rho_f + 3.0 * rho) / rho;
+    const double tmin = (f.topo1 < f.topo3) ? f.topo1 : f.topo3;
+    if (eta < tmin) return;
+    const double sq = sqrt(g * h);
+    const double s[3] = {v - sq, v, v + sq};
+    const double* asdq = x + 6;
+    const double d1 = asdq[0], d2 = asdq[mu], d3 = asdq[mv];
+    const double D = s[2] - s[0];
+    const double be[3] = {(s[2] * d1 - d3) / D, -(u * d1) + d2, (d3 - s[0] * d1) / D};
+    const double r5 = gamma * rho * g;
+    double r[3][6];
+    r[0][0] = 1.0; r[0][mu] = u; r[0][mv] = s[0]; r[0][3] = m; r[0][4] = r5; r[0][5] = chi;
+    r[2][0] = 1.0; r[2][mu] = u; r[2][mv] = s[2]; r[2][3] = m; r[2][4] = r5; r[2][5] = chi;
+    for (int k = 0; k < 6; k++) r[1][k] = 0.0;
+    r[1][mu] = 1.0;
+    double ls[3] = {0, 0, 0}, lbe[3] = {0, 0, 0}, lr[3][6];
+    for (int w = 0; w < 3; w++)
+        for (int k = 0; k < 6; k++) lr[w][k] = 0.0;
+    for (int w = 0; w < 3; w++) {
+        const double* lt = nullptr;
+        if (s[w] < 0.0 && eta >= f.topo1) lt = ly;
+        else if (s[w] > 0.0 && eta >= f.topo3) lt = ly + 6;
+        if (!lt) continue;
+        // out[k] += ((1 * s) * be) * r[k]
+        for (int k = 0; k < 6; k++) {
+            ls[w] += lt[k] * be[w] * r[w][k];
+            lbe[w] += lt[k] * s[w] * r[w][k];
+            lr[w][k] += lt[k] * s[w] * be[w];
+        }
+    }
+    double lu = lr[0][mu] + lr[2][mu], lm = lr[0][3] + lr[2][3];
+    double lr5 = lr[0][4] + lr[2][4], lchi = lr[0][5] + lr[2][5];
+    ls[0] += lr[0][mv];
+    ls[2] += lr[2][mv];
+    // betas
+    double ld1 = 0.0, ld2 = 0.0, ld3 = 0.0;
+    ls[2] += lbe[0] * (d1 - be[0]) / D;
+    ls[0] += lbe[0] * be[0] / D;
+    ld1 += lbe[0] * s[2] / D;
+    ld3 += -lbe[0] / D;
+    lu += -(lbe[1] * d1);
+    ld1 += -(lbe[1] * u);
+    ld2 += lbe[1];
+    ls[0] += lbe[2] * (be[2] - d1) / D;
+    ls[2] += -(lbe[2] * be[2]) / D;
+    ld3 += lbe[2] / D;
+    ld1 += -(lbe[2] * s[0]) / D;
+    lx[6 + 0] += ld1;
+    lx[6 + mu] += ld2;
+    lx[6 + mv] += ld3;
+    // speeds: s1 = v - sq, s2 = v, s3 = v + sq; sq = sqrt(g h)
+    const double lv = ls[0] + ls[1] + ls[2];
+    const double lsq = ls[2] - ls[0];
+    double lh = 0.0, lg = 0.0;
+    if (sq > 0.0) {
+        lh += lsq * g / (2.0 * sq);
+        lg += lsq * h / (2.0 * sq);
+    }
+    // r5 = gamma rho g; gamma(rho, rho_f); rho = m rho_s + (1 - m) rho_f
+    double lrho = lr5 * gamma * g, lrhof = 0.0;
+    lg += lr5 * gamma * rho;
+    const double lgam = lr5 * rho * g;
+    if (P.src2method == -1) {
+        lrho += -(lgam * gamma) / rho;
+        lrhof += lgam / rho;
+    } else {
+        lrho += -(lgam * 0.25 * P.rho_f) / (rho * rho);
+        lrhof += lgam * 0.25 / rho;
+    }
+    lm += lrho * (P.rho_s - P.rho_f);
+    const double lrhos = lrho * m;
+    lrhof += lrho * (1.0 - m);
+    // u = nu / h, v = nv / h, m = hm / h, chi = hchi / h
+    lx[xd ? 1 : 2] += lu / h;
+    lx[xd ? 2 : 1] += lv / h;
+    lx[3] += lm / h;
+    lx[4] += lchi / h;
+    lh += -(lu * u + lv * v + lm * m + lchi * chi) / h;
+    lx[0] += lh;
+    for (int p = 0; p < A.np; p++)
+        gp[p] += lg * A.dP[p].grav + lrhos * A.dP[p].rho_s + lrhof * A.dP[p].rho_f;
+}
+
 // B2: the four transverse splits of the cell.
 DC_HD void rev_split(const AdjArgs& A, const TP& Pz, int i, int j)
 {
@@ -693,10 +799,8 @@ DC_HD void rev_split(const AdjArgs& A, const TP& Pz, int i, int j)
                 ly[6 + z] = -hy * (lam_fm(A, i + 1, j, z) + lam_fp(A, i + 1, j, z));
             }
         }
-        // rpt2 splits nothing in an inactive or dry cell (zero Jacobian)
-        if (!f.active || !(x[0] > A.P.dry_tolerance)) continue;
         double lx[NSI];
-        vjp<NSI, 12>(A, f, Pz, x, ly, lx, g);
+        split_vjp(A, f, x, ly, lx, g);
         for (int k = 0; k < 6; k++) ls[k] += lx[k];
         for (int k = 0; k < 6; k++) la[s][k] = lx[6 + k];
     }
@@ -708,6 +812,155 @@ DC_HD void rev_split(const AdjArgs& A, const TP& Pz, int i, int j)
         ws(A, W_LYA + 6 + k, n, la[3][k]);
     }
     gp_add(A, n, g);
+}
+
+// d limiter_phi / dr on the branch the value takes (max/min ties as dmax and
+// dmin resolve them: the second argument).
+DC_HD double limiter_dphi(int mthlim, double r)
+{
+    switch (mthlim) {
+    case 1: {
+        const double m = (1.0 < r) ? 1.0 : r;
+        const double dm = (1.0 < r) ? 0.0 : 1.0;
+        return (0.0 > m) ? 0.0 : dm;
+    }
+    case 2: {
+        const double i = (1.0 < 2.0 * r) ? 1.0 : 2.0 * r;
+        const double di = (1.0 < 2.0 * r) ? 0.0 : 2.0;
+        const double a = (0.0 > i) ? 0.0 : i;
+        const double da = (0.0 > i) ? 0.0 : di;
+        const double b = (2.0 < r) ? 2.0 : r;
+        const double db = (2.0 < r) ? 0.0 : 1.0;
+        return (a > b) ? da : db;
+    }
+    case 3: {
+        const double a = fabs(r), da = signbit(r) ? -1.0 : 1.0;
+        const double den = 1.0 + a;
+        const double q = (r + a) / den;
+        return ((1.0 + da) - q * da) / den;
+    }
+    case 4: {
+        const double c = (1.0 + r) / 2.0;
+        const double m1 = (c < 2.0) ? c : 2.0;
+        const double dm1 = (c < 2.0) ? 0.5 : 0.0;
+        const double m2 = (m1 < 2.0 * r) ? m1 : 2.0 * r;
+        const double dm2 = (m1 < 2.0 * r) ? dm1 : 2.0;
+        return (0.0 > m2) ? 0.0 : dm2;
+    }
+    default: return 1.0;
+    }
+}
+
+// Vector-Jacobian product of the flux2 edge body (FluxF, dclaw_strip_body.inc
+// edge_flux) written out: the function is piecewise linear in the waves given
+// the signs of the speeds, the limiter ratios and the wet-dry gate, so its
+// transpose is exact and cheap.  x: the 23 inputs (waves, dotl, dotr); ly:
+// adjoints of faddm, faddp, amdq, apdq; lx: adjoints of the inputs.  It
+// equals vjp<NEI, NEF> of FluxF up to rounding (the derivatives are those of
+// the branch the values take, as the tangent-linear evaluation defines them).
+DC_HD void flux_vjp(const dclaw_params& P, const FluxF& f, const double* x, const double* ly,
+                    double* lx)
+{
+    const int nrm = (f.ixy == 1) ? 1 : 2;
+    const double* s = x;
+    const double* w1 = x + 3;
+    const double* w5 = x + 9;
+    const double w2h = x[15], w2n = x[16];
+    const double* dotl = x + NW;
+    const double* dotr = x + NW + 3;
+    for (int k = 0; k < NEI; k++) lx[k] = 0.0;
+    double* ls = lx;
+    double* lw1 = lx + 3;
+    double* lw5 = lx + 9;
+    double* ldotl = lx + NW;
+    double* ldotr = lx + NW + 3;
+    const bool fold = (P.order > 1 && P.transverse == 2);
+    double lam[6], lap[6], lcq[6];
+    for (int k = 0; k < 6; k++) {
+        const double lfm = ly[k], lfp = ly[6 + k], lam_o = ly[12 + k], lap_o = ly[18 + k];
+        lam[k] = lfm + lam_o;
+        lap[k] = -lfp + lap_o;
+        lcq[k] = 0.5 * lfm + 0.5 * lfp + (fold ? lam_o - lap_o : 0.0);
+    }
+    // fluctuations: each wave joins amdq or apdq by the sign of its speed
+    if (s[0] < 0.0) { for (int k = 0; k < 6; k++) lw1[k] += lam[k]; }
+    else if (s[0] > 0.0) { for (int k = 0; k < 6; k++) lw1[k] += lap[k]; }
+    if (s[1] < 0.0) { lx[15] += lam[0]; lx[16] += lam[nrm]; }
+    else if (s[1] > 0.0) { lx[15] += lap[0]; lx[16] += lap[nrm]; }
+    if (s[2] < 0.0) { for (int k = 0; k < 6; k++) lw5[k] += lam[k]; }
+    else if (s[2] > 0.0) { for (int k = 0; k < 6; k++) lw5[k] += lap[k]; }
+    if (P.order == 1 || !f.gate) return;   // no correction: cq = 0
+    // limiter
+    double wn2[3];
+    {
+        double d = 0.0;
+        for (int k = 0; k < 6; k++) d = d + w1[k] * w1[k];
+        wn2[0] = d;
+        d = 0.0;
+        d = d + w2h * w2h;
+        d = d + w2n * w2n;
+        wn2[1] = d;
+        d = 0.0;
+        for (int k = 0; k < 6; k++) d = d + w5[k] * w5[k];
+        wn2[2] = d;
+    }
+    const int fam_mw[3] = {0, 1, 4};
+    double phi[3] = {1.0, 1.0, 1.0}, rr[3] = {0.0, 0.0, 0.0};
+    bool lim_on[3];
+    for (int q = 0; q < 3; q++) {
+        const int lim = P.mthlim[fam_mw[q]];
+        lim_on[q] = (lim != 0 && wn2[q] != 0.0);
+        if (!lim_on[q]) continue;
+        rr[q] = (s[q] > 0.0 ? dotl[q] : dotr[q]) / wn2[q];
+        // limiter_phi's value, as dclaw::limiter_phi computes it
+        phi[q] = dclaw::limiter_phi(lim, rr[q]);
+    }
+    // cq[k] = c0 l1[k] + [k = 0] c1 l2h + [k = nrm] c1 l2n + c2 l5[k]
+    double c[3], dcds[3];
+    for (int q = 0; q < 3; q++) {
+        const double sg = copysign(1.0, s[q]);
+        c[q] = sg * (1.0 - fabs(s[q]) * f.dtdx);
+        dcds[q] = sg * (-((signbit(s[q]) ? -1.0 : 1.0) * f.dtdx));
+    }
+    double lc[3] = {0.0, 0.0, 0.0}, lphi[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 6; k++) {
+        const double l1 = lim_on[0] ? phi[0] * w1[k] : w1[k];
+        const double l5 = lim_on[2] ? phi[2] * w5[k] : w5[k];
+        lc[0] += l1 * lcq[k];
+        lc[2] += l5 * lcq[k];
+        const double ll1 = c[0] * lcq[k], ll5 = c[2] * lcq[k];
+        lw1[k] += lim_on[0] ? phi[0] * ll1 : ll1;
+        lw5[k] += lim_on[2] ? phi[2] * ll5 : ll5;
+        if (lim_on[0]) lphi[0] += w1[k] * ll1;
+        if (lim_on[2]) lphi[2] += w5[k] * ll5;
+    }
+    {
+        const double l2h = lim_on[1] ? phi[1] * w2h : w2h;
+        const double l2n = lim_on[1] ? phi[1] * w2n : w2n;
+        lc[1] += l2h * lcq[0] + l2n * lcq[nrm];
+        const double ll2h = c[1] * lcq[0], ll2n = c[1] * lcq[nrm];
+        lx[15] += lim_on[1] ? phi[1] * ll2h : ll2h;
+        lx[16] += lim_on[1] ? phi[1] * ll2n : ll2n;
+        if (lim_on[1]) lphi[1] += w2h * ll2h + w2n * ll2n;
+    }
+    for (int q = 0; q < 3; q++) ls[q] += lc[q] * dcds[q];
+    // back through phi(r), r = dot / wn2, wn2 = sum of squares of the family
+    for (int q = 0; q < 3; q++) {
+        if (!lim_on[q] || lphi[q] == 0.0) continue;
+        const double lr = lphi[q] * limiter_dphi(P.mthlim[fam_mw[q]], rr[q]);
+        const double ldot = lr / wn2[q];
+        const double lwn2 = -(lr * rr[q]) / wn2[q];
+        if (s[q] > 0.0) ldotl[q] += ldot;
+        else ldotr[q] += ldot;
+        if (q == 0) {
+            for (int k = 0; k < 6; k++) lw1[k] += 2.0 * w1[k] * lwn2;
+        } else if (q == 1) {
+            lx[15] += 2.0 * w2h * lwn2;
+            lx[16] += 2.0 * w2n * lwn2;
+        } else {
+            for (int k = 0; k < 6; k++) lw5[k] += 2.0 * w5[k] * lwn2;
+        }
+    }
 }
 
 // B3: the flux body of the x- and y-edge at (i, j).
@@ -735,10 +988,8 @@ DC_HD void rev_flux(const AdjArgs& A, const TP& Pz, int i, int j)
                     ly[18 + z] = wv(A, W_LYA + 6 + z, n);
                 }
             }
-            // with the gate dry and every speed 0 the edge body adds nothing
-            // (waves with s = 0 are dropped, no correction): zero Jacobian
-            if (f.gate || x[0] != 0.0 || x[1] != 0.0 || x[2] != 0.0)
-                vjp<NEI, NEF>(A, f, Pz, x, ly, lx, g);
+            // the edge body has no parameter dependence
+            flux_vjp(A.P, f, x, ly, lx);
         }
         const int f0 = ixy == 1 ? W_LXE : W_LYE;
         for (int k = 0; k < NEI; k++) ws(A, f0 + k, n, lx[k]);

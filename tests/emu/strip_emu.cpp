@@ -551,3 +551,108 @@ extern "C" int emu_adjoint_step(const dclaw_params* P, const dclaw_params* dP, i
     }
     return 0;
 }
+
+// The hand-written flux2 edge-body adjoint (flux_vjp) against the seeded
+// tangent-linear vector-Jacobian product of the same function: n random
+// cases (waves, limiter dot products, adjoints) for the given scheme switches;
+// returns the largest |difference| relative to the largest |entry|.
+extern "C" double emu_flux_vjp_check(const dclaw_params* P, int n, unsigned seed)
+{
+    namespace D = dclaw_adj;
+    unsigned long long st = seed * 2862933555777941757ull + 3037000493ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)((st >> 11) & ((1ull << 53) - 1)) / (double)(1ull << 53) * 2.0 - 1.0;
+    };
+    D::AdjArgs A{};
+    A.P = *P;
+    A.np = 0;
+    const D::TP Pz = D::zero_params(*P);
+    double worst = 0.0;
+    for (int c = 0; c < n; c++) {
+        double x[D::NEI], ly[D::NEF], la[D::NEI], lb[D::NEI];
+        for (int k = 0; k < D::NEI; k++) x[k] = rnd();
+        // some speeds exactly 0 and both signs; dot products of either sign
+        if (c % 7 == 0) x[c % 3] = 0.0;
+        for (int k = 0; k < D::NEF; k++) ly[k] = rnd();
+        D::FluxF f;
+        f.ixy = 1 + (c & 1);
+        f.gate = (c % 5) != 0;
+        f.dtdx = 0.3 + 0.2 * rnd();
+        D::vjp<D::NEI, D::NEF>(A, f, Pz, x, ly, la, nullptr);
+        D::flux_vjp(*P, f, x, ly, lb);
+        double scale = 0.0, diff = 0.0;
+        for (int k = 0; k < D::NEI; k++) {
+            scale = std::max(scale, std::fabs(la[k]));
+            diff = std::max(diff, std::fabs(la[k] - lb[k]));
+        }
+        if (scale > 0.0) worst = std::max(worst, diff / scale);
+        else worst = std::max(worst, diff);
+    }
+    return worst;
+}
+
+// The hand-written rpt2 split adjoint (split_vjp) against the seeded
+// tangent-linear vector-Jacobian product (inputs and the grav, rho_s, rho_f
+// directions): n random wet cells with random fluctuations, both sweep
+// directions and random neighbour beds; returns the largest difference
+// relative to the largest input adjoint (parameters: to the largest
+// sensitivity to a relative parameter change).
+extern "C" double emu_split_vjp_check(const dclaw_params* P, int n, unsigned seed)
+{
+    namespace D = dclaw_adj;
+    unsigned long long st = seed * 2862933555777941757ull + 3037000493ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)((st >> 11) & ((1ull << 53) - 1)) / (double)(1ull << 53) * 2.0 - 1.0;
+    };
+    dclaw_params dP[3] = {};
+    dP[0].grav = 1.0;
+    dP[1].rho_s = 1.0;
+    dP[2].rho_f = 1.0;
+    D::AdjArgs A{};
+    A.P = *P;
+    A.dP = dP;
+    A.np = 3;
+    const D::TP Pz = D::zero_params(*P);
+    double worst = 0.0;
+    for (int c = 0; c < n; c++) {
+        double x[D::NSI], ly[12], la[D::NSI], lb[D::NSI];
+        x[0] = 0.05 + 2.0 * (rnd() + 1.0);                    // h
+        x[1] = 3.0 * rnd() * x[0];
+        x[2] = 3.0 * rnd() * x[0];
+        x[3] = (0.5 + 0.2 * rnd()) * x[0];
+        x[4] = 0.3 * (rnd() + 1.0) * x[0];
+        x[5] = 0.1 * rnd();
+        for (int k = 6; k < D::NSI; k++) x[k] = rnd();
+        for (int k = 0; k < 12; k++) ly[k] = rnd();
+        D::SplitF f;
+        f.ixy = 1 + (c & 1);
+        f.bedc = rnd();
+        f.topo1 = f.bedc + 2.0 * rnd();
+        f.topo3 = f.bedc + 2.0 * rnd();
+        f.active = true;
+        double ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+        D::vjp<D::NSI, 12>(A, f, Pz, x, ly, la, ga);
+        D::split_vjp(A, f, x, ly, lb, gb);
+        double scale = 0.0, diff = 0.0;
+        for (int k = 0; k < D::NSI; k++) {
+            scale = std::max(scale, std::fabs(la[k]));
+            diff = std::max(diff, std::fabs(la[k] - lb[k]));
+        }
+        // parameter derivatives compared as sensitivities to a relative change
+        // of each parameter (|dL/dp| |p|), against the largest of the three: a
+        // derivative that is 0 in exact arithmetic (d/drho_s with src2method
+        // -1, where gamma rho = rho_f) is rounding residue in both forms
+        const double pm[3] = {P->grav, P->rho_s, P->rho_f};
+        double gs = 0.0;
+        for (int k = 0; k < 3; k++) gs = std::max(gs, std::fabs(ga[k] * pm[k]));
+        for (int k = 0; k < 3; k++) {
+            const double d = std::fabs((ga[k] - gb[k]) * pm[k]);
+            if (gs > 0.0) worst = std::max(worst, d / gs);
+        }
+        if (scale > 0.0) worst = std::max(worst, diff / scale);
+        else worst = std::max(worst, diff);
+    }
+    return worst;
+}
