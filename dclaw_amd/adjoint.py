@@ -283,6 +283,103 @@ class EnsembleAdjoint:
         return self.grad[:, :self.np]
 
 
+class EnsembleRollout:
+    """N steps of an ensemble (one batched launch per step, each member with
+    its own parameters and dt rule) with per-member gauge records, and the
+    reverse pass of all members together (one batched adjoint launch per
+    pass per step, checkpointed recompute): the particle-filter form of
+    ``Rollout``.  Member k's results equal a ``Rollout`` of that member alone
+    bit for bit (the batched step and adjoint equal the single-patch ones).
+
+    gauges_xy: one (ng, 2) array of gauge positions per member (each member's
+    patch has xlower = ylower = 0)."""
+
+    def __init__(self, members: List, nsteps: int, gauges_xy: Sequence[np.ndarray],
+                 parameters: Sequence[str] = ("phi", "kref"), checkpoint_every: int = 100,
+                 dt0: float = 1e-3, cfl_desired: float = 0.9, dt_max: float = 1e9,
+                 bc=(BC_EXTRAP,) * 4, seg_rows: int = 16):
+        from .step import Ensemble
+        if nsteps < 1 or checkpoint_every < 1:
+            raise ValueError("nsteps and checkpoint_every must be >= 1")
+        if len(gauges_xy) != len(members):
+            raise ValueError("one gauge array per member")
+        self.members = members
+        self.nsteps = nsteps
+        self.every = checkpoint_every
+        self.dt0, self.cfl_desired, self.dt_max = dt0, cfl_desired, dt_max
+        self.ens = Ensemble(members, bc=bc, seg_rows=seg_rows)
+        self.adj = EnsembleAdjoint(self.ens, parameters)
+        dev = self.ens.dt.device
+        self.xy = [torch.from_numpy(np.ascontiguousarray(
+            np.asarray(g, dtype=np.float64).reshape(-1, 2))).to(dev) for g in gauges_xy]
+        self.drytol = [m.patch.params.dry_tolerance for m in members]
+
+    def _states(self) -> List[torch.Tensor]:
+        return [self.ens.current(k) for k in range(self.ens.n)]
+
+    def _gauges(self, series: List[torch.Tensor], n: int) -> None:
+        e = self.ens
+        for k in range(e.n):
+            gauges(e.current(k), e.bed[k], e.maux[k], e.mx[k], e.my[k], 0.0, 0.0, e.dx[k],
+                   e.dy[k], self.drytol[k], self.xy[k], series[k][n])
+
+    def forward(self):
+        """Runs the ensemble from its members' initial states; returns the
+        gauge records (one (nsteps, ng_k, 8) tensor per member) and the final
+        states."""
+        e = self.ens
+        e.dt.fill_(float(self.dt0))
+        e.cfl.zero_()
+        dev = e.dt.device
+        self.dts = torch.zeros((self.nsteps, e.n), dtype=torch.float64, device=dev)
+        self.ckpt = {0: [x.clone() for x in self._states()]}
+        series = [torch.zeros((self.nsteps, xy.shape[0], 8), dtype=torch.float64, device=dev)
+                  for xy in self.xy]
+        for n in range(self.nsteps):
+            self.dts[n].copy_(e.dt)
+            e.step()
+            dt_update(e.dt, e.cfl, self.cfl_desired, self.dt_max)
+            self._gauges(series, n)
+            if (n + 1) % self.every == 0 and n + 1 < self.nsteps:
+                self.ckpt[n + 1] = [x.clone() for x in self._states()]
+        return series, [x.clone() for x in self._states()]
+
+    def backward(self, g_series: Optional[Sequence[torch.Tensor]] = None,
+                 g_final: Optional[Sequence[torch.Tensor]] = None):
+        """Adjoint of a loss given its gradients with respect to the gauge
+        records and the final states: returns (lambda_0 per member, the (n, np)
+        parameter gradients)."""
+        e = self.ens
+        n_m = e.n
+        lam = [g.detach().clone().contiguous() if g is not None else torch.zeros_like(x)
+               for g, x in zip(g_final or [None] * n_m, self.ckpt[0])]
+        lam_b = [torch.empty_like(x) for x in lam]
+        grad = torch.zeros((n_m, self.adj.np), dtype=torch.float64, device=e.dt.device)
+        for b0 in reversed(sorted(self.ckpt)):
+            b1 = min(b0 + self.every, self.nsteps)
+            for k in range(n_m):
+                e.current(k).copy_(self.ckpt[b0][k])
+            states = [[x.clone() for x in self._states()]]
+            for n in range(b0, b1):
+                e.dt.copy_(self.dts[n])
+                e.cfl.zero_()
+                e.step()
+                states.append([x.clone() for x in self._states()])
+            for n in range(b1 - 1, b0 - 1, -1):
+                if g_series is not None:
+                    qn1 = states[n + 1 - b0]
+                    for k in range(n_m):
+                        gauges_adj(qn1[k], e.bed[k], e.maux[k], e.mx[k], e.my[k], 0.0, 0.0,
+                                   e.dx[k], e.dy[k], self.drytol[k], self.xy[k],
+                                   g_series[k][n].detach().contiguous(), lam[k])
+                e.dt.copy_(self.dts[n])
+                g = self.adj.step(states[n - b0], lam, lam_b)
+                grad += g
+                lam, lam_b = lam_b, lam
+            del states
+        return lam, grad
+
+
 class _RolloutFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, q0, theta, spec):

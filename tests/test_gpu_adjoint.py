@@ -299,3 +299,45 @@ def test_batched_adjoint_equals_per_patch(dev):
         assert torch.equal(gb, gp), call
         assert float(gb.abs().max()) > 0.0
         ens.run(5, ens.dt.clone(), 0.9, 1e9)
+
+
+def test_ensemble_rollout_equals_member_rollouts(dev):
+    """Four members of uneven sizes, each with its own gauges and targets:
+    the batched reverse sweep (EnsembleRollout, checkpoints every 5 steps)
+    gives each member's gauge-misfit gradient (lambda_0 and d/dphi, d/dkref)
+    bit for bit as that member's own Rollout."""
+    from dclaw_amd import adjoint as AD
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    torch = dev
+    members = W.ensemble(4, 60, 140, seed=9)
+    nsteps = 12
+    rng = np.random.default_rng(3)
+    gxy, tg = [], []
+    for m in members:
+        p = m.patch
+        xy = np.column_stack([rng.uniform(2.0, (p.mx - 2) * p.dx, 3),
+                              rng.uniform(2.0, (p.my - 2) * p.dy, 3)])
+        gxy.append(xy)
+        tg.append(torch.tensor(rng.uniform(0.0, 0.5, (nsteps, 3)), device="cuda"))
+    er = AD.EnsembleRollout(members, nsteps, gxy, parameters=("phi", "kref"), checkpoint_every=5)
+    series, qf = er.forward()
+    gs = [torch.zeros_like(s) for s in series]
+    for k in range(len(members)):
+        gs[k][:, :, 0] = 2.0 * (series[k][:, :, 0] - tg[k])
+    lam0, grad = er.backward(gs, None)
+    torch.cuda.synchronize()
+    for k, m in enumerate(members):
+        p = m.patch
+        spec = AD.RolloutSpec(aux=p.aux, params=p.params, dx=p.dx, dy=p.dy, nsteps=nsteps,
+                              gauges_xy=gxy[k], parameters=("phi", "kref"), checkpoint_every=5)
+        r = AD.Rollout(spec)
+        s1, q1 = r.forward(S.to_device_q(p.q), r.record())
+        assert torch.equal(s1, series[k]) and torch.equal(q1, qf[k]), k
+        g1 = torch.zeros_like(s1)
+        g1[:, :, 0] = 2.0 * (s1[:, :, 0] - tg[k])
+        l1, gr1 = r.backward(g1, None)
+        torch.cuda.synchronize()
+        assert torch.equal(l1, lam0[k]), k
+        assert torch.equal(gr1, grad[k]), k
+    assert float(grad.abs().max()) > 0.0
