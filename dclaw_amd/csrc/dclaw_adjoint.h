@@ -1027,6 +1027,786 @@ DC_HD void lam_waves(const AdjArgs& A, int ixy, int i, int j, double* lw)
 }
 
 // B4: rpn2 of the x- and y-edge at (i, j).
+// The inputs and wave adjoints of one edge's rpn2 product; false when the
+// product is zero (no such edge, both sides dry: rpn2 returns constant zero
+// waves, or a zero wave adjoint).
+DC_HD bool wave_edge(const AdjArgs& A, int ixy, int i, int j, double* x, Rpn2F& f, double* ly)
+{
+    if (!(ixy == 1 ? xedge(A, i, j) : yedge(A, i, j))) return false;
+    rpn2_inputs(A, ixy, i, j, x, f);
+    const double drytol = A.P.dry_tolerance;
+    if (!(x[SD_H] > drytol || x[NSD + SD_H] > drytol)) return false;
+    lam_waves(A, ixy, i, j, ly);
+    return !all_zero<NW>(ly);
+}
+
+// d sqrt(a) = da / (2 sqrt(a)), 0 where sqrt(a) = 0 (the tangent-linear
+// convention): the adjoint of a from that of s = sqrt(a).
+DC_HD double sqrt_adj(double s, double sb) { return s > 0.0 ? sb / (2.0 * s) : 0.0; }
+
+// Adjoint of riemanntype's speeds s1m, s2m (riemannsolvers_dclaw.f:526-637)
+// with respect to hL, hR, uL, uR and g, added to hLb ... gb.  hm enters rpn2
+// only through these speeds and comparisons.  Same branches as riemanntype;
+// in the mixed branch the term (h0 - h_min) sqrt(...) is 0 with derivative 0.
+DC_HD void riemanntype_vjp(double hL, double hR, double uL, double uR, double drytol, double g,
+                           double s1b, double s2b, double& hLb, double& hRb, double& uLb,
+                           double& uRb, double& gb)
+{
+    if (s1b == 0.0 && s2b == 0.0) return;
+    // s = sqrt(g a): adjoints of g and a from that of s
+    auto sg = [&](double a, double sb, double& ab) {
+        const double t = sqrt_adj(sqrt(g * a), sb);
+        gb += t * a;
+        ab += t * g;
+    };
+    const bool minR = hR < hL, maxR = hR > hL;
+    const double h_min = minR ? hR : hL, h_max = maxR ? hR : hL;
+    const double delu = uR - uL;
+    double delub = 0.0, hminb = 0.0, hmaxb = 0.0;
+    if (h_min <= drytol) {
+        const double S = s1b + s2b;
+        uRb += S;
+        uLb += S;
+        sg(hR, -2.0 * S, hRb);
+        sg(hL, 2.0 * S, hLb);
+        return;
+    }
+    const double F_min = delu + 2.0 * (sqrt(g * h_min) - sqrt(g * h_max));
+    const double FQ = sqrt(.5 * g * (h_max + h_min) / (h_max * h_min));
+    const double F_max = delu + (h_max - h_min) * FQ;
+    if (F_min > 0.0) {
+        const double arg = -delu + 2.0 * (sqrt(g * hL) + sqrt(g * hR));
+        const double t = 0.0 > arg ? 0.0 : arg;
+        const double c16 = 1.0 / (16.0 * g);
+        const double hm = c16 * (t * t);
+        uLb += s1b;
+        uRb += s2b;
+        double sLgb = 2.0 * s1b, sRgb = -2.0 * s2b, hmb = 0.0;
+        sg(hm, -3.0 * s1b + 3.0 * s2b, hmb);
+        const double c16b = hmb * (t * t);
+        const double tb = hmb * c16 * 2.0 * t;
+        gb += -(c16b * c16) / g;
+        if (!(0.0 > arg)) {
+            delub -= tb;
+            sLgb += 2.0 * tb;
+            sRgb += 2.0 * tb;
+        }
+        sg(hL, sLgb, hLb);
+        sg(hR, sRgb, hRb);
+    } else if (F_max <= 0.0) {
+        const double h0 = h_max;
+        const double iA = 1.0 / h0 + 1.0 / hL, iB = 1.0 / h0 + 1.0 / hR;
+        const double gL = sqrt(.5 * g * iA), gR = sqrt(.5 * g * iB);
+        const double dL = h0 - hL, dR = h0 - hR;
+        const double F0 = delu + dL * gL + dR * gR;
+        const double h02 = h0 * h0;
+        const double DL = 4.0 * h02 * gL, DR = 4.0 * h02 * gR;
+        const double dfdh = gL - g * dL / DL + gR - g * dR / DR;
+        const double sq0 = sqrt(h0);
+        const double slope = 2.0 * sq0 * dfdh;
+        const double q = F0 / slope;
+        const double t = sq0 - q;
+        const double hm = t * t;
+        const double iC = 1.0 / hm + 1.0 / hL, iD = 1.0 / hm + 1.0 / hR;
+        const double qC = sqrt((.5 * g) * iC), qD = sqrt((.5 * g) * iD);
+        // s1m = uL - (hm - hL) qC - sqrt(g hm), s2m = uR + (hm - hR) qD + sqrt(g hm)
+        double hmb = 0.0;
+        sg(hm, -s1b + s2b, hmb);
+        uRb += s2b;
+        hmb += s2b * qD;
+        hRb -= s2b * qD;
+        const double qDb = s2b * (hm - hR);
+        uLb += s1b;
+        hmb -= s1b * qC;
+        hLb += s1b * qC;
+        const double qCb = -(s1b * (hm - hL));
+        {
+            const double tc = sqrt_adj(qC, qCb), td = sqrt_adj(qD, qDb);
+            gb += tc * .5 * iC + td * .5 * iD;
+            const double iCb = tc * .5 * g, iDb = td * .5 * g;
+            hmb += -(iCb / hm) / hm - (iDb / hm) / hm;
+            hLb += -(iCb / hL) / hL;
+            hRb += -(iDb / hR) / hR;
+        }
+        // hm = t^2, t = sq0 - F0 / slope, slope = 2 sq0 dfdh
+        const double tb = hmb * 2.0 * t;
+        double sq0b = tb;
+        const double F0b = -tb / slope;
+        const double slopeb = tb * q / slope;
+        sq0b += slopeb * 2.0 * dfdh;
+        const double dfdhb = slopeb * 2.0 * sq0;
+        double h0b = 0.0, gLb = dfdhb, gRb = dfdhb, dLb = 0.0, dRb = 0.0, h02b = 0.0;
+        {
+            // dfdh -= g dL / DL (and the same with R)
+            const double TL = g * dL / DL, TR = g * dR / DR;
+            const double nLb = -dfdhb / DL, nRb = -dfdhb / DR;
+            const double DLb = dfdhb * TL / DL, DRb = dfdhb * TR / DR;
+            gb += nLb * dL + nRb * dR;
+            dLb += nLb * g;
+            dRb += nRb * g;
+            h02b += DLb * 4.0 * gL + DRb * 4.0 * gR;
+            gLb += DLb * 4.0 * h02;
+            gRb += DRb * 4.0 * h02;
+        }
+        h0b += h02b * 2.0 * h0;
+        // F0 = delu + dL gL + dR gR
+        delub += F0b;
+        dLb += F0b * gL;
+        gLb += F0b * dL;
+        dRb += F0b * gR;
+        gRb += F0b * dR;
+        h0b += dLb + dRb;
+        hLb -= dLb;
+        hRb -= dRb;
+        {
+            const double ta = sqrt_adj(gL, gLb), tb2 = sqrt_adj(gR, gRb);
+            gb += ta * .5 * iA + tb2 * .5 * iB;
+            const double iAb = ta * .5 * g, iBb = tb2 * .5 * g;
+            h0b += -(iAb / h0) / h0 - (iBb / h0) / h0;
+            hLb += -(iAb / hL) / hL;
+            hRb += -(iBb / hR) / hR;
+        }
+        h0b += sqrt_adj(sq0, sq0b);
+        hmaxb += h0b;
+    } else {
+        const double h0 = h_min;
+        const double sg0 = sqrt(g * h0), sgM = sqrt(g * h_max);
+        const double F0 = delu + 2.0 * (sg0 - sgM)
+                          + (h0 - h_min) * sqrt(.5 * g * (1.0 / h0 + 1.0 / h_min));
+        const double dH = h_max - h_min;
+        const double slope = (F_max - F0) / dH;
+        const double q = F0 / slope;
+        const double hm = h0 - q;
+        double hmb = 0.0;
+        if (hL > hR) {
+            // s1m = uL + 2 sqrt(g hL) - 3 sqrt(g hm), s2m = uL + 2 sqrt(g hL) - sqrt(g hm)
+            uLb += s1b + s2b;
+            sg(hL, 2.0 * (s1b + s2b), hLb);
+            sg(hm, -3.0 * s1b - s2b, hmb);
+        } else {
+            // s2m = uR - 2 sqrt(g hR) + 3 sqrt(g hm), s1m = uR - 2 sqrt(g hR) + sqrt(g hm)
+            uRb += s1b + s2b;
+            sg(hR, -2.0 * (s1b + s2b), hRb);
+            sg(hm, 3.0 * s2b + s1b, hmb);
+        }
+        // hm = h0 - F0 / slope, slope = (F_max - F0) / dH
+        double h0b = hmb;
+        const double qb = -hmb;
+        double F0b = qb / slope;
+        const double slopeb = -(qb * q) / slope;
+        const double nb = slopeb / dH;
+        const double F_maxb = nb;
+        F0b -= nb;
+        double dHb = -(slopeb * slope) / dH;
+        // F_max = delu + dH FQ, FQ = sqrt(.5 g (h_max + h_min) / (h_max h_min))
+        delub += F_maxb;
+        dHb += F_maxb * FQ;
+        {
+            const double t = sqrt_adj(FQ, F_maxb * dH);
+            const double NQ = .5 * g * (h_max + h_min), DQ = h_max * h_min;
+            const double NQb = t / DQ, DQb = -(t * (NQ / DQ)) / DQ;
+            gb += NQb * .5 * (h_max + h_min);
+            hmaxb += NQb * .5 * g + DQb * h_min;
+            hminb += NQb * .5 * g + DQb * h_max;
+        }
+        hmaxb += dHb;
+        hminb -= dHb;
+        // F0 = delu + 2 (sqrt(g h0) - sqrt(g h_max))
+        delub += F0b;
+        sg(h0, 2.0 * F0b, h0b);
+        sg(h_max, -2.0 * F0b, hmaxb);
+        hminb += h0b;
+    }
+    if (minR) hRb += hminb;
+    else hLb += hminb;
+    if (maxR) hRb += hmaxb;
+    else hLb += hmaxb;
+    uRb += delub;
+    uLb -= delub;
+}
+
+// Vector-Jacobian product of rpn2 (Rpn2F, rpn2_edge: rpn2_dclaw.f:125-315 and
+// riemann_dig2_aug_sswave_ez, riemannsolvers_dclaw.f:3-521) written out for
+// dry_tolerance > 0: given the adjoints ly of the 17 wave values, the
+// adjoints lx of the 30 side inputs and, in gp[k], the parameter directions'
+// contributions (rpn2 reads grav, rho_f, kappa and beta_seg).  The forward
+// part replays rpn2_edge's arithmetic exactly, so every branch (dry resets,
+// wall mirror, the rho/tau side choice, riemanntype's structure, the rare
+// corrector, sonic, the source2dx and deldelh clamps, the static test,
+// delta_huvm_revert's fold) is the one the step took; the tangent-linear
+// conventions at kinks (max/min ties, sqrt at 0, copysign) are followed.
+// Equals vjp<2 NSD, NW> of Rpn2F up to rounding.
+DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const double* ly,
+                    double* lx, double* gp)
+{
+    const dclaw_params& P = A.P;
+    for (int k = 0; k < 2 * NSD; k++) lx[k] = 0.0;
+    const double drytol = P.dry_tolerance, grav = P.grav;
+    const double* XL = x;
+    const double* XR = x + NSD;
+    if (XL[SD_H] <= drytol && XR[SD_H] <= drytol) return;
+    const bool xd = (f.ixy == 1);
+    const int NU = xd ? SD_HU : SD_HV, TU = xd ? SD_HV : SD_HU;
+    // effective side values and where each comes from: the index of the
+    // input (-1: a constant) and a sign (a mirrored velocity / momentum)
+    double hL = XL[SD_H], hR = XR[SD_H], huL = XL[NU], huR = XR[NU], hvL = XL[TU], hvR = XR[TU];
+    double hmL = XL[SD_HM], hmR = XR[SD_HM], pL = XL[SD_P], pR = XR[SD_P];
+    double bL = f.bedL - XL[SD_BDIF], bR = f.bedR - XR[SD_BDIF];
+    double uL = XL[SD_U], uR = XR[SD_U], vL = XL[SD_V], vR = XR[SD_V];
+    double mL = XL[SD_M], mR = XR[SD_M], chiL = XL[SD_CHI], chiR = XR[SD_CHI];
+    double svrL = XL[SD_RHO], svtL = XL[SD_TAU], svrR = XR[SD_RHO], svtR = XR[SD_TAU];
+    const double taudirR = XR[SD_TAUDIR], fsL = XL[SD_FS], fsR = XR[SD_FS];
+    int oh[2] = {SD_H, NSD + SD_H}, ohu[2] = {NU, NSD + NU}, ohv[2] = {TU, NSD + TU};
+    int ohm[2] = {SD_HM, NSD + SD_HM}, op[2] = {SD_P, NSD + SD_P};
+    int ob[2] = {SD_BDIF, NSD + SD_BDIF}, ou[2] = {SD_U, NSD + SD_U}, ov[2] = {SD_V, NSD + SD_V};
+    int om[2] = {SD_M, NSD + SD_M}, ochi[2] = {SD_CHI, NSD + SD_CHI};
+    int osr[2] = {SD_RHO, NSD + SD_RHO}, ost[2] = {SD_TAU, NSD + SD_TAU};
+    double shu[2] = {1.0, 1.0}, su[2] = {1.0, 1.0};
+    double wall0 = 1.0, wall1 = 1.0, wall2 = 1.0;
+    bool wallprob = false;
+    if (hR <= drytol) {
+        hR = 0.0; pR = 0.0; mR = mL; chiR = chiL;
+        oh[1] = -1; op[1] = -1; om[1] = om[0]; ochi[1] = ochi[0];
+        const dclaw::RType rt = dclaw::riemanntype(hL, hL, uL, -uL, drytol, grav);
+        const double hstartest = dclaw::dmax(hL, rt.hm);
+        if (hstartest + bL < bR) {
+            wall1 = 0.0; wall2 = 0.0; wallprob = true;
+            hR = hL; huR = -huL; hvR = hvL; hmR = hmL; bR = bL;
+            uR = -uL; vR = vL; mR = mL; pR = pL; chiR = chiL; svrR = svrL; svtR = svtL;
+            oh[1] = oh[0]; ohu[1] = ohu[0]; shu[1] = -shu[0]; ohv[1] = ohv[0]; ohm[1] = ohm[0];
+            ob[1] = ob[0]; ou[1] = ou[0]; su[1] = -su[0]; ov[1] = ov[0]; om[1] = om[0];
+            op[1] = op[0]; ochi[1] = ochi[0]; osr[1] = osr[0]; ost[1] = ost[0];
+        }
+    } else if (hL <= drytol) {
+        hL = 0.0; pL = 0.0; mL = mR; chiL = chiR;
+        oh[0] = -1; op[0] = -1; om[0] = om[1]; ochi[0] = ochi[1];
+        const dclaw::RType rt = dclaw::riemanntype(hR, hR, -uR, uR, drytol, grav);
+        const double hstartest = dclaw::dmax(hR, rt.hm);
+        if (hstartest + bR < bL) {
+            wall0 = 0.0; wall1 = 0.0; wallprob = true;
+            hL = hR; huL = -huR; hvL = hvR; hmL = hmR; mL = mR; bL = bR;
+            uL = -uR; vL = vR; pL = pR; chiL = chiR; svrL = svrR; svtL = svtR;
+            oh[0] = oh[1]; ohu[0] = ohu[1]; shu[0] = -shu[1]; ohv[0] = ohv[1]; ohm[0] = ohm[1];
+            om[0] = om[1]; ob[0] = ob[1]; ou[0] = ou[1]; su[0] = -su[1]; ov[0] = ov[1];
+            op[0] = op[1]; ochi[0] = ochi[1]; osr[0] = osr[1]; ost[0] = ost[1];
+        }
+    }
+
+    // ---- forward replay of the solver
+    const double rho_f = P.rho_f, kappa = P.kappa;
+    const double criticaltol = dclaw::dmax(drytol * grav, 1e-6);
+    const double criticaltol_2 = sqrt(criticaltol);
+    const double gz = grav * cos(0.0);
+    const double alpha_seg = 1.0 - P.beta_seg;
+    int rtcase;
+    double rhoL, rhoR, tauLs, tauRs, h;
+    if (hL >= drytol && hR >= drytol) {
+        rtcase = 0; rhoL = svrL; tauLs = svtL; rhoR = svrR; tauRs = svtR; h = 0.5 * (hL + hR);
+    } else if (hL >= drytol) {
+        rtcase = 1; rhoL = svrL; tauLs = svtL; rhoR = svrL; tauRs = 0.5 * tauLs; h = 0.5 * hL;
+    } else {
+        rtcase = 2; rhoL = svrR; rhoR = svrR; tauRs = svtR; tauLs = 0.5 * tauRs; h = 0.5 * hR;
+    }
+    const double tauL = fsL * tauLs, tauR = fsR * tauRs;
+    const double rho = 0.5 * (rhoL + rhoR);
+    const bool m1 = P.src2method == -1;
+    double gamma, gammaL, gammaR;
+    if (m1) {
+        gamma = rho_f / rho; gammaL = rho_f / rhoL; gammaR = rho_f / rhoR;
+    } else {
+        gamma = 0.25 * (rho_f + 3.0 * rho) / rho;
+        gammaL = 0.25 * (rho_f + 3.0 * rhoL) / rhoL;
+        gammaR = 0.25 * (rho_f + 3.0 * rhoR) / rhoR;
+    }
+    const double eps = kappa + (1.0 - kappa) * gamma;
+    const double geps = gz * eps;
+    const double sqhL = sqrt(hL), sqhR = sqrt(hR);
+    const double gsL = sqrt(geps * hL), gsR = sqrt(geps * hR);
+    const double sL = uL - gsL, sR = uR + gsR;
+    const double uhat = (sqhL * uL + sqhR * uR) / (sqhR + sqhL);
+    const double chat = sqrt(geps * 0.5 * (hR + hL));
+    const double sRoe1 = uhat - chat, sRoe2 = uhat + chat;
+    const double sE1 = dclaw::dmin(sL, sRoe1), sE2 = dclaw::dmax(sR, sRoe2);
+    const double u = uhat;
+    const dclaw::RType rt = dclaw::riemanntype(hL, hR, uL, uR, drytol, geps);
+    const double sw0 = dclaw::dmin(sE1, rt.s2m), sw2 = dclaw::dmax(sE2, rt.s1m);
+    double sw1 = 0.5 * (sw2 + sw0);
+    int sw1src = 0;
+    const double hsQ = (huL - huR + sE2 * hR - sE1 * hL) / (sE2 - sE1);
+    const double hstarHLL = dclaw::dmax(hsQ, 0.0);
+    bool rarecorrector = false;
+    if (P.rarecorrectortest) {
+        const double sdelta = sw2 - sw0;
+        double raremin = 0.5;
+        const double raremax = 0.9;
+        if (rt.rare1 && sE1 * rt.s1m < 0.0) raremin = 0.2;
+        if (rt.rare2 && sE2 * rt.s2m < 0.0) raremin = 0.2;
+        if (rt.rare1 || rt.rare2) {
+            const double rare1st = 3.0 * (sqrt(geps * hL) - sqrt(geps * rt.hm));
+            const double rare2st = 3.0 * (sqrt(geps * hR) - sqrt(geps * rt.hm));
+            if (dclaw::dmax(rare1st, rare2st) > raremin * sdelta &&
+                dclaw::dmax(rare1st, rare2st) < raremax * sdelta) {
+                rarecorrector = true;
+                if (rare1st > rare2st) { sw1 = rt.s1m; sw1src = 1; }
+                else if (rare2st > rare1st) { sw1 = rt.s2m; sw1src = 2; }
+                else { sw1 = 0.5 * (rt.s1m + rt.s2m); sw1src = 3; }
+            }
+        }
+        if (hstarHLL < dclaw::dmin(hL, hR) / 5.0) rarecorrector = false;
+    }
+    const double delb = bR - bL;
+    const double hbar = 0.5 * (hL + hR);
+    const double tuu = uL + uR;
+    const double s1s2bar = 0.25 * (tuu * tuu) - gz * hbar;
+    const double uLuR = uL * uR;
+    const double s1s2tilde = dclaw::dmax(0.0, uLuR) - gz * hbar;
+    bool sonic = false;
+    if (fabs(s1s2bar) <= criticaltol) sonic = true;
+    else if (s1s2bar * s1s2tilde <= criticaltol * criticaltol) sonic = true;
+    else if (s1s2bar * sE1 * sE2 <= criticaltol * criticaltol) sonic = true;
+    else if (dclaw::dmin(fabs(sE1), fabs(sE2)) < criticaltol_2) sonic = true;
+    else if (sE1 < criticaltol_2 && rt.s1m > -criticaltol_2) sonic = true;
+    else if (sE2 > -criticaltol_2 && rt.s2m < criticaltol_2) sonic = true;
+    else if ((uL + sqrt(geps * hL)) * (uR + sqrt(geps * hR)) < 0.0) sonic = true;
+    else if ((uL - sqrt(geps * hL)) * (uR - sqrt(geps * hR)) < 0.0) sonic = true;
+
+    // source2dx: which expression it ends as (0/4: -(gz hbar delb), 1: the
+    // non-sonic ratio, 2: gz max(-hL delb, -hR delb), 3: gz min(...))
+    int s2src;
+    double source2dx;
+    if (sonic) { source2dx = -(gz * hbar * delb); s2src = 0; }
+    else { source2dx = -(gz * hbar * delb * s1s2tilde / s1s2bar); s2src = 1; }
+    const double mLd = -(hL * delb), mRd = -(hR * delb);
+    const double cl1 = gz * dclaw::dmax(mLd, mRd);
+    if (!(source2dx < cl1)) { source2dx = cl1; s2src = 2; }
+    const double cl2 = gz * dclaw::dmin(mLd, mRd);
+    if (!(source2dx > cl2)) { source2dx = cl2; s2src = 3; }
+    if (fabs(u) <= 0.0) { source2dx = -(hbar * gz * delb); s2src = 4; }
+
+    // deldelh: 0 -delb, 1 the non-sonic ratio, 2 hstarHLL dE / sE2,
+    // 3 hstarHLL dE / sE1, 4 -hL, 5 hR
+    int ddsrc;
+    double deldelh;
+    if (sonic) { deldelh = -delb; ddsrc = 0; }
+    else { deldelh = delb * gz * hbar / s1s2bar; ddsrc = 1; }
+    const double dE = sE2 - sE1;
+    if (sE1 < -criticaltol && sE2 > criticaltol) {
+        const double a1 = hstarHLL * dE / sE2;
+        if (!(deldelh < a1)) { deldelh = a1; ddsrc = 2; }
+        const double a2 = hstarHLL * dE / sE1;
+        if (!(deldelh > a2)) { deldelh = a2; ddsrc = 3; }
+    } else if (sE1 >= criticaltol) {
+        const double a1 = hstarHLL * dE / sE1;
+        if (!(deldelh < a1)) { deldelh = a1; ddsrc = 3; }
+        if (!(deldelh > -hL)) { deldelh = -hL; ddsrc = 4; }
+    } else if (sE2 <= -criticaltol) {
+        if (!(deldelh < hR)) { deldelh = hR; ddsrc = 5; }
+        const double a2 = hstarHLL * dE / sE2;
+        if (!(deldelh > a2)) { deldelh = a2; ddsrc = 2; }
+    }
+
+    double del0 = hR - hL - deldelh;
+    double del1 = huR - huL;
+    double del2 = hR * (uR * uR) + 0.5 * kappa * gz * (hR * hR)
+                  - (hL * (uL * uL) + 0.5 * kappa * gz * (hL * hL));
+    del2 = del2 + (1.0 - kappa) * h * (pR - pL) / rho;
+    const double del2pre = del2;
+    const double vnorm = sqrt(uR * uR + uL * uL + vR * vR + vL * vL);
+    const double T1 = taudirR * tauR / rhoR, T2 = tauL * taudirR / rhoL;
+    bool is_static = false;
+    int tscase;  // 0: tausource constant, 1: del2 - source2dx, 2: dsign(...)
+    double ts0 = 0.0;
+    if (vnorm > 0.0) {
+        tscase = 0;
+    } else if (0.5 * fabs(T1 + T2) > fabs(del2 - source2dx)) {
+        tscase = 1;
+        del1 = 0.0;
+        del0 = 0.0;
+        is_static = true;
+    } else {
+        tscase = 2;
+        ts0 = 0.5 * (T1 + T2);
+    }
+    if (wallprob) tscase = 0;
+
+    const double a = sw0, b = sw1, c = sw2;
+    double beta1, beta2, beta3, R12, R22;
+    if (!rarecorrector) {
+        beta1 = (c * del0 - del1) / (c - a);
+        beta2 = 0.0;  // its value is not needed
+        beta3 = (del1 - a * del0) / (c - a);
+        R12 = 0.0;
+        R22 = 1.0;
+    } else {
+        beta1 = 0.0;
+        beta2 = 0.0;
+        beta3 = 0.0;
+        R12 = sw1;
+        R22 = sw1 * sw1;
+    }
+    double D1 = 0.0, D2 = 0.0, D3 = 0.0;
+    if (rarecorrector) {
+        // del2 after the tausource subtraction
+        double tausource = 0.0;
+        if (tscase == 1) tausource = del2pre - source2dx;
+        else if (tscase == 2) tausource = dclaw::dsign(ts0, del2pre - source2dx);
+        const double d2 = del2pre - source2dx - tausource;
+        D1 = a * a - a * b - a * c + b * c;
+        D2 = a * b - a * c - b * b + b * c;
+        D3 = a * b - a * c - b * c + c * c;
+        beta1 = (b * c * del0 - (b + c) * del1 + d2) / D1;
+        beta2 = (-(a * c * del0) + (a + c) * del1 - d2) / D2;
+        beta3 = (a * b * del0 - (a + b) * del1 + d2) / D3;
+    } else {
+        double tausource = 0.0;
+        if (tscase == 1) tausource = del2pre - source2dx;
+        else if (tscase == 2) tausource = dclaw::dsign(ts0, del2pre - source2dx);
+        const double d2 = del2pre - source2dx - tausource;
+        beta2 = a * c * del0 - (a + c) * del1 + d2;
+    }
+    const double f11 = beta1 * a, f13 = beta3 * c;
+    const double AL = 1.0 + (1.0 - alpha_seg) * (1.0 - chiL);
+    const double AR = 1.0 + (1.0 - alpha_seg) * (1.0 - chiR);
+    const double hustar = huL + f11;
+    const double u_i = is_static ? 0.0 : hustar / hstarHLL;
+    const double K = gamma * rho * gz;
+
+    // ---- reverse sweep
+    double hLb = 0, hRb = 0, huLb = 0, huRb = 0, hvLb = 0, hvRb = 0, hmLb = 0, hmRb = 0;
+    double pLb = 0, pRb = 0, bLb = 0, bRb = 0, uLb = 0, uRb = 0, vLb = 0, vRb = 0;
+    double mLb = 0, mRb = 0, chiLb = 0, chiRb = 0, svrLb = 0, svrRb = 0, svtLb = 0, svtRb = 0;
+    double fsLb = 0, fsRb = 0, taudirRb = 0;
+    double gravb = 0, rhofb = 0, kappab = 0, betasegb = 0;
+    double gzb = 0, gammab = 0, gammaLb = 0, gammaRb = 0, rhob = 0, rhoLb = 0, rhoRb = 0;
+    double tauLb = 0, tauRb = 0, hb = 0, hstarb = 0, sE1b = 0, sE2b = 0;
+    // outputs (the wall factors are constants)
+    const double* lw1 = ly + 3;
+    const double* lw5 = ly + 9;
+    double sw0b = ly[0] * wall0, sw1b = ly[1] * wall1, sw2b = ly[2] * wall2;
+    double f11b = lw1[0] * wall0;
+    const double f21b = (xd ? lw1[1] : lw1[2]) * wall0;
+    const double f31Fb = (xd ? lw1[2] : lw1[1]) * wall0;
+    const double f41Fb = lw1[3] * wall0, f51Fb = lw1[4] * wall0, f61Fb = lw1[5] * wall0;
+    double f13b = lw5[0] * wall2;
+    const double f23b = (xd ? lw5[1] : lw5[2]) * wall2;
+    const double f33Fb = (xd ? lw5[2] : lw5[1]) * wall2;
+    const double f43Fb = lw5[3] * wall2, f53Fb = lw5[4] * wall2, f63Fb = lw5[5] * wall2;
+    const double f12b = ly[15] * wall1, f22b = ly[16] * wall1;
+    // delta_huvm_revert's fold of the middle wave into wave 1 or 3
+    double f31b = f31Fb, f33b = f33Fb, f41b = f41Fb, f43b = f43Fb;
+    double f51b = f51Fb, f53b = f53Fb, f61b = f61Fb, f63b = f63Fb;
+    const bool left = hustar <= 0.0;
+    const double f32b = left ? f31Fb : f33Fb, f42b = left ? f41Fb : f43Fb;
+    const double f52b = left ? f51Fb : f53Fb, f62b = left ? f61Fb : f63Fb;
+    double del0b = 0, del1b = 0, del2b = 0;
+    // f52 = del4 - f53 - f51, del4 = 0 (static) or recomputed with u_interface
+    f53b -= f52b;
+    f51b -= f52b;
+    if (!is_static) {
+        const double dhRL = hR - hL, dp = pR - pL;
+        const double Kb = f52b * (-(u_i * dhRL) + del1);
+        const double uib = f52b * (-(K * dhRL) + dp);
+        hRb -= f52b * K * u_i;
+        hLb += f52b * K * u_i;
+        del1b += f52b * K;
+        pRb += f52b * u_i;
+        pLb -= f52b * u_i;
+        gammab += Kb * rho * gz;
+        rhob += Kb * gamma * gz;
+        gzb += Kb * gamma * rho;
+        const double hsb = uib / hstarHLL;
+        hstarb -= uib * u_i / hstarHLL;
+        huLb += hsb;
+        f11b += hsb;
+    }
+    // f62 = seg_R - seg_L - f61 - f63; f42, f32 likewise
+    const double segRb = f62b, segLb = -f62b;
+    f61b -= f62b;
+    f63b -= f62b;
+    hmRb += f42b * uR; uRb += f42b * hmR; hmLb -= f42b * uL; uLb -= f42b * hmL;
+    f41b -= f42b;
+    f43b -= f42b;
+    hvRb += f32b * uR; uRb += f32b * hvR; hvLb -= f32b * uL; uLb -= f32b * hvL;
+    f31b -= f32b;
+    f33b -= f32b;
+    double ALb = 0.0, ARb = 0.0;
+    f11b += f61b * chiL * AL; chiLb += f61b * f11 * AL; ALb += f61b * f11 * chiL;
+    f13b += f63b * chiR * AR; chiRb += f63b * f13 * AR; ARb += f63b * f13 * chiR;
+    chiLb += segLb * hL * uL * AL; hLb += segLb * chiL * uL * AL;
+    uLb += segLb * chiL * hL * AL; ALb += segLb * chiL * hL * uL;
+    chiRb += segRb * hR * uR * AR; hRb += segRb * chiR * uR * AR;
+    uRb += segRb * chiR * hR * AR; ARb += segRb * chiR * hR * uR;
+    betasegb += ALb * (1.0 - chiL) + ARb * (1.0 - chiR);  // alpha_seg = 1 - beta_seg
+    chiLb -= ALb * (1.0 - alpha_seg);
+    chiRb -= ARb * (1.0 - alpha_seg);
+    // f51 = f11 gammaL rhoL grav, f53 = f13 gammaR rhoR grav
+    f11b += f51b * gammaL * rhoL * grav; gammaLb += f51b * f11 * rhoL * grav;
+    rhoLb += f51b * f11 * gammaL * grav; gravb += f51b * f11 * gammaL * rhoL;
+    f13b += f53b * gammaR * rhoR * grav; gammaRb += f53b * f13 * rhoR * grav;
+    rhoRb += f53b * f13 * gammaR * grav; gravb += f53b * f13 * gammaR * rhoR;
+    // f41 = f11 mL, f31 = f11 vL, f43 = f13 mR, f33 = f13 vR
+    f11b += f41b * mL + f31b * vL; mLb += f41b * f11; vLb += f31b * f11;
+    f13b += f43b * mR + f33b * vR; mRb += f43b * f13; vRb += f33b * f13;
+    // f11 = beta1 a, f21 = beta1 a^2, f12 = beta2 R12, f22 = beta2 R22,
+    // f13 = beta3 c, f23 = beta3 c^2
+    const double beta1b = f11b * a + f21b * (a * a);
+    double ab = f11b * beta1 + f21b * beta1 * 2.0 * a;
+    const double beta2b = f12b * R12 + f22b * R22;
+    const double R12b = f12b * beta2, R22b = f22b * beta2;
+    const double beta3b = f13b * c + f23b * (c * c);
+    double cb = f13b * beta3 + f23b * beta3 * 2.0 * c;
+    double bb = 0.0;
+    if (!rarecorrector) {
+        const double D = c - a;
+        const double N1b = beta1b / D, N3b = beta3b / D;
+        const double Db = -(beta1b * beta1) / D - (beta3b * beta3) / D;
+        cb += N1b * del0; del0b += N1b * c; del1b -= N1b;
+        ab += beta2b * (c * del0 - del1); cb += beta2b * (a * del0 - del1);
+        del0b += beta2b * a * c; del1b -= beta2b * (a + c); del2b += beta2b;
+        del1b += N3b; ab -= N3b * del0; del0b -= N3b * a;
+        cb += Db;
+        ab -= Db;
+    } else {
+        const double N1b = beta1b / D1, D1b = -(beta1b * beta1) / D1;
+        bb += N1b * (c * del0 - del1); cb += N1b * (b * del0 - del1);
+        del0b += N1b * b * c; del1b -= N1b * (b + c); del2b += N1b;
+        ab += D1b * (2.0 * a - b - c); bb += D1b * (c - a); cb += D1b * (b - a);
+        const double N2b = beta2b / D2, D2b = -(beta2b * beta2) / D2;
+        ab += N2b * (del1 - c * del0); cb += N2b * (del1 - a * del0);
+        del0b -= N2b * a * c; del1b += N2b * (a + c); del2b -= N2b;
+        ab += D2b * (b - c); bb += D2b * (a - 2.0 * b + c); cb += D2b * (b - a);
+        const double N3b = beta3b / D3, D3b = -(beta3b * beta3) / D3;
+        ab += N3b * (b * del0 - del1); bb += N3b * (a * del0 - del1);
+        del0b += N3b * a * b; del1b -= N3b * (a + b); del2b += N3b;
+        ab += D3b * (b - c); bb += D3b * (a - c); cb += D3b * (2.0 * c - a - b);
+        bb += R12b + R22b * 2.0 * b;
+    }
+    if (is_static) {
+        // del0 and del1 were replaced by constants
+        del0b = 0.0;
+        del1b = 0.0;
+    }
+    sw0b += ab;
+    sw1b += bb;
+    sw2b += cb;
+    // del2 = del2pre - source2dx - tausource
+    double del2preb = del2b, s2dxb = -del2b;
+    const double tsb = -del2b;
+    if (tscase == 1) {
+        del2preb += tsb;
+        s2dxb -= tsb;
+    } else if (tscase == 2) {
+        const double sgn = (signbit(ts0) ? -1.0 : 1.0) * (signbit(del2pre - source2dx) ? -1.0 : 1.0);
+        const double T12b = 0.5 * tsb * sgn;
+        taudirRb += T12b * tauR / rhoR + T12b * tauL / rhoL;
+        tauRb += T12b * taudirR / rhoR;
+        rhoRb -= T12b * T1 / rhoR;
+        tauLb += T12b * taudirR / rhoL;
+        rhoLb -= T12b * T2 / rhoL;
+    }
+    // del2pre
+    {
+        const double KG = 0.5 * kappa * gz;
+        hRb += del2preb * (uR * uR + KG * 2.0 * hR);
+        uRb += del2preb * hR * 2.0 * uR;
+        hLb -= del2preb * (uL * uL + KG * 2.0 * hL);
+        uLb -= del2preb * hL * 2.0 * uL;
+        const double KGb = del2preb * (hR * hR - hL * hL);
+        kappab += KGb * 0.5 * gz;
+        gzb += KGb * 0.5 * kappa;
+        const double dp = pR - pL;
+        const double E = (1.0 - kappa) * h * dp / rho;
+        kappab -= del2preb * h * dp / rho;
+        hb += del2preb * (1.0 - kappa) * dp / rho;
+        pRb += del2preb * (1.0 - kappa) * h / rho;
+        pLb -= del2preb * (1.0 - kappa) * h / rho;
+        rhob -= del2preb * E / rho;
+    }
+    huRb += del1b;
+    huLb -= del1b;
+    hRb += del0b;
+    hLb -= del0b;
+    const double deldelhb = -del0b;
+    double delbb = 0.0, hbarb = 0.0, s1s2barb = 0.0, s1s2tildeb = 0.0;
+    switch (ddsrc) {
+    case 0: delbb -= deldelhb; break;
+    case 1: {
+        const double v = delb * gz * hbar / s1s2bar;
+        delbb += deldelhb * gz * hbar / s1s2bar;
+        gzb += deldelhb * delb * hbar / s1s2bar;
+        hbarb += deldelhb * delb * gz / s1s2bar;
+        s1s2barb -= deldelhb * v / s1s2bar;
+        break;
+    }
+    case 2: {
+        const double v = hstarHLL * dE / sE2, dEb = deldelhb * hstarHLL / sE2;
+        hstarb += deldelhb * dE / sE2;
+        sE2b += dEb - deldelhb * v / sE2;
+        sE1b -= dEb;
+        break;
+    }
+    case 3: {
+        const double v = hstarHLL * dE / sE1, dEb = deldelhb * hstarHLL / sE1;
+        hstarb += deldelhb * dE / sE1;
+        sE2b += dEb;
+        sE1b += -dEb - deldelhb * v / sE1;
+        break;
+    }
+    case 4: hLb -= deldelhb; break;
+    default: hRb += deldelhb; break;
+    }
+    switch (s2src) {
+    case 1: {
+        const double r = s1s2tilde / s1s2bar;
+        const double w = gz * hbar * delb * s1s2tilde / s1s2bar;
+        gzb -= s2dxb * hbar * delb * r;
+        hbarb -= s2dxb * gz * delb * r;
+        delbb -= s2dxb * gz * hbar * r;
+        s1s2tildeb -= s2dxb * gz * hbar * delb / s1s2bar;
+        s1s2barb += s2dxb * w / s1s2bar;
+        break;
+    }
+    case 2:
+    case 3: {
+        const bool lsel = s2src == 2 ? (mLd > mRd) : (mLd < mRd);
+        const double mv = lsel ? mLd : mRd;
+        gzb += s2dxb * mv;
+        const double mb = s2dxb * gz;
+        if (lsel) { hLb -= mb * delb; delbb -= mb * hL; }
+        else { hRb -= mb * delb; delbb -= mb * hR; }
+        break;
+    }
+    default:
+        gzb -= s2dxb * hbar * delb;
+        hbarb -= s2dxb * gz * delb;
+        delbb -= s2dxb * gz * hbar;
+        break;
+    }
+    // s1s2bar = tuu^2 / 4 - gz hbar, s1s2tilde = max(0, uL uR) - gz hbar
+    const double tuub = s1s2barb * 0.5 * tuu;
+    gzb -= (s1s2barb + s1s2tildeb) * hbar;
+    hbarb -= (s1s2barb + s1s2tildeb) * gz;
+    if (!(0.0 > uLuR)) {
+        uLb += s1s2tildeb * uR;
+        uRb += s1s2tildeb * uL;
+    }
+    uLb += tuub;
+    uRb += tuub;
+    hLb += 0.5 * hbarb;
+    hRb += 0.5 * hbarb;
+    bRb += delbb;
+    bLb -= delbb;
+    // sw1 and the speeds
+    double s1mb = 0.0, s2mb = 0.0;
+    switch (sw1src) {
+    case 0: sw2b += 0.5 * sw1b; sw0b += 0.5 * sw1b; break;
+    case 1: s1mb += sw1b; break;
+    case 2: s2mb += sw1b; break;
+    default: s1mb += 0.5 * sw1b; s2mb += 0.5 * sw1b; break;
+    }
+    if (sE1 < rt.s2m) sE1b += sw0b;
+    else s2mb += sw0b;
+    if (sE2 > rt.s1m) sE2b += sw2b;
+    else s1mb += sw2b;
+    if (hsQ > 0.0) {
+        const double Dq = sE2 - sE1;
+        const double Nb = hstarb / Dq, Dqb = -(hstarb * hsQ) / Dq;
+        huLb += Nb;
+        huRb -= Nb;
+        sE2b += Nb * hR + Dqb;
+        hRb += Nb * sE2;
+        sE1b -= Nb * hL + Dqb;
+        hLb -= Nb * sE1;
+    }
+    double gepsb = 0.0;
+    riemanntype_vjp(hL, hR, uL, uR, drytol, geps, s1mb, s2mb, hLb, hRb, uLb, uRb, gepsb);
+    double sLb = 0.0, sRb = 0.0, uhatb = 0.0, chatb = 0.0;
+    if (sL < sRoe1) sLb += sE1b;
+    else { uhatb += sE1b; chatb -= sE1b; }
+    if (sR > sRoe2) sRb += sE2b;
+    else { uhatb += sE2b; chatb += sE2b; }
+    uLb += sLb;
+    uRb += sRb;
+    {
+        const double tl = sqrt_adj(gsL, -sLb), tr = sqrt_adj(gsR, sRb);
+        gepsb += tl * hL + tr * hR;
+        hLb += tl * geps;
+        hRb += tr * geps;
+        const double tc = sqrt_adj(chat, chatb);
+        gepsb += tc * 0.5 * (hR + hL);
+        hRb += tc * geps * 0.5;
+        hLb += tc * geps * 0.5;
+    }
+    {
+        const double Du = sqhR + sqhL;
+        const double Nb = uhatb / Du, Dub = -(uhatb * uhat) / Du;
+        uLb += Nb * sqhL;
+        uRb += Nb * sqhR;
+        hLb += sqrt_adj(sqhL, Nb * uL + Dub);
+        hRb += sqrt_adj(sqhR, Nb * uR + Dub);
+    }
+    // geps = gz eps, eps = kappa + (1 - kappa) gamma
+    gzb += gepsb * eps;
+    const double epsb = gepsb * gz;
+    kappab += epsb * (1.0 - gamma);
+    gammab += epsb * (1.0 - kappa);
+    if (m1) {
+        rhofb += gammab / rho + gammaLb / rhoL + gammaRb / rhoR;
+        rhob -= gammab * gamma / rho;
+        rhoLb -= gammaLb * gammaL / rhoL;
+        rhoRb -= gammaRb * gammaR / rhoR;
+    } else {
+        rhofb += 0.25 * (gammab / rho + gammaLb / rhoL + gammaRb / rhoR);
+        rhob += gammab * (0.75 - gamma) / rho;
+        rhoLb += gammaLb * (0.75 - gammaL) / rhoL;
+        rhoRb += gammaRb * (0.75 - gammaR) / rhoR;
+    }
+    rhoLb += 0.5 * rhob;
+    rhoRb += 0.5 * rhob;
+    fsLb += tauLb * tauLs;
+    const double tauLsb = tauLb * fsL;
+    fsRb += tauRb * tauRs;
+    const double tauRsb = tauRb * fsR;
+    gravb += gzb;
+    switch (rtcase) {
+    case 0:
+        svrLb += rhoLb; svtLb += tauLsb; svrRb += rhoRb; svtRb += tauRsb;
+        hLb += 0.5 * hb; hRb += 0.5 * hb;
+        break;
+    case 1:
+        svrLb += rhoLb + rhoRb; svtLb += tauLsb + 0.5 * tauRsb; hLb += 0.5 * hb;
+        break;
+    default:
+        svrRb += rhoLb + rhoRb; svtRb += tauRsb + 0.5 * tauLsb; hRb += 0.5 * hb;
+        break;
+    }
+    // back to the side inputs
+    auto put = [&](int o, double s, double v) {
+        if (o >= 0) lx[o] += s * v;
+    };
+    put(oh[0], 1.0, hLb); put(oh[1], 1.0, hRb);
+    put(ohu[0], shu[0], huLb); put(ohu[1], shu[1], huRb);
+    put(ohv[0], 1.0, hvLb); put(ohv[1], 1.0, hvRb);
+    put(ohm[0], 1.0, hmLb); put(ohm[1], 1.0, hmRb);
+    put(op[0], 1.0, pLb); put(op[1], 1.0, pRb);
+    put(ob[0], -1.0, bLb); put(ob[1], -1.0, bRb);
+    put(ou[0], su[0], uLb); put(ou[1], su[1], uRb);
+    put(ov[0], 1.0, vLb); put(ov[1], 1.0, vRb);
+    put(om[0], 1.0, mLb); put(om[1], 1.0, mRb);
+    put(ochi[0], 1.0, chiLb); put(ochi[1], 1.0, chiRb);
+    put(osr[0], 1.0, svrLb); put(osr[1], 1.0, svrRb);
+    put(ost[0], 1.0, svtLb); put(ost[1], 1.0, svtRb);
+    lx[SD_FS] += fsLb;
+    lx[NSD + SD_FS] += fsRb;
+    lx[NSD + SD_TAUDIR] += taudirRb;
+    for (int p = 0; p < A.np; p++)
+        gp[p] += gravb * A.dP[p].grav + rhofb * A.dP[p].rho_f + kappab * A.dP[p].kappa +
+                 betasegb * A.dP[p].beta_seg;
+}
+
+// B4: rpn2 of the x- and y-edge at (i, j): the hand-written product for
+// dry_tolerance > 0, the seeded one otherwise (a reset dry side then reaches
+// setvars).
 DC_HD void rev_waves(const AdjArgs& A, const TP& Pz, int i, int j)
 {
     const size_t n = cix(A, i, j);
@@ -1034,14 +1814,11 @@ DC_HD void rev_waves(const AdjArgs& A, const TP& Pz, int i, int j)
     for (int ixy = 1; ixy <= 2; ixy++) {
         double lx[2 * NSD];
         for (int k = 0; k < 2 * NSD; k++) lx[k] = 0.0;
-        if (ixy == 1 ? xedge(A, i, j) : yedge(A, i, j)) {
-            double x[2 * NSD], ly[NW];
-            Rpn2F f;
-            rpn2_inputs(A, ixy, i, j, x, f);
-            lam_waves(A, ixy, i, j, ly);
-            const double drytol = A.P.dry_tolerance;
-            // both sides dry: rpn2 returns constant zero waves
-            if (x[SD_H] > drytol || x[NSD + SD_H] > drytol) vjp<2 * NSD, NW>(A, f, Pz, x, ly, lx, g);
+        double x[2 * NSD], ly[NW];
+        Rpn2F f;
+        if (wave_edge(A, ixy, i, j, x, f, ly)) {
+            if (A.P.dry_tolerance > 0.0) rpn2_vjp(A, f, x, ly, lx, g);
+            else vjp<2 * NSD, NW>(A, f, Pz, x, ly, lx, g);
         }
         const int f0 = ixy == 1 ? W_LXS : W_LYS;
         for (int k = 0; k < 2 * NSD; k++) ws(A, f0 + k, n, lx[k]);

@@ -656,3 +656,167 @@ extern "C" double emu_split_vjp_check(const dclaw_params* P, int n, unsigned see
     }
     return worst;
 }
+
+// The hand-written rpn2 adjoint (rpn2_vjp) against the seeded tangent-linear
+// vector-Jacobian product (30 side inputs and the grav, rho_f, kappa,
+// beta_seg directions) on n random edges of 8 kinds, c % 8: both sides wet
+// at moderate (0) and large (1) speeds, a dry right side below (2) and above
+// (3, the wall) the left surface, the same on the left (4, 5), both wet at
+// rest (6: the static / tausource branches) and near-critical speeds (7).
+// Returns the largest difference relative to the largest input adjoint
+// (parameters: as sensitivities |dL/dp| |p| against the largest of the
+// four); worst[k] gets the same per kind when not null.
+extern "C" double emu_rpn2_vjp_check(const dclaw_params* P, int n, unsigned seed, double* worst8)
+{
+    namespace D = dclaw_adj;
+    unsigned long long st = seed * 2862933555777941757ull + 3037000493ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)((st >> 11) & ((1ull << 53) - 1)) / (double)(1ull << 53) * 2.0 - 1.0;
+    };
+    dclaw_params dP[4] = {};
+    dP[0].grav = 1.0;
+    dP[1].rho_f = 1.0;
+    dP[2].kappa = 1.0;
+    dP[3].beta_seg = 1.0;
+    D::AdjArgs A{};
+    A.P = *P;
+    A.dP = dP;
+    A.np = 4;
+    const D::TP Pz = D::zero_params(*P);
+    const double g = P->grav;
+    if (worst8)
+        for (int k = 0; k < 8; k++) worst8[k] = 0.0;
+    double worst = 0.0;
+    for (int c = 0; c < n; c++) {
+        const int kind = c % 8;
+        D::Rpn2F f;
+        f.ixy = 1 + ((c / 8) & 1);
+        const bool xd = f.ixy == 1;
+        double x[2 * D::NSD], ly[D::NW], la[2 * D::NSD], lb[2 * D::NSD];
+        const double bedc = 2.0 * rnd();
+        for (int s = 0; s < 2; s++) {
+            double* X = x + s * D::NSD;
+            bool dry = (s == 1 && (kind == 2 || kind == 3)) || (s == 0 && (kind == 4 || kind == 5));
+            double h = 0.05 + 1.5 * (rnd() + 1.0);
+            const double cs = sqrt(g * h);
+            double u = (kind == 1 ? 4.0 : 0.8) * cs * rnd(), v = 0.5 * cs * rnd();
+            if (kind == 6) u = v = 0.0;
+            if (kind >= 2 && kind <= 5) {
+                // the wet side at rest, flowing away from the dry side, or either way
+                const int vk = (c / 16) % 4;
+                if (vk == 0) u = v = 0.0;
+                else if (vk == 1) u = (s == 0 ? -1.0 : 1.0) * 2.0 * cs * std::fabs(rnd());
+                else u = 2.0 * cs * rnd();
+            }
+            if (kind == 7) u = (s == 0 ? 1.0 : -1.0) * cs * (1.0 + 0.02 * rnd());
+            if (dry) {
+                h = (c % 3 == 0) ? 0.0 : 0.5 * P->dry_tolerance * (rnd() + 1.0);
+                u = v = 0.0;
+            }
+            const double m = dry ? 0.0 : 0.5 + 0.2 * rnd(), chi = dry ? 0.0 : 0.5 + 0.5 * rnd();
+            const double rho = m * P->rho_s + (1.0 - m) * P->rho_f;
+            X[D::SD_H] = h;
+            X[xd ? D::SD_HU : D::SD_HV] = h * u;
+            X[xd ? D::SD_HV : D::SD_HU] = h * v;
+            X[D::SD_HM] = h * m;
+            X[D::SD_P] = dry ? 0.0 : P->rho_f * g * h * (0.8 + 0.3 * rnd());
+            X[D::SD_HCHI] = h * chi;
+            X[D::SD_U] = u;
+            X[D::SD_V] = v;
+            X[D::SD_M] = m;
+            X[D::SD_CHI] = chi;
+            X[D::SD_BDIF] = 0.1 * (rnd() + 1.0);
+            X[D::SD_TAUDIR] = rnd();
+            X[D::SD_FS] = 0.5 * (rnd() + 1.0);
+            X[D::SD_RHO] = dry ? P->rho_f : rho;
+            X[D::SD_TAU] = dry ? 0.0 : 2e3 * h * (rnd() + 1.0);
+        }
+        // beds: the dry side's bed below the wet surface (2, 4) or well above it (3, 5)
+        f.bedL = bedc;
+        f.bedR = bedc + 0.3 * rnd();
+        const double hw = kind == 2 || kind == 3 ? x[D::SD_H] : x[D::NSD + D::SD_H];
+        if (kind == 3) f.bedR = bedc + x[D::SD_BDIF] - x[D::NSD + D::SD_BDIF] + hw + 1.0;
+        if (kind == 2) f.bedR = bedc - 1.0;
+        if (kind == 5) f.bedL = f.bedR + x[D::NSD + D::SD_BDIF] - x[D::SD_BDIF] + hw + 1.0;
+        if (kind == 4) f.bedL = f.bedR - 1.0;
+        for (int k = 0; k < D::NW; k++) ly[k] = rnd();
+        double ga[4] = {0, 0, 0, 0}, gb[4] = {0, 0, 0, 0};
+        D::vjp<2 * D::NSD, D::NW>(A, f, Pz, x, ly, la, ga);
+        D::rpn2_vjp(A, f, x, ly, lb, gb);
+        double scale = 0.0, diff = 0.0;
+        for (int k = 0; k < 2 * D::NSD; k++) {
+            scale = std::max(scale, std::fabs(la[k]));
+            diff = std::max(diff, std::fabs(la[k] - lb[k]));
+        }
+        double w = scale > 0.0 ? diff / scale : diff;
+        // (a parameter that is 0 is scaled as 1)
+        const double pm[4] = {std::max(std::fabs(P->grav), 1.0), std::max(std::fabs(P->rho_f), 1.0),
+                              std::max(std::fabs(P->kappa), 1.0),
+                              std::max(std::fabs(P->beta_seg), 1.0)};
+        double gs = 0.0;
+        for (int k = 0; k < 4; k++) gs = std::max(gs, std::fabs(ga[k] * pm[k]));
+        for (int k = 0; k < 4; k++) {
+            const double d = std::fabs((ga[k] - gb[k]) * pm[k]);
+            if (gs > 0.0) w = std::max(w, d / gs);
+            else w = std::max(w, d);
+        }
+        worst = std::max(worst, w);
+        if (worst8) worst8[kind] = std::max(worst8[kind], w);
+    }
+    return worst;
+}
+
+// The hand-written riemanntype adjoint (riemanntype_vjp: s1m, s2m with
+// respect to hL, hR, uL, uR, g) against the seeded tangent-linear
+// riemanntype on n random states; counts[4] gets how many fell in each
+// branch (a dry side, two rarefactions, two shocks, mixed).  Returns the
+// largest difference relative to the largest input adjoint.
+extern "C" double emu_riemanntype_vjp_check(double drytol, int n, unsigned seed, int* counts)
+{
+    namespace T = dclaw_tl;
+    unsigned long long st = seed * 2862933555777941757ull + 3037000493ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)((st >> 11) & ((1ull << 53) - 1)) / (double)(1ull << 53) * 2.0 - 1.0;
+    };
+    for (int k = 0; k < 4; k++) counts[k] = 0;
+    double worst = 0.0;
+    for (int c = 0; c < n; c++) {
+        double v[5];
+        v[0] = (c % 11 == 0) ? 0.5 * drytol * (rnd() + 1.0) : 0.02 + 2.0 * (rnd() + 1.0);
+        v[1] = (c % 13 == 0) ? 0.0 : 0.02 + 2.0 * (rnd() + 1.0);
+        const double g = 9.81 * (0.5 + 0.4 * rnd());
+        v[4] = g;
+        const double cs = sqrt(g * 0.5 * (v[0] + v[1]));
+        v[2] = 3.0 * cs * rnd();
+        v[3] = 3.0 * cs * rnd();
+        const double s1b = rnd(), s2b = rnd();
+        double la[5];
+        for (int k = 0; k < 5; k++) {
+            T::Dual d[5];
+            for (int a = 0; a < 5; a++) d[a] = T::Dual(v[a], a == k ? 1.0 : 0.0);
+            const T::RType r = T::riemanntype(d[0], d[1], d[2], d[3], T::Dual(drytol), d[4]);
+            la[k] = s1b * r.s1m.d + s2b * r.s2m.d;
+        }
+        double lb[5] = {0, 0, 0, 0, 0};
+        dclaw_adj::riemanntype_vjp(v[0], v[1], v[2], v[3], drytol, g, s1b, s2b, lb[0], lb[1], lb[2],
+                                   lb[3], lb[4]);
+        // branch of this state (the same tests as riemanntype)
+        const double hmin = std::min(v[0], v[1]), hmax = std::max(v[0], v[1]);
+        const double delu = v[3] - v[2];
+        int br;
+        if (hmin <= drytol) br = 0;
+        else if (delu + 2.0 * (sqrt(g * hmin) - sqrt(g * hmax)) > 0.0) br = 1;
+        else if (delu + (hmax - hmin) * (sqrt(.5 * g * (hmax + hmin) / (hmax * hmin))) <= 0.0) br = 2;
+        else br = 3;
+        counts[br]++;
+        double scale = 0.0, diff = 0.0;
+        for (int k = 0; k < 5; k++) {
+            scale = std::max(scale, std::fabs(la[k]));
+            diff = std::max(diff, std::fabs(la[k] - lb[k]));
+        }
+        worst = std::max(worst, scale > 0.0 ? diff / scale : diff);
+    }
+    return worst;
+}

@@ -204,3 +204,33 @@ def test_hand_split_adjoint_equals_seeded(emu_lib, src2method):
     emu_lib.emu_split_vjp_check.restype = ctypes.c_double
     err = emu_lib.emu_split_vjp_check(ctypes.byref(p), 2000, 5 + src2method)
     assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("rarecorrector", [0, 1])
+@pytest.mark.parametrize("src2method", [0, -1])
+def test_hand_rpn2_adjoint_equals_seeded(emu_lib, rarecorrector, src2method):
+    """The hand-written adjoint of rpn2 (the 30 side inputs and the grav,
+    rho_f, kappa, beta_seg directions) equals the seeded tangent-linear
+    vector-Jacobian product on random edges: both sides wet at moderate,
+    large, zero and near-critical speeds, a dry side below and above the wet
+    surface (inundation and the wall mirror) on either side, with the rare
+    corrector on and off."""
+    from dclaw_amd import workloads as W
+    p = to_c(W.config0_params())
+    p.rarecorrectortest = rarecorrector
+    p.src2method = src2method
+    worst = (ctypes.c_double * 8)()
+    emu_lib.emu_rpn2_vjp_check.restype = ctypes.c_double
+    err = emu_lib.emu_rpn2_vjp_check(ctypes.byref(p), 8000, 3 + rarecorrector, worst)
+    assert err <= 1e-12, list(worst)
+
+
+def test_hand_riemanntype_adjoint_equals_seeded(emu_lib):
+    """The hand-written adjoint of riemanntype's speeds equals the seeded
+    tangent-linear one in each of its four branches (a dry side, two
+    rarefactions, two shocks, mixed); every branch is visited."""
+    counts = (ctypes.c_int * 4)()
+    emu_lib.emu_riemanntype_vjp_check.restype = ctypes.c_double
+    err = emu_lib.emu_riemanntype_vjp_check(ctypes.c_double(1e-3), 20000, 7, counts)
+    assert min(counts) >= 1000, list(counts)
+    assert err <= 1e-12, err
