@@ -39,6 +39,7 @@
 #pragma once
 
 #include "dclaw_tangent.h"
+#include "dclaw_tangent_k.h"
 
 namespace dclaw_adj {
 
@@ -149,22 +150,37 @@ DC_HD TP zero_params(const dclaw_params& P)
 // ---------------------------------------------------------------- stages
 
 // F1: q of the cell (after the boundary map) -> prepared fields.
+// The record types of the namespace that evaluates a stage over R (dclaw_tl
+// for one tangent, dclaw_tlk for K).
+template <class R> struct PhysOf;
+template <> struct PhysOf<Dual> {
+    using RowCell = dclaw_tl::RowCell;
+    using Cell = dclaw_tl::Cell;
+    using RhoTau = dclaw_tl::RhoTau;
+};
+template <> struct PhysOf<dclaw_tlk::DualK> {
+    using RowCell = dclaw_tlk::RowCell;
+    using Cell = dclaw_tlk::Cell;
+    using RhoTau = dclaw_tlk::RhoTau;
+};
+
 struct PrepF {
     double bed;
-    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    template <class PP, class R>
+    DC_HD void operator()(const PP& P, const R* x, R* y) const
     {
-        dclaw_tl::RowCell r;
-        dclaw_tl::prepare_cell(P, x, Dual(bed), r);
+        typename PhysOf<R>::RowCell r;
+        prepare_cell(P, x, R(bed), r);
         y[PC_H] = r.h; y[PC_HU] = r.hu; y[PC_HV] = r.hv; y[PC_BDIF] = r.bdif;
         y[PC_HM0] = r.hm0; y[PC_P0] = r.p0; y[PC_HCHI0] = r.hchi0;
         y[PC_HM1] = r.hm1; y[PC_P1] = r.p1; y[PC_HCHI1] = r.hchi1; y[PC_M1] = r.m1;
         y[PC_CHI1] = r.chi1; y[PC_U] = r.u; y[PC_V] = r.v;
         y[PC_HM2] = r.hm2; y[PC_P2] = r.p2; y[PC_HCHI2] = r.hchi2; y[PC_M2] = r.m2;
         y[PC_CHI2] = r.chi2;
-        y[PC_RHO1] = y[PC_TAU1] = y[PC_RHO2] = y[PC_TAU2] = Dual(0.0);
+        y[PC_RHO1] = y[PC_TAU1] = y[PC_RHO2] = y[PC_TAU2] = R(0.0);
         if (r.h > P.dry_tolerance) {
-            const dclaw_tl::RhoTau v1 = dclaw_tl::rho_tau(P, r.h, r.u, r.v, r.m1, r.p1, r.chi1);
-            const dclaw_tl::RhoTau v2 = dclaw_tl::rho_tau(P, r.h, r.u, r.v, r.m2, r.p2, r.chi2);
+            const auto v1 = rho_tau(P, r.h, r.u, r.v, r.m1, r.p1, r.chi1);
+            const auto v2 = rho_tau(P, r.h, r.u, r.v, r.m2, r.p2, r.chi2);
             y[PC_RHO1] = v1.rho; y[PC_TAU1] = v1.tau;
             y[PC_RHO2] = v2.rho; y[PC_TAU2] = v2.tau;
         }
@@ -177,21 +193,21 @@ struct PrepF {
 struct TaudirF {
     double dx, dy;
     double b[5];
-    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    template <class PP, class R>
+    DC_HD void operator()(const PP& P, const R* x, R* y) const
     {
-        dclaw_tl::Cell c[5];
-        Dual bb[5];
+        typename PhysOf<R>::Cell c[5];
+        R bb[5];
         for (int s = 0; s < 5; s++) {
             c[s].h = x[7 * s + 0]; c[s].hu = x[7 * s + 1]; c[s].hv = x[7 * s + 2];
             c[s].hm = x[7 * s + 3]; c[s].p = x[7 * s + 4]; c[s].hchi = x[7 * s + 5];
-            bb[s] = Dual(b[s]) - x[7 * s + 6];
+            bb[s] = R(b[s]) - x[7 * s + 6];
         }
-        dclaw_tl::RhoTau sv;
+        typename PhysOf<R>::RhoTau sv;
         sv.rho = x[35];
         sv.tau = x[36];
-        const dclaw_tl::Taudir t = dclaw_tl::calc_taudir_cell(P, Dual(dx), Dual(dy), c[0], bb[0],
-                                                              c[1], bb[1], c[2], bb[2], c[3],
-                                                              bb[3], c[4], bb[4], sv);
+        const auto t = calc_taudir_cell(P, R(dx), R(dy), c[0], bb[0], c[1], bb[1], c[2], bb[2],
+                                        c[3], bb[3], c[4], bb[4], sv);
         y[0] = t.fsphi;
         y[1] = t.tx;
         y[2] = t.ty;
@@ -269,10 +285,11 @@ struct SplitF {
 // F6: src2 of the updated state (6 values and bdif).
 struct Src2F {
     double dt, h_e;
-    DC_HD void operator()(const TP& P, const Dual* x, Dual* y) const
+    template <class PP, class R>
+    DC_HD void operator()(const PP& P, const R* x, R* y) const
     {
         for (int k = 0; k < 7; k++) y[k] = x[k];
-        dclaw_tl::src2_cell(P, Dual(dt), y[0], y[1], y[2], y[3], y[4], y[5], y[6], Dual(h_e));
+        src2_cell(P, R(dt), y[0], y[1], y[2], y[3], y[4], y[5], y[6], R(h_e));
     }
 };
 
@@ -322,6 +339,46 @@ DC_HD void vjp(const AdjArgs& A, const F& f, const TP& Pz, const double* x, cons
         double s = 0.0;
         for (int o = 0; o < NO; o++) s = s + ly[o] * Y[o].d;
         gp[p] += s;
+    }
+}
+
+// The same products as vjp, K seeds per evaluation (dclaw_tlk): the NI input
+// seeds, then one seed per parameter direction, dealt K at a time.  Each
+// tangent component is bit for bit the tangent of vjp's evaluation with that
+// seed, so lx and gp equal vjp's exactly.
+template <int NI, int NO, class F>
+DC_HD void vjpk(const AdjArgs& A, const F& f, const double* x, const double* ly, double* lx,
+                double* gp)
+{
+    using dclaw_tlk::DualK;
+    constexpr int KK = dclaw_tlk::K;
+    for (int k = 0; k < NI; k++) lx[k] = 0.0;
+    if (all_zero<NO>(ly)) return;
+    const int ns = NI + A.np;
+#pragma unroll 1
+    for (int s0 = 0; s0 < ns; s0 += KK) {
+        const dclaw_params* dp[KK];
+        for (int k = 0; k < KK; k++) {
+            const int sd = s0 + k;
+            dp[k] = (sd >= NI && sd < ns) ? &A.dP[sd - NI] : nullptr;
+        }
+        const dclaw_tlk::Params Pk = dclaw_tlk::make_params(A.P, dp);
+        DualK X[NI], Y[NO];
+        for (int a = 0; a < NI; a++) {
+            X[a] = DualK(x[a]);
+#pragma unroll
+            for (int k = 0; k < KK; k++) X[a].d[k] = (a == s0 + k) ? 1.0 : 0.0;
+        }
+        f(Pk, X, Y);
+#pragma unroll
+        for (int k = 0; k < KK; k++) {
+            const int sd = s0 + k;
+            if (sd >= ns) break;
+            double t = 0.0;
+            for (int o = 0; o < NO; o++) t = t + ly[o] * Y[o].d[k];
+            if (sd < NI) lx[sd] = t;
+            else gp[sd - NI] += t;
+        }
     }
 }
 
@@ -661,7 +718,7 @@ DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
         } else {
             const double h_e = A.P.entrainment ? A.aux[aidx(i, j, A.mx, A.maux) + 6] : 0.0;
             Src2F f{A.dt, h_e};
-            vjp<7, 7>(A, f, Pz, x, ly, lx, g);
+            vjpk<7, 7>(A, f, x, ly, lx, g);
             gp_add(A, n, g);
         }
     }
@@ -1854,7 +1911,7 @@ DC_HD void rev_taudir(const AdjArgs& A, const TP& Pz, int i, int j)
         const bool dry5 = dtol > 0.0 && x[0] <= dtol && x[7] <= dtol && x[14] <= dtol &&
                           x[21] <= dtol && x[28] <= dtol;
         if (!dry5) {
-            vjp<NTI, 3>(A, f, Pz, x, ly, lx, g);
+            vjpk<NTI, 3>(A, f, x, ly, lx, g);
             gp_add(A, n, g);
         }
     }
@@ -1949,7 +2006,7 @@ DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
         lx[6] = q7[6] > 0.0 ? lp[PC_BDIF] : 0.0;
     } else {
         PrepF f{bed(A, i, j)};
-        vjp<7, NPC>(A, f, Pz, q7, lp, lx, g);
+        vjpk<7, NPC>(A, f, q7, lp, lx, g);
         gp_add(A, n, g);
     }
     for (int m = 0; m < 7; m++) ws(A, W_LQ7 + m, n, lx[m]);

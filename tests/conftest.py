@@ -24,7 +24,8 @@ def build_emulator() -> str:
     GPU test process can load it beside torch's runtime."""
     srcs = [EMU_SRC] + [os.path.join(ROOT, "dclaw_amd", "csrc", f)
                         for f in ("dclaw_strip.h", "dclaw_physics.h", "dclaw_amr.h",
-                                  "dclaw_tangent.h", "dclaw_adjoint.h", "dclaw_strip_body.inc",
+                                  "dclaw_tangent.h", "dclaw_tangent_k.h", "dclaw_adjoint.h",
+                                  "dclaw_strip_body.inc", "dclaw_prepare_body.inc",
                                   "dclaw_physics_body.inc", "dcmath.h")]
     if os.path.exists(EMU_LIB) and all(os.path.getmtime(EMU_LIB) >= os.path.getmtime(s)
                                        for s in srcs):

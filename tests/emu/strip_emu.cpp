@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "dclaw_adjoint.h"
@@ -819,4 +820,83 @@ extern "C" double emu_riemanntype_vjp_check(double drytol, int n, unsigned seed,
         worst = std::max(worst, scale > 0.0 ? diff / scale : diff);
     }
     return worst;
+}
+
+// The K-direction products (vjpk) against the one-direction seeded products
+// (vjp) of the three stages that use them -- prepare (F1), calc_taudir (F2)
+// and src2 -- on n random wet cells with np parameter directions (phi, kref,
+// m_crit, mu): returns the number of input or parameter adjoints that differ
+// in any bit.
+extern "C" int emu_vjpk_check(const dclaw_params* P, int n, int np, unsigned seed)
+{
+    namespace D = dclaw_adj;
+    unsigned long long st = seed * 2862933555777941757ull + 3037000493ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)((st >> 11) & ((1ull << 53) - 1)) / (double)(1ull << 53) * 2.0 - 1.0;
+    };
+    dclaw_params dP[4] = {};
+    dP[0].phi = 1.0;
+    dP[1].kref = 1.0;
+    dP[2].m_crit = 1.0;
+    dP[3].mu = 1.0;
+    D::AdjArgs A{};
+    A.P = *P;
+    A.dP = dP;
+    A.np = np;
+    const D::TP Pz = D::zero_params(*P);
+    const double g = P->grav;
+    auto state = [&](double* q, bool wet) {
+        const double h = wet ? 0.05 + 2.0 * (rnd() + 1.0) : 0.5 * P->dry_tolerance * (rnd() + 1.0);
+        const double m = 0.55 + 0.1 * rnd();
+        q[0] = h;
+        q[1] = h * 2.0 * rnd();
+        q[2] = h * 2.0 * rnd();
+        q[3] = h * m;
+        q[4] = P->rho_f * g * h * (0.7 + 0.3 * rnd());
+        q[5] = h * 0.5 * (rnd() + 1.0);
+        q[6] = 0.05 * rnd();
+    };
+    int bad = 0;
+    auto cmp = [&](const double* a, const double* b, int m) {
+        for (int k = 0; k < m; k++)
+            if (std::memcmp(&a[k], &b[k], sizeof(double)) != 0) bad++;
+    };
+    for (int c = 0; c < n; c++) {
+        double ga[4] = {0, 0, 0, 0}, gb[4] = {0, 0, 0, 0};
+        {
+            double q[7], ly[D::NPC], la[7], lb[7];
+            state(q, c % 5 != 0);
+            for (int k = 0; k < D::NPC; k++) ly[k] = rnd();
+            D::PrepF f{rnd()};
+            D::vjp<7, D::NPC>(A, f, Pz, q, ly, la, ga);
+            D::vjpk<7, D::NPC>(A, f, q, ly, lb, gb);
+            cmp(la, lb, 7);
+        }
+        {
+            double x[D::NTI], ly[3], la[D::NTI], lb[D::NTI];
+            for (int s = 0; s < 5; s++) state(x + 7 * s, s == 0 || c % 3 != 0);
+            x[35] = P->rho_f + 1000.0 * (rnd() + 1.0);
+            x[36] = 1e3 * (rnd() + 1.0);
+            for (int k = 0; k < 3; k++) ly[k] = rnd();
+            D::TaudirF f;
+            f.dx = 1.0 + 0.5 * rnd();
+            f.dy = 1.0 + 0.5 * rnd();
+            for (int s = 0; s < 5; s++) f.b[s] = rnd();
+            D::vjp<D::NTI, 3>(A, f, Pz, x, ly, la, ga);
+            D::vjpk<D::NTI, 3>(A, f, x, ly, lb, gb);
+            cmp(la, lb, D::NTI);
+        }
+        {
+            double q[7], ly[7], la[7], lb[7];
+            state(q, true);
+            for (int k = 0; k < 7; k++) ly[k] = rnd();
+            D::Src2F f{0.01 * (rnd() + 1.5), 0.5 * (rnd() + 1.0)};
+            D::vjp<7, 7>(A, f, Pz, q, ly, la, ga);
+            D::vjpk<7, 7>(A, f, q, ly, lb, gb);
+            cmp(la, lb, 7);
+        }
+        cmp(ga, gb, np);
+    }
+    return bad;
 }

@@ -234,3 +234,19 @@ def test_hand_riemanntype_adjoint_equals_seeded(emu_lib):
     err = emu_lib.emu_riemanntype_vjp_check(ctypes.c_double(1e-3), 20000, 7, counts)
     assert min(counts) >= 1000, list(counts)
     assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("variant", ["default", "segregation_m1", "entrainment_m1"])
+@pytest.mark.parametrize("np_", [0, 1, 3, 4])
+def test_k_direction_products_equal_seeded_bitwise(emu_lib, variant, np_):
+    """The K-directions-per-evaluation products of the prepare, calc_taudir
+    and src2 stages equal the one-direction seeded products bit for bit (every
+    input adjoint and parameter contribution), with 0 to 4 parameter
+    directions, across the segregation, src2method and entrainment variants."""
+    from dclaw_amd import workloads as W
+    p = to_c(W.config0_params())
+    if variant == "segregation_m1":
+        p.segregation, p.src2method = 1, -1
+    elif variant == "entrainment_m1":
+        p.src2method, p.entrainment = 1, 1
+    assert emu_lib.emu_vjpk_check(ctypes.byref(p), 300, np_, 11) == 0
