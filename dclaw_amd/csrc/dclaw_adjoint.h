@@ -364,6 +364,7 @@ DC_HD void vjpk(const AdjArgs& A, const F& f, const double* x, const double* ly,
         }
         const dclaw_tlk::Params Pk = dclaw_tlk::make_params(A.P, dp);
         DualK X[NI], Y[NO];
+#pragma unroll
         for (int a = 0; a < NI; a++) {
             X[a] = DualK(x[a]);
 #pragma unroll
@@ -373,11 +374,16 @@ DC_HD void vjpk(const AdjArgs& A, const F& f, const double* x, const double* ly,
 #pragma unroll
         for (int k = 0; k < KK; k++) {
             const int sd = s0 + k;
-            if (sd >= ns) break;
             double t = 0.0;
+#pragma unroll
             for (int o = 0; o < NO; o++) t = t + ly[o] * Y[o].d[k];
-            if (sd < NI) lx[sd] = t;
-            else gp[sd - NI] += t;
+            // compile-time indices (predicated), so lx and gp stay in registers
+#pragma unroll
+            for (int a = 0; a < NI; a++)
+                if (a == sd) lx[a] = t;
+#pragma unroll
+            for (int p = 0; p < MAXP; p++)
+                if (p == sd - NI && sd < ns) gp[p] += t;
         }
     }
 }
@@ -732,15 +738,16 @@ DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
 // branches as the split (dry cell, eta below both neighbours' bed, the sign
 // of each speed and the eta test of its side); equals vjp<NSI, 12> of SplitF
 // up to rounding.
-DC_HD void split_vjp(const AdjArgs& A, const SplitF& f, const double* x, const double* ly,
-                     double* lx, double* gp)
+template <int MU>
+DC_HD void split_vjp_t(const AdjArgs& A, const SplitF& f, const double* x, const double* ly,
+                       double* lx, double* gp)
 {
     const dclaw_params& P = A.P;
     for (int k = 0; k < NSI; k++) lx[k] = 0.0;
     const double h = x[0];
     if (!f.active || !(h > P.dry_tolerance)) return;
-    const bool xd = (f.ixy == 1);
-    const int mu = xd ? 1 : 2, mv = xd ? 2 : 1;
+    constexpr bool xd = (MU == 1);
+    constexpr int mu = MU, mv = 3 - MU;
     const double nu = xd ? x[1] : x[2], nv = xd ? x[2] : x[1];
     const double u = nu / h, v = nv / h, m = x[3] / h, chi = x[4] / h;
     const double eta = h + f.bedc - x[5];
@@ -831,19 +838,32 @@ DC_HD void split_vjp(const AdjArgs& A, const SplitF& f, const double* x, const d
         gp[p] += lg * A.dP[p].grav + lrhos * A.dP[p].rho_s + lrhof * A.dP[p].rho_f;
 }
 
+// The normal momentum's slot (1 for an x-sweep split, 2 for y) as a
+// compile-time constant, so no array is indexed at run time.
+DC_HD void split_vjp(const AdjArgs& A, const SplitF& f, const double* x, const double* ly,
+                     double* lx, double* gp)
+{
+    if (f.ixy == 1) split_vjp_t<1>(A, f, x, ly, lx, gp);
+    else split_vjp_t<2>(A, f, x, ly, lx, gp);
+}
+
 // B2: the four transverse splits of the cell.
 DC_HD void rev_split(const AdjArgs& A, const TP& Pz, int i, int j)
 {
     const size_t n = cix(A, i, j);
     double ls[6] = {0, 0, 0, 0, 0, 0};
-    double la[4][6];
     double g[MAXP] = {0, 0, 0, 0};
     const double hx = 0.5 * (A.dt / A.dx), hy = 0.5 * (A.dt / A.dy);
     for (int s = 0; s < 4; s++) {
-        for (int k = 0; k < 6; k++) la[s][k] = 0.0;
+        // adjoint of the split's asdq: amdq of x-edge i+1 / apdq of x-edge i
+        // (s = 0, 1), the same for y-edges (s = 2, 3)
+        const int fa = s < 2 ? W_LXA + 6 * s : W_LYA + 6 * (s - 2);
         double x[NSI];
         SplitF f;
-        if (!split_inputs(A, s, i, j, x, f)) continue;
+        if (!split_inputs(A, s, i, j, x, f)) {
+            for (int k = 0; k < 6; k++) ws(A, fa + k, n, 0.0);
+            continue;
+        }
         double ly[12];
         for (int z = 0; z < 6; z++) {
             if (s < 2) {
@@ -859,15 +879,9 @@ DC_HD void rev_split(const AdjArgs& A, const TP& Pz, int i, int j)
         double lx[NSI];
         split_vjp(A, f, x, ly, lx, g);
         for (int k = 0; k < 6; k++) ls[k] += lx[k];
-        for (int k = 0; k < 6; k++) la[s][k] = lx[6 + k];
+        for (int k = 0; k < 6; k++) ws(A, fa + k, n, lx[6 + k]);
     }
     for (int k = 0; k < 6; k++) ws(A, W_LS + k, n, ls[k]);
-    for (int k = 0; k < 6; k++) {
-        ws(A, W_LXA + k, n, la[0][k]);
-        ws(A, W_LXA + 6 + k, n, la[1][k]);
-        ws(A, W_LYA + k, n, la[2][k]);
-        ws(A, W_LYA + 6 + k, n, la[3][k]);
-    }
     gp_add(A, n, g);
 }
 
@@ -1303,49 +1317,42 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     const double* XR = x + NSD;
     if (XL[SD_H] <= drytol && XR[SD_H] <= drytol) return;
     const bool xd = (f.ixy == 1);
-    const int NU = xd ? SD_HU : SD_HV, TU = xd ? SD_HV : SD_HU;
-    // effective side values and where each comes from: the index of the
-    // input (-1: a constant) and a sign (a mirrored velocity / momentum)
-    double hL = XL[SD_H], hR = XR[SD_H], huL = XL[NU], huR = XR[NU], hvL = XL[TU], hvR = XR[TU];
+    // effective side values after the dry resets / wall mirror; `sides`
+    // records which inputs they come from (indices stay compile-time
+    // constants, so the side arrays live in registers)
+    double hL = XL[SD_H], hR = XR[SD_H];
+    double huL = xd ? XL[SD_HU] : XL[SD_HV], huR = xd ? XR[SD_HU] : XR[SD_HV];  // normal
+    double hvL = xd ? XL[SD_HV] : XL[SD_HU], hvR = xd ? XR[SD_HV] : XR[SD_HU];  // transverse
     double hmL = XL[SD_HM], hmR = XR[SD_HM], pL = XL[SD_P], pR = XR[SD_P];
     double bL = f.bedL - XL[SD_BDIF], bR = f.bedR - XR[SD_BDIF];
     double uL = XL[SD_U], uR = XR[SD_U], vL = XL[SD_V], vR = XR[SD_V];
     double mL = XL[SD_M], mR = XR[SD_M], chiL = XL[SD_CHI], chiR = XR[SD_CHI];
     double svrL = XL[SD_RHO], svtL = XL[SD_TAU], svrR = XR[SD_RHO], svtR = XR[SD_TAU];
     const double taudirR = XR[SD_TAUDIR], fsL = XL[SD_FS], fsR = XR[SD_FS];
-    int oh[2] = {SD_H, NSD + SD_H}, ohu[2] = {NU, NSD + NU}, ohv[2] = {TU, NSD + TU};
-    int ohm[2] = {SD_HM, NSD + SD_HM}, op[2] = {SD_P, NSD + SD_P};
-    int ob[2] = {SD_BDIF, NSD + SD_BDIF}, ou[2] = {SD_U, NSD + SD_U}, ov[2] = {SD_V, NSD + SD_V};
-    int om[2] = {SD_M, NSD + SD_M}, ochi[2] = {SD_CHI, NSD + SD_CHI};
-    int osr[2] = {SD_RHO, NSD + SD_RHO}, ost[2] = {SD_TAU, NSD + SD_TAU};
-    double shu[2] = {1.0, 1.0}, su[2] = {1.0, 1.0};
+    enum { OWN, RESET_R, WALL_R, RESET_L, WALL_L } sides = OWN;
     double wall0 = 1.0, wall1 = 1.0, wall2 = 1.0;
     bool wallprob = false;
     if (hR <= drytol) {
         hR = 0.0; pR = 0.0; mR = mL; chiR = chiL;
-        oh[1] = -1; op[1] = -1; om[1] = om[0]; ochi[1] = ochi[0];
+        sides = RESET_R;
         const dclaw::RType rt = dclaw::riemanntype(hL, hL, uL, -uL, drytol, grav);
         const double hstartest = dclaw::dmax(hL, rt.hm);
         if (hstartest + bL < bR) {
             wall1 = 0.0; wall2 = 0.0; wallprob = true;
             hR = hL; huR = -huL; hvR = hvL; hmR = hmL; bR = bL;
             uR = -uL; vR = vL; mR = mL; pR = pL; chiR = chiL; svrR = svrL; svtR = svtL;
-            oh[1] = oh[0]; ohu[1] = ohu[0]; shu[1] = -shu[0]; ohv[1] = ohv[0]; ohm[1] = ohm[0];
-            ob[1] = ob[0]; ou[1] = ou[0]; su[1] = -su[0]; ov[1] = ov[0]; om[1] = om[0];
-            op[1] = op[0]; ochi[1] = ochi[0]; osr[1] = osr[0]; ost[1] = ost[0];
+            sides = WALL_R;
         }
     } else if (hL <= drytol) {
         hL = 0.0; pL = 0.0; mL = mR; chiL = chiR;
-        oh[0] = -1; op[0] = -1; om[0] = om[1]; ochi[0] = ochi[1];
+        sides = RESET_L;
         const dclaw::RType rt = dclaw::riemanntype(hR, hR, -uR, uR, drytol, grav);
         const double hstartest = dclaw::dmax(hR, rt.hm);
         if (hstartest + bR < bL) {
             wall0 = 0.0; wall1 = 0.0; wallprob = true;
             hL = hR; huL = -huR; hvL = hvR; hmL = hmR; mL = mR; bL = bR;
             uL = -uR; vL = vR; pL = pR; chiL = chiR; svrL = svrR; svtL = svtR;
-            oh[0] = oh[1]; ohu[0] = ohu[1]; shu[0] = -shu[1]; ohv[0] = ohv[1]; ohm[0] = ohm[1];
-            om[0] = om[1]; ob[0] = ob[1]; ou[0] = ou[1]; su[0] = -su[1]; ov[0] = ov[1];
-            op[0] = op[1]; ochi[0] = ochi[1]; osr[0] = osr[1]; ost[0] = ost[1];
+            sides = WALL_L;
         }
     }
 
@@ -1837,22 +1844,53 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
         svrRb += rhoLb + rhoRb; svtRb += tauRsb + 0.5 * tauLsb; hRb += 0.5 * hb;
         break;
     }
-    // back to the side inputs
-    auto put = [&](int o, double s, double v) {
-        if (o >= 0) lx[o] += s * v;
+    // back to the side inputs: b = bed - bdif; a reset side's h and p are
+    // constants and its m, chi the other side's; a mirrored side is the other
+    // side with the normal momentum and velocity negated
+    auto side = [&](double* l, double h_, double hu_, double hv_, double hm_, double p_,
+                    double b_, double u_, double v_, double m_, double chi_, double svr_,
+                    double svt_) {
+        l[SD_H] += h_;
+        if (xd) {
+            l[SD_HU] += hu_;
+            l[SD_HV] += hv_;
+        } else {
+            l[SD_HV] += hu_;
+            l[SD_HU] += hv_;
+        }
+        l[SD_HM] += hm_;
+        l[SD_P] += p_;
+        l[SD_BDIF] -= b_;
+        l[SD_U] += u_;
+        l[SD_V] += v_;
+        l[SD_M] += m_;
+        l[SD_CHI] += chi_;
+        l[SD_RHO] += svr_;
+        l[SD_TAU] += svt_;
     };
-    put(oh[0], 1.0, hLb); put(oh[1], 1.0, hRb);
-    put(ohu[0], shu[0], huLb); put(ohu[1], shu[1], huRb);
-    put(ohv[0], 1.0, hvLb); put(ohv[1], 1.0, hvRb);
-    put(ohm[0], 1.0, hmLb); put(ohm[1], 1.0, hmRb);
-    put(op[0], 1.0, pLb); put(op[1], 1.0, pRb);
-    put(ob[0], -1.0, bLb); put(ob[1], -1.0, bRb);
-    put(ou[0], su[0], uLb); put(ou[1], su[1], uRb);
-    put(ov[0], 1.0, vLb); put(ov[1], 1.0, vRb);
-    put(om[0], 1.0, mLb); put(om[1], 1.0, mRb);
-    put(ochi[0], 1.0, chiLb); put(ochi[1], 1.0, chiRb);
-    put(osr[0], 1.0, svrLb); put(osr[1], 1.0, svrRb);
-    put(ost[0], 1.0, svtLb); put(ost[1], 1.0, svtRb);
+    switch (sides) {
+    case OWN:
+        side(lx, hLb, huLb, hvLb, hmLb, pLb, bLb, uLb, vLb, mLb, chiLb, svrLb, svtLb);
+        side(lx + NSD, hRb, huRb, hvRb, hmRb, pRb, bRb, uRb, vRb, mRb, chiRb, svrRb, svtRb);
+        break;
+    case RESET_R:
+        side(lx, hLb, huLb, hvLb, hmLb, pLb, bLb, uLb, vLb, mLb + mRb, chiLb + chiRb, svrLb, svtLb);
+        side(lx + NSD, 0.0, huRb, hvRb, hmRb, 0.0, bRb, uRb, vRb, 0.0, 0.0, svrRb, svtRb);
+        break;
+    case WALL_R:
+        side(lx, hLb + hRb, huLb - huRb, hvLb + hvRb, hmLb + hmRb, pLb + pRb, bLb + bRb,
+             uLb - uRb, vLb + vRb, mLb + mRb, chiLb + chiRb, svrLb + svrRb, svtLb + svtRb);
+        break;
+    case RESET_L:
+        side(lx, 0.0, huLb, hvLb, hmLb, 0.0, bLb, uLb, vLb, 0.0, 0.0, svrLb, svtLb);
+        side(lx + NSD, hRb, huRb, hvRb, hmRb, pRb, bRb, uRb, vRb, mRb + mLb, chiRb + chiLb, svrRb,
+             svtRb);
+        break;
+    default:
+        side(lx + NSD, hRb + hLb, huRb - huLb, hvRb + hvLb, hmRb + hmLb, pRb + pLb, bRb + bLb,
+             uRb - uLb, vRb + vLb, mRb + mLb, chiRb + chiLb, svrRb + svrLb, svtRb + svtLb);
+        break;
+    }
     lx[SD_FS] += fsLb;
     lx[NSD + SD_FS] += fsRb;
     lx[NSD + SD_TAUDIR] += taudirRb;
@@ -1874,8 +1912,13 @@ DC_HD void rev_waves(const AdjArgs& A, const TP& Pz, int i, int j)
         double x[2 * NSD], ly[NW];
         Rpn2F f;
         if (wave_edge(A, ixy, i, j, x, f, ly)) {
-            if (A.P.dry_tolerance > 0.0) rpn2_vjp(A, f, x, ly, lx, g);
-            else vjp<2 * NSD, NW>(A, f, Pz, x, ly, lx, g);
+            if (A.P.dry_tolerance > 0.0) {
+                rpn2_vjp(A, f, x, ly, lx, g);
+            } else {
+                double ls[2 * NSD];
+                vjp<2 * NSD, NW>(A, f, Pz, x, ly, ls, g);
+                for (int k = 0; k < 2 * NSD; k++) lx[k] = ls[k];
+            }
         }
         const int f0 = ixy == 1 ? W_LXS : W_LYS;
         for (int k = 0; k < 2 * NSD; k++) ws(A, f0 + k, n, lx[k]);
