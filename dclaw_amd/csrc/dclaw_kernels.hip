@@ -1069,9 +1069,18 @@ __device__ __forceinline__ void adj_gp_part(const double* __restrict__ W, size_t
     const size_t chunk = (N + ADJ_GP_PARTS - 1) / ADJ_GP_PARTS;
     const size_t lo = (size_t)b * chunk;
     const size_t hi = lo + chunk < N ? lo + chunk : N;
-    double s = 0.0;
-    for (size_t n = lo + threadIdx.x; n < hi; n += 256) s += g[n];
-    sh[threadIdx.x] = s;
+    // four independent sums per thread (four loads in flight), combined in a
+    // fixed order
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    size_t n = lo + threadIdx.x;
+    for (; n + 768 < hi; n += 1024) {
+        s0 += g[n];
+        s1 += g[n + 256];
+        s2 += g[n + 512];
+        s3 += g[n + 768];
+    }
+    for (; n < hi; n += 256) s0 += g[n];
+    sh[threadIdx.x] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
@@ -1160,19 +1169,45 @@ __device__ dclaw_adj::AdjArgs adj_member(const AdjBatch& B, int m, AdjTiles& T)
     return A;
 }
 
-__global__ void k_adjb_prefix(AdjBatch B)
+// Cell and tile prefixes of the members: one block, 1024 members at a time,
+// an inclusive scan in LDS plus the running carry (integer sums: the result
+// does not depend on the order).
+__global__ __launch_bounds__(1024) void k_adjb_prefix(AdjBatch B)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    long long c = 0, t = 0;
-    for (int m = 0; m < B.npatch; m++) {
-        B.cell_prefix[m] = c;
-        B.tile_prefix[m] = t;
-        const int mx = B.pt[m].mx, my = B.pt[m].my;
-        c += (long long)(mx + 4) * (my + 4);
-        t += (long long)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) * ((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+    __shared__ long long sc[1024], st[1024];
+    const int tid = (int)threadIdx.x;
+    long long carry_c = 0, carry_t = 0;
+    for (int base = 0; base < B.npatch; base += 1024) {
+        const int m = base + tid;
+        long long c = 0, t = 0;
+        if (m < B.npatch) {
+            const int mx = B.pt[m].mx, my = B.pt[m].my;
+            c = (long long)(mx + 4) * (my + 4);
+            t = (long long)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) * ((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+        }
+        sc[tid] = c;
+        st[tid] = t;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const long long vc = tid >= off ? sc[tid - off] : 0;
+            const long long vt = tid >= off ? st[tid - off] : 0;
+            __syncthreads();
+            sc[tid] += vc;
+            st[tid] += vt;
+            __syncthreads();
+        }
+        if (m < B.npatch) {
+            B.cell_prefix[m] = carry_c + sc[tid] - c;
+            B.tile_prefix[m] = carry_t + st[tid] - t;
+        }
+        carry_c += sc[1023];
+        carry_t += st[1023];
+        __syncthreads();
     }
-    B.cell_prefix[B.npatch] = c;
-    B.tile_prefix[B.npatch] = t;
+    if (tid == 0) {
+        B.cell_prefix[B.npatch] = carry_c;
+        B.tile_prefix[B.npatch] = carry_t;
+    }
 }
 
 __global__ __launch_bounds__(ADJ_BLOCK) void k_adjb_dense(AdjBatch B, long long ncells)
@@ -1682,7 +1717,7 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     B.glist = (int*)(batch_work + 2 * (npatch + 1) + 1);
     hipStream_t st = (hipStream_t)stream;
     const dim3 tg((unsigned)std::min<long long>(ntiles, ADJ_GRID)), tb(64);
-    hipLaunchKernelGGL(k_adjb_prefix, dim3(1), dim3(1), 0, st, B);
+    hipLaunchKernelGGL(k_adjb_prefix, dim3(1), dim3(1024), 0, st, B);
     hipLaunchKernelGGL(k_adjb_dense, dim3((unsigned)((ncells + ADJ_BLOCK - 1) / ADJ_BLOCK)),
                        dim3(ADJ_BLOCK), 0, st, B, ncells);
     hipLaunchKernelGGL(k_adjb_clear_stale, dim3(32, npatch), tb, 0, st, B);
