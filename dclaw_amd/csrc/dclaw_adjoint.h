@@ -346,21 +346,23 @@ DC_HD void vjp(const AdjArgs& A, const F& f, const TP& Pz, const double* x, cons
 // seeds, then one seed per parameter direction, dealt K at a time.  Each
 // tangent component is bit for bit the tangent of vjp's evaluation with that
 // seed, so lx and gp equal vjp's exactly.
+// Only the first nis inputs are seeded (the caller writes the adjoint of an
+// input that passes through the stage untouched); lx of the others is 0.
 template <int NI, int NO, class F>
 DC_HD void vjpk(const AdjArgs& A, const F& f, const double* x, const double* ly, double* lx,
-                double* gp)
+                double* gp, int nis = NI)
 {
     using dclaw_tlk::DualK;
     constexpr int KK = dclaw_tlk::K;
     for (int k = 0; k < NI; k++) lx[k] = 0.0;
     if (all_zero<NO>(ly)) return;
-    const int ns = NI + A.np;
+    const int ns = nis + A.np;
 #pragma unroll 1
     for (int s0 = 0; s0 < ns; s0 += KK) {
         const dclaw_params* dp[KK];
         for (int k = 0; k < KK; k++) {
             const int sd = s0 + k;
-            dp[k] = (sd >= NI && sd < ns) ? &A.dP[sd - NI] : nullptr;
+            dp[k] = (sd >= nis && sd < ns) ? &A.dP[sd - nis] : nullptr;
         }
         const dclaw_tlk::Params Pk = dclaw_tlk::make_params(A.P, dp);
         DualK X[NI], Y[NO];
@@ -368,7 +370,7 @@ DC_HD void vjpk(const AdjArgs& A, const F& f, const double* x, const double* ly,
         for (int a = 0; a < NI; a++) {
             X[a] = DualK(x[a]);
 #pragma unroll
-            for (int k = 0; k < KK; k++) X[a].d[k] = (a == s0 + k) ? 1.0 : 0.0;
+            for (int k = 0; k < KK; k++) X[a].d[k] = (a == s0 + k && a < nis) ? 1.0 : 0.0;
         }
         f(Pk, X, Y);
 #pragma unroll
@@ -380,10 +382,10 @@ DC_HD void vjpk(const AdjArgs& A, const F& f, const double* x, const double* ly,
             // compile-time indices (predicated), so lx and gp stay in registers
 #pragma unroll
             for (int a = 0; a < NI; a++)
-                if (a == sd) lx[a] = t;
+                if (a == sd && sd < nis) lx[a] = t;
 #pragma unroll
             for (int p = 0; p < MAXP; p++)
-                if (p == sd - NI && sd < ns) gp[p] += t;
+                if (p == sd - nis && sd < ns) gp[p] += t;
         }
     }
 }
@@ -724,7 +726,10 @@ DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
         } else {
             const double h_e = A.P.entrainment ? A.aux[aidx(i, j, A.mx, A.maux) + 6] : 0.0;
             Src2F f{A.dt, h_e};
-            vjpk<7, 7>(A, f, x, ly, lx, g);
+            // without ent_dclaw4's erosion src2 passes bdif through unchanged
+            const bool erodes = A.P.entrainment == 1 && A.P.entrainment_method == 0;
+            vjpk<7, 7>(A, f, x, ly, lx, g, erodes ? 7 : 6);
+            if (!erodes) lx[6] = ly[6];
             gp_add(A, n, g);
         }
     }
@@ -2049,7 +2054,9 @@ DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
         lx[6] = q7[6] > 0.0 ? lp[PC_BDIF] : 0.0;
     } else {
         PrepF f{bed(A, i, j)};
-        vjpk<7, NPC>(A, f, q7, lp, lx, g);
+        // bdif enters the prepared fields only as max(bdif, 0)
+        vjpk<7, NPC>(A, f, q7, lp, lx, g, 6);
+        lx[6] = q7[6] > 0.0 ? lp[PC_BDIF] : 0.0;
         gp_add(A, n, g);
     }
     for (int m = 0; m < 7; m++) ws(A, W_LQ7 + m, n, lx[m]);
