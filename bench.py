@@ -378,13 +378,19 @@ def bench_flume(args):
     centres = rng.uniform(0.1 * nsteps, 0.9 * nsteps, len(gxy))
     n = np.arange(1, nsteps + 1)[:, None]
     targets = 0.3 * np.exp(-((n - centres[None, :]) / (0.1 * nsteps)) ** 2)
-    seg = args.seg_rows if args.seg_rows > 0 else 4
-    _lib.check(min(0, _lib.lib().dclaw_set_seg_rows(seg)))
     cells = pt.mx * pt.my
 
     r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
     r.run(5, 1e-3, 0.9, 1e9)
     torch.cuda.synchronize()
+    seg = args.seg_rows
+    if seg <= 0:
+        # segment height by the autotuner (untimed); the channel is 35 strips
+        # wide, so short segments give the launch its parallelism
+        from dclaw_amd import autotune
+        seg, _ = autotune.tune_patch(r, split_modes=(r.split_mode,))
+    _lib.check(min(0, _lib.lib().dclaw_set_seg_rows(seg)))
+    print(f"[bench] flume segment rows {seg}", file=sys.stderr, flush=True)
     r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy)
     t0 = time.perf_counter()
     r.run(nsteps, 1e-3, 0.9, 1e9)
