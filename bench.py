@@ -91,7 +91,7 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the path is sharded by "
+        print(json.dumps({"error": "bench.py runs on one GPU only; the path is sharded by "
                                    "member (ensembles) or by strips with a 2-3 cell halo "
                                    "exchange (single patch) on more GPUs -- not built here",
                           "n_gpus": args.gpus}))

@@ -30,9 +30,11 @@
 // workspace; its q_out is the step's result bit for bit.  The reverse pass
 // runs the stages backwards (B1 src2, B2 splits, B3 flux, B4 waves, B5
 // taudir, B6 prepare, B7 the boundary map), each computing the
-// vector-Jacobian product of its stage from the stage's local Jacobian:
-// one tangent-linear evaluation of the stage per input, seeded with that
-// input (and one per parameter direction).  Every stage output has exactly
+// vector-Jacobian product of its stage.  waves (rpn2), flux and split have
+// hand-written reverses (rpn2_vjp, flux_vjp, split_vjp); prepare, taudir and
+// src2 take theirs from the stage's local Jacobian: tangent-linear
+// evaluations of the stage seeded with each input and each parameter
+// direction, four directions per evaluation (vjpk).  Every stage output has exactly
 // one consumer slot, so each reverse kernel gathers from its neighbours'
 // slots and writes only its own cell: no atomics, and the result does not
 // depend on scheduling.

@@ -1,7 +1,8 @@
 /*
  * dclaw_oracle.c -- CPU restatement of D-Claw's per-patch finite-volume step.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Never linked into the product.
+ * TEST INFRASTRUCTURE ONLY (the oracle of the tests).  Never linked into the
+ * product.
  *
  * Each routine restates the reference Fortran it cites, statement by
  * statement and in the same floating-point evaluation order (Fortran

@@ -1,7 +1,7 @@
 /*
  * dclaw_oracle.h -- CPU restatement of D-Claw's per-patch finite-volume step.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle the HIP path is compared
  * against; it is never linked into, loaded by, or called from the product
  * library (dclaw_amd/).  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.

@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/dclaw_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+TEST INFRASTRUCTURE ONLY: the oracle the HIP path is compared against.  Only tests/,
 ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg import this
 module; the product package ``dclaw_amd`` never does.
 """

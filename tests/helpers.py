@@ -12,7 +12,7 @@ from dclaw_amd.params import CParams, to_c
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
-# Parity bars (north star): per step 1e-12, after 1000 steps 1e-9, relative
+# Parity bars: per step 1e-12, after 1000 steps 1e-9, relative
 # L-inf over wet cells (normwise per component); wet/dry mask bit-identical.
 TOL_STEP = 1e-12
 TOL_LONG = 1e-9

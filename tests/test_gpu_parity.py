@@ -1,10 +1,9 @@
 """GPU parity: the fused gfx950 stepgrid against the CPU oracle.
 
-Bars (north star): per step relative L-inf over wet cells <= 1e-12, after
-1000 steps <= 1e-9, wet/dry mask identical; CFL numbers to 1e-12.  The GPU
-uses the device math library (OCML) for exp/atan/tan/tanh where the oracle
-uses glibc, so bits may differ at the ulp level; everything else (operation
-order, no FMA contraction) is the same.
+Bars: per step relative L-inf over wet cells <= 1e-12, after 1000 steps
+<= 1e-9, wet/dry mask identical; CFL numbers to 1e-12.  The GPU and the
+oracle share the elementary functions (dcmath.h), the operation order and
+the absence of FMA contraction, so in practice the results are bit-identical.
 """
 import numpy as np
 import pytest

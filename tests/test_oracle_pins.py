@@ -2,8 +2,8 @@
 
 Each check is chosen so that a plausible transcription mistake (a dropped
 term, a sign, a swapped index or operand) fails it; none of them retypes the
-oracle's own formula.  Sources: the survey's compiled-reference probe of
-riemanntype (known answer), the reference's eigenstructure tests
+oracle's own formula.  Sources: a known answer of riemanntype from a run of
+the reference's compiled Fortran, the reference's eigenstructure tests
 (tests/eigenstructure_check/check.py:41-121), textbook shallow-water
 identities (f-wave sums, the transverse Jacobian, lake at rest), and the
 closed-form solutions of the source sub-steps.
