@@ -3,7 +3,7 @@ it flagged) against the fused step, single patch and batched, bit for bit."""
 import numpy as np
 import pytest
 
-from helpers import advance_oracle, slope_patch
+from helpers import TOL_STEP, advance_oracle, mask_equal, rel_linf_wet, slope_patch
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,8 @@ def dev():
 
 
 @pytest.mark.parametrize("mx,my,seg,bc", [(130, 70, 8, (3, 3, 3, 3)), (59, 33, 4, (1, 3, 3, 1)),
-                                          (200, 150, 16, (1, 1, 1, 1)), (7, 90, 32, None)])
+                                          (200, 150, 16, (1, 1, 1, 1)), (7, 90, 32, None),
+                                          (200, 40, 1, (1, 1, 1, 1)), (90, 37, 2, (3, 1, 3, 1))])
 def test_split_equals_fused(dev, oracle_mod, mx, my, seg, bc):
     from dclaw_amd import _lib
     from dclaw_amd import step as S
@@ -267,7 +268,8 @@ def test_tuner_restores_state(dev):
 
 
 @pytest.mark.parametrize("mx,my,seg,bc", [(130, 70, 8, (3, 3, 3, 3)), (59, 33, 4, (1, 3, 3, 1)),
-                                          (600, 500, 6, (1, 1, 1, 1)), (7, 90, 32, None)])
+                                          (600, 500, 6, (1, 1, 1, 1)), (7, 90, 32, None),
+                                          (200, 40, 1, (1, 1, 1, 1)), (90, 37, 2, (3, 1, 3, 1))])
 def test_ordered_equals_fused(dev, oracle_mod, mx, my, seg, bc):
     """The split step with the wet-first tile order, starting from arbitrary
     flags in the workspace, equals the fused step on every cell; the order it
@@ -318,7 +320,8 @@ def test_ordered_equals_fused(dev, oracle_mod, mx, my, seg, bc):
 
 
 @pytest.mark.parametrize("mx,my,seg,bc", [(130, 70, 8, (3, 3, 3, 3)), (59, 33, 4, (1, 3, 3, 1)),
-                                          (600, 500, 6, (1, 1, 1, 1)), (7, 90, 5, None)])
+                                          (600, 500, 6, (1, 1, 1, 1)), (7, 90, 5, None),
+                                          (200, 40, 1, (1, 1, 1, 1)), (90, 37, 2, (3, 1, 3, 1))])
 def test_hybrid_equals_fused(dev, oracle_mod, mx, my, seg, bc):
     """The hybrid split step (the previous step's flagged 4-segment groups
     as single segments first, the others as one tall segment each), from
@@ -378,3 +381,41 @@ def test_hybrid_equals_fused(dev, oracle_mod, mx, my, seg, bc):
     assert len(dry_list) + len({t // nstrips // 4 * nstrips + t % nstrips for t in flagged}) \
         == nstrips * ngi
     assert set(w[:ntiles].tolist()) <= {0, 1}
+
+
+@pytest.mark.parametrize("mode", ["single", "ordered", "dry", "hybrid"])
+@pytest.mark.parametrize("seg", [1, 2])
+def test_short_segments_match_oracle(dev, oracle_mod, mode, seg):
+    """One- and two-row segments (the autotuner's pick on narrow patches,
+    where few strips leave the launch short of work) give the oracle's step
+    in every split mode, on a 200 x 40 slope patch; the second call runs with
+    the flags the first one left."""
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    O = oracle_mod
+    mx, my = 200, 40
+    pt = slope_patch(mx, my, slope=0.8, phi_deg=25.0, seed=7)
+    q, aux, dt = advance_oracle(O, pt, 8, bc=(1, 1, 1, 1))
+    print(f"[short segments] {mode} seg {seg}", flush=True)
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(seg)
+    try:
+        ws = dev.zeros((L.dclaw_split_workspace_size(mx, my),), dtype=dev.int32, device="cuda")
+        pdev = S.params_tensor([pt.params])
+        dtd = dev.tensor([dt], dtype=dev.float64, device="cuda")
+        bed = S.bed_tensor(aux)
+        qi = S.to_device_q(q)
+        qs = dev.full_like(qi, float("nan"))
+        cs = dev.zeros(1, dtype=dev.float64, device="cuda")
+        for _ in range(2):
+            S.stepgrid_split(qi, qs, bed, 1, mx, my, pt.dx, pt.dy, None, dtd, pdev, cs, ws,
+                             mode=mode)
+        dev.cuda.synchronize()
+    finally:
+        L.dclaw_set_seg_rows(prev)
+    qg = S.to_host_q(qs)
+    qo, _, cflo = O.stepgrid(O.params_from(pt.params), q, aux, pt.dx, pt.dy, dt)
+    assert not np.isnan(qg).any()
+    assert mask_equal(qg, qo, pt.params.dry_tolerance)
+    assert rel_linf_wet(qg, qo, pt.params.dry_tolerance) <= TOL_STEP
+    assert abs(cs.item() - cflo) <= TOL_STEP * cflo
