@@ -709,7 +709,9 @@ DC_HD void gp_zero(const AdjArgs& A, size_t n)
 }
 DC_HD void gp_add(const AdjArgs& A, size_t n, const double* g)
 {
-    for (int p = 0; p < A.np; p++) ws(A, W_GP + p, n, wv(A, W_GP + p, n) + g[p]);
+#pragma unroll
+    for (int p = 0; p < MAXP; p++)
+        if (p < A.np) ws(A, W_GP + p, n, wv(A, W_GP + p, n) + g[p]);
 }
 
 // B1: src2 (and the parameter gradient is started here).
@@ -780,16 +782,19 @@ DC_HD void split_vjp_t(const AdjArgs& A, const SplitF& f, const double* x, const
     double ls[3] = {0, 0, 0}, lbe[3] = {0, 0, 0}, lr[3][6];
     for (int w = 0; w < 3; w++)
         for (int k = 0; k < 6; k++) lr[w][k] = 0.0;
+#pragma unroll
     for (int w = 0; w < 3; w++) {
-        const double* lt = nullptr;
-        if (s[w] < 0.0 && eta >= f.topo1) lt = ly;
-        else if (s[w] > 0.0 && eta >= f.topo3) lt = ly + 6;
-        if (!lt) continue;
+        // the wave goes to bm (ly[0..5]) or bp (ly[6..11]), or nowhere
+        const bool to_m = s[w] < 0.0 && eta >= f.topo1;
+        const bool to_p = !to_m && s[w] > 0.0 && eta >= f.topo3;
+        if (!to_m && !to_p) continue;
         // out[k] += ((1 * s) * be) * r[k]
+#pragma unroll
         for (int k = 0; k < 6; k++) {
-            ls[w] += lt[k] * be[w] * r[w][k];
-            lbe[w] += lt[k] * s[w] * r[w][k];
-            lr[w][k] += lt[k] * s[w] * be[w];
+            const double lt = to_m ? ly[k] : ly[6 + k];
+            ls[w] += lt * be[w] * r[w][k];
+            lbe[w] += lt * s[w] * r[w][k];
+            lr[w][k] += lt * s[w] * be[w];
         }
     }
     double lu = lr[0][mu] + lr[2][mu], lm = lr[0][3] + lr[2][3];
@@ -841,8 +846,9 @@ DC_HD void split_vjp_t(const AdjArgs& A, const SplitF& f, const double* x, const
     lx[4] += lchi / h;
     lh += -(lu * u + lv * v + lm * m + lchi * chi) / h;
     lx[0] += lh;
-    for (int p = 0; p < A.np; p++)
-        gp[p] += lg * A.dP[p].grav + lrhos * A.dP[p].rho_s + lrhof * A.dP[p].rho_f;
+#pragma unroll
+    for (int p = 0; p < MAXP; p++)
+        if (p < A.np) gp[p] += lg * A.dP[p].grav + lrhos * A.dP[p].rho_s + lrhof * A.dP[p].rho_f;
 }
 
 // The normal momentum's slot (1 for an x-sweep split, 2 for y) as a
@@ -1901,9 +1907,11 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     lx[SD_FS] += fsLb;
     lx[NSD + SD_FS] += fsRb;
     lx[NSD + SD_TAUDIR] += taudirRb;
-    for (int p = 0; p < A.np; p++)
-        gp[p] += gravb * A.dP[p].grav + rhofb * A.dP[p].rho_f + kappab * A.dP[p].kappa +
-                 betasegb * A.dP[p].beta_seg;
+#pragma unroll
+    for (int p = 0; p < MAXP; p++)
+        if (p < A.np)
+            gp[p] += gravb * A.dP[p].grav + rhofb * A.dP[p].rho_f + kappab * A.dP[p].kappa +
+                     betasegb * A.dP[p].beta_seg;
 }
 
 // B4: rpn2 of the x- and y-edge at (i, j): the hand-written product for
@@ -1922,9 +1930,11 @@ DC_HD void rev_waves(const AdjArgs& A, const TP& Pz, int i, int j)
             if (A.P.dry_tolerance > 0.0) {
                 rpn2_vjp(A, f, x, ly, lx, g);
             } else {
-                double ls[2 * NSD];
-                vjp<2 * NSD, NW>(A, f, Pz, x, ly, ls, g);
+                // (its own arrays: the seeded product indexes them at run time)
+                double ls[2 * NSD], gs[MAXP] = {0, 0, 0, 0};
+                vjp<2 * NSD, NW>(A, f, Pz, x, ly, ls, gs);
                 for (int k = 0; k < 2 * NSD; k++) lx[k] = ls[k];
+                for (int p = 0; p < MAXP; p++) g[p] += gs[p];
             }
         }
         const int f0 = ixy == 1 ? W_LXS : W_LYS;
