@@ -1273,6 +1273,23 @@ __global__ __launch_bounds__(64) void k_adjb(AdjBatch B)
     }
 }
 
+// The same, held to two waves per SIMD (for the stages whose batched form
+// needs just over half the register file).
+template <int STAGE>
+__global__ __launch_bounds__(64, 2) void k_adjb2(AdjBatch B)
+{
+    const int nact = *B.gcount;
+    for (int k = blockIdx.x; k < nact; k += gridDim.x) {
+        const long long g = B.glist[k];
+        const int m = adj_find(B.tile_prefix, B.npatch, g);
+        AdjTiles T;
+        const dclaw_adj::AdjArgs A = adj_member(B, m, T);
+        const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
+        int i, j;
+        if (adj_tile_cell(A, T, (int)(g - B.tile_prefix[m]), i, j)) adj_stage<STAGE>(A, Pz, i, j);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_adjb_gp(AdjBatch B)
 {
     const dclaw_adj_patch& p = B.pt[blockIdx.z];
@@ -1730,10 +1747,10 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     hipLaunchKernelGGL(k_adjb<ADJ_REC_SPLIT>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_UPDATE>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_SRC2>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb<ADJ_REV_SPLIT>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb2<ADJ_REV_SPLIT>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_FLUX>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_WAVES>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb<ADJ_REV_TAUDIR>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb2<ADJ_REV_TAUDIR>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_PREPARE>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_BC>, tg, tb, 0, st, B);
     if (np > 0) {
