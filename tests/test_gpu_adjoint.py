@@ -1,8 +1,9 @@
 """GPU: the adjoint step (dclaw_adjoint_step_dev) against the host emulator
 of the same stages; the transpose identity against the GPU tangent-linear
 step; and gradients of multi-step gauge losses by the checkpointed reverse
-sweep (and through torch.autograd) against the tangent-linear gradients,
-which test_gpu_tangent.py pins to finite differences."""
+sweep (and through torch.autograd) against the tangent-linear gradients
+(which test_gpu_tangent.py pins to finite differences) and directly against
+central differences of the forward runs."""
 import numpy as np
 import pytest
 
@@ -145,6 +146,56 @@ def test_reverse_gradient_equals_tangent_gradients(dev):
         pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy, targets, parameters=("phi", "kref"),
         checkpoint_every=1000)
     assert np.array_equal(lam0_b, lam0) and grads_b == grads_a
+
+
+def test_reverse_gradient_matches_finite_differences(dev):
+    """The reverse sweep's gradient of a gauge misfit over 120 steps of a
+    flume release (checkpoints every 10 steps) against central differences of
+    the forward runs on the same time grid: d/dphi and d/dkref at a 1e-6
+    relative step, and the initial-state gradient along the m0 direction
+    (hm = m0 h)."""
+    from dclaw_amd import adjoint as AD
+    from dclaw_amd import step as S
+    torch = dev
+    pt, gxy, nsteps, targets = _flume_case()
+    loss, grads, lam0, dts = AD.gauge_loss_gradient_adjoint(
+        pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy, targets, parameters=("phi", "kref"),
+        checkpoint_every=10)
+
+    def fwd_loss(c=None, q0=None):
+        r = S.PatchRunner(pt.q if q0 is None else q0, pt.aux, pt.params, pt.dx, pt.dy)
+        if c is not None:
+            r.params = torch.frombuffer(bytearray(bytes(c)), dtype=torch.uint8).to("cuda")
+        xy = torch.tensor(gxy, device="cuda")
+        out = torch.zeros((len(gxy), 8), dtype=torch.float64, device="cuda")
+        tot = 0.0
+        for n, d in enumerate(dts):
+            r.set_dt(float(d))
+            r.cfl.zero_()
+            r.step()
+            S.gauges(r.qa, r.bed, r.maux, r.mx, r.my, 0.0, 0.0, r.dx, r.dy,
+                     pt.params.dry_tolerance, xy, out)
+            res = out[:, 0].cpu().numpy() - targets[n]
+            tot += float((res * res).sum())
+        return tot
+
+    assert fwd_loss() == pytest.approx(loss, rel=1e-13)
+    for name in ("phi", "kref"):
+        c0 = getattr(to_c(pt.params), name)
+        eps = 1e-6 * c0
+        cp, cm = to_c(pt.params), to_c(pt.params)
+        setattr(cp, name, c0 + eps)
+        setattr(cm, name, c0 - eps)
+        fd = (fwd_loss(c=cp) - fwd_loss(c=cm)) / (2 * eps)
+        print(f"[adjoint fd] {name}: reverse {grads[name]:.10e} central {fd:.10e}", flush=True)
+        assert grads[name] == pytest.approx(fd, rel=1e-4), name
+    dq_m0 = np.zeros_like(pt.q)
+    dq_m0[3] = pt.q[0]
+    eps = 1e-6
+    fd = (fwd_loss(q0=pt.q + eps * dq_m0) - fwd_loss(q0=pt.q - eps * dq_m0)) / (2 * eps)
+    g_m0 = float((lam0 * dq_m0).sum())
+    print(f"[adjoint fd] m0: reverse {g_m0:.10e} central {fd:.10e}", flush=True)
+    assert g_m0 == pytest.approx(fd, rel=1e-4)
 
 
 def test_autograd_rollout(dev):
