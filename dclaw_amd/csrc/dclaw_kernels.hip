@@ -2127,6 +2127,8 @@ static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int ma
     if (maux < 6) return fail(DCLAW_EINVAL, "the host drop-in needs maux >= 6");
     std::lock_guard<std::mutex> lk(g_host.mu);
     if (!g_host.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
+    if (g_host.params.entrainment && maux < 7)
+        return fail(DCLAW_EINVAL, "entrainment needs maux >= 7 (aux(7) holds h_e)");
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
     const size_t na = (size_t)maux * (mx + 4) * (my + 4);
     if ((rc = grow(&g_host.d_qa, &g_host.cap_qa, nq))) return rc;
@@ -2188,6 +2190,8 @@ static int check_host_stage(int mbc, int meqn, int mx, int my, int maux, bool ne
     if (meqn != DCLAW_MEQN) return fail(DCLAW_EINVAL, "meqn must be 7");
     if (int rc = check_stage(mx, my, maux, need_taudir)) return rc;
     if (!g_host.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
+    if (g_host.params.entrainment && maux < 7)
+        return fail(DCLAW_EINVAL, "entrainment needs maux >= 7 (aux(7) holds h_e)");
     return DCLAW_OK;
 }
 
@@ -2247,15 +2251,14 @@ static int src2_host(int meqn, int mbc, int mx, int my, double* q, int maux, con
 {
     std::lock_guard<std::mutex> lk(g_host.mu);
     if (int rc = check_host_stage(mbc, meqn, mx, my, maux, false)) return rc;
-    if (int rc = host_upload(q, nullptr, mx, my, maux)) return rc;
+    // aux travels with q: with entrainment on src2 reads aux(7) = h_e
+    // (src2.f90:253-263), and the device buffer may hold another patch's aux
+    if (int rc = host_upload(q, aux, mx, my, maux)) return rc;
     double scal[2] = {dt, 0.0};
     HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
-    // src2 reads no aux field on this path (bed_normal, curvature and
-    // entrainment off); the device buffer only satisfies the contract
     if (int rc = dclaw_src2_dev(g_host.d_qa, g_host.d_aux, maux, mx, my, g_host.d_scalars,
                                 g_host.d_params, g_host.d_nan, nullptr))
         return rc;
-    (void)aux;
     HIP_TRY(hipDeviceSynchronize());
     host_nan_stop(q, mx, my, t, 2);
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);

@@ -411,6 +411,7 @@ class Ensemble:
         self.dy = [m.patch.dy for m in members]
         self._patches = [self._patch_table(False, device), self._patch_table(True, device)]
         self.flip = False
+        self._plain_flags = None
 
     def _patch_table(self, flipped: bool, device) -> torch.Tensor:
         recs = []
@@ -435,16 +436,25 @@ class Ensemble:
         self.flip = not self.flip
 
     def step_plain(self) -> None:
-        """The plain per-member loop: one bound + stepgrid launch pair per member
-        (the reference's one-stepgrid-call-per-patch structure), for comparison."""
+        """The plain per-member loop, for comparison: one single-patch step per
+        member, the reference's one stepgrid call per patch (AMRClaw advanc).
+        Each call is the step PatchRunner takes: boundary conditions on load,
+        src2 in a second pass over the wet tiles, the previous step's wet
+        tiles dispatched first (split mode "ordered", each member with its own
+        tile-flag workspace), at the library's single-patch segment height
+        (dclaw_set_seg_rows; the caller tunes it for this loop)."""
+        if self._plain_flags is None:
+            w = _lib.lib().dclaw_strip_width()
+            self._plain_flags = [
+                torch.zeros(3 * (-(-self.mx[k] // w)) * self.my[k] + 4, dtype=torch.int32,
+                            device=self.dt.device) for k in range(self.n)]
         for k in range(self.n):
             qi = self.current(k)
             qo = self.qa[k] if self.flip else self.qb[k]
-            bc_fill(qi, self.mx[k], self.my[k], self.bc)
-            stepgrid(qi, qo, self.bed[k], self.maux[k], self.mx[k], self.my[k], self.dx[k],
-                     self.dy[k],
-                     self.dt[k:k + 1], self.params[k * self._prec:(k + 1) * self._prec],
-                     self.cfl[k:k + 1])
+            stepgrid_split(qi, qo, self.bed[k], self.maux[k], self.mx[k], self.my[k], self.dx[k],
+                           self.dy[k], self.bc, self.dt[k:k + 1],
+                           self.params[k * self._prec:(k + 1) * self._prec], self.cfl[k:k + 1],
+                           self._plain_flags[k], "ordered")
         self.flip = not self.flip
 
     def run(self, nsteps: int, dt0, cfl_desired: float, dt_max: float,

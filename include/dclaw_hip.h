@@ -9,9 +9,14 @@
  * 1-mbc:my+mbc) and aux(maux, same), Fortran column-major, component
  * fastest (rpn2_dclaw.f:67-74, src2.f90:47-49).  q components: 1 h, 2 hu,
  * 3 hv, 4 hm, 5 pb, 6 hchi, 7 bdif (digclaw_module.f90:96-102).  aux(1) is
- * the bed b; the device entry points read only aux(1) (stride maux), since
- * phi comes from the parameter record and fsphi/taudir are recomputed inside
- * the step exactly as b4step2 would (b4step2.f90:111).  mbc must be 2.
+ * the bed b; the stepping entry points read aux(1) (stride maux), since phi
+ * comes from the parameter record and fsphi/taudir are recomputed inside the
+ * step exactly as b4step2 would (b4step2.f90:111).  With entrainment on they
+ * also read aux(7) = h_e, the erodible depth (src2.f90:253-263), so such
+ * calls need maux >= 7: the device entry points receive the parameters as a
+ * device record and cannot check this, the caller must (the host drop-ins,
+ * which hold a host copy of the parameters, fail with DCLAW_EINVAL).  mbc
+ * must be 2.
  *
  * All pointers passed to the device entry points are device (HBM) pointers;
  * all take an explicit hipStream_t (passed as void*).  Every call returns
@@ -71,7 +76,8 @@ typedef struct dclaw_params {
 typedef struct dclaw_patch {
     const double* q_in;         /* q(7, mx+4, my+4): state at t_n, ghosts filled */
     double* q_out;              /* q(7, mx+4, my+4): state at t_n+dt (may not alias q_in) */
-    const double* aux;          /* aux(maux, mx+4, my+4); only aux(1) = b is read */
+    const double* aux;          /* aux(maux, mx+4, my+4); aux(1) = b is read, and
+                                   aux(7) = h_e with entrainment on (maux >= 7) */
     int mx, my, maux, member;   /* member indexes params[], dt[], cfl[] */
     double dx, dy;
 } dclaw_patch;
@@ -380,8 +386,8 @@ int dclaw_step2_dev(const double* q, const double* aux, int maux, int mx, int my
                     double* fp, double* gm, double* gp, double* cfl_dev, void* stream);
 
 /* src2 (src2.f90:27-306) in place on the interior cells; check4nans (:73)
- * first when first_nan_dev is not NULL.  aux is not read on this path
- * (bed_normal, curvature and entrainment off). */
+ * first when first_nan_dev is not NULL.  aux is read only with entrainment
+ * on, as aux(7) = h_e (src2.f90:253-263; maux >= 7 then). */
 int dclaw_src2_dev(double* q, const double* aux, int maux, int mx, int my, const double* dt_dev,
                    const dclaw_params* params_dev, int* first_nan_dev, void* stream);
 

@@ -39,7 +39,7 @@ def rel_linf_wet(a: np.ndarray, b: np.ndarray, drytol: float) -> float:
     if not wet.any():
         return float(np.abs(a - b).max())
     worst = 0.0
-    for m in range(6):
+    for m in range(a.shape[0]):
         bb = b[m][wet]
         scale = np.abs(bb).max()
         d = np.abs(a[m][wet] - bb).max()

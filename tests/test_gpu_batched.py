@@ -67,6 +67,39 @@ def test_batched_equals_plain_loop(dev):
     assert dev.equal(a.dt, b.dt)
 
 
+def test_c2_ensemble_256(dev, oracle_mod):
+    """The 256-member ensemble (sides log-uniform in [64, 512], per-member
+    fractal bed, Gaussian source, phi, m0, kref): the smallest and the largest
+    member equal the oracle bit for bit after 25 batched steps, and every
+    member of the batched run equals its own single-patch steps (the plain
+    per-member loop) bit for bit, dt included."""
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    O = oracle_mod
+    members = W.ensemble(256, 64, 512, seed=1)
+    nsteps = 25
+    cells = [m.patch.mx * m.patch.my for m in members]
+    print(f"[c2] 256 members, {sum(cells)} cells", flush=True)
+    a = S.Ensemble(members, seg_rows=16)
+    b = S.Ensemble(members, seg_rows=16)
+    a.run(nsteps, 1e-3, 0.9, 1e9)
+    b.run(nsteps, 1e-3, 0.9, 1e9, plain=True)
+    dev.cuda.synchronize()
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    for k in range(len(members)):
+        assert np.array_equal(a.q(k)[I], b.q(k)[I]), k
+    assert dev.equal(a.dt, b.dt)
+    for k in (int(np.argmin(cells)), int(np.argmax(cells))):
+        pt = members[k].patch
+        print(f"[c2] member {k}: {pt.mx}x{pt.my} against the oracle", flush=True)
+        qo, _, dto, _, _ = O.run(O.params_from(pt.params), pt.q, pt.aux, pt.dx, pt.dy, nsteps,
+                                 1e-3, 0.9, 1e9)
+        qg = a.q(k)
+        assert np.array_equal(qg[I], qo[I]), k
+        assert a.dt[k].item() == dto, k
+        assert (qo[0][2:-2, 2:-2] > pt.params.dry_tolerance).sum() > 100
+
+
 def test_fused_bc_equals_bound_then_step(dev):
     from dclaw_amd import step as S
     pt = slope_patch(90, 70, slope=0.8, phi_deg=25.0, seed=2)

@@ -167,3 +167,50 @@ def test_host_stage_dropins(dev, oracle_mod):
     L.src2_(ci(7), ci(2), ci(mx), ci(my), cd(0.0), cd(0.0), cd(pt.dx), cd(pt.dy),
             q2.ctypes.data_as(dp), ci(maux), ah.ctypes.data_as(dp), cd(0.0), cd(dt))
     assert np.array_equal(q2, O.src2(P, qo, auxo, dt))
+
+
+def test_host_src2_entrainment_reads_its_own_aux(dev, oracle_mod):
+    """src2_ with entrainment on reads aux(7) = h_e of the patch it is given,
+    also right after b4step2_/step2_ ran on another patch (the library's
+    device aux buffer then holds that patch's aux)."""
+    from dclaw_amd import _lib
+    from dclaw_amd import workloads as W
+    from dclaw_amd.params import to_c
+    O = oracle_mod
+    L = _lib.lib()
+    mx, my = 48, 36
+    pt, q, aux, dt = _state(O, mx, my, 21)
+    rng = np.random.default_rng(8)
+    pe = W.make_patch(np.zeros((mx, my)), np.zeros((mx, my)), 0.6, 0.8, pt.params, 1.0, 1.0,
+                      maux=7, h_e=rng.uniform(0.05, 0.6, (mx, my)))
+    aux7 = np.array(np.concatenate([aux, pe.aux[6:7]], axis=0), order="F")
+    pt.params.entrainment, pt.params.entrainment_method = 1, 0
+    pt.params.entrainment_rate = 0.3
+    P = O.params_from(pt.params)
+    _lib.check(L.dclaw_set_params(ctypes.byref(to_c(pt.params))))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    # another (larger) patch through b4step2_ and step2_ first: its aux is
+    # what the device buffer holds when src2_ is called
+    ox, oy = 70, 52
+    po, qo_, auxo_, dto = _state(O, ox, oy, 22)
+    auxo7 = np.array(np.concatenate([auxo_, np.full((1, ox + 4, oy + 4), 5.0)], axis=0),
+                     order="F")
+    qh = np.array(qo_, order="F", copy=True)
+    print(f"[host src2 entrainment] {mx}x{my} after {ox}x{oy}", flush=True)
+    L.b4step2_(ci(2), ci(ox), ci(oy), ci(7), qh.ctypes.data_as(dp), cd(0.0), cd(0.0),
+               cd(po.dx), cd(po.dy), cd(0.0), cd(dto), ci(7), auxo7.ctypes.data_as(dp), ci(1))
+    f = [np.zeros_like(qh, order="F") for _ in range(4)]
+    cfl = ctypes.c_double(0.0)
+    L.step2_(ci(max(ox, oy)), ci(7), ci(7), ci(2), ci(ox), ci(oy), qh.ctypes.data_as(dp),
+             auxo7.ctypes.data_as(dp), cd(po.dx), cd(po.dy), cd(dto), ctypes.byref(cfl),
+             *[a.ctypes.data_as(dp) for a in f], None, None)
+    qb, auxb = O.b4step2(P, q, aux7, pt.dx, pt.dy)
+    q2 = np.array(qb, order="F", copy=True)
+    L.src2_(ci(7), ci(2), ci(mx), ci(my), cd(0.0), cd(0.0), cd(pt.dx), cd(pt.dy),
+            q2.ctypes.data_as(dp), ci(7), np.asfortranarray(auxb).ctypes.data_as(dp), cd(0.0),
+            cd(dt))
+    want = O.src2(P, qb, auxb, dt)
+    assert np.array_equal(q2, want)
+    assert not np.array_equal(want[6], qb[6])   # the erodible layer was used
