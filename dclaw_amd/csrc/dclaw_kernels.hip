@@ -1819,6 +1819,63 @@ int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* au
                            cfl_dev, bc, (hipStream_t)stream, tile_flags_dev, n_flags);
 }
 
+}  // extern "C"
+
+// Tile-flag workspace of dclaw_stepgrid_split_static_dev: a fixed array in
+// the library's own device memory instead of a caller workspace, for callers
+// that cannot allocate (graph capture, serving loops).  Each GPU the library
+// runs on holds its own copy until the process ends; calls that overlap on
+// one GPU share it and race.  The host never names the array in a kernel
+// argument: it resolves the address once per device.
+constexpr int STATIC_FLAG_INTS = 1 << 18;    // 1 MiB
+__device__ __align__(256) int scratch[STATIC_FLAG_INTS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device.
+int static_scratch(int** out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(DCLAW_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<int*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return DCLAW_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int dclaw_stepgrid_split_static_dev(const double* q_in, double* q_out, const double* aux,
+                                    int maux, int mx, int my, double dx, double dy,
+                                    const int* bc, const double* dt_dev,
+                                    const dclaw_params* params_dev, double* cfl_dev,
+                                    void* stream)
+{
+    const int need = dclaw_split_workspace_size(mx, my);
+    if (need < 0) return need;
+    if (need > STATIC_FLAG_INTS)
+        return fail(DCLAW_EINVAL, "patch has more tiles than the static workspace holds ("
+                                  "dclaw_split_workspace_size > " +
+                                      std::to_string(STATIC_FLAG_INTS) +
+                                      " ints): use dclaw_stepgrid_split_dev");
+    int* flags = nullptr;
+    if (int rc = static_scratch(&flags)) return rc;
+    return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
+                           cfl_dev, bc, (hipStream_t)stream, flags, need);
+}
+
+int dclaw_static_workspace_ints(void) { return STATIC_FLAG_INTS; }
+
 int dclaw_worklist_size(const int* mx, const int* my, int npatch, int seg_rows)
 {
     if (!mx || !my || npatch < 0 || seg_rows < 1) return fail(DCLAW_EINVAL, "bad work-list args");

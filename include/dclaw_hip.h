@@ -196,6 +196,27 @@ int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* au
                              const double* dt_dev, const dclaw_params* params_dev,
                              double* cfl_dev, int* tile_flags_dev, int n_flags, void* stream);
 
+/* The split step of dclaw_stepgrid_split_dev with the tile-flag workspace
+ * taken from a static array in the library's device memory instead of a
+ * caller buffer: no allocation and no workspace argument.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Limit: dclaw_split_workspace_size(mx, my) (at the current
+ * segment height) must not exceed dclaw_static_workspace_ints() (262144
+ * ints, 1 MiB); larger patches fail with DCLAW_EINVAL.  Each GPU holds one
+ * such array until the process ends and every call on that GPU shares it,
+ * so calls that overlap on one GPU (concurrent streams or threads) race;
+ * the array keeps the last call's flags, which only reorder the next call's
+ * tiles (results never depend on them). */
+int dclaw_stepgrid_split_static_dev(const double* q_in, double* q_out, const double* aux,
+                                    int maux, int mx, int my, double dx, double dy,
+                                    const int* bc, const double* dt_dev,
+                                    const dclaw_params* params_dev, double* cfl_dev,
+                                    void* stream);
+
+/* Ints of the static workspace of dclaw_stepgrid_split_static_dev. */
+int dclaw_static_workspace_ints(void);
+
 /* Batched fused step over npatch patches (device array of dclaw_patch).
  * One launch; the work list of (patch, strip, segment) tiles is built on
  * the host by dclaw_worklist_size / dclaw_worklist_build into device memory.
