@@ -15,8 +15,12 @@
  * that is not part of the reference tree (step2, the inline MC limiter,
  * stepgrid's update, bc2amr extrap/wall, the dt rule) is restated from
  * Clawpack 5's published algorithm; those parts are pinned by invariants
- * (conservation, lake at rest, symmetry, closed forms), not by reference
- * source.
+ * (conservation, lake at rest, symmetry, closed forms, observed order of
+ * accuracy), not by reference source.  The physics of every routine is
+ * pinned by consequences of the governing equations (tests/test_oracle_*.py),
+ * and tests/test_oracle_mutants.py checks that those pins reject altered
+ * copies of this file.  Not restated (parity unpinned, not on the product
+ * path): mp_update_FE_4quad (src2method 2) and the curvature term of src2.
  */
 #include <math.h>
 #include <stdlib.h>

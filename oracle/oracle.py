@@ -77,7 +77,7 @@ class libm:
     portable elementary functions shared with the GPU) or "glibc"."""
 
     def __init__(self, variant: str):
-        assert variant in ("dcmath", "glibc")
+        assert variant in ("dcmath", "glibc") or os.path.isabs(variant)
         self.variant = variant
 
     def __enter__(self):
@@ -91,9 +91,16 @@ class libm:
         _variant = self.prev
 
 
+def library(path: str) -> libm:
+    """Context manager running the oracle functions of this module against
+    another build of the oracle source at `path` (an absolute .so path): the
+    mutation tests use it to check that the pins reject altered copies."""
+    return libm(os.path.abspath(path))
+
+
 def lib():
     if _variant not in _libs:
-        path = LIB_PATH if _variant == "dcmath" else LIB_PATH_GLIBC
+        path = {"dcmath": LIB_PATH, "glibc": LIB_PATH_GLIBC}.get(_variant, _variant)
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
