@@ -1,22 +1,30 @@
 """Benchmark of the gfx950 D-Claw step.
 
-Default workload: the plain drop-in GPU step at scale -- one 4096x4096 patch,
-dx = 2 m, seeded fractal bed (spectral slope -2, 200 m relief) on a 20 deg
-tilt, 64 seeded elliptical sources, D-Claw material parameters of the
-200x200 case; FP64.  A step is one device-resident pass of the hot path:
-the physical boundary conditions applied on load (bc2amr extrap), stepgrid
-(b4step2, x/y sweeps with rpn2/rpt2 and MC limiter, update, src2; src2 as a
-second pass over the wet tiles unless --fused, optionally after a separate
-dry pass) and the device dt rule.  Segment height and dry pass are picked by
-the autotuner (untimed) unless given.
+Default workload (the metric's own): batched forward + adjoint of an
+ensemble of 64 members of 1024 x 1024 cells (dx = 2 m), each with its own
+seeded fractal bed on a 20 deg tilt, one Gaussian source mass and perturbed
+phi ~ U[32,42] deg, m0 ~ U[0.55,0.64], log10 kref ~ U[-11,-9]; FP64.  A step
+is one batched forward step (all members in one launch per pass: boundary
+conditions on load, stepgrid, src2 over the wet tiles, each member's dt
+rule) plus one batched adjoint step (lambda_n = J^T lambda_{n+1} and the
+gradient with respect to phi and kref of every member, at its state; lambda
+a seeded random field).  The same pair is timed as the plain per-member
+loop (one single-patch forward step and one single-patch adjoint step per
+member, the reference's one stepgrid call per patch); `batched_over_plain`
+is the ratio.
 
-Prints one JSON line (rank 0) with the throughput in interior cell-updates
-per second, the roofline of the step's kernels (HBM, algorithmic bytes 120 B
-per cell-update: 7 q in + 1 bed in + 7 q out, doubles) timed with HIP events
-on the stream the kernels run on, the measured HBM traffic of one step from
-the committed counter summary, and a CPU baseline: the oracle (scalar C port
-of the reference algorithm, one core) on a bounded sample of the same
-workload.  --workload ensemble runs the batched 256-member step instead.
+Prints one JSON line with the throughput in interior cell-updates per second
+of the pair (every interior cell of every member), the HBM roofline of the
+pair (algorithmic bytes 296 B per ACTIVE cell-update: cells within 3 cells
+of a wet cell, the only cells whose state can change; 120 B forward + 176 B
+adjoint), timed with HIP events on the stream the kernels run on, the
+measured HBM traffic from the committed counter summary when it was taken
+on this configuration, and a CPU baseline (the oracle, a scalar C port of
+the reference algorithm, forward step only, OpenMP over members on the
+host's cores and on one core).  The 4096^2 single-patch forward step is
+timed beside it (`c1_forward`).  Other workloads: --workload patch (4096^2
+forward), ensemble (256 members, sides 64-512), wet (fully wet plane),
+flume (2000 x 40 gradient).
 """
 from __future__ import annotations
 
@@ -33,19 +41,24 @@ METRIC = "cell-updates/s fwd+adjoint FP64, batched vs plain GPU, at 1e-9 rel vs 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_CELL_FWD = 8 * (7 + 1 + 7)
+BYTES_PER_CELL_ADJ = 8 * (7 + 1 + 7 + 7)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 20 for the batched fwd+adjoint workload, 200 "
+                         "otherwise)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 5 / 10)")
     ap.add_argument("--n", type=int, default=4096, help="patch side (cells)")
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--workload", choices=["patch", "ensemble", "wet", "flume"], default="patch",
-                    help="patch: one 4096^2 patch (default); ensemble: 256 members with sides "
+    ap.add_argument("--workload", choices=["c4", "patch", "ensemble", "wet", "flume"],
+                    default="c4",
+                    help="c4: batched forward + adjoint of 64 members of 1024^2 against the "
+                         "plain per-member loop (default); patch: one 4096^2 patch; ensemble: 256 members with sides "
                          "log-uniform in [64, 512], batched launch vs plain per-member loop; "
                          "wet: a fully wet --wet-n^2 inclined plane (the FP64-bound regime); "
                          "flume: 2000 x 40 release, gradient of a gauge misfit w.r.t. phi, kref, "
@@ -55,7 +68,16 @@ def parse():
     ap.add_argument("--ens-list-order", action="store_true",
                     help="ensemble: dispatch the work list in its own order instead of the "
                          "previous step's wet tiles first")
-    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--members", type=int, default=None,
+                    help="ensemble members (default 64 for c4, 256 for ensemble)")
+    ap.add_argument("--c4-n", type=int, default=1024, help="c4: member side (cells)")
+    ap.add_argument("--plain-steps", type=int, default=4,
+                    help="c4: timed steps of the plain per-member loop")
+    ap.add_argument("--c1-steps", type=int, default=100,
+                    help="c4: timed steps of the 4096^2 forward step reported beside it "
+                         "(0 disables)")
+    ap.add_argument("--cpu-members", type=int, default=64,
+                    help="c4: members in the CPU baseline sample (one step each)")
     ap.add_argument("--ens-lo", type=int, default=64, help="ensemble: smallest patch side")
     ap.add_argument("--ens-hi", type=int, default=512, help="ensemble: largest patch side "
                     "(sides log-uniform in [lo, hi]; lo = hi = 1024 with 64 members is the "
@@ -90,6 +112,13 @@ def parse():
 
 def main():
     args = parse()
+    c4 = args.workload == "c4"
+    if args.steps is None:
+        args.steps = 20 if c4 else 200
+    if args.warmup is None:
+        args.warmup = 5 if c4 else 10
+    if args.members is None:
+        args.members = 64 if c4 else 256
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py runs on one GPU only; the path is sharded by "
                                    "member (ensembles) or by strips with a 2-3 cell halo "
@@ -106,6 +135,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     _lib.lib()
+    if c4:
+        return bench_c4(args)
     if args.workload == "ensemble":
         return bench_ensemble(args)
     if args.workload == "flume":
@@ -654,6 +685,277 @@ def bench_ensemble(args):
         "graph": graph,
         "tangent": tangent,
         "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def active_cells(q: "torch.Tensor", drytol: float) -> int:
+    """Interior cells within 3 cells of a wet cell (the cells whose state a
+    step can change: the stencil radius with calc_taudir fused) of a device
+    state of layout (my+4, mx+4, 7)."""
+    import torch
+    wet = (q[2:-2, 2:-2, 0] > drytol).double()[None, None]
+    near = torch.nn.functional.max_pool2d(wet, 7, stride=1, padding=3)
+    return int((near > 0).sum().item())
+
+
+def cpu_baseline_members(members, nthreads, nmembers, nsteps=1):
+    """The oracle's forward step (the reference has no adjoint) on a sample
+    of the members: OpenMP over members on nthreads host threads, and one
+    member on one thread.  Returns (value all cores, value one core, seconds
+    of the sample on all cores)."""
+    from oracle import oracle as O
+    sample = members[:nmembers]
+    prm = [O.params_from(m.patch.params) for m in sample]
+    cells = sum(m.patch.mx * m.patch.my for m in sample) * nsteps
+    with O.libm("omp"):
+        c0 = time.perf_counter()
+        O.run_patches(prm, [m.patch.q for m in sample], [m.patch.aux for m in sample],
+                      [m.patch.dx for m in sample], [m.patch.dy for m in sample], nsteps, 1e-3,
+                      0.9, 1e9, nthreads=nthreads)
+        t_all = time.perf_counter() - c0
+        one = sample[:1]
+        c0 = time.perf_counter()
+        O.run_patches(prm[:1], [one[0].patch.q], [one[0].patch.aux], [one[0].patch.dx],
+                      [one[0].patch.dy], nsteps, 1e-3, 0.9, 1e9, nthreads=1)
+        t_one = time.perf_counter() - c0
+    cells_one = one[0].patch.mx * one[0].patch.my * nsteps
+    return cells / t_all, cells_one / t_one, t_all
+
+
+def c1_forward(args):
+    """The 4096^2 single-patch forward step (plain drop-in step at scale),
+    ordered split step at the tuned segment height; HIP-event time per step."""
+    import torch
+
+    from dclaw_amd import _lib
+    from dclaw_amd import autotune
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    pt = W.fractal_sources(4096, 4096, dx=2.0, seed=args.seed)
+    r = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, split_mode="ordered")
+    r.run(5, 1e-3, 0.9, 1e9)
+    seg, _ = autotune.tune_patch(r, candidates=(12, 16, 20, 24, 28, 32),
+                                 split_modes=("ordered",))
+    r.set_dt(1e-3)
+    r.cfl.zero_()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record()
+    for _ in range(args.c1_steps):
+        r.step()
+        S.dt_update(r.dt, r.cfl, 0.9, 1e9)
+    b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = a.elapsed_time(b) / args.c1_steps
+    cells = r.mx * r.my
+    act = active_cells(r.qa, pt.params.dry_tolerance)
+    out = {"workload": "single 4096x4096 patch, dx=2 m, seeded fractal bed + 20 deg tilt, 64 "
+                       "elliptical sources, forward step (ordered split step)",
+           "steps": args.c1_steps, "ms_per_step": ms, "wall_ms_per_step": wall / args.c1_steps * 1e3,
+           "value": cells / (ms * 1e-3), "unit": "cell-updates/s", "seg_rows": seg,
+           "active_cells_end": act,
+           "active_roofline_frac": act * BYTES_PER_CELL_FWD / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    _lib.check(min(0, _lib.lib().dclaw_set_seg_rows(seg)))
+    del r
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_c4(args):
+    """Batched forward + adjoint of the 64 x 1024^2 ensemble against the plain
+    per-member loop (see the module docstring)."""
+    import torch
+
+    from dclaw_amd import _lib
+    from dclaw_amd import adjoint as AD
+    from dclaw_amd import autotune
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+
+    n = args.c4_n
+    t0 = time.time()
+    members = W.ensemble(args.members, n, n, seed=args.seed)
+    cells = sum(m.patch.mx * m.patch.my for m in members)
+    drytol = members[0].patch.params.dry_tolerance
+    print(f"[bench] c4: {len(members)} members of {n}x{n} built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    seg = args.seg_rows
+    tune_times = None
+    if seg <= 0:
+        seg, tune_times = autotune.tune_ensemble(
+            members, lambda h: S.Ensemble(members, seg_rows=h), candidates=(8, 12, 16, 24, 32),
+            reps=2)
+        tune_times = {h: round(v, 3) for h, v in tune_times.items()}
+        torch.cuda.empty_cache()
+    ens = S.Ensemble(members, seg_rows=seg)
+    adj = AD.EnsembleAdjoint(ens, parameters=("phi", "kref"))
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    lam = [torch.randn(ens.qa[k].shape, dtype=torch.float64, device="cuda", generator=gen)
+           for k in range(ens.n)]
+    lin = [torch.empty_like(x) for x in lam]
+    ens.dt.fill_(1e-3)
+    ens.cfl.zero_()
+    print(f"[bench] c4: segment rows {seg} (tuner {tune_times}), adjoint workspace "
+          f"{sum(w.numel() for w in adj.work) * 8 / 1e9:.1f} GB", file=sys.stderr, flush=True)
+
+    def pair(plain=False, ev=None):
+        qs = [ens.current(k) for k in range(ens.n)]
+        if ev is not None:
+            ev[0].record()
+        if plain:
+            adj.step_plain(qs, lam, lin)
+        else:
+            adj.step(qs, lam, lin)
+        if ev is not None:
+            ev[1].record()
+        if plain:
+            ens.step_plain()
+        else:
+            ens.step()
+        if ev is not None:
+            ev[2].record()
+        S.dt_update(ens.dt, ens.cfl, 0.9, 1e9)
+
+    for _ in range(args.warmup):
+        pair()
+    torch.cuda.synchronize()
+    act0 = sum(active_cells(ens.current(k), drytol) for k in range(ens.n))
+    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        pair(ev=evs[k])
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t_start
+    act1 = sum(active_cells(ens.current(k), drytol) for k in range(ens.n))
+    adj_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
+    fwd_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    pair_gpu_s = (adj_ms + fwd_ms) * 1e-3
+    value = cells * args.steps / t_b
+    print(f"[bench] c4 batched: {t_b / args.steps * 1e3:.2f} ms per pair (adjoint "
+          f"{adj_ms:.2f} ms, forward {fwd_ms:.2f} ms), active cells {act0} -> {act1}",
+          file=sys.stderr, flush=True)
+
+    if args.copy_calib:
+        # one plain 1 GiB device copy: the calibration dispatch of the HBM
+        # counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    # the plain per-member loop from the same state: single-patch steps at
+    # the segment height tuned for that loop
+    L = _lib.lib()
+    plain_tune = {}
+    for h in ((4, 6, 8, 12, 16, 24) if args.plain_steps > 0 else ()):
+        _lib.check(min(0, L.dclaw_set_seg_rows(h)))
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ens.step_plain()
+        a.record()
+        ens.step_plain()
+        b.record()
+        torch.cuda.synchronize()
+        plain_tune[h] = round(a.elapsed_time(b), 3)
+    plain_loop = None
+    if args.plain_steps > 0:
+        hp = min(plain_tune, key=plain_tune.get)
+        _lib.check(min(0, L.dclaw_set_seg_rows(hp)))
+        pair(plain=True)
+        torch.cuda.synchronize()
+        pevs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3))
+                for _ in range(args.plain_steps)]
+        t_start = time.perf_counter()
+        for k in range(args.plain_steps):
+            pair(plain=True, ev=pevs[k])
+        torch.cuda.synchronize()
+        t_p = time.perf_counter() - t_start
+        padj_ms = sum(e[0].elapsed_time(e[1]) for e in pevs) / args.plain_steps
+        pfwd_ms = sum(e[1].elapsed_time(e[2]) for e in pevs) / args.plain_steps
+        plain = cells * args.plain_steps / t_p
+        print(f"[bench] c4 plain loop: {t_p / args.plain_steps * 1e3:.2f} ms per pair (adjoint "
+              f"{padj_ms:.2f} ms, forward {pfwd_ms:.2f} ms at height {hp}; tuner "
+              f"{plain_tune})", file=sys.stderr, flush=True)
+        plain_loop = {"value": plain, "steps": args.plain_steps,
+                      "ms_per_step": t_p / args.plain_steps * 1e3, "adjoint_ms": padj_ms,
+                      "forward_ms": pfwd_ms, "seg_rows": hp, "tuner_ms": plain_tune,
+                      "forward": "single-patch ordered split step per member",
+                      "adjoint": "single-patch adjoint step per member"}
+
+    act = 0.5 * (act0 + act1)
+    achieved = act * (BYTES_PER_CELL_FWD + BYTES_PER_CELL_ADJ) / pair_gpu_s / 1e9
+    run_cfg = f"--c4-n {n} --members {len(members)} --seg-rows {seg}"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"fwd_adjoint_batched_c4_{len(members)}x{n}_traffic.txt")
+    if os.path.exists(tpath) and run_cfg in open(tpath).read():
+        for line in open(tpath).read().splitlines():
+            if line.startswith("traffic_bytes_per_step"):
+                traffic = float(line.split("=")[1])
+    del adj, lam, lin
+    torch.cuda.empty_cache()
+
+    c1 = c1_forward(args) if args.c1_steps > 0 else None
+    if c1:
+        print(f"[bench] c1 forward {c1['ms_per_step']:.3f} ms per step", file=sys.stderr,
+              flush=True)
+
+    cpu = None
+    if args.cpu_steps > 0:
+        nthr = max(1, min(16, os.cpu_count() or 1))
+        v_all, v_one, t_all = cpu_baseline_members(members, nthr, min(args.cpu_members,
+                                                                        len(members)))
+        model = ""
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = {"value": v_all, "unit": "cell-updates/s", "cores": nthr, "kind": "port",
+               "value_one_core": v_one, "cpu_model": model, "nproc": os.cpu_count(),
+               "sample": f"forward step only (the reference has no adjoint): one step of "
+                         f"{min(args.cpu_members, len(members))} members of {n}x{n} from their "
+                         f"initial states, oracle (scalar C port, gcc -O2) with OpenMP over "
+                         f"members on {nthr} threads ({t_all:.1f}s); one core: one member"}
+        print(f"[bench] cpu baseline {v_all:.3e} cell-updates/s on {nthr} threads, "
+              f"{v_one:.3e} on one", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_b / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched forward + adjoint, {len(members)} members of {n}x{n}, "
+                               f"dx=2 m, per-member seeded fractal bed + 20 deg tilt, one "
+                               f"Gaussian source, phi/m0/kref perturbed; step = one batched "
+                               f"forward step + one batched adjoint step (lambda_n and "
+                               f"d/d(phi, kref) per member, lambda a seeded random field)",
+                   "members": len(members), "side": n, "cells": cells, "seed": args.seed,
+                   "seg_rows": seg, "active_cells_start": act0, "active_cells_end": act1},
+        "batched": {"ms_per_step": t_b / args.steps * 1e3, "adjoint_ms": adj_ms,
+                    "forward_ms": fwd_ms},
+        "plain_loop": plain_loop,
+        "batched_over_plain": value / plain_loop["value"] if plain_loop else None,
+        "forward_batched_over_plain": plain_loop["forward_ms"] / fwd_ms if plain_loop else None,
+        "adjoint_batched_over_plain": plain_loop["adjoint_ms"] / adj_ms if plain_loop else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_active_cell": BYTES_PER_CELL_FWD + BYTES_PER_CELL_ADJ,
+                     "active_cells": act,
+                     "kernel": "one batched pair: dclaw_adjoint_batched (k_adjb_*) + "
+                               "dclaw_step_batched_ws (k_order_tiles, k_step_batched<1>, "
+                               "k_src2_batched)",
+                     "kernel_avg_ms": adj_ms + fwd_ms},
+        "c1_forward": c1,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

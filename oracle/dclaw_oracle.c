@@ -1270,7 +1270,7 @@ int oracle_run(oracle_params* P, int mx, int my, int mbc, int maux, double dx, d
                double cfl_desired, double dt_max, double* cfl_hist, double* dt_hist,
                int nthreads)
 {
-    (void)nthreads;
+    (void)nthreads;   /* one patch runs on one thread; oracle_run_patches spreads patches */
     const size_t ncell = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);
     double* work = (double*)malloc(sizeof(double) * 4 * MEQN * ncell);
     double dt = *dt_inout;
@@ -1286,6 +1286,37 @@ int oracle_run(oracle_params* P, int mx, int my, int mbc, int maux, double dx, d
     *dt_inout = dt;
     free(work);
     return n;
+}
+
+/* Many patches, each with its own parameter record and dt rule, advanced
+ * nsteps each: the reference's OpenMP over patches (AMRClaw advanc runs
+ * stepgrid for the grids of a level in an OpenMP loop with a dynamic
+ * schedule, update.f90:50-60 and amr2.f90:592 name the same structure).
+ * Built with OpenMP only in the timing variant of the library; elsewhere the
+ * loop runs on one thread.  Patches are independent here (ensemble members
+ * or patches with fixed ghost rules), so no ghost exchange is needed. */
+int oracle_run_patches(oracle_params* P, int npatch, const int* mx, const int* my, int mbc,
+                       const int* maux, const double* dx, const double* dy, const int* bc,
+                       double** q, double** aux, int nsteps, double* dt_inout,
+                       double cfl_desired, double dt_max, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (int k = 0; k < npatch; k++)
+        oracle_run(&P[k], mx[k], my[k], mbc, maux[k], dx[k], dy[k], bc, q[k], aux[k], nsteps,
+                   &dt_inout[k], cfl_desired, dt_max, NULL, NULL, 1);
+    return npatch;
+}
+
+int oracle_openmp_threads(void)
+{
+#ifdef _OPENMP
+    return 1 << 20;
+#else
+    return 1;
+#endif
 }
 
 void oracle_dt_rule(double dt_used, double cfl, double t_before, double cfl_desired,

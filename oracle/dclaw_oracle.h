@@ -94,6 +94,16 @@ int oracle_run(oracle_params* p, int mx, int my, int mbc, int maux, double dx,
                double* dt_inout, double cfl_desired, double dt_max,
                double* cfl_hist, double* dt_hist, int nthreads);
 
+/* nsteps of oracle_run on each of npatch independent patches (P[k],
+ * mx[k], ...; per-patch dt in dt_inout[k]) over nthreads OpenMP threads
+ * (dynamic schedule) when the library is built with OpenMP. */
+int oracle_run_patches(oracle_params* P, int npatch, const int* mx, const int* my, int mbc,
+                       const int* maux, const double* dx, const double* dy, const int* bc,
+                       double** q, double** aux, int nsteps, double* dt_inout,
+                       double cfl_desired, double dt_max, int nthreads);
+/* > 1 if this build runs oracle_run_patches on several threads. */
+int oracle_openmp_threads(void);
+
 /* Time loop pieces (device-resident loop of the build, restated):
  * oracle_dt_rule: the dt rule after a step of length dt_used taken at time
  * t_before with CFL number cfl: t_after = t_before + dt_used;

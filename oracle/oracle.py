@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libdclaw_oracle.so")
 LIB_PATH_GLIBC = os.path.join(HERE, "_build", "libdclaw_oracle_glibc.so")
+LIB_PATH_OMP = os.path.join(HERE, "_build", "libdclaw_oracle_omp.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -77,7 +78,7 @@ class libm:
     portable elementary functions shared with the GPU) or "glibc"."""
 
     def __init__(self, variant: str):
-        assert variant in ("dcmath", "glibc") or os.path.isabs(variant)
+        assert variant in ("dcmath", "glibc", "omp") or os.path.isabs(variant)
         self.variant = variant
 
     def __enter__(self):
@@ -100,7 +101,8 @@ def library(path: str) -> libm:
 
 def lib():
     if _variant not in _libs:
-        path = {"dcmath": LIB_PATH, "glibc": LIB_PATH_GLIBC}.get(_variant, _variant)
+        path = {"dcmath": LIB_PATH, "glibc": LIB_PATH_GLIBC, "omp": LIB_PATH_OMP}.get(_variant,
+                                                                              _variant)
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
@@ -133,6 +135,13 @@ def lib():
                                  ctypes.c_double, ctypes.c_double, _ip, _dp, _dp, ctypes.c_int,
                                  _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
         L.oracle_run.restype = ctypes.c_int
+        if hasattr(L, "oracle_run_patches"):
+            _pp = ctypes.POINTER(_dp)
+            L.oracle_run_patches.argtypes = [P, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip, _dp, _dp,
+                                             _ip, _pp, _pp, ctypes.c_int, _dp, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int]
+            L.oracle_run_patches.restype = ctypes.c_int
+            L.oracle_openmp_threads.restype = ctypes.c_int
         L.oracle_dt_rule.argtypes = [ctypes.c_double] * 6 + [_dp, _dp, _ip]
         L.oracle_gauges.argtypes = [ctypes.c_int] * 4 + [_dp, _dp] + [ctypes.c_double] * 5 + [
             _dp, _dp, ctypes.c_int, _dp]
@@ -317,6 +326,31 @@ def run(params, q, aux, dx, dy, nsteps, dt0, cfl_desired, dt_max, bc=(1, 1, 1, 1
                      ctypes.cast(bcs, _ip), _ptr(q), _ptr(aux), nsteps, ctypes.byref(dt),
                      cfl_desired, dt_max, _ptr(cfl_hist), _ptr(dt_hist), 1)
     return q, aux, dt.value, cfl_hist[:nsteps], dt_hist[:nsteps]
+
+
+def run_patches(params_list, qs, auxs, dxs, dys, nsteps, dt0, cfl_desired, dt_max,
+                bc=(1, 1, 1, 1), nthreads=1, mbc=2):
+    """nsteps of ``run`` on each of several independent patches at once, over
+    nthreads OpenMP threads (the "omp" build of the library; the patches are
+    spread with a dynamic schedule, as AMRClaw spreads grids).  Returns the
+    final states and the next dt of each patch."""
+    n = len(qs)
+    qs = [np.array(q, dtype=np.float64, order="F", copy=True) for q in qs]
+    auxs = [np.array(a, dtype=np.float64, order="F", copy=True) for a in auxs]
+    prm = (OracleParams * n)(*params_list)
+    mx = (ctypes.c_int * n)(*[q.shape[1] - 2 * mbc for q in qs])
+    my = (ctypes.c_int * n)(*[q.shape[2] - 2 * mbc for q in qs])
+    maux = (ctypes.c_int * n)(*[a.shape[0] for a in auxs])
+    dx = np.array(dxs, dtype=np.float64)
+    dy = np.array(dys, dtype=np.float64)
+    dt = np.full(n, float(dt0))
+    qp = (_dp * n)(*[_ptr(q) for q in qs])
+    ap = (_dp * n)(*[_ptr(a) for a in auxs])
+    bcs = (ctypes.c_int * 4)(*bc)
+    lib().oracle_run_patches(prm, n, ctypes.cast(mx, _ip), ctypes.cast(my, _ip), mbc,
+                             ctypes.cast(maux, _ip), _ptr(dx), _ptr(dy), ctypes.cast(bcs, _ip),
+                             qp, ap, nsteps, _ptr(dt), cfl_desired, dt_max, int(nthreads))
+    return qs, dt
 
 
 def dt_rule(dt_used, cfl, t_before, cfl_desired, dt_max, tout=float("inf")):

@@ -86,3 +86,20 @@ def test_gauges_1d_flume_convention(oracle_mod):
     q[0, 6 + 1, 2 + 1] = 0.0
     out = O.gauges(q, aux, 0.0, 0.0, 1.0, 1.0, 1e-3, xg, yg)
     assert np.array_equal(out[0, :7], q[:, 7 + 1, 2 + 1])
+
+
+def test_run_patches_equals_run_per_patch(oracle_mod):
+    """The OpenMP-over-patches loop of the CPU baseline (the "omp" build)
+    gives each patch exactly the run it gets alone."""
+    from dclaw_amd import workloads as W
+    O = oracle_mod
+    ms = W.ensemble(6, 24, 60, seed=3)
+    P = [O.params_from(m.patch.params) for m in ms]
+    with O.libm("omp"):
+        qs, dts = O.run_patches(P, [m.patch.q for m in ms], [m.patch.aux for m in ms],
+                                [m.patch.dx for m in ms], [m.patch.dy for m in ms], 4, 1e-3,
+                                0.9, 1e9, nthreads=4)
+    for k, m in enumerate(ms):
+        q, _, dt, _, _ = O.run(O.params_from(m.patch.params), m.patch.q, m.patch.aux,
+                               m.patch.dx, m.patch.dy, 4, 1e-3, 0.9, 1e9)
+        assert np.array_equal(q, qs[k]) and dt == dts[k], k
