@@ -897,11 +897,36 @@ constexpr int ADJ_BLOCK = 256;    // dense passes
 constexpr int ADJ_GRID = 8192;    // persistent grid of the tile passes
 constexpr int ADJ_GP_PARTS = 256; // partial sums per parameter direction
 
+// The tile passes cut the active list into ADJ_XCDS contiguous chunks and
+// give chunk x to the workgroups b with b % ADJ_XCDS == x.  The dispatcher
+// deals workgroups to the 8 XCDs round-robin by index, so one XCD works on a
+// run of consecutive list entries -- neighbouring tiles, whose halo reads
+// share cache lines -- and they meet in that XCD's L2 instead of being
+// fetched once per XCD.  Each entry is still taken by exactly one workgroup
+// (the grid is a multiple of ADJ_XCDS), so results do not depend on it.
+constexpr int ADJ_XCDS = 8;
+
+__device__ __forceinline__ void adj_span(int n, int& k, int& end, int& step)
+{
+    const int x = (int)(blockIdx.x % ADJ_XCDS);
+    step = (int)(gridDim.x / ADJ_XCDS);
+    k = (int)((long long)n * x / ADJ_XCDS) + (int)(blockIdx.x / ADJ_XCDS);
+    end = (int)((long long)n * (x + 1) / ADJ_XCDS);
+}
+
+// Grid of the tile passes: up to ADJ_GRID workgroups, a multiple of ADJ_XCDS.
+inline unsigned adj_grid(long long ntiles)
+{
+    const long long g = std::min<long long>(ntiles, ADJ_GRID);
+    return (unsigned)std::max<long long>(ADJ_XCDS, (g + ADJ_XCDS - 1) / ADJ_XCDS * ADJ_XCDS);
+}
+
 struct AdjTiles {
     int ntx, nty;
     int* flags;       // ntx * nty, 0 between calls
-    int* list;        // active tiles of the last call
+    int* list;        // active tiles of the last call, in tile order
     int* count;       // [0]: their number
+    int* bcount;      // marked tiles per block of 64 tiles, then their list offsets
 };
 
 // A patch's tile bookkeeping at the end of its workspace: after the stage
@@ -918,7 +943,39 @@ __host__ __device__ inline AdjTiles adj_tiles(double* work, int mx, int my, size
     T.flags = ints;
     T.list = ints + ntiles;
     T.count = ints + 2 * ntiles;
+    T.bcount = ints + 2 * ntiles + 1;
     return T;
+}
+
+// Ordered compaction of marked tiles (the active list comes out in tile
+// order, so a run of list entries is a run of neighbouring tiles):
+// k_*_count_marks counts the marks of each block of 64 tiles, k_scan_blocks
+// turns the counts into list offsets (and writes the total), and the list
+// kernel writes each marked tile at its block's offset plus its rank in the
+// block.
+__global__ __launch_bounds__(1024) void k_scan_blocks(int* a, int n, int* total)
+{
+    __shared__ int sh[1024];
+    const int tid = (int)threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int lo = min(n, tid * per), hi = min(n, lo + per);
+    int sum = 0;
+    for (int k = lo; k < hi; k++) sum += a[k];
+    sh[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += v;
+        __syncthreads();
+    }
+    int run = sh[tid] - sum;
+    for (int k = lo; k < hi; k++) {
+        const int c = a[k];
+        a[k] = run;
+        run += c;
+    }
+    if (tid == 1023) *total = sh[1023];
 }
 
 template <int STAGE>
@@ -1004,8 +1061,9 @@ __global__ __launch_bounds__(64) void k_adjoint(dclaw_adj::AdjArgs A, AdjTiles T
     A.P = *A.Pd;
     A.dt = *A.dt_dev;
     const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
-    const int nact = *T.count;
-    for (int k = blockIdx.x; k < nact; k += gridDim.x) {
+    int k, end, step;
+    adj_span(*T.count, k, end, step);
+    for (; k < end; k += step) {
         int i, j;
         if (adj_tile_cell(A, T, T.list[k], i, j)) adj_stage<STAGE>(A, Pz, i, j);
     }
@@ -1033,28 +1091,27 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint_dense(dclaw_adj::AdjArgs 
     adj_dense_cell(A, T, n);
 }
 
-// Compact the marked tiles into the list (ballot within the wavefront, one
-// counter atomic per wavefront) and reset the marks.  The list's order does
-// not matter: every tile pass writes only its own cells.
+// Marked tiles per block of 64 (pass 1 of the ordered compaction).
+__global__ __launch_bounds__(64) void k_adjoint_count_marks(AdjTiles T, int ntiles)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const unsigned long long b = __ballot(t < ntiles && T.flags[t] != 0);
+    if (threadIdx.x == 0) T.bcount[blockIdx.x] = (int)__popcll(b);
+}
+
+// Write the marked tiles into the list in tile order (T.bcount holds each
+// block's offset) and reset the marks.  Every tile pass writes only its own
+// cells, so the order only affects cache locality.
 __global__ __launch_bounds__(64) void k_adjoint_list(AdjTiles T, int ntiles)
 {
     const int t = blockIdx.x * 64 + threadIdx.x;
     const bool f = t < ntiles && T.flags[t] != 0;
-    if (f) T.flags[t] = 0;
     const unsigned long long b = __ballot(f);
-    if (b == 0ull) return;
-    int base = 0;
-    if (threadIdx.x == 0) base = atomicAdd(T.count, (int)__popcll(b));
-    base = __shfl(base, 0);
     if (f) {
+        T.flags[t] = 0;
         const unsigned long long below = b & ((1ull << threadIdx.x) - 1ull);
-        T.list[base + (int)__popcll(below)] = t;
+        T.list[T.bcount[blockIdx.x] + (int)__popcll(below)] = t;
     }
-}
-
-__global__ void k_adjoint_count_zero(int* count)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) count[0] = 0;
 }
 
 // Sum of the per-cell parameter gradients, in a fixed order: block (b, p)
@@ -1126,8 +1183,9 @@ struct AdjBatch {
     const double* dt;              // per member
     long long* cell_prefix;        // npatch + 1
     long long* tile_prefix;        // npatch + 1
-    int* glist;                    // active (patch, tile) pairs as global tile ids
+    int* glist;                    // active (patch, tile) pairs as global tile ids, in order
     int* gcount;
+    int* bcount;                   // marks per block of 64 global tiles, then list offsets
 };
 
 // Largest m with prefix[m] <= g.
@@ -1242,27 +1300,51 @@ __global__ __launch_bounds__(256) void k_adjb_count_zero(AdjBatch B)
     adj_tiles(p.work, p.mx, p.my, (size_t)(p.mx + 4) * (size_t)(p.my + 4)).count[0] = 0;
 }
 
+// Whether global tile g (< ntiles) is marked, and its patch's bookkeeping.
+__device__ __forceinline__ bool adjb_marked(const AdjBatch& B, long long g, AdjTiles& T, int& t)
+{
+    const int m = adj_find(B.tile_prefix, B.npatch, g);
+    const dclaw_adj_patch& p = B.pt[m];
+    T = adj_tiles(p.work, p.mx, p.my, (size_t)(p.mx + 4) * (size_t)(p.my + 4));
+    t = (int)(g - B.tile_prefix[m]);
+    return T.flags[t] != 0;
+}
+
+// Marked tiles per block of 64 global tiles (pass 1 of the ordered
+// compaction).
+__global__ __launch_bounds__(64) void k_adjb_count_marks(AdjBatch B, long long ntiles)
+{
+    const long long g = (long long)blockIdx.x * 64 + threadIdx.x;
+    AdjTiles T;
+    int t = 0;
+    const bool f = g < ntiles && adjb_marked(B, g, T, t);
+    const unsigned long long b = __ballot(f);
+    if (threadIdx.x == 0) B.bcount[blockIdx.x] = (int)__popcll(b);
+}
+
 // Compaction over every patch's marks: each marked tile goes to its patch's
-// own list (for the next call's clearing) and to the global list.
+// own list (for the next call's clearing; any order) and to the global list,
+// in global tile order (B.bcount holds each block's offset).
 __global__ __launch_bounds__(64) void k_adjb_list(AdjBatch B, long long ntiles)
 {
     const long long g = (long long)blockIdx.x * 64 + threadIdx.x;
-    if (g >= ntiles) return;
-    const int m = adj_find(B.tile_prefix, B.npatch, g);
-    const dclaw_adj_patch& p = B.pt[m];
-    const AdjTiles T = adj_tiles(p.work, p.mx, p.my, (size_t)(p.mx + 4) * (size_t)(p.my + 4));
-    const int t = (int)(g - B.tile_prefix[m]);
-    if (T.flags[t] == 0) return;
+    AdjTiles T;
+    int t = 0;
+    const bool f = g < ntiles && adjb_marked(B, g, T, t);
+    const unsigned long long b = __ballot(f);
+    if (!f) return;
     T.flags[t] = 0;
     T.list[atomicAdd(T.count, 1)] = t;
-    B.glist[atomicAdd(B.gcount, 1)] = (int)g;
+    const unsigned long long below = b & ((1ull << threadIdx.x) - 1ull);
+    B.glist[B.bcount[blockIdx.x] + (int)__popcll(below)] = (int)g;
 }
 
 template <int STAGE>
 __global__ __launch_bounds__(64) void k_adjb(AdjBatch B)
 {
-    const int nact = *B.gcount;
-    for (int k = blockIdx.x; k < nact; k += gridDim.x) {
+    int k, end, step;
+    adj_span(*B.gcount, k, end, step);
+    for (; k < end; k += step) {
         const long long g = B.glist[k];
         const int m = adj_find(B.tile_prefix, B.npatch, g);
         AdjTiles T;
@@ -1278,8 +1360,9 @@ __global__ __launch_bounds__(64) void k_adjb(AdjBatch B)
 template <int STAGE>
 __global__ __launch_bounds__(64, 2) void k_adjb2(AdjBatch B)
 {
-    const int nact = *B.gcount;
-    for (int k = blockIdx.x; k < nact; k += gridDim.x) {
+    int k, end, step;
+    adj_span(*B.gcount, k, end, step);
+    for (; k < end; k += step) {
         const long long g = B.glist[k];
         const int m = adj_find(B.tile_prefix, B.npatch, g);
         AdjTiles T;
@@ -1590,11 +1673,12 @@ size_t dclaw_adjoint_workspace_doubles(int mx, int my)
 {
     if (mx < 1 || my < 1) return 0;
     // the stage planes, the parameter-gradient partial sums, then the tile
-    // marks, the active-tile list and its counter (ints)
+    // marks, the active-tile list, its counter and the per-block counts (ints)
     const size_t ntiles = (size_t)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) *
                           (size_t)((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+    const size_t ints = 2 * ntiles + 1 + (ntiles + 63) / 64;
     return (size_t)dclaw_adj::W_FIELDS * (size_t)(mx + 4) * (size_t)(my + 4) +
-           (size_t)dclaw_adj::MAXP * ADJ_GP_PARTS + (2 * ntiles + 2 + 1) / 2;
+           (size_t)dclaw_adj::MAXP * ADJ_GP_PARTS + (ints + 1) / 2;
 }
 
 int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int mx, int my,
@@ -1642,11 +1726,13 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
     const AdjTiles T = adj_tiles(work, mx, my, A.N);
     const int ntiles = T.ntx * T.nty;
     hipStream_t st = (hipStream_t)stream;
-    const dim3 tg((unsigned)std::min(ntiles, ADJ_GRID)), tb(64);
+    const dim3 tg(adj_grid(ntiles)), tb(64);
     hipLaunchKernelGGL(k_adjoint_dense, dim3((unsigned)nb), dim3(ADJ_BLOCK), 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint_clear_stale, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint_count_zero, dim3(1), dim3(1), 0, st, T.count);
-    hipLaunchKernelGGL(k_adjoint_list, dim3((ntiles + 63) / 64), dim3(64), 0, st, T, ntiles);
+    const int nblk = (ntiles + 63) / 64;
+    hipLaunchKernelGGL(k_adjoint_count_marks, dim3(nblk), dim3(64), 0, st, T, ntiles);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, T.bcount, nblk, T.count);
+    hipLaunchKernelGGL(k_adjoint_list, dim3(nblk), dim3(64), 0, st, T, ntiles);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_PREPARE>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_TAUDIR>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_WAVES>, tg, tb, 0, st, A, T);
@@ -1687,8 +1773,9 @@ size_t dclaw_adjoint_batch_doubles(const dclaw_adj_patch* patches_host, int npat
 {
     if (!patches_host || npatch < 1) return 0;
     const long long t = adj_batch_tiles(patches_host, npatch, nullptr);
-    // cell and tile prefixes, the global counter, then the global list (ints)
-    return (size_t)(2 * (npatch + 1) + 1) + (size_t)((t + 1) / 2);
+    // cell and tile prefixes, the global counter, then the global list and
+    // the per-block counts (ints)
+    return (size_t)(2 * (npatch + 1) + 1) + (size_t)((t + (t + 63) / 64 + 1) / 2);
 }
 
 int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_patch* patches_host,
@@ -1732,14 +1819,18 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     B.tile_prefix = B.cell_prefix + (npatch + 1);
     B.gcount = (int*)(B.tile_prefix + (npatch + 1));
     B.glist = (int*)(batch_work + 2 * (npatch + 1) + 1);
+    B.bcount = B.glist + ntiles;
+    const long long nblk = (ntiles + 63) / 64;
     hipStream_t st = (hipStream_t)stream;
-    const dim3 tg((unsigned)std::min<long long>(ntiles, ADJ_GRID)), tb(64);
+    const dim3 tg(adj_grid(ntiles)), tb(64);
     hipLaunchKernelGGL(k_adjb_prefix, dim3(1), dim3(1024), 0, st, B);
     hipLaunchKernelGGL(k_adjb_dense, dim3((unsigned)((ncells + ADJ_BLOCK - 1) / ADJ_BLOCK)),
                        dim3(ADJ_BLOCK), 0, st, B, ncells);
     hipLaunchKernelGGL(k_adjb_clear_stale, dim3(32, npatch), tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb_count_zero, dim3((npatch + 255) / 256), dim3(256), 0, st, B);
-    hipLaunchKernelGGL(k_adjb_list, dim3((unsigned)((ntiles + 63) / 64)), tb, 0, st, B, ntiles);
+    hipLaunchKernelGGL(k_adjb_count_marks, dim3((unsigned)nblk), tb, 0, st, B, ntiles);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, B.bcount, (int)nblk, B.gcount);
+    hipLaunchKernelGGL(k_adjb_list, dim3((unsigned)nblk), tb, 0, st, B, ntiles);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_PREPARE>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_TAUDIR>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_WAVES>, tg, tb, 0, st, B);
