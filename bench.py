@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--workload", choices=["c4", "patch", "ensemble", "wet", "flume"],
+    ap.add_argument("--workload", choices=["c4", "c4-gradient", "patch", "ensemble", "wet",
+                                           "flume"],
                     default="c4",
                     help="c4: batched forward + adjoint of 64 members of 1024^2 against the "
                          "plain per-member loop (default); patch: one 4096^2 patch; ensemble: 256 members with sides "
@@ -76,6 +77,10 @@ def parse():
     ap.add_argument("--c1-steps", type=int, default=100,
                     help="c4: timed steps of the 4096^2 forward step reported beside it "
                          "(0 disables)")
+    ap.add_argument("--grad-steps", type=int, default=1000,
+                    help="c4-gradient: steps of the checkpointed gradient run")
+    ap.add_argument("--grad-every", type=int, default=50,
+                    help="c4-gradient: block checkpoint interval")
     ap.add_argument("--cpu-members", type=int, default=64,
                     help="c4: members in the CPU baseline sample (one step each)")
     ap.add_argument("--ens-lo", type=int, default=64, help="ensemble: smallest patch side")
@@ -118,7 +123,7 @@ def main():
     if args.warmup is None:
         args.warmup = 5 if c4 else 10
     if args.members is None:
-        args.members = 64 if c4 else 256
+        args.members = 64 if args.workload in ("c4", "c4-gradient") else 256
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py runs on one GPU only; the path is sharded by "
                                    "member (ensembles) or by strips with a 2-3 cell halo "
@@ -137,6 +142,8 @@ def main():
     _lib.lib()
     if c4:
         return bench_c4(args)
+    if args.workload == "c4-gradient":
+        return bench_c4_gradient(args)
     if args.workload == "ensemble":
         return bench_ensemble(args)
     if args.workload == "flume":
@@ -956,6 +963,83 @@ def bench_c4(args):
                      "kernel_avg_ms": adj_ms + fwd_ms},
         "c1_forward": c1,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_c4_gradient(args):
+    """The whole batched gradient of the 64 x 1024^2 ensemble: --grad-steps
+    steps (1000) of the batched forward with per-member gauge records (three
+    gauges around each member's source, loss = sum of squared depth misfits
+    against a seeded synthetic target), then the checkpointed batched reverse
+    sweep (block checkpoints every --grad-every steps in host memory,
+    sub-checkpoints every ~sqrt of that on the device) giving every member's
+    gradient with respect to its initial state, phi and kref.  Reports the
+    wall time of each phase, the peak HBM allocated, and whether every
+    gradient is finite."""
+    import numpy as np
+    import torch
+
+    from dclaw_amd import adjoint as AD
+    from dclaw_amd import workloads as W
+
+    n = args.c4_n
+    members = W.ensemble(args.members, n, n, seed=args.seed)
+    cells = sum(m.patch.mx * m.patch.my for m in members)
+    N = args.grad_steps
+    rng = np.random.default_rng(args.seed)
+    gxy = []
+    for m in members:
+        p = m.patch
+        i, j = np.unravel_index(np.argmax(p.q[0][2:-2, 2:-2]), (p.mx, p.my))
+        off = rng.uniform(-20.0, 20.0, (3, 2))
+        gxy.append(np.column_stack([(i + 0.5 + off[:, 0]) * p.dx, (j + 0.5 + off[:, 1]) * p.dy]))
+    steps = np.arange(1, N + 1)[:, None]
+    tgt = [torch.tensor(1.5 * np.exp(-((steps - c) / (0.2 * N)) ** 2), device="cuda")
+           for c in rng.uniform(0.2 * N, 0.8 * N, (len(members), 3))]
+    seg = args.seg_rows if args.seg_rows > 0 else 32
+    torch.cuda.reset_peak_memory_stats()
+    er = AD.EnsembleRollout(members, N, gxy, parameters=("phi", "kref"),
+                            checkpoint_every=args.grad_every, seg_rows=seg, host_checkpoints=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    series, _ = er.forward()
+    gs = [2.0 * (s_[:, :, 0] - t_) for s_, t_ in zip(series, tgt)]
+    loss = float(sum(float((g * g).sum()) for g in gs) / 4.0)
+    g_series = []
+    for s_, g in zip(series, gs):
+        z = torch.zeros_like(s_)
+        z[:, :, 0] = g
+        g_series.append(z)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    print(f"[bench] c4 gradient: forward {N} steps {t_fwd:.1f}s, loss {loss:.6e}",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    lam0, grad = er.backward(g_series, None)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    peak = torch.cuda.max_memory_allocated()
+    finite = bool(torch.isfinite(grad).all()) and all(bool(torch.isfinite(x).all()) for x in lam0)
+    gmax = float(grad.abs().max())
+    print(f"[bench] c4 gradient: reverse {t_bwd:.1f}s, peak HBM {peak / 1e9:.1f} GB, finite "
+          f"{finite}, max |dloss/dparam| {gmax:.3e}", file=sys.stderr, flush=True)
+    total = t_fwd + t_bwd
+    out = {
+        "metric": METRIC, "value": cells * N / total, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": N, "warmup": 0, "ms_per_step": total / N * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"whole batched gradient: {len(members)} members of {n}x{n}, "
+                               f"{N} steps, gauge-misfit loss per member, block checkpoints "
+                               f"every {args.grad_every} steps in host memory, sub-checkpoints "
+                               f"on the device; value = cell-updates of the forward steps per "
+                               f"second of forward + recompute + reverse sweep",
+                   "members": len(members), "side": n, "seed": args.seed, "seg_rows": seg,
+                   "checkpoint_every": args.grad_every,
+                   "sub_every": AD.sub_interval(args.grad_every, 0)},
+        "forward_s": t_fwd, "reverse_s": t_bwd, "peak_hbm_bytes": peak, "loss": loss,
+        "gradient_finite": finite, "max_abs_param_gradient": gmax,
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
 

@@ -10,10 +10,15 @@ differentiated, as for a fixed time grid).  J is the tangent-linear step's
 the gradient of a scalar loss with respect to the whole initial state and
 the parameters together.
 
-A run of N steps is reversed with checkpoints: the forward pass keeps the
-state every ``checkpoint_every`` steps; the reverse pass recomputes each
-block's states from its checkpoint (bit for bit: the step is deterministic)
-and runs the adjoint steps back through it.
+A run of N steps is reversed with two levels of checkpoints: the forward
+pass keeps the state every ``checkpoint_every`` steps; the reverse pass
+recomputes, from a block's checkpoint, the states at the block's
+sub-checkpoints (every ``sub_every`` steps, by default about the square root
+of the block length), then each sub-block's states from its sub-checkpoint
+(bit for bit: the step is deterministic), and runs the adjoint steps back
+through them.  At most N / checkpoint_every + checkpoint_every / sub_every +
+sub_every + 1 states are held at once, for about two extra forward steps per
+step.
 
 ``rollout`` is the ``torch.autograd.Function`` form: gauge records and the
 final state of an N-step run as functions of the initial state and of named
@@ -102,7 +107,47 @@ class RolloutSpec:
     ylower: float = 0.0
     parameters: Tuple[str, ...] = ()
     checkpoint_every: int = 100
+    sub_every: int = 0
     device: str = "cuda"
+
+
+def sub_interval(every: int, sub: int) -> int:
+    """Sub-checkpoint interval inside a block of `every` steps (sub <= 0: the
+    integer nearest sqrt(every), which minimises the states held)."""
+    if sub > 0:
+        return min(sub, every)
+    return max(1, int(round(every ** 0.5)))
+
+
+def reverse_blocks(nsteps: int, every: int, sub: int, ckpt: Dict, advance, reverse_step,
+                   load=None) -> None:
+    """Drive the reverse sweep with two checkpoint levels.
+
+    ckpt[b0] is the state at each block start b0 (multiples of `every`);
+    advance(x, n) returns a new state q_{n+1} from q_n = x; reverse_step(n,
+    q_n, q_{n+1}) runs the adjoint of step n+1.  Steps are reversed from the
+    last to the first; within a block the states at the sub-checkpoints are
+    recomputed first, then each sub-block's states, which are dropped once
+    it has been reversed.  load(ckpt[b0]) gives the device state of a block
+    start (checkpoints may be kept in host memory)."""
+    s = sub_interval(every, sub)
+    for b0 in sorted(ckpt, reverse=True):
+        b1 = min(b0 + every, nsteps)
+        starts = list(range(b0, b1, s))
+        x = ckpt[b0] if load is None else load(ckpt[b0])
+        sc = {b0: x}
+        for n in range(b0, starts[-1]):
+            x = advance(x, n)
+            if n + 1 in starts:
+                sc[n + 1] = x
+        for c0 in reversed(starts):
+            c1 = min(c0 + s, b1)
+            states = [sc.pop(c0)]
+            for n in range(c0, c1):
+                states.append(advance(states[-1], n))
+            for n in range(c1 - 1, c0 - 1, -1):
+                reverse_step(n, states[n - c0], states[n + 1 - c0])
+            del states
 
 
 class Rollout:
@@ -185,32 +230,32 @@ class Rollout:
         gstep = torch.zeros_like(grad)
         cfl = torch.zeros(1, dtype=torch.float64, device=dev)
         dt = torch.zeros(1, dtype=torch.float64, device=dev)
-        starts = sorted(self.ckpt)
-        for b0 in reversed(starts):
-            b1 = min(b0 + s.checkpoint_every, N)
-            # states q_{b0} .. q_{b1} of the block, recomputed from its checkpoint
-            states = [self.ckpt[b0]]
-            for n in range(b0, b1):
-                dt.copy_(self.dts[n:n + 1])
-                x = torch.empty_like(q0)
-                stepgrid_bc(states[-1], x, self.aux, self.maux, self.mx, self.my, s.dx, s.dy,
-                            s.bc, dt, self.params_dev, cfl)
-                states.append(x)
-            for n in range(b1 - 1, b0 - 1, -1):
-                # step n+1 maps q_n -> q_{n+1}; its gauge record's adjoint joins first
-                if self.ng and g_series is not None:
-                    gauges_adj(states[n + 1 - b0], self.aux, self.maux, self.mx, self.my,
-                               s.xlower, s.ylower, s.dx, s.dy, s.params.dry_tolerance, self.xy,
-                               g_series[n].detach().contiguous(), lam)
-                dt.copy_(self.dts[n:n + 1])
-                adjoint_step(states[n - b0], self.aux, self.maux, self.mx, self.my, s.dx, s.dy,
-                             s.bc, dt, self.params_dev, self.dparams, self.np, lam, lam_b,
-                             None, gstep if self.np else None, self.work)
-                if self.np:
-                    grad += gstep
-                lam, lam_b = lam_b, lam
-            del states
-        return lam, grad[:self.np]
+        lams = [lam, lam_b]
+
+        def advance(x, n):
+            dt.copy_(self.dts[n:n + 1])
+            y = torch.empty_like(q0)
+            stepgrid_bc(x, y, self.aux, self.maux, self.mx, self.my, s.dx, s.dy, s.bc, dt,
+                        self.params_dev, cfl)
+            return y
+
+        def reverse_step(n, qn, qn1):
+            # step n+1 maps q_n -> q_{n+1}; its gauge record's adjoint joins first
+            lo, li = lams
+            if self.ng and g_series is not None:
+                gauges_adj(qn1, self.aux, self.maux, self.mx, self.my, s.xlower, s.ylower, s.dx,
+                           s.dy, s.params.dry_tolerance, self.xy,
+                           g_series[n].detach().contiguous(), lo)
+            dt.copy_(self.dts[n:n + 1])
+            adjoint_step(qn, self.aux, self.maux, self.mx, self.my, s.dx, s.dy, s.bc, dt,
+                         self.params_dev, self.dparams, self.np, lo, li, None,
+                         gstep if self.np else None, self.work)
+            if self.np:
+                grad.add_(gstep)
+            lams.reverse()
+
+        reverse_blocks(N, s.checkpoint_every, s.sub_every, self.ckpt, advance, reverse_step)
+        return lams[0], grad[:self.np]
 
 
 def _adj_records(rows) -> tuple:
@@ -297,7 +342,8 @@ class EnsembleRollout:
     def __init__(self, members: List, nsteps: int, gauges_xy: Sequence[np.ndarray],
                  parameters: Sequence[str] = ("phi", "kref"), checkpoint_every: int = 100,
                  dt0: float = 1e-3, cfl_desired: float = 0.9, dt_max: float = 1e9,
-                 bc=(BC_EXTRAP,) * 4, seg_rows: int = 16):
+                 bc=(BC_EXTRAP,) * 4, seg_rows: int = 16, sub_every: int = 0,
+                 host_checkpoints: bool = False):
         from .step import Ensemble
         if nsteps < 1 or checkpoint_every < 1:
             raise ValueError("nsteps and checkpoint_every must be >= 1")
@@ -306,6 +352,11 @@ class EnsembleRollout:
         self.members = members
         self.nsteps = nsteps
         self.every = checkpoint_every
+        self.sub_every = sub_every
+        # block checkpoints in host memory (copied back when a block is
+        # reversed): HBM then holds only the sub-checkpoints and one
+        # sub-block's states besides the step's own buffers and workspaces
+        self.host_checkpoints = host_checkpoints
         self.dt0, self.cfl_desired, self.dt_max = dt0, cfl_desired, dt_max
         self.ens = Ensemble(members, bc=bc, seg_rows=seg_rows)
         self.adj = EnsembleAdjoint(self.ens, parameters)
@@ -316,6 +367,19 @@ class EnsembleRollout:
 
     def _states(self) -> List[torch.Tensor]:
         return [self.ens.current(k) for k in range(self.ens.n)]
+
+    def _checkpoint(self) -> List[torch.Tensor]:
+        if not self.host_checkpoints:
+            return [x.clone() for x in self._states()]
+        # pageable host memory: page-locked allocations of this size are not
+        # granted everywhere, and a block checkpoint is taken only every
+        # checkpoint_every steps
+        return [x.to("cpu") for x in self._states()]
+
+    def _load(self, ck: List[torch.Tensor]) -> List[torch.Tensor]:
+        if not self.host_checkpoints:
+            return ck
+        return [h.to(self.ens.dt.device) for h in ck]
 
     def _gauges(self, series: List[torch.Tensor], n: int) -> None:
         e = self.ens
@@ -332,7 +396,7 @@ class EnsembleRollout:
         e.cfl.zero_()
         dev = e.dt.device
         self.dts = torch.zeros((self.nsteps, e.n), dtype=torch.float64, device=dev)
-        self.ckpt = {0: [x.clone() for x in self._states()]}
+        self.ckpt = {0: self._checkpoint()}
         series = [torch.zeros((self.nsteps, xy.shape[0], 8), dtype=torch.float64, device=dev)
                   for xy in self.xy]
         for n in range(self.nsteps):
@@ -341,7 +405,7 @@ class EnsembleRollout:
             dt_update(e.dt, e.cfl, self.cfl_desired, self.dt_max)
             self._gauges(series, n)
             if (n + 1) % self.every == 0 and n + 1 < self.nsteps:
-                self.ckpt[n + 1] = [x.clone() for x in self._states()]
+                self.ckpt[n + 1] = self._checkpoint()
         return series, [x.clone() for x in self._states()]
 
     def backward(self, g_series: Optional[Sequence[torch.Tensor]] = None,
@@ -351,33 +415,37 @@ class EnsembleRollout:
         parameter gradients)."""
         e = self.ens
         n_m = e.n
-        lam = [g.detach().clone().contiguous() if g is not None else torch.zeros_like(x)
+        lam = [g.detach().clone().contiguous() if g is not None
+               else torch.zeros_like(x, device=e.dt.device)
                for g, x in zip(g_final or [None] * n_m, self.ckpt[0])]
         lam_b = [torch.empty_like(x) for x in lam]
         grad = torch.zeros((n_m, self.adj.np), dtype=torch.float64, device=e.dt.device)
-        for b0 in reversed(sorted(self.ckpt)):
-            b1 = min(b0 + self.every, self.nsteps)
+        lams = [lam, lam_b]
+
+        def advance(x, n):
+            # one batched step from the states x (the ensemble's own buffers
+            # serve as scratch); returns copies of q_{n+1}
             for k in range(n_m):
-                e.current(k).copy_(self.ckpt[b0][k])
-            states = [[x.clone() for x in self._states()]]
-            for n in range(b0, b1):
-                e.dt.copy_(self.dts[n])
-                e.cfl.zero_()
-                e.step()
-                states.append([x.clone() for x in self._states()])
-            for n in range(b1 - 1, b0 - 1, -1):
-                if g_series is not None:
-                    qn1 = states[n + 1 - b0]
-                    for k in range(n_m):
-                        gauges_adj(qn1[k], e.bed[k], e.maux[k], e.mx[k], e.my[k], 0.0, 0.0,
-                                   e.dx[k], e.dy[k], self.drytol[k], self.xy[k],
-                                   g_series[k][n].detach().contiguous(), lam[k])
-                e.dt.copy_(self.dts[n])
-                g = self.adj.step(states[n - b0], lam, lam_b)
-                grad += g
-                lam, lam_b = lam_b, lam
-            del states
-        return lam, grad
+                e.current(k).copy_(x[k])
+            e.dt.copy_(self.dts[n])
+            e.cfl.zero_()
+            e.step()
+            return [y.clone() for y in self._states()]
+
+        def reverse_step(n, qn, qn1):
+            lo, li = lams
+            if g_series is not None:
+                for k in range(n_m):
+                    gauges_adj(qn1[k], e.bed[k], e.maux[k], e.mx[k], e.my[k], 0.0, 0.0,
+                               e.dx[k], e.dy[k], self.drytol[k], self.xy[k],
+                               g_series[k][n].detach().contiguous(), lo[k])
+            e.dt.copy_(self.dts[n])
+            grad.add_(self.adj.step(qn, lo, li))
+            lams.reverse()
+
+        reverse_blocks(self.nsteps, self.every, self.sub_every, self.ckpt, advance, reverse_step,
+                       load=self._load)
+        return lams[0], grad
 
 
 class _RolloutFn(torch.autograd.Function):
@@ -414,7 +482,7 @@ def gauge_loss_gradient_adjoint(q: np.ndarray, aux: np.ndarray, params: DigParam
                                 targets: np.ndarray, parameters: Sequence[str] = ("phi", "kref"),
                                 dt0: float = 1e-3, cfl_desired: float = 0.9, dt_max: float = 1e9,
                                 component: int = 0, bc=(BC_EXTRAP,) * 4,
-                                checkpoint_every: int = 100):
+                                checkpoint_every: int = 100, sub_every: int = 0):
     """The gauge misfit of ``tangent.gauge_loss_gradient`` (sum over steps and
     gauges of (gauge value[component] - target)^2) and its gradient by one
     reverse sweep: returns (loss, {parameter: dloss/dparameter}, dloss/dq0 as
@@ -425,7 +493,7 @@ def gauge_loss_gradient_adjoint(q: np.ndarray, aux: np.ndarray, params: DigParam
                        cfl_desired=cfl_desired, dt_max=dt_max, bc=tuple(bc),
                        gauges_xy=np.asarray(gauges_xy, dtype=np.float64).reshape(-1, 2),
                        xlower=xlower, ylower=ylower, parameters=tuple(parameters),
-                       checkpoint_every=checkpoint_every)
+                       checkpoint_every=checkpoint_every, sub_every=sub_every)
     if spec.gauges_xy.shape[0] != ng:
         raise ValueError("targets must have one column per gauge")
     r = Rollout(spec)

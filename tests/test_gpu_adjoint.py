@@ -119,7 +119,8 @@ def test_reverse_gradient_equals_tangent_gradients(dev):
     gives d/dphi and d/dkref equal to the tangent-linear runs', and its
     initial-state gradient dotted with the m0 direction (hm = m0 h) and with
     a random depth perturbation equals the tangent-linear derivatives along
-    them; checkpoints every 7 steps give the same result as one block."""
+    them; any checkpoint interval and sub-checkpoint interval give the same
+    result bit for bit."""
     from dclaw_amd import adjoint as AD
     from dclaw_amd import tangent as T
     pt, gxy, nsteps, targets = _flume_case()
@@ -142,10 +143,11 @@ def test_reverse_gradient_equals_tangent_gradients(dev):
         assert grads_a[n] == pytest.approx(grads_t[n], rel=1e-9), n
     assert float((lam0 * dq_m0).sum()) == pytest.approx(grads_t["m0"], rel=1e-9)
     assert float((lam0 * dq_h).sum()) == pytest.approx(grads_t["h"], rel=1e-9)
-    _, grads_b, lam0_b, _ = AD.gauge_loss_gradient_adjoint(
-        pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy, targets, parameters=("phi", "kref"),
-        checkpoint_every=1000)
-    assert np.array_equal(lam0_b, lam0) and grads_b == grads_a
+    for every, sub in [(1000, 0), (1000, 1), (7, 3), (40, 40)]:
+        _, grads_b, lam0_b, _ = AD.gauge_loss_gradient_adjoint(
+            pt.q, pt.aux, pt.params, pt.dx, pt.dy, 0.0, 0.0, gxy, targets,
+            parameters=("phi", "kref"), checkpoint_every=every, sub_every=sub)
+        assert np.array_equal(lam0_b, lam0) and grads_b == grads_a, (every, sub)
 
 
 def test_reverse_gradient_matches_finite_differences(dev):
