@@ -503,15 +503,45 @@ def bench_flume(args):
                     "graph_seconds": t_fwd_graph,
                     "graph_ms_per_step": None if t_fwd_graph is None
                     else t_fwd_graph / nsteps * 1e3},
-        "gradient": {"seconds": t_adj, "loss": loss_a, "dloss": grads_a,
-                     "method": "reverse sweep (one forward run, recompute, adjoint steps)",
-                     "units": {"phi": "per radian", "kref": "per m^2", "m0": "per unit m0"}},
-        "tangent_gradient": {"seconds": t_grad, "loss": loss, "dloss": grads,
-                             "rel_diff_vs_adjoint": rel},
+        "gradient": gradient_record(t_adj, loss_a, grads_a, nsteps,
+                                    "reverse sweep (one forward run, recompute, adjoint steps)"),
+        "tangent_gradient": dict(gradient_record(t_grad, loss, grads, nsteps,
+                                                 "three tangent-linear runs"),
+                                 rel_diff_vs_adjoint=rel),
         "roofline": None,
         "cpu_baseline": None,
     }
     print(json.dumps(out))
+
+
+# Steps over which the flume gradient is validated against central
+# differences (GPU tests: the tangent after 150 steps, the reverse gradient of
+# the gauge misfit over 150 steps).  Beyond that the derivative of the
+# MC-limited second-order correction grows by ~13 orders of magnitude per 100
+# steps in the supercritical flow (first order: no growth; minmod: ~1.5).
+FLUME_FD_VALIDATED_STEPS = 150
+
+
+def gradient_record(seconds, loss, grads, nsteps, method):
+    """A gradient result with its status: values inside the validated horizon
+    are reported as results; beyond it, or when not finite, they are moved
+    to 'dloss_unchecked' and flagged."""
+    import math
+    finite = all(math.isfinite(v) for v in grads.values())
+    rec = {"seconds": seconds, "loss": loss, "method": method, "finite": finite,
+           "fd_validated_steps": FLUME_FD_VALIDATED_STEPS,
+           "units": {"phi": "per radian", "kref": "per m^2", "m0": "per unit m0"}}
+    if finite and nsteps <= FLUME_FD_VALIDATED_STEPS:
+        rec["status"] = "validated: within the horizon checked against central differences"
+        rec["dloss"] = grads
+    else:
+        rec["status"] = ("unchecked: " + ("not finite" if not finite else
+                         f"{nsteps} steps exceed the {FLUME_FD_VALIDATED_STEPS}-step horizon "
+                         f"validated by central differences; the derivative of the MC-limited "
+                         f"second-order correction grows exponentially in this supercritical "
+                         f"flow, so the values measure that growth, not a usable gradient"))
+        rec["dloss_unchecked"] = grads
+    return rec
 
 
 def bench_ensemble(args):
@@ -1038,7 +1068,10 @@ def bench_c4_gradient(args):
                    "checkpoint_every": args.grad_every,
                    "sub_every": AD.sub_interval(args.grad_every, 0)},
         "forward_s": t_fwd, "reverse_s": t_bwd, "peak_hbm_bytes": peak, "loss": loss,
-        "gradient_finite": finite, "max_abs_param_gradient": gmax,
+        "gradient_finite": finite, "max_abs_param_gradient_unchecked": gmax,
+        "gradient_status": ("unchecked: no finite-difference check at this horizon; the "
+                            "magnitude reflects the exponential growth of the MC-limited "
+                            "scheme's derivative in fast flow (see the flume workload)"),
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
