@@ -1471,8 +1471,10 @@ int validate_params(const dclaw_params& P)
     if (P.mwaves != 5) return fail(DCLAW_EINVAL, "mwaves must be 5 (rpn2_dclaw.f:276-315)");
     if (P.order != 1 && P.order != 2) return fail(DCLAW_EINVAL, "order must be 1 or 2");
     if (P.transverse < 0 || P.transverse > 2) return fail(DCLAW_EINVAL, "transverse must be 0..2");
-    if (P.src2method < -1 || P.src2method > 1)
-        return fail(DCLAW_EINVAL, "src2method must be -1, 0 or 1 on this path");
+    if (P.src2method < -1 || P.src2method > 2)
+        return fail(DCLAW_EINVAL, "src2method must be -1, 0, 1 or 2");
+    if (P.src2method == 2 && P.alphamethod != 1)
+        return fail(DCLAW_EINVAL, "src2method 2 requires alphamethod 1 (data.py:362-365)");
     if (P.entrainment < 0 || P.entrainment > 1 || P.entrainment_method < 0 ||
         P.entrainment_method > 1)
         return fail(DCLAW_EINVAL, "entrainment and entrainment_method must be 0 or 1");

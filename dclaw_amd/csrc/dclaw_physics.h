@@ -30,6 +30,10 @@ namespace dclaw {
 
 using Real = double;
 using Params = dclaw_params;
+// src2method 2 (mp_update_FE_4quad) is compiled into the step itself; the
+// derivative instantiations leave it out (its substep logic over values
+// with tangents would spill the shared src2 body of every method)
+constexpr bool kFe4quad = true;
 
 #include "dclaw_physics_body.inc"
 

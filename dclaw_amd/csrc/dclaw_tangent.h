@@ -154,6 +154,8 @@ DC_HD Params make_params(const dclaw_params& p, const dclaw_params& dp)
 }
 
 using Real = Dual;
+// src2method 2 is not differentiated (src2_cell writes NaN for it here)
+constexpr bool kFe4quad = false;
 
 #include "dclaw_physics_body.inc"
 

@@ -280,6 +280,8 @@ DC_HD Params make_params(const dclaw_params& p, const dclaw_params* const* dp)
 }
 
 using Real = DualK;
+// src2method 2 is not differentiated (src2_cell writes NaN for it here)
+constexpr bool kFe4quad = false;
 
 #include "dclaw_physics_body.inc"
 #include "dclaw_prepare_body.inc"

@@ -81,8 +81,8 @@ class DigParams:
                              "data.py:357-361)")
         if self.src2method == 2 and self.alphamethod < 1:
             raise ValueError("src2method == 2 requires alphamethod = 1 (data.py:362-365)")
-        if self.src2method not in (-1, 0, 1):
-            raise ValueError("src2method must be -1, 0 or 1 on this path")
+        if self.src2method not in (-1, 0, 1, 2):
+            raise ValueError("src2method must be -1, 0, 1 or 2")
         if self.curvature != 0:
             raise ValueError("curvature is not implemented on this path (the reference "
                              "accumulates gz over the cells of src2's loop, src2.f90:87,156)")

@@ -19,8 +19,8 @@
  * accuracy), not by reference source.  The physics of every routine is
  * pinned by consequences of the governing equations (tests/test_oracle_*.py),
  * and tests/test_oracle_mutants.py checks that those pins reject altered
- * copies of this file.  Not restated (parity unpinned, not on the product
- * path): mp_update_FE_4quad (src2method 2) and the curvature term of src2.
+ * copies of this file.  Not restated (not on the product path): the
+ * curvature term of src2.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -38,6 +38,7 @@
 #define D_TAN tan
 #define D_TANH tanh
 #define D_POW pow
+#define D_LOG log
 #else
 #define DCM_FN static inline
 #include "../dclaw_amd/csrc/dcmath.h"
@@ -46,6 +47,7 @@
 #define D_TAN dcm_tan
 #define D_TANH dcm_tanh
 #define D_POW dcm_pow
+#define D_LOG dcm_log
 #endif
 
 #define MEQN 7
@@ -1093,8 +1095,231 @@ static void mp_update_relax_Dclaw4(const oracle_params* P, double dt, double* h,
     }
 }
 
-/* src2: src2.f90:27-306 (bed_normal = curvature = entrainment = 0;
- * src2method -1, 0, 1) */
+/* mp_update_FE_4quad: mp_update.f90:38-610 (src2method 2).  One substep
+ * of the quadrant-based explicit integration of (m, p_exc) on the rho h =
+ * const manifold; returns the substep dtk taken out of dt (the time still to
+ * integrate).  The branch for vnorm == 0 or m == m_eq (:226-254) reads dtm
+ * before any assignment, so its result is not defined by the source: here it
+ * sets p to a quiet NaN (check4nans then stops the run, as it would for any
+ * NaN) and ends the substeps (dtk = dt).  Quirks kept: region 1B integrates
+ * p over dtp but m and the time over dtm (:360-382); region 3A m over dtm,
+ * p and the time over dtp (:490-509); case 4 reports dtk = dtp although it
+ * integrated over min(dtr, dtp, dtm) (:589-604). */
+static double mp_update_FE_4quad(const oracle_params* P, double dt, double* h, double u,
+                                 double v, double* m, double* p, double chi, double rhoh,
+                                 double gz)
+{
+    const double rho_f = P->rho_f, rho_s = P->rho_s, mu = P->mu;
+    double rho, kperm, alphainv, m_eq, tanpsi, tau, hu, hv, hm, p_exc = 0.0;
+    double dtk, dtr, dtp, dtm, dts = 0.0, p_exc_ave;
+    const double vnorm = sqrt(u * u + v * v);
+    setvars(P, *h, u, v, *m, *p, chi, gz, &rho, &kperm, &alphainv, &m_eq, &tanpsi, &tau);
+    const double h0 = *h, m_0 = *m;
+    const double rho0 = m_0 * (rho_s - rho_f) + rho_f;
+    const double p0 = *p;
+    const double p_eq0 = rho_f * gz * h0;
+    const double p_exc0 = p0 - p_eq0;
+    dtk = 0.0;
+    dtr = dt;
+    /* :188-190 */
+    const double km0 = ((2.0 * kperm * (rho0 * rho0)) / (mu * (rhoh * rhoh)));
+    const double c_d0 = -(3.0 * vnorm * (alphainv * rho0 / (rhoh)) * tanpsi);
+    double kp0 = (kperm / (h0 * mu)) *
+                 (3.0 * alphainv * rho0 / rhoh - 1.5 * rho_f * gz * h0 * (rho0 - rho_f) / rhoh);
+    kp0 = dmax(kp0, 1.e-3);                                                   /* :208 */
+    if (*m == 0.0) {                                                          /* :212-217 */
+        p_exc = p_exc0 * D_EXP(-(kp0 * dtr));
+        qfix_cmass(P, h, m, p, &rho, &p_exc, &hu, &hv, &hm, u, v, rhoh, gz);
+        return dtr;
+    }
+    if ((fabs(p_exc0 / (rho_f * gz * h0)) < 1.e-12) && (fabs(*m - m_eq) < 1.e-12))
+        return dtr;                                                           /* :220-223 */
+    if ((vnorm == 0.0) || (*m == m_eq)) {                                     /* :226-254 */
+        *p = NAN;   /* dtm is read unset by the reference: undefined result */
+        return dtr;
+    }
+    /* critical point (m_c, p_eq_c): fixed-point iteration, :257-293 */
+    double m_c = P->m_crit;
+    double rho_c = m_c * (rho_s - rho_f) + rho_f;
+    double h_c = rhoh / rho_c;
+    double p_eq_c = rho_f * gz * h_c;
+    const double shear = 2.0 * vnorm / h0;
+    const double convtol = 1.e-16;
+    const int itermax = 100;
+    double normc = 0.0;
+    int iter;
+    for (iter = 1; iter <= itermax; iter++) {
+        const double m_c0 = m_c, p_eq_c0 = p_eq_c;
+        rho_c = m_c * (rho_s - rho_f) + rho_f;
+        h_c = rhoh / rho_c;
+        p_eq_c = rho_f * gz * h_c;
+        const double sig_c = dmax(rhoh * gz - p_eq_c, 0.0);
+        const double sd = shear * P->delta;
+        const double Nd = rho_s * (sd * sd) + sig_c;
+        const double Nn = mu * shear;
+        m_c = P->m_crit * (sqrt(Nd) / (sqrt(Nd) + sqrt(Nn)));
+        const double a = 1.e2 * (m_c - m_c0), b = 1.e-3 * (p_eq_c - p_eq_c0);
+        normc = a * a + b * b;
+        if (normc < convtol) break;
+    }
+    /* a Fortran DO loop leaves iter = itermax + 1 when it runs to the end */
+    if ((iter >= itermax) && (normc > 1.0)) {
+        m_c = P->m_crit;
+        rho_c = m_c * (rho_s - rho_f) + rho_f;
+        h_c = rhoh / rho_c;
+        p_eq_c = rho_f * gz * h_c;
+    } else {
+        rho_c = m_c * (rho_s - rho_f) + rho_f;
+        h_c = rhoh / rho_c;
+        p_eq_c = rho_f * gz * h_c;
+    }
+    int quad0 = 0;                                                            /* :297-305 */
+    if ((*m < m_c) && (*p >= p_eq_c)) quad0 = 1;
+    else if ((*m >= m_c) && (*p > p_eq_c)) quad0 = 2;
+    else if ((*m > m_c) && (*p <= p_eq_c)) quad0 = 3;
+    else if ((*m <= m_c) && (*p < p_eq_c)) quad0 = 4;
+
+    switch (quad0) {
+    case 1:                                                                   /* :311-422 */
+        if (p_exc0 < -(1.e-3 * rhoh * gz)) {                                  /* region 1A */
+            if (c_d0 > 0.0) dtp = -(1.0 / kp0) * D_LOG(c_d0 / (c_d0 - kp0 * p_exc0));
+            else dtp = dtr;
+            dtm = D_LOG(0.98) / (kp0 * p_exc0);
+            dts = dmin(dmin(dtr, dtp), dtm);
+            dts = dmax(dts, 0.0);
+            p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dts)) + c_d0);
+            *m = m_0 * D_EXP(dts * km0 * p_exc);
+        } else if ((-(kp0 * p_exc0) + c_d0) > 0.0) {                          /* region 1B */
+            if ((c_d0 / kp0) > (rhoh * gz - p_eq_c))
+                dtp = dmin(dtr, -(1.0 / kp0) * D_LOG((-(kp0 * rhoh * gz) + kp0 * p_eq_c + c_d0) /
+                                                     (-(kp0 * p_exc0) + c_d0)));
+            else
+                dtp = dtr;
+            p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dtp)) + c_d0);
+            p_exc_ave = 0.5 * (p_exc + p_exc0);
+            if (p_exc_ave <= 0.0) {
+                dtm = dtr;
+                *m = m_0 * D_EXP(km0 * 0.5 * (p_exc + p_exc0) * dtm);
+            } else {
+                if ((km0 * p_exc_ave * m_0) < 1.e-99) dtm = dtp;
+                else dtm = dmin(dtp, (m_eq - m_0) / (km0 * p_exc_ave * m_0));
+            }
+            *m = m_0 + km0 * m_0 * p_exc_ave * dtm;
+            dts = dtm;
+        } else {                                                              /* region 1C */
+            if (c_d0 > 0.0) {
+                dtp = dtr;
+                if ((km0 * p_exc0 * m_0) > 1.e-99) dtm = (m_c - m_0) / (km0 * p_exc0 * m_0);
+                else dtm = dtp;
+                dts = dmin(dtm, dtp);
+                dts = dmax(dts, 0.0);
+                *m = m_0 + km0 * m_0 * p_exc0 * dts;
+                p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dts)) + c_d0);
+            } else {
+                dts = dtr;
+                dtp = dtr;
+                if ((km0 * p_exc0 * m_0) > 1.e-99) dts = dmin(dtr, (m_c - m_0) / (km0 * p_exc0 * m_0));
+                if ((kp0 * p_exc0 - c_d0) != 0.0) {
+                    if ((-(kp0 * p_eq0) - c_d0) / (kp0 * p_exc0 - c_d0) > 1.e-99)
+                        dtp = dmin(dtr, -(1.0 / kp0) * D_LOG((c_d0) / (c_d0 - kp0 * p_exc0)));
+                }
+                dts = dmin(dts, dtp);
+                dts = dmax(dts, 0.0);
+                *m = m_0 + km0 * m_0 * p_exc0 * dts;
+                p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dts)) + c_d0);
+            }
+        }
+        qfix_cmass(P, h, m, p, &rho, &p_exc, &hu, &hv, &hm, u, v, rhoh, gz);
+        dtk = dts;
+        break;
+    case 2:                                                                   /* :424-476 */
+        dtp = dtr;
+        if ((c_d0 - kp0 * (p_eq_c - p_eq0)) / (c_d0 - kp0 * p_exc0) > 1.e-99)
+            dtp = dmin(dtr, -(1.0 / kp0) * D_LOG((c_d0 - kp0 * (p_eq_c - p_eq0)) / (c_d0 - kp0 * p_exc0)));
+        dtm = dtp;
+        if (km0 * p_exc0 * m_0 > 1.e-99)
+            dtm = dmin(dtp, (dmin(1.0 - m_0, 0.02) / (km0 * p_exc0 * m_0)));
+        dts = dmin(dtp, dtm);
+        p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dts)) + c_d0);
+        *m = m_0 + km0 * m_0 * p_exc * dts;
+        qfix_cmass(P, h, m, p, &rho, &p_exc, &hu, &hv, &hm, u, v, rhoh, gz);
+        dtk = dts;
+        break;
+    case 3:                                                                   /* :478-571 */
+        if (p_exc0 > 1.e-3 * rhoh * gz) {                                     /* region 3A */
+            dtp = dtr;
+            if ((c_d0 / (-(kp0 * p_exc0) + c_d0)) > 1.e-99)
+                dtp = dmin(dtr, -(1.0 / kp0) * D_LOG(c_d0 / (c_d0 - kp0 * p_exc0)));
+            dtm = dtp;
+            if ((km0 * p_exc0 * m_0) > 1.e-99)
+                dtm = dmin(dtp, (dmin(1.0 - m_0, 0.02)) / (km0 * p_exc0 * m_0));
+            p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dtp)) + c_d0);
+            *m = m_0 + km0 * m_0 * 0.5 * (p_exc0 + p_exc) * dtm;
+            dts = dtp;
+        } else if ((-(kp0 * p_exc0) + c_d0) < 0.0) {                          /* region 3B */
+            if (c_d0 / kp0 < -p_eq0) {
+                dtp = dtr;
+                if (((c_d0 + kp0 * p_eq0) / (-(kp0 * p_exc0) + c_d0)) > 1.e-99)
+                    dtp = dmin(dtr, -(1.0 / kp0) * D_LOG((c_d0 + kp0 * p_eq0) / (-(kp0 * p_exc0) + c_d0)));
+            } else {
+                dtp = dtr;
+            }
+            p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dtp)) + c_d0);
+            p_exc_ave = 0.5 * (p_exc + p_exc0);
+            if (p_exc_ave > 0.0) {
+                dtm = dmin(dtr, (0.02) / (km0 * p_exc_ave * m_0));
+            } else {
+                const double m_lower = m_eq + dmax(0.0, (1.0 / 3.0) * kp0 * h0 * (-p_exc_ave) /
+                                                            (vnorm * alphainv));
+                dtm = dmin(dtr, (m_lower - m_0) / (km0 * p_exc_ave * m_0));
+            }
+            *m = m_0 + km0 * m_0 * p_exc_ave * dtm;
+            dts = dtm;
+        } else {                                                              /* region 3C */
+            if (c_d0 < 0.0) {
+                dtp = dtr;
+                dtm = dtr;
+                if (-(km0 * p_exc0 * m_0) > 1.e-99) dtm = (m_c - m_0) / (km0 * p_exc0 * m_0);
+                dts = dmin(dtp, dtm);
+                dts = dmax(dts, 0.0);
+                p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dts)) + c_d0);
+                *m = m_0 + km0 * m_0 * p_exc0 * dts;
+            } else {
+                dtm = dtr;
+                dtp = dtr;
+                if ((km0 * p_exc0 * m_0) > 1.e-99) dtm = (m_c - m_0) / (km0 * p_exc0 * m_0);
+                if ((-c_d0 / (kp0 * p_exc0 - c_d0)) > 1.e-99)
+                    dtp = -(1.0 / kp0) * D_LOG(c_d0 / (c_d0 - kp0 * p_exc0));
+                dts = dmin(dtp, dtm);
+                dts = dmax(dts, 0.0);
+                p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dts)) + c_d0);
+                *m = m_0 + km0 * m_0 * p_exc0 * dts;
+            }
+        }
+        qfix_cmass(P, h, m, p, &rho, &p_exc, &hu, &hv, &hm, u, v, rhoh, gz);
+        dtk = dts;
+        break;
+    case 4:                                                                   /* :573-605 */
+        dtp = dtr;
+        if ((c_d0 - kp0 * (p_eq_c - p_eq0)) / (c_d0 - kp0 * p_exc0) > 1.e-99)
+            dtp = dmin(dtr, -(1.0 / kp0) * D_LOG((c_d0 - kp0 * (p_eq_c - p_eq0)) / (c_d0 - kp0 * p_exc0)));
+        dtm = D_LOG(0.98) / (kp0 * p_exc0);
+        dts = dmin(dtr, dmin(dtp, dtm));
+        p_exc = (1.0 / kp0) * ((kp0 * p_exc0 - c_d0) * D_EXP(-(kp0 * dts)) + c_d0);
+        *m = m_0 * D_EXP(dts * km0 * 0.5 * (p_exc0 + p_exc));
+        dts = dtp;
+        qfix_cmass(P, h, m, p, &rho, &p_exc, &hu, &hv, &hm, u, v, rhoh, gz);
+        dtk = dts;
+        break;
+    default:
+        break;
+    }
+    (void)tau;
+    return dtk;
+}
+
+/* src2: src2.f90:27-306 (bed_normal = curvature = 0; src2method -1, 0, 1,
+ * 2) */
 /* ent_dclaw4: entrainment.f90:27-100.  b_x and b_y are arguments of the
  * reference routine that it never reads; they are left out. */
 static void ent_dclaw4(const oracle_params* P, double dt, double* h, double u, double v,
@@ -1169,6 +1394,22 @@ void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, doub
 
             if (P->src2method == -1) {
                 hu = h * u; hv = h * v; hm = h * m; hchi = h * chi;
+            } else if (P->src2method == 2) {
+                /* src2.f90:209-226: substeps until dt is used up */
+                double dtremaining = dt;
+                const int itercountmax = 100;
+                int itercount = 0;
+                while (dtremaining > 1.e-99) {
+                    const double dtk = mp_update_FE_4quad(P, dtremaining, &h, u, v, &m, &p, chi,
+                                                          rhoh, gz);
+                    dtremaining = dtremaining - dtk;
+                    itercount = itercount + 1;
+                    if (dtk == 0.0) break;
+                    if (h < drytol) break;
+                    if (itercount >= itercountmax) break;
+                }
+                hu = h * u; hv = h * v; hm = h * m; hchi = h * chi;
+                qfix(P, &h, &hu, &hv, &hm, &p, &hchi, &u, &v, &m, &chi, &rho, gz);
             } else {
                 mp_update_relax_Dclaw4(P, dt, &h, &u, &v, &m, &p, &chi, rhoh, gz);
                 hu = h * u; hv = h * v; hm = h * m; hchi = h * chi;

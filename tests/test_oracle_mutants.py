@@ -100,6 +100,14 @@ MUTANTS = [
     ("setvars effective stress from rho_f",
      "double sig_eff = dmax(0.0, *rho * gz * h - p);",
      "double sig_eff = dmax(0.0, P->rho_f * gz * h - p);"),
+    ("FE_4quad kp0 with the sign of its second term flipped",
+     "(3.0 * alphainv * rho0 / rhoh - 1.5 * rho_f * gz * h0 * (rho0 - rho_f) / rhoh);",
+     "(3.0 * alphainv * rho0 / rhoh + 1.5 * rho_f * gz * h0 * (rho0 - rho_f) / rhoh);"),
+    ("FE_4quad dilatancy forcing 3|v| -> 1.5|v|",
+     "const double c_d0 = -(3.0 * vnorm", "const double c_d0 = -(1.5 * vnorm"),
+    ("FE_4quad m rate with rho0 for rho0^2",
+     "const double km0 = ((2.0 * kperm * (rho0 * rho0)) / (mu * (rhoh * rhoh)));",
+     "const double km0 = ((2.0 * kperm * (rho0)) / (mu * (rhoh * rhoh)));"),
     ("qfix pb bound rho g h -> rho_f g h",
      "double pmax = *rho * gz * *h;", "double pmax = P->rho_f * gz * *h;"),
 ]

@@ -444,6 +444,43 @@ def pin_calc_taudir_still_cell(O):
         assert out[3, 3, 3] == 1.0
 
 
+def pin_fe4quad_rates(O):
+    """src2method 2 (mp_update_FE_4quad) integrates the same source terms on
+    the rho h = const manifold: at vanishing dt its rates equal theory.md's
+    (dpb/dt = zeta D - 3 |v| tan(psi)/(alpha h), dh/dt = D (rho - rho_f)/rho,
+    d(hm)/dt = -D m rho_f/rho, with sigma_0 of alphamethod 1,
+    theory.md:139-190) in each of the four quadrants of (m, pb) around the
+    critical point; and a still cell (vnorm = 0, where the reference reads an
+    unset time step, mp_update.f90:226-254) comes out as NaN."""
+    seen = set()
+    for m, frac, speed in [(0.50, 1.2, 1.0), (0.66, 1.2, 1.0), (0.66, 0.6, 1.0), (0.50, 0.6, 1.0),
+                           (0.55, 1.1, 3.0), (0.68, 0.3, 0.5)]:
+        p, prm = _params(O, kref=1e-9, src2method=2, alphamethod=1)
+        h = 1.0
+        rho = m * p.rho_s + (1 - m) * p.rho_f
+        pb = frac * p.rho_f * G * h
+        u, v = 0.8 * speed, 0.6 * speed
+        q, aux = _cell(h, h * u, h * v, m, pb)
+        sig = rho * G * h - pb
+        sig0 = 0.5 * p.alpha_c * p.rho_f * (p.rho_s - p.rho_f) * G * h / rho
+        ainv = m * (sig + sig0) / p.alpha_c
+        gdot = 2 * speed / h
+        N = p.mu * gdot / (p.rho_s * gdot ** 2 * p.delta ** 2 + sig)
+        tanpsi = (m - p.m_crit / (1 + math.sqrt(N))) / p.c1
+        k = p.kref * math.exp(-(m - p.mref) / 0.04)
+        D = -(2 * k / (h * p.mu)) * (pb - p.rho_f * G * h)
+        zeta = 1.5 * ainv / h + G * p.rho_f * (rho - p.rho_f) / (4 * rho)
+        r = _rates(O, prm, q, aux, 1e-6)
+        assert r[4] == pytest.approx(zeta * D - 3 * speed * tanpsi * ainv / h, rel=1e-5), (m, frac)
+        assert r[0] == pytest.approx(D * (rho - p.rho_f) / rho, rel=1e-5), (m, frac)
+        assert r[3] == pytest.approx(-D * m * p.rho_f / rho, rel=1e-5), (m, frac)
+        seen.add((m > 0.6, frac > 1.0))
+    assert len(seen) == 4
+    p, prm = _params(O, src2method=2, alphamethod=1)
+    q, aux = _cell(1.0, 0.0, 0.0, 0.6, 0.5 * 1000 * G)
+    assert math.isnan(O.src2(prm, q, aux, 0.01)[4, 2, 2])
+
+
 def pin_speed_limit(O):
     """b4step2: a cell faster than speed_limit is slowed to exactly the limit
     with its direction kept; slower cells are untouched."""
@@ -494,7 +531,7 @@ PINS = [pin_rpt2_gravity_eigenvectors, pin_rpt2_uses_the_receiving_cell,
         pin_solver_stationary_bed_step, pin_rpn2_fluctuations, pin_basal_friction_rates,
         pin_pressure_relaxation_rate, pin_dilatancy_rate, pin_speed_limit,
         pin_still_impermeable_cell_keeps_pressure, pin_static_heap_threshold,
-        pin_calc_taudir_still_cell,
+        pin_calc_taudir_still_cell, pin_fe4quad_rates,
         pin_second_order_1d, pin_second_order_2d]
 
 
