@@ -102,6 +102,11 @@ SIGNATURES = {
     "dclaw_stepgrid_split_static_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _ip,
                                                        _vp, _vp, _vp, _vp]),
     "dclaw_static_workspace_ints": (ctypes.c_int, []),
+    "dclaw_curvature_workspace_doubles": (ctypes.c_size_t, [_i, _i]),
+    "dclaw_stepgrid_curvature_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _ip, _vp,
+                                                    _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "dclaw_src2_curvature_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _d, _d, _vp, _vp, _vp,
+                                                ctypes.c_size_t, _vp]),
     "dclaw_dt_update": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, _vp]),
     "dclaw_set_params": (ctypes.c_int, [ctypes.POINTER(CParams)]),

@@ -690,6 +690,75 @@ __global__ void k_src2(double* q, const double* __restrict__ aux, int maux, int 
 
 // Smallest flat index (component fastest) of a NaN in q, min-reduced into
 // *first (which the caller initialises to INT32_MAX).
+// ---- curvature (src2.f90:133-156): gz runs over src2's sweep
+// The reference sets gz once and adds each wet cell's curvature acceleration
+// to it without a reset, so a cell's gravity includes every earlier cell's
+// in loop order (j outer, i inner).  The step forms that running sum exactly
+// in the reference's order: the accelerations of all cells in parallel, the
+// sum in one workgroup (a cell that adds nothing adds +0.0, which leaves the
+// sum bit-identical), then src2 on every cell in parallel with its own gz.
+__global__ void k_curv_gacc(const double* __restrict__ q, const double* __restrict__ aux,
+                            int maux, int mx, int my, double dx, double dy,
+                            const dclaw_params* __restrict__ params, double* __restrict__ g)
+{
+    const dclaw_params P = *params;
+    const long n = (long)mx * my;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(t % mx), j = 1 + (int)(t / mx);
+        g[t] = curvature_gacc(P, q, aux, maux, mx, i, j, dx, dy);
+    }
+}
+
+// g[t] <- grav + g[0] + ... + g[t], added one by one in index order (one
+// workgroup; chunks of 1024 through LDS; a chunk of zeros is skipped, since
+// adding +0.0 does not change the sum).
+__global__ __launch_bounds__(64) void k_curv_prefix(double* g, long n,
+                                                    const dclaw_params* __restrict__ params)
+{
+    __shared__ double buf[1024];
+    double gz = params->grav;
+    for (long base = 0; base < n; base += 1024) {
+        const int cnt = (int)min(1024L, n - base);
+        bool nz = false;
+        for (int k = threadIdx.x; k < cnt; k += 64) {
+            const double v = g[base + k];
+            buf[k] = v;
+            nz = nz || v != 0.0;
+        }
+        __syncthreads();
+        const bool any = __ballot(nz) != 0ull;   // uniform over the wavefront
+        if (any && threadIdx.x == 0) {
+            for (int k = 0; k < cnt; k++) {
+                gz = gz + buf[k];
+                buf[k] = gz;
+            }
+        }
+        __syncthreads();
+        gz = __shfl(gz, 0);
+        for (int k = threadIdx.x; k < cnt; k += 64) g[base + k] = any ? buf[k] : gz;
+        __syncthreads();
+    }
+}
+
+__global__ void k_src2_curv(double* q, const double* __restrict__ aux, int maux, int mx, int my,
+                            const double* __restrict__ dt_dev,
+                            const dclaw_params* __restrict__ params,
+                            const double* __restrict__ gz)
+{
+    const dclaw_params P = *params;
+    const double dt = *dt_dev;
+    const long n = (long)mx * my;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(t % mx), j = 1 + (int)(t / mx);
+        double* c = q + qidx(0, i, j, mx);
+        double h = c[0], hu = c[1], hv = c[2], hm = c[3], p = c[4], hchi = c[5], bdif = c[6];
+        src2_cell_gz(P, dt, h, hu, hv, hm, p, hchi, bdif, 0.0, gz[t]);
+        c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi; c[6] = bdif;
+    }
+}
+
 __global__ void k_check4nans(const double* __restrict__ q, long n, int* first)
 {
     int mine = 0x7fffffff;
@@ -1451,6 +1520,8 @@ struct HostCtx {
     size_t cap_qa = 0, cap_qb = 0, cap_aux = 0;
     double* d_flux[4] = {nullptr, nullptr, nullptr, nullptr};   // step2_ fm, fp, gm, gp
     size_t cap_flux[4] = {0, 0, 0, 0};
+    double* d_curv = nullptr;                                    // curvature workspace
+    size_t cap_curv = 0;
     int* d_nan = nullptr;                                        // check4nans result
 };
 HostCtx g_host;
@@ -1475,6 +1546,10 @@ int validate_params(const dclaw_params& P)
         return fail(DCLAW_EINVAL, "src2method must be -1, 0, 1 or 2");
     if (P.src2method == 2 && P.alphamethod != 1)
         return fail(DCLAW_EINVAL, "src2method 2 requires alphamethod 1 (data.py:362-365)");
+    if (P.curvature < 0 || P.curvature > 1) return fail(DCLAW_EINVAL, "curvature must be 0 or 1");
+    if (P.curvature == 1 && P.entrainment == 1)
+        return fail(DCLAW_EINVAL, "curvature with entrainment is not supported (src2 would read "
+                                  "beds it eroded earlier in the same sweep)");
     if (P.entrainment < 0 || P.entrainment > 1 || P.entrainment_method < 0 ||
         P.entrainment_method > 1)
         return fail(DCLAW_EINVAL, "entrainment and entrainment_method must be 0 or 1");
@@ -1969,6 +2044,86 @@ int dclaw_stepgrid_split_static_dev(const double* q_in, double* q_out, const dou
 
 int dclaw_static_workspace_ints(void) { return STATIC_FLAG_INTS; }
 
+size_t dclaw_curvature_workspace_doubles(int mx, int my)
+{
+    if (mx < 1 || my < 1) return 0;
+    // the running gz of every interior cell, then one tile flag per
+    // (strip, row) -- the most tiles any segment height gives
+    const size_t flags = (size_t)((mx + STRIP_W - 1) / STRIP_W) * (size_t)my;
+    return (size_t)mx * (size_t)my + (flags + 1) / 2;
+}
+
+static int launch_curvature_src2(double* q, const double* aux, int maux, int mx, int my,
+                                 double dx, double dy, const double* dt_dev,
+                                 const dclaw_params* params_dev, double* g, hipStream_t st)
+{
+    const long n = (long)mx * my;
+    hipLaunchKernelGGL(k_curv_gacc, dim3(grid_for(n)), dim3(256), 0, st, q, aux, maux, mx, my,
+                       dx, dy, params_dev, g);
+    hipLaunchKernelGGL(k_curv_prefix, dim3(1), dim3(64), 0, st, g, n, params_dev);
+    hipLaunchKernelGGL(k_src2_curv, dim3(grid_for(n)), dim3(256), 0, st, q, aux, maux, mx, my,
+                       dt_dev, params_dev, g);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+static int check_stage(int mx, int my, int maux, bool need_taudir);
+
+// The curvature step; aux_out (the host drop-in's aux, or null) receives
+// aux(4..6) as b4step2's calc_taudir leaves them.
+static int launch_curvature_step(const double* q_in, double* q_out, const double* aux,
+                                 double* aux_out, int maux, int mx, int my, double dx, double dy,
+                                 const int* bc, const double* dt_dev,
+                                 const dclaw_params* params_dev, double* cfl_dev, double* work,
+                                 size_t work_doubles, hipStream_t st)
+{
+    if (int rcb = check_bc(bc)) return rcb;
+    if (int rc = check_patch(mx, my, maux)) return rc;
+    if (maux < 3) return fail(DCLAW_EINVAL, "curvature needs maux >= 3 (aux(3) = theta)");
+    if (!q_in || !q_out || !aux || !dt_dev || !params_dev || !cfl_dev || !work)
+        return fail(DCLAW_EINVAL, "null pointer");
+    if (q_in == q_out) return fail(DCLAW_EINVAL, "q_out must not alias q_in");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+    if (work_doubles < dclaw_curvature_workspace_doubles(mx, my))
+        return fail(DCLAW_EINVAL, "curvature workspace too small");
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int seg_rows = g_seg_rows;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
+    const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
+    int* flags = (int*)(work + (size_t)mx * (size_t)my);
+    // the step without src2 (its dry reset included), then src2 with each
+    // cell's running gz
+    hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips * nseg), dim3(64), 0, st, q_in,
+                       q_out, aux, aux_out, maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev,
+                       seg_rows, nstrips, nseg, b0, b1, b2, b3, flags, nullptr);
+    HIP_TRY(hipGetLastError());
+    return launch_curvature_src2(q_out, aux, maux, mx, my, dx, dy, dt_dev, params_dev, work, st);
+}
+
+int dclaw_stepgrid_curvature_dev(const double* q_in, double* q_out, const double* aux, int maux,
+                                 int mx, int my, double dx, double dy, const int* bc,
+                                 const double* dt_dev, const dclaw_params* params_dev,
+                                 double* cfl_dev, double* work, size_t work_doubles,
+                                 void* stream)
+{
+    return launch_curvature_step(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, bc, dt_dev,
+                                 params_dev, cfl_dev, work, work_doubles, (hipStream_t)stream);
+}
+
+int dclaw_src2_curvature_dev(double* q, const double* aux, int maux, int mx, int my, double dx,
+                             double dy, const double* dt_dev, const dclaw_params* params_dev,
+                             double* work, size_t work_doubles, void* stream)
+{
+    if (int rc = check_stage(mx, my, maux, false)) return rc;
+    if (maux < 3) return fail(DCLAW_EINVAL, "curvature needs maux >= 3 (aux(3) = theta)");
+    if (!q || !aux || !dt_dev || !params_dev || !work) return fail(DCLAW_EINVAL, "null pointer");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
+    if (work_doubles < (size_t)mx * (size_t)my)
+        return fail(DCLAW_EINVAL, "curvature workspace too small");
+    return launch_curvature_src2(q, aux, maux, mx, my, dx, dy, dt_dev, params_dev, work,
+                                 (hipStream_t)stream);
+}
+
 int dclaw_worklist_size(const int* mx, const int* my, int npatch, int seg_rows)
 {
     if (!mx || !my || npatch < 0 || seg_rows < 1) return fail(DCLAW_EINVAL, "bad work-list args");
@@ -2288,9 +2443,17 @@ static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int ma
     HIP_TRY(hipMemcpy(g_host.d_qa, q, nq * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g_host.d_aux, aux, na * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
-    rc = launch_stepgrid(g_host.d_qa, g_host.d_qb, g_host.d_aux, g_host.d_aux, maux, mx, my, dx,
-                         dy, g_host.d_scalars, g_host.d_params, g_host.d_scalars + 1, nullptr,
-                         nullptr);
+    if (g_host.params.curvature == 1) {
+        const size_t nw = dclaw_curvature_workspace_doubles(mx, my);
+        if ((rc = grow(&g_host.d_curv, &g_host.cap_curv, nw))) return rc;
+        rc = launch_curvature_step(g_host.d_qa, g_host.d_qb, g_host.d_aux, g_host.d_aux, maux, mx,
+                                   my, dx, dy, nullptr, g_host.d_scalars, g_host.d_params,
+                                   g_host.d_scalars + 1, g_host.d_curv, nw, nullptr);
+    } else {
+        rc = launch_stepgrid(g_host.d_qa, g_host.d_qb, g_host.d_aux, g_host.d_aux, maux, mx, my,
+                             dx, dy, g_host.d_scalars, g_host.d_params, g_host.d_scalars + 1,
+                             nullptr, nullptr);
+    }
     if (rc) return rc;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(q, g_host.d_qb, nq * sizeof(double), hipMemcpyDeviceToHost));
@@ -2342,6 +2505,8 @@ static int check_host_stage(int mbc, int meqn, int mx, int my, int maux, bool ne
     if (!g_host.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
     if (g_host.params.entrainment && maux < 7)
         return fail(DCLAW_EINVAL, "entrainment needs maux >= 7 (aux(7) holds h_e)");
+    if (g_host.params.curvature && maux < 3)
+        return fail(DCLAW_EINVAL, "curvature needs maux >= 3 (aux(3) = theta)");
     return DCLAW_OK;
 }
 
@@ -2397,7 +2562,7 @@ static int step2_host(int meqn, int maux, int mbc, int mx, int my, const double*
 }
 
 static int src2_host(int meqn, int mbc, int mx, int my, double* q, int maux, const double* aux,
-                     double t, double dt)
+                     double dx, double dy, double t, double dt)
 {
     std::lock_guard<std::mutex> lk(g_host.mu);
     if (int rc = check_host_stage(mbc, meqn, mx, my, maux, false)) return rc;
@@ -2406,9 +2571,18 @@ static int src2_host(int meqn, int mbc, int mx, int my, double* q, int maux, con
     if (int rc = host_upload(q, aux, mx, my, maux)) return rc;
     double scal[2] = {dt, 0.0};
     HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
-    if (int rc = dclaw_src2_dev(g_host.d_qa, g_host.d_aux, maux, mx, my, g_host.d_scalars,
-                                g_host.d_params, g_host.d_nan, nullptr))
+    if (g_host.params.curvature == 1) {
+        const size_t nw = (size_t)mx * (size_t)my;
+        if (int rc = grow(&g_host.d_curv, &g_host.cap_curv, nw)) return rc;
+        if (int rc = dclaw_check4nans_dev(g_host.d_qa, mx, my, g_host.d_nan, nullptr)) return rc;
+        if (int rc = dclaw_src2_curvature_dev(g_host.d_qa, g_host.d_aux, maux, mx, my, dx, dy,
+                                              g_host.d_scalars, g_host.d_params, g_host.d_curv,
+                                              nw, nullptr))
+            return rc;
+    } else if (int rc = dclaw_src2_dev(g_host.d_qa, g_host.d_aux, maux, mx, my, g_host.d_scalars,
+                                       g_host.d_params, g_host.d_nan, nullptr)) {
         return rc;
+    }
     HIP_TRY(hipDeviceSynchronize());
     host_nan_stop(q, mx, my, t, 2);
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
@@ -2439,8 +2613,8 @@ void src2_(const int* meqn, const int* mbc, const int* mx, const int* my, const 
            const double* ylower, const double* dx, const double* dy, double* q, const int* maux,
            const double* aux, const double* t, const double* dt)
 {
-    (void)xlower; (void)ylower; (void)dx; (void)dy;
-    host_stop_on_error("src2", src2_host(*meqn, *mbc, *mx, *my, q, *maux, aux, *t, *dt));
+    (void)xlower; (void)ylower;
+    host_stop_on_error("src2", src2_host(*meqn, *mbc, *mx, *my, q, *maux, aux, *dx, *dy, *t, *dt));
 }
 
 void stepgrid_dclaw_(double* q, double* aux, const int* mbc, const int* mx, const int* my,

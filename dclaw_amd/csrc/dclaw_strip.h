@@ -210,6 +210,34 @@ DC_HD void src2_tile_cell(const dclaw_params& P, double dt, double* q, const dou
     c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi; c[6] = bdif;
 }
 
+// Curvature acceleration of interior cell (i, j) at src2's entry state
+// (src2.f90:133-156): centred second differences of the bed b = aux(1) -
+// bdif, the squared velocity along them and the bed-angle gradient of
+// aux(3), clipped at 0.  0 for a cell src2 skips (dry, or hm < 0), so
+// adding it to the sweep's running gz changes nothing there, as the skip
+// does.  maux >= 3.
+DC_HD double curvature_gacc(const dclaw_params& P, const double* q, const double* aux,
+                            int maux, int mx, int i, int j, double dx, double dy)
+{
+    const double* c = q + qidx(0, i, j, mx);
+    const double h = c[0];
+    if (h <= P.dry_tolerance) return 0.0;
+    if (c[3] < 0.0) return 0.0;
+#define DC_BD(ii, jj) (aux[aidx(ii, jj, mx, maux)] - q[qidx(6, ii, jj, mx)])
+    const double b = DC_BD(i, j), bL = DC_BD(i - 1, j), bR = DC_BD(i + 1, j);
+    const double bT = DC_BD(i, j + 1), bB = DC_BD(i, j - 1);
+    const double bTR = DC_BD(i + 1, j + 1), bTL = DC_BD(i - 1, j + 1);
+    const double bBR = DC_BD(i + 1, j - 1), bBL = DC_BD(i - 1, j - 1);
+#undef DC_BD
+    const double b_xx = (bR - 2.0 * b + bL) / (dx * dx);
+    const double b_yy = (bT - 2.0 * b + bB) / (dy * dy);
+    const double b_xy = ((bTR - bTL) - (bBR - bBL)) / (4.0 * dx * dy);
+    const double u = c[1] / h, v = c[2] / h;
+    const double dtheta = -((aux[aidx(i + 1, j, mx, maux) + 2] -
+                             aux[aidx(i - 1, j, mx, maux) + 2]) / (2.0 * dx));
+    return dmax(u * u * b_xx + v * v * b_yy + 2.0 * u * v * b_xy + u * u * dtheta, 0.0);
+}
+
 // Dry pass of the split step over a group of consecutive segments of one
 // strip: A.jA..A.jB span the group, cut into segments of seg_rows rows from
 // A.jA (the last possibly shorter).  One scan of rows A.jA-1..A.jB+1 answers

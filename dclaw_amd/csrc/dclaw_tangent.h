@@ -122,6 +122,7 @@ struct Params {
     Dual entrainment_rate, me, chie;
     int entrainment;
     int entrainment_method;
+    int curvature;
 };
 
 DC_HD Params make_params(const dclaw_params& p, const dclaw_params& dp)
@@ -150,6 +151,7 @@ DC_HD Params make_params(const dclaw_params& p, const dclaw_params& dp)
     o.mwaves = p.mwaves;
     o.entrainment = p.entrainment;
     o.entrainment_method = p.entrainment_method;
+    o.curvature = p.curvature;
     return o;
 }
 

@@ -245,6 +245,7 @@ struct Params {
     DualK entrainment_rate, me, chie;
     int entrainment;
     int entrainment_method;
+    int curvature;
 };
 
 // Values p; tangent k is dp[k]'s field, or 0 where dp[k] is null.
@@ -276,6 +277,7 @@ DC_HD Params make_params(const dclaw_params& p, const dclaw_params* const* dp)
     o.mwaves = p.mwaves;
     o.entrainment = p.entrainment;
     o.entrainment_method = p.entrainment_method;
+    o.curvature = p.curvature;
     return o;
 }
 

@@ -83,9 +83,11 @@ class DigParams:
             raise ValueError("src2method == 2 requires alphamethod = 1 (data.py:362-365)")
         if self.src2method not in (-1, 0, 1, 2):
             raise ValueError("src2method must be -1, 0, 1 or 2")
-        if self.curvature != 0:
-            raise ValueError("curvature is not implemented on this path (the reference "
-                             "accumulates gz over the cells of src2's loop, src2.f90:87,156)")
+        if self.curvature not in (0, 1):
+            raise ValueError("curvature must be 0 or 1")
+        if self.curvature and self.entrainment:
+            raise ValueError("curvature with entrainment is not supported: src2 then reads "
+                             "the bed of cells it has already eroded in the same sweep")
         if self.entrainment not in (0, 1) or self.entrainment_method not in (0, 1):
             raise ValueError("entrainment and entrainment_method must be 0 or 1")
         if self.mwaves != 5:
@@ -135,6 +137,7 @@ class CParams(ctypes.Structure):
         ("chie", ctypes.c_double),
         ("entrainment", ctypes.c_int),
         ("entrainment_method", ctypes.c_int),
+        ("curvature", ctypes.c_int),
     ]
 
 

@@ -67,6 +67,10 @@ def aux_planes(aux: np.ndarray, params, device="cuda"):
         if aux.shape[0] < 7:
             raise ValueError("entrainment needs maux >= 7 (aux(7) = h_e)")
         return aux_tensor(aux, device), aux.shape[0]
+    if getattr(params, "curvature", 0):
+        if aux.shape[0] < 3:
+            raise ValueError("curvature needs maux >= 3 (aux(3) = theta)")
+        return aux_tensor(aux, device), aux.shape[0]
     return bed_tensor(aux, device), 1
 
 
@@ -176,6 +180,23 @@ def stepgrid_split(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, m
         _ptr(params_dev), _ptr(cfl_dev), _ptr(flags), n, _stream()))
 
 
+def stepgrid_curvature(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int,
+                       mx: int, my: int, dx: float, dy: float, bc, dt_dev: torch.Tensor,
+                       params_dev: torch.Tensor, cfl_dev: torch.Tensor,
+                       work: torch.Tensor) -> None:
+    """The step with src2's curvature term (each cell's gravity includes the
+    curvature accelerations of the cells before it in src2's sweep);
+    work: float64 tensor of at least curvature_workspace_doubles(mx, my)."""
+    arr = None if bc is None else (ctypes.c_int * 4)(*bc)
+    _lib.check(_lib.lib().dclaw_stepgrid_curvature_dev(
+        _ptr(q_in), _ptr(q_out), _ptr(aux), maux, mx, my, dx, dy, arr, _ptr(dt_dev),
+        _ptr(params_dev), _ptr(cfl_dev), _ptr(work), work.numel(), _stream()))
+
+
+def curvature_workspace_doubles(mx: int, my: int) -> int:
+    return int(_lib.lib().dclaw_curvature_workspace_doubles(mx, my))
+
+
 def dt_update(dt_dev: torch.Tensor, cfl_dev: torch.Tensor, cfl_desired: float,
               dt_max: float) -> None:
     _lib.check(_lib.lib().dclaw_dt_update(_ptr(dt_dev), _ptr(cfl_dev), dt_dev.numel(),
@@ -240,6 +261,10 @@ class PatchRunner:
         # most): tile flags, the wet list or tile order, and counters
         nstrips = -(-self.mx // _lib.lib().dclaw_strip_width())
         self.flags = torch.zeros(3 * nstrips * self.my + 4, dtype=torch.int32, device=device)
+        # curvature: the step with src2's running gravity (its own workspace)
+        self.curv_work = (torch.zeros(curvature_workspace_doubles(self.mx, self.my),
+                                      dtype=torch.float64, device=device)
+                          if getattr(params, "curvature", 0) else None)
 
     def set_dt(self, dt: float) -> None:
         self.dt.fill_(float(dt))
@@ -247,7 +272,13 @@ class PatchRunner:
     def step(self, fused_bc: bool = True) -> None:
         """bound + stepgrid (boundary conditions applied on load when fused_bc);
         the result is left in self.qa and cflgrid in self.cfl."""
-        if fused_bc and self.split_src2:
+        if self.curv_work is not None:
+            if not fused_bc:
+                bc_fill(self.qa, self.mx, self.my, self.bc)
+            stepgrid_curvature(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx,
+                               self.dy, self.bc if fused_bc else None, self.dt, self.params,
+                               self.cfl, self.curv_work)
+        elif fused_bc and self.split_src2:
             stepgrid_split(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
                            self.bc, self.dt, self.params, self.cfl, self.flags, self.split_mode)
         elif fused_bc:
@@ -381,6 +412,9 @@ class Ensemble:
         self.split_src2 = split_src2
         self.ordered = ordered
         self.n = len(members)
+        if any(getattr(m.patch.params, "curvature", 0) for m in members):
+            raise ValueError("the batched step has no curvature path (src2's running gravity "
+                             "is a sweep over each patch); step such members with PatchRunner")
         self.seg_rows = seg_rows
         self.bc = tuple(bc)
         self.mx = [m.patch.mx for m in members]

@@ -70,6 +70,9 @@ typedef struct dclaw_params {
     double entrainment_rate, me, chie;
     int entrainment;            /* 0 or 1 */
     int entrainment_method;     /* 0: ent_dclaw4; 1: no update (the reference's default) */
+    int curvature;              /* 1: bed-curvature acceleration in src2 (src2.f90:150-156);
+                                   only dclaw_stepgrid_curvature_dev and the host drop-ins
+                                   apply it (needs aux(3) = theta: maux >= 3) */
 } dclaw_params;
 
 /* One patch of a (possibly batched) step.  All pointers are device pointers. */
@@ -216,6 +219,32 @@ int dclaw_stepgrid_split_static_dev(const double* q_in, double* q_out, const dou
 
 /* Ints of the static workspace of dclaw_stepgrid_split_static_dev. */
 int dclaw_static_workspace_ints(void);
+
+/* The step with src2's curvature term (params.curvature = 1; src2.f90:133-156).
+ * The reference sets gz once per src2 call (:87) and adds each wet cell's
+ * curvature acceleration max(u^2 b_xx + v^2 b_yy + 2uv b_xy + u^2 dtheta, 0)
+ * to it without a reset (:156), so a cell's gravity includes every earlier
+ * cell's in src2's loop order (j outer, i inner).  This entry point forms
+ * that running sum exactly in that order (one workgroup), between the step
+ * without src2 and a parallel src2 that gives each cell its own gz; the
+ * other stepping entry points, which cannot, mark a wet cell's pb NaN when
+ * curvature is on.  aux(3) = theta is read (maux >= 3), and aux(4..6) are
+ * not written.  work: device workspace of
+ * dclaw_curvature_workspace_doubles(mx, my) doubles, overwritten.  Curvature
+ * with entrainment is rejected (src2 would read beds eroded earlier in the
+ * same sweep).  Replaces stepgrid with src2's curvature branch. */
+size_t dclaw_curvature_workspace_doubles(int mx, int my);
+int dclaw_stepgrid_curvature_dev(const double* q_in, double* q_out, const double* aux, int maux,
+                                 int mx, int my, double dx, double dy, const int* bc,
+                                 const double* dt_dev, const dclaw_params* params_dev,
+                                 double* cfl_dev, double* work, size_t work_doubles,
+                                 void* stream);
+
+/* src2 alone with the curvature term (src2.f90:27-306 with curvature = 1) on
+ * the interior of q in place; work: mx * my doubles, overwritten. */
+int dclaw_src2_curvature_dev(double* q, const double* aux, int maux, int mx, int my, double dx,
+                             double dy, const double* dt_dev, const dclaw_params* params_dev,
+                             double* work, size_t work_doubles, void* stream);
 
 /* Batched fused step over npatch patches (device array of dclaw_patch).
  * One launch; the work list of (patch, strip, segment) tiles is built on

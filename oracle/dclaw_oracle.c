@@ -19,8 +19,7 @@
  * accuracy), not by reference source.  The physics of every routine is
  * pinned by consequences of the governing equations (tests/test_oracle_*.py),
  * and tests/test_oracle_mutants.py checks that those pins reject altered
- * copies of this file.  Not restated (not on the product path): the
- * curvature term of src2.
+ * copies of this file.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -1355,12 +1354,16 @@ static void ent_dclaw4(const oracle_params* P, double dt, double* h, double u, d
     }
 }
 
-void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, double* q,
-                 const double* aux, double dt)
+void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, double dx,
+                 double dy, double* q, const double* aux, double dt)
 {
     const double drytol = P->dry_tolerance, rho_s = P->rho_s, rho_f = P->rho_f, mu = P->mu;
     double coeffmanning = P->friction_forcing ? P->manning_coefficient : 0.0;
-    const double gz = P->grav, gx = 0.0;
+    /* gz is set once (src2.f90:87) and, with curvature, each wet cell adds its
+     * curvature acceleration to it (:156) without a reset: a cell's gravity
+     * includes every earlier cell's in loop order (j outer, i inner) */
+    double gz = P->grav;
+    const double gx = 0.0;
     for (int j = 1; j <= my; j++)
         for (int i = 1; i <= mx; i++) {
             double h = P2(q, MEQN, I_H, i, j);
@@ -1373,6 +1376,23 @@ void oracle_src2(const oracle_params* P, int mx, int my, int mbc, int maux, doub
             double hchi = P2(q, MEQN, I_HCHI, i, j);
             double rhoh = hm * rho_s + (h - hm) * rho_f;
             if (hm < 0.0) continue;
+            if (P->curvature == 1) {                                   /* :133-156 */
+#define BD(ii, jj) (P2(aux, maux, 1, ii, jj) - P2(q, MEQN, I_BDIF, ii, jj))
+                const double b = BD(i, j), bL = BD(i - 1, j), bR = BD(i + 1, j);
+                const double bT = BD(i, j + 1), bB = BD(i, j - 1);
+                const double bTR = BD(i + 1, j + 1), bTL = BD(i - 1, j + 1);
+                const double bBR = BD(i + 1, j - 1), bBL = BD(i - 1, j - 1);
+#undef BD
+                const double b_xx = (bR - 2.0 * b + bL) / (dx * dx);
+                const double b_yy = (bT - 2.0 * b + bB) / (dy * dy);
+                const double b_xy = ((bTR - bTL) - (bBR - bBL)) / (4.0 * dx * dy);
+                const double uc = hu / h, vc = hv / h;
+                const double dtheta = -((P2(aux, maux, I_THETA, i + 1, j) -
+                                         P2(aux, maux, I_THETA, i - 1, j)) / (2.0 * dx));
+                const double gacc = dmax(uc * uc * b_xx + vc * vc * b_yy + 2.0 * uc * vc * b_xy +
+                                         uc * uc * dtheta, 0.0);
+                gz = gz + gacc;
+            }
 
             double u, v, m, chi, rho, kperm, alphainv, m_eq, tanpsi, tau;
             qfix(P, &h, &hu, &hv, &hm, &p, &hchi, &u, &v, &m, &chi, &rho, gz);
@@ -1475,7 +1495,7 @@ double oracle_stepgrid(oracle_params* P, int mx, int my, int mbc, int maux, doub
                 P2(q, MEQN, m, i, j) = P2(q, MEQN, m, i, j)
                     - dtdx * (P2(fm, MEQN, m, i + 1, j) - P2(fp, MEQN, m, i, j))
                     - dtdy * (P2(gm, MEQN, m, i, j + 1) - P2(gp, MEQN, m, i, j));
-    oracle_src2(P, mx, my, mbc, maux, q, aux, dt);
+    oracle_src2(P, mx, my, mbc, maux, dx, dy, q, aux, dt);
     return cflgrid;
 }
 

@@ -44,6 +44,7 @@ typedef struct oracle_params {
     double entrainment_rate, me, chie;
     int entrainment;            /* 0 or 1 */
     int entrainment_method;     /* 0: ent_dclaw4; 1: no update (the reference's default) */
+    int curvature;              /* src2.f90:150-156 */
 } oracle_params;
 
 /* ---- point routines (for unit / known-answer tests) ---- */
@@ -75,8 +76,8 @@ void oracle_b4step2(oracle_params* p, int mx, int my, int mbc, int maux,
 void oracle_step2(const oracle_params* p, int mx, int my, int mbc, int maux,
                   const double* qold, const double* aux, double dx, double dy, double dt,
                   double* cflgrid, double* fm, double* fp, double* gm, double* gp);
-void oracle_src2(const oracle_params* p, int mx, int my, int mbc, int maux,
-                 double* q, const double* aux, double dt);
+void oracle_src2(const oracle_params* p, int mx, int my, int mbc, int maux, double dx,
+                 double dy, double* q, const double* aux, double dt);
 /* One stepgrid: b4step2, step2, conservative update, src2.  `work` must hold
  * 4*7*(mx+2mbc)*(my+2mbc) doubles (fm, fp, gm, gp).  Returns cflgrid. */
 double oracle_stepgrid(oracle_params* p, int mx, int my, int mbc, int maux,

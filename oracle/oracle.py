@@ -60,6 +60,7 @@ class OracleParams(ctypes.Structure):
         ("chie", ctypes.c_double),
         ("entrainment", ctypes.c_int),
         ("entrainment_method", ctypes.c_int),
+        ("curvature", ctypes.c_int),
     ]
 
 
@@ -125,7 +126,7 @@ def lib():
                                    _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    _dp, _dp, _dp, _dp, _dp]
         L.oracle_src2.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                  _dp, _dp, ctypes.c_double]
+                                  ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_double]
         L.oracle_stepgrid.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                       _dp, _dp]
@@ -286,11 +287,13 @@ def step2(params, q, aux, dx, dy, dt, mbc=2):
     return cfl.value, fm, fp, gm, gp
 
 
-def src2(params, q, aux, dt, mbc=2):
+def src2(params, q, aux, dt, mbc=2, dx=1.0, dy=1.0):
+    """src2 on the interior; dx, dy are read only with curvature on."""
     q = np.array(q, dtype=np.float64, order="F", copy=True)
     aux = np.asfortranarray(aux, dtype=np.float64)
     mx, my = _dims(q, mbc)
-    lib().oracle_src2(ctypes.byref(params), mx, my, mbc, aux.shape[0], _ptr(q), _ptr(aux), dt)
+    lib().oracle_src2(ctypes.byref(params), mx, my, mbc, aux.shape[0], dx, dy, _ptr(q),
+                      _ptr(aux), dt)
     return q
 
 

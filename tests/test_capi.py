@@ -61,7 +61,7 @@ def test_params_struct_layout():
     from dclaw_amd.params import CParams
     from oracle.oracle import OracleParams
     assert [f[0] for f in CParams._fields_] == [f[0] for f in OracleParams._fields_]
-    assert ctypes.sizeof(CParams) == 20 * 8 + 14 * 4 + 3 * 8 + 2 * 4
+    assert ctypes.sizeof(CParams) == 20 * 8 + 14 * 4 + 3 * 8 + 3 * 4 + 4     # + padding
 
 
 def test_stage_argument_validation(L):

@@ -108,6 +108,10 @@ MUTANTS = [
     ("FE_4quad m rate with rho0 for rho0^2",
      "const double km0 = ((2.0 * kperm * (rho0 * rho0)) / (mu * (rhoh * rhoh)));",
      "const double km0 = ((2.0 * kperm * (rho0)) / (mu * (rhoh * rhoh)));"),
+    ("curvature gravity reset for every cell",
+     "                gz = gz + gacc;", "                gz = P->grav + gacc;"),
+    ("curvature acceleration with u instead of u^2",
+     "dmax(uc * uc * b_xx + vc * vc * b_yy", "dmax(uc * b_xx + vc * vc * b_yy"),
     ("qfix pb bound rho g h -> rho_f g h",
      "double pmax = *rho * gz * *h;", "double pmax = P->rho_f * gz * *h;"),
 ]

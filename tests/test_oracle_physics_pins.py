@@ -481,6 +481,32 @@ def pin_fe4quad_rates(O):
     assert math.isnan(O.src2(prm, q, aux, 0.01)[4, 2, 2])
 
 
+def pin_curvature_running_gravity(O):
+    """src2 with curvature on a parabolic bed b = kappa x^2 / 2 (b_xx = kappa,
+    the centred second difference of a quadratic is exact) under a uniform
+    flow u: each wet cell's curvature acceleration is u^2 kappa, and the
+    gravity src2 uses is set once and raised by every wet cell in loop order
+    (src2.f90:87, 152-156: j outer, i inner).  A pore pressure above
+    lithostatic is clipped by qfix to rho gz h, which makes cell n of the
+    sweep show pb = rho h (g + n u^2 kappa) (src2method -1: nothing else
+    changes pb)."""
+    p, prm = _params(O, src2method=-1, curvature=1)
+    mx, my, dx, dy = 10, 6, 2.0, 1.5
+    kappa, u, h, m = 0.05, 2.0, 1.0, 0.6
+    x = (np.arange(-2, mx + 2) + 0.5) * dx
+    aux = np.zeros((3, mx + 4, my + 4), order="F")
+    aux[0] = (0.5 * kappa * x * x)[:, None]
+    q = np.zeros((7, mx + 4, my + 4), order="F")
+    q[0], q[1], q[3], q[4] = h, h * u, m * h, 1e7
+    out = O.src2(prm, q, aux, 1e-3, dx=dx, dy=dy)
+    rho = m * p.rho_s + (1 - m) * p.rho_f
+    gacc = u * u * kappa
+    for j in range(1, my + 1):
+        for i in range(1, mx + 1):
+            n = (j - 1) * mx + i
+            assert out[4, i + 1, j + 1] == pytest.approx(rho * (G + n * gacc) * h, rel=1e-12), (i, j)
+
+
 def pin_speed_limit(O):
     """b4step2: a cell faster than speed_limit is slowed to exactly the limit
     with its direction kept; slower cells are untouched."""
@@ -531,7 +557,7 @@ PINS = [pin_rpt2_gravity_eigenvectors, pin_rpt2_uses_the_receiving_cell,
         pin_solver_stationary_bed_step, pin_rpn2_fluctuations, pin_basal_friction_rates,
         pin_pressure_relaxation_rate, pin_dilatancy_rate, pin_speed_limit,
         pin_still_impermeable_cell_keeps_pressure, pin_static_heap_threshold,
-        pin_calc_taudir_still_cell, pin_fe4quad_rates,
+        pin_calc_taudir_still_cell, pin_fe4quad_rates, pin_curvature_running_gravity,
         pin_second_order_1d, pin_second_order_2d]
 
 
