@@ -72,8 +72,10 @@ constexpr int NTI = 37;     // taudir inputs: 5 x (Q0 state 6 + bdif), rho1, tau
 constexpr int NSI = 12;     // split inputs: h, hu, hv, hm2, hchi2, bdif, asdq[6]
 constexpr int MAXP = 4;     // parameter directions per call
 
-// Workspace fields (each a plane of N = (mx+4)(my+4) doubles, cell (i, j)
-// at (i+1) + (mx+4)(j+1); an edge's values sit at its right/top cell).  A
+// Workspace fields (each a plane of N = ld (my+4) doubles, cell (i, j) at
+// (i+1) + ld (j+1) with the row stride ld = mx+4 rounded up to 16 doubles, so
+// that every row of a plane, and every 16-cell row of a tile, starts on a
+// 128-byte line; an edge's values sit at its right/top cell).  A
 // reverse-pass field shares planes with record fields that are dead by the
 // time it is written (no stage reads a plane it or a neighbour writes in the
 // same pass):
@@ -121,14 +123,18 @@ struct AdjArgs {
     double* lam_in;          // lambda_n, layout of q
     double* q_out;           // q_{n+1} (the record pass; optional)
     double* W;               // workspace, W_FIELDS planes
-    size_t N;
+    size_t N;                // plane size, plane_size(mx, my)
+    int ld;                  // plane row stride, plane_ld(mx)
     const dclaw_params* Pd;  // device copies of P and dt read by each GPU thread (or null)
     const double* dt_dev;
 };
 
+DC_HD int plane_ld(int mx) { return (mx + 4 + 15) / 16 * 16; }
+DC_HD size_t plane_size(int mx, int my) { return (size_t)plane_ld(mx) * (size_t)(my + 4); }
+
 DC_HD size_t cix(const AdjArgs& A, int i, int j)
 {
-    return (size_t)(i + 1) + (size_t)(A.mx + 4) * (size_t)(j + 1);
+    return (size_t)(i + 1) + (size_t)A.ld * (size_t)(j + 1);
 }
 DC_HD double* plane(const AdjArgs& A, int f) { return A.W + (size_t)f * A.N; }
 DC_HD double wv(const AdjArgs& A, int f, size_t n) { return A.W[(size_t)f * A.N + n]; }

@@ -946,7 +946,7 @@ __global__ void k_gauges(const double* __restrict__ q, const double* __restrict_
 
 
 // ---- adjoint step (dclaw_adjoint.h), tile-sparse: the stages run over the
-// 8 x 8 cell tiles within ADJ_HALO cells of a wet cell (and the tiles of the
+// 16 x 4 cell tiles within ADJ_HALO cells of a wet cell (and the tiles of the
 // border band, where the boundary map gathers ghost adjoints), one wavefront
 // per tile, tiles taken from a compacted list by a persistent grid.  Outside
 // those tiles the step maps a cell to its dry state and every intermediate of
@@ -960,7 +960,11 @@ enum AdjStage {
     ADJ_REV_SRC2, ADJ_REV_SPLIT, ADJ_REV_FLUX, ADJ_REV_WAVES, ADJ_REV_TAUDIR, ADJ_REV_PREPARE,
     ADJ_REV_BC, ADJ_NSTAGES
 };
-constexpr int ADJ_TILE = 8;       // tile side (64 cells: one wavefront)
+// Tiles are 16 cells wide and 4 high (64 cells: one wavefront): a tile row
+// of a workspace plane is one 128-byte line (the planes' row stride is a
+// multiple of 16 doubles), so a tile reads whole lines of its own cells.
+constexpr int ADJ_TW = 16;
+constexpr int ADJ_TH = 4;
 constexpr int ADJ_HALO = 5;       // cells around a wet cell whose tiles are active
 constexpr int ADJ_BLOCK = 256;    // dense passes
 constexpr int ADJ_GRID = 8192;    // persistent grid of the tile passes
@@ -1004,8 +1008,8 @@ struct AdjTiles {
 __host__ __device__ inline AdjTiles adj_tiles(double* work, int mx, int my, size_t N)
 {
     AdjTiles T;
-    T.ntx = (mx + 4 + ADJ_TILE - 1) / ADJ_TILE;
-    T.nty = (my + 4 + ADJ_TILE - 1) / ADJ_TILE;
+    T.ntx = (mx + 4 + ADJ_TW - 1) / ADJ_TW;
+    T.nty = (my + 4 + ADJ_TH - 1) / ADJ_TH;
     const size_t ntiles = (size_t)T.ntx * (size_t)T.nty;
     int* ints = (int*)(work + (size_t)dclaw_adj::W_FIELDS * N +
                        (size_t)dclaw_adj::MAXP * ADJ_GP_PARTS);
@@ -1070,8 +1074,8 @@ __device__ __forceinline__ void adj_stage(const dclaw_adj::AdjArgs& A, const dcl
 __device__ __forceinline__ bool adj_tile_cell(const dclaw_adj::AdjArgs& A, const AdjTiles& T,
                                               int t, int& i, int& j)
 {
-    i = -1 + (t % T.ntx) * ADJ_TILE + (int)(threadIdx.x % ADJ_TILE);
-    j = -1 + (t / T.ntx) * ADJ_TILE + (int)(threadIdx.x / ADJ_TILE);
+    i = -1 + (t % T.ntx) * ADJ_TW + (int)(threadIdx.x % ADJ_TW);
+    j = -1 + (t / T.ntx) * ADJ_TH + (int)(threadIdx.x / ADJ_TW);
     return i <= A.mx + 2 && j <= A.my + 2;
 }
 
@@ -1090,8 +1094,8 @@ __device__ __forceinline__ void adj_clear_cell(const dclaw_adj::AdjArgs& A, size
 __device__ __forceinline__ void adj_dense_cell(const dclaw_adj::AdjArgs& A, const AdjTiles& T,
                                                size_t n)
 {
-    const int w = A.mx + 4;
-    const int i = (int)(n % (size_t)w) - 1, j = (int)(n / (size_t)w) - 1;
+    const int i = (int)(n % (size_t)A.ld) - 1, j = (int)(n / (size_t)A.ld) - 1;
+    if (i > A.mx + 2) return;  // row padding: never read, stays 0
     const double drytol = A.P.dry_tolerance;
     double q7[7], fu, fv;
     dclaw_adj::load_q(A, i, j, q7, fu, fv);
@@ -1112,12 +1116,12 @@ __device__ __forceinline__ void adj_dense_cell(const dclaw_adj::AdjArgs& A, cons
     const bool border = i <= 2 || i >= A.mx - 1 || j <= 2 || j >= A.my - 1;
     const bool wet = !(drytol > 0.0) || q7[0] > drytol;
     if (border && !wet) {
-        T.flags[((j + 1) / ADJ_TILE) * T.ntx + (i + 1) / ADJ_TILE] = 1;
+        T.flags[((j + 1) / ADJ_TH) * T.ntx + (i + 1) / ADJ_TW] = 1;
     } else if (wet) {
         const int ia = max(i - ADJ_HALO, -1), ib = min(i + ADJ_HALO, A.mx + 2);
         const int ja = max(j - ADJ_HALO, -1), jb = min(j + ADJ_HALO, A.my + 2);
-        for (int ty = (ja + 1) / ADJ_TILE; ty <= (jb + 1) / ADJ_TILE; ty++)
-            for (int tx = (ia + 1) / ADJ_TILE; tx <= (ib + 1) / ADJ_TILE; tx++)
+        for (int ty = (ja + 1) / ADJ_TH; ty <= (jb + 1) / ADJ_TH; ty++)
+            for (int tx = (ia + 1) / ADJ_TW; tx <= (ib + 1) / ADJ_TW; tx++)
                 T.flags[ty * T.ntx + tx] = 1;
     }
 }
@@ -1291,7 +1295,8 @@ __device__ dclaw_adj::AdjArgs adj_member(const AdjBatch& B, int m, AdjTiles& T)
     A.lam_in = p.lam_in;
     A.q_out = p.q_out;
     A.W = p.work;
-    A.N = (size_t)(p.mx + 4) * (size_t)(p.my + 4);
+    A.N = dclaw_adj::plane_size(p.mx, p.my);
+    A.ld = dclaw_adj::plane_ld(p.mx);
     T = adj_tiles(p.work, p.mx, p.my, A.N);
     return A;
 }
@@ -1309,8 +1314,8 @@ __global__ __launch_bounds__(1024) void k_adjb_prefix(AdjBatch B)
         long long c = 0, t = 0;
         if (m < B.npatch) {
             const int mx = B.pt[m].mx, my = B.pt[m].my;
-            c = (long long)(mx + 4) * (my + 4);
-            t = (long long)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) * ((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+            c = (long long)dclaw_adj::plane_size(mx, my);
+            t = (long long)((mx + 4 + ADJ_TW - 1) / ADJ_TW) * ((my + 4 + ADJ_TH - 1) / ADJ_TH);
         }
         sc[tid] = c;
         st[tid] = t;
@@ -1366,7 +1371,7 @@ __global__ __launch_bounds__(256) void k_adjb_count_zero(AdjBatch B)
     if (m == 0) *B.gcount = 0;
     if (m >= B.npatch) return;
     const dclaw_adj_patch& p = B.pt[m];
-    adj_tiles(p.work, p.mx, p.my, (size_t)(p.mx + 4) * (size_t)(p.my + 4)).count[0] = 0;
+    adj_tiles(p.work, p.mx, p.my, dclaw_adj::plane_size(p.mx, p.my)).count[0] = 0;
 }
 
 // Whether global tile g (< ntiles) is marked, and its patch's bookkeeping.
@@ -1374,7 +1379,7 @@ __device__ __forceinline__ bool adjb_marked(const AdjBatch& B, long long g, AdjT
 {
     const int m = adj_find(B.tile_prefix, B.npatch, g);
     const dclaw_adj_patch& p = B.pt[m];
-    T = adj_tiles(p.work, p.mx, p.my, (size_t)(p.mx + 4) * (size_t)(p.my + 4));
+    T = adj_tiles(p.work, p.mx, p.my, dclaw_adj::plane_size(p.mx, p.my));
     t = (int)(g - B.tile_prefix[m]);
     return T.flags[t] != 0;
 }
@@ -1445,14 +1450,14 @@ __global__ __launch_bounds__(64, 2) void k_adjb2(AdjBatch B)
 __global__ __launch_bounds__(256) void k_adjb_gp(AdjBatch B)
 {
     const dclaw_adj_patch& p = B.pt[blockIdx.z];
-    const size_t N = (size_t)(p.mx + 4) * (size_t)(p.my + 4);
+    const size_t N = dclaw_adj::plane_size(p.mx, p.my);
     adj_gp_part(p.work, N, blockIdx.y, blockIdx.x, p.work + (size_t)dclaw_adj::W_FIELDS * N);
 }
 
 __global__ __launch_bounds__(ADJ_GP_PARTS) void k_adjb_gp_final(AdjBatch B)
 {
     const dclaw_adj_patch& p = B.pt[blockIdx.y];
-    const size_t N = (size_t)(p.mx + 4) * (size_t)(p.my + 4);
+    const size_t N = dclaw_adj::plane_size(p.mx, p.my);
     adj_gp_final(p.work + (size_t)dclaw_adj::W_FIELDS * N, blockIdx.x, p.grad);
 }
 
@@ -1751,10 +1756,10 @@ size_t dclaw_adjoint_workspace_doubles(int mx, int my)
     if (mx < 1 || my < 1) return 0;
     // the stage planes, the parameter-gradient partial sums, then the tile
     // marks, the active-tile list, its counter and the per-block counts (ints)
-    const size_t ntiles = (size_t)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) *
-                          (size_t)((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+    const size_t ntiles = (size_t)((mx + 4 + ADJ_TW - 1) / ADJ_TW) *
+                          (size_t)((my + 4 + ADJ_TH - 1) / ADJ_TH);
     const size_t ints = 2 * ntiles + 1 + (ntiles + 63) / 64;
-    return (size_t)dclaw_adj::W_FIELDS * (size_t)(mx + 4) * (size_t)(my + 4) +
+    return (size_t)dclaw_adj::W_FIELDS * dclaw_adj::plane_size(mx, my) +
            (size_t)dclaw_adj::MAXP * ADJ_GP_PARTS + (ints + 1) / 2;
 }
 
@@ -1795,7 +1800,8 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
     A.lam_in = lam_in;
     A.q_out = q_out;
     A.W = work;
-    A.N = (size_t)(mx + 4) * (size_t)(my + 4);
+    A.N = dclaw_adj::plane_size(mx, my);
+    A.ld = dclaw_adj::plane_ld(mx);
     A.Pd = params_dev;
     A.dt_dev = dt_dev;
     const size_t nb = (A.N + ADJ_BLOCK - 1) / ADJ_BLOCK;
@@ -1839,8 +1845,8 @@ static long long adj_batch_tiles(const dclaw_adj_patch* ph, int npatch, long lon
     long long t = 0, c = 0;
     for (int k = 0; k < npatch; k++) {
         const int mx = ph[k].mx, my = ph[k].my;
-        c += (long long)(mx + 4) * (my + 4);
-        t += (long long)((mx + 4 + ADJ_TILE - 1) / ADJ_TILE) * ((my + 4 + ADJ_TILE - 1) / ADJ_TILE);
+        c += (long long)dclaw_adj::plane_size(mx, my);
+        t += (long long)((mx + 4 + ADJ_TW - 1) / ADJ_TW) * ((my + 4 + ADJ_TH - 1) / ADJ_TH);
     }
     if (cells) *cells = c;
     return t;

@@ -334,7 +334,7 @@ int dclaw_gauges_tl_dev(const double* q, const double* dq, const double* aux, in
  * my) doubles, zero-filled before its first use; each call leaves it as the
  * next call with the same mx, my needs it (do not modify it in between).
  * q_out and lam_in must not alias q_in, lam_out or each other.  mx, my >= 2.
- * The work is confined to the 8 x 8 tiles within 5 cells of a wet cell (and
+ * The work is confined to the 16 x 4 tiles within 5 cells of a wet cell (and
  * the border band); elsewhere the step is the dry map, whose adjoint passes
  * only bdif.  The reference has no adjoint; this is the "adjoint step"
  * call of the device API a gradient-based caller (parameter fitting, data

@@ -521,7 +521,8 @@ extern "C" int emu_adjoint_step(const dclaw_params* P, const dclaw_params* dP, i
     A.lam_out = lam_out;
     A.lam_in = lam_in;
     A.q_out = q_out;
-    A.N = (size_t)(mx + 4) * (size_t)(my + 4);
+    A.N = D::plane_size(mx, my);
+    A.ld = D::plane_ld(mx);
     A.Pd = nullptr;
     A.dt_dev = nullptr;
     std::vector<double> W((size_t)D::W_FIELDS * A.N, 0.0);
