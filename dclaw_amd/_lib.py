@@ -28,6 +28,17 @@ _i = ctypes.c_int
 _d = ctypes.c_double
 
 
+class CAmrRegion(ctypes.Structure):
+    """Mirror of ``dclaw_amr_region`` in include/dclaw_hip.h."""
+
+    _fields_ = [
+        ("q", _vp), ("aux", _vp), ("flag", _vp),
+        ("mx", ctypes.c_int), ("my", ctypes.c_int), ("maux", ctypes.c_int),
+        ("r0", ctypes.c_int), ("c0", ctypes.c_int),
+        ("ilo", ctypes.c_int), ("ihi", ctypes.c_int), ("jlo", ctypes.c_int), ("jhi", ctypes.c_int),
+    ]
+
+
 class CAdjPatch(ctypes.Structure):
     """Mirror of ``dclaw_adj_patch`` in include/dclaw_hip.h."""
 
@@ -94,6 +105,15 @@ SIGNATURES = {
                                             _vp, _vp]),
     "dclaw_amr_average_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i,
                                              _i, _i, _i, _i, _i, _d, _vp]),
+    "dclaw_amr_region_init_dev": (ctypes.c_int, [_vp, _i, _i, _vp]),
+    "dclaw_amr_copy_dev": (ctypes.c_int, [_vp, _vp, _vp, _d, _vp, _i, _i, _i, _i, _i, _i, _vp]),
+    "dclaw_amr_unset_box_dev": (ctypes.c_int, [_vp, _vp, _vp]),
+    "dclaw_amr_coarse_box": (None, [_i, _i, _i, _i, _i, _i, _vp]),
+    "dclaw_amr_filpatch_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i]
+                               + [_d] * 10 + [_vp]),
+    "dclaw_amr_region_bc_dev": (ctypes.c_int, [_vp, _i, _i, _vp, _vp]),
+    "dclaw_amr_setaux_dev": (ctypes.c_int, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _d, _d,
+                                            _vp]),
     "dclaw_amr_interp_dev": (ctypes.c_int, [_vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _i,
                                             _i, _d, _d, _d, _d, _vp]),
     "dclaw_stepgrid_split_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,

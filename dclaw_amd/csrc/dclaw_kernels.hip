@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -883,6 +884,123 @@ __global__ void k_amr_interp(const double* __restrict__ valc, const double* __re
         const int i = 1 + (int)(t % ni), j = 1 + (int)(t / ni);
         amr_interp_cell(valc, auxc, mauxc, mic, val, auxf, mauxf, mitot, needs_set, rx, ry, i, j,
                         dry_tolerance, sea_level, rho_f, grav);
+    }
+}
+
+// ---- region fill (filpatch.f90's filrecur on device)
+
+__global__ void k_amr_region_init(dclaw_amr_region R, int iregsz, int jregsz)
+{
+    const int ni = R.ihi - R.ilo + 1;
+    const long n = (long)ni * (R.jhi - R.jlo + 1);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int gi = R.ilo + (int)(t % ni), gj = R.jlo + (int)(t / ni);
+        R.flag[t] = amr_in_domain(gi, gj, iregsz, jregsz) ? 0 : 1;
+    }
+}
+
+// One thread per cell of the intersection ia..ib x ja..jb.
+__global__ void k_amr_copy(dclaw_amr_region R, const double* __restrict__ sq,
+                           const double* __restrict__ sq_old, double w_old,
+                           const double* __restrict__ saux, int smaux, int silo, int sjlo, int smx,
+                           int copy_aux, int ia, int ib, int ja, int jb)
+{
+    const int ni = ib - ia + 1;
+    const long n = (long)ni * (jb - ja + 1);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x)
+        amr_copy_cell(R, sq, sq_old, w_old, saux, smaux, silo, sjlo, smx, copy_aux != 0,
+                      ia + (int)(t % ni), ja + (int)(t / ni));
+}
+
+// trimbd: one workgroup; per-thread extremes of the unset cells, then a tree
+// in LDS.
+__global__ __launch_bounds__(1024) void k_amr_unset_box(dclaw_amr_region R, int* box)
+{
+    __shared__ int sh[4][1024];
+    const int ni = R.ihi - R.ilo + 1;
+    const long n = (long)ni * (R.jhi - R.jlo + 1);
+    int lo_i = INT_MAX, hi_i = INT_MIN, lo_j = INT_MAX, hi_j = INT_MIN;
+    for (long t = threadIdx.x; t < n; t += 1024) {
+        if (R.flag[t] != 0) continue;
+        const int gi = R.ilo + (int)(t % ni), gj = R.jlo + (int)(t / ni);
+        lo_i = min(lo_i, gi); hi_i = max(hi_i, gi);
+        lo_j = min(lo_j, gj); hi_j = max(hi_j, gj);
+    }
+    const int tid = (int)threadIdx.x;
+    sh[0][tid] = lo_i; sh[1][tid] = hi_i; sh[2][tid] = lo_j; sh[3][tid] = hi_j;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (tid < h) {
+            sh[0][tid] = min(sh[0][tid], sh[0][tid + h]);
+            sh[1][tid] = max(sh[1][tid], sh[1][tid + h]);
+            sh[2][tid] = min(sh[2][tid], sh[2][tid + h]);
+            sh[3][tid] = max(sh[3][tid], sh[3][tid + h]);
+        }
+        __syncthreads();
+    }
+    if (tid < 4) box[tid] = sh[tid][0];
+}
+
+// filpatch.f90's interpolation: one thread per interior coarse cell.
+__global__ void k_amr_filpatch(dclaw_amr_region R, const double* __restrict__ valc,
+                               const double* __restrict__ auxc, AmrFill G)
+{
+    const int ni = G.mxc - 2;
+    const long n = (long)ni * (G.myc - 2);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x)
+        amr_filpatch_cell(valc, auxc, R.q, R.aux, R.flag, G, 2 + (int)(t % ni),
+                          2 + (int)(t / ni));
+}
+
+__global__ void k_amr_region_bc(dclaw_amr_region R, int iregsz, int jregsz, int b0, int b1,
+                                int b2, int b3)
+{
+    const int bc[4] = {b0, b1, b2, b3};
+    const int ni = R.ihi - R.ilo + 1;
+    const long n = (long)ni * (R.jhi - R.jlo + 1);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int gi = R.ilo + (int)(t % ni), gj = R.jlo + (int)(t / ni);
+        if (!amr_in_domain(gi, gj, iregsz, jregsz))
+            amr_region_bc_cell(R, bc, iregsz, jregsz, gi, gj);
+    }
+}
+
+// setaux.f90:59-139 (bed of the cells not copied from an old patch), then
+// :154-184 (cells outside the domain take the bed of the nearest domain
+// cell) in a second launch, then :186-258 (the D-Claw fields).
+__global__ void k_amr_setaux_bed(double* aux, int maux, int mx, int my, int ilo, int jlo,
+                                 const unsigned char* __restrict__ bed_set,
+                                 const double* __restrict__ bed, int iregsz, int jregsz)
+{
+    const int nx = mx + 4;
+    const long n = (long)nx * (my + 4);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int gi = ilo - 2 + (int)(t % nx), gj = jlo - 2 + (int)(t / nx);
+        if (!bed_set[t] && amr_in_domain(gi, gj, iregsz, jregsz)) aux[(size_t)maux * t] = bed[t];
+    }
+}
+
+__global__ void k_amr_setaux_fields(double* aux, int maux, int mx, int my, int ilo, int jlo,
+                                    int iregsz, int jregsz, double phi, double theta)
+{
+    const int nx = mx + 4;
+    const long n = (long)nx * (my + 4);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int gi = ilo - 2 + (int)(t % nx), gj = jlo - 2 + (int)(t / nx);
+        double* a = aux + (size_t)maux * t;
+        if (!amr_in_domain(gi, gj, iregsz, jregsz)) {
+            const int si = min(max(gi, 0), iregsz - 1), sj = min(max(gj, 0), jregsz - 1);
+            a[0] = aux[(size_t)maux * ((size_t)(si - ilo + 2) + (size_t)nx * (size_t)(sj - jlo + 2))];
+        }
+        for (int k = 1; k < maux; k++) a[k] = 0.0;
+        a[1] = phi;
+        a[2] = theta;
     }
 }
 
@@ -2409,6 +2527,139 @@ int dclaw_amr_interp_dev(const double* valc, const double* auxc, int mauxc, int 
     hipLaunchKernelGGL(k_amr_interp, dim3(blocks), dim3(256), 0, (hipStream_t)stream, valc, auxc,
                        mauxc, mic, mjc, val, auxf, mauxf, mitot, needs_set, rx, ry, dry_tolerance,
                        sea_level, rho_f, grav);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+static int check_region(const dclaw_amr_region* r)
+{
+    if (!r || !r->q || !r->aux || !r->flag) return fail(DCLAW_EINVAL, "null region pointer");
+    if (r->maux < 1 || r->mx < 1 || r->my < 1 || r->ihi < r->ilo || r->jhi < r->jlo)
+        return fail(DCLAW_EINVAL, "bad AMR region sizes");
+    if (r->r0 < 0 || r->c0 < 0 || (long)r->r0 + (r->ihi - r->ilo) >= r->mx ||
+        (long)r->c0 + (r->jhi - r->jlo) >= r->my)
+        return fail(DCLAW_EINVAL, "AMR region outside its arrays");
+    return DCLAW_OK;
+}
+
+static int amr_blocks(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 65535)); }
+
+int dclaw_amr_region_init_dev(const dclaw_amr_region* r, int iregsz, int jregsz, void* stream)
+{
+    if (int rc = check_region(r)) return rc;
+    if (iregsz < 1 || jregsz < 1) return fail(DCLAW_EINVAL, "bad domain size");
+    const long n = (long)(r->ihi - r->ilo + 1) * (r->jhi - r->jlo + 1);
+    hipLaunchKernelGGL(k_amr_region_init, dim3(amr_blocks(n)), dim3(256), 0,
+                       (hipStream_t)stream, *r, iregsz, jregsz);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_amr_copy_dev(const dclaw_amr_region* r, const double* src_q, const double* src_q_old,
+                       double w_old, const double* src_aux, int src_maux, int src_ilo,
+                       int src_jlo, int src_mx, int src_my, int copy_aux, void* stream)
+{
+    if (int rc = check_region(r)) return rc;
+    if (!src_q || (copy_aux && !src_aux)) return fail(DCLAW_EINVAL, "null source pointer");
+    if (src_mx < 1 || src_my < 1 || (copy_aux && src_maux < 1))
+        return fail(DCLAW_EINVAL, "bad source patch sizes");
+    if (!(w_old >= 0.0 && w_old <= 1.0)) return fail(DCLAW_EINVAL, "w_old must be in [0, 1]");
+    if (src_q == r->q && (src_q_old || copy_aux))
+        return fail(DCLAW_EINVAL, "a patch marking its own cells copies nothing");
+    const int ia = std::max(r->ilo, src_ilo), ib = std::min(r->ihi, src_ilo + src_mx - 1);
+    const int ja = std::max(r->jlo, src_jlo), jb = std::min(r->jhi, src_jlo + src_my - 1);
+    if (ia > ib || ja > jb) return DCLAW_OK;
+    const long n = (long)(ib - ia + 1) * (jb - ja + 1);
+    hipLaunchKernelGGL(k_amr_copy, dim3(amr_blocks(n)), dim3(256), 0, (hipStream_t)stream, *r,
+                       src_q, src_q_old, w_old, src_aux, src_maux, src_ilo, src_jlo, src_mx,
+                       copy_aux, ia, ib, ja, jb);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_amr_unset_box_dev(const dclaw_amr_region* r, int* box_dev, void* stream)
+{
+    if (int rc = check_region(r)) return rc;
+    if (!box_dev) return fail(DCLAW_EINVAL, "null box pointer");
+    hipLaunchKernelGGL(k_amr_unset_box, dim3(1), dim3(1024), 0, (hipStream_t)stream, *r,
+                       box_dev);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+void dclaw_amr_coarse_box(int isl, int isr, int jsb, int jst, int rx, int ry, int* box)
+{
+    if (box && rx > 0 && ry > 0) amr_coarse_box(isl, isr, jsb, jst, rx, ry, box);
+}
+
+int dclaw_amr_filpatch_dev(const dclaw_amr_region* r, const double* valc, const double* auxc,
+                           int mauxc, int iplo, int iphi, int jplo, int jphi, int rx, int ry,
+                           double xlower, double ylower, double dx_fine, double dy_fine,
+                           double dx_coarse, double dy_coarse, double sea_level,
+                           double dry_tolerance, double rho_f, double grav, void* stream)
+{
+    if (int rc = check_region(r)) return rc;
+    if (!valc || !auxc) return fail(DCLAW_EINVAL, "null coarse pointer");
+    if (mauxc < 1 || rx < 1 || ry < 1 || iphi - iplo < 2 || jphi - jplo < 2)
+        return fail(DCLAW_EINVAL, "bad coarse rectangle");
+    if (!(dx_fine > 0.0 && dy_fine > 0.0 && dx_coarse > 0.0 && dy_coarse > 0.0))
+        return fail(DCLAW_EINVAL, "cell sizes must be > 0");
+    if (!(dry_tolerance >= 0.0)) return fail(DCLAW_EINVAL, "dry_tolerance must be >= 0");
+    AmrFill G;
+    G.ilo = r->ilo; G.ihi = r->ihi; G.jlo = r->jlo; G.jhi = r->jhi;
+    G.r0 = r->r0; G.c0 = r->c0; G.mx = r->mx; G.maux = r->maux;
+    G.iplo = iplo; G.jplo = jplo; G.mxc = iphi - iplo + 1; G.myc = jphi - jplo + 1;
+    G.mauxc = mauxc; G.rx = rx; G.ry = ry;
+    G.xlower = xlower; G.ylower = ylower; G.dxf = dx_fine; G.dyf = dy_fine;
+    G.dxc = dx_coarse; G.dyc = dy_coarse;
+    G.sea_level = sea_level; G.drytol = dry_tolerance; G.rho_f = rho_f; G.grav = grav;
+    const long n = (long)(G.mxc - 2) * (G.myc - 2);
+    hipLaunchKernelGGL(k_amr_filpatch, dim3(amr_blocks(n)), dim3(256), 0, (hipStream_t)stream,
+                       *r, valc, auxc, G);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_amr_region_bc_dev(const dclaw_amr_region* r, int iregsz, int jregsz, const int* bc,
+                            void* stream)
+{
+    if (int rc = check_region(r)) return rc;
+    if (int rcb = check_bc(bc)) return rcb;
+    if (!bc) return fail(DCLAW_EINVAL, "null bc");
+    if (iregsz < 1 || jregsz < 1) return fail(DCLAW_EINVAL, "bad domain size");
+    // every source cell the rules name must be a domain cell of the region
+    const int xs[2] = {r->ilo, r->ihi}, ys[2] = {r->jlo, r->jhi};
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            int si, sj;
+            double fu, fv;
+            amr_domain_source(bc, iregsz, jregsz, xs[a], ys[b], si, sj, fu, fv);
+            if (si < r->ilo || si > r->ihi || sj < r->jlo || sj > r->jhi ||
+                !amr_in_domain(si, sj, iregsz, jregsz))
+                return fail(DCLAW_EINVAL, "AMR region: boundary source cells outside the region");
+        }
+    const long n = (long)(r->ihi - r->ilo + 1) * (r->jhi - r->jlo + 1);
+    hipLaunchKernelGGL(k_amr_region_bc, dim3(amr_blocks(n)), dim3(256), 0, (hipStream_t)stream,
+                       *r, iregsz, jregsz, bc[0], bc[1], bc[2], bc[3]);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_amr_setaux_dev(double* aux, int maux, int mx, int my, int ilo, int jlo,
+                         const unsigned char* bed_set, const double* bed, int iregsz,
+                         int jregsz, double phi, double theta, void* stream)
+{
+    if (!aux || !bed_set || !bed) return fail(DCLAW_EINVAL, "null pointer");
+    if (maux < 3 || mx < 1 || my < 1) return fail(DCLAW_EINVAL, "setaux needs maux >= 3");
+    if (iregsz < 1 || jregsz < 1) return fail(DCLAW_EINVAL, "bad domain size");
+    if (ilo + mx - 1 < 0 || ilo > iregsz - 1 || jlo + my - 1 < 0 || jlo > jregsz - 1)
+        return fail(DCLAW_EINVAL, "patch outside the domain");
+    const long n = (long)(mx + 4) * (my + 4);
+    hipLaunchKernelGGL(k_amr_setaux_bed, dim3(amr_blocks(n)), dim3(256), 0, (hipStream_t)stream,
+                       aux, maux, mx, my, ilo, jlo, bed_set, bed, iregsz, jregsz);
+    hipLaunchKernelGGL(k_amr_setaux_fields, dim3(amr_blocks(n)), dim3(256), 0,
+                       (hipStream_t)stream, aux, maux, mx, my, ilo, jlo, iregsz, jregsz, phi,
+                       theta);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }

@@ -409,6 +409,93 @@ int dclaw_amr_interp_dev(const double* valc, const double* auxc, int mauxc, int 
                          const unsigned char* needs_set, int rx, int ry, double dry_tolerance,
                          double sea_level, double rho_f, double grav, void* stream);
 
+/* ---- AMR ghost fill and regrid on device.  The D-Claw parts are
+ * filpatch.f90's interpolation from the enlarged coarse rectangle
+ * (dclaw_amr_filpatch_dev), filval.f90's interpolation loop
+ * (dclaw_amr_interp_dev) and setaux.f90's per-patch fields
+ * (dclaw_amr_setaux_dev); the AMRClaw steps around them (not in the
+ * reference: AMRClaw 5.x intfil/icall, trimbd, bc2amr) are restated as
+ * dclaw_amr_copy_dev, dclaw_amr_unset_box_dev and dclaw_amr_region_bc_dev.
+ * Non-periodic domains, coordinate_system 1, topography fixed in time,
+ * force_dry and variable_eta_init off. ---- */
+
+/* A region of one refinement level being filled (filpatch.f90's valbig,
+ * aux and flaguse): cells ilo..ihi x jlo..jhi in the level's index space
+ * (cell i covers [xlower + i dx, xlower + (i+1) dx)), stored from column r0,
+ * row c0 of q(7, mx, my) and aux(maux, mx, my) (device, component fastest);
+ * flag: (ihi-ilo+1) x (jhi-jlo+1) device bytes, column fastest, nonzero =
+ * set.  Passed by pointer to host memory. */
+typedef struct {
+    double* q;
+    double* aux;
+    unsigned char* flag;
+    int mx, my, maux;
+    int r0, c0;
+    int ilo, ihi, jlo, jhi;
+} dclaw_amr_region;
+
+/* Start a fill (AMRClaw intfil's marking): flag = 1 on the region's cells
+ * outside the level's domain 0..iregsz-1 x 0..jregsz-1 (bc2amr sets them),
+ * 0 elsewhere. */
+int dclaw_amr_region_init_dev(const dclaw_amr_region* r, int iregsz, int jregsz, void* stream);
+
+/* Copy from one patch of the same level (AMRClaw intfil/icall): the cells of
+ * the source's interior (src_ilo.., src_jlo.., src_mx x src_my cells; arrays
+ * with 2 ghost cells, interior cell (src_ilo + a, src_jlo + b) at column
+ * a + 2, row b + 2) that lie in the region get the source's values and flag
+ * 1.  With src_q_old (the source's values at its previous time), the value
+ * is q (1 - w_old) + q_old w_old, the linear interpolation in time (w_old =
+ * (t_new - t) / (t_new - t_old)); w_old = 0 copies q.  copy_aux != 0 also
+ * copies the aux components (min(maux, src_maux) of them).  src_q == r->q
+ * marks the cells only (a patch filling its own ghosts). */
+int dclaw_amr_copy_dev(const dclaw_amr_region* r, const double* src_q, const double* src_q_old,
+                       double w_old, const double* src_aux, int src_maux, int src_ilo,
+                       int src_jlo, int src_mx, int src_my, int copy_aux, void* stream);
+
+/* trimbd: the bounding box (isl, isr, jsb, jst) in level index space of the
+ * region's unset cells into box_dev[0..3] (device ints); isl > isr when every
+ * cell is set. */
+int dclaw_amr_unset_box_dev(const dclaw_amr_region* r, int* box_dev, void* stream);
+
+/* The coarse rectangle (iplo, iphi, jplo, jphi) filpatch.f90:182-187 builds
+ * around the unset cells isl..isr x jsb..jst at refinement rx, ry (host). */
+void dclaw_amr_coarse_box(int isl, int isr, int jsb, int jst, int rx, int ry, int* box);
+
+/* filpatch.f90:256-520 (replaces filrecur's interpolation): the region's
+ * unset cells (flag 0) from the filled coarse rectangle valc/auxc
+ * (iplo..iphi x jplo..jphi, component fastest, no ghost cells; its bed is
+ * auxc component 1): depth from the dry-limited surface by minmod slopes
+ * minus the fine bed aux(1), the other components limited-linear within the
+ * coarse velocity bounds, else conservatively spread.  Every unset cell
+ * must lie in the interior of the coarse rectangle (the rectangle
+ * dclaw_amr_coarse_box gives for the unset box does).  dx_coarse, dy_coarse:
+ * the coarser level's cell size (hxposs(level-1)). */
+int dclaw_amr_filpatch_dev(const dclaw_amr_region* r, const double* valc, const double* auxc,
+                           int mauxc, int iplo, int iphi, int jplo, int jphi, int rx, int ry,
+                           double xlower, double ylower, double dx_fine, double dy_fine,
+                           double dx_coarse, double dy_coarse, double sea_level,
+                           double dry_tolerance, double rho_f, double grav, void* stream);
+
+/* bc2amr on a region: its cells outside the domain take q (and all aux
+ * components) from the cell the boundary rule names -- extrap: the nearest
+ * domain cell; wall (bc 3): the mirror cell with the normal momentum
+ * negated -- which must lie in the region.  bc: xlo, xhi, ylo, yhi. */
+int dclaw_amr_region_bc_dev(const dclaw_amr_region* r, int iregsz, int jregsz, const int* bc,
+                            void* stream);
+
+/* setaux.f90:59-258 for a new patch (interior ilo.., jlo.., mx x my cells, 2
+ * ghost cells; aux(maux, mx+4, my+4), maux >= 3): aux(1) = bed where
+ * bed_set is 0 and the cell is in the domain (bed: the cell integral of the
+ * topography / cell area, cellgridintegrate's result, computed by the
+ * caller; bed_set marks cells whose aux(1) was copied from an old patch),
+ * then cells outside the domain copy aux(1) of the nearest domain cell
+ * (:154-184), then aux(2..maux) = 0, aux(2) = phi (i_phi), aux(3) = theta
+ * (i_theta).  bed_set and bed: (mx+4) x (my+4), column fastest.  Auxinit
+ * files (aux(i_ent) and others from files) are the caller's to add. */
+int dclaw_amr_setaux_dev(double* aux, int maux, int mx, int my, int ilo, int jlo,
+                         const unsigned char* bed_set, const double* bed, int iregsz,
+                         int jregsz, double phi, double theta, void* stream);
+
 /* ---- Per-stage device entry points (GeoClaw stepgrid's stages one at a
  * time, on device memory; the fused dclaw_stepgrid_dev equals b4step2, step2,
  * the conservative update and src2 in sequence). ---- */

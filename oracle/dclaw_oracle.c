@@ -1865,3 +1865,364 @@ void oracle_amr_interp(const double* valc, const double* auxc, int mauxc, int mi
 #undef BCX
 #undef VF
 }
+
+/* ------------------------------------------------------------------ filpatch
+ * filpatch.f90:137-243 (the coarse rectangle around the unset cells),
+ * Fortran integer division (truncation toward zero) as the reference has it. */
+void oracle_amr_coarse_box(int isl, int isr, int jsb, int jst, int rx, int ry, int* box)
+{
+    const int nghost = 2;
+    box[0] = (isl - rx + nghost * rx) / rx - nghost;   /* iplo */
+    box[1] = (isr + rx) / rx;                          /* iphi */
+    box[2] = (jsb - ry + nghost * ry) / ry - nghost;   /* jplo */
+    box[3] = (jst + ry) / ry;                          /* jphi */
+}
+
+/* filpatch.f90:256-520: interpolation of the unset cells (flaguse == 0) of
+ * the region ilo..ihi x jlo..jhi (level index space) from the coarse
+ * rectangle valcrse (mxc x myc cells from (iplo, jplo)), force_dry and
+ * variable_eta_init off (sea level constant), coordinate_system 1.  The
+ * region's cell (ilo + a, jlo + b) sits at column r0 + a, row c0 + b of
+ * valbig / aux (0-based, mx columns); flaguse is (ihi-ilo+1) x (jhi-jlo+1),
+ * column fastest.  Loops as the reference has them: the depth over every
+ * fine cell (j outer), then per component the coarse slopes and velocity
+ * bounds, the fine cells, and the reset of flagged coarse cells.  Returns 0,
+ * or -1 if an unset cell falls outside the interior of the coarse
+ * rectangle. */
+int oracle_amr_filpatch(const double* valcrse, const double* auxcrse, int mauxc, int mxc, int myc,
+                        int iplo, int jplo, double* valbig, const double* aux, int maux, int mx,
+                        int r0, int c0, int ilo, int ihi, int jlo, int jhi,
+                        const unsigned char* flaguse, int rx, int ry, double xlower,
+                        double ylower, double dx_fine, double dy_fine, double dx_coarse,
+                        double dy_coarse, double sea_level, double dry_tolerance, double rho_f,
+                        double grav)
+{
+    const int mx_patch = ihi - ilo + 1, my_patch = jhi - jlo + 1;
+    const size_t nc = (size_t)mxc * (size_t)myc;
+#define VCR(n, i, j) valcrse[(n) + 7 * ((size_t)((i) - 1) + (size_t)mxc * (size_t)((j) - 1))]
+#define BCR(i, j) auxcrse[(size_t)mauxc * ((size_t)((i) - 1) + (size_t)mxc * (size_t)((j) - 1))]
+#define VB(n, i, j) valbig[(n) + 7 * ((size_t)(r0 + (i) - 1) + (size_t)mx * (size_t)(c0 + (j) - 1))]
+#define BF(i, j) aux[(size_t)maux * ((size_t)(r0 + (i) - 1) + (size_t)mx * (size_t)(c0 + (j) - 1))]
+#define FU(i, j) flaguse[(size_t)((i) - 1) + (size_t)mx_patch * (size_t)((j) - 1)]
+#define CX(a, i, j) a[(size_t)((i) - 1) + (size_t)mxc * (size_t)((j) - 1)]
+    double* eta_coarse = calloc(nc, sizeof(double));
+    double* slope1 = calloc(nc, sizeof(double));
+    double* slope2 = calloc(nc, sizeof(double));
+    double* fine_mass = calloc(nc, sizeof(double));
+    double* vel_max = calloc(nc, sizeof(double));
+    double* vel_min = calloc(nc, sizeof(double));
+    int* fine_cell_count = calloc(nc, sizeof(int));
+    unsigned char* flag1 = calloc(nc, 1);
+    unsigned char* flagn = calloc(nc, 1);
+    int rc = 0;
+    const double xlow_coarse = xlower + iplo * dx_coarse;
+    const double ylow_coarse = ylower + jplo * dy_coarse;
+
+    /* :274-287 surface with dry limiting */
+    for (int j = 1; j <= myc; j++)
+        for (int i = 1; i <= mxc; i++) {
+            const double h = VCR(0, i, j), b = BCR(i, j);
+            if (h < dry_tolerance) CX(eta_coarse, i, j) = sea_level;
+            else CX(eta_coarse, i, j) = h + b;
+        }
+    /* :294-313 limited gradients of eta */
+    for (int j = 2; j <= myc - 1; j++)
+        for (int i = 2; i <= mxc - 1; i++) {
+            double down = CX(eta_coarse, i, j) - CX(eta_coarse, i - 1, j);
+            double up = CX(eta_coarse, i + 1, j) - CX(eta_coarse, i, j);
+            if (up * down > 0.0)
+                CX(slope1, i, j) = dmin(fabs(up), fabs(down)) *
+                                   dsign(1.0, CX(eta_coarse, i + 1, j) - CX(eta_coarse, i - 1, j));
+            down = CX(eta_coarse, i, j) - CX(eta_coarse, i, j - 1);
+            up = CX(eta_coarse, i, j + 1) - CX(eta_coarse, i, j);
+            if (up * down > 0.0)
+                CX(slope2, i, j) = dmin(fabs(up), fabs(down)) *
+                                   dsign(1.0, CX(eta_coarse, i, j + 1) - CX(eta_coarse, i, j - 1));
+        }
+    const double ratio_y = (double)ry, ratio_x = (double)rx;
+    int reloop = 0;
+    /* :317-376 depth */
+    for (int jf = 1; jf <= my_patch; jf++) {
+        const int jc = (int)floor((double)(jf + jlo - 1) / ratio_y) - jplo + 1;
+        const double ycent_coarse = ylow_coarse + ((double)jc - 0.5) * dy_coarse;
+        const double ycent_fine = ylower + ((double)(jf - 1 + jlo) + 0.5) * dy_fine;
+        const double eta2 = (ycent_fine - ycent_coarse) / dy_coarse;
+        for (int ifn = 1; ifn <= mx_patch; ifn++) {
+            const int ic = (int)floor((double)(ifn + ilo - 1) / ratio_x) - iplo + 1;
+            const double xcent_coarse = xlow_coarse + ((double)ic - 0.5) * dx_coarse;
+            const double xcent_fine = xlower + ((double)(ifn - 1 + ilo) + 0.5) * dx_fine;
+            const double eta1 = (xcent_fine - xcent_coarse) / dx_coarse;
+            if (FU(ifn, jf) != 0) continue;
+            if (ic < 2 || ic > mxc - 1 || jc < 2 || jc > myc - 1) { rc = -1; goto done; }
+            CX(fine_cell_count, ic, jc) += 1;
+            const double eta_fine = (CX(eta_coarse, ic, jc) + eta1 * CX(slope1, ic, jc)) +
+                                    eta2 * CX(slope2, ic, jc);
+            const double h_fine = dmax(eta_fine - BF(ifn, jf), 0.0);
+            VB(0, ifn, jf) = h_fine;
+            CX(fine_mass, ic, jc) = CX(fine_mass, ic, jc) + h_fine;
+            if (h_fine < dry_tolerance) {
+                CX(flag1, ic, jc) = 1;
+                reloop = 1;
+            }
+        }
+    }
+    /* :379-516 the other components */
+    for (int n = 2; n <= 7; n++) {
+        const int m = n - 1;
+        memset(slope1, 0, nc * sizeof(double));
+        memset(slope2, 0, nc * sizeof(double));
+        memset(flagn, 0, nc);
+        for (int j = 2; j <= myc - 1; j++)
+            for (int i = 2; i <= mxc - 1; i++) {
+                double down = VCR(m, i, j) - VCR(m, i - 1, j);
+                double up = VCR(m, i + 1, j) - VCR(m, i, j);
+                if (up * down > 0.0)
+                    CX(slope1, i, j) = dmin(fabs(up), fabs(down)) *
+                                       dsign(1.0, VCR(m, i + 1, j) - VCR(m, i - 1, j));
+                down = VCR(m, i, j) - VCR(m, i, j - 1);
+                up = VCR(m, i, j + 1) - VCR(m, i, j);
+                if (up * down > 0.0)
+                    CX(slope2, i, j) = dmin(fabs(up), fabs(down)) *
+                                       dsign(1.0, VCR(m, i, j + 1) - VCR(m, i, j - 1));
+                if (VCR(0, i, j) > dry_tolerance) {
+                    CX(vel_max, i, j) = VCR(m, i, j) / VCR(0, i, j);
+                    CX(vel_min, i, j) = VCR(m, i, j) / VCR(0, i, j);
+                } else {
+                    CX(vel_min, i, j) = 0.0;
+                    CX(vel_max, i, j) = 0.0;
+                    if (n == 4) {
+                        CX(vel_max, i, j) = 0.0;
+                        CX(vel_min, i, j) = 0.0;
+                    } else if (n == 5) {
+                        CX(vel_max, i, j) = rho_f * grav;
+                        CX(vel_min, i, j) = rho_f * grav;
+                    }
+                }
+                for (int s = -1; s <= 1; s += 2) {
+                    if (VCR(0, i + s, j) > dry_tolerance) {
+                        CX(vel_max, i, j) = dmax(CX(vel_max, i, j), VCR(m, i + s, j) / VCR(0, i + s, j));
+                        CX(vel_min, i, j) = dmin(CX(vel_min, i, j), VCR(m, i + s, j) / VCR(0, i + s, j));
+                    }
+                    if (VCR(0, i, j + s) > dry_tolerance) {
+                        CX(vel_max, i, j) = dmax(CX(vel_max, i, j), VCR(m, i, j + s) / VCR(0, i, j + s));
+                        CX(vel_min, i, j) = dmin(CX(vel_min, i, j), VCR(m, i, j + s) / VCR(0, i, j + s));
+                    }
+                }
+            }
+        for (int jf = 1; jf <= my_patch; jf++) {
+            const int jc = (int)floor((double)(jf + jlo - 1) / ratio_y) - jplo + 1;
+            const double ycent_coarse = ylow_coarse + ((double)jc - 0.5) * dy_coarse;
+            const double ycent_fine = ylower + ((double)(jf - 1 + jlo) + 0.5) * dy_fine;
+            const double eta2 = (ycent_fine - ycent_coarse) / dy_coarse;
+            for (int ifn = 1; ifn <= mx_patch; ifn++) {
+                const int ic = (int)floor((double)(ifn + ilo - 1) / ratio_x) - iplo + 1;
+                const double xcent_coarse = xlow_coarse + ((double)ic - 0.5) * dx_coarse;
+                const double xcent_fine = xlower + ((double)(ifn - 1 + ilo) + 0.5) * dx_fine;
+                const double eta1 = (xcent_fine - xcent_coarse) / dx_coarse;
+                if (FU(ifn, jf) != 0 || CX(flag1, ic, jc)) continue;
+                const double hv_fine = (VCR(m, ic, jc) + eta1 * CX(slope1, ic, jc)) +
+                                       eta2 * CX(slope2, ic, jc);
+                const double v_fine = hv_fine / VB(0, ifn, jf);
+                if (v_fine < CX(vel_min, ic, jc) || v_fine > CX(vel_max, ic, jc)) {
+                    CX(flagn, ic, jc) = 1;
+                    reloop = 1;
+                } else {
+                    VB(m, ifn, jf) = hv_fine;
+                }
+            }
+        }
+        if (reloop) {
+            for (int jf = 1; jf <= my_patch; jf++) {
+                const int jc = (int)floor((double)(jf + jlo - 1) / ratio_y) - jplo + 1;
+                for (int ifn = 1; ifn <= mx_patch; ifn++) {
+                    const int ic = (int)floor((double)(ifn + ilo - 1) / ratio_x) - iplo + 1;
+                    if (FU(ifn, jf) != 0) continue;
+                    if (!(CX(flag1, ic, jc) || CX(flagn, ic, jc))) continue;
+                    if (CX(fine_mass, ic, jc) > dry_tolerance) {
+                        const double h_coarse = VCR(0, ic, jc);
+                        const double h_count = (double)CX(fine_cell_count, ic, jc);
+                        const double h_fine_average = CX(fine_mass, ic, jc) / h_count;
+                        const double divide_mass = dmax(h_coarse, h_fine_average);
+                        double v_new = VCR(m, ic, jc) / divide_mass;
+                        if (n > 3) {
+                            v_new = dmax(CX(vel_min, ic, jc), v_new);
+                            v_new = dmin(CX(vel_max, ic, jc), v_new);
+                        }
+                        VB(m, ifn, jf) = v_new * VB(0, ifn, jf);
+                    } else {
+                        VB(m, ifn, jf) = 0.0;
+                    }
+                }
+            }
+        }
+    }
+done:
+    free(eta_coarse); free(slope1); free(slope2); free(fine_mass); free(vel_max); free(vel_min);
+    free(fine_cell_count); free(flag1); free(flagn);
+#undef VCR
+#undef BCR
+#undef VB
+#undef BF
+#undef FU
+#undef CX
+    return rc;
+}
+
+/* ------------------------------------------------ region fill (AMRClaw steps)
+ * Not in the reference: AMRClaw 5.x intfil/icall (copy from the grids of a
+ * level, linear in time between a grid's two stored times), trimbd
+ * (bounding box of the unset cells) and bc2amr (extrap / wall fill of the
+ * cells outside the domain), restated from their published behaviour.
+ * Region layout: cell (ilo + a, jlo + b) at column r0 + a, row c0 + b of
+ * q(7, mx, .) and aux(maux, mx, .); flag (ihi-ilo+1) x (jhi-jlo+1). */
+void oracle_amr_intfil(double* q, double* aux, int maux, int mx, int r0, int c0, int ilo,
+                       int ihi, int jlo, int jhi, unsigned char* flag, const double* sq,
+                       const double* sq_old, double w_old, const double* saux, int smaux,
+                       int silo, int sjlo, int smx, int smy, int copy_aux)
+{
+    const int mxp = ihi - ilo + 1;
+    for (int j = jlo; j <= jhi; j++)
+        for (int i = ilo; i <= ihi; i++) {
+            if (i < silo || i > silo + smx - 1 || j < sjlo || j > sjlo + smy - 1) continue;
+            flag[(size_t)(i - ilo) + (size_t)mxp * (j - jlo)] = 1;
+            const size_t d = (size_t)(r0 + i - ilo) + (size_t)mx * (size_t)(c0 + j - jlo);
+            const size_t s = (size_t)(i - silo + 2) + (size_t)(smx + 4) * (size_t)(j - sjlo + 2);
+            for (int m = 0; m < 7; m++) {
+                if (sq_old && w_old != 0.0)
+                    q[7 * d + m] = sq[7 * s + m] * (1.0 - w_old) + sq_old[7 * s + m] * w_old;
+                else
+                    q[7 * d + m] = sq[7 * s + m];
+            }
+            if (copy_aux)
+                for (int k = 0; k < (maux < smaux ? maux : smaux); k++)
+                    aux[(size_t)maux * d + k] = saux[(size_t)smaux * s + k];
+        }
+}
+
+/* trimbd: box = (isl, isr, jsb, jst) of the unset cells in level index
+ * space; returns 1 if every cell is set (box untouched), else 0. */
+int oracle_amr_trimbd(const unsigned char* flag, int ilo, int ihi, int jlo, int jhi, int* box)
+{
+    const int mxp = ihi - ilo + 1;
+    int isl = ihi + 1, isr = ilo - 1, jsb = jhi + 1, jst = jlo - 1;
+    for (int j = jlo; j <= jhi; j++)
+        for (int i = ilo; i <= ihi; i++)
+            if (flag[(size_t)(i - ilo) + (size_t)mxp * (j - jlo)] == 0) {
+                if (i < isl) isl = i;
+                if (i > isr) isr = i;
+                if (j < jsb) jsb = j;
+                if (j > jst) jst = j;
+            }
+    if (isl > isr) return 1;
+    box[0] = isl; box[1] = isr; box[2] = jsb; box[3] = jst;
+    return 0;
+}
+
+/* bc2amr on a region: first the columns outside the domain in x (rows
+ * inside in y) from the x rule, then the rows outside in y (every column)
+ * from the y rule, as the patch ghost fill orders them; q and every aux
+ * component copied, the normal momentum negated at walls (bc 3). */
+void oracle_amr_bc2amr(double* q, double* aux, int maux, int mx, int r0, int c0, int ilo,
+                       int ihi, int jlo, int jhi, int iregsz, int jregsz, const int* bc)
+{
+#define RC(i, j) ((size_t)(r0 + (i) - ilo) + (size_t)mx * (size_t)(c0 + (j) - jlo))
+    for (int j = jlo; j <= jhi; j++) {
+        if (j < 0 || j >= jregsz) continue;
+        for (int i = ilo; i <= ihi; i++) {
+            int si;
+            double f = 1.0;
+            if (i < 0) {
+                if (bc[0] == 3) { si = -1 - i; f = -1.0; } else si = 0;
+            } else if (i >= iregsz) {
+                if (bc[1] == 3) { si = 2 * iregsz - 1 - i; f = -1.0; } else si = iregsz - 1;
+            } else {
+                continue;
+            }
+            for (int m = 0; m < 7; m++)
+                q[7 * RC(i, j) + m] = (m == 1 ? f : 1.0) * q[7 * RC(si, j) + m];
+            for (int k = 0; k < maux; k++) aux[(size_t)maux * RC(i, j) + k] = aux[(size_t)maux * RC(si, j) + k];
+        }
+    }
+    for (int j = jlo; j <= jhi; j++) {
+        int sj;
+        double f = 1.0;
+        if (j < 0) {
+            if (bc[2] == 3) { sj = -1 - j; f = -1.0; } else sj = 0;
+        } else if (j >= jregsz) {
+            if (bc[3] == 3) { sj = 2 * jregsz - 1 - j; f = -1.0; } else sj = jregsz - 1;
+        } else {
+            continue;
+        }
+        for (int i = ilo; i <= ihi; i++) {
+            for (int m = 0; m < 7; m++)
+                q[7 * RC(i, j) + m] = (m == 2 ? f : 1.0) * q[7 * RC(i, sj) + m];
+            for (int k = 0; k < maux; k++) aux[(size_t)maux * RC(i, j) + k] = aux[(size_t)maux * RC(i, sj) + k];
+        }
+    }
+#undef RC
+}
+
+/* setaux.f90:59-258 for a patch (interior cells ilo.., jlo.., mx x my, 2
+ * ghost cells), coordinate_system 1, no auxinit files: aux(1) from bed
+ * where bed_set is 0 and the cell centre lies in the domain (the topography
+ * loop, :75-139, with cellgridintegrate's result supplied), then the rows
+ * and the columns whose centres lie outside the domain copy aux(1) from the
+ * cell the ceiling shifts name (:154-184), aux(2..maux) = 0 (:190-198),
+ * aux(i_phi = 2) = phi (:260-264), aux(i_theta = 3) = theta (:267-273). */
+void oracle_amr_setaux(double* aux, int maux, int mx, int my, int ilo, int jlo,
+                       const unsigned char* bed_set, const double* bed, double xlower,
+                       double xupper, double ylower, double yupper, double dx, double dy,
+                       double phi, double theta)
+{
+    const int mbc = 2, nx = mx + 4;
+#define A1(ii, jj) aux[(size_t)maux * ((size_t)((ii) + mbc - 1) + (size_t)nx * (size_t)((jj) + mbc - 1))]
+#define CI(ii, jj) ((size_t)((ii) + mbc - 1) + (size_t)nx * (size_t)((jj) + mbc - 1))
+    /* ilo, jlo here are the level indices of the first interior cell, which
+     * is setaux's floor((xlow - xlower + 0.05 dx)/dx) */
+    for (int jj = 1 - mbc; jj <= my + mbc; jj++) {
+        const double ym = ylower + (jlo + jj - 1.0) * dy, yp = ylower + (jlo + jj) * dy;
+        const double y = 0.5 * (ym + yp);
+        for (int ii = 1 - mbc; ii <= mx + mbc; ii++) {
+            const double xm = xlower + (ilo + ii - 1.0) * dx, xp = xlower + (ilo + ii) * dx;
+            const double x = 0.5 * (xm + xp);
+            if (bed_set[CI(ii, jj)]) continue;
+            if (y > yupper || y < ylower || x > xupper || x < xlower) continue;
+            A1(ii, jj) = bed[CI(ii, jj)];
+        }
+    }
+    for (int jj = 1 - mbc; jj <= my + mbc; jj++) {
+        const double y = ylower + (jlo + jj - 0.5) * dy;
+        if (y < ylower || y > yupper) {
+            for (int ii = 1 - mbc; ii <= mx + mbc; ii++) {
+                const double x = xlower + (ilo + ii - 0.5) * dx;
+                const int iint = ii + (int)dmax(0.0, ceil((xlower - x) / dx)) -
+                                 (int)dmax(0.0, ceil((x - xupper) / dx));
+                const int jint = jj + (int)dmax(0.0, ceil((ylower - y) / dy)) -
+                                 (int)dmax(0.0, ceil((y - yupper) / dy));
+                A1(ii, jj) = A1(iint, jint);
+            }
+        }
+    }
+    for (int ii = 1 - mbc; ii <= mx + mbc; ii++) {
+        const double x = xlower + (ilo + ii - 0.5) * dx;
+        if (x < xlower || x > xupper) {
+            for (int jj = 1 - mbc; jj <= my + mbc; jj++) {
+                const double y = ylower + (jlo + jj - 0.5) * dy;
+                const int iint = ii + (int)dmax(0.0, ceil((xlower - x) / dx)) -
+                                 (int)dmax(0.0, ceil((x - xupper) / dx));
+                const int jint = jj + (int)dmax(0.0, ceil((ylower - y) / dy)) -
+                                 (int)dmax(0.0, ceil((y - yupper) / dy));
+                A1(ii, jj) = A1(iint, jint);
+            }
+        }
+    }
+    for (int jj = 1 - mbc; jj <= my + mbc; jj++)
+        for (int ii = 1 - mbc; ii <= mx + mbc; ii++) {
+            double* a = aux + (size_t)maux * CI(ii, jj);
+            for (int k = 1; k < maux; k++) a[k] = 0.0;
+            a[1] = phi;
+            a[2] = theta;
+        }
+#undef A1
+#undef CI
+}

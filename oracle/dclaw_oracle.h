@@ -142,6 +142,37 @@ void oracle_amr_interp(const double* valc, const double* auxc, int mauxc, int mi
                        const unsigned char* needs_set, int rx, int ry, double dry_tolerance,
                        double sea_level, double rho_f, double grav);
 
+/* filpatch.f90:182-187: coarse rectangle (iplo, iphi, jplo, jphi) around
+ * the unset cells isl..isr x jsb..jst. */
+void oracle_amr_coarse_box(int isl, int isr, int jsb, int jst, int rx, int ry, int* box);
+
+/* filpatch.f90:256-520: interpolation of a region's unset cells from the
+ * coarse rectangle (mxc x myc from (iplo, jplo)).  0 on success, -1 if an
+ * unset cell is outside the rectangle's interior. */
+int oracle_amr_filpatch(const double* valcrse, const double* auxcrse, int mauxc, int mxc, int myc,
+                        int iplo, int jplo, double* valbig, const double* aux, int maux, int mx,
+                        int r0, int c0, int ilo, int ihi, int jlo, int jhi,
+                        const unsigned char* flaguse, int rx, int ry, double xlower,
+                        double ylower, double dx_fine, double dy_fine, double dx_coarse,
+                        double dy_coarse, double sea_level, double dry_tolerance, double rho_f,
+                        double grav);
+
+/* AMRClaw steps of a region fill (not in the reference; restated). */
+void oracle_amr_intfil(double* q, double* aux, int maux, int mx, int r0, int c0, int ilo,
+                       int ihi, int jlo, int jhi, unsigned char* flag, const double* sq,
+                       const double* sq_old, double w_old, const double* saux, int smaux,
+                       int silo, int sjlo, int smx, int smy, int copy_aux);
+int oracle_amr_trimbd(const unsigned char* flag, int ilo, int ihi, int jlo, int jhi, int* box);
+void oracle_amr_bc2amr(double* q, double* aux, int maux, int mx, int r0, int c0, int ilo,
+                       int ihi, int jlo, int jhi, int iregsz, int jregsz, const int* bc);
+
+/* setaux.f90:59-273 for a patch with 2 ghost cells, the topography integral
+ * supplied as bed (used where bed_set is 0). */
+void oracle_amr_setaux(double* aux, int maux, int mx, int my, int ilo, int jlo,
+                       const unsigned char* bed_set, const double* bed, double xlower,
+                       double xupper, double ylower, double yupper, double dx, double dy,
+                       double phi, double theta);
+
 #ifdef __cplusplus
 }
 #endif

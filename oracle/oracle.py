@@ -151,6 +151,19 @@ def lib():
             ctypes.c_double]
         L.oracle_amr_interp.argtypes = [_dp, _dp, _ci, _ci, _ci, _dp, _dp, _ci, _ci,
                                         ctypes.c_void_p, _ci, _ci] + [ctypes.c_double] * 4
+        _cd = ctypes.c_double
+        _vp = ctypes.c_void_p
+        L.oracle_amr_coarse_box.argtypes = [_ci] * 6 + [_vp]
+        L.oracle_amr_filpatch.argtypes = [_dp, _dp, _ci, _ci, _ci, _ci, _ci, _dp, _dp, _ci, _ci,
+                                          _ci, _ci, _ci, _ci, _ci, _ci, _vp, _ci, _ci] + [_cd] * 10
+        L.oracle_amr_filpatch.restype = _ci
+        L.oracle_amr_intfil.argtypes = [_dp, _dp, _ci, _ci, _ci, _ci, _ci, _ci, _ci, _ci, _vp,
+                                        _dp, _vp, _cd, _dp, _ci, _ci, _ci, _ci, _ci, _ci]
+        L.oracle_amr_trimbd.argtypes = [_vp, _ci, _ci, _ci, _ci, _vp]
+        L.oracle_amr_trimbd.restype = _ci
+        L.oracle_amr_bc2amr.argtypes = [_dp, _dp, _ci, _ci, _ci, _ci, _ci, _ci, _ci, _ci, _ci,
+                                        _ci, _vp]
+        L.oracle_amr_setaux.argtypes = [_dp, _ci, _ci, _ci, _ci, _ci, _vp, _dp] + [_cd] * 8
         _libs[_variant] = L
     return _libs[_variant]
 
@@ -410,3 +423,79 @@ def amr_interp(valc, auxc, val, auxf, needs_set, rx, ry, dry_tolerance, sea_leve
                             auxf.shape[0], val.shape[1], ns.ctypes.data, rx, ry, dry_tolerance,
                             sea_level, rho_f, grav)
     return val
+
+
+# ------------------------------------------------------------ AMR region fill
+# Regions: q (7, nx, ny) and aux (maux, nx, ny) Fortran-ordered arrays that
+# hold cells ilo..ihi x jlo..jhi of a level from column r0, row c0; flag
+# (ihi-ilo+1, jhi-jlo+1) uint8 Fortran-ordered.  These functions work in
+# place on the arrays given (which must already be float64 / uint8 and
+# Fortran-ordered).
+
+def _fchk(a, dtype):
+    assert a.dtype == dtype and a.flags.f_contiguous, "need a Fortran-ordered array"
+
+
+def amr_coarse_box(isl, isr, jsb, jst, rx, ry):
+    box = np.zeros(4, dtype=np.int32)
+    lib().oracle_amr_coarse_box(isl, isr, jsb, jst, rx, ry, box.ctypes.data)
+    return [int(v) for v in box]
+
+
+def amr_filpatch(valc, auxc, iplo, jplo, q, aux, r0, c0, ilo, ihi, jlo, jhi, flag, rx, ry,
+                 xlower, ylower, dxf, dyf, dxc, dyc, sea_level, dry_tolerance, rho_f, grav):
+    for a in (valc, auxc, q, aux):
+        _fchk(a, np.float64)
+    _fchk(flag, np.uint8)
+    rc = lib().oracle_amr_filpatch(_ptr(valc), _ptr(auxc), auxc.shape[0], valc.shape[1],
+                                   valc.shape[2], iplo, jplo, _ptr(q), _ptr(aux), aux.shape[0],
+                                   q.shape[1], r0, c0, ilo, ihi, jlo, jhi, flag.ctypes.data, rx,
+                                   ry, xlower, ylower, dxf, dyf, dxc, dyc, sea_level,
+                                   dry_tolerance, rho_f, grav)
+    if rc != 0:
+        raise ValueError("filpatch: an unset cell lies outside the coarse rectangle")
+
+
+def amr_intfil(q, aux, r0, c0, ilo, ihi, jlo, jhi, flag, sq, saux, silo, sjlo, sq_old=None,
+               w_old=0.0, copy_aux=True):
+    """Copy from a source patch (sq: (7, smx+4, smy+4) with 2 ghost cells,
+    interior cell (silo + a, sjlo + b) at column a + 2, row b + 2)."""
+    for a in (q, aux, sq, saux):
+        _fchk(a, np.float64)
+    _fchk(flag, np.uint8)
+    old = None
+    if sq_old is not None:
+        _fchk(sq_old, np.float64)
+        old = sq_old.ctypes.data
+    lib().oracle_amr_intfil(_ptr(q), _ptr(aux), aux.shape[0], q.shape[1], r0, c0, ilo, ihi, jlo,
+                            jhi, flag.ctypes.data, _ptr(sq), old, w_old, _ptr(saux),
+                            saux.shape[0], silo, sjlo, sq.shape[1] - 4, sq.shape[2] - 4,
+                            1 if copy_aux else 0)
+
+
+def amr_trimbd(flag, ilo, ihi, jlo, jhi):
+    """Bounding box (isl, isr, jsb, jst) of the unset cells, or None."""
+    _fchk(flag, np.uint8)
+    box = np.zeros(4, dtype=np.int32)
+    if lib().oracle_amr_trimbd(flag.ctypes.data, ilo, ihi, jlo, jhi, box.ctypes.data):
+        return None
+    return [int(v) for v in box]
+
+
+def amr_bc2amr(q, aux, r0, c0, ilo, ihi, jlo, jhi, iregsz, jregsz, bc):
+    for a in (q, aux):
+        _fchk(a, np.float64)
+    b = np.asarray(bc, dtype=np.int32)
+    lib().oracle_amr_bc2amr(_ptr(q), _ptr(aux), aux.shape[0], q.shape[1], r0, c0, ilo, ihi, jlo,
+                            jhi, iregsz, jregsz, b.ctypes.data)
+
+
+def amr_setaux(aux, ilo, jlo, bed_set, bed, xlower, xupper, ylower, yupper, dx, dy, phi, theta):
+    """setaux for a patch: aux (maux, mx+4, my+4); bed_set (mx+4, my+4)
+    uint8, bed (mx+4, my+4) float64, both Fortran-ordered."""
+    _fchk(aux, np.float64)
+    _fchk(bed_set, np.uint8)
+    _fchk(bed, np.float64)
+    lib().oracle_amr_setaux(_ptr(aux), aux.shape[0], aux.shape[1] - 4, aux.shape[2] - 4, ilo, jlo,
+                            bed_set.ctypes.data, _ptr(bed), xlower, xupper, ylower, yupper, dx,
+                            dy, phi, theta)

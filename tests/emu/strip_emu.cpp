@@ -457,6 +457,26 @@ extern "C" void emu_amr_interp(const double* valc, const double* auxc, int mauxc
                             j, dry_tolerance, sea_level, rho_f, grav);
 }
 
+// filpatch's interpolation (amr_filpatch_cell) over every interior coarse
+// cell of the rectangle, in reverse order (cells are independent).
+extern "C" void emu_amr_filpatch(const double* valc, const double* auxc, int mauxc, int mxc,
+                                 int myc, int iplo, int jplo, double* q, const double* aux,
+                                 int maux, int mx, int r0, int c0, int ilo, int ihi, int jlo,
+                                 int jhi, const unsigned char* flag, int rx, int ry,
+                                 double xlower, double ylower, double dxf, double dyf,
+                                 double dxc, double dyc, double sea_level, double drytol,
+                                 double rho_f, double grav)
+{
+    AmrFill G;
+    G.ilo = ilo; G.ihi = ihi; G.jlo = jlo; G.jhi = jhi; G.r0 = r0; G.c0 = c0; G.mx = mx;
+    G.maux = maux; G.iplo = iplo; G.jplo = jplo; G.mxc = mxc; G.myc = myc; G.mauxc = mauxc;
+    G.rx = rx; G.ry = ry; G.xlower = xlower; G.ylower = ylower; G.dxf = dxf; G.dyf = dyf;
+    G.dxc = dxc; G.dyc = dyc; G.sea_level = sea_level; G.drytol = drytol; G.rho_f = rho_f;
+    G.grav = grav;
+    for (int jc = myc - 1; jc >= 2; jc--)
+        for (int ic = mxc - 1; ic >= 2; ic--) amr_filpatch_cell(valc, auxc, q, aux, flag, G, ic, jc);
+}
+
 // The tangent-linear fused step: q_out, cflgrid as emu_stepgrid (aux not
 // written); dq_out = J_q dq_in + J_P dP at dt.
 extern "C" int emu_stepgrid_tl(const dclaw_params* P, const dclaw_params* dP, const double* q_in,
