@@ -131,6 +131,8 @@ SIGNATURES = {
                                        ctypes.c_double, _vp]),
     "dclaw_set_params": (ctypes.c_int, [ctypes.POINTER(CParams)]),
     "stepgrid_dclaw_": (None, [_dp, _dp, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),
+    "stepgrid_fluxes_dclaw_": (None, [_dp, _dp, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      _dp, _dp]),
     "dclaw_dt_update_out": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int,
                                            ctypes.c_double, ctypes.c_double, _vp]),
     "dclaw_gauges_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,

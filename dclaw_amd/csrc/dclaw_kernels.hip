@@ -1643,6 +1643,8 @@ struct HostCtx {
     size_t cap_qa = 0, cap_qb = 0, cap_aux = 0;
     double* d_flux[4] = {nullptr, nullptr, nullptr, nullptr};   // step2_ fm, fp, gm, gp
     size_t cap_flux[4] = {0, 0, 0, 0};
+    double* d_qc = nullptr;                                      // b4step2 copy for the fluxes
+    size_t cap_qc = 0;
     double* d_curv = nullptr;                                    // curvature workspace
     size_t cap_curv = 0;
     int* d_nan = nullptr;                                        // check4nans result
@@ -2681,7 +2683,7 @@ int dclaw_set_params(const dclaw_params* p)
 }
 
 static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int maux, double dx,
-                         double dy, double dt, double* cflgrid)
+                         double dy, double dt, double* cflgrid, double* const* flux = nullptr)
 {
     if (mbc != DCLAW_MBC) return fail(DCLAW_EINVAL, "mbc must be 2");
     int rc = check_patch(mx, my, maux);
@@ -2700,6 +2702,23 @@ static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int ma
     HIP_TRY(hipMemcpy(g_host.d_qa, q, nq * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g_host.d_aux, aux, na * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
+    if (flux) {
+        // fm, fp, gm, gp of the step: step2 on the b4step2 state of a copy
+        // (the fused step below computes the same values internally)
+        for (int k = 0; k < 4; k++)
+            if ((rc = grow(&g_host.d_flux[k], &g_host.cap_flux[k], nq))) return rc;
+        if ((rc = grow(&g_host.d_qc, &g_host.cap_qc, nq))) return rc;
+        HIP_TRY(hipMemcpy(g_host.d_qc, g_host.d_qa, nq * sizeof(double),
+                          hipMemcpyDeviceToDevice));
+        if ((rc = dclaw_b4step2_dev(g_host.d_qc, g_host.d_aux, maux, mx, my, dx, dy,
+                                    g_host.d_params, nullptr, nullptr)))
+            return rc;
+        if ((rc = dclaw_step2_dev(g_host.d_qc, g_host.d_aux, maux, mx, my, dx, dy,
+                                  g_host.d_scalars, g_host.d_params, g_host.d_flux[0],
+                                  g_host.d_flux[1], g_host.d_flux[2], g_host.d_flux[3],
+                                  g_host.d_scalars + 1, nullptr)))
+            return rc;
+    }
     if (g_host.params.curvature == 1) {
         const size_t nw = dclaw_curvature_workspace_doubles(mx, my);
         if ((rc = grow(&g_host.d_curv, &g_host.cap_curv, nw))) return rc;
@@ -2717,6 +2736,10 @@ static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int ma
     HIP_TRY(hipMemcpy(aux, g_host.d_aux, na * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(scal, g_host.d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost));
     *cflgrid = scal[1];
+    if (flux)
+        for (int k = 0; k < 4; k++)
+            HIP_TRY(hipMemcpy(flux[k], g_host.d_flux[k], nq * sizeof(double),
+                              hipMemcpyDeviceToHost));
     return DCLAW_OK;
 }
 
@@ -2883,6 +2906,21 @@ void stepgrid_dclaw_(double* q, double* aux, const int* mbc, const int* mx, cons
         // The Fortran caller has no status argument; like the reference's
         // fatal paths (check4nans, set_dig) this stops the run.
         std::fprintf(stderr, "stepgrid_dclaw: %s\n", g_err.c_str());
+        std::abort();
+    }
+}
+
+void stepgrid_fluxes_dclaw_(double* q, double* aux, const int* mbc, const int* mx, const int* my,
+                            const int* maux, const double* dx, const double* dy,
+                            const double* dt, double* cflgrid, double* fm, double* fp,
+                            double* gm, double* gp)
+{
+    double* const flux[4] = {fm, fp, gm, gp};
+    int rc = (fm && fp && gm && gp) ? DCLAW_OK : fail(DCLAW_EINVAL, "null flux array");
+    if (rc == DCLAW_OK)
+        rc = stepgrid_host(q, aux, *mbc, *mx, *my, *maux, *dx, *dy, *dt, cflgrid, flux);
+    if (rc != DCLAW_OK) {
+        std::fprintf(stderr, "stepgrid_fluxes_dclaw: %s\n", g_err.c_str());
         std::abort();
     }
 }

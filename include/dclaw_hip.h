@@ -539,6 +539,16 @@ int dclaw_set_params(const dclaw_params* params_host);
 void stepgrid_dclaw_(double* q, double* aux, const int* mbc, const int* mx, const int* my,
                      const int* maux, const double* dx, const double* dy, const double* dt,
                      double* cflgrid);
+/* stepgrid_dclaw_ that also returns the step's fm, fp, gm, gp (host arrays
+ * of q's shape, every entry written, in step2's summation order, equal to
+ * what the step applied): what AMRClaw's advanc hands to its conservation
+ * fix-up (fluxsv on a coarse patch, qad and fluxad on a fine one) after
+ * GeoClaw's stepgrid, whose fm..gp arguments this fills (one extra b4step2
+ * and step2 pass on the device). */
+void stepgrid_fluxes_dclaw_(double* q, double* aux, const int* mbc, const int* mx, const int* my,
+                            const int* maux, const double* dx, const double* dy,
+                            const double* dt, double* cflgrid, double* fm, double* fp,
+                            double* gm, double* gp);
 
 /* Per-stage Fortran drop-ins with the reference's own argument lists, for
  * executables that keep GeoClaw's stepgrid and exclude the D-Claw stage

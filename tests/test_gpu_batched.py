@@ -137,3 +137,39 @@ def test_host_dropin_stepgrid(dev, oracle_mod):
     assert mask_equal(qh, qo, pt.params.dry_tolerance)
     assert np.allclose(ah, auxo, rtol=1e-12, atol=1e-300)
     assert abs(cfl.value - cflo) <= TOL_STEP * cflo
+
+
+def test_host_dropin_stepgrid_with_fluxes(dev, oracle_mod):
+    """stepgrid_fluxes_dclaw_: the same q, aux and cflgrid as stepgrid_dclaw_,
+    and fm, fp, gm, gp equal to the oracle's step2 on the oracle's b4step2
+    state bit for bit (what AMRClaw's fix-up reads after stepgrid)."""
+    from dclaw_amd import _lib
+    from dclaw_amd.params import to_c
+    from helpers import advance_oracle
+    O = oracle_mod
+    L = _lib.lib()
+    pt = slope_patch(61, 44, slope=0.7, phi_deg=26.0, seed=5)
+    q, aux, dt = advance_oracle(O, pt, 6)
+    cp = to_c(pt.params)
+    _lib.check(L.dclaw_set_params(ctypes.byref(cp)))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    print("[host stepgrid with fluxes] 61x44", flush=True)
+    q1, a1 = np.array(q, order="F", copy=True), np.array(aux, order="F", copy=True)
+    c1 = ctypes.c_double(0.0)
+    L.stepgrid_dclaw_(q1.ctypes.data_as(dp), a1.ctypes.data_as(dp), ci(2), ci(pt.mx), ci(pt.my),
+                      ci(a1.shape[0]), cd(pt.dx), cd(pt.dy), cd(dt), ctypes.byref(c1))
+    q2, a2 = np.array(q, order="F", copy=True), np.array(aux, order="F", copy=True)
+    c2 = ctypes.c_double(0.0)
+    f = [np.full_like(q2, np.nan, order="F") for _ in range(4)]
+    L.stepgrid_fluxes_dclaw_(q2.ctypes.data_as(dp), a2.ctypes.data_as(dp), ci(2), ci(pt.mx),
+                             ci(pt.my), ci(a2.shape[0]), cd(pt.dx), cd(pt.dy), cd(dt),
+                             ctypes.byref(c2), *[a.ctypes.data_as(dp) for a in f])
+    assert np.array_equal(q1, q2) and np.array_equal(a1, a2) and c1.value == c2.value
+    P = O.params_from(pt.params)
+    qb, ab = O.b4step2(P, q, aux, pt.dx, pt.dy)
+    cflo, *fo = O.step2(P, qb, ab, pt.dx, pt.dy, dt)
+    for k, (a, b) in enumerate(zip(f, fo)):
+        assert np.array_equal(a, b), f"flux {k}"
+    assert c2.value == cflo
