@@ -42,6 +42,7 @@
 
 #include "dclaw_tangent.h"
 #include "dclaw_tangent_k.h"
+#include "dclaw_adjoint_src2.h"
 
 namespace dclaw_adj {
 
@@ -720,9 +721,14 @@ DC_HD void gp_add(const AdjArgs& A, size_t n, const double* g)
         if (p < A.np) ws(A, W_GP + p, n, wv(A, W_GP + p, n) + g[p]);
 }
 
-// B1: src2 (and the parameter gradient is started here).
-DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
+// B1: src2 (and the parameter gradient is started here).  WRITTEN selects
+// the configurations src2_vjp covers (the written-out reverse) or the others
+// (seeded products); a call for the other kind does nothing, so the two run
+// as separate passes and each keeps only its own registers.
+template <bool WRITTEN>
+DC_HD void rev_src2_t(const AdjArgs& A, const TP& Pz, int i, int j)
 {
+    if (src2_vjp_covers(A.P) != WRITTEN) return;
     const size_t n = cix(A, i, j);
     gp_zero(A, n);
     double lx[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -735,15 +741,32 @@ DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
             lx[6] = ly[6];
         } else {
             const double h_e = A.P.entrainment ? A.aux[aidx(i, j, A.mx, A.maux) + 6] : 0.0;
-            Src2F f{A.dt, h_e};
-            // without ent_dclaw4's erosion src2 passes bdif through unchanged
-            const bool erodes = A.P.entrainment == 1 && A.P.entrainment_method == 0;
-            vjpk<7, 7>(A, f, x, ly, lx, g, erodes ? 7 : 6);
-            if (!erodes) lx[6] = ly[6];
+            if constexpr (WRITTEN) {
+                // the written-out reverse (bdif passes through)
+                ParAdj pa;
+                pa_zero(pa);
+                src2_vjp(A.P, A.dt, x, ly, lx, pa);
+                lx[6] = ly[6];
+#pragma unroll
+                for (int p = 0; p < MAXP; p++)
+                    if (p < A.np) g[p] = pa_dot(pa, A.dP[p]);
+            } else {
+                Src2F f{A.dt, h_e};
+                // without ent_dclaw4's erosion src2 passes bdif through unchanged
+                const bool erodes = A.P.entrainment == 1 && A.P.entrainment_method == 0;
+                vjpk<7, 7>(A, f, x, ly, lx, g, erodes ? 7 : 6);
+                if (!erodes) lx[6] = ly[6];
+            }
             gp_add(A, n, g);
         }
     }
     for (int m = 0; m < 7; m++) ws(A, W_LQN + m, n, lx[m]);
+}
+
+DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    rev_src2_t<true>(A, Pz, i, j);
+    rev_src2_t<false>(A, Pz, i, j);
 }
 
 // Vector-Jacobian product of one rpt2 split (SplitF, rpt2_cell_split)

@@ -1076,7 +1076,7 @@ __global__ void k_gauges(const double* __restrict__ q, const double* __restrict_
 enum AdjStage {
     ADJ_REC_PREPARE, ADJ_REC_TAUDIR, ADJ_REC_WAVES, ADJ_REC_FLUX, ADJ_REC_SPLIT, ADJ_REC_UPDATE,
     ADJ_REV_SRC2, ADJ_REV_SPLIT, ADJ_REV_FLUX, ADJ_REV_WAVES, ADJ_REV_TAUDIR, ADJ_REV_PREPARE,
-    ADJ_REV_BC, ADJ_NSTAGES
+    ADJ_REV_BC, ADJ_REV_SRC2_SEEDED, ADJ_NSTAGES
 };
 // Tiles are 16 cells wide and 4 high (64 cells: one wavefront): a tile row
 // of a workspace plane is one 128-byte line (the planes' row stride is a
@@ -1179,7 +1179,8 @@ __device__ __forceinline__ void adj_stage(const dclaw_adj::AdjArgs& A, const dcl
     else if constexpr (STAGE == ADJ_REC_FLUX) dclaw_adj::rec_flux(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REC_SPLIT) dclaw_adj::rec_split(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REC_UPDATE) dclaw_adj::rec_update(A, Pz, i, j);
-    else if constexpr (STAGE == ADJ_REV_SRC2) dclaw_adj::rev_src2(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REV_SRC2) dclaw_adj::rev_src2_t<true>(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_REV_SRC2_SEEDED) dclaw_adj::rev_src2_t<false>(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REV_SPLIT) dclaw_adj::rev_split(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REV_FLUX) dclaw_adj::rev_flux(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REV_WAVES) dclaw_adj::rev_waves(A, Pz, i, j);
@@ -1943,6 +1944,7 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_SPLIT>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_UPDATE>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_SRC2>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint<ADJ_REV_SRC2_SEEDED>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_SPLIT>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_FLUX>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_WAVES>, tg, tb, 0, st, A, T);
@@ -2041,6 +2043,7 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     hipLaunchKernelGGL(k_adjb<ADJ_REC_SPLIT>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_UPDATE>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_SRC2>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb<ADJ_REV_SRC2_SEEDED>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb2<ADJ_REV_SPLIT>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_FLUX>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_WAVES>, tg, tb, 0, st, B);

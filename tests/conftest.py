@@ -25,6 +25,7 @@ def build_emulator() -> str:
     srcs = [EMU_SRC] + [os.path.join(ROOT, "dclaw_amd", "csrc", f)
                         for f in ("dclaw_strip.h", "dclaw_physics.h", "dclaw_amr.h",
                                   "dclaw_tangent.h", "dclaw_tangent_k.h", "dclaw_adjoint.h",
+                                  "dclaw_adjoint_src2.h",
                                   "dclaw_strip_body.inc", "dclaw_prepare_body.inc",
                                   "dclaw_physics_body.inc", "dcmath.h")]
     if os.path.exists(EMU_LIB) and all(os.path.getmtime(EMU_LIB) >= os.path.getmtime(s)

@@ -679,6 +679,95 @@ extern "C" double emu_split_vjp_check(const dclaw_params* P, int n, unsigned see
     return worst;
 }
 
+// The written-out src2 reverse (src2_vjp) against the seeded vector-Jacobian
+// product vjp<7, 7>(Src2F) on n random wet cells (velocities of both signs
+// and at rest, m and chi inside and outside [0, 1], pb below and above
+// rho g h, a few with hm < 0), for the six state inputs and the 13
+// parameter fields it covers (one unit direction each).  Returns the
+// largest difference relative to the largest input adjoint (parameters:
+// as sensitivities |dL/dp| |p| against the largest of them); worst2[0..1]
+// get the input and the parameter parts when not null.
+extern "C" double emu_src2_vjp_check(const dclaw_params* P, int n, unsigned seed, double dt,
+                                     double* worst2)
+{
+    namespace D = dclaw_adj;
+    unsigned long long st = seed * 2862933555777941757ull + 3037000493ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)((st >> 11) & ((1ull << 53) - 1)) / (double)(1ull << 53) * 2.0 - 1.0;
+    };
+    constexpr int NP = 13;
+    dclaw_params dP[4];
+    const double pv[NP] = {P->grav, P->rho_s, P->rho_f, P->mu, P->phi, P->kref, P->mref,
+                           P->delta_kr_order, P->alpha_c, P->sigma_0, P->delta, P->m_crit,
+                           P->c1};
+    D::AdjArgs A{};
+    A.P = *P;
+    A.dP = dP;
+    const D::TP Pz = D::zero_params(*P);
+    double wi = 0.0, wp = 0.0;
+    for (int c = 0; c < n; c++) {
+        double x[7], ly[7], la[7], lb[7];
+        const double h = 0.01 + 1.5 * (rnd() + 1.0);
+        x[0] = h;
+        const bool still = c % 9 == 0;
+        x[1] = still ? 0.0 : 3.0 * rnd() * h;
+        x[2] = still ? 0.0 : 3.0 * rnd() * h;
+        double m = 0.5 + 0.25 * rnd();
+        if (c % 11 == 3) m = 1.0 + 0.05 * (rnd() + 1.0);
+        if (c % 13 == 5) m = -0.05 * (rnd() + 1.0);
+        x[3] = m * h;
+        const double rho = P->rho_s * m + (1.0 - m) * P->rho_f;
+        x[4] = (0.6 + 0.5 * rnd()) * rho * P->grav * h;
+        if (c % 7 == 2) x[4] = -0.1 * rho * P->grav * h;
+        x[5] = (0.5 + 0.6 * rnd()) * h;
+        x[6] = 0.1 * rnd();
+        for (int k = 0; k < 7; k++) ly[k] = rnd();
+        D::Src2F f{dt, 0.0};
+        // inputs
+        A.np = 0;
+        D::vjp<7, 7>(A, f, Pz, x, ly, la, nullptr);
+        D::ParAdj pa;
+        D::pa_zero(pa);
+        D::src2_vjp(*P, dt, x, ly, lb, pa);
+        double scale = 0.0, diff = 0.0;
+        for (int k = 0; k < 6; k++) {
+            scale = std::max(scale, std::fabs(la[k]));
+            diff = std::max(diff, std::fabs(la[k] - lb[k]));
+        }
+        wi = std::max(wi, scale > 0.0 ? diff / scale : diff);
+        // parameters, four directions per seeded call
+        double ga[NP], gb[NP];
+        for (int p0 = 0; p0 < NP; p0 += 4) {
+            const int np = std::min(4, NP - p0);
+            for (int k = 0; k < np; k++) {
+                std::memset(&dP[k], 0, sizeof(dclaw_params));
+                double* fld[NP] = {&dP[k].grav, &dP[k].rho_s, &dP[k].rho_f, &dP[k].mu,
+                                   &dP[k].phi, &dP[k].kref, &dP[k].mref, &dP[k].delta_kr_order,
+                                   &dP[k].alpha_c, &dP[k].sigma_0, &dP[k].delta,
+                                   &dP[k].m_crit, &dP[k].c1};
+                *fld[p0 + k] = 1.0;
+            }
+            A.np = np;
+            double g4[4] = {0, 0, 0, 0}, tmp[7];
+            D::vjp<7, 7>(A, f, Pz, x, ly, tmp, g4);
+            for (int k = 0; k < np; k++) {
+                ga[p0 + k] = g4[k];
+                gb[p0 + k] = D::pa_dot(pa, dP[k]);
+            }
+        }
+        double gs = 0.0;
+        for (int k = 0; k < NP; k++) gs = std::max(gs, std::fabs(ga[k] * pv[k]));
+        for (int k = 0; k < NP; k++)
+            if (gs > 0.0) wp = std::max(wp, std::fabs((ga[k] - gb[k]) * pv[k]) / gs);
+    }
+    if (worst2) {
+        worst2[0] = wi;
+        worst2[1] = wp;
+    }
+    return std::max(wi, wp);
+}
+
 // The hand-written rpn2 adjoint (rpn2_vjp) against the seeded tangent-linear
 // vector-Jacobian product (30 side inputs and the grav, rho_f, kappa,
 // beta_seg directions) on n random edges of 8 kinds, c % 8: both sides wet

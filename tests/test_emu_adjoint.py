@@ -250,3 +250,50 @@ def test_k_direction_products_equal_seeded_bitwise(emu_lib, variant, np_):
     elif variant == "entrainment_m1":
         p.src2method, p.entrainment = 1, 1
     assert emu_lib.emu_vjpk_check(ctypes.byref(p), 300, np_, 11) == 0
+
+
+@pytest.mark.parametrize("variant", [
+    dict(), dict(src2method=-1), dict(src2method=1), dict(alphamethod=1),
+    dict(segregation=1, delta_kr_order=1.5), dict(kref=1.0e-4, mu=0.001),
+])
+@pytest.mark.parametrize("dt", [1e-3, 0.05])
+def test_hand_src2_adjoint_equals_seeded(emu_lib, variant, dt):
+    """The written-out reverse of src2 (src2_vjp: friction, the relaxation
+    source with its three qfix passes and qfix_cmass, setvars twice) equals
+    the seeded vector-Jacobian product on random wet cells: the six state
+    adjoints to 1e-10 of the largest, the 13 parameter sensitivities to 1e-7
+    of the largest.  The parameter tolerance is wider because the seeded
+    forward-mode product loses digits where a parameter enters a nearly
+    cancelling pair (p_eq + (p4 - p_eq) E1 with E1 -> 1 at small dt, the
+    rho_s and rho_f terms of rho h); the full step's transpose identity with
+    these parameters holds to 1e-12 (test_src2_parameters_transpose)."""
+    from dclaw_amd import workloads as W
+    p = W.config0_params()
+    for k, v in variant.items():
+        setattr(p, k, v)
+    c = to_c(p)
+    worst = (ctypes.c_double * 2)()
+    emu_lib.emu_src2_vjp_check.restype = ctypes.c_double
+    emu_lib.emu_src2_vjp_check(ctypes.byref(c), 3000, 17, ctypes.c_double(dt), worst)
+    assert worst[0] <= 1e-10 and worst[1] <= 1e-7, list(worst)
+
+
+@pytest.mark.parametrize("variant", [
+    dict(), dict(src2method=1), dict(alphamethod=1), dict(segregation=1, delta_kr_order=1.5),
+    dict(src2method=-1),
+])
+def test_src2_parameters_transpose(emu_lib, oracle_mod, variant):
+    """<lambda, J dP> = g . a over every parameter field src2 reads (grav,
+    rho_s, rho_f, alpha_c, c1, delta, mref and sigma_0 or delta_kr_order),
+    each one against the tangent-linear step as well."""
+    pt = slope_patch(60, 36, slope=0.9, phi_deg=22.0, seed=5)
+    for k, v in variant.items():
+        setattr(pt.params, k, v)
+    if variant.get("segregation"):
+        pt.q[5] = 0.4 * pt.q[0]
+    q, aux, dt = advance_oracle(oracle_mod, pt, 6)
+    q, aux = np.asfortranarray(q), np.asfortranarray(aux)
+    _dot_identity(emu_lib, pt, q, aux, dt, (1, 1, 3, 3), ["grav", "rho_s", "rho_f", "alpha_c"],
+                  seed=2)
+    last = "delta_kr_order" if variant.get("segregation") else "sigma_0"
+    _dot_identity(emu_lib, pt, q, aux, dt, (1, 1, 3, 3), ["c1", "delta", "mref", last], seed=3)
