@@ -2025,6 +2025,88 @@ DC_HD void side_to_pc(const double* ls, bool second, int ixy, double* lp)
     lp[second ? PC_TAU2 : PC_TAU1] += ls[SD_TAU];
 }
 
+// ------------------------------------------------------------------ prepare
+// The prepare stage (PrepF: prepare_cell, dclaw_prepare_body.inc, then
+// rho_tau of F1 and F2) on a wet loaded cell (h > dry_tolerance): b4step2's
+// qfix (digclaw_module.f90:285-345) and speed limit (b4step2.f90:74-81),
+// rpn2's two qfix passes F1 and F2, setvars of both.  lp: adjoints of the
+// NPC prepared fields; lx: adjoints of h, hu, hv, hm, pb, hchi (bdif is the
+// caller's: it enters as max(bdif, 0) only); pa: parameter adjoints (added).
+DC_HD void prepare_vjp(const dclaw_params& P, const double* q7, const double* lp, double* lx,
+                       ParAdj& pa)
+{
+    const double gz = P.grav;
+    QfFwd q0;
+    qfix_fwd(P, q7[0], q7[1], q7[2], q7[3], q7[4], q7[5], gz, q0);
+    const double h = q0.h;
+    const double hm0 = h * q0.m, p0 = q0.p, hchi0 = h * q0.chi;
+    // speed limit
+    const double n0 = sqrt((q7[1] * q7[1] + q7[2] * q7[2]));
+    const double sp = n0 / h;
+    const bool lim = h > 0.0 && sp > P.speed_limit;
+    const double sratio = lim ? P.speed_limit / sp : 1.0;
+    const double hu = lim ? q7[1] * sratio : q7[1], hv = lim ? q7[2] * sratio : q7[2];
+    QfFwd f1, f2;
+    qfix_fwd(P, h, hu, hv, hm0, p0, hchi0, gz, f1);
+    const double hm1 = h * f1.m, p1 = f1.p, hchi1 = h * f1.chi;
+    qfix_fwd(P, h, hu, hv, hm1, p1, hchi1, gz, f2);
+    SvFwd s1, s2;
+    setvars_fwd(P, h, f1.u, f1.v, f1.m, f1.p, f1.chi, gz, s1);
+    setvars_fwd(P, h, f1.u, f1.v, f2.m, f2.p, f2.chi, gz, s2);
+    double b_gz = 0.0;
+    // setvars of F2 and F1 (rho, tau)
+    double bh = lp[PC_H], bu = lp[PC_U], bv = lp[PC_V];
+    double bm2 = lp[PC_M2], bp2 = lp[PC_P2], bchi2 = lp[PC_CHI2];
+    setvars_rev(P, h, f1.u, f1.v, f2.m, f2.chi, gz, s2, lp[PC_RHO2], 0.0, 0.0, 0.0, 0.0,
+                lp[PC_TAU2], bh, bu, bv, bm2, bp2, bchi2, b_gz, pa);
+    double bm1 = lp[PC_M1], bp1 = lp[PC_P1], bchi1 = lp[PC_CHI1];
+    setvars_rev(P, h, f1.u, f1.v, f1.m, f1.chi, gz, s1, lp[PC_RHO1], 0.0, 0.0, 0.0, 0.0,
+                lp[PC_TAU1], bh, bu, bv, bm1, bp1, bchi1, b_gz, pa);
+    // F2 (its h, hu, hv, u, v, rho are not read)
+    double t_h = 0.0, b_hu = 0.0, b_hv = 0.0, t_hm = lp[PC_HM2], t_hchi = lp[PC_HCHI2];
+    qfix_rev(P, f2, gz, t_h, b_hu, b_hv, t_hm, bp2, t_hchi, 0.0, 0.0, bm2, bchi2, 0.0, b_gz,
+             pa);
+    bh += t_h;
+    // F1: hm1, p1, hchi1 feed F2 and are outputs
+    double u_h = 0.0, b_hm0 = lp[PC_HM1] + t_hm, b_hchi0 = lp[PC_HCHI1] + t_hchi;
+    bp1 += bp2;
+    qfix_rev(P, f1, gz, u_h, b_hu, b_hv, b_hm0, bp1, b_hchi0, bu, bv, bm1, bchi1, 0.0, b_gz, pa);
+    bh += u_h;
+    // Q0's hm, p, hchi are outputs and feed F1; hu, hv (after the limit) too
+    b_hm0 += lp[PC_HM0];
+    double b_p0 = bp1 + lp[PC_P0];
+    b_hchi0 += lp[PC_HCHI0];
+    b_hu += lp[PC_HU];
+    b_hv += lp[PC_HV];
+    double b_hu0 = b_hu, b_hv0 = b_hv;
+    if (lim) {
+        // hu = hu0 L / s, s = |(hu0, hv0)| / h
+        b_hu0 = b_hu * sratio;
+        b_hv0 = b_hv * sratio;
+        const double b_sr = b_hu * q7[1] + b_hv * q7[2];
+        pa.speed_limit += b_sr / sp;
+        const double b_s = -b_sr * sratio / sp;
+        const double b_n = b_s / h;
+        bh -= b_s * sp / h;
+        if (n0 > 0.0) {
+            b_hu0 += b_n * q7[1] / n0;
+            b_hv0 += b_n * q7[2] / n0;
+        }
+    }
+    // Q0: the loaded cell's qfix (its u, v, m, chi, rho are not read)
+    double b_hm = b_hm0, b_hchi = b_hchi0;
+    qfix_rev(P, q0, gz, bh, b_hu0, b_hv0, b_hm, b_p0, b_hchi, 0.0, 0.0, 0.0, 0.0, 0.0, b_gz,
+             pa);
+    lx[0] = bh;
+    lx[1] = b_hu0;
+    lx[2] = b_hv0;
+    lx[3] = b_hm;
+    lx[4] = b_p0;
+    lx[5] = b_hchi;
+    pa.grav += b_gz;
+}
+
+
 // B6: gather the adjoint of the prepared fields, then b4step2's pointwise part.
 DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
 {
@@ -2094,10 +2176,14 @@ DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
         for (int m = 0; m < 6; m++) lx[m] = 0.0;
         lx[6] = q7[6] > 0.0 ? lp[PC_BDIF] : 0.0;
     } else {
-        PrepF f{bed(A, i, j)};
         // bdif enters the prepared fields only as max(bdif, 0)
-        vjpk<7, NPC>(A, f, q7, lp, lx, g, 6);
+        ParAdj pa;
+        pa_zero(pa);
+        prepare_vjp(A.P, q7, lp, lx, pa);
         lx[6] = q7[6] > 0.0 ? lp[PC_BDIF] : 0.0;
+#pragma unroll
+        for (int p = 0; p < MAXP; p++)
+            if (p < A.np) g[p] = pa_dot(pa, A.dP[p]);
         gp_add(A, n, g);
     }
     for (int m = 0; m < 7; m++) ws(A, W_LQ7 + m, n, lx[m]);

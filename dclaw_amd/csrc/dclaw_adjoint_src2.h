@@ -1,5 +1,7 @@
 // dclaw_adjoint_src2.h -- the reverse (vector-Jacobian product) of the src2
-// cell body written out, for the adjoint step's src2 stage.
+// cell body written out, for the adjoint step's src2 stage, and the reverses
+// of qfix, qfix_cmass and setvars it is built from (the prepare stage's
+// reverse, prepare_vjp in dclaw_adjoint.h, uses them too).
 //
 // src2_cell (dclaw_physics_body.inc; src2.f90:100-300) with the relaxation
 // source (mp_update_relax_Dclaw4, mp_update.f90:710-789; src2method -1, 0,
@@ -20,13 +22,13 @@ namespace dclaw_adj {
 // Adjoints of the parameter fields the cell physics reads.
 struct ParAdj {
     double grav, rho_s, rho_f, mu, phi, kref, mref, delta_kr_order, alpha_c, sigma_0, delta,
-        m_crit, c1;
+        m_crit, c1, speed_limit;
 };
 
 DC_HD void pa_zero(ParAdj& a)
 {
     a.grav = a.rho_s = a.rho_f = a.mu = a.phi = a.kref = a.mref = a.delta_kr_order = 0.0;
-    a.alpha_c = a.sigma_0 = a.delta = a.m_crit = a.c1 = 0.0;
+    a.alpha_c = a.sigma_0 = a.delta = a.m_crit = a.c1 = a.speed_limit = 0.0;
 }
 
 // ParAdj . dP: the directional derivative along a parameter direction.
@@ -45,6 +47,7 @@ DC_HD double pa_dot(const ParAdj& a, const dclaw_params& d)
     s = s + a.delta * d.delta;
     s = s + a.m_crit * d.m_crit;
     s = s + a.c1 * d.c1;
+    s = s + a.speed_limit * d.speed_limit;
     return s;
 }
 

@@ -768,6 +768,89 @@ extern "C" double emu_src2_vjp_check(const dclaw_params* P, int n, unsigned seed
     return std::max(wi, wp);
 }
 
+// The written-out prepare reverse (prepare_vjp) against the seeded product
+// vjp<7, NPC>(PrepF) on n random wet cells (speeds above and below the
+// speed limit, m, chi and pb inside and outside their bounds), for the six
+// state inputs and 14 parameter fields; worst2 as emu_src2_vjp_check.
+extern "C" double emu_prepare_vjp_check(const dclaw_params* P, int n, unsigned seed,
+                                        double* worst2)
+{
+    namespace D = dclaw_adj;
+    unsigned long long st = seed * 2862933555777941757ull + 3037000493ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)((st >> 11) & ((1ull << 53) - 1)) / (double)(1ull << 53) * 2.0 - 1.0;
+    };
+    constexpr int NP = 14;
+    dclaw_params dP[4];
+    const double pv[NP] = {P->grav, P->rho_s, P->rho_f, P->mu, P->phi, P->kref, P->mref,
+                           P->delta_kr_order, P->alpha_c, P->sigma_0, P->delta, P->m_crit,
+                           P->c1, P->speed_limit};
+    D::AdjArgs A{};
+    A.P = *P;
+    A.dP = dP;
+    const D::TP Pz = D::zero_params(*P);
+    double wi = 0.0, wp = 0.0;
+    for (int c = 0; c < n; c++) {
+        double x[7], ly[D::NPC], la[7], lb[7];
+        const double h = 0.01 + 1.5 * (rnd() + 1.0);
+        x[0] = h;
+        const double sp = (c % 4 == 0) ? 30.0 : 5.0;
+        x[1] = (c % 9 == 0) ? 0.0 : sp * rnd() * h;
+        x[2] = (c % 9 == 0) ? 0.0 : sp * rnd() * h;
+        double m = 0.5 + 0.25 * rnd();
+        if (c % 11 == 3) m = 1.0 + 0.05 * (rnd() + 1.0);
+        if (c % 13 == 5) m = -0.05 * (rnd() + 1.0);
+        x[3] = m * h;
+        const double rho = P->rho_s * m + (1.0 - m) * P->rho_f;
+        x[4] = (0.6 + 0.5 * rnd()) * rho * P->grav * h;
+        if (c % 7 == 2) x[4] = -0.1 * rho * P->grav * h;
+        x[5] = (0.5 + 0.6 * rnd()) * h;
+        x[6] = 0.1 * rnd();
+        for (int k = 0; k < D::NPC; k++) ly[k] = rnd();
+        D::PrepF f{0.3 * rnd()};
+        A.np = 0;
+        D::vjp<7, D::NPC>(A, f, Pz, x, ly, la, nullptr);
+        D::ParAdj pa;
+        D::pa_zero(pa);
+        D::prepare_vjp(*P, x, ly, lb, pa);
+        double scale = 0.0, diff = 0.0;
+        for (int k = 0; k < 6; k++) {
+            scale = std::max(scale, std::fabs(la[k]));
+            diff = std::max(diff, std::fabs(la[k] - lb[k]));
+        }
+        wi = std::max(wi, scale > 0.0 ? diff / scale : diff);
+        double ga[NP], gb[NP];
+        for (int p0 = 0; p0 < NP; p0 += 4) {
+            const int np = std::min(4, NP - p0);
+            for (int k = 0; k < np; k++) {
+                std::memset(&dP[k], 0, sizeof(dclaw_params));
+                double* fld[NP] = {&dP[k].grav, &dP[k].rho_s, &dP[k].rho_f, &dP[k].mu,
+                                   &dP[k].phi, &dP[k].kref, &dP[k].mref, &dP[k].delta_kr_order,
+                                   &dP[k].alpha_c, &dP[k].sigma_0, &dP[k].delta,
+                                   &dP[k].m_crit, &dP[k].c1, &dP[k].speed_limit};
+                *fld[p0 + k] = 1.0;
+            }
+            A.np = np;
+            double g4[4] = {0, 0, 0, 0}, tmp[7];
+            D::vjp<7, D::NPC>(A, f, Pz, x, ly, tmp, g4);
+            for (int k = 0; k < np; k++) {
+                ga[p0 + k] = g4[k];
+                gb[p0 + k] = D::pa_dot(pa, dP[k]);
+            }
+        }
+        double gs = 0.0;
+        for (int k = 0; k < NP; k++) gs = std::max(gs, std::fabs(ga[k] * pv[k]));
+        for (int k = 0; k < NP; k++)
+            if (gs > 0.0) wp = std::max(wp, std::fabs((ga[k] - gb[k]) * pv[k]) / gs);
+    }
+    if (worst2) {
+        worst2[0] = wi;
+        worst2[1] = wp;
+    }
+    return std::max(wi, wp);
+}
+
 // The hand-written rpn2 adjoint (rpn2_vjp) against the seeded tangent-linear
 // vector-Jacobian product (30 side inputs and the grav, rho_f, kappa,
 // beta_seg directions) on n random edges of 8 kinds, c % 8: both sides wet

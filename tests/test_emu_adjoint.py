@@ -297,3 +297,22 @@ def test_src2_parameters_transpose(emu_lib, oracle_mod, variant):
                   seed=2)
     last = "delta_kr_order" if variant.get("segregation") else "sigma_0"
     _dot_identity(emu_lib, pt, q, aux, dt, (1, 1, 3, 3), ["c1", "delta", "mref", last], seed=3)
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(alphamethod=1), dict(segregation=1),
+                                     dict(kref=1.0e-4, mu=0.001, speed_limit=8.0)])
+def test_hand_prepare_adjoint_equals_seeded(emu_lib, variant):
+    """The written-out reverse of the prepare stage (b4step2's qfix and speed
+    limit, rpn2's two qfix passes, setvars of both) equals the seeded
+    vector-Jacobian product of PrepF on random wet cells, speeds above and
+    below the limit: state adjoints and the 14 parameter sensitivities to
+    1e-11 of the largest."""
+    from dclaw_amd import workloads as W
+    p = W.config0_params()
+    for k, v in variant.items():
+        setattr(p, k, v)
+    c = to_c(p)
+    worst = (ctypes.c_double * 2)()
+    emu_lib.emu_prepare_vjp_check.restype = ctypes.c_double
+    emu_lib.emu_prepare_vjp_check(ctypes.byref(c), 3000, 23, worst)
+    assert worst[0] <= 1e-11 and worst[1] <= 1e-11, list(worst)
