@@ -1459,6 +1459,14 @@ __global__ __launch_bounds__(1024) void k_adjb_prefix(AdjBatch B)
         B.cell_prefix[B.npatch] = carry_c;
         B.tile_prefix[B.npatch] = carry_t;
     }
+    // gcount[1]: whether some member's src2 reverse takes the seeded products
+    __shared__ int seeded;
+    if (tid == 0) seeded = 0;
+    __syncthreads();
+    for (int m = tid; m < B.npatch; m += 1024)
+        if (!dclaw_adj::src2_vjp_covers(B.params[B.pt[m].member])) seeded = 1;
+    __syncthreads();
+    if (tid == 0) B.gcount[1] = seeded;
 }
 
 __global__ __launch_bounds__(ADJ_BLOCK) void k_adjb_dense(AdjBatch B, long long ncells)
@@ -1535,6 +1543,8 @@ __global__ __launch_bounds__(64) void k_adjb_list(AdjBatch B, long long ntiles)
 template <int STAGE>
 __global__ __launch_bounds__(64) void k_adjb(AdjBatch B)
 {
+    if constexpr (STAGE == ADJ_REV_SRC2_SEEDED)
+        if (B.gcount[1] == 0) return;
     int k, end, step;
     adj_span(*B.gcount, k, end, step);
     for (; k < end; k += step) {
@@ -2022,7 +2032,7 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     B.dt = dt_dev;
     B.cell_prefix = (long long*)batch_work;
     B.tile_prefix = B.cell_prefix + (npatch + 1);
-    B.gcount = (int*)(B.tile_prefix + (npatch + 1));
+    B.gcount = (int*)(B.tile_prefix + (npatch + 1));   // [0] active tiles, [1] see k_adjb_prefix
     B.glist = (int*)(batch_work + 2 * (npatch + 1) + 1);
     B.bcount = B.glist + ntiles;
     const long long nblk = (ntiles + 63) / 64;
@@ -2042,13 +2052,13 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     hipLaunchKernelGGL(k_adjb<ADJ_REC_FLUX>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_SPLIT>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_UPDATE>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb<ADJ_REV_SRC2>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb2<ADJ_REV_SRC2>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_SRC2_SEEDED>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb2<ADJ_REV_SPLIT>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_FLUX>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_WAVES>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb2<ADJ_REV_TAUDIR>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb<ADJ_REV_PREPARE>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb2<ADJ_REV_PREPARE>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_BC>, tg, tb, 0, st, B);
     if (np > 0) {
         hipLaunchKernelGGL(k_adjb_gp, dim3(ADJ_GP_PARTS, np, npatch), dim3(256), 0, st, B);

@@ -333,22 +333,29 @@ def test_sparse_tiles_and_workspace_reuse(dev, emu_lib):
                                                           lam[6][far[1:]], 0.0))
 
 
-def test_batched_adjoint_equals_per_patch(dev):
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batched_adjoint_equals_per_patch(dev, mixed):
     """Six members of uneven sizes, each with its own parameters and dt: the
     batched adjoint step (one launch per pass) equals the per-member
     single-patch adjoint steps bit for bit (lambda_n and the phi, kref
-    gradients), on two successive states with the same workspaces."""
+    gradients), on two successive states with the same workspaces.  mixed:
+    two members with Manning friction, whose src2 reverse takes the seeded
+    products while the others take the written-out one."""
     from dclaw_amd import adjoint as AD
     from dclaw_amd import step as S
     from dclaw_amd import workloads as W
     torch = dev
     members = W.ensemble(6, 60, 200, seed=4)
+    if mixed:
+        for k in (1, 4):
+            members[k].patch.params.friction_forcing = 1
+            members[k].patch.params.manning_coefficient = 0.06
     ens = S.Ensemble(members, seg_rows=16)
     ens.run(8, 1e-3, 0.9, 1e9)
     adj = AD.EnsembleAdjoint(ens, parameters=("phi", "kref"))
     g = torch.Generator(device="cuda").manual_seed(1)
     for call in range(2):
-        print(f"[adjoint batched] call {call}", flush=True)
+        print(f"[adjoint batched] mixed {mixed} call {call}", flush=True)
         q = [ens.current(k).clone() for k in range(ens.n)]
         lam = [torch.randn(x.shape, dtype=torch.float64, device="cuda", generator=g) for x in q]
         lb = [torch.zeros_like(x) for x in q]
