@@ -100,7 +100,7 @@ enum {
     W_GP = W_LYS + 2 * NSD,   // parameter gradient per cell (MAXP)
     W_FIELDS = W_GP + MAXP,
     // aliases
-    W_LTD = W_TD,             // adjoint of the taudir inputs (NTI = 3 + 2 NW planes)
+    W_LTD = W_TD,             // adjoint of the taudir inputs that enter it (NLT planes)
     W_LXE = W_XE,             // adjoint of the x-edge flux inputs (NEI)
     W_LYE = W_XE + NEI,       // y-edges (NEI; 2 NEI <= 2 NEF)
     W_LXA = W_G,              // adjoint of amdq of x-edge i+1, apdq of x-edge i (12)
@@ -108,7 +108,12 @@ enum {
     W_LQ7 = W_G,              // adjoint of the loaded q of the cell (7)
     W_LS = W_QN               // adjoint of the split inputs of the cell state (6)
 };
-static_assert(NTI == 3 + 2 * NW, "taudir adjoint must fit the taudir and wave planes");
+// The taudir inputs with a derivative (taudir_vjp): depth and bdif of each of
+// the five cells, the centre's hu, hv, rho1 and tau1.
+enum { LT_HU = 10, LT_HV = 11, LT_RHO = 12, LT_TAU = 13, NLT = 14 };
+DC_HD int lt_h(int s) { return 2 * s; }
+DC_HD int lt_bdif(int s) { return 2 * s + 1; }
+static_assert(NLT <= 3 + 2 * NW, "taudir adjoint must fit the taudir and wave planes");
 static_assert(2 * NEI <= 2 * NEF, "flux adjoint must fit the flux planes");
 
 struct AdjArgs {
@@ -430,6 +435,27 @@ DC_HD void taudir_inputs(const AdjArgs& A, int i, int j, double* x, TaudirF& f)
         f.b[s] = bed(A, i + di[s], j + dj[s]);
     }
     const size_t n = cix(A, i, j);
+    x[35] = wv(A, W_PC + PC_RHO1, n);
+    x[36] = wv(A, W_PC + PC_TAU1, n);
+    f.dx = A.dx;
+    f.dy = A.dy;
+}
+
+// The inputs taudir_vjp reads (depths, bdifs, the centre's hu, hv, rho1,
+// tau1; the others 0).
+DC_HD void taudir_inputs_live(const AdjArgs& A, int i, int j, double* x, TaudirF& f)
+{
+    const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
+    for (int k = 0; k < NTI; k++) x[k] = 0.0;
+    for (int s = 0; s < 5; s++) {
+        const size_t n = cix(A, i + di[s], j + dj[s]);
+        x[7 * s + 0] = wv(A, W_PC + PC_H, n);
+        x[7 * s + 6] = wv(A, W_PC + PC_BDIF, n);
+        f.b[s] = bed(A, i + di[s], j + dj[s]);
+    }
+    const size_t n = cix(A, i, j);
+    x[1] = wv(A, W_PC + PC_HU, n);
+    x[2] = wv(A, W_PC + PC_HV, n);
     x[35] = wv(A, W_PC + PC_RHO1, n);
     x[36] = wv(A, W_PC + PC_TAU1, n);
     f.dx = A.dx;
@@ -1977,6 +2003,165 @@ DC_HD double lxs(const AdjArgs& A, int f0, int i, int j, int k)
     return in_grid(A, i, j) ? wv(A, f0 + k, cix(A, i, j)) : 0.0;
 }
 
+// calc_taudir's reverse written out (TaudirF: calc_taudir_cell on the five
+// Q0 states, digclaw_module.f90:546-713).  Of its 37 inputs only the depths
+// and bdif of the five cells, the centre's hu, hv and its rho1, tau1 enter
+// the outputs (the qfix passes leave h, hu, hv as they are on a wet cell and
+// zero them on a dry one); the rest get 0.  lx: adjoints of the 37 inputs;
+// b_grav: the adjoint of grav (added).
+DC_HD void taudir_vjp(const dclaw_params& P, double dx, double dy, const double* bed5,
+                      const double* x, const double* ly, double* lx, double& b_grav)
+{
+    for (int k = 0; k < NTI; k++) lx[k] = 0.0;
+    const double gz = P.grav, drytol = P.dry_tolerance;
+    double hs[5], e0[5];
+    bool wet[5];
+    for (int s = 0; s < 5; s++) {
+        wet[s] = x[7 * s] > drytol;
+        hs[s] = wet[s] ? x[7 * s] : 0.0;
+        e0[s] = hs[s] + (bed5[s] - x[7 * s + 6]);
+    }
+    const double h = hs[0];
+    const double hu = wet[0] ? x[1] : 0.0, hv = wet[0] ? x[2] : 0.0;
+    // neighbours' surfaces, limited to the centre's where they are dry
+    double en[5];
+    bool nlim[5];
+    for (int s = 1; s < 5; s++) {
+        en[s] = e0[s];
+        nlim[s] = false;
+        if (hs[s] < drytol) {
+            nlim[s] = !(en[s] < e0[0]);   // dmin picks the centre's surface
+            en[s] = nlim[s] ? e0[0] : en[s];
+        }
+    }
+    // the centre's surface, limited by the left then the bottom one when dry
+    double eta = e0[0];
+    bool c1 = false, c2 = false;
+    if (h < drytol) {
+        c1 = en[1] < eta;
+        eta = c1 ? en[1] : eta;
+        c2 = en[3] < eta;
+        eta = c2 ? en[3] : eta;
+    }
+    if ((h + hs[1] + hs[3] + hs[2] + hs[4]) < drytol) return;
+    const double A = gz * 0.5;
+    const double FxL = -(A * (h + hs[1]) * (eta - en[1]) / dx);
+    const double FyL = -(A * (h + hs[3]) * (eta - en[3]) / dy);
+    const double FxR = -(A * (h + hs[2]) * (en[2] - eta) / dx);
+    const double FyR = -(A * (h + hs[4]) * (en[4] - eta) / dy);
+    const bool fx_on = FxL * FxR > 0.0, fy_on = FyL * FyR > 0.0;
+    const bool fx_l = fabs(FxL) < fabs(FxR), fy_l = fabs(FyL) < fabs(FyR);
+    const double Fx = fx_on ? (fx_l ? FxL : FxR) : 0.0;
+    const double Fy = fy_on ? (fy_l ? FyL : FyR) : 0.0;
+    double bFx = 0.0, bFy = 0.0, bFxL = 0.0, bFyL = 0.0, bhu = 0.0, bhv = 0.0;
+    const double hvn = sqrt(hu * hu + hv * hv);
+    if (hvn > 0.0) {
+        double bhvn = 0.0;
+        const double tx = -(dx * hu / hvn), ty = -(dy * hv / hvn);
+        bhu -= ly[1] * dx / hvn;
+        bhvn -= ly[1] * tx / hvn;
+        bhv -= ly[2] * dy / hvn;
+        bhvn -= ly[2] * ty / hvn;
+        const double a1p = Fx * hu, a2p = Fy * hv;
+        const double a1 = 0.0 > a1p ? 0.0 : a1p, a2 = 0.0 > a2p ? 0.0 : a2p;
+        const double dot = a1 < a2 ? a1 : a2;
+        if (dot > 0.0) {
+            const double Fproj = dot / hvn;
+            const double tau = x[36], rho = x[35];
+            const double tmax = tau > 1e-16 ? tau : 1e-16;
+            const double X = Fproj * rho / tmax;
+            if (!(1.0 < X)) {
+                const double bX = ly[0];
+                const double bFproj = bX * rho / tmax;
+                lx[35] = bX * Fproj / tmax;
+                if (tau > 1e-16) lx[36] = -bX * X / tmax;
+                const double bdot = bFproj / hvn;
+                bhvn -= bFproj * Fproj / hvn;
+                double ba1 = 0.0, ba2 = 0.0;
+                if (a1 < a2) ba1 = bdot;
+                else ba2 = bdot;
+                if (!(0.0 > a1p)) {
+                    bFx += ba1 * hu;
+                    bhu += ba1 * Fx;
+                }
+                if (!(0.0 > a2p)) {
+                    bFy += ba2 * hv;
+                    bhv += ba2 * Fy;
+                }
+            }
+        }
+        bhu += bhvn * hu / hvn;
+        bhv += bhvn * hv / hvn;
+    } else {
+        const double S1 = FxL * FxL + Fy * Fy, S2 = Fx * Fx + FyL * FyL;
+        if (S1 > 0.0) {
+            const double r1 = sqrt(S1);
+            bFxL -= ly[1] * dx / r1;
+            const double br1 = ly[1] * dx * FxL / (r1 * r1);
+            bFxL += br1 * FxL / r1;
+            bFy += br1 * Fy / r1;
+        }
+        if (S2 > 0.0) {
+            const double r2 = sqrt(S2);
+            bFyL -= ly[2] * dy / r2;
+            const double br2 = ly[2] * dy * FyL / (r2 * r2);
+            bFx += br2 * Fx / r2;
+            bFyL += br2 * FyL / r2;
+        }
+    }
+    double bFxR = 0.0, bFyR = 0.0;
+    if (fx_on) {
+        if (fx_l) bFxL += bFx;
+        else bFxR += bFx;
+    }
+    if (fy_on) {
+        if (fy_l) bFyL += bFy;
+        else bFyR += bFy;
+    }
+    // F = -(A (h + h_n) d_eta / d); A = grav / 2
+    double bh[5] = {0, 0, 0, 0, 0}, ben[5] = {0, 0, 0, 0, 0}, beta = 0.0;
+    {
+        const double sL = A * (h + hs[1]) / dx, dL = eta - en[1];
+        bh[0] -= bFxL * A * dL / dx; bh[1] -= bFxL * A * dL / dx;
+        beta -= bFxL * sL; ben[1] += bFxL * sL;
+        b_grav -= bFxL * 0.5 * (h + hs[1]) * dL / dx;
+        const double sB = A * (h + hs[3]) / dy, dB = eta - en[3];
+        bh[0] -= bFyL * A * dB / dy; bh[3] -= bFyL * A * dB / dy;
+        beta -= bFyL * sB; ben[3] += bFyL * sB;
+        b_grav -= bFyL * 0.5 * (h + hs[3]) * dB / dy;
+        const double sR = A * (h + hs[2]) / dx, dR = en[2] - eta;
+        bh[0] -= bFxR * A * dR / dx; bh[2] -= bFxR * A * dR / dx;
+        ben[2] -= bFxR * sR; beta += bFxR * sR;
+        b_grav -= bFxR * 0.5 * (h + hs[2]) * dR / dx;
+        const double sT = A * (h + hs[4]) / dy, dT = en[4] - eta;
+        bh[0] -= bFyR * A * dT / dy; bh[4] -= bFyR * A * dT / dy;
+        ben[4] -= bFyR * sT; beta += bFyR * sT;
+        b_grav -= bFyR * 0.5 * (h + hs[4]) * dT / dy;
+    }
+    // the centre's limits (bottom, then left), then the neighbours'
+    double be0[5] = {0, 0, 0, 0, 0};
+    if (h < drytol) {
+        if (c2) ben[3] += beta;
+        else if (c1) ben[1] += beta;
+        else be0[0] += beta;
+    } else {
+        be0[0] += beta;
+    }
+    for (int s = 1; s < 5; s++) {
+        if (nlim[s]) be0[0] += ben[s];
+        else be0[s] += ben[s];
+    }
+    for (int s = 0; s < 5; s++) {
+        // e0 = h + (bed - bdif); the depth's derivative passes on a wet cell
+        if (wet[s]) lx[7 * s] = bh[s] + be0[s];
+        lx[7 * s + 6] = -be0[s];
+    }
+    if (wet[0]) {
+        lx[1] = bhu;
+        lx[2] = bhv;
+    }
+}
+
 // B5: calc_taudir of the cell.
 DC_HD void rev_taudir(const AdjArgs& A, const TP& Pz, int i, int j)
 {
@@ -1993,18 +2178,29 @@ DC_HD void rev_taudir(const AdjArgs& A, const TP& Pz, int i, int j)
         ly[2] = lxs(A, W_LYS, i, j, NSD + SD_TAUDIR) + lxs(A, W_LYS, i, j + 1, SD_TAUDIR);
         double x[NTI], g[MAXP] = {0, 0, 0, 0};
         TaudirF f;
-        taudir_inputs(A, i, j, x, f);
+        taudir_inputs_live(A, i, j, x, f);
         // five dry states: calc_taudir returns zeros (its h-sum test, with
         // dry_tolerance > 0)
         const double dtol = A.P.dry_tolerance;
         const bool dry5 = dtol > 0.0 && x[0] <= dtol && x[7] <= dtol && x[14] <= dtol &&
                           x[21] <= dtol && x[28] <= dtol;
         if (!dry5) {
-            vjpk<NTI, 3>(A, f, x, ly, lx, g);
+            double b_grav = 0.0;
+            taudir_vjp(A.P, A.dx, A.dy, f.b, x, ly, lx, b_grav);
+#pragma unroll
+            for (int p = 0; p < MAXP; p++)
+                if (p < A.np) g[p] = b_grav * A.dP[p].grav;
             gp_add(A, n, g);
         }
     }
-    for (int k = 0; k < NTI; k++) ws(A, W_LTD + k, n, lx[k]);
+    for (int s5 = 0; s5 < 5; s5++) {
+        ws(A, W_LTD + lt_h(s5), n, lx[7 * s5]);
+        ws(A, W_LTD + lt_bdif(s5), n, lx[7 * s5 + 6]);
+    }
+    ws(A, W_LTD + LT_HU, n, lx[1]);
+    ws(A, W_LTD + LT_HV, n, lx[2]);
+    ws(A, W_LTD + LT_RHO, n, lx[35]);
+    ws(A, W_LTD + LT_TAU, n, lx[36]);
 }
 
 // Add a side's adjoint to the prepared fields it was read from.
@@ -2157,17 +2353,14 @@ DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
         for (int s = 0; s < 5; s++) {
             if (!in_grid(A, ci[s], cj[s])) continue;
             const size_t c = cix(A, ci[s], cj[s]);
-            const int o = 7 * s;
-            lp[PC_H] += wv(A, W_LTD + o + 0, c);
-            lp[PC_HU] += wv(A, W_LTD + o + 1, c);
-            lp[PC_HV] += wv(A, W_LTD + o + 2, c);
-            lp[PC_HM0] += wv(A, W_LTD + o + 3, c);
-            lp[PC_P0] += wv(A, W_LTD + o + 4, c);
-            lp[PC_HCHI0] += wv(A, W_LTD + o + 5, c);
-            lp[PC_BDIF] += wv(A, W_LTD + o + 6, c);
+            lp[PC_H] += wv(A, W_LTD + lt_h(s), c);
+            lp[PC_BDIF] += wv(A, W_LTD + lt_bdif(s), c);
         }
-        lp[PC_RHO1] += wv(A, W_LTD + 35, n);
-        lp[PC_TAU1] += wv(A, W_LTD + 36, n);
+        // the other inputs of calc_taudir have no derivative (taudir_vjp)
+        lp[PC_HU] += wv(A, W_LTD + LT_HU, n);
+        lp[PC_HV] += wv(A, W_LTD + LT_HV, n);
+        lp[PC_RHO1] += wv(A, W_LTD + LT_RHO, n);
+        lp[PC_TAU1] += wv(A, W_LTD + LT_TAU, n);
     }
     double q7[7], fu, fv, lx[7], g[MAXP] = {0, 0, 0, 0};
     load_q(A, i, j, q7, fu, fv);

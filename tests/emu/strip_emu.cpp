@@ -851,6 +851,75 @@ extern "C" double emu_prepare_vjp_check(const dclaw_params* P, int n, unsigned s
     return std::max(wi, wp);
 }
 
+// The written-out calc_taudir reverse (taudir_vjp) against the seeded
+// product vjp<NTI, 3>(TaudirF) on n random five-cell stencils (kind c % 4:
+// all wet and moving, a still centre, a dry centre, dry neighbours; beds,
+// surfaces and flows random), the 37 inputs and grav.  Returns the largest
+// difference relative to the output adjoints' size times the outputs' own
+// scale (|ly| (1 + dx + dy): many entries are 0 in exact arithmetic and
+// rounding residue in both forms), or to the largest input adjoint if
+// larger; grav as a sensitivity |dL/dg| g against the same scale or the
+// largest |dL/dx| |x|.
+extern "C" double emu_taudir_vjp_check(const dclaw_params* P, int n, unsigned seed)
+{
+    namespace D = dclaw_adj;
+    unsigned long long st = seed * 2862933555777941757ull + 3037000493ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)((st >> 11) & ((1ull << 53) - 1)) / (double)(1ull << 53) * 2.0 - 1.0;
+    };
+    dclaw_params dP[1];
+    std::memset(&dP[0], 0, sizeof(dclaw_params));
+    dP[0].grav = 1.0;
+    D::AdjArgs A{};
+    A.P = *P;
+    A.dP = dP;
+    const D::TP Pz = D::zero_params(*P);
+    double worst = 0.0;
+    for (int c = 0; c < n; c++) {
+        const int kind = c % 4;
+        double x[D::NTI], ly[3], la[D::NTI], lb[D::NTI];
+        D::TaudirF f;
+        f.dx = 0.5 + (rnd() + 1.0);
+        f.dy = 0.5 + (rnd() + 1.0);
+        for (int s = 0; s < 5; s++) {
+            f.b[s] = 0.5 * rnd();
+            double h = 0.05 + (rnd() + 1.0);
+            if ((kind == 2 && s == 0) || (kind == 3 && s > 0 && rnd() > 0.0)) h = 0.5e-3 * (rnd() + 1.0);
+            x[7 * s] = h;
+            x[7 * s + 1] = (kind == 1 && s == 0) ? 0.0 : 2.0 * rnd() * h;
+            x[7 * s + 2] = (kind == 1 && s == 0) ? 0.0 : 2.0 * rnd() * h;
+            x[7 * s + 3] = 0.6 * h;
+            x[7 * s + 4] = 0.8 * 1500.0 * P->grav * h;
+            x[7 * s + 5] = 0.3 * h;
+            x[7 * s + 6] = 0.05 * rnd();
+        }
+        x[35] = 1500.0 + 500.0 * rnd();
+        x[36] = 5000.0 * (rnd() + 1.1);
+        for (int k = 0; k < 3; k++) ly[k] = rnd();
+        A.np = 1;
+        double ga[1] = {0.0};
+        D::vjp<D::NTI, 3>(A, f, Pz, x, ly, la, ga);
+        double gb = 0.0;
+        D::taudir_vjp(*P, f.dx, f.dy, f.b, x, ly, lb, gb);
+        // scale: the output adjoints times the outputs' own scale (tx, ty ~
+        // dx, dy; fsphi ~ 1) per unit of input (depths ~ 1)
+        double lmax = 0.0;
+        for (int k = 0; k < 3; k++) lmax = std::max(lmax, std::fabs(ly[k]));
+        double scale = lmax * (1.0 + f.dx + f.dy), diff = 0.0;
+        double sens = std::fabs(ga[0] * P->grav);
+        for (int k = 0; k < D::NTI; k++) {
+            scale = std::max(scale, std::fabs(la[k]));
+            diff = std::max(diff, std::fabs(la[k] - lb[k]));
+            sens = std::max(sens, std::fabs(la[k] * x[k]));
+        }
+        sens = std::max(sens, lmax * (1.0 + f.dx + f.dy));
+        worst = std::max(worst, diff / scale);
+        worst = std::max(worst, std::fabs((ga[0] - gb) * P->grav) / sens);
+    }
+    return worst;
+}
+
 // The hand-written rpn2 adjoint (rpn2_vjp) against the seeded tangent-linear
 // vector-Jacobian product (30 side inputs and the grav, rho_f, kappa,
 // beta_seg directions) on n random edges of 8 kinds, c % 8: both sides wet

@@ -316,3 +316,17 @@ def test_hand_prepare_adjoint_equals_seeded(emu_lib, variant):
     emu_lib.emu_prepare_vjp_check.restype = ctypes.c_double
     emu_lib.emu_prepare_vjp_check(ctypes.byref(c), 3000, 23, worst)
     assert worst[0] <= 1e-11 and worst[1] <= 1e-11, list(worst)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_hand_taudir_adjoint_equals_seeded(emu_lib, seed):
+    """The written-out reverse of calc_taudir (five depths and bdifs, the
+    centre's momentum, rho1, tau1 and grav) equals the seeded
+    vector-Jacobian product of TaudirF on random five-cell stencils: all wet
+    and moving, a still centre (the driving-force direction), a dry centre
+    (its surface limited by the left and bottom ones), dry neighbours."""
+    from dclaw_amd import workloads as W
+    c = to_c(W.config0_params())
+    emu_lib.emu_taudir_vjp_check.restype = ctypes.c_double
+    err = emu_lib.emu_taudir_vjp_check(ctypes.byref(c), 20000, seed)
+    assert err <= 1e-11, err
