@@ -48,6 +48,9 @@ namespace dclaw_adj {
 
 using dclaw_tl::Dual;
 using TP = dclaw_tl::Params;
+// the step's own (value-only) bodies, for the record pass over double
+using dclaw::rho_tau;
+using dclaw::src2_cell;
 using dclaw::aidx;
 using dclaw::bc_source;
 using dclaw::qidx;
@@ -167,6 +170,11 @@ DC_HD TP zero_params(const dclaw_params& P)
 // The record types of the namespace that evaluates a stage over R (dclaw_tl
 // for one tangent, dclaw_tlk for K).
 template <class R> struct PhysOf;
+template <> struct PhysOf<double> {
+    using RowCell = dclaw::RowCell;
+    using Cell = dclaw::Cell;
+    using RhoTau = dclaw::RhoTau;
+};
 template <> struct PhysOf<Dual> {
     using RowCell = dclaw_tl::RowCell;
     using Cell = dclaw_tl::Cell;
@@ -236,6 +244,14 @@ DC_HD void side_from(const Dual* x, double bedv, dclaw_tl::Side& s)
     s.taudir = x[SD_TAUDIR]; s.fs = x[SD_FS]; s.rho = x[SD_RHO]; s.tau = x[SD_TAU];
 }
 
+DC_HD void side_from_d(const double* x, double bedv, dclaw::Side& s)
+{
+    s.h = x[SD_H]; s.hu = x[SD_HU]; s.hv = x[SD_HV]; s.hm = x[SD_HM]; s.p = x[SD_P];
+    s.hchi = x[SD_HCHI]; s.u = x[SD_U]; s.v = x[SD_V]; s.m = x[SD_M]; s.chi = x[SD_CHI];
+    s.b = bedv - x[SD_BDIF];
+    s.taudir = x[SD_TAUDIR]; s.fs = x[SD_FS]; s.rho = x[SD_RHO]; s.tau = x[SD_TAU];
+}
+
 // F3: rpn2 of one edge from its two sides (inputs 0..14 left, 15..29 right).
 struct Rpn2F {
     int ixy;
@@ -246,6 +262,18 @@ struct Rpn2F {
         side_from(x, bedL, L);
         side_from(x + NSD, bedR, R);
         const dclaw_tl::EdgeWaves w = dclaw_tl::rpn2_edge(P, ixy, L, R);
+        for (int k = 0; k < 3; k++) y[k] = w.s[k];
+        for (int k = 0; k < 6; k++) y[3 + k] = w.w1[k];
+        for (int k = 0; k < 6; k++) y[9 + k] = w.w5[k];
+        y[15] = w.w2h;
+        y[16] = w.w2n;
+    }
+    DC_HD void operator()(const dclaw_params& P, const double* x, double* y) const
+    {
+        dclaw::Side L, R;
+        side_from_d(x, bedL, L);
+        side_from_d(x + NSD, bedR, R);
+        const dclaw::EdgeWaves w = dclaw::rpn2_edge(P, ixy, L, R);
         for (int k = 0; k < 3; k++) y[k] = w.s[k];
         for (int k = 0; k < 6; k++) y[3 + k] = w.w1[k];
         for (int k = 0; k < 6; k++) y[9 + k] = w.w5[k];
@@ -282,6 +310,18 @@ struct FluxF {
             y[18 + k] = e.apdq[k];
         }
     }
+    DC_HD void operator()(const dclaw_params& P, const double* x, double* y) const
+    {
+        dclaw::EdgeWaves w;
+        waves_from(x, w);
+        const dclaw::EdgeFlux e = dclaw::edge_flux(P, ixy, w, x + NW, x + NW + 3, gate, dtdx);
+        for (int k = 0; k < 6; k++) {
+            y[k] = e.faddm[k];
+            y[6 + k] = e.faddp[k];
+            y[12 + k] = e.amdq[k];
+            y[18 + k] = e.apdq[k];
+        }
+    }
 };
 
 // F5: one rpt2 split of a fluctuation in a cell.
@@ -293,6 +333,11 @@ struct SplitF {
     {
         dclaw_tl::rpt2_cell_split(P, ixy, x[0], x[1], x[2], x[3], x[4], Dual(bedc), x[5],
                                   Dual(topo1), Dual(topo3), x + 6, active, y, y + 6);
+    }
+    DC_HD void operator()(const dclaw_params& P, const double* x, double* y) const
+    {
+        dclaw::rpt2_cell_split(P, ixy, x[0], x[1], x[2], x[3], x[4], bedc, x[5], topo1, topo3,
+                               x + 6, active, y, y + 6);
     }
 };
 
@@ -307,7 +352,15 @@ struct Src2F {
     }
 };
 
-// Values of a stage (tangents zero).
+// Values of a stage: the step's own double arithmetic (the tangent types
+// compute these values operation for operation).
+template <int NI, int NO, class F>
+DC_HD void evald(const F& f, const dclaw_params& P, const double* x, double* y)
+{
+    f(P, x, y);
+}
+
+// Values of a stage over the tangent type (tangents zero).
 template <int NI, int NO, class F>
 DC_HD void eval(const F& f, const TP& Pz, const double* x, double* y)
 {
@@ -602,7 +655,7 @@ DC_HD void rec_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
     double q7[7], fu, fv, y[NPC];
     load_q(A, i, j, q7, fu, fv);
     PrepF f{bed(A, i, j)};
-    eval<7, NPC>(f, Pz, q7, y);
+    evald<7, NPC>(f, A.P, q7, y);
     const size_t n = cix(A, i, j);
     for (int k = 0; k < NPC; k++) ws(A, W_PC + k, n, y[k]);
 }
@@ -615,7 +668,7 @@ DC_HD void rec_taudir(const AdjArgs& A, const TP& Pz, int i, int j)
         double x[NTI];
         TaudirF f;
         taudir_inputs(A, i, j, x, f);
-        eval<NTI, 3>(f, Pz, x, y);
+        evald<NTI, 3>(f, A.P, x, y);
     }
     for (int k = 0; k < 3; k++) ws(A, W_TD + k, n, y[k]);
 }
@@ -630,7 +683,7 @@ DC_HD void rec_waves(const AdjArgs& A, const TP& Pz, int i, int j)
             double x[2 * NSD];
             Rpn2F f;
             rpn2_inputs(A, ixy, i, j, x, f);
-            eval<2 * NSD, NW>(f, Pz, x, y);
+            evald<2 * NSD, NW>(f, A.P, x, y);
         }
         const int f0 = ixy == 1 ? W_XW : W_YW;
         for (int k = 0; k < NW; k++) ws(A, f0 + k, n, y[k]);
@@ -647,7 +700,7 @@ DC_HD void rec_flux(const AdjArgs& A, const TP& Pz, int i, int j)
             double x[NEI];
             FluxF f;
             flux_inputs(A, ixy, i, j, x, f);
-            eval<NEI, NEF>(f, Pz, x, y);
+            evald<NEI, NEF>(f, A.P, x, y);
         }
         const int f0 = ixy == 1 ? W_XE : W_YE;
         for (int k = 0; k < NEF; k++) ws(A, f0 + k, n, y[k]);
@@ -662,7 +715,7 @@ DC_HD void rec_split(const AdjArgs& A, const TP& Pz, int i, int j)
         for (int k = 0; k < 12; k++) b[s][k] = 0.0;
         double x[NSI];
         SplitF f;
-        if (split_inputs(A, s, i, j, x, f)) eval<NSI, 12>(f, Pz, x, b[s]);
+        if (split_inputs(A, s, i, j, x, f)) evald<NSI, 12>(f, A.P, x, b[s]);
     }
     // G = (0 - (dt/2dx) b_imp1) - (dt/2dx) b_imp2 (step2's transverse terms)
     const double dtdx = A.dt / A.dx, dtdy = A.dt / A.dy;
@@ -705,7 +758,10 @@ DC_HD void rec_update(const AdjArgs& A, const TP& Pz, int i, int j)
     const double h_e = A.P.entrainment ? A.aux[aidx(i, j, A.mx, A.maux) + 6] : 0.0;
     Src2F f{A.dt, h_e};
     double y[7];
-    eval<7, 7>(f, Pz, q, y);
+    // src2method 2 is not differentiated: the tangent type's src2 writes NaN,
+    // and the record keeps that (the step itself integrates it)
+    if (A.P.src2method == 2) eval<7, 7>(f, Pz, q, y);
+    else evald<7, 7>(f, A.P, q, y);
     if (o)
         for (int m = 0; m < 7; m++) o[m] = y[m];
 }
