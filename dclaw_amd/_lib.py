@@ -87,6 +87,7 @@ SIGNATURES = {
                                           _ip, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "dclaw_tile_count": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dclaw_split_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "dclaw_split_resident_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dclaw_set_dry_rows": (ctypes.c_int, [ctypes.c_int]),
     "dclaw_set_split_mode": (ctypes.c_int, [ctypes.c_int]),
     "dclaw_gauges_tl_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _vp, _i,

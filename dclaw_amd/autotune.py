@@ -57,6 +57,7 @@ def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 5,
         for h in candidates:
             _lib.check(min(0, L.dclaw_set_seg_rows(h)))
             runner.qa.copy_(qa)
+            runner.state_written()
 
             def one():
                 runner.step()
@@ -66,6 +67,7 @@ def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 5,
     _lib.check(min(0, L.dclaw_set_seg_rows(best)))
     runner.qa.copy_(qa)
     runner.qb.copy_(qb)
+    runner.state_written()
     runner.dt.copy_(dt)
     runner.cfl.copy_(cfl)
     return best, times

@@ -119,13 +119,17 @@ __global__ __launch_bounds__(64) void k_stepgrid(const double* __restrict__ q_in
                                                  double* cfl_dev, int seg_rows, int nstrips,
                                                  int nseg, int bx0, int bx1, int by0, int by1,
                                                  int* tile_flags,
-                                                 const int* __restrict__ order)
+                                                 const int* __restrict__ order,
+                                                 const int* __restrict__ dry_age)
 {
     // tile order: row-major, or the permutation `order` (wet tiles of the
     // previous step first, so the long tiles start first); the dispatcher
     // deals consecutive workgroups to the XCDs round-robin, which spreads a
     // cluster of wet tiles over the whole chip
     const int T = order ? order[blockIdx.x] : (int)blockIdx.x;
+    // a tile whose neighbourhood was dry on the last three inputs already
+    // holds its result in q_out (k_dry_age); its flag is 0 and stays 0
+    if (dry_age && dry_age[T] >= 3) return;
     const int strip = T % nstrips, seg = T / nstrips;
     const dclaw_params P = *params;
     StripArgs A;
@@ -430,6 +434,64 @@ __global__ __launch_bounds__(64) void k_src2_list(double* q, const double* __res
         const int k = (int)(task % nchunk) * 64 + (int)threadIdx.x;
         if (k < STRIP_W * (jB - jA + 1)) src2_tile_cell(P, dt, q, aux, maux, mx, i0, jA, jB, k);
     }
+}
+
+// Dry-tile skipping of the ordered split step (a "resident" workspace of
+// split_resident_ints(ntiles) ints: flags, order, counters, then SKIP_META
+// ints of bookkeeping and one age per tile).  The strip kernel's tile flag
+// marks a tile whose output has a cell wet before src2 (src2 only dries
+// cells), so after a step the flags cover every wet cell of the new state.
+// If no tile within the strip kernel's read halo of tile T (one strip, and
+// ceil(3 / segment rows) segments, each way) is flagged, every cell the
+// kernel reads for T is dry and it writes the dry map of its input, whose
+// repeated application changes nothing.  Three such inputs in a row (D_n,
+// D_{n-1}, D_{n-2}), with q_in and q_out swapping roles between calls,
+// leave q_out's tile (step n-1's input) equal to the result, so the tile is
+// not read or written.  meta: [0] segment rows, [1] strips, [2] segments,
+// [3..4] the previous call's q_out, [5..6] its q_in, [7] how many
+// consecutive calls (capped at 4) continued one such sequence -- the flags
+// describe this call's input only when it is at least 1.
+constexpr int SKIP_META = 8;
+inline long split_resident_ints(long ntiles) { return 4 * ntiles + 4 + SKIP_META; }
+
+__global__ void k_skip_meta(int* meta, const double* q_in, const double* q_out, int seg_rows,
+                            int nstrips, int nseg)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned long long in = (unsigned long long)q_in, out = (unsigned long long)q_out;
+    const unsigned long long last_out =
+        (unsigned long long)(unsigned)meta[3] | ((unsigned long long)(unsigned)meta[4] << 32);
+    const unsigned long long last_in =
+        (unsigned long long)(unsigned)meta[5] | ((unsigned long long)(unsigned)meta[6] << 32);
+    const bool cont = meta[0] == seg_rows && meta[1] == nstrips && meta[2] == nseg &&
+                      in == last_out && out == last_in && meta[7] >= 0;
+    meta[7] = cont ? min(meta[7] + 1, 4) : 0;
+    meta[0] = seg_rows;
+    meta[1] = nstrips;
+    meta[2] = nseg;
+    meta[3] = (int)(unsigned)(out & 0xffffffffull);
+    meta[4] = (int)(unsigned)(out >> 32);
+    meta[5] = (int)(unsigned)(in & 0xffffffffull);
+    meta[6] = (int)(unsigned)(in >> 32);
+}
+
+// Age of each tile: consecutive inputs with no flagged tile within the read
+// halo (0 when the flags do not describe this call's input).
+__global__ void k_dry_age(const int* __restrict__ flags, int* age, const int* __restrict__ meta,
+                          int nstrips, int nseg, int ry)
+{
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= nstrips * nseg) return;
+    if (meta[7] == 0) {
+        age[t] = 0;
+        return;
+    }
+    const int strip = t % nstrips, seg = t / nstrips;
+    bool dry = true;
+    for (int s = max(0, seg - ry); s <= min(nseg - 1, seg + ry); s++)
+        for (int x = max(0, strip - 1); x <= min(nstrips - 1, strip + 1); x++)
+            dry = dry && flags[s * nstrips + x] == 0;
+    age[t] = dry ? min(age[t] + 1, 3) : 0;
 }
 
 // Permutation of the n tiles with the flagged ones first (flags from the
@@ -1758,13 +1820,21 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     if (!tile_flags) {
         hipLaunchKernelGGL(k_stepgrid<MODE_STEP>, dim3(nstrips * nseg), dim3(64), 0, stream,
                            q_in, q_out, aux, aux_out, maux, mx, my, dx, dy, dt_dev, params_dev,
-                           cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, nullptr, nullptr);
+                           cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, nullptr, nullptr,
+                           nullptr);
         HIP_TRY(hipGetLastError());
         return DCLAW_OK;
     }
     const int ntiles = nstrips * nseg;
     if (n_flags < ntiles) return fail(DCLAW_EINVAL, "tile_flags holds fewer than dclaw_tile_count() ints");
     if (aux_out) return fail(DCLAW_EINVAL, "the split step does not write aux");
+    const bool resident = (long)n_flags >= split_resident_ints(ntiles);
+    if (resident && !(g_split_mode == 1 && b0 && b1 && b2 && b3)) {
+        // another mode (or stored ghosts) writes q_out without the dry-tile
+        // bookkeeping: the next resident call starts over
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, stream, tile_flags + 3 * ntiles + 4,
+                           SKIP_META);
+    }
     if ((long)n_flags >= 3L * ntiles + 4 && g_split_mode == 3) {
         // one strip launch: the previous step's wet segments singly and first,
         // the other segments in tall dry groups
@@ -1790,13 +1860,26 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
         // one strip launch with the previous step's wet tiles first
         int* order = tile_flags + ntiles;
         int* counters = tile_flags + 2 * ntiles + 2;
+        const int* dry_age = nullptr;
+        if (resident && b0 && b1 && b2 && b3) {
+            // resident workspace, boundary conditions on load: tiles that
+            // stayed dry are not read or written
+            int* meta = tile_flags + 3 * ntiles + 4;
+            int* age = meta + SKIP_META;
+            hipLaunchKernelGGL(k_skip_meta, dim3(1), dim3(64), 0, stream, meta, q_in, q_out,
+                               seg_rows, nstrips, nseg);
+            hipLaunchKernelGGL(k_dry_age, dim3((ntiles + 255) / 256), dim3(256), 0, stream,
+                               tile_flags, age, meta, nstrips, nseg,
+                               (STRIP_HALO + seg_rows - 1) / seg_rows);
+            dry_age = age;
+        }
         hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, stream, counters, 2);
         hipLaunchKernelGGL(k_order_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, stream,
                            tile_flags, ntiles, order, counters);
         hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips * nseg), dim3(64), 0,
                            stream, q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev,
                            params_dev, cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3,
-                           tile_flags, order);
+                           tile_flags, order, dry_age);
         HIP_TRY(hipGetLastError());
         const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
         hipLaunchKernelGGL(k_src2_tiles, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0,
@@ -1829,7 +1912,8 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     }
     hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips * nseg), dim3(64), 0, stream,
                        q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
-                       cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, tile_flags, nullptr);
+                       cfl_dev, seg_rows, nstrips, nseg, b0, b1, b2, b3, tile_flags, nullptr,
+                       nullptr);
     HIP_TRY(hipGetLastError());
     const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
     hipLaunchKernelGGL(k_src2_tiles, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, stream,
@@ -2092,6 +2176,14 @@ int dclaw_tile_count(int mx, int my)
     return (int)n;
 }
 
+int dclaw_split_resident_size(int mx, int my)
+{
+    const int n = dclaw_tile_count(mx, my);
+    if (n < 0) return n;
+    if (split_resident_ints(n) > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
+    return (int)split_resident_ints(n);
+}
+
 int dclaw_split_workspace_size(int mx, int my)
 {
     const int n = dclaw_tile_count(mx, my);
@@ -2234,7 +2326,7 @@ static int launch_curvature_step(const double* q_in, double* q_out, const double
     // cell's running gz
     hipLaunchKernelGGL(k_stepgrid<MODE_STEP_NOSRC2>, dim3(nstrips * nseg), dim3(64), 0, st, q_in,
                        q_out, aux, aux_out, maux, mx, my, dx, dy, dt_dev, params_dev, cfl_dev,
-                       seg_rows, nstrips, nseg, b0, b1, b2, b3, flags, nullptr);
+                       seg_rows, nstrips, nseg, b0, b1, b2, b3, flags, nullptr, nullptr);
     HIP_TRY(hipGetLastError());
     return launch_curvature_src2(q_out, aux, maux, mx, my, dx, dy, dt_dev, params_dev, work, st);
 }

@@ -258,9 +258,11 @@ class PatchRunner:
         self.dt = torch.zeros(1, dtype=torch.float64, device=device)
         self.cfl = torch.zeros(1, dtype=torch.float64, device=device)
         # split-step workspace for any segment height (one row per tile at
-        # most): tile flags, the wet list or tile order, and counters
+        # most): tile flags, the wet list or tile order, counters, and the
+        # dry-tile ages of the resident wet-first step (this loop swaps qa and
+        # qb every step and writes neither in between)
         nstrips = -(-self.mx // _lib.lib().dclaw_strip_width())
-        self.flags = torch.zeros(3 * nstrips * self.my + 4, dtype=torch.int32, device=device)
+        self.flags = torch.zeros(4 * nstrips * self.my + 12, dtype=torch.int32, device=device)
         # curvature: the step with src2's running gravity (its own workspace)
         self.curv_work = (torch.zeros(curvature_workspace_doubles(self.mx, self.my),
                                       dtype=torch.float64, device=device)
@@ -268,6 +270,12 @@ class PatchRunner:
 
     def set_dt(self, dt: float) -> None:
         self.dt.fill_(float(dt))
+
+    def state_written(self) -> None:
+        """Call after writing qa or qb directly: the split step's dry-tile
+        skipping (which assumes the state changes only through step) starts
+        over."""
+        self.flags.zero_()
 
     def step(self, fused_bc: bool = True) -> None:
         """bound + stepgrid (boundary conditions applied on load when fused_bc);
@@ -446,6 +454,7 @@ class Ensemble:
         self._patches = [self._patch_table(False, device), self._patch_table(True, device)]
         self.flip = False
         self._plain_flags = None
+        self._plain_dirty = False
 
     def _patch_table(self, flipped: bool, device) -> torch.Tensor:
         recs = []
@@ -468,6 +477,8 @@ class Ensemble:
             self.seg_rows, bc, _ptr(self.dt), _ptr(self.params), _ptr(self.cfl),
             _ptr(self.flags) if self.split_src2 else None, self.flags.numel(), _stream()))
         self.flip = not self.flip
+        # the per-member loop's dry-tile ages no longer describe the state
+        self._plain_dirty = True
 
     def step_plain(self) -> None:
         """The plain per-member loop, for comparison: one single-patch step per
@@ -480,8 +491,12 @@ class Ensemble:
         if self._plain_flags is None:
             w = _lib.lib().dclaw_strip_width()
             self._plain_flags = [
-                torch.zeros(3 * (-(-self.mx[k] // w)) * self.my[k] + 4, dtype=torch.int32,
+                torch.zeros(4 * (-(-self.mx[k] // w)) * self.my[k] + 12, dtype=torch.int32,
                             device=self.dt.device) for k in range(self.n)]
+        elif self._plain_dirty:
+            for f in self._plain_flags:
+                f.zero_()
+        self._plain_dirty = False
         for k in range(self.n):
             qi = self.current(k)
             qo = self.qa[k] if self.flip else self.qb[k]

@@ -161,6 +161,11 @@ int dclaw_tile_count(int mx, int my);
  * counterpart. */
 int dclaw_split_workspace_size(int mx, int my);
 
+/* Workspace ints (4 * dclaw_tile_count(mx, my) + 12) of the "resident"
+ * wet-first split step: see dclaw_stepgrid_split_dev.  No reference
+ * counterpart. */
+int dclaw_split_resident_size(int mx, int my);
+
 /* Mode of the split step when its workspace holds dclaw_split_workspace_size
  * ints: 1 = one strip launch with the tiles the previous call flagged first
  * (the workspace carries the flags between calls; any contents give a valid
@@ -193,7 +198,14 @@ int dclaw_set_dry_rows(int rows);
  * kernel and src2 run over that list only) or the wet-first tile order (the
  * previous call's flagged tiles dispatched first).  The rest of the
  * workspace holds the list or order and counters.  Same result in every
- * mode.  No reference counterpart (a GPU scheduling choice). */
+ * mode.  With n_flags >= dclaw_split_resident_size(mx, my), the wet-first
+ * order and boundary conditions on load (every bc code nonzero), tiles whose
+ * neighbourhood stayed dry on the last three inputs are neither read nor
+ * written: for a time loop that swaps q_in and q_out every call and changes
+ * neither in between (the library checks the swap and the segment height on
+ * the device and starts over when they break; a caller that writes q between
+ * calls zeroes the workspace).  No reference counterpart (a GPU scheduling
+ * choice). */
 int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* aux, int maux,
                              int mx, int my, double dx, double dy, const int* bc,
                              const double* dt_dev, const dclaw_params* params_dev,

@@ -419,3 +419,59 @@ def test_short_segments_match_oracle(dev, oracle_mod, mode, seg):
     assert mask_equal(qg, qo, pt.params.dry_tolerance)
     assert rel_linf_wet(qg, qo, pt.params.dry_tolerance) <= TOL_STEP
     assert abs(cs.item() - cflo) <= TOL_STEP * cflo
+
+
+@pytest.mark.parametrize("seg,bc", [(8, (1, 1, 1, 1)), (2, (3, 1, 3, 3)), (1, (1, 3, 1, 3))])
+def test_resident_split_skips_dry_tiles(dev, seg, bc):
+    """The resident wet-first split step (PatchRunner's workspace) leaves the
+    tiles whose neighbourhood stayed dry unread and unwritten, and its states
+    and dt equal the same loop's without skipping (a workspace too small for
+    the dry-tile ages) bit for bit over 160 steps, through a tuner run in the
+    middle (which rewrites the state and must restart the bookkeeping)."""
+    from dclaw_amd import _lib, autotune
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    pt = W.fractal_sources(300, 200, dx=2.0, seed=5, nsrc=6)
+    L = _lib.lib()
+    prev = L.dclaw_set_seg_rows(seg)
+    try:
+        print(f"[resident] seg {seg} bc {bc}", flush=True)
+        ra = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, bc=bc, split_mode="ordered")
+        rb = S.PatchRunner(pt.q, pt.aux, pt.params, pt.dx, pt.dy, bc=bc, split_mode="ordered")
+        ntiles = L.dclaw_tile_count(300, 200)
+        rb.flags = dev.zeros(L.dclaw_split_workspace_size(300, 200), dtype=dev.int32,
+                             device="cuda")
+        assert ra.flags.numel() >= L.dclaw_split_resident_size(300, 200)
+        for r in (ra, rb):
+            r.set_dt(1e-3)
+        for n in range(80):
+            ra.step()
+            rb.step()
+            if n == 40:
+                h, _ = autotune.tune_patch(ra, candidates=(seg, 2 * seg), reps=2)
+                L.dclaw_set_seg_rows(seg)
+                ra.split_mode = "ordered"
+        ra.run(80, ra.dt.clone(), 0.9, 1e9)
+        rb.run(80, rb.dt.clone(), 0.9, 1e9)
+        dev.cuda.synchronize()
+        assert np.array_equal(ra.q(), rb.q()), f"{(ra.q() != rb.q()).sum()} values differ"
+        assert ra.dt.item() == rb.dt.item()
+        # a dry corner tile far from every source is skipped: a marker put into
+        # the buffer the next step writes survives that step
+        age = ra.flags[3 * ntiles + 12:4 * ntiles + 12].cpu().numpy()
+        assert (age >= 3).any() and (age < 3).any()
+        nstrips = -(-300 // L.dclaw_strip_width())
+        # the skipped tile farthest from any tile that is not
+        a2 = age.reshape(-1, nstrips)
+        busy = np.argwhere(a2 < 3)
+        far = [(min(max(abs(s - bs) * seg, abs(x - bx) * 58) for bs, bx in busy), s * nstrips + x)
+               for s, x in np.argwhere(a2 >= 3)]
+        t = max(far)[1]
+        i = 1 + (t % nstrips) * L.dclaw_strip_width()
+        j = 1 + (t // nstrips) * seg
+        ra.qb[j + 1, i + 1, 6] = 12345.0
+        ra.step()
+        dev.cuda.synchronize()
+        assert ra.qa[j + 1, i + 1, 6].item() == 12345.0
+    finally:
+        L.dclaw_set_seg_rows(prev)
