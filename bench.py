@@ -177,10 +177,12 @@ def main():
         r.qa, r.qb = r.qb, r.qa
         S.dt_update(r.dt, r.cfl, cfl_desired, dt_max)
 
+    seg = args.seg_rows
+    if seg > 0:
+        _lib.lib().dclaw_set_seg_rows(seg)
     for _ in range(args.warmup):
         one_step()
     torch.cuda.synchronize()
-    seg = args.seg_rows
     tune_times = None
     if seg <= 0:
         from dclaw_amd import autotune
@@ -188,8 +190,11 @@ def main():
             r, split_modes=(("single", "ordered", "dry", "hybrid") if args.split_mode == "auto"
                             else (args.split_mode,)))
         tune_times = {f"{h}/{m}": round(v, 4) for (h, m), v in tune_times.items()}
-    else:
-        _lib.lib().dclaw_set_seg_rows(seg)
+        # the tuner restored the state, so the resident step's dry-tile
+        # bookkeeping starts over: three untimed steps rebuild it
+        for _ in range(3):
+            one_step()
+        torch.cuda.synchronize()
     print(f"[bench] warmup done, segment rows {seg}, split mode {r.split_mode} "
           f"(tuner: {tune_times})",
           file=sys.stderr,
