@@ -98,9 +98,11 @@ enum {
     W_G = W_YE + NEF,         // Gx1, Gx2, Gy1, Gy2 (24)
     W_QN = W_G + 24,          // updated state before src2 (7)
     W_LQN = W_QN + 7,         // adjoint of that state (7)
-    W_LXS = W_LQN + 7,        // adjoint of the x-edge rpn2 sides, left then right (2 NSD)
-    W_LYS = W_LXS + 2 * NSD,  // y-edges (2 NSD)
-    W_GP = W_LYS + 2 * NSD,   // parameter gradient per cell (MAXP)
+    W_LR = W_LQN + 7,         // adjoint of the cell's F1 fields from the right (top)
+                              // sides of its x- and y-edge, summed (NLR)
+    W_LXL = W_LR + 16,        // adjoint of the x-edge's left side (NSD)
+    W_LYL = W_LXL + NSD,      // adjoint of the y-edge's bottom side (NSD)
+    W_GP = W_LYL + NSD,       // parameter gradient per cell (MAXP)
     W_FIELDS = W_GP + MAXP,
     // aliases
     W_LTD = W_TD,             // adjoint of the taudir inputs that enter it (NLT planes)
@@ -111,6 +113,13 @@ enum {
     W_LQ7 = W_G,              // adjoint of the loaded q of the cell (7)
     W_LS = W_QN               // adjoint of the split inputs of the cell state (6)
 };
+// The fields of W_LR: the F1 prepared fields a right (top) side reads, then
+// the side's fsphi and the taudir of each direction.
+enum {
+    LR_H, LR_HU, LR_HV, LR_HM1, LR_P1, LR_HCHI1, LR_U, LR_V, LR_M1, LR_CHI1, LR_BDIF,
+    LR_RHO1, LR_TAU1, LR_FS, LR_TDX, LR_TDY, NLR
+};
+static_assert(NLR == 16, "W_LR holds NLR planes");
 // The taudir inputs with a derivative (taudir_vjp): depth and bdif of each of
 // the five cells, the centre's hu, hv, rho1 and tau1.
 enum { LT_HU = 10, LT_HV = 11, LT_RHO = 12, LT_TAU = 13, NLT = 14 };
@@ -676,16 +685,23 @@ DC_HD void rec_taudir(const AdjArgs& A, const TP& Pz, int i, int j)
 DC_HD void rec_waves(const AdjArgs& A, const TP& Pz, int i, int j)
 {
     const size_t n = cix(A, i, j);
-    for (int ixy = 1; ixy <= 2; ixy++) {
+    // both edges' inputs are loaded before the first store: every plane goes
+    // through one pointer, so a load after a store stays behind it
+    double x[2][2 * NSD];
+    Rpn2F f[2];
+    bool e[2];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        const int ixy = d + 1;
+        e[d] = ixy == 1 ? xedge(A, i, j) : yedge(A, i, j);
+        if (e[d]) rpn2_inputs(A, ixy, i, j, x[d], f[d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
         double y[NW];
         for (int k = 0; k < NW; k++) y[k] = 0.0;
-        if (ixy == 1 ? xedge(A, i, j) : yedge(A, i, j)) {
-            double x[2 * NSD];
-            Rpn2F f;
-            rpn2_inputs(A, ixy, i, j, x, f);
-            evald<2 * NSD, NW>(f, A.P, x, y);
-        }
-        const int f0 = ixy == 1 ? W_XW : W_YW;
+        if (e[d]) evald<2 * NSD, NW>(f[d], A.P, x[d], y);
+        const int f0 = d == 0 ? W_XW : W_YW;
         for (int k = 0; k < NW; k++) ws(A, f0 + k, n, y[k]);
     }
 }
@@ -693,16 +709,22 @@ DC_HD void rec_waves(const AdjArgs& A, const TP& Pz, int i, int j)
 DC_HD void rec_flux(const AdjArgs& A, const TP& Pz, int i, int j)
 {
     const size_t n = cix(A, i, j);
-    for (int ixy = 1; ixy <= 2; ixy++) {
+    // both edges' inputs before the first store (as rec_waves)
+    double x[2][NEI];
+    FluxF f[2];
+    bool e[2];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        const int ixy = d + 1;
+        e[d] = ixy == 1 ? xflux(A, i, j) : yflux(A, i, j);
+        if (e[d]) flux_inputs(A, ixy, i, j, x[d], f[d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
         double y[NEF];
         for (int k = 0; k < NEF; k++) y[k] = 0.0;
-        if (ixy == 1 ? xflux(A, i, j) : yflux(A, i, j)) {
-            double x[NEI];
-            FluxF f;
-            flux_inputs(A, ixy, i, j, x, f);
-            evald<NEI, NEF>(f, A.P, x, y);
-        }
-        const int f0 = ixy == 1 ? W_XE : W_YE;
+        if (e[d]) evald<NEI, NEF>(f[d], A.P, x[d], y);
+        const int f0 = d == 0 ? W_XE : W_YE;
         for (int k = 0; k < NEF; k++) ws(A, f0 + k, n, y[k]);
     }
 }
@@ -727,18 +749,20 @@ DC_HD void rec_split(const AdjArgs& A, const TP& Pz, int i, int j)
     }
 }
 
-DC_HD void rec_update(const AdjArgs& A, const TP& Pz, int i, int j)
+// The conservative update of cell (i, j) into q (the state src2 receives);
+// false for a ghost cell, whose q_out (when wanted) is its prepared state.
+DC_HD bool update_state(const AdjArgs& A, int i, int j, double* q)
 {
     const size_t n = cix(A, i, j);
-    double q[7];
     q[0] = wv(A, W_PC + PC_H, n); q[1] = wv(A, W_PC + PC_HU, n); q[2] = wv(A, W_PC + PC_HV, n);
     q[3] = wv(A, W_PC + PC_HM0, n); q[4] = wv(A, W_PC + PC_P0, n);
     q[5] = wv(A, W_PC + PC_HCHI0, n); q[6] = wv(A, W_PC + PC_BDIF, n);
-    double* o = A.q_out ? A.q_out + qidx(0, i, j, A.mx) : nullptr;
     if (!interior(A, i, j)) {
-        if (o)
+        if (A.q_out) {
+            double* o = A.q_out + qidx(0, i, j, A.mx);
             for (int m = 0; m < 7; m++) o[m] = q[m];
-        return;
+        }
+        return false;
     }
     const double dtdx = A.dt / A.dx, dtdy = A.dt / A.dy;
     const size_t nl = cix(A, i - 1, j), nr = cix(A, i + 1, j);
@@ -754,7 +778,14 @@ DC_HD void rec_update(const AdjArgs& A, const TP& Pz, int i, int j)
                                wv(A, W_YE + 6 + z, n);
         q[z] = q[z] - dtdx * (fm_next - fpc) - dtdy * (gm_next - gp_prev);
     }
-    for (int m = 0; m < 7; m++) ws(A, W_QN + m, n, q[m]);
+    return true;
+}
+
+// q_out of an interior cell: src2 of its updated state q (only when the
+// caller asked for q_out).
+DC_HD void update_out(const AdjArgs& A, const TP& Pz, int i, int j, const double* q)
+{
+    if (!A.q_out) return;
     const double h_e = A.P.entrainment ? A.aux[aidx(i, j, A.mx, A.maux) + 6] : 0.0;
     Src2F f{A.dt, h_e};
     double y[7];
@@ -762,8 +793,17 @@ DC_HD void rec_update(const AdjArgs& A, const TP& Pz, int i, int j)
     // and the record keeps that (the step itself integrates it)
     if (A.P.src2method == 2) eval<7, 7>(f, Pz, q, y);
     else evald<7, 7>(f, A.P, q, y);
-    if (o)
-        for (int m = 0; m < 7; m++) o[m] = y[m];
+    double* o = A.q_out + qidx(0, i, j, A.mx);
+    for (int m = 0; m < 7; m++) o[m] = y[m];
+}
+
+DC_HD void rec_update(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    double q[7];
+    if (!update_state(A, i, j, q)) return;
+    const size_t n = cix(A, i, j);
+    for (int m = 0; m < 7; m++) ws(A, W_QN + m, n, q[m]);
+    update_out(A, Pz, i, j, q);
 }
 
 // --------------------------------------------------------- reverse pass
@@ -792,10 +832,6 @@ DC_HD double lam_gp(const AdjArgs& A, int i, int j, int z)
     return (A.dt / A.dy) * lam_qn(A, i, j, z);
 }
 
-DC_HD void gp_zero(const AdjArgs& A, size_t n)
-{
-    for (int p = 0; p < MAXP; p++) ws(A, W_GP + p, n, 0.0);
-}
 DC_HD void gp_add(const AdjArgs& A, size_t n, const double* g)
 {
 #pragma unroll
@@ -807,16 +843,15 @@ DC_HD void gp_add(const AdjArgs& A, size_t n, const double* g)
 // the configurations src2_vjp covers (the written-out reverse) or the others
 // (seeded products); a call for the other kind does nothing, so the two run
 // as separate passes and each keeps only its own registers.
+// The reverse of src2 at cell (i, j) given its updated state x (interior
+// cells; x unused for ghost cells).
 template <bool WRITTEN>
-DC_HD void rev_src2_t(const AdjArgs& A, const TP& Pz, int i, int j)
+DC_HD void rev_src2_cell(const AdjArgs& A, int i, int j, const double* x)
 {
-    if (src2_vjp_covers(A.P) != WRITTEN) return;
     const size_t n = cix(A, i, j);
-    gp_zero(A, n);
-    double lx[7] = {0, 0, 0, 0, 0, 0, 0};
+    double lx[7] = {0, 0, 0, 0, 0, 0, 0}, g[MAXP] = {0, 0, 0, 0};
     if (interior(A, i, j)) {
-        double x[7], ly[7], g[MAXP] = {0, 0, 0, 0};
-        for (int m = 0; m < 7; m++) x[m] = wv(A, W_QN + m, n);
+        double ly[7];
         for (int m = 0; m < 7; m++) ly[m] = A.lam_out[qidx(m, i, j, A.mx)];
         if (x[0] <= A.P.dry_tolerance) {
             // src2 zeroes a dry cell and leaves bdif (src2.f90:103-111)
@@ -839,16 +874,50 @@ DC_HD void rev_src2_t(const AdjArgs& A, const TP& Pz, int i, int j)
                 vjpk<7, 7>(A, f, x, ly, lx, g, erodes ? 7 : 6);
                 if (!erodes) lx[6] = ly[6];
             }
-            gp_add(A, n, g);
         }
     }
+    // the parameter gradient of the cell starts here (0 + g: the first term
+    // of its sum)
+#pragma unroll
+    for (int p = 0; p < MAXP; p++) ws(A, W_GP + p, n, 0.0 + g[p]);
     for (int m = 0; m < 7; m++) ws(A, W_LQN + m, n, lx[m]);
+}
+
+template <bool WRITTEN>
+DC_HD void rev_src2_t(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    if (src2_vjp_covers(A.P) != WRITTEN) return;
+    const size_t n = cix(A, i, j);
+    double x[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (interior(A, i, j))
+        for (int m = 0; m < 7; m++) x[m] = wv(A, W_QN + m, n);
+    rev_src2_cell<WRITTEN>(A, i, j, x);
 }
 
 DC_HD void rev_src2(const AdjArgs& A, const TP& Pz, int i, int j)
 {
     rev_src2_t<true>(A, Pz, i, j);
     rev_src2_t<false>(A, Pz, i, j);
+}
+
+// F6 and B1 in one pass: the update (kept in registers instead of going
+// through the QN planes), q_out when wanted, and src2's written-out reverse.
+// A member whose src2 needs the seeded products gets its QN planes instead,
+// and rev_src2_t<false> runs after.  Equals rec_update then rev_src2 bit for
+// bit.
+DC_HD void upd_src2(const AdjArgs& A, const TP& Pz, int i, int j)
+{
+    double q[7] = {0, 0, 0, 0, 0, 0, 0};
+    const bool in = update_state(A, i, j, q);
+    if (in) update_out(A, Pz, i, j, q);
+    if (src2_vjp_covers(A.P)) {
+        rev_src2_cell<true>(A, i, j, q);
+        return;
+    }
+    if (in) {
+        const size_t n = cix(A, i, j);
+        for (int m = 0; m < 7; m++) ws(A, W_QN + m, n, q[m]);
+    }
 }
 
 // Vector-Jacobian product of one rpt2 split (SplitF, rpt2_cell_split)
@@ -971,42 +1040,98 @@ DC_HD void split_vjp(const AdjArgs& A, const SplitF& f, const double* x, const d
     else split_vjp_t<2>(A, f, x, ly, lx, gp);
 }
 
-// B2: the four transverse splits of the cell.
+// B2: the four transverse splits of the cell.  Every value the splits read
+// is loaded before the first store: all planes share one pointer, so the
+// compiler keeps a load that follows a store behind it, and each split's
+// loads would wait for the previous split's stores.
 DC_HD void rev_split(const AdjArgs& A, const TP& Pz, int i, int j)
 {
     const size_t n = cix(A, i, j);
+    const bool tr = A.P.transverse != 0;
+    // the x-splits (s = 0, 1) and y-splits (s = 2, 3) take place (split_inputs)
+    const bool sx = tr && i >= 1 && i <= A.mx && j >= 0 && j <= A.my + 1;
+    const bool sy = tr && i >= 0 && i <= A.mx + 1 && j >= -1 && j <= A.my + 1;
+    double x[4][NSI];
+    split_state(A, n, x[0]);
+    double ax[2][6], ay[2][6];
+    for (int z = 0; z < 6; z++) {
+        ax[0][z] = sx ? wv(A, W_XE + 12 + z, cix(A, i + 1, j)) : 0.0;   // amdq of x-edge i+1
+        ax[1][z] = sx ? wv(A, W_XE + 18 + z, n) : 0.0;                  // apdq of x-edge i
+        ay[0][z] = sy ? wv(A, W_YE + 12 + z, cix(A, i, j + 1)) : 0.0;   // amdq of y-edge j+1
+        ay[1][z] = sy ? wv(A, W_YE + 18 + z, n) : 0.0;                  // apdq of y-edge j
+    }
+    const double bc = bed(A, i, j);
+    const double bjm = sx ? bed(A, i, j - 1) : 0.0, bjp = sx ? bed(A, i, j + 1) : 0.0;
+    const double bim = sy ? bed(A, i - 1, j) : 0.0, bip = sy ? bed(A, i + 1, j) : 0.0;
+    double lc[6], ljm[6], ljp[6], lim[6], lip[6], gp0[MAXP];
+    for (int z = 0; z < 6; z++) {
+        lc[z] = lam_qn(A, i, j, z);
+        ljm[z] = sx ? lam_qn(A, i, j - 1, z) : 0.0;
+        ljp[z] = sx ? lam_qn(A, i, j + 1, z) : 0.0;
+        lim[z] = sy ? lam_qn(A, i - 1, j, z) : 0.0;
+        lip[z] = sy ? lam_qn(A, i + 1, j, z) : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < MAXP; p++) gp0[p] = p < A.np ? wv(A, W_GP + p, n) : 0.0;
+
+    for (int s = 1; s < 4; s++)
+        for (int k = 0; k < 6; k++) x[s][k] = x[0][k];
+    for (int k = 0; k < 6; k++) {
+        x[0][6 + k] = ax[0][k];
+        x[1][6 + k] = ax[1][k];
+        x[2][6 + k] = ay[0][k];
+        x[3][6 + k] = ay[1][k];
+    }
+    // adjoints of bm, bp of each split: Gx1(i, j) feeds gm(i, j) and gp(i, j),
+    // Gx2(i, j) feeds gm, gp(i, j+1); Gy1(i, j) feeds fm(i, j) and fp(i, j),
+    // Gy2(i, j) feeds fm, fp(i+1, j) (lam_gm, lam_gp, lam_fm, lam_fp)
+    const double hx = 0.5 * (A.dt / A.dx), hy = 0.5 * (A.dt / A.dy);
+    const double ry = A.dt / A.dy, rx = A.dt / A.dx;
+    double lyx[12], lyy[12];
+    for (int z = 0; z < 6; z++) {
+        lyx[z] = -hx * (-ry * ljm[z] + ry * lc[z]);
+        lyx[6 + z] = -hx * (-ry * lc[z] + ry * ljp[z]);
+        lyy[z] = -hy * (-rx * lim[z] + rx * lc[z]);
+        lyy[6 + z] = -hy * (-rx * lc[z] + rx * lip[z]);
+    }
     double ls[6] = {0, 0, 0, 0, 0, 0};
     double g[MAXP] = {0, 0, 0, 0};
-    const double hx = 0.5 * (A.dt / A.dx), hy = 0.5 * (A.dt / A.dy);
+    double la[4][6];
+#pragma unroll
     for (int s = 0; s < 4; s++) {
-        // adjoint of the split's asdq: amdq of x-edge i+1 / apdq of x-edge i
-        // (s = 0, 1), the same for y-edges (s = 2, 3)
-        const int fa = s < 2 ? W_LXA + 6 * s : W_LYA + 6 * (s - 2);
-        double x[NSI];
+        for (int k = 0; k < 6; k++) la[s][k] = 0.0;
+        if (!(s < 2 ? sx : sy)) continue;
         SplitF f;
-        if (!split_inputs(A, s, i, j, x, f)) {
-            for (int k = 0; k < 6; k++) ws(A, fa + k, n, 0.0);
-            continue;
-        }
-        double ly[12];
-        for (int z = 0; z < 6; z++) {
-            if (s < 2) {
-                // Gx1(i, j) feeds gm(i, j) and gp(i, j); Gx2(i, j) feeds gm, gp(i, j+1)
-                ly[z] = -hx * (lam_gm(A, i, j, z) + lam_gp(A, i, j, z));
-                ly[6 + z] = -hx * (lam_gm(A, i, j + 1, z) + lam_gp(A, i, j + 1, z));
-            } else {
-                // Gy1(i, j) feeds fm(i, j) and fp(i, j); Gy2(i, j) feeds fm, fp(i+1, j)
-                ly[z] = -hy * (lam_fm(A, i, j, z) + lam_fp(A, i, j, z));
-                ly[6 + z] = -hy * (lam_fm(A, i + 1, j, z) + lam_fp(A, i + 1, j, z));
-            }
+        f.bedc = bc;
+        if (s < 2) {
+            f.ixy = 1;
+            f.topo1 = bjm;
+            f.topo3 = bjp;
+            f.active = true;
+        } else {
+            f.ixy = 2;
+            f.topo1 = bim;
+            f.topo3 = bip;
+            const int k = s == 2 ? j + 1 : j;
+            f.active = (k >= 1 && k <= A.my + 1);
         }
         double lx[NSI];
-        split_vjp(A, f, x, ly, lx, g);
+        split_vjp(A, f, x[s], s < 2 ? lyx : lyy, lx, g);
         for (int k = 0; k < 6; k++) ls[k] += lx[k];
-        for (int k = 0; k < 6; k++) ws(A, fa + k, n, lx[6 + k]);
+        for (int k = 0; k < 6; k++) la[s][k] = lx[6 + k];
+    }
+    // adjoint of the split's asdq: amdq of x-edge i+1 / apdq of x-edge i
+    // (s = 0, 1), the same for y-edges (s = 2, 3)
+    for (int k = 0; k < 6; k++) {
+        ws(A, W_LXA + k, n, la[0][k]);
+        ws(A, W_LXA + 6 + k, n, la[1][k]);
+        ws(A, W_LYA + k, n, la[2][k]);
+        ws(A, W_LYA + 6 + k, n, la[3][k]);
     }
     for (int k = 0; k < 6; k++) ws(A, W_LS + k, n, ls[k]);
-    gp_add(A, n, g);
+#pragma unroll
+    for (int p = 0; p < MAXP; p++)
+        if (p < A.np) ws(A, W_GP + p, n, gp0[p] + g[p]);
 }
 
 // d limiter_phi / dr on the branch the value takes (max/min ties as dmax and
@@ -1158,38 +1283,43 @@ DC_HD void flux_vjp(const dclaw_params& P, const FluxF& f, const double* x, cons
     }
 }
 
-// B3: the flux body of the x- and y-edge at (i, j).
+// B3: the flux body of the x- and y-edge at (i, j).  The edge body has no
+// parameter dependence.  Both edges' inputs are loaded before the first
+// store (as rec_waves).
 DC_HD void rev_flux(const AdjArgs& A, const TP& Pz, int i, int j)
 {
     const size_t n = cix(A, i, j);
-    double g[MAXP] = {0, 0, 0, 0};
-    for (int ixy = 1; ixy <= 2; ixy++) {
+    double x[2][NEI], ly[2][NEF];
+    FluxF f[2];
+    bool e[2];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        const int ixy = d + 1;
+        e[d] = ixy == 1 ? xflux(A, i, j) : yflux(A, i, j);
+        if (!e[d]) continue;
+        flux_inputs(A, ixy, i, j, x[d], f[d]);
+        for (int z = 0; z < 6; z++) {
+            if (ixy == 1) {
+                ly[d][z] = lam_fm(A, i, j, z);
+                ly[d][6 + z] = lam_fp(A, i, j, z);
+                ly[d][12 + z] = wv(A, W_LXA + z, cix(A, i - 1, j));   // amdq: cell i-1's split
+                ly[d][18 + z] = wv(A, W_LXA + 6 + z, n);              // apdq: cell i's split
+            } else {
+                ly[d][z] = lam_gm(A, i, j, z);
+                ly[d][6 + z] = lam_gp(A, i, j, z);
+                ly[d][12 + z] = wv(A, W_LYA + z, cix(A, i, j - 1));
+                ly[d][18 + z] = wv(A, W_LYA + 6 + z, n);
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
         double lx[NEI];
         for (int k = 0; k < NEI; k++) lx[k] = 0.0;
-        if (ixy == 1 ? xflux(A, i, j) : yflux(A, i, j)) {
-            double x[NEI], ly[NEF];
-            FluxF f;
-            flux_inputs(A, ixy, i, j, x, f);
-            for (int z = 0; z < 6; z++) {
-                if (ixy == 1) {
-                    ly[z] = lam_fm(A, i, j, z);
-                    ly[6 + z] = lam_fp(A, i, j, z);
-                    ly[12 + z] = wv(A, W_LXA + z, cix(A, i - 1, j));   // amdq: cell i-1's split
-                    ly[18 + z] = wv(A, W_LXA + 6 + z, n);              // apdq: cell i's split
-                } else {
-                    ly[z] = lam_gm(A, i, j, z);
-                    ly[6 + z] = lam_gp(A, i, j, z);
-                    ly[12 + z] = wv(A, W_LYA + z, cix(A, i, j - 1));
-                    ly[18 + z] = wv(A, W_LYA + 6 + z, n);
-                }
-            }
-            // the edge body has no parameter dependence
-            flux_vjp(A.P, f, x, ly, lx);
-        }
-        const int f0 = ixy == 1 ? W_LXE : W_LYE;
+        if (e[d]) flux_vjp(A.P, f[d], x[d], ly[d], lx);
+        const int f0 = d == 0 ? W_LXE : W_LYE;
         for (int k = 0; k < NEI; k++) ws(A, f0 + k, n, lx[k]);
     }
-    gp_add(A, n, g);
 }
 
 // Adjoint of the waves of an edge: its own flux body's, plus the limiter dot
@@ -2032,6 +2162,7 @@ DC_HD void rev_waves(const AdjArgs& A, const TP& Pz, int i, int j)
 {
     const size_t n = cix(A, i, j);
     double g[MAXP] = {0, 0, 0, 0};
+    double lr[NLR];
     for (int ixy = 1; ixy <= 2; ixy++) {
         double lx[2 * NSD];
         for (int k = 0; k < 2 * NSD; k++) lx[k] = 0.0;
@@ -2048,13 +2179,32 @@ DC_HD void rev_waves(const AdjArgs& A, const TP& Pz, int i, int j)
                 for (int p = 0; p < MAXP; p++) g[p] += gs[p];
             }
         }
-        const int f0 = ixy == 1 ? W_LXS : W_LYS;
-        for (int k = 0; k < 2 * NSD; k++) ws(A, f0 + k, n, lx[k]);
+        // the left (bottom) side belongs to the neighbour; the right (top)
+        // side is this cell's F1 state for both edges
+        const int f0 = ixy == 1 ? W_LXL : W_LYL;
+        for (int k = 0; k < NSD; k++) ws(A, f0 + k, n, lx[k]);
+        const double* r = lx + NSD;
+        if (ixy == 1) {
+            lr[LR_H] = r[SD_H]; lr[LR_HU] = r[SD_HU]; lr[LR_HV] = r[SD_HV];
+            lr[LR_HM1] = r[SD_HM]; lr[LR_P1] = r[SD_P]; lr[LR_HCHI1] = r[SD_HCHI];
+            lr[LR_U] = r[SD_U]; lr[LR_V] = r[SD_V]; lr[LR_M1] = r[SD_M];
+            lr[LR_CHI1] = r[SD_CHI]; lr[LR_BDIF] = r[SD_BDIF]; lr[LR_RHO1] = r[SD_RHO];
+            lr[LR_TAU1] = r[SD_TAU]; lr[LR_FS] = r[SD_FS]; lr[LR_TDX] = r[SD_TAUDIR];
+        } else {
+            // a y-edge's side has u and v swapped
+            lr[LR_H] += r[SD_H]; lr[LR_HU] += r[SD_HU]; lr[LR_HV] += r[SD_HV];
+            lr[LR_HM1] += r[SD_HM]; lr[LR_P1] += r[SD_P]; lr[LR_HCHI1] += r[SD_HCHI];
+            lr[LR_U] += r[SD_V]; lr[LR_V] += r[SD_U]; lr[LR_M1] += r[SD_M];
+            lr[LR_CHI1] += r[SD_CHI]; lr[LR_BDIF] += r[SD_BDIF]; lr[LR_RHO1] += r[SD_RHO];
+            lr[LR_TAU1] += r[SD_TAU]; lr[LR_FS] += r[SD_FS]; lr[LR_TDY] = r[SD_TAUDIR];
+        }
     }
+    for (int k = 0; k < NLR; k++) ws(A, W_LR + k, n, lr[k]);
     gp_add(A, n, g);
 }
 
-DC_HD double lxs(const AdjArgs& A, int f0, int i, int j, int k)
+// Field k of a neighbour's left (bottom) side adjoint, 0 outside the grid.
+DC_HD double lside(const AdjArgs& A, int f0, int i, int j, int k)
 {
     return in_grid(A, i, j) ? wv(A, f0 + k, cix(A, i, j)) : 0.0;
 }
@@ -2228,10 +2378,10 @@ DC_HD void rev_taudir(const AdjArgs& A, const TP& Pz, int i, int j)
         double ly[3];
         // the cell is the right side of x-edge i, the left of x-edge i+1, the
         // right (top) side of y-edge j, the left of y-edge j+1
-        ly[0] = lxs(A, W_LXS, i, j, NSD + SD_FS) + lxs(A, W_LXS, i + 1, j, SD_FS) +
-                lxs(A, W_LYS, i, j, NSD + SD_FS) + lxs(A, W_LYS, i, j + 1, SD_FS);
-        ly[1] = lxs(A, W_LXS, i, j, NSD + SD_TAUDIR) + lxs(A, W_LXS, i + 1, j, SD_TAUDIR);
-        ly[2] = lxs(A, W_LYS, i, j, NSD + SD_TAUDIR) + lxs(A, W_LYS, i, j + 1, SD_TAUDIR);
+        ly[0] = (wv(A, W_LR + LR_FS, n) + lside(A, W_LXL, i + 1, j, SD_FS)) +
+                lside(A, W_LYL, i, j + 1, SD_FS);
+        ly[1] = wv(A, W_LR + LR_TDX, n) + lside(A, W_LXL, i + 1, j, SD_TAUDIR);
+        ly[2] = wv(A, W_LR + LR_TDY, n) + lside(A, W_LYL, i, j + 1, SD_TAUDIR);
         double x[NTI], g[MAXP] = {0, 0, 0, 0};
         TaudirF f;
         taudir_inputs_live(A, i, j, x, f);
@@ -2359,6 +2509,15 @@ DC_HD void prepare_vjp(const dclaw_params& P, const double* q7, const double* lp
 }
 
 
+// Whether lambda_n of cell (i, j) comes from the boundary map's pass (rev_bc):
+// with the map on load, the ghost cells and the interior cells within 2 of
+// the border; the prepare pass writes every other cell's lambda_n itself.
+DC_HD bool bc_gathers(const AdjArgs& A, int i, int j)
+{
+    if (A.bc[0] == 0) return false;
+    return !interior(A, i, j) || i <= 2 || i >= A.mx - 1 || j <= 2 || j >= A.my - 1;
+}
+
 // B6: gather the adjoint of the prepared fields, then b4step2's pointwise part.
 DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
 {
@@ -2380,24 +2539,24 @@ DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
     lp[PC_H] += wv(A, W_LS + 0, n); lp[PC_HU] += wv(A, W_LS + 1, n);
     lp[PC_HV] += wv(A, W_LS + 2, n); lp[PC_HM2] += wv(A, W_LS + 3, n);
     lp[PC_HCHI2] += wv(A, W_LS + 4, n); lp[PC_BDIF] += wv(A, W_LS + 5, n);
-    // rpn2 sides: right side of x-edge i and y-edge j (F1), left side of
+    // rpn2 sides: right side of x-edge i and y-edge j (F1, summed by the
+    // waves' reverse pass; 0 where the edge does not exist), left side of
     // x-edge i+1 and y-edge j+1 (F2 unless the cell is the slice's first)
     {
+        lp[PC_H] += wv(A, W_LR + LR_H, n); lp[PC_HU] += wv(A, W_LR + LR_HU, n);
+        lp[PC_HV] += wv(A, W_LR + LR_HV, n); lp[PC_HM1] += wv(A, W_LR + LR_HM1, n);
+        lp[PC_P1] += wv(A, W_LR + LR_P1, n); lp[PC_HCHI1] += wv(A, W_LR + LR_HCHI1, n);
+        lp[PC_U] += wv(A, W_LR + LR_U, n); lp[PC_V] += wv(A, W_LR + LR_V, n);
+        lp[PC_M1] += wv(A, W_LR + LR_M1, n); lp[PC_CHI1] += wv(A, W_LR + LR_CHI1, n);
+        lp[PC_BDIF] += wv(A, W_LR + LR_BDIF, n); lp[PC_RHO1] += wv(A, W_LR + LR_RHO1, n);
+        lp[PC_TAU1] += wv(A, W_LR + LR_TAU1, n);
         double ls[NSD];
-        if (xedge(A, i, j)) {
-            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LXS + NSD + k, n);
-            side_to_pc(ls, false, 1, lp);
-        }
         if (xedge(A, i + 1, j)) {
-            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LXS + k, cix(A, i + 1, j));
+            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LXL + k, cix(A, i + 1, j));
             side_to_pc(ls, i >= 0, 1, lp);
         }
-        if (yedge(A, i, j)) {
-            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LYS + NSD + k, n);
-            side_to_pc(ls, false, 2, lp);
-        }
         if (yedge(A, i, j + 1)) {
-            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LYS + k, cix(A, i, j + 1));
+            for (int k = 0; k < NSD; k++) ls[k] = wv(A, W_LYL + k, cix(A, i, j + 1));
             side_to_pc(ls, j >= 0, 2, lp);
         }
     }
@@ -2435,7 +2594,13 @@ DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
             if (p < A.np) g[p] = pa_dot(pa, A.dP[p]);
         gp_add(A, n, g);
     }
-    for (int m = 0; m < 7; m++) ws(A, W_LQ7 + m, n, lx[m]);
+    if (bc_gathers(A, i, j)) {
+        for (int m = 0; m < 7; m++) ws(A, W_LQ7 + m, n, lx[m]);
+    } else {
+        // lambda_n of a cell the boundary map does not touch
+        double* o = A.lam_in + qidx(0, i, j, A.mx);
+        for (int m = 0; m < 7; m++) o[m] = lx[m];
+    }
 }
 
 // B7: lambda_n.  With the boundary map on load a ghost cell's q is its source
@@ -2443,37 +2608,36 @@ DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
 // source; the stored ghosts of q_n are not read and get 0.
 DC_HD void rev_bc(const AdjArgs& A, int i, int j)
 {
+    if (!bc_gathers(A, i, j)) return;   // written by the prepare pass
     const size_t n = cix(A, i, j);
     double l[7];
     for (int m = 0; m < 7; m++) l[m] = wv(A, W_LQ7 + m, n);
-    if (A.bc[0] != 0) {
-        if (!interior(A, i, j)) {
-            for (int m = 0; m < 7; m++) l[m] = 0.0;
-        } else if (i <= 2 || i >= A.mx - 1 || j <= 2 || j >= A.my - 1) {
-            // ghost cells whose source is (i, j): columns -1, 0 / mx+1, mx+2
-            // and rows -1, 0 / my+1, my+2 around the cell's own
-            int gi[5], gj[5], ni = 0, nj = 0;
-            gi[ni++] = i;
-            if (i <= 2) { gi[ni++] = -1; gi[ni++] = 0; }
-            if (i >= A.mx - 1) { gi[ni++] = A.mx + 1; gi[ni++] = A.mx + 2; }
-            gj[nj++] = j;
-            if (j <= 2) { gj[nj++] = -1; gj[nj++] = 0; }
-            if (j >= A.my - 1) { gj[nj++] = A.my + 1; gj[nj++] = A.my + 2; }
-            for (int b = 0; b < nj; b++)
-                for (int a = 0; a < ni; a++) {
-                    const int ci = gi[a], cj = gj[b];
-                    if (interior(A, ci, cj)) continue;
-                    int si, sj;
-                    double fu, fv;
-                    bc_source(A.bc, A.mx, A.my, ci, cj, si, sj, fu, fv);
-                    if (si != i || sj != j) continue;
-                    const size_t c = cix(A, ci, cj);
-                    for (int m = 0; m < 7; m++) {
-                        const double s = m == 1 ? fu : (m == 2 ? fv : 1.0);
-                        l[m] += s * wv(A, W_LQ7 + m, c);
-                    }
+    if (!interior(A, i, j)) {
+        for (int m = 0; m < 7; m++) l[m] = 0.0;
+    } else {
+        // ghost cells whose source is (i, j): columns -1, 0 / mx+1, mx+2
+        // and rows -1, 0 / my+1, my+2 around the cell's own
+        int gi[5], gj[5], ni = 0, nj = 0;
+        gi[ni++] = i;
+        if (i <= 2) { gi[ni++] = -1; gi[ni++] = 0; }
+        if (i >= A.mx - 1) { gi[ni++] = A.mx + 1; gi[ni++] = A.mx + 2; }
+        gj[nj++] = j;
+        if (j <= 2) { gj[nj++] = -1; gj[nj++] = 0; }
+        if (j >= A.my - 1) { gj[nj++] = A.my + 1; gj[nj++] = A.my + 2; }
+        for (int b = 0; b < nj; b++)
+            for (int a = 0; a < ni; a++) {
+                const int ci = gi[a], cj = gj[b];
+                if (interior(A, ci, cj)) continue;
+                int si, sj;
+                double fu, fv;
+                bc_source(A.bc, A.mx, A.my, ci, cj, si, sj, fu, fv);
+                if (si != i || sj != j) continue;
+                const size_t c = cix(A, ci, cj);
+                for (int m = 0; m < 7; m++) {
+                    const double s = m == 1 ? fu : (m == 2 ? fv : 1.0);
+                    l[m] += s * wv(A, W_LQ7 + m, c);
                 }
-        }
+            }
     }
     double* o = A.lam_in + qidx(0, i, j, A.mx);
     for (int m = 0; m < 7; m++) o[m] = l[m];

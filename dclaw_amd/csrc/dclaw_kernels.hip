@@ -1138,7 +1138,7 @@ __global__ void k_gauges(const double* __restrict__ q, const double* __restrict_
 enum AdjStage {
     ADJ_REC_PREPARE, ADJ_REC_TAUDIR, ADJ_REC_WAVES, ADJ_REC_FLUX, ADJ_REC_SPLIT, ADJ_REC_UPDATE,
     ADJ_REV_SRC2, ADJ_REV_SPLIT, ADJ_REV_FLUX, ADJ_REV_WAVES, ADJ_REV_TAUDIR, ADJ_REV_PREPARE,
-    ADJ_REV_BC, ADJ_REV_SRC2_SEEDED, ADJ_NSTAGES
+    ADJ_REV_BC, ADJ_REV_SRC2_SEEDED, ADJ_UPD_SRC2, ADJ_NSTAGES
 };
 // Tiles are 16 cells wide and 4 high (64 cells: one wavefront): a tile row
 // of a workspace plane is one 128-byte line (the planes' row stride is a
@@ -1243,6 +1243,7 @@ __device__ __forceinline__ void adj_stage(const dclaw_adj::AdjArgs& A, const dcl
     else if constexpr (STAGE == ADJ_REC_UPDATE) dclaw_adj::rec_update(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REV_SRC2) dclaw_adj::rev_src2_t<true>(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REV_SRC2_SEEDED) dclaw_adj::rev_src2_t<false>(A, Pz, i, j);
+    else if constexpr (STAGE == ADJ_UPD_SRC2) dclaw_adj::upd_src2(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REV_SPLIT) dclaw_adj::rev_split(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REV_FLUX) dclaw_adj::rev_flux(A, Pz, i, j);
     else if constexpr (STAGE == ADJ_REV_WAVES) dclaw_adj::rev_waves(A, Pz, i, j);
@@ -1311,6 +1312,22 @@ __device__ __forceinline__ void adj_dense_cell(const dclaw_adj::AdjArgs& A, cons
 
 template <int STAGE>
 __global__ __launch_bounds__(64) void k_adjoint(dclaw_adj::AdjArgs A, AdjTiles T)
+{
+    A.P = *A.Pd;
+    A.dt = *A.dt_dev;
+    const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
+    int k, end, step;
+    adj_span(*T.count, k, end, step);
+    for (; k < end; k += step) {
+        int i, j;
+        if (adj_tile_cell(A, T, T.list[k], i, j)) adj_stage<STAGE>(A, Pz, i, j);
+    }
+}
+
+// The same, held to two waves per SIMD (the stages the batched step holds
+// there too).
+template <int STAGE>
+__global__ __launch_bounds__(64, 2) void k_adjoint2(dclaw_adj::AdjArgs A, AdjTiles T)
 {
     A.P = *A.Pd;
     A.dt = *A.dt_dev;
@@ -2033,17 +2050,16 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
     hipLaunchKernelGGL(k_adjoint_list, dim3(nblk), dim3(64), 0, st, T, ntiles);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_PREPARE>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_TAUDIR>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint<ADJ_REC_WAVES>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint2<ADJ_REC_WAVES>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_FLUX>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_SPLIT>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint<ADJ_REC_UPDATE>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint<ADJ_REV_SRC2>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint2<ADJ_UPD_SRC2>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_SRC2_SEEDED>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint<ADJ_REV_SPLIT>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint<ADJ_REV_FLUX>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint2<ADJ_REV_SPLIT>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint2<ADJ_REV_FLUX>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_WAVES>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint<ADJ_REV_TAUDIR>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint<ADJ_REV_PREPARE>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint2<ADJ_REV_TAUDIR>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL(k_adjoint2<ADJ_REV_PREPARE>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_BC>, tg, tb, 0, st, A, T);
     if (np > 0) {
         double* part = work + (size_t)dclaw_adj::W_FIELDS * A.N;
@@ -2132,14 +2148,13 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     hipLaunchKernelGGL(k_adjb_list, dim3((unsigned)nblk), tb, 0, st, B, ntiles);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_PREPARE>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_TAUDIR>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb<ADJ_REC_WAVES>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb2<ADJ_REC_WAVES>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_FLUX>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_SPLIT>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb<ADJ_REC_UPDATE>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb2<ADJ_REV_SRC2>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb2<ADJ_UPD_SRC2>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_SRC2_SEEDED>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb2<ADJ_REV_SPLIT>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb<ADJ_REV_FLUX>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL(k_adjb2<ADJ_REV_FLUX>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_WAVES>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb2<ADJ_REV_TAUDIR>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb2<ADJ_REV_PREPARE>, tg, tb, 0, st, B);

@@ -518,6 +518,11 @@ extern "C" int emu_stepgrid_tl(const dclaw_params* P, const dclaw_params* dP, co
 // The adjoint step (dclaw_adjoint.h) with its stages run over the grid in
 // order on the host: q_out is the record pass's step, lam_in = J_q^T lam_out
 // and gp[k] = lam_out . J_P dP[k] summed over the cells in index order.
+// emu_set_adjoint_fused(1) runs the update and src2's reverse as the GPU
+// does (one fused pass); 0 (default) as two passes through the QN planes.
+static int g_emu_fused_update = 0;
+extern "C" void emu_set_adjoint_fused(int on) { g_emu_fused_update = on; }
+
 extern "C" int emu_adjoint_step(const dclaw_params* P, const dclaw_params* dP, int np,
                                 const double* q_in, const double* aux, int maux, int mx, int my,
                                 double dx, double dy, double dt, const int* bc,
@@ -557,8 +562,15 @@ extern "C" int emu_adjoint_step(const dclaw_params* P, const dclaw_params* dP, i
     all(D::rec_waves);
     all(D::rec_flux);
     all(D::rec_split);
-    all(D::rec_update);
-    all(D::rev_src2);
+    if (g_emu_fused_update) {
+        // the GPU passes: update and src2's reverse in one pass, then the
+        // seeded src2 reverse for configurations the written one does not cover
+        all(D::upd_src2);
+        all(D::rev_src2_t<false>);
+    } else {
+        all(D::rec_update);
+        all(D::rev_src2);
+    }
     all(D::rev_split);
     all(D::rev_flux);
     all(D::rev_waves);

@@ -127,6 +127,29 @@ def test_adjoint_with_entrainment(emu_lib, oracle_mod):
     _dot_identity(emu_lib, pt, q, aux, dt, (1, 1, 1, 1), ["phi", "entrainment_rate"], seed=3)
 
 
+@pytest.mark.parametrize("variant", [dict(), dict(friction_forcing=1, manning_coefficient=0.06)])
+def test_fused_update_src2_pass_equals_two_passes(emu_lib, moving, variant):
+    """The GPU runs the update and src2's reverse as one pass (the updated
+    state stays in registers); the result is the two-pass form's bit for bit,
+    also when src2 takes the seeded products (Manning friction)."""
+    pt, q, aux, dt = moving
+    c = to_c(pt.params)
+    for k, v in variant.items():
+        setattr(c, k, v)
+    rng = np.random.default_rng(4)
+    lam = _scaled(rng, q)
+    dPs = [CParams(phi=1.0), CParams(kref=1e-10)]
+    try:
+        emu_lib.emu_set_adjoint_fused(1)
+        fused = _adj(emu_lib, c, dPs, q, aux, pt.dx, pt.dy, dt, (1, 1, 3, 3), lam)
+    finally:
+        emu_lib.emu_set_adjoint_fused(0)
+    two = _adj(emu_lib, c, dPs, q, aux, pt.dx, pt.dy, dt, (1, 1, 3, 3), lam)
+    assert np.array_equal(fused[0], two[0]) and np.array_equal(fused[1], two[1])
+    assert np.array_equal(fused[2], two[2])
+    assert np.abs(fused[1]).max() > 0.0
+
+
 def test_adjoint_on_a_dry_patch_is_the_bdif_clip(emu_lib, oracle_mod):
     """All dry: the step maps q to the dry state (zeros, bdif clipped at 0),
     so lambda_n is lambda_{n+1}'s bdif entry where bdif > 0 and 0 elsewhere."""
