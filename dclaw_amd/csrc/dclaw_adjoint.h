@@ -942,9 +942,12 @@ DC_HD void split_vjp_t(const AdjArgs& A, const SplitF& f, const double* x, const
     const double eta = h + f.bedc - x[5];
     const double g = P.grav;
     const double rho = m * P.rho_s + (1.0 - m) * P.rho_f;
+    // the reverse divides by D, h, rho and 2 sq through their reciprocals
+    // (one division each; an FP64 division is a chain of 11 instructions)
+    const double rrho = 1.0 / rho;
     double gamma;
-    if (P.src2method == -1) gamma = P.rho_f / rho;
-    else gamma = 0.25 * (P.rho_f + 3.0 * rho) / rho;
+    if (P.src2method == -1) gamma = P.rho_f * rrho;
+    else gamma = 0.25 * (P.rho_f + 3.0 * rho) * rrho;
     const double tmin = (f.topo1 < f.topo3) ? f.topo1 : f.topo3;
     if (eta < tmin) return;
     const double sq = sqrt(g * h);
@@ -952,7 +955,8 @@ DC_HD void split_vjp_t(const AdjArgs& A, const SplitF& f, const double* x, const
     const double* asdq = x + 6;
     const double d1 = asdq[0], d2 = asdq[mu], d3 = asdq[mv];
     const double D = s[2] - s[0];
-    const double be[3] = {(s[2] * d1 - d3) / D, -(u * d1) + d2, (d3 - s[0] * d1) / D};
+    const double rD = 1.0 / D;
+    const double be[3] = {(s[2] * d1 - d3) * rD, -(u * d1) + d2, (d3 - s[0] * d1) * rD};
     const double r5 = gamma * rho * g;
     double r[3][6];
     r[0][0] = 1.0; r[0][mu] = u; r[0][mv] = s[0]; r[0][3] = m; r[0][4] = r5; r[0][5] = chi;
@@ -983,17 +987,17 @@ DC_HD void split_vjp_t(const AdjArgs& A, const SplitF& f, const double* x, const
     ls[2] += lr[2][mv];
     // betas
     double ld1 = 0.0, ld2 = 0.0, ld3 = 0.0;
-    ls[2] += lbe[0] * (d1 - be[0]) / D;
-    ls[0] += lbe[0] * be[0] / D;
-    ld1 += lbe[0] * s[2] / D;
-    ld3 += -lbe[0] / D;
+    ls[2] += lbe[0] * (d1 - be[0]) * rD;
+    ls[0] += lbe[0] * be[0] * rD;
+    ld1 += lbe[0] * s[2] * rD;
+    ld3 += -lbe[0] * rD;
     lu += -(lbe[1] * d1);
     ld1 += -(lbe[1] * u);
     ld2 += lbe[1];
-    ls[0] += lbe[2] * (be[2] - d1) / D;
-    ls[2] += -(lbe[2] * be[2]) / D;
-    ld3 += lbe[2] / D;
-    ld1 += -(lbe[2] * s[0]) / D;
+    ls[0] += lbe[2] * (be[2] - d1) * rD;
+    ls[2] += -(lbe[2] * be[2]) * rD;
+    ld3 += lbe[2] * rD;
+    ld1 += -(lbe[2] * s[0]) * rD;
     lx[6 + 0] += ld1;
     lx[6 + mu] += ld2;
     lx[6 + mv] += ld3;
@@ -1002,29 +1006,31 @@ DC_HD void split_vjp_t(const AdjArgs& A, const SplitF& f, const double* x, const
     const double lsq = ls[2] - ls[0];
     double lh = 0.0, lg = 0.0;
     if (sq > 0.0) {
-        lh += lsq * g / (2.0 * sq);
-        lg += lsq * h / (2.0 * sq);
+        const double r2sq = 0.5 / sq;
+        lh += lsq * g * r2sq;
+        lg += lsq * h * r2sq;
     }
     // r5 = gamma rho g; gamma(rho, rho_f); rho = m rho_s + (1 - m) rho_f
     double lrho = lr5 * gamma * g, lrhof = 0.0;
     lg += lr5 * gamma * rho;
     const double lgam = lr5 * rho * g;
     if (P.src2method == -1) {
-        lrho += -(lgam * gamma) / rho;
-        lrhof += lgam / rho;
+        lrho += -(lgam * gamma) * rrho;
+        lrhof += lgam * rrho;
     } else {
-        lrho += -(lgam * 0.25 * P.rho_f) / (rho * rho);
-        lrhof += lgam * 0.25 / rho;
+        lrho += -(lgam * 0.25 * P.rho_f) * (rrho * rrho);
+        lrhof += lgam * 0.25 * rrho;
     }
     lm += lrho * (P.rho_s - P.rho_f);
     const double lrhos = lrho * m;
     lrhof += lrho * (1.0 - m);
     // u = nu / h, v = nv / h, m = hm / h, chi = hchi / h
-    lx[xd ? 1 : 2] += lu / h;
-    lx[xd ? 2 : 1] += lv / h;
-    lx[3] += lm / h;
-    lx[4] += lchi / h;
-    lh += -(lu * u + lv * v + lm * m + lchi * chi) / h;
+    const double rh = 1.0 / h;
+    lx[xd ? 1 : 2] += lu * rh;
+    lx[xd ? 2 : 1] += lv * rh;
+    lx[3] += lm * rh;
+    lx[4] += lchi * rh;
+    lh += -(lu * u + lv * v + lm * m + lchi * chi) * rh;
     lx[0] += lh;
 #pragma unroll
     for (int p = 0; p < MAXP; p++)
@@ -2261,13 +2267,17 @@ DC_HD void taudir_vjp(const dclaw_params& P, double dx, double dy, const double*
     const double Fy = fy_on ? (fy_l ? FyL : FyR) : 0.0;
     double bFx = 0.0, bFy = 0.0, bFxL = 0.0, bFyL = 0.0, bhu = 0.0, bhv = 0.0;
     const double hvn = sqrt(hu * hu + hv * hv);
+    // The reverse divides through reciprocals (one division each; an FP64
+    // division is a chain of 11 instructions).  The forward values that
+    // select a branch (the F's, Fproj, X) keep calc_taudir's own divisions.
     if (hvn > 0.0) {
         double bhvn = 0.0;
-        const double tx = -(dx * hu / hvn), ty = -(dy * hv / hvn);
-        bhu -= ly[1] * dx / hvn;
-        bhvn -= ly[1] * tx / hvn;
-        bhv -= ly[2] * dy / hvn;
-        bhvn -= ly[2] * ty / hvn;
+        const double rhv = 1.0 / hvn;
+        const double tx = -(dx * hu * rhv), ty = -(dy * hv * rhv);
+        bhu -= ly[1] * dx * rhv;
+        bhvn -= ly[1] * tx * rhv;
+        bhv -= ly[2] * dy * rhv;
+        bhvn -= ly[2] * ty * rhv;
         const double a1p = Fx * hu, a2p = Fy * hv;
         const double a1 = 0.0 > a1p ? 0.0 : a1p, a2 = 0.0 > a2p ? 0.0 : a2p;
         const double dot = a1 < a2 ? a1 : a2;
@@ -2278,11 +2288,12 @@ DC_HD void taudir_vjp(const dclaw_params& P, double dx, double dy, const double*
             const double X = Fproj * rho / tmax;
             if (!(1.0 < X)) {
                 const double bX = ly[0];
-                const double bFproj = bX * rho / tmax;
-                lx[35] = bX * Fproj / tmax;
-                if (tau > 1e-16) lx[36] = -bX * X / tmax;
-                const double bdot = bFproj / hvn;
-                bhvn -= bFproj * Fproj / hvn;
+                const double rtm = 1.0 / tmax;
+                const double bFproj = bX * rho * rtm;
+                lx[35] = bX * Fproj * rtm;
+                if (tau > 1e-16) lx[36] = -bX * X * rtm;
+                const double bdot = bFproj * rhv;
+                bhvn -= bFproj * Fproj * rhv;
                 double ba1 = 0.0, ba2 = 0.0;
                 if (a1 < a2) ba1 = bdot;
                 else ba2 = bdot;
@@ -2296,23 +2307,23 @@ DC_HD void taudir_vjp(const dclaw_params& P, double dx, double dy, const double*
                 }
             }
         }
-        bhu += bhvn * hu / hvn;
-        bhv += bhvn * hv / hvn;
+        bhu += bhvn * hu * rhv;
+        bhv += bhvn * hv * rhv;
     } else {
         const double S1 = FxL * FxL + Fy * Fy, S2 = Fx * Fx + FyL * FyL;
         if (S1 > 0.0) {
-            const double r1 = sqrt(S1);
-            bFxL -= ly[1] * dx / r1;
-            const double br1 = ly[1] * dx * FxL / (r1 * r1);
-            bFxL += br1 * FxL / r1;
-            bFy += br1 * Fy / r1;
+            const double r1 = sqrt(S1), rr1 = 1.0 / r1;
+            bFxL -= ly[1] * dx * rr1;
+            const double br1 = ly[1] * dx * FxL * (rr1 * rr1);
+            bFxL += br1 * FxL * rr1;
+            bFy += br1 * Fy * rr1;
         }
         if (S2 > 0.0) {
-            const double r2 = sqrt(S2);
-            bFyL -= ly[2] * dy / r2;
-            const double br2 = ly[2] * dy * FyL / (r2 * r2);
-            bFx += br2 * Fx / r2;
-            bFyL += br2 * FyL / r2;
+            const double r2 = sqrt(S2), rr2 = 1.0 / r2;
+            bFyL -= ly[2] * dy * rr2;
+            const double br2 = ly[2] * dy * FyL * (rr2 * rr2);
+            bFx += br2 * Fx * rr2;
+            bFyL += br2 * FyL * rr2;
         }
     }
     double bFxR = 0.0, bFyR = 0.0;
@@ -2327,22 +2338,23 @@ DC_HD void taudir_vjp(const dclaw_params& P, double dx, double dy, const double*
     // F = -(A (h + h_n) d_eta / d); A = grav / 2
     double bh[5] = {0, 0, 0, 0, 0}, ben[5] = {0, 0, 0, 0, 0}, beta = 0.0;
     {
-        const double sL = A * (h + hs[1]) / dx, dL = eta - en[1];
-        bh[0] -= bFxL * A * dL / dx; bh[1] -= bFxL * A * dL / dx;
+        const double rdx = 1.0 / dx, rdy = 1.0 / dy;
+        const double sL = A * (h + hs[1]) * rdx, dL = eta - en[1];
+        bh[0] -= bFxL * A * dL * rdx; bh[1] -= bFxL * A * dL * rdx;
         beta -= bFxL * sL; ben[1] += bFxL * sL;
-        b_grav -= bFxL * 0.5 * (h + hs[1]) * dL / dx;
-        const double sB = A * (h + hs[3]) / dy, dB = eta - en[3];
-        bh[0] -= bFyL * A * dB / dy; bh[3] -= bFyL * A * dB / dy;
+        b_grav -= bFxL * 0.5 * (h + hs[1]) * dL * rdx;
+        const double sB = A * (h + hs[3]) * rdy, dB = eta - en[3];
+        bh[0] -= bFyL * A * dB * rdy; bh[3] -= bFyL * A * dB * rdy;
         beta -= bFyL * sB; ben[3] += bFyL * sB;
-        b_grav -= bFyL * 0.5 * (h + hs[3]) * dB / dy;
-        const double sR = A * (h + hs[2]) / dx, dR = en[2] - eta;
-        bh[0] -= bFxR * A * dR / dx; bh[2] -= bFxR * A * dR / dx;
+        b_grav -= bFyL * 0.5 * (h + hs[3]) * dB * rdy;
+        const double sR = A * (h + hs[2]) * rdx, dR = en[2] - eta;
+        bh[0] -= bFxR * A * dR * rdx; bh[2] -= bFxR * A * dR * rdx;
         ben[2] -= bFxR * sR; beta += bFxR * sR;
-        b_grav -= bFxR * 0.5 * (h + hs[2]) * dR / dx;
-        const double sT = A * (h + hs[4]) / dy, dT = en[4] - eta;
-        bh[0] -= bFyR * A * dT / dy; bh[4] -= bFyR * A * dT / dy;
+        b_grav -= bFxR * 0.5 * (h + hs[2]) * dR * rdx;
+        const double sT = A * (h + hs[4]) * rdy, dT = en[4] - eta;
+        bh[0] -= bFyR * A * dT * rdy; bh[4] -= bFyR * A * dT * rdy;
         ben[4] -= bFyR * sT; beta += bFyR * sT;
-        b_grav -= bFyR * 0.5 * (h + hs[4]) * dT / dy;
+        b_grav -= bFyR * 0.5 * (h + hs[4]) * dT * rdy;
     }
     // the centre's limits (bottom, then left), then the neighbours'
     double be0[5] = {0, 0, 0, 0, 0};

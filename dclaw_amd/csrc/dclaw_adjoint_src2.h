@@ -127,12 +127,12 @@ DC_HD void setvars_rev(const dclaw_params& P, double h, double u, double v, doub
     if (s.meq_const) {
         pa.m_crit += bme;
     } else {
-        const double S = s.sN + s.sM;
-        pa.m_crit += bme * (s.sN / S);
+        const double S = s.sN + s.sM, rS = 1.0 / S;
+        pa.m_crit += bme * (s.sN * rS);
         const double bf = bme * P.m_crit;
-        const double b_sN = bf * s.sM / (S * S), b_sM = -bf * s.sN / (S * S);
-        if (s.sN > 0.0) b_Nden += b_sN / (2.0 * s.sN);
-        if (s.sM > 0.0) b_Nnum += b_sM / (2.0 * s.sM);
+        const double b_sN = bf * s.sM * (rS * rS), b_sM = -bf * s.sN * (rS * rS);
+        if (s.sN > 0.0) b_Nden += b_sN * (0.5 / s.sN);
+        if (s.sM > 0.0) b_Nnum += b_sM * (0.5 / s.sM);
     }
     // Nnum = mu shear; Nden = rho_s sd^2 + sig_eff; sd = shear delta
     pa.mu += b_Nnum * s.shear;
@@ -143,29 +143,33 @@ DC_HD void setvars_rev(const dclaw_params& P, double h, double u, double v, doub
     b_shear += b_sd * P.delta;
     pa.delta += b_sd * s.shear;
     if (s.shear_on) {
-        const double b_vn = b_shear * 2.0 / h;
-        bh -= b_shear * s.shear / h;
+        const double rh = 1.0 / h;
+        const double b_vn = b_shear * 2.0 * rh;
+        bh -= b_shear * s.shear * rh;
         if (s.vn > 0.0) {
-            bu += b_vn * u / s.vn;
-            bv += b_vn * v / s.vn;
+            const double rvn = 1.0 / s.vn;
+            bu += b_vn * u * rvn;
+            bv += b_vn * v * rvn;
         }
     }
     // alphainv = m (sig_eff + sig_0) / alpha_c
-    bm += b_ai * (s.sig_eff + s.sig_0) / P.alpha_c;
-    const double b_S = b_ai * m / P.alpha_c;
-    pa.alpha_c -= b_ai * s.alphainv / P.alpha_c;
+    const double rac = 1.0 / P.alpha_c;
+    bm += b_ai * (s.sig_eff + s.sig_0) * rac;
+    const double b_S = b_ai * m * rac;
+    pa.alpha_c -= b_ai * s.alphainv * rac;
     b_sig += b_S;
     double b_r = b_rho;
     if (s.am1) {
         // sig_0 = 0.5 alpha_c rho_f (rho_s - rho_f) gz h / rho
-        const double ghr = gz * h / s.rho;
+        const double rrho = 1.0 / s.rho;
+        const double ghr = gz * h * rrho;
         const double c0 = 0.5 * P.alpha_c * P.rho_f * s.r1;
         pa.alpha_c += b_S * 0.5 * P.rho_f * s.r1 * ghr;
         pa.rho_f += b_S * 0.5 * P.alpha_c * (s.r1 - P.rho_f) * ghr;
         pa.rho_s += b_S * 0.5 * P.alpha_c * P.rho_f * ghr;
-        bgz += b_S * c0 * h / s.rho;
-        bh += b_S * c0 * gz / s.rho;
-        b_r -= b_S * s.sig_0 / s.rho;
+        bgz += b_S * c0 * h * rrho;
+        bh += b_S * c0 * gz * rrho;
+        b_r -= b_S * s.sig_0 * rrho;
     } else {
         pa.sigma_0 += b_S;
     }
@@ -179,8 +183,8 @@ DC_HD void setvars_rev(const dclaw_params& P, double h, double u, double v, doub
     // kperm = kr kref exp(-(m - mref) / 0.04)
     pa.kref += b_kperm * s.kr * s.ex;
     const double b_a = b_kperm * s.kr * P.kref * s.ex;
-    bm -= b_a / 0.04;
-    pa.mref += b_a / 0.04;
+    bm -= b_a * 25.0;   // d/dm of -(m - mref) / 0.04
+    pa.mref += b_a * 25.0;
     if (s.seg) {
         const double b_t = b_kperm * P.kref * s.ex * s.kr * ::dcm_log(10.0);
         pa.delta_kr_order += b_t * 2.0 * (chi - 0.5);
@@ -249,14 +253,15 @@ DC_HD void qfix_rev(const dclaw_params& P, const QfFwd& f, double gz, double& bh
     bh += bhm * f.m;
     bmt += bhm * f.h;
     const double bm0 = (f.m0 > 0.0 && f.m0 < 1.0) ? bmt : 0.0;
-    bhm = bm0 / f.h;
-    bh -= bm0 * f.m0 / f.h;
-    bhchi = bchi0 / f.h;
-    bh -= bchi0 * f.chi0 / f.h;
-    bhu += bu / f.h;
-    bh -= bu * f.u / f.h;
-    bhv += bv / f.h;
-    bh -= bv * f.v / f.h;
+    const double rh = 1.0 / f.h;   // one division (an FP64 division is 11 instructions)
+    bhm = bm0 * rh;
+    bh -= bm0 * f.m0 * rh;
+    bhchi = bchi0 * rh;
+    bh -= bchi0 * f.chi0 * rh;
+    bhu += bu * rh;
+    bh -= bu * f.u * rh;
+    bhv += bv * rh;
+    bh -= bv * f.v * rh;
     bp = bp_in;
 }
 
@@ -312,8 +317,9 @@ DC_HD void qcm_rev(const dclaw_params& P, const QcFwd& c, double rhoh, double gz
     bgz += bpemax * (rhoh - P.rho_f * c.h);
     pa.rho_f -= bpemax * c.h * gz;
     bht -= bpemax * P.rho_f * gz;
-    brhoh += bht / c.rho;
-    const double brt = brho - bht * c.h / c.rho;
+    const double rrho = 1.0 / c.rho;
+    brhoh += bht * rrho;
+    const double brt = brho - bht * c.h * rrho;
     bm1 += brt * (P.rho_s - P.rho_f);
     pa.rho_s += brt * c.m1;
     pa.rho_f += brt * (1.0 - c.m1);
@@ -367,6 +373,10 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         u2 = hu2 / h0;
         v2 = hv2 / h0;
     }
+    // the reverse divides through reciprocals (one division each; an FP64
+    // division is a chain of 11 instructions); the forward values above keep
+    // src2's own divisions, since they select its branches
+    const double rh0 = 1.0 / h0, rrhoh = 1.0 / rhoh, rhv0 = fr ? 1.0 / hvn0 : 0.0;
     double b_h = 0.0, b_u2 = 0.0, b_v2 = 0.0, b_m1 = 0.0, b_p1 = 0.0, b_chi1 = 0.0;
     double b_rhoh = 0.0, b_gz = 0.0;
     if (P.src2method == -1) {
@@ -427,6 +437,8 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
             for (int k = 0; k < 6; k++) lx[k] = 0.0;
             return;
         }
+        const double rrf = 1.0 / P.rho_f, rrho4 = 1.0 / rho4, rh6 = 1.0 / h6;
+        const double rmu = 1.0 / P.mu, rmuc = 1.0 / muc;
         // ---- reverse of src2's outputs h7, h7 uF, h7 vF, h7 mF, pF, h7 chiF
         double bh7 = ly[0] + ly[1] * q3.u + ly[2] * q3.v + ly[3] * q3.m + ly[5] * q3.chi;
         double bhu7 = 0.0, bhv7 = 0.0, bhm7 = 0.0, bp6 = ly[4], bhchi7 = 0.0;
@@ -448,28 +460,28 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
             b_chi4 += bhchi7 * h7;
             b_u4 += bhu7 * h7;
             b_v4 += bhv7 * h7;
-            b_rhoh += bh7 / P.rho_f;
-            b_hm7 -= bh7 * (P.rho_s - P.rho_f) / P.rho_f;
-            pa.rho_s -= bh7 * hm7 / P.rho_f;
-            pa.rho_f += bh7 * (hm7 - h7) / P.rho_f;
+            b_rhoh += bh7 * rrf;
+            b_hm7 -= bh7 * (P.rho_s - P.rho_f) * rrf;
+            pa.rho_s -= bh7 * hm7 * rrf;
+            pa.rho_f += bh7 * (hm7 - h7) * rrf;
         }
         // hm7 = hm6 E2, E2 = exp(-(dt D rho_f / rhoh))
         const double b_hm6 = b_hm7 * E2;
         const double b_e2 = b_hm7 * hm6 * E2;
-        double b_D = -b_e2 * dt * P.rho_f / rhoh;
-        pa.rho_f -= b_e2 * dt * D / rhoh;
-        b_rhoh -= b_e2 * e2 / rhoh;
+        double b_D = -b_e2 * dt * P.rho_f * rrhoh;
+        pa.rho_f -= b_e2 * dt * D * rrhoh;
+        b_rhoh -= b_e2 * e2 * rrhoh;
         // kr2 = D (rho6 - rho_f) / rhoh
-        b_D += b_kr2 * (qc.rho - P.rho_f) / rhoh;
-        const double b_rho6 = b_kr2 * D / rhoh;
-        pa.rho_f -= b_kr2 * D / rhoh;
-        b_rhoh -= b_kr2 * kr2 / rhoh;
+        b_D += b_kr2 * (qc.rho - P.rho_f) * rrhoh;
+        const double b_rho6 = b_kr2 * D * rrhoh;
+        pa.rho_f -= b_kr2 * D * rrhoh;
+        b_rhoh -= b_kr2 * kr2 * rrhoh;
         // D = -2 g6 pe6, g6 = kperm / (mu h6)
         const double b_pe6 = -2.0 * b_D * g6;
         const double b_g6 = -2.0 * b_D * qc.pe;
-        double b_kperm = b_g6 / (P.mu * h6);
-        pa.mu -= b_g6 * g6 / P.mu;
-        b_h6 -= b_g6 * g6 / h6;
+        double b_kperm = b_g6 * (rmu * rh6);
+        pa.mu -= b_g6 * g6 * rmu;
+        b_h6 -= b_g6 * g6 * rh6;
         // qfix_cmass
         double b_m4 = 0.0, b_pexc = 0.0;
         qcm_rev(P, qc, rhoh, gz, b_h6, 0.0, bp6, b_rho6, b_pe6, b_hu6, b_hv6, b_hm6, b_m4,
@@ -485,17 +497,18 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         b_h += b_peq * P.rho_f * gz;
         b_gz += b_peq * P.rho_f * h0;
         // kr1 = -(zeta 2 kperm / (h0 muc))
-        const double b_zeta = -b_kr1 * 2.0 * s2.kperm / (h0 * muc);
-        b_kperm -= b_kr1 * zeta * 2.0 / (h0 * muc);
-        b_h -= b_kr1 * kr1 / h0;
-        if (mu_on) pa.mu -= b_kr1 * kr1 / muc;
+        const double b_zeta = -b_kr1 * 2.0 * s2.kperm * (rh0 * rmuc);
+        b_kperm -= b_kr1 * zeta * 2.0 * (rh0 * rmuc);
+        b_h -= b_kr1 * kr1 * rh0;
+        if (mu_on) pa.mu -= b_kr1 * kr1 * rmuc;
         // zeta = 3 alphainv / (2 h0) + (rho4 - rho_f) rho_f gz / (4 rho4)
-        double b_ai = b_zeta * 3.0 / (h0 * 2.0);
-        b_h -= b_zeta * (3.0 * s2.alphainv / (h0 * 2.0)) / h0;
-        const double t2 = t2n / (4.0 * rho4);
-        const double b_rho4 = b_zeta * (P.rho_f * gz / (4.0 * rho4) - t2 / rho4);
-        pa.rho_f += b_zeta * (rho4 - 2.0 * P.rho_f) * gz / (4.0 * rho4);
-        b_gz += b_zeta * (rho4 - P.rho_f) * P.rho_f / (4.0 * rho4);
+        double b_ai = b_zeta * 1.5 * rh0;
+        b_h -= b_zeta * (1.5 * s2.alphainv * rh0) * rh0;
+        const double q4 = 0.25 * rrho4;
+        const double t2 = t2n * q4;
+        const double b_rho4 = b_zeta * (P.rho_f * gz * q4 - t2 * rrho4);
+        pa.rho_f += b_zeta * (rho4 - 2.0 * P.rho_f) * gz * q4;
+        b_gz += b_zeta * (rho4 - P.rho_f) * P.rho_f * q4;
         // qfix of mp_update (its h, hu, hv, hm, hchi outputs are not read later)
         double bq2h = 0.0, bq2hu = 0.0, bq2hv = 0.0, bq2hm = 0.0, bq2p = b_p4, bq2hchi = 0.0;
         qfix_rev(P, q2, gz, bq2h, bq2hu, bq2hv, bq2hm, bq2p, bq2hchi, b_u4, b_v4, b_m4, b_chi4,
@@ -503,17 +516,18 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         b_h += bq2h;
         // p3 = p1 - (dt 3 vn3 tanpsi alphainv) / h0
         b_p1 += bq2p;
-        const double b_c3 = -bq2p / h0;
-        b_h += bq2p * c3 / (h0 * h0);
+        const double b_c3 = -bq2p * rh0;
+        b_h += bq2p * c3 * (rh0 * rh0);
         const double b_vn3 = b_c3 * dt * 3.0 * s2.w * s2.alphainv;
         const double b_tanpsi = b_c3 * dt * 3.0 * vn3 * s2.alphainv;
         b_ai += b_c3 * dt * 3.0 * vn3 * s2.w;
         double b_hu3 = bq2hu, b_hv3 = bq2hv;
         const double b_hm3 = bq2hm, b_hchi3 = bq2hchi;
-        b_h -= b_vn3 * vn3 / h0;
+        b_h -= b_vn3 * vn3 * rh0;
         if (sq3 > 0.0) {
-            b_hu3 += b_vn3 / h0 * hu3 / sq3;
-            b_hv3 += b_vn3 / h0 * hv3 / sq3;
+            const double rsq3 = 1.0 / sq3;
+            b_hu3 += b_vn3 * rh0 * hu3 * rsq3;
+            b_hv3 += b_vn3 * rh0 * hv3 * rsq3;
         }
         b_h += b_hu3 * u2 + b_hv3 * v2 + b_hm3 * m1 + b_hchi3 * chi1;
         b_u2 += b_hu3 * h0;
@@ -526,28 +540,28 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
     // ---- friction (src2.f90:160-176)
     double b_hu0 = 0.0, b_hv0 = 0.0, b_u1 = 0.0, b_v1 = 0.0, b_tau1 = 0.0;
     if (fr) {
-        const double b_hu2 = b_u2 / h0, b_hv2 = b_v2 / h0;
-        b_h -= b_u2 * u2 / h0 + b_v2 * v2 / h0;
-        const double b_hvn = b_hu2 * hu0 / hvn0 + b_hv2 * hv0 / hvn0;
-        b_hu0 += b_hu2 * hvn / hvn0;
-        b_hv0 += b_hv2 * hvn / hvn0;
-        double b_hvn0 = -(b_hu2 * hu2 + b_hv2 * hv2) / hvn0;
+        const double b_hu2 = b_u2 * rh0, b_hv2 = b_v2 * rh0;
+        b_h -= b_u2 * u2 * rh0 + b_v2 * v2 * rh0;
+        const double b_hvn = b_hu2 * hu0 * rhv0 + b_hv2 * hv0 * rhv0;
+        b_hu0 += b_hu2 * hvn * rhv0;
+        b_hv0 += b_hv2 * hvn * rhv0;
+        double b_hvn0 = -(b_hu2 * hu2 + b_hv2 * hv2) * rhv0;
         b_h += b_hvn * vn_c;
         const double b_vnc = b_hvn * h0;
         const double b_vnb = b_vnc * E0;
         const double b_ea = b_vnc * vn_b * E0;
-        b_m1 += b_ea * 2.0 * P.mu * dt / (h0 * rhoh);
-        pa.mu -= b_ea * (1.0 - m1) * 2.0 * dt / (h0 * rhoh);
-        b_h -= b_ea * ea / h0;
-        b_rhoh -= b_ea * ea / rhoh;
+        b_m1 += b_ea * 2.0 * P.mu * dt * (rh0 * rrhoh);
+        pa.mu -= b_ea * (1.0 - m1) * 2.0 * dt * (rh0 * rrhoh);
+        b_h -= b_ea * ea * rh0;
+        b_rhoh -= b_ea * ea * rrhoh;
         if (!(0.0 > pre)) {
-            b_hvn0 += b_vnb / h0;
-            b_h -= b_vnb * vn_a / h0;
-            b_tau1 -= b_vnb * dt / rhoh;
-            b_rhoh += b_vnb * dt * s1.tau / (rhoh * rhoh);
+            b_hvn0 += b_vnb * rh0;
+            b_h -= b_vnb * vn_a * rh0;
+            b_tau1 -= b_vnb * dt * rrhoh;
+            b_rhoh += b_vnb * dt * s1.tau * (rrhoh * rrhoh);
         }
-        b_hu0 += b_hvn0 * hu0 / hvn0;
-        b_hv0 += b_hvn0 * hv0 / hvn0;
+        b_hu0 += b_hvn0 * hu0 * rhv0;
+        b_hv0 += b_hvn0 * hv0 * rhv0;
     } else {
         b_u1 += b_u2;
         b_v1 += b_v2;

@@ -1324,10 +1324,10 @@ __global__ __launch_bounds__(64) void k_adjoint(dclaw_adj::AdjArgs A, AdjTiles T
     }
 }
 
-// The same, held to two waves per SIMD (the stages the batched step holds
+// The same, held to WAVES waves per SIMD (the stages the batched step holds
 // there too).
-template <int STAGE>
-__global__ __launch_bounds__(64, 2) void k_adjoint2(dclaw_adj::AdjArgs A, AdjTiles T)
+template <int STAGE, int WAVES>
+__global__ __launch_bounds__(64, WAVES) void k_adjointw(dclaw_adj::AdjArgs A, AdjTiles T)
 {
     A.P = *A.Pd;
     A.dt = *A.dt_dev;
@@ -1637,10 +1637,10 @@ __global__ __launch_bounds__(64) void k_adjb(AdjBatch B)
     }
 }
 
-// The same, held to two waves per SIMD (for the stages whose batched form
-// needs just over half the register file).
-template <int STAGE>
-__global__ __launch_bounds__(64, 2) void k_adjb2(AdjBatch B)
+// The same, held to WAVES waves per SIMD (for the stages whose batched form
+// needs just over 1/WAVES of the register file).
+template <int STAGE, int WAVES>
+__global__ __launch_bounds__(64, WAVES) void k_adjbw(AdjBatch B)
 {
     int k, end, step;
     adj_span(*B.gcount, k, end, step);
@@ -2050,16 +2050,16 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
     hipLaunchKernelGGL(k_adjoint_list, dim3(nblk), dim3(64), 0, st, T, ntiles);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_PREPARE>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_TAUDIR>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint2<ADJ_REC_WAVES>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL((k_adjointw<ADJ_REC_WAVES, 2>), tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_FLUX>, tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REC_SPLIT>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint2<ADJ_UPD_SRC2>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL((k_adjointw<ADJ_UPD_SRC2, 2>), tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_SRC2_SEEDED>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint2<ADJ_REV_SPLIT>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint2<ADJ_REV_FLUX>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL((k_adjointw<ADJ_REV_SPLIT, 2>), tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL((k_adjointw<ADJ_REV_FLUX, 2>), tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_WAVES>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint2<ADJ_REV_TAUDIR>, tg, tb, 0, st, A, T);
-    hipLaunchKernelGGL(k_adjoint2<ADJ_REV_PREPARE>, tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL((k_adjointw<ADJ_REV_TAUDIR, 2>), tg, tb, 0, st, A, T);
+    hipLaunchKernelGGL((k_adjointw<ADJ_REV_PREPARE, 2>), tg, tb, 0, st, A, T);
     hipLaunchKernelGGL(k_adjoint<ADJ_REV_BC>, tg, tb, 0, st, A, T);
     if (np > 0) {
         double* part = work + (size_t)dclaw_adj::W_FIELDS * A.N;
@@ -2148,16 +2148,16 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     hipLaunchKernelGGL(k_adjb_list, dim3((unsigned)nblk), tb, 0, st, B, ntiles);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_PREPARE>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_TAUDIR>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb2<ADJ_REC_WAVES>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL((k_adjbw<ADJ_REC_WAVES, 2>), tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_FLUX>, tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REC_SPLIT>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb2<ADJ_UPD_SRC2>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL((k_adjbw<ADJ_UPD_SRC2, 2>), tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_SRC2_SEEDED>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb2<ADJ_REV_SPLIT>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb2<ADJ_REV_FLUX>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL((k_adjbw<ADJ_REV_SPLIT, 2>), tg, tb, 0, st, B);
+    hipLaunchKernelGGL((k_adjbw<ADJ_REV_FLUX, 2>), tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_WAVES>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb2<ADJ_REV_TAUDIR>, tg, tb, 0, st, B);
-    hipLaunchKernelGGL(k_adjb2<ADJ_REV_PREPARE>, tg, tb, 0, st, B);
+    hipLaunchKernelGGL((k_adjbw<ADJ_REV_TAUDIR, 2>), tg, tb, 0, st, B);
+    hipLaunchKernelGGL((k_adjbw<ADJ_REV_PREPARE, 2>), tg, tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb<ADJ_REV_BC>, tg, tb, 0, st, B);
     if (np > 0) {
         hipLaunchKernelGGL(k_adjb_gp, dim3(ADJ_GP_PARTS, np, npatch), dim3(256), 0, st, B);
