@@ -1441,6 +1441,10 @@ DC_HD void riemanntype_vjp(double hL, double hR, double uL, double uR, double dr
         const double iC = 1.0 / hm + 1.0 / hL, iD = 1.0 / hm + 1.0 / hR;
         const double qC = sqrt((.5 * g) * iC), qD = sqrt((.5 * g) * iD);
         // s1m = uL - (hm - hL) qC - sqrt(g hm), s2m = uR + (hm - hR) qD + sqrt(g hm)
+        // the reverse divides through reciprocals (an FP64 division is a
+        // chain of 11 instructions)
+        const double rhm = 1.0 / hm, rhL = 1.0 / hL, rhR = 1.0 / hR, rh0 = 1.0 / h0;
+        const double rslope = 1.0 / slope, rDL = 1.0 / DL, rDR = 1.0 / DR;
         double hmb = 0.0;
         sg(hm, -s1b + s2b, hmb);
         uRb += s2b;
@@ -1455,23 +1459,23 @@ DC_HD void riemanntype_vjp(double hL, double hR, double uL, double uR, double dr
             const double tc = sqrt_adj(qC, qCb), td = sqrt_adj(qD, qDb);
             gb += tc * .5 * iC + td * .5 * iD;
             const double iCb = tc * .5 * g, iDb = td * .5 * g;
-            hmb += -(iCb / hm) / hm - (iDb / hm) / hm;
-            hLb += -(iCb / hL) / hL;
-            hRb += -(iDb / hR) / hR;
+            hmb += -(iCb * rhm) * rhm - (iDb * rhm) * rhm;
+            hLb += -(iCb * rhL) * rhL;
+            hRb += -(iDb * rhR) * rhR;
         }
         // hm = t^2, t = sq0 - F0 / slope, slope = 2 sq0 dfdh
         const double tb = hmb * 2.0 * t;
         double sq0b = tb;
-        const double F0b = -tb / slope;
-        const double slopeb = tb * q / slope;
+        const double F0b = -tb * rslope;
+        const double slopeb = tb * q * rslope;
         sq0b += slopeb * 2.0 * dfdh;
         const double dfdhb = slopeb * 2.0 * sq0;
         double h0b = 0.0, gLb = dfdhb, gRb = dfdhb, dLb = 0.0, dRb = 0.0, h02b = 0.0;
         {
             // dfdh -= g dL / DL (and the same with R)
-            const double TL = g * dL / DL, TR = g * dR / DR;
-            const double nLb = -dfdhb / DL, nRb = -dfdhb / DR;
-            const double DLb = dfdhb * TL / DL, DRb = dfdhb * TR / DR;
+            const double TL = g * dL * rDL, TR = g * dR * rDR;
+            const double nLb = -dfdhb * rDL, nRb = -dfdhb * rDR;
+            const double DLb = dfdhb * TL * rDL, DRb = dfdhb * TR * rDR;
             gb += nLb * dL + nRb * dR;
             dLb += nLb * g;
             dRb += nRb * g;
@@ -1493,9 +1497,9 @@ DC_HD void riemanntype_vjp(double hL, double hR, double uL, double uR, double dr
             const double ta = sqrt_adj(gL, gLb), tb2 = sqrt_adj(gR, gRb);
             gb += ta * .5 * iA + tb2 * .5 * iB;
             const double iAb = ta * .5 * g, iBb = tb2 * .5 * g;
-            h0b += -(iAb / h0) / h0 - (iBb / h0) / h0;
-            hLb += -(iAb / hL) / hL;
-            hRb += -(iBb / hR) / hR;
+            h0b += -(iAb * rh0) * rh0 - (iBb * rh0) * rh0;
+            hLb += -(iAb * rhL) * rhL;
+            hRb += -(iBb * rhR) * rhR;
         }
         h0b += sqrt_adj(sq0, sq0b);
         hmaxb += h0b;
@@ -1523,19 +1527,21 @@ DC_HD void riemanntype_vjp(double hL, double hR, double uL, double uR, double dr
         // hm = h0 - F0 / slope, slope = (F_max - F0) / dH
         double h0b = hmb;
         const double qb = -hmb;
-        double F0b = qb / slope;
-        const double slopeb = -(qb * q) / slope;
-        const double nb = slopeb / dH;
+        const double rslope = 1.0 / slope, rdH = 1.0 / dH;
+        double F0b = qb * rslope;
+        const double slopeb = -(qb * q) * rslope;
+        const double nb = slopeb * rdH;
         const double F_maxb = nb;
         F0b -= nb;
-        double dHb = -(slopeb * slope) / dH;
+        double dHb = -(slopeb * slope) * rdH;
         // F_max = delu + dH FQ, FQ = sqrt(.5 g (h_max + h_min) / (h_max h_min))
         delub += F_maxb;
         dHb += F_maxb * FQ;
         {
             const double t = sqrt_adj(FQ, F_maxb * dH);
             const double NQ = .5 * g * (h_max + h_min), DQ = h_max * h_min;
-            const double NQb = t / DQ, DQb = -(t * (NQ / DQ)) / DQ;
+            const double rDQ = 1.0 / DQ;
+            const double NQb = t * rDQ, DQb = -(t * (NQ * rDQ)) * rDQ;
             gb += NQb * .5 * (h_max + h_min);
             hmaxb += NQb * .5 * g + DQb * h_min;
             hminb += NQb * .5 * g + DQb * h_max;
@@ -1795,7 +1801,9 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     const double u_i = is_static ? 0.0 : hustar / hstarHLL;
     const double K = gamma * rho * gz;
 
-    // ---- reverse sweep
+    // ---- reverse sweep (it divides through reciprocals: one division per
+    // divisor, an FP64 division being a chain of 11 instructions; the forward
+    // replay above keeps rpn2's own divisions, since they select branches)
     double hLb = 0, hRb = 0, huLb = 0, huRb = 0, hvLb = 0, hvRb = 0, hmLb = 0, hmRb = 0;
     double pLb = 0, pRb = 0, bLb = 0, bRb = 0, uLb = 0, uRb = 0, vLb = 0, vRb = 0;
     double mLb = 0, mRb = 0, chiLb = 0, chiRb = 0, svrLb = 0, svrRb = 0, svtLb = 0, svtRb = 0;
@@ -1838,8 +1846,9 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
         gammab += Kb * rho * gz;
         rhob += Kb * gamma * gz;
         gzb += Kb * gamma * rho;
-        const double hsb = uib / hstarHLL;
-        hstarb -= uib * u_i / hstarHLL;
+        const double rhs = 1.0 / hstarHLL;
+        const double hsb = uib * rhs;
+        hstarb -= uib * u_i * rhs;
         huLb += hsb;
         f11b += hsb;
     }
@@ -1881,9 +1890,9 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     double cb = f13b * beta3 + f23b * beta3 * 2.0 * c;
     double bb = 0.0;
     if (!rarecorrector) {
-        const double D = c - a;
-        const double N1b = beta1b / D, N3b = beta3b / D;
-        const double Db = -(beta1b * beta1) / D - (beta3b * beta3) / D;
+        const double rD = 1.0 / (c - a);
+        const double N1b = beta1b * rD, N3b = beta3b * rD;
+        const double Db = -(beta1b * beta1) * rD - (beta3b * beta3) * rD;
         cb += N1b * del0; del0b += N1b * c; del1b -= N1b;
         ab += beta2b * (c * del0 - del1); cb += beta2b * (a * del0 - del1);
         del0b += beta2b * a * c; del1b -= beta2b * (a + c); del2b += beta2b;
@@ -1891,15 +1900,18 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
         cb += Db;
         ab -= Db;
     } else {
-        const double N1b = beta1b / D1, D1b = -(beta1b * beta1) / D1;
+        const double rD1 = 1.0 / D1;
+        const double N1b = beta1b * rD1, D1b = -(beta1b * beta1) * rD1;
         bb += N1b * (c * del0 - del1); cb += N1b * (b * del0 - del1);
         del0b += N1b * b * c; del1b -= N1b * (b + c); del2b += N1b;
         ab += D1b * (2.0 * a - b - c); bb += D1b * (c - a); cb += D1b * (b - a);
-        const double N2b = beta2b / D2, D2b = -(beta2b * beta2) / D2;
+        const double rD2 = 1.0 / D2;
+        const double N2b = beta2b * rD2, D2b = -(beta2b * beta2) * rD2;
         ab += N2b * (del1 - c * del0); cb += N2b * (del1 - a * del0);
         del0b -= N2b * a * c; del1b += N2b * (a + c); del2b -= N2b;
         ab += D2b * (b - c); bb += D2b * (a - 2.0 * b + c); cb += D2b * (b - a);
-        const double N3b = beta3b / D3, D3b = -(beta3b * beta3) / D3;
+        const double rD3 = 1.0 / D3;
+        const double N3b = beta3b * rD3, D3b = -(beta3b * beta3) * rD3;
         ab += N3b * (b * del0 - del1); bb += N3b * (a * del0 - del1);
         del0b += N3b * a * b; del1b -= N3b * (a + b); del2b += N3b;
         ab += D3b * (b - c); bb += D3b * (a - c); cb += D3b * (2.0 * c - a - b);
@@ -1922,11 +1934,12 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     } else if (tscase == 2) {
         const double sgn = (signbit(ts0) ? -1.0 : 1.0) * (signbit(del2pre - source2dx) ? -1.0 : 1.0);
         const double T12b = 0.5 * tsb * sgn;
-        taudirRb += T12b * tauR / rhoR + T12b * tauL / rhoL;
-        tauRb += T12b * taudirR / rhoR;
-        rhoRb -= T12b * T1 / rhoR;
-        tauLb += T12b * taudirR / rhoL;
-        rhoLb -= T12b * T2 / rhoL;
+        const double rR = 1.0 / rhoR, rL = 1.0 / rhoL;
+        taudirRb += T12b * tauR * rR + T12b * tauL * rL;
+        tauRb += T12b * taudirR * rR;
+        rhoRb -= T12b * T1 * rR;
+        tauLb += T12b * taudirR * rL;
+        rhoLb -= T12b * T2 * rL;
     }
     // del2pre
     {
@@ -1938,13 +1951,13 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
         const double KGb = del2preb * (hR * hR - hL * hL);
         kappab += KGb * 0.5 * gz;
         gzb += KGb * 0.5 * kappa;
-        const double dp = pR - pL;
-        const double E = (1.0 - kappa) * h * dp / rho;
-        kappab -= del2preb * h * dp / rho;
-        hb += del2preb * (1.0 - kappa) * dp / rho;
-        pRb += del2preb * (1.0 - kappa) * h / rho;
-        pLb -= del2preb * (1.0 - kappa) * h / rho;
-        rhob -= del2preb * E / rho;
+        const double dp = pR - pL, rr = 1.0 / rho;
+        const double E = (1.0 - kappa) * h * dp * rr;
+        kappab -= del2preb * h * dp * rr;
+        hb += del2preb * (1.0 - kappa) * dp * rr;
+        pRb += del2preb * (1.0 - kappa) * h * rr;
+        pLb -= del2preb * (1.0 - kappa) * h * rr;
+        rhob -= del2preb * E * rr;
     }
     huRb += del1b;
     huLb -= del1b;
@@ -1955,25 +1968,28 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     switch (ddsrc) {
     case 0: delbb -= deldelhb; break;
     case 1: {
-        const double v = delb * gz * hbar / s1s2bar;
-        delbb += deldelhb * gz * hbar / s1s2bar;
-        gzb += deldelhb * delb * hbar / s1s2bar;
-        hbarb += deldelhb * delb * gz / s1s2bar;
-        s1s2barb -= deldelhb * v / s1s2bar;
+        const double r = 1.0 / s1s2bar;
+        const double v = delb * gz * hbar * r;
+        delbb += deldelhb * gz * hbar * r;
+        gzb += deldelhb * delb * hbar * r;
+        hbarb += deldelhb * delb * gz * r;
+        s1s2barb -= deldelhb * v * r;
         break;
     }
     case 2: {
-        const double v = hstarHLL * dE / sE2, dEb = deldelhb * hstarHLL / sE2;
-        hstarb += deldelhb * dE / sE2;
-        sE2b += dEb - deldelhb * v / sE2;
+        const double r = 1.0 / sE2;
+        const double v = hstarHLL * dE * r, dEb = deldelhb * hstarHLL * r;
+        hstarb += deldelhb * dE * r;
+        sE2b += dEb - deldelhb * v * r;
         sE1b -= dEb;
         break;
     }
     case 3: {
-        const double v = hstarHLL * dE / sE1, dEb = deldelhb * hstarHLL / sE1;
-        hstarb += deldelhb * dE / sE1;
+        const double r = 1.0 / sE1;
+        const double v = hstarHLL * dE * r, dEb = deldelhb * hstarHLL * r;
+        hstarb += deldelhb * dE * r;
         sE2b += dEb;
-        sE1b += -dEb - deldelhb * v / sE1;
+        sE1b += -dEb - deldelhb * v * r;
         break;
     }
     case 4: hLb -= deldelhb; break;
@@ -1981,13 +1997,14 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     }
     switch (s2src) {
     case 1: {
-        const double r = s1s2tilde / s1s2bar;
-        const double w = gz * hbar * delb * s1s2tilde / s1s2bar;
+        const double rb = 1.0 / s1s2bar;
+        const double r = s1s2tilde * rb;
+        const double w = gz * hbar * delb * s1s2tilde * rb;
         gzb -= s2dxb * hbar * delb * r;
         hbarb -= s2dxb * gz * delb * r;
         delbb -= s2dxb * gz * hbar * r;
-        s1s2tildeb -= s2dxb * gz * hbar * delb / s1s2bar;
-        s1s2barb += s2dxb * w / s1s2bar;
+        s1s2tildeb -= s2dxb * gz * hbar * delb * rb;
+        s1s2barb += s2dxb * w * rb;
         break;
     }
     case 2:
@@ -2033,8 +2050,8 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     if (sE2 > rt.s1m) sE2b += sw2b;
     else s1mb += sw2b;
     if (hsQ > 0.0) {
-        const double Dq = sE2 - sE1;
-        const double Nb = hstarb / Dq, Dqb = -(hstarb * hsQ) / Dq;
+        const double rDq = 1.0 / (sE2 - sE1);
+        const double Nb = hstarb * rDq, Dqb = -(hstarb * hsQ) * rDq;
         huLb += Nb;
         huRb -= Nb;
         sE2b += Nb * hR + Dqb;
@@ -2062,8 +2079,8 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
         hLb += tc * geps * 0.5;
     }
     {
-        const double Du = sqhR + sqhL;
-        const double Nb = uhatb / Du, Dub = -(uhatb * uhat) / Du;
+        const double rDu = 1.0 / (sqhR + sqhL);
+        const double Nb = uhatb * rDu, Dub = -(uhatb * uhat) * rDu;
         uLb += Nb * sqhL;
         uRb += Nb * sqhR;
         hLb += sqrt_adj(sqhL, Nb * uL + Dub);
@@ -2074,16 +2091,19 @@ DC_HD void rpn2_vjp(const AdjArgs& A, const Rpn2F& f, const double* x, const dou
     const double epsb = gepsb * gz;
     kappab += epsb * (1.0 - gamma);
     gammab += epsb * (1.0 - kappa);
-    if (m1) {
-        rhofb += gammab / rho + gammaLb / rhoL + gammaRb / rhoR;
-        rhob -= gammab * gamma / rho;
-        rhoLb -= gammaLb * gammaL / rhoL;
-        rhoRb -= gammaRb * gammaR / rhoR;
-    } else {
-        rhofb += 0.25 * (gammab / rho + gammaLb / rhoL + gammaRb / rhoR);
-        rhob += gammab * (0.75 - gamma) / rho;
-        rhoLb += gammaLb * (0.75 - gammaL) / rhoL;
-        rhoRb += gammaRb * (0.75 - gammaR) / rhoR;
+    {
+        const double rr = 1.0 / rho, rL = 1.0 / rhoL, rR = 1.0 / rhoR;
+        if (m1) {
+            rhofb += gammab * rr + gammaLb * rL + gammaRb * rR;
+            rhob -= gammab * gamma * rr;
+            rhoLb -= gammaLb * gammaL * rL;
+            rhoRb -= gammaRb * gammaR * rR;
+        } else {
+            rhofb += 0.25 * (gammab * rr + gammaLb * rL + gammaRb * rR);
+            rhob += gammab * (0.75 - gamma) * rr;
+            rhoLb += gammaLb * (0.75 - gammaL) * rL;
+            rhoRb += gammaRb * (0.75 - gammaR) * rR;
+        }
     }
     rhoLb += 0.5 * rhob;
     rhoRb += 0.5 * rhob;
