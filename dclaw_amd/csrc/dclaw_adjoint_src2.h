@@ -350,19 +350,25 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         for (int k = 0; k < 6; k++) lx[k] = ly[k];
         return;
     }
-    // ---- forward values
-    QfFwd q1;
-    qfix_fwd(P, h0, hu0, hv0, hm0, p0, hchi0, gz, q1);
-    const double u1 = q1.u, v1 = q1.v, m1 = q1.m, chi1 = q1.chi, p1 = q1.p;
-    SvFwd s1;
-    setvars_fwd(P, h0, u1, v1, m1, p1, chi1, gz, s1);
+    // ---- forward values.  The first qfix and setvars are evaluated again
+    // for their reverses at the end (fewer values held across the whole
+    // reverse: registers, not scratch, at two waves per SIMD)
+    double u1, v1, m1, chi1, p1, tau1;
+    {
+        QfFwd q1;
+        qfix_fwd(P, h0, hu0, hv0, hm0, p0, hchi0, gz, q1);
+        u1 = q1.u; v1 = q1.v; m1 = q1.m; chi1 = q1.chi; p1 = q1.p;
+        SvFwd s1;
+        setvars_fwd(P, h0, u1, v1, m1, p1, chi1, gz, s1);
+        tau1 = s1.tau;
+    }
     const double hvn0 = sqrt(hu0 * hu0 + hv0 * hv0);
     const bool fr = hvn0 > 0.0;
     double vn_a = 0.0, pre = 0.0, vn_b = 0.0, ea = 0.0, E0 = 0.0, vn_c = 0.0, hvn = 0.0;
     double hu2 = hu0, hv2 = hv0, u2 = u1, v2 = v1;
     if (fr) {
         vn_a = hvn0 / h0;
-        pre = vn_a - dt * s1.tau / rhoh;
+        pre = vn_a - dt * tau1 / rhoh;
         vn_b = 0.0 > pre ? 0.0 : pre;
         ea = -((1.0 - m1) * 2.0 * P.mu * dt / (h0 * rhoh));
         E0 = ::dcm_exp(ea);
@@ -389,21 +395,28 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         b_p1 += ly[4];
     } else {
         // ---- mp_update_relax forward
-        SvFwd s2;
-        setvars_fwd(P, h0, u2, v2, m1, p1, chi1, gz, s2);
+        double s2w, s2ai, s2kp;   // the second setvars' tanpsi, alphainv, kperm
+        {
+            SvFwd s2;
+            setvars_fwd(P, h0, u2, v2, m1, p1, chi1, gz, s2);
+            s2w = s2.w; s2ai = s2.alphainv; s2kp = s2.kperm;
+        }
         const double hu3 = h0 * u2, hv3 = h0 * v2, hm3 = h0 * m1, hchi3 = h0 * chi1;
         const double sq3 = sqrt(hu3 * hu3 + hv3 * hv3);
         const double vn3 = sq3 / h0;
-        const double c3 = dt * 3.0 * vn3 * s2.w * s2.alphainv;
+        const double c3 = dt * 3.0 * vn3 * s2w * s2ai;
         const double p3 = p1 - c3 / h0;
-        QfFwd q2;
-        qfix_fwd(P, h0, hu3, hv3, hm3, p3, hchi3, gz, q2);
-        const double u4 = q2.u, v4 = q2.v, m4 = q2.m, chi4 = q2.chi, rho4 = q2.rho, p4 = q2.p;
+        double u4, v4, m4, chi4, rho4, p4;
+        {
+            QfFwd q2;
+            qfix_fwd(P, h0, hu3, hv3, hm3, p3, hchi3, gz, q2);
+            u4 = q2.u; v4 = q2.v; m4 = q2.m; chi4 = q2.chi; rho4 = q2.rho; p4 = q2.p;
+        }
         const double t2n = (rho4 - P.rho_f) * P.rho_f * gz;
-        const double zeta = 3.0 * s2.alphainv / (h0 * 2.0) + t2n / (4.0 * rho4);
+        const double zeta = 3.0 * s2ai / (h0 * 2.0) + t2n / (4.0 * rho4);
         const bool mu_on = P.mu > 1e-16;
         const double muc = mu_on ? P.mu : 1e-16;
-        const double kr1 = -(zeta * 2.0 * s2.kperm / (h0 * muc));
+        const double kr1 = -(zeta * 2.0 * s2kp / (h0 * muc));
         const double p_eq = P.rho_f * h0 * gz;
         const double E1 = ::dcm_exp(kr1 * dt);
         const double p5 = p_eq + (p4 - p_eq) * E1;
@@ -411,7 +424,7 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         QcFwd qc;
         qcm_fwd(P, m4, pexc, u4, v4, rhoh, gz, qc);
         const double h6 = qc.h, hu6 = h6 * u4, hv6 = h6 * v4, hm6 = h6 * qc.m1;
-        const double g6 = s2.kperm / (P.mu * h6);
+        const double g6 = s2kp / (P.mu * h6);
         const double D = -(2.0 * g6 * qc.pe);
         const double kr2 = D * (qc.rho - P.rho_f) / rhoh;
         const double e2 = -(dt * D * P.rho_f / rhoh);
@@ -497,13 +510,13 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         b_h += b_peq * P.rho_f * gz;
         b_gz += b_peq * P.rho_f * h0;
         // kr1 = -(zeta 2 kperm / (h0 muc))
-        const double b_zeta = -b_kr1 * 2.0 * s2.kperm * (rh0 * rmuc);
+        const double b_zeta = -b_kr1 * 2.0 * s2kp * (rh0 * rmuc);
         b_kperm -= b_kr1 * zeta * 2.0 * (rh0 * rmuc);
         b_h -= b_kr1 * kr1 * rh0;
         if (mu_on) pa.mu -= b_kr1 * kr1 * rmuc;
         // zeta = 3 alphainv / (2 h0) + (rho4 - rho_f) rho_f gz / (4 rho4)
         double b_ai = b_zeta * 1.5 * rh0;
-        b_h -= b_zeta * (1.5 * s2.alphainv * rh0) * rh0;
+        b_h -= b_zeta * (1.5 * s2ai * rh0) * rh0;
         const double q4 = 0.25 * rrho4;
         const double t2 = t2n * q4;
         const double b_rho4 = b_zeta * (P.rho_f * gz * q4 - t2 * rrho4);
@@ -511,6 +524,8 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         b_gz += b_zeta * (rho4 - P.rho_f) * P.rho_f * q4;
         // qfix of mp_update (its h, hu, hv, hm, hchi outputs are not read later)
         double bq2h = 0.0, bq2hu = 0.0, bq2hv = 0.0, bq2hm = 0.0, bq2p = b_p4, bq2hchi = 0.0;
+        QfFwd q2;   // evaluated again (the forward kept only its outputs)
+        qfix_fwd(P, h0, hu3, hv3, hm3, p3, hchi3, gz, q2);
         qfix_rev(P, q2, gz, bq2h, bq2hu, bq2hv, bq2hm, bq2p, bq2hchi, b_u4, b_v4, b_m4, b_chi4,
                  b_rho4, b_gz, pa);
         b_h += bq2h;
@@ -518,9 +533,9 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         b_p1 += bq2p;
         const double b_c3 = -bq2p * rh0;
         b_h += bq2p * c3 * (rh0 * rh0);
-        const double b_vn3 = b_c3 * dt * 3.0 * s2.w * s2.alphainv;
-        const double b_tanpsi = b_c3 * dt * 3.0 * vn3 * s2.alphainv;
-        b_ai += b_c3 * dt * 3.0 * vn3 * s2.w;
+        const double b_vn3 = b_c3 * dt * 3.0 * s2w * s2ai;
+        const double b_tanpsi = b_c3 * dt * 3.0 * vn3 * s2ai;
+        b_ai += b_c3 * dt * 3.0 * vn3 * s2w;
         double b_hu3 = bq2hu, b_hv3 = bq2hv;
         const double b_hm3 = bq2hm, b_hchi3 = bq2hchi;
         b_h -= b_vn3 * vn3 * rh0;
@@ -534,6 +549,8 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         b_v2 += b_hv3 * h0;
         b_m1 += b_hm3 * h0;
         b_chi1 += b_hchi3 * h0;
+        SvFwd s2;   // evaluated again
+        setvars_fwd(P, h0, u2, v2, m1, p1, chi1, gz, s2);
         setvars_rev(P, h0, u2, v2, m1, chi1, gz, s2, 0.0, b_kperm, b_ai, 0.0, b_tanpsi, 0.0,
                     b_h, b_u2, b_v2, b_m1, b_p1, b_chi1, b_gz, pa);
     }
@@ -558,7 +575,7 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
             b_hvn0 += b_vnb * rh0;
             b_h -= b_vnb * vn_a * rh0;
             b_tau1 -= b_vnb * dt * rrhoh;
-            b_rhoh += b_vnb * dt * s1.tau * (rrhoh * rrhoh);
+            b_rhoh += b_vnb * dt * tau1 * (rrhoh * rrhoh);
         }
         b_hu0 += b_hvn0 * hu0 * rhv0;
         b_hv0 += b_hvn0 * hv0 * rhv0;
@@ -566,10 +583,14 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         b_u1 += b_u2;
         b_v1 += b_v2;
     }
+    SvFwd s1;
+    setvars_fwd(P, h0, u1, v1, m1, p1, chi1, gz, s1);
     setvars_rev(P, h0, u1, v1, m1, chi1, gz, s1, 0.0, 0.0, 0.0, 0.0, 0.0, b_tau1, b_h, b_u1,
                 b_v1, b_m1, b_p1, b_chi1, b_gz, pa);
     // ---- the first qfix (its hm, hchi, rho outputs are not read later)
     double bhm = 0.0, bhchi = 0.0, bp = b_p1;
+    QfFwd q1;
+    qfix_fwd(P, h0, hu0, hv0, hm0, p0, hchi0, gz, q1);
     qfix_rev(P, q1, gz, b_h, b_hu0, b_hv0, bhm, bp, bhchi, b_u1, b_v1, b_m1, b_chi1, 0.0, b_gz,
              pa);
     // ---- rhoh = hm0 rho_s + (h0 - hm0) rho_f
