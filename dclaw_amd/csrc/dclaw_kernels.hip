@@ -1454,7 +1454,8 @@ struct AdjBatch {
     const double* dt;              // per member
     long long* cell_prefix;        // npatch + 1
     long long* tile_prefix;        // npatch + 1
-    int* glist;                    // active (patch, tile) pairs as global tile ids, in order
+    int* glist;                    // active tiles in global tile order: the tile within its patch
+    int* gmem;                     // and its patch (parallel to glist)
     int* gcount;
     int* bcount;                   // marks per block of 64 global tiles, then list offsets
 };
@@ -1604,19 +1605,66 @@ __global__ __launch_bounds__(64) void k_adjb_count_marks(AdjBatch B, long long n
 
 // Compaction over every patch's marks: each marked tile goes to its patch's
 // own list (for the next call's clearing; any order) and to the global list,
-// in global tile order (B.bcount holds each block's offset).
+// in global tile order (B.bcount holds each block's offset).  The lanes whose
+// tiles belong to the wave's first marked patch (all of them but at a
+// patch boundary) take their slots of that patch's list with one atomic.
 __global__ __launch_bounds__(64) void k_adjb_list(AdjBatch B, long long ntiles)
 {
     const long long g = (long long)blockIdx.x * 64 + threadIdx.x;
     AdjTiles T;
-    int t = 0;
-    const bool f = g < ntiles && adjb_marked(B, g, T, t);
+    int t = 0, m = -1;
+    bool f = false;
+    if (g < ntiles) {
+        m = adj_find(B.tile_prefix, B.npatch, g);
+        const dclaw_adj_patch& p = B.pt[m];
+        T = adj_tiles(p.work, p.mx, p.my, dclaw_adj::plane_size(p.mx, p.my));
+        t = (int)(g - B.tile_prefix[m]);
+        f = T.flags[t] != 0;
+    }
     const unsigned long long b = __ballot(f);
+    if (b == 0ull) return;
+    const int lead = __ffsll((long long)b) - 1;
+    const int m0 = __shfl(m, lead);
+    const bool agg = f && m == m0;
+    const unsigned long long ba = __ballot(agg);
+    const unsigned long long below = (1ull << threadIdx.x) - 1ull;
+    int base = 0;
+    if ((int)threadIdx.x == lead) base = atomicAdd(T.count, (int)__popcll(ba));
+    base = __shfl(base, lead);
     if (!f) return;
     T.flags[t] = 0;
-    T.list[atomicAdd(T.count, 1)] = t;
-    const unsigned long long below = b & ((1ull << threadIdx.x) - 1ull);
-    B.glist[B.bcount[blockIdx.x] + (int)__popcll(below)] = (int)g;
+    if (agg) T.list[base + (int)__popcll(ba & below)] = t;
+    else T.list[atomicAdd(T.count, 1)] = t;
+    const int slot = B.bcount[blockIdx.x] + (int)__popcll(b & below);
+    B.glist[slot] = t;
+    B.gmem[slot] = m;
+}
+
+// The tile loop of a batched stage pass.  Each list entry carries its patch,
+// and the next entry is requested before the current tile's work, so a tile
+// starts without a chain of dependent lookups.
+template <int STAGE>
+__device__ __forceinline__ void adjb_tiles(const AdjBatch& B)
+{
+    int k, end, step;
+    adj_span(*B.gcount, k, end, step);
+    if (k >= end) return;
+    int t = B.glist[k], m = B.gmem[k];
+    for (; k < end; k += step) {
+        const int kn = k + step;
+        int tn = 0, mn = 0;
+        if (kn < end) {
+            tn = B.glist[kn];
+            mn = B.gmem[kn];
+        }
+        AdjTiles T;
+        const dclaw_adj::AdjArgs A = adj_member(B, m, T);
+        const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
+        int i, j;
+        if (adj_tile_cell(A, T, t, i, j)) adj_stage<STAGE>(A, Pz, i, j);
+        t = tn;
+        m = mn;
+    }
 }
 
 template <int STAGE>
@@ -1624,17 +1672,7 @@ __global__ __launch_bounds__(64) void k_adjb(AdjBatch B)
 {
     if constexpr (STAGE == ADJ_REV_SRC2_SEEDED)
         if (B.gcount[1] == 0) return;
-    int k, end, step;
-    adj_span(*B.gcount, k, end, step);
-    for (; k < end; k += step) {
-        const long long g = B.glist[k];
-        const int m = adj_find(B.tile_prefix, B.npatch, g);
-        AdjTiles T;
-        const dclaw_adj::AdjArgs A = adj_member(B, m, T);
-        const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
-        int i, j;
-        if (adj_tile_cell(A, T, (int)(g - B.tile_prefix[m]), i, j)) adj_stage<STAGE>(A, Pz, i, j);
-    }
+    adjb_tiles<STAGE>(B);
 }
 
 // The same, held to WAVES waves per SIMD (for the stages whose batched form
@@ -1642,17 +1680,7 @@ __global__ __launch_bounds__(64) void k_adjb(AdjBatch B)
 template <int STAGE, int WAVES>
 __global__ __launch_bounds__(64, WAVES) void k_adjbw(AdjBatch B)
 {
-    int k, end, step;
-    adj_span(*B.gcount, k, end, step);
-    for (; k < end; k += step) {
-        const long long g = B.glist[k];
-        const int m = adj_find(B.tile_prefix, B.npatch, g);
-        AdjTiles T;
-        const dclaw_adj::AdjArgs A = adj_member(B, m, T);
-        const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
-        int i, j;
-        if (adj_tile_cell(A, T, (int)(g - B.tile_prefix[m]), i, j)) adj_stage<STAGE>(A, Pz, i, j);
-    }
+    adjb_tiles<STAGE>(B);
 }
 
 __global__ __launch_bounds__(256) void k_adjb_gp(AdjBatch B)
@@ -2088,9 +2116,9 @@ size_t dclaw_adjoint_batch_doubles(const dclaw_adj_patch* patches_host, int npat
 {
     if (!patches_host || npatch < 1) return 0;
     const long long t = adj_batch_tiles(patches_host, npatch, nullptr);
-    // cell and tile prefixes, the global counter, then the global list and
-    // the per-block counts (ints)
-    return (size_t)(2 * (npatch + 1) + 1) + (size_t)((t + (t + 63) / 64 + 1) / 2);
+    // cell and tile prefixes, the global counter, then the global list, the
+    // per-block counts and the list's patch indices (ints)
+    return (size_t)(2 * (npatch + 1) + 1) + (size_t)((2 * t + (t + 63) / 64 + 1) / 2);
 }
 
 int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_patch* patches_host,
@@ -2133,9 +2161,10 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     B.cell_prefix = (long long*)batch_work;
     B.tile_prefix = B.cell_prefix + (npatch + 1);
     B.gcount = (int*)(B.tile_prefix + (npatch + 1));   // [0] active tiles, [1] see k_adjb_prefix
+    const long long nblk = (ntiles + 63) / 64;
     B.glist = (int*)(batch_work + 2 * (npatch + 1) + 1);
     B.bcount = B.glist + ntiles;
-    const long long nblk = (ntiles + 63) / 64;
+    B.gmem = B.bcount + nblk;
     hipStream_t st = (hipStream_t)stream;
     const dim3 tg(adj_grid(ntiles)), tb(64);
     hipLaunchKernelGGL(k_adjb_prefix, dim3(1), dim3(1024), 0, st, B);
