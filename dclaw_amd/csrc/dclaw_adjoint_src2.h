@@ -336,6 +336,35 @@ DC_HD bool src2_vjp_covers(const dclaw_params& P)
     return P.curvature == 0;
 }
 
+// src2's Coulomb friction (src2.f90:160-176) on the qfix'd state: the
+// forward values its reverse reads.
+struct FrFwd {
+    double hvn0, vn_a, pre, vn_b, ea, E0, vn_c, hvn, hu2, hv2, u2, v2;
+    bool fr;
+};
+
+DC_HD void friction_fwd(const dclaw_params& P, double dt, double h0, double hu0, double hv0,
+                        double m1, double u1, double v1, double tau1, double rhoh, FrFwd& f)
+{
+    f.hvn0 = sqrt(hu0 * hu0 + hv0 * hv0);
+    f.fr = f.hvn0 > 0.0;
+    f.vn_a = f.pre = f.vn_b = f.ea = f.E0 = f.vn_c = f.hvn = 0.0;
+    f.hu2 = hu0; f.hv2 = hv0; f.u2 = u1; f.v2 = v1;
+    if (f.fr) {
+        f.vn_a = f.hvn0 / h0;
+        f.pre = f.vn_a - dt * tau1 / rhoh;
+        f.vn_b = 0.0 > f.pre ? 0.0 : f.pre;
+        f.ea = -((1.0 - m1) * 2.0 * P.mu * dt / (h0 * rhoh));
+        f.E0 = ::dcm_exp(f.ea);
+        f.vn_c = f.vn_b * f.E0;
+        f.hvn = h0 * f.vn_c;
+        f.hu2 = f.hvn * hu0 / f.hvn0;
+        f.hv2 = f.hvn * hv0 / f.hvn0;
+        f.u2 = f.hu2 / h0;
+        f.v2 = f.hv2 / h0;
+    }
+}
+
 // x: the cell's state (h > dry_tolerance) before src2; ly: adjoints of the
 // six outputs h, hu, hv, hm, pb, hchi; lx: adjoints of the six inputs (bdif
 // is the caller's: it passes through); pa: parameter adjoints (added).
@@ -362,27 +391,20 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
         setvars_fwd(P, h0, u1, v1, m1, p1, chi1, gz, s1);
         tau1 = s1.tau;
     }
-    const double hvn0 = sqrt(hu0 * hu0 + hv0 * hv0);
-    const bool fr = hvn0 > 0.0;
-    double vn_a = 0.0, pre = 0.0, vn_b = 0.0, ea = 0.0, E0 = 0.0, vn_c = 0.0, hvn = 0.0;
-    double hu2 = hu0, hv2 = hv0, u2 = u1, v2 = v1;
-    if (fr) {
-        vn_a = hvn0 / h0;
-        pre = vn_a - dt * tau1 / rhoh;
-        vn_b = 0.0 > pre ? 0.0 : pre;
-        ea = -((1.0 - m1) * 2.0 * P.mu * dt / (h0 * rhoh));
-        E0 = ::dcm_exp(ea);
-        vn_c = vn_b * E0;
-        hvn = h0 * vn_c;
-        hu2 = hvn * hu0 / hvn0;
-        hv2 = hvn * hv0 / hvn0;
-        u2 = hu2 / h0;
-        v2 = hv2 / h0;
+    // friction (evaluated again for its reverse at the end)
+    double u2, v2;
+    bool fr;
+    {
+        FrFwd f;
+        friction_fwd(P, dt, h0, hu0, hv0, m1, u1, v1, tau1, rhoh, f);
+        u2 = f.u2;
+        v2 = f.v2;
+        fr = f.fr;
     }
     // the reverse divides through reciprocals (one division each; an FP64
     // division is a chain of 11 instructions); the forward values above keep
     // src2's own divisions, since they select its branches
-    const double rh0 = 1.0 / h0, rrhoh = 1.0 / rhoh, rhv0 = fr ? 1.0 / hvn0 : 0.0;
+    const double rh0 = 1.0 / h0, rrhoh = 1.0 / rhoh;
     double b_h = 0.0, b_u2 = 0.0, b_v2 = 0.0, b_m1 = 0.0, b_p1 = 0.0, b_chi1 = 0.0;
     double b_rhoh = 0.0, b_gz = 0.0;
     if (P.src2method == -1) {
@@ -557,6 +579,11 @@ DC_HD void src2_vjp(const dclaw_params& P, double dt, const double* x, const dou
     // ---- friction (src2.f90:160-176)
     double b_hu0 = 0.0, b_hv0 = 0.0, b_u1 = 0.0, b_v1 = 0.0, b_tau1 = 0.0;
     if (fr) {
+        FrFwd f;
+        friction_fwd(P, dt, h0, hu0, hv0, m1, u1, v1, tau1, rhoh, f);
+        const double hvn0 = f.hvn0, vn_a = f.vn_a, pre = f.pre, vn_b = f.vn_b, ea = f.ea;
+        const double E0 = f.E0, vn_c = f.vn_c, hvn = f.hvn, hu2 = f.hu2, hv2 = f.hv2;
+        const double rhv0 = 1.0 / hvn0;
         const double b_hu2 = b_u2 * rh0, b_hv2 = b_v2 * rh0;
         b_h -= b_u2 * u2 * rh0 + b_v2 * v2 * rh0;
         const double b_hvn = b_hu2 * hu0 * rhv0 + b_hv2 * hv0 * rhv0;
