@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -1748,12 +1749,21 @@ int check_patch(int mx, int my, int maux)
     return DCLAW_OK;
 }
 
-// Host drop-in state (library-owned device buffers, grown on demand).
-struct HostCtx {
-    std::mutex mu;
+// Host drop-in state.  The parameters are process-wide configuration (the
+// reference's module globals: dclaw_set_params after set_dig); each calling
+// thread has its own device buffers and HIP stream (HostCtx), so threads
+// that step different patches at once (AMRClaw advanc's OpenMP loop over
+// patches) run concurrently.  dclaw_set_params waits for calls in progress.
+struct HostCfg {
+    std::shared_mutex mu;
     dclaw_params params;
     bool have_params = false;
     dclaw_params* d_params = nullptr;
+};
+HostCfg g_cfg;
+
+struct HostCtx {
+    hipStream_t st = nullptr;
     double* d_scalars = nullptr;   // [0] dt, [1] cfl
     double* d_qa = nullptr;
     double* d_qb = nullptr;
@@ -1767,7 +1777,26 @@ struct HostCtx {
     size_t cap_curv = 0;
     int* d_nan = nullptr;                                        // check4nans result
 };
-HostCtx g_host;
+
+// The calling thread's context, created on its first call and kept until the
+// process ends (a thread pool keeps its threads; the registry holds them).
+int host_ctx(HostCtx** out)
+{
+    static std::mutex reg_mu;
+    static std::vector<HostCtx*> reg;
+    thread_local HostCtx* mine = nullptr;
+    if (!mine) {
+        HostCtx* c = new HostCtx;
+        HIP_TRY(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        HIP_TRY(hipMalloc(&c->d_scalars, 2 * sizeof(double)));
+        HIP_TRY(hipMalloc(&c->d_nan, sizeof(int)));
+        std::lock_guard<std::mutex> lk(reg_mu);
+        reg.push_back(c);
+        mine = c;
+    }
+    *out = mine;
+    return DCLAW_OK;
+}
 
 int grow(double** p, size_t* cap, size_t need)
 {
@@ -2820,14 +2849,23 @@ int dclaw_set_params(const dclaw_params* p)
     if (!p) return fail(DCLAW_EINVAL, "null params");
     int rc = validate_params(*p);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(g_host.mu);
-    if (!g_host.d_params) {
-        HIP_TRY(hipMalloc(&g_host.d_params, sizeof(dclaw_params)));
-        HIP_TRY(hipMalloc(&g_host.d_scalars, 2 * sizeof(double)));
-    }
-    HIP_TRY(hipMemcpy(g_host.d_params, p, sizeof(dclaw_params), hipMemcpyHostToDevice));
-    g_host.params = *p;
-    g_host.have_params = true;
+    std::unique_lock<std::shared_mutex> lk(g_cfg.mu);
+    if (!g_cfg.d_params) HIP_TRY(hipMalloc(&g_cfg.d_params, sizeof(dclaw_params)));
+    HIP_TRY(hipMemcpy(g_cfg.d_params, p, sizeof(dclaw_params), hipMemcpyHostToDevice));
+    g_cfg.params = *p;
+    g_cfg.have_params = true;
+    return DCLAW_OK;
+}
+
+// Host <-> device copies on the calling thread's stream.
+static int h2d(HostCtx* c, double* d, const double* h, size_t n)
+{
+    HIP_TRY(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    return DCLAW_OK;
+}
+static int d2h(HostCtx* c, double* h, const double* d, size_t n)
+{
+    HIP_TRY(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     return DCLAW_OK;
 }
 
@@ -2838,83 +2876,84 @@ static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int ma
     int rc = check_patch(mx, my, maux);
     if (rc) return rc;
     if (maux < 6) return fail(DCLAW_EINVAL, "the host drop-in needs maux >= 6");
-    std::lock_guard<std::mutex> lk(g_host.mu);
-    if (!g_host.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
-    if (g_host.params.entrainment && maux < 7)
+    std::shared_lock<std::shared_mutex> lk(g_cfg.mu);
+    if (!g_cfg.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
+    if (g_cfg.params.entrainment && maux < 7)
         return fail(DCLAW_EINVAL, "entrainment needs maux >= 7 (aux(7) holds h_e)");
+    HostCtx* c;
+    if ((rc = host_ctx(&c))) return rc;
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
     const size_t na = (size_t)maux * (mx + 4) * (my + 4);
-    if ((rc = grow(&g_host.d_qa, &g_host.cap_qa, nq))) return rc;
-    if ((rc = grow(&g_host.d_qb, &g_host.cap_qb, nq))) return rc;
-    if ((rc = grow(&g_host.d_aux, &g_host.cap_aux, na))) return rc;
+    if ((rc = grow(&c->d_qa, &c->cap_qa, nq))) return rc;
+    if ((rc = grow(&c->d_qb, &c->cap_qb, nq))) return rc;
+    if ((rc = grow(&c->d_aux, &c->cap_aux, na))) return rc;
     double scal[2] = {dt, 0.0};
-    HIP_TRY(hipMemcpy(g_host.d_qa, q, nq * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(g_host.d_aux, aux, na * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = h2d(c, c->d_qa, q, nq)) || (rc = h2d(c, c->d_aux, aux, na)) ||
+        (rc = h2d(c, c->d_scalars, scal, 2)))
+        return rc;
     if (flux) {
         // fm, fp, gm, gp of the step: step2 on the b4step2 state of a copy
         // (the fused step below computes the same values internally)
         for (int k = 0; k < 4; k++)
-            if ((rc = grow(&g_host.d_flux[k], &g_host.cap_flux[k], nq))) return rc;
-        if ((rc = grow(&g_host.d_qc, &g_host.cap_qc, nq))) return rc;
-        HIP_TRY(hipMemcpy(g_host.d_qc, g_host.d_qa, nq * sizeof(double),
-                          hipMemcpyDeviceToDevice));
-        if ((rc = dclaw_b4step2_dev(g_host.d_qc, g_host.d_aux, maux, mx, my, dx, dy,
-                                    g_host.d_params, nullptr, nullptr)))
+            if ((rc = grow(&c->d_flux[k], &c->cap_flux[k], nq))) return rc;
+        if ((rc = grow(&c->d_qc, &c->cap_qc, nq))) return rc;
+        HIP_TRY(hipMemcpyAsync(c->d_qc, c->d_qa, nq * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->st));
+        if ((rc = dclaw_b4step2_dev(c->d_qc, c->d_aux, maux, mx, my, dx, dy, g_cfg.d_params,
+                                    nullptr, c->st)))
             return rc;
-        if ((rc = dclaw_step2_dev(g_host.d_qc, g_host.d_aux, maux, mx, my, dx, dy,
-                                  g_host.d_scalars, g_host.d_params, g_host.d_flux[0],
-                                  g_host.d_flux[1], g_host.d_flux[2], g_host.d_flux[3],
-                                  g_host.d_scalars + 1, nullptr)))
+        if ((rc = dclaw_step2_dev(c->d_qc, c->d_aux, maux, mx, my, dx, dy, c->d_scalars,
+                                  g_cfg.d_params, c->d_flux[0], c->d_flux[1], c->d_flux[2],
+                                  c->d_flux[3], c->d_scalars + 1, c->st)))
             return rc;
     }
-    if (g_host.params.curvature == 1) {
+    if (g_cfg.params.curvature == 1) {
         const size_t nw = dclaw_curvature_workspace_doubles(mx, my);
-        if ((rc = grow(&g_host.d_curv, &g_host.cap_curv, nw))) return rc;
-        rc = launch_curvature_step(g_host.d_qa, g_host.d_qb, g_host.d_aux, g_host.d_aux, maux, mx,
-                                   my, dx, dy, nullptr, g_host.d_scalars, g_host.d_params,
-                                   g_host.d_scalars + 1, g_host.d_curv, nw, nullptr);
+        if ((rc = grow(&c->d_curv, &c->cap_curv, nw))) return rc;
+        rc = launch_curvature_step(c->d_qa, c->d_qb, c->d_aux, c->d_aux, maux, mx, my, dx, dy,
+                                   nullptr, c->d_scalars, g_cfg.d_params, c->d_scalars + 1,
+                                   c->d_curv, nw, c->st);
     } else {
-        rc = launch_stepgrid(g_host.d_qa, g_host.d_qb, g_host.d_aux, g_host.d_aux, maux, mx, my,
-                             dx, dy, g_host.d_scalars, g_host.d_params, g_host.d_scalars + 1,
-                             nullptr, nullptr);
+        rc = launch_stepgrid(c->d_qa, c->d_qb, c->d_aux, c->d_aux, maux, mx, my, dx, dy,
+                             c->d_scalars, g_cfg.d_params, c->d_scalars + 1, nullptr, c->st);
     }
     if (rc) return rc;
-    HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(q, g_host.d_qb, nq * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(aux, g_host.d_aux, na * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(scal, g_host.d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost));
-    *cflgrid = scal[1];
+    if ((rc = d2h(c, q, c->d_qb, nq)) || (rc = d2h(c, aux, c->d_aux, na)) ||
+        (rc = d2h(c, scal, c->d_scalars, 2)))
+        return rc;
     if (flux)
         for (int k = 0; k < 4; k++)
-            HIP_TRY(hipMemcpy(flux[k], g_host.d_flux[k], nq * sizeof(double),
-                              hipMemcpyDeviceToHost));
+            if ((rc = d2h(c, flux[k], c->d_flux[k], nq))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->st));
+    *cflgrid = scal[1];
     return DCLAW_OK;
 }
 
 // ---- Fortran-ABI per-stage drop-ins (host memory) ----------------------
 
-// Copies q (and aux) of one patch to the library's device buffers.
-static int host_upload(const double* q, const double* aux, int mx, int my, int maux)
+// Copies q (and aux) of one patch to the calling thread's device buffers.
+static int host_upload(HostCtx* c, const double* q, const double* aux, int mx, int my, int maux)
 {
     int rc;
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
     const size_t na = (size_t)maux * (mx + 4) * (my + 4);
-    if ((rc = grow(&g_host.d_qa, &g_host.cap_qa, nq))) return rc;
-    if ((rc = grow(&g_host.d_aux, &g_host.cap_aux, na))) return rc;
-    if (!g_host.d_nan) HIP_TRY(hipMalloc(&g_host.d_nan, sizeof(int)));
-    HIP_TRY(hipMemcpy(g_host.d_qa, q, nq * sizeof(double), hipMemcpyHostToDevice));
-    if (aux) HIP_TRY(hipMemcpy(g_host.d_aux, aux, na * sizeof(double), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, nullptr, g_host.d_nan, 0x7fffffff);
+    if ((rc = grow(&c->d_qa, &c->cap_qa, nq))) return rc;
+    if ((rc = grow(&c->d_aux, &c->cap_aux, na))) return rc;
+    if ((rc = h2d(c, c->d_qa, q, nq))) return rc;
+    if (aux && (rc = h2d(c, c->d_aux, aux, na))) return rc;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, c->st, c->d_nan, 0x7fffffff);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
 }
 
 // check4nans' stop (AMRClaw check4nans): the first NaN found by the scan.
-static void host_nan_stop(const double* q, int mx, int my, double t, int icheck)
+static void host_nan_stop(HostCtx* c, const double* q, int mx, int my, double t, int icheck)
 {
     int first = 0x7fffffff;
-    if (hipMemcpy(&first, g_host.d_nan, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return;
+    if (hipMemcpyAsync(&first, c->d_nan, sizeof(int), hipMemcpyDeviceToHost, c->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+        return;
     if (first == 0x7fffffff) return;
     const int m = first % 7, cell = first / 7;
     const int i = cell % (mx + 4) - 1, j = cell / (mx + 4) - 1;
@@ -2931,10 +2970,10 @@ static int check_host_stage(int mbc, int meqn, int mx, int my, int maux, bool ne
     if (mbc != DCLAW_MBC) return fail(DCLAW_EINVAL, "mbc must be 2");
     if (meqn != DCLAW_MEQN) return fail(DCLAW_EINVAL, "meqn must be 7");
     if (int rc = check_stage(mx, my, maux, need_taudir)) return rc;
-    if (!g_host.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
-    if (g_host.params.entrainment && maux < 7)
+    if (!g_cfg.have_params) return fail(DCLAW_EINVAL, "dclaw_set_params was not called");
+    if (g_cfg.params.entrainment && maux < 7)
         return fail(DCLAW_EINVAL, "entrainment needs maux >= 7 (aux(7) holds h_e)");
-    if (g_host.params.curvature && maux < 3)
+    if (g_cfg.params.curvature && maux < 3)
         return fail(DCLAW_EINVAL, "curvature needs maux >= 3 (aux(3) = theta)");
     return DCLAW_OK;
 }
@@ -2949,18 +2988,20 @@ static void host_stop_on_error(const char* who, int rc)
 static int b4step2_host(int mbc, int mx, int my, int meqn, double* q, double dx, double dy,
                         double t, int maux, double* aux)
 {
-    std::lock_guard<std::mutex> lk(g_host.mu);
+    std::shared_lock<std::shared_mutex> lk(g_cfg.mu);
     if (int rc = check_host_stage(mbc, meqn, mx, my, maux, true)) return rc;
-    if (int rc = host_upload(q, aux, mx, my, maux)) return rc;
-    if (int rc = dclaw_b4step2_dev(g_host.d_qa, g_host.d_aux, maux, mx, my, dx, dy,
-                                   g_host.d_params, g_host.d_nan, nullptr))
+    HostCtx* c;
+    if (int rc = host_ctx(&c)) return rc;
+    if (int rc = host_upload(c, q, aux, mx, my, maux)) return rc;
+    if (int rc = dclaw_b4step2_dev(c->d_qa, c->d_aux, maux, mx, my, dx, dy, g_cfg.d_params,
+                                   c->d_nan, c->st))
         return rc;
-    HIP_TRY(hipDeviceSynchronize());
-    host_nan_stop(q, mx, my, t, 1);
+    host_nan_stop(c, q, mx, my, t, 1);
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
     const size_t na = (size_t)maux * (mx + 4) * (my + 4);
-    HIP_TRY(hipMemcpy(q, g_host.d_qa, nq * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(aux, g_host.d_aux, na * sizeof(double), hipMemcpyDeviceToHost));
+    if (int rc = d2h(c, q, c->d_qa, nq)) return rc;
+    if (int rc = d2h(c, aux, c->d_aux, na)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->st));
     return DCLAW_OK;
 }
 
@@ -2968,24 +3009,25 @@ static int step2_host(int meqn, int maux, int mbc, int mx, int my, const double*
                       const double* aux, double dx, double dy, double dt, double* cflgrid,
                       double* fm, double* fp, double* gm, double* gp)
 {
-    std::lock_guard<std::mutex> lk(g_host.mu);
+    std::shared_lock<std::shared_mutex> lk(g_cfg.mu);
     if (int rc = check_host_stage(mbc, meqn, mx, my, maux, true)) return rc;
-    if (int rc = host_upload(qold, aux, mx, my, maux)) return rc;
+    HostCtx* c;
+    if (int rc = host_ctx(&c)) return rc;
+    if (int rc = host_upload(c, qold, aux, mx, my, maux)) return rc;
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
     for (int k = 0; k < 4; k++)
-        if (int rc = grow(&g_host.d_flux[k], &g_host.cap_flux[k], nq)) return rc;
+        if (int rc = grow(&c->d_flux[k], &c->cap_flux[k], nq)) return rc;
     double scal[2] = {dt, 0.0};
-    HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
-    if (int rc = dclaw_step2_dev(g_host.d_qa, g_host.d_aux, maux, mx, my, dx, dy,
-                                 g_host.d_scalars, g_host.d_params, g_host.d_flux[0],
-                                 g_host.d_flux[1], g_host.d_flux[2], g_host.d_flux[3],
-                                 g_host.d_scalars + 1, nullptr))
+    if (int rc = h2d(c, c->d_scalars, scal, 2)) return rc;
+    if (int rc = dclaw_step2_dev(c->d_qa, c->d_aux, maux, mx, my, dx, dy, c->d_scalars,
+                                 g_cfg.d_params, c->d_flux[0], c->d_flux[1], c->d_flux[2],
+                                 c->d_flux[3], c->d_scalars + 1, c->st))
         return rc;
-    HIP_TRY(hipDeviceSynchronize());
     double* outs[4] = {fm, fp, gm, gp};
     for (int k = 0; k < 4; k++)
-        HIP_TRY(hipMemcpy(outs[k], g_host.d_flux[k], nq * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(scal, g_host.d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost));
+        if (int rc = d2h(c, outs[k], c->d_flux[k], nq)) return rc;
+    if (int rc = d2h(c, scal, c->d_scalars, 2)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->st));
     *cflgrid = scal[1];
     return DCLAW_OK;
 }
@@ -2993,29 +3035,31 @@ static int step2_host(int meqn, int maux, int mbc, int mx, int my, const double*
 static int src2_host(int meqn, int mbc, int mx, int my, double* q, int maux, const double* aux,
                      double dx, double dy, double t, double dt)
 {
-    std::lock_guard<std::mutex> lk(g_host.mu);
+    std::shared_lock<std::shared_mutex> lk(g_cfg.mu);
     if (int rc = check_host_stage(mbc, meqn, mx, my, maux, false)) return rc;
+    HostCtx* c;
+    if (int rc = host_ctx(&c)) return rc;
     // aux travels with q: with entrainment on src2 reads aux(7) = h_e
     // (src2.f90:253-263), and the device buffer may hold another patch's aux
-    if (int rc = host_upload(q, aux, mx, my, maux)) return rc;
+    if (int rc = host_upload(c, q, aux, mx, my, maux)) return rc;
     double scal[2] = {dt, 0.0};
-    HIP_TRY(hipMemcpy(g_host.d_scalars, scal, 2 * sizeof(double), hipMemcpyHostToDevice));
-    if (g_host.params.curvature == 1) {
+    if (int rc = h2d(c, c->d_scalars, scal, 2)) return rc;
+    if (g_cfg.params.curvature == 1) {
         const size_t nw = (size_t)mx * (size_t)my;
-        if (int rc = grow(&g_host.d_curv, &g_host.cap_curv, nw)) return rc;
-        if (int rc = dclaw_check4nans_dev(g_host.d_qa, mx, my, g_host.d_nan, nullptr)) return rc;
-        if (int rc = dclaw_src2_curvature_dev(g_host.d_qa, g_host.d_aux, maux, mx, my, dx, dy,
-                                              g_host.d_scalars, g_host.d_params, g_host.d_curv,
-                                              nw, nullptr))
+        if (int rc = grow(&c->d_curv, &c->cap_curv, nw)) return rc;
+        if (int rc = dclaw_check4nans_dev(c->d_qa, mx, my, c->d_nan, c->st)) return rc;
+        if (int rc = dclaw_src2_curvature_dev(c->d_qa, c->d_aux, maux, mx, my, dx, dy,
+                                              c->d_scalars, g_cfg.d_params, c->d_curv, nw,
+                                              c->st))
             return rc;
-    } else if (int rc = dclaw_src2_dev(g_host.d_qa, g_host.d_aux, maux, mx, my, g_host.d_scalars,
-                                       g_host.d_params, g_host.d_nan, nullptr)) {
+    } else if (int rc = dclaw_src2_dev(c->d_qa, c->d_aux, maux, mx, my, c->d_scalars,
+                                       g_cfg.d_params, c->d_nan, c->st)) {
         return rc;
     }
-    HIP_TRY(hipDeviceSynchronize());
-    host_nan_stop(q, mx, my, t, 2);
+    host_nan_stop(c, q, mx, my, t, 2);
     const size_t nq = (size_t)7 * (mx + 4) * (my + 4);
-    HIP_TRY(hipMemcpy(q, g_host.d_qa, nq * sizeof(double), hipMemcpyDeviceToHost));
+    if (int rc = d2h(c, q, c->d_qa, nq)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->st));
     return DCLAW_OK;
 }
 

@@ -546,7 +546,11 @@ int dclaw_src2_dev(double* q, const double* aux, int maux, int mx, int my, const
  * (Makefile_template:45-63).  Parameters come from dclaw_set_params()
  * called once after set_dig.  stepgrid_dclaw_ performs b4step2 + step2 +
  * update + src2 in place on q and returns cflgrid; aux(4..6) are written as
- * b4step2/calc_taudir would (digclaw_module.f90:641-713) when maux >= 6. */
+ * b4step2/calc_taudir would (digclaw_module.f90:641-713) when maux >= 6.
+ * The drop-ins are re-entrant: each calling thread has its own device
+ * buffers and HIP stream, so AMRClaw's advanc can step different patches
+ * from its OpenMP threads at once; dclaw_set_params waits for the calls in
+ * progress. */
 int dclaw_set_params(const dclaw_params* params_host);
 void stepgrid_dclaw_(double* q, double* aux, const int* mbc, const int* mx, const int* my,
                      const int* maux, const double* dx, const double* dy, const double* dt,

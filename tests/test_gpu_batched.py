@@ -139,6 +139,59 @@ def test_host_dropin_stepgrid(dev, oracle_mod):
     assert abs(cfl.value - cflo) <= TOL_STEP * cflo
 
 
+def test_host_dropin_stepgrid_concurrent_threads(dev, oracle_mod):
+    """stepgrid_dclaw_ called from four threads at once (AMRClaw advanc's
+    OpenMP loop over patches; ctypes releases the GIL during the call), each
+    stepping its own patch three times: every result equals the same calls
+    made one after another on one thread, bit for bit."""
+    import threading
+
+    from dclaw_amd import _lib
+    from dclaw_amd.params import to_c
+    from helpers import advance_oracle
+    O = oracle_mod
+    L = _lib.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    patches = []
+    for k, (mx, my) in enumerate([(75, 52), (130, 41), (48, 96), (101, 77)]):
+        pt = slope_patch(mx, my, slope=0.7, phi_deg=26.0, seed=10 + k)
+        q, aux, dt = advance_oracle(O, pt, 4)
+        patches.append((pt, q, aux, dt))
+    cp = to_c(patches[0][0].params)
+    _lib.check(L.dclaw_set_params(ctypes.byref(cp)))
+
+    def run(k, out):
+        pt, q, aux, dt = patches[k]
+        qh = np.array(q, order="F", copy=True)
+        ah = np.array(aux, order="F", copy=True)
+        cfls = []
+        for _ in range(3):
+            c = ctypes.c_double(0.0)
+            L.stepgrid_dclaw_(qh.ctypes.data_as(dp), ah.ctypes.data_as(dp), ci(2), ci(pt.mx),
+                              ci(pt.my), ci(ah.shape[0]), cd(pt.dx), cd(pt.dy), cd(dt),
+                              ctypes.byref(c))
+            cfls.append(c.value)
+        out[k] = (qh, ah, cfls)
+
+    print("[host stepgrid threads] serial", flush=True)
+    serial = {}
+    for k in range(len(patches)):
+        run(k, serial)
+    print("[host stepgrid threads] 4 threads", flush=True)
+    conc = {}
+    ths = [threading.Thread(target=run, args=(k, conc)) for k in range(len(patches))]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    for k in range(len(patches)):
+        assert np.array_equal(conc[k][0], serial[k][0]), k
+        assert np.array_equal(conc[k][1], serial[k][1]), k
+        assert conc[k][2] == serial[k][2], k
+
+
 def test_host_dropin_stepgrid_with_fluxes(dev, oracle_mod):
     """stepgrid_fluxes_dclaw_: the same q, aux and cflgrid as stepgrid_dclaw_,
     and fm, fp, gm, gp equal to the oracle's step2 on the oracle's b4step2
