@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <cstring>
@@ -88,10 +89,12 @@ __device__ __forceinline__ void wave_max_to(double v, double* dst)
                   (unsigned long long)__double_as_longlong(v));
 }
 
-int g_seg_rows = 32;   // rows per segment of the single-patch launch (tunable)
-int g_dry_rows = 48;   // rows per wavefront of the split step's dry pass (whole segments)
-int g_split_mode = 1;  // split step with the full workspace: 1 wet-first order, 2 dry pass,
-                       // 3 wet-first short segments + tall dry groups
+// Performance settings of the single-patch launches (results do not depend
+// on them); atomic, so a thread may change one while others launch.
+std::atomic<int> g_seg_rows{32};   // rows per segment of the single-patch launch
+std::atomic<int> g_dry_rows{48};   // rows per wavefront of the split step's dry pass
+std::atomic<int> g_split_mode{1};  // split step with the full workspace: 1 wet-first order,
+                                   // 2 dry pass, 3 wet-first short segments + tall dry groups
 
 // Compute units of the current device (cached per device).
 int cu_count()
