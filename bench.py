@@ -1007,7 +1007,8 @@ def bench_c4_gradient(args):
     steps (1000) of the batched forward with per-member gauge records (three
     gauges around each member's source, loss = sum of squared depth misfits
     against a seeded synthetic target), then the checkpointed batched reverse
-    sweep (block checkpoints every --grad-every steps in host memory,
+    sweep (block checkpoints every --grad-every steps, on the device while
+    it keeps room for the reverse sweep and in host memory after that,
     sub-checkpoints every ~sqrt of that on the device) giving every member's
     gradient with respect to its initial state, phi and kref.  Reports the
     wall time of each phase, the peak HBM allocated, and whether every
@@ -1035,7 +1036,8 @@ def bench_c4_gradient(args):
     seg = args.seg_rows if args.seg_rows > 0 else 32
     torch.cuda.reset_peak_memory_stats()
     er = AD.EnsembleRollout(members, N, gxy, parameters=("phi", "kref"),
-                            checkpoint_every=args.grad_every, seg_rows=seg, host_checkpoints=True)
+                            checkpoint_every=args.grad_every, seg_rows=seg,
+                            host_checkpoints="auto")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     series, _ = er.forward()
@@ -1048,7 +1050,9 @@ def bench_c4_gradient(args):
         g_series.append(z)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
-    print(f"[bench] c4 gradient: forward {N} steps {t_fwd:.1f}s, loss {loss:.6e}",
+    on_host = sum(1 for ck in er.ckpt.values() if ck[0].device.type == "cpu")
+    print(f"[bench] c4 gradient: forward {N} steps {t_fwd:.1f}s, loss {loss:.6e}, "
+          f"{len(er.ckpt)} block checkpoints ({on_host} in host memory)",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     lam0, grad = er.backward(g_series, None)
@@ -1066,9 +1070,11 @@ def bench_c4_gradient(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"whole batched gradient: {len(members)} members of {n}x{n}, "
                                f"{N} steps, gauge-misfit loss per member, block checkpoints "
-                               f"every {args.grad_every} steps in host memory, sub-checkpoints "
-                               f"on the device; value = cell-updates of the forward steps per "
-                               f"second of forward + recompute + reverse sweep",
+                               f"every {args.grad_every} steps (on the device while it keeps "
+                               f"room for the reverse sweep, then in host memory), "
+                               f"sub-checkpoints on the device; value = cell-updates of the "
+                               f"forward steps per second of forward + recompute + reverse sweep",
+                   "block_checkpoints": len(er.ckpt), "block_checkpoints_in_host_memory": on_host,
                    "members": len(members), "side": n, "seed": args.seed, "seg_rows": seg,
                    "checkpoint_every": args.grad_every,
                    "sub_every": AD.sub_interval(args.grad_every, 0)},

@@ -343,7 +343,7 @@ class EnsembleRollout:
                  parameters: Sequence[str] = ("phi", "kref"), checkpoint_every: int = 100,
                  dt0: float = 1e-3, cfl_desired: float = 0.9, dt_max: float = 1e9,
                  bc=(BC_EXTRAP,) * 4, seg_rows: int = 16, sub_every: int = 0,
-                 host_checkpoints: bool = False):
+                 host_checkpoints=False, hbm_reserve: Optional[int] = None):
         from .step import Ensemble
         if nsteps < 1 or checkpoint_every < 1:
             raise ValueError("nsteps and checkpoint_every must be >= 1")
@@ -353,10 +353,18 @@ class EnsembleRollout:
         self.nsteps = nsteps
         self.every = checkpoint_every
         self.sub_every = sub_every
-        # block checkpoints in host memory (copied back when a block is
-        # reversed): HBM then holds only the sub-checkpoints and one
-        # sub-block's states besides the step's own buffers and workspaces
+        # block checkpoints: on the device (False), in host memory (True,
+        # copied back when a block is reversed: HBM then holds only the
+        # sub-checkpoints and one sub-block's states besides the step's own
+        # buffers and workspaces), or "auto": on the device while it keeps
+        # hbm_reserve bytes free after the copy (default: what the reverse
+        # sweep allocates -- the sub-checkpoints of a block, one sub-block's
+        # states, two adjoint states and two step temporaries -- plus 32 GB),
+        # in host memory after that
+        if host_checkpoints not in (False, True, "auto"):
+            raise ValueError("host_checkpoints must be False, True or 'auto'")
         self.host_checkpoints = host_checkpoints
+        self.hbm_reserve = hbm_reserve
         self.dt0, self.cfl_desired, self.dt_max = dt0, cfl_desired, dt_max
         self.ens = Ensemble(members, bc=bc, seg_rows=seg_rows)
         self.adj = EnsembleAdjoint(self.ens, parameters)
@@ -369,17 +377,26 @@ class EnsembleRollout:
         return [self.ens.current(k) for k in range(self.ens.n)]
 
     def _checkpoint(self) -> List[torch.Tensor]:
-        if not self.host_checkpoints:
-            return [x.clone() for x in self._states()]
+        states = self._states()
+        to_host = self.host_checkpoints is True
+        if self.host_checkpoints == "auto":
+            nbytes = sum(x.numel() * x.element_size() for x in states)
+            reserve = self.hbm_reserve
+            if reserve is None:
+                s = sub_interval(self.every, self.sub_every)
+                reserve = (-(-self.every // s) + s + 4) * nbytes + (32 << 30)
+            free, _ = torch.cuda.mem_get_info(self.ens.dt.device)
+            to_host = free - nbytes < reserve
+        if not to_host:
+            return [x.clone() for x in states]
         # pageable host memory: page-locked allocations of this size are not
         # granted everywhere, and a block checkpoint is taken only every
         # checkpoint_every steps
-        return [x.to("cpu") for x in self._states()]
+        return [x.to("cpu") for x in states]
 
     def _load(self, ck: List[torch.Tensor]) -> List[torch.Tensor]:
-        if not self.host_checkpoints:
-            return ck
-        return [h.to(self.ens.dt.device) for h in ck]
+        dev = self.ens.dt.device
+        return [h if h.device == dev else h.to(dev) for h in ck]
 
     def _gauges(self, series: List[torch.Tensor], n: int) -> None:
         e = self.ens
