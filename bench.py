@@ -77,6 +77,9 @@ def parse():
     ap.add_argument("--c1-steps", type=int, default=100,
                     help="c4: timed steps of the 4096^2 forward step reported beside it "
                          "(0 disables)")
+    ap.add_argument("--c2-steps", type=int, default=10,
+                    help="c4: timed batched pairs of the 256-member C2 ensemble reported beside "
+                         "it against its plain loop (0 disables)")
     ap.add_argument("--grad-steps", type=int, default=1000,
                     help="c4-gradient: steps of the checkpointed gradient run")
     ap.add_argument("--grad-every", type=int, default=50,
@@ -709,8 +712,7 @@ def bench_ensemble(args):
                                f"phi/m0/kref, one batched "
                                f"launch per step (plus the src2 pass over wet tiles unless "
                                f"fused); the tangent-linear batched step is timed under "
-                               f"'tangent'; the adjoint is per patch (timed on the patch and "
-                               f"flume workloads), not batched",
+                               f"'tangent', the batched adjoint step under 'adjoint'",
                    "members": len(members),
                    "cells": cells, "seed": args.seed, "seg_rows": seg,
                    "src2": "fused" if args.fused else "split",
@@ -802,6 +804,73 @@ def c1_forward(args):
            "active_roofline_frac": act * BYTES_PER_CELL_FWD / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     _lib.check(min(0, _lib.lib().dclaw_set_seg_rows(seg)))
     del r
+    torch.cuda.empty_cache()
+    return out
+
+
+def c2_pair(args):
+    """The forward + adjoint pair of the C2 ensemble (256 members, sides
+    log-uniform in [64, 512]), batched against the plain per-member loop,
+    timed like the headline pair: where members are small, batching is what
+    fills the GPU."""
+    import torch
+
+    from dclaw_amd import _lib
+    from dclaw_amd import adjoint as AD
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+
+    members = W.ensemble(256, 64, 512, seed=args.seed)
+    cells = sum(m.patch.mx * m.patch.my for m in members)
+    ens = S.Ensemble(members, seg_rows=16)
+    adj = AD.EnsembleAdjoint(ens, parameters=("phi", "kref"))
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    lam = [torch.randn(ens.qa[k].shape, dtype=torch.float64, device="cuda", generator=gen)
+           for k in range(ens.n)]
+    lin = [torch.empty_like(x) for x in lam]
+    ens.dt.fill_(1e-3)
+    ens.cfl.zero_()
+    _lib.check(min(0, _lib.lib().dclaw_set_seg_rows(16)))
+
+    def pair(plain):
+        qs = [ens.current(k) for k in range(ens.n)]
+        if plain:
+            adj.step_plain(qs, lam, lin)
+            ens.step_plain()
+        else:
+            adj.step(qs, lam, lin)
+            ens.step()
+        S.dt_update(ens.dt, ens.cfl, 0.9, 1e9)
+
+    out = {"workload": "256 members, sides log-uniform in [64, 512], seed "
+                       f"{args.seed}, {cells} cells; forward + adjoint pair as the headline's",
+           "cells": cells, "seg_rows": 16}
+    L = _lib.lib()
+    for plain, nst in ((False, args.c2_steps), (True, max(2, args.c2_steps // 4))):
+        if plain:
+            # the plain loop at the single-patch segment height that suits it
+            tune = {}
+            for h in (4, 6, 8, 12, 16):
+                _lib.check(min(0, L.dclaw_set_seg_rows(h)))
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ens.step_plain()
+                torch.cuda.synchronize()
+                tune[h] = time.perf_counter() - t0
+            out["plain_seg_rows"] = min(tune, key=tune.get)
+            _lib.check(min(0, L.dclaw_set_seg_rows(out["plain_seg_rows"])))
+        for _ in range(2):
+            pair(plain)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nst):
+            pair(plain)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / nst
+        out["plain_ms_per_step" if plain else "ms_per_step"] = t * 1e3
+        out["plain_value" if plain else "value"] = cells / t
+    out["batched_over_plain"] = out["plain_ms_per_step"] / out["ms_per_step"]
+    del adj, lam, lin, ens
     torch.cuda.empty_cache()
     return out
 
@@ -947,6 +1016,11 @@ def bench_c4(args):
     if c1:
         print(f"[bench] c1 forward {c1['ms_per_step']:.3f} ms per step", file=sys.stderr,
               flush=True)
+    c2 = c2_pair(args) if args.c2_steps > 0 else None
+    if c2:
+        print(f"[bench] c2 pair: batched {c2['ms_per_step']:.2f} ms, plain "
+              f"{c2['plain_ms_per_step']:.2f} ms ({c2['batched_over_plain']:.1f} x)",
+              file=sys.stderr, flush=True)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -997,6 +1071,7 @@ def bench_c4(args):
                                "k_src2_batched)",
                      "kernel_avg_ms": adj_ms + fwd_ms},
         "c1_forward": c1,
+        "c2_pair": c2,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
