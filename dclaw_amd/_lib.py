@@ -123,6 +123,16 @@ SIGNATURES = {
     "dclaw_stepgrid_split_static_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _ip,
                                                        _vp, _vp, _vp, _vp]),
     "dclaw_static_workspace_ints": (ctypes.c_int, []),
+    "dclaw_batched_resident_size": (ctypes.c_int, [_i, _i]),
+    "dclaw_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "dclaw_destroy": (ctypes.c_int, [_vp]),
+    "dclaw_handle_set_seg_rows": (ctypes.c_int, [_vp, _i]),
+    "dclaw_handle_set_split_mode": (ctypes.c_int, [_vp, _i]),
+    "dclaw_handle_set_dry_rows": (ctypes.c_int, [_vp, _i]),
+    "dclaw_handle_tile_count": (ctypes.c_int, [_vp, _i, _i]),
+    "dclaw_handle_stepgrid_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _ip,
+                                                 _vp, _vp, _vp, _vp]),
+    "dclaw_handle_state_written": (ctypes.c_int, [_vp, _vp]),
     "dclaw_curvature_workspace_doubles": (ctypes.c_size_t, [_i, _i]),
     "dclaw_stepgrid_curvature_dev": (ctypes.c_int, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _ip, _vp,
                                                     _vp, _vp, _vp, ctypes.c_size_t, _vp]),

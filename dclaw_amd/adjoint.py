@@ -444,6 +444,7 @@ class EnsembleRollout:
             # serve as scratch); returns copies of q_{n+1}
             for k in range(n_m):
                 e.current(k).copy_(x[k])
+            e.state_written()
             e.dt.copy_(self.dts[n])
             e.cfl.zero_()
             e.step()

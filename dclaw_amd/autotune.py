@@ -48,14 +48,24 @@ def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 5,
     steps at a fixed dt; the runner's state and dt are restored afterwards.
     Returns the height and the times in ms keyed by (height, mode)."""
     L = _lib.lib()
+    handle = getattr(runner, "handle", None)
+
+    def use(h, mode):
+        # the runner's own handle if it has one, else the library-wide setting
+        runner.split_mode = mode
+        if handle is not None:
+            handle.set_seg_rows(h)
+            handle.set_split_mode(mode)
+        else:
+            _lib.check(min(0, L.dclaw_set_seg_rows(h)))
+
     qa, qb = runner.qa.clone(), runner.qb.clone()
     dt, cfl = runner.dt.clone(), runner.cfl.clone()
     modes = tuple(split_modes) if runner.split_src2 else (runner.split_mode,)
     times = {}
     for mode in modes:
-        runner.split_mode = mode
         for h in candidates:
-            _lib.check(min(0, L.dclaw_set_seg_rows(h)))
+            use(h, mode)
             runner.qa.copy_(qa)
             runner.state_written()
 
@@ -63,8 +73,8 @@ def tune_patch(runner, candidates: Iterable[int] = CANDIDATES, reps: int = 5,
                 runner.step()
                 runner.cfl.zero_()
             times[(h, mode)] = _time(one, reps)
-    best, runner.split_mode = min(times, key=times.get)
-    _lib.check(min(0, L.dclaw_set_seg_rows(best)))
+    best, mode = min(times, key=times.get)
+    use(best, mode)
     runner.qa.copy_(qa)
     runner.qb.copy_(qb)
     runner.state_written()

@@ -96,6 +96,19 @@ std::atomic<int> g_dry_rows{48};   // rows per wavefront of the split step's dry
 std::atomic<int> g_split_mode{1};  // split step with the full workspace: 1 wet-first order,
                                    // 2 dry pass, 3 wet-first short segments + tall dry groups
 
+// The settings one launch uses, read once per call: from the library-wide
+// values above, or from a dclaw_handle's own.
+struct Tuning {
+    int seg_rows, split_mode, dry_rows;
+};
+inline Tuning global_tuning() { return Tuning{g_seg_rows.load(), g_split_mode.load(), g_dry_rows.load()}; }
+
+// Tiles (58-column strip x segment) of a patch at a segment height.
+inline long tiles_at(int mx, int my, int seg_rows)
+{
+    return (long)((mx + STRIP_W - 1) / STRIP_W) * (long)((my + seg_rows - 1) / seg_rows);
+}
+
 // Compute units of the current device (cached per device).
 int cu_count()
 {
@@ -169,9 +182,13 @@ __global__ __launch_bounds__(64) void k_step_batched(const dclaw_patch* __restri
                                                      const dclaw_params* __restrict__ params,
                                                      double* cfl_dev, int bx0, int bx1, int by0,
                                                      int by1, int* tile_flags,
-                                                     const int* __restrict__ order)
+                                                     const int* __restrict__ order,
+                                                     const int* __restrict__ dry_age)
 {
     const int t = order ? order[blockIdx.x] : (int)blockIdx.x;
+    // a tile whose neighbourhood was dry on its patch's last three inputs
+    // already holds its result in q_out (k_bdry_age); its flag stays 0
+    if (dry_age && dry_age[t] >= 3) return;
     const int pidx = tiles[4 * t + 0];
     const dclaw_patch pt = patches[pidx];
     const dclaw_params P = params[pt.member];
@@ -518,6 +535,85 @@ __global__ void k_order_tiles(const int* __restrict__ flags, int n, int* order, 
     base_d = __shfl(base_d, 0, 64);
     if (wet) order[base_w + __popcll(bw & below)] = t;
     else if (in) order[n - 1 - (base_d + __popcll(bd & below))] = t;
+}
+
+// Dry-tile skipping of the batched split step (n_ws >= batched_resident_ints
+// in the workspace: flags, order, counters, then BSKIP_GMETA ints of the
+// work list's shape, BSKIP_PMETA ints per patch and one age per tile).  Per
+// patch the rule of the single-patch step above: the tile (patch, strip,
+// segment) of a work list from dclaw_worklist_build sits at the patch's
+// first tile + strip * segments + segment, and a tile whose neighbours
+// within the strip kernel's read halo (one strip, ceil(3 / segment rows)
+// segments) were unflagged on the last three inputs of that patch -- with
+// its q_in and q_out swapping roles between calls -- holds its result in
+// q_out already.  A neighbour is used only where the list has it at that
+// place (any other list only skips nothing).  gmeta: [0] segment rows,
+// [1] tiles, [2] patches, [3] 1 once written; pmeta per patch: [0..1] the
+// previous call's q_out, [2..3] its q_in, [4] consecutive calls (capped at
+// 4) that continued one such sequence.
+constexpr int BSKIP_GMETA = 4;
+constexpr int BSKIP_PMETA = 5;
+inline long batched_resident_ints(long ntiles, long npatch)
+{
+    return 3 * ntiles + 4 + BSKIP_GMETA + BSKIP_PMETA * npatch;
+}
+
+__global__ __launch_bounds__(256) void k_bskip_meta(const dclaw_patch* __restrict__ patches,
+                                                    int npatch, int* gmeta, int* pmeta,
+                                                    int seg_rows, int ntiles)
+{
+    __shared__ int same;
+    if (threadIdx.x == 0)
+        same = gmeta[3] == 1 && gmeta[0] == seg_rows && gmeta[1] == ntiles && gmeta[2] == npatch;
+    __syncthreads();
+    for (int p = (int)threadIdx.x; p < npatch; p += (int)blockDim.x) {
+        int* m = pmeta + BSKIP_PMETA * p;
+        const unsigned long long in = (unsigned long long)patches[p].q_in;
+        const unsigned long long out = (unsigned long long)patches[p].q_out;
+        const unsigned long long last_out =
+            (unsigned long long)(unsigned)m[0] | ((unsigned long long)(unsigned)m[1] << 32);
+        const unsigned long long last_in =
+            (unsigned long long)(unsigned)m[2] | ((unsigned long long)(unsigned)m[3] << 32);
+        const bool cont = same && in == last_out && out == last_in && m[4] >= 0;
+        m[4] = cont ? min(m[4] + 1, 4) : 0;
+        m[0] = (int)(unsigned)(out & 0xffffffffull);
+        m[1] = (int)(unsigned)(out >> 32);
+        m[2] = (int)(unsigned)(in & 0xffffffffull);
+        m[3] = (int)(unsigned)(in >> 32);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        gmeta[0] = seg_rows;
+        gmeta[1] = ntiles;
+        gmeta[2] = npatch;
+        gmeta[3] = 1;
+    }
+}
+
+__global__ void k_bdry_age(const dclaw_patch* __restrict__ patches,
+                           const int32_t* __restrict__ tiles, const int* __restrict__ flags,
+                           int* age, const int* __restrict__ pmeta, int ntiles, int seg_rows,
+                           int ry)
+{
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= ntiles) return;
+    const int p = tiles[4 * t + 0], s = tiles[4 * t + 1], g = tiles[4 * t + 2];
+    if (pmeta[BSKIP_PMETA * p + 4] == 0) {
+        age[t] = 0;
+        return;
+    }
+    const int ns = (patches[p].mx + STRIP_W - 1) / STRIP_W;
+    const int ng = (patches[p].my + seg_rows - 1) / seg_rows;
+    const long base = (long)t - ((long)s * ng + g);
+    bool dry = true;
+    for (int sg = max(0, g - ry); sg <= min(ng - 1, g + ry); sg++)
+        for (int x = max(0, s - 1); x <= min(ns - 1, s + 1); x++) {
+            const long nb = base + (long)x * ng + sg;
+            const bool placed = nb >= 0 && nb < ntiles && tiles[4 * nb + 0] == p &&
+                                tiles[4 * nb + 1] == x && tiles[4 * nb + 2] == sg;
+            dry = dry && placed && flags[nb] == 0;
+        }
+    age[t] = dry ? min(age[t] + 1, 3) : 0;
 }
 
 // Work list of the hybrid split step: each group of HYB_GROUP consecutive
@@ -1846,9 +1942,7 @@ int dclaw_strip_width(void) { return STRIP_W; }
 int dclaw_set_seg_rows(int seg_rows)
 {
     if (seg_rows < 1 || seg_rows > 65536) return fail(DCLAW_EINVAL, "seg_rows must be in [1, 65536]");
-    const int prev = g_seg_rows;
-    g_seg_rows = seg_rows;
-    return prev;
+    return g_seg_rows.exchange(seg_rows);
 }
 
 int dclaw_bc_fill(double* q, int mx, int my, const int* bc, void* stream)
@@ -1876,11 +1970,11 @@ static int check_bc(const int* bc)
     return DCLAW_OK;
 }
 
-static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
-                           double* aux_out, int maux, int mx, int my, double dx, double dy,
-                           const double* dt_dev, const dclaw_params* params_dev,
-                           double* cfl_dev, const int* bc, hipStream_t stream,
-                           int* tile_flags = nullptr, int n_flags = 0)
+static int launch_stepgrid(const Tuning& tn, const double* q_in, double* q_out,
+                           const double* aux, double* aux_out, int maux, int mx, int my,
+                           double dx, double dy, const double* dt_dev,
+                           const dclaw_params* params_dev, double* cfl_dev, const int* bc,
+                           hipStream_t stream, int* tile_flags = nullptr, int n_flags = 0)
 {
     if (int rcb = check_bc(bc)) return rcb;
     int rc = check_patch(mx, my, maux);
@@ -1890,7 +1984,7 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     if (q_in == q_out) return fail(DCLAW_EINVAL, "q_out must not alias q_in");
     if (!(dx > 0.0) || !(dy > 0.0)) return fail(DCLAW_EINVAL, "dx and dy must be > 0");
     const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
-    const int seg_rows = g_seg_rows;
+    const int seg_rows = tn.seg_rows;
     const int nseg = (my + seg_rows - 1) / seg_rows;
     if ((long)nstrips * nseg > 0x7fffffffL) return fail(DCLAW_EINVAL, "patch too large for one launch");
     const int b0 = bc ? bc[0] : 0, b1 = bc ? bc[1] : 0, b2 = bc ? bc[2] : 0, b3 = bc ? bc[3] : 0;
@@ -1906,13 +2000,13 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
     if (n_flags < ntiles) return fail(DCLAW_EINVAL, "tile_flags holds fewer than dclaw_tile_count() ints");
     if (aux_out) return fail(DCLAW_EINVAL, "the split step does not write aux");
     const bool resident = (long)n_flags >= split_resident_ints(ntiles);
-    if (resident && !(g_split_mode == 1 && b0 && b1 && b2 && b3)) {
+    if (resident && !(tn.split_mode == 1 && b0 && b1 && b2 && b3)) {
         // another mode (or stored ghosts) writes q_out without the dry-tile
         // bookkeeping: the next resident call starts over
         hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, stream, tile_flags + 3 * ntiles + 4,
                            SKIP_META);
     }
-    if ((long)n_flags >= 3L * ntiles + 4 && g_split_mode == 3) {
+    if ((long)n_flags >= 3L * ntiles + 4 && tn.split_mode == 3) {
         // one strip launch: the previous step's wet segments singly and first,
         // the other segments in tall dry groups
         int* wet_list = tile_flags + ntiles;
@@ -1933,7 +2027,7 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
         HIP_TRY(hipGetLastError());
         return DCLAW_OK;
     }
-    if ((long)n_flags >= 2L * ntiles + 4 && g_split_mode == 1) {
+    if ((long)n_flags >= 2L * ntiles + 4 && tn.split_mode == 1) {
         // one strip launch with the previous step's wet tiles first
         int* order = tile_flags + ntiles;
         int* counters = tile_flags + 2 * ntiles + 2;
@@ -1965,11 +2059,11 @@ static int launch_stepgrid(const double* q_in, double* q_out, const double* aux,
         HIP_TRY(hipGetLastError());
         return DCLAW_OK;
     }
-    if ((long)n_flags >= 2L * ntiles + 2 && g_split_mode == 2) {
+    if ((long)n_flags >= 2L * ntiles + 2 && tn.split_mode == 2) {
         // dry pass, strip body over the wet list, src2 over the wet list
         int* wet_list = tile_flags + ntiles;
         int* counters = tile_flags + 2 * ntiles;
-        const int group = std::max(1, std::min(32, g_dry_rows / seg_rows));
+        const int group = std::max(1, std::min(32, tn.dry_rows / seg_rows));
         const int ngroups = (nseg + group - 1) / group;
         const int ncu = cu_count();
         hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, stream, counters, 2);
@@ -2004,8 +2098,8 @@ int dclaw_stepgrid_dev(const double* q_in, double* q_out, const double* aux, int
                        int my, double dx, double dy, const double* dt_dev,
                        const dclaw_params* params_dev, double* cfl_dev, void* stream)
 {
-    return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
-                           cfl_dev, nullptr, (hipStream_t)stream);
+    return launch_stepgrid(global_tuning(), q_in, q_out, aux, nullptr, maux, mx, my, dx, dy,
+                           dt_dev, params_dev, cfl_dev, nullptr, (hipStream_t)stream);
 }
 
 int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, int maux,
@@ -2014,8 +2108,8 @@ int dclaw_stepgrid_bc_dev(const double* q_in, double* q_out, const double* aux, 
                           double* cfl_dev, void* stream)
 {
     if (!bc) return fail(DCLAW_EINVAL, "null bc");
-    return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
-                           cfl_dev, bc, (hipStream_t)stream);
+    return launch_stepgrid(global_tuning(), q_in, q_out, aux, nullptr, maux, mx, my, dx, dy,
+                           dt_dev, params_dev, cfl_dev, bc, (hipStream_t)stream);
 }
 
 int dclaw_stepgrid_tl_dev(const double* q_in, const double* dq_in, double* q_out, double* dq_out,
@@ -2247,7 +2341,7 @@ int dclaw_gauges_adj_dev(const double* q, const double* aux, int maux, int mx, i
 int dclaw_tile_count(int mx, int my)
 {
     if (mx < 1 || my < 1) return fail(DCLAW_EINVAL, "mx and my must be >= 1");
-    const long n = (long)((mx + STRIP_W - 1) / STRIP_W) * ((my + g_seg_rows - 1) / g_seg_rows);
+    const long n = tiles_at(mx, my, g_seg_rows.load());
     if (n > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
     return (int)n;
 }
@@ -2271,17 +2365,13 @@ int dclaw_split_workspace_size(int mx, int my)
 int dclaw_set_split_mode(int mode)
 {
     if (mode < 0 || mode > 3) return fail(DCLAW_EINVAL, "split mode must be 0, 1, 2 or 3");
-    const int prev = g_split_mode;
-    g_split_mode = mode;
-    return prev;
+    return g_split_mode.exchange(mode);
 }
 
 int dclaw_set_dry_rows(int rows)
 {
     if (rows < 1 || rows > 65536) return fail(DCLAW_EINVAL, "dry rows must be in [1, 65536]");
-    const int prev = g_dry_rows;
-    g_dry_rows = rows;
-    return prev;
+    return g_dry_rows.exchange(rows);
 }
 
 int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* aux, int maux,
@@ -2290,8 +2380,9 @@ int dclaw_stepgrid_split_dev(const double* q_in, double* q_out, const double* au
                              double* cfl_dev, int* tile_flags_dev, int n_flags, void* stream)
 {
     if (!tile_flags_dev) return fail(DCLAW_EINVAL, "null tile_flags");
-    return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
-                           cfl_dev, bc, (hipStream_t)stream, tile_flags_dev, n_flags);
+    return launch_stepgrid(global_tuning(), q_in, q_out, aux, nullptr, maux, mx, my, dx, dy,
+                           dt_dev, params_dev, cfl_dev, bc, (hipStream_t)stream, tile_flags_dev,
+                           n_flags);
 }
 
 }  // extern "C"
@@ -2336,20 +2427,148 @@ int dclaw_stepgrid_split_static_dev(const double* q_in, double* q_out, const dou
                                     const dclaw_params* params_dev, double* cfl_dev,
                                     void* stream)
 {
-    const int need = dclaw_split_workspace_size(mx, my);
-    if (need < 0) return need;
-    if (need > STATIC_FLAG_INTS)
+    const Tuning tn = global_tuning();
+    if (mx < 1 || my < 1) return fail(DCLAW_EINVAL, "mx and my must be >= 1");
+    const long ntiles = tiles_at(mx, my, tn.seg_rows);
+    if (3L * ntiles + 4 > STATIC_FLAG_INTS)
         return fail(DCLAW_EINVAL, "patch has more tiles than the static workspace holds ("
                                   "dclaw_split_workspace_size > " +
                                       std::to_string(STATIC_FLAG_INTS) +
                                       " ints): use dclaw_stepgrid_split_dev");
     int* flags = nullptr;
     if (int rc = static_scratch(&flags)) return rc;
-    return launch_stepgrid(q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev, params_dev,
-                           cfl_dev, bc, (hipStream_t)stream, flags, need);
+    return launch_stepgrid(tn, q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev,
+                           params_dev, cfl_dev, bc, (hipStream_t)stream, flags,
+                           (int)(3L * ntiles + 4));
 }
 
 int dclaw_static_workspace_ints(void) { return STATIC_FLAG_INTS; }
+
+}  // extern "C"
+
+// A caller's own library context: the settings its launches read instead of
+// the library-wide ones, and the resident split step's tile workspace,
+// allocated on the device current at the handle's first step and grown when
+// a patch (or a shorter segment) needs more tiles.  One handle serves one
+// time loop at a time: its calls are issued in order by one thread, on one
+// stream, as the resident bookkeeping assumes.
+struct dclaw_handle_s {
+    Tuning tn;
+    int* ws = nullptr;
+    long ws_ints = 0;
+    int device = -1;
+};
+
+extern "C" {
+
+int dclaw_create(dclaw_handle* out)
+{
+    if (!out) return fail(DCLAW_EINVAL, "null handle pointer");
+    *out = nullptr;
+    dclaw_handle h = new (std::nothrow) dclaw_handle_s;
+    if (!h) return fail(DCLAW_ENOMEM, "out of host memory");
+    h->tn = global_tuning();
+    *out = h;
+    return DCLAW_OK;
+}
+
+int dclaw_destroy(dclaw_handle h)
+{
+    if (!h) return DCLAW_OK;
+    int rc = DCLAW_OK;
+    if (h->ws) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != h->device) e = hipSetDevice(h->device);
+        // hipFree waits for the device's work, the last step included
+        if (e == hipSuccess) e = hipFree(h->ws);
+        if (e != hipSuccess) rc = hip_fail(e, "freeing the handle's workspace");
+        if (cur >= 0 && cur != h->device) (void)hipSetDevice(cur);
+    }
+    delete h;
+    return rc;
+}
+
+int dclaw_handle_set_seg_rows(dclaw_handle h, int seg_rows)
+{
+    if (!h) return fail(DCLAW_EINVAL, "null handle");
+    if (seg_rows < 1 || seg_rows > 65536) return fail(DCLAW_EINVAL, "seg_rows must be in [1, 65536]");
+    const int prev = h->tn.seg_rows;
+    h->tn.seg_rows = seg_rows;
+    return prev;
+}
+
+int dclaw_handle_set_split_mode(dclaw_handle h, int mode)
+{
+    if (!h) return fail(DCLAW_EINVAL, "null handle");
+    if (mode < 0 || mode > 3) return fail(DCLAW_EINVAL, "split mode must be 0, 1, 2 or 3");
+    const int prev = h->tn.split_mode;
+    h->tn.split_mode = mode;
+    return prev;
+}
+
+int dclaw_handle_set_dry_rows(dclaw_handle h, int rows)
+{
+    if (!h) return fail(DCLAW_EINVAL, "null handle");
+    if (rows < 1 || rows > 65536) return fail(DCLAW_EINVAL, "dry rows must be in [1, 65536]");
+    const int prev = h->tn.dry_rows;
+    h->tn.dry_rows = rows;
+    return prev;
+}
+
+int dclaw_handle_tile_count(dclaw_handle h, int mx, int my)
+{
+    if (!h) return fail(DCLAW_EINVAL, "null handle");
+    if (mx < 1 || my < 1) return fail(DCLAW_EINVAL, "mx and my must be >= 1");
+    const long n = tiles_at(mx, my, h->tn.seg_rows);
+    if (split_resident_ints(n) > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
+    return (int)n;
+}
+
+int dclaw_handle_state_written(dclaw_handle h, void* stream)
+{
+    if (!h) return fail(DCLAW_EINVAL, "null handle");
+    if (!h->ws) return DCLAW_OK;
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != h->device) return fail(DCLAW_EINVAL, "the handle's workspace is on another device");
+    hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((h->ws_ints + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, h->ws, (int)h->ws_ints);
+    HIP_TRY(hipGetLastError());
+    return DCLAW_OK;
+}
+
+int dclaw_handle_stepgrid_dev(dclaw_handle h, const double* q_in, double* q_out,
+                              const double* aux, int maux, int mx, int my, double dx, double dy,
+                              const int* bc, const double* dt_dev,
+                              const dclaw_params* params_dev, double* cfl_dev, void* stream)
+{
+    if (!h) return fail(DCLAW_EINVAL, "null handle");
+    const int ntiles = dclaw_handle_tile_count(h, mx, my);
+    if (ntiles < 0) return ntiles;
+    const long need = split_resident_ints(ntiles);
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (h->ws && dev != h->device)
+        return fail(DCLAW_EINVAL, "the handle's workspace is on another device");
+    const hipStream_t st = (hipStream_t)stream;
+    if (h->ws_ints < need) {
+        if (h->ws) {
+            HIP_TRY(hipFree(h->ws));
+            h->ws = nullptr;
+            h->ws_ints = 0;
+        }
+        HIP_TRY(hipMalloc(&h->ws, (size_t)need * sizeof(int)));
+        h->ws_ints = need;
+        h->device = dev;
+        // zero bookkeeping: the first step runs every tile
+        hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((need + 255) / 256)), dim3(256), 0, st,
+                           h->ws, (int)need);
+        HIP_TRY(hipGetLastError());
+    }
+    return launch_stepgrid(h->tn, q_in, q_out, aux, nullptr, maux, mx, my, dx, dy, dt_dev,
+                           params_dev, cfl_dev, bc, st, h->ws, (int)need);
+}
 
 size_t dclaw_curvature_workspace_doubles(int mx, int my)
 {
@@ -2488,12 +2707,33 @@ int dclaw_step_batched_ws(const dclaw_patch* patches_dev, int npatch, const int3
     if (!tile_flags_dev) {
         hipLaunchKernelGGL(k_step_batched<MODE_STEP>, dim3(ntiles), dim3(64), 0, st, patches_dev,
                            tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev, b0, b1, b2, b3,
-                           nullptr, nullptr);
+                           nullptr, nullptr, nullptr);
         HIP_TRY(hipGetLastError());
         return DCLAW_OK;
     }
     const int* order = nullptr;
+    const int* dry_age = nullptr;
     if ((long)n_ws >= 2L * ntiles + 4) {
+        const bool resident = (long)n_ws >= batched_resident_ints(ntiles, npatch);
+        if (resident) {
+            int* gmeta = tile_flags_dev + 2 * ntiles + 4;
+            int* pmeta = gmeta + BSKIP_GMETA;
+            if (b0 && b1 && b2 && b3) {
+                // resident workspace, boundary conditions on load: tiles that
+                // stayed dry are not read or written
+                int* age = pmeta + (size_t)BSKIP_PMETA * npatch;
+                hipLaunchKernelGGL(k_bskip_meta, dim3(1), dim3(256), 0, st, patches_dev, npatch,
+                                   gmeta, pmeta, seg_rows, ntiles);
+                hipLaunchKernelGGL(k_bdry_age, dim3((ntiles + 255) / 256), dim3(256), 0, st,
+                                   patches_dev, tiles_dev, tile_flags_dev, age, pmeta, ntiles,
+                                   seg_rows, (STRIP_HALO + seg_rows - 1) / seg_rows);
+                dry_age = age;
+            } else {
+                // stored ghosts: every tile runs, and the next resident call
+                // starts over
+                hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, gmeta, BSKIP_GMETA);
+            }
+        }
         // the tiles the previous call flagged first
         int* ord = tile_flags_dev + ntiles;
         int* counters = tile_flags_dev + 2 * ntiles + 2;
@@ -2504,7 +2744,7 @@ int dclaw_step_batched_ws(const dclaw_patch* patches_dev, int npatch, const int3
     }
     hipLaunchKernelGGL(k_step_batched<MODE_STEP_NOSRC2>, dim3(ntiles), dim3(64), 0, st,
                        patches_dev, tiles_dev, seg_rows, dt_dev, params_dev, cfl_dev, b0, b1, b2,
-                       b3, tile_flags_dev, order);
+                       b3, tile_flags_dev, order, dry_age);
     HIP_TRY(hipGetLastError());
     const long ntask = (long)ntiles * ((STRIP_W * seg_rows + 63) / 64);
     hipLaunchKernelGGL(k_src2_batched, dim3((int)std::min<long>(ntask, 16384)), dim3(64), 0, st,
@@ -2512,6 +2752,14 @@ int dclaw_step_batched_ws(const dclaw_patch* patches_dev, int npatch, const int3
                        params_dev);
     HIP_TRY(hipGetLastError());
     return DCLAW_OK;
+}
+
+int dclaw_batched_resident_size(int ntiles, int npatch)
+{
+    if (ntiles < 1 || npatch < 1) return fail(DCLAW_EINVAL, "empty batch");
+    const long n = batched_resident_ints(ntiles, npatch);
+    if (n > 0x7fffffff) return fail(DCLAW_EINVAL, "too many tiles");
+    return (int)n;
 }
 
 int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
@@ -2917,8 +3165,9 @@ static int stepgrid_host(double* q, double* aux, int mbc, int mx, int my, int ma
                                    nullptr, c->d_scalars, g_cfg.d_params, c->d_scalars + 1,
                                    c->d_curv, nw, c->st);
     } else {
-        rc = launch_stepgrid(c->d_qa, c->d_qb, c->d_aux, c->d_aux, maux, mx, my, dx, dy,
-                             c->d_scalars, g_cfg.d_params, c->d_scalars + 1, nullptr, c->st);
+        rc = launch_stepgrid(global_tuning(), c->d_qa, c->d_qb, c->d_aux, c->d_aux, maux, mx,
+                             my, dx, dy, c->d_scalars, g_cfg.d_params, c->d_scalars + 1, nullptr,
+                             c->st);
     }
     if (rc) return rc;
     if ((rc = d2h(c, q, c->d_qb, nq)) || (rc = d2h(c, aux, c->d_aux, na)) ||

@@ -180,6 +180,70 @@ def stepgrid_split(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, m
         _ptr(params_dev), _ptr(cfl_dev), _ptr(flags), n, _stream()))
 
 
+class Handle:
+    """A library handle (dclaw_create): its own segment height, split mode and
+    dry-pass rows instead of the library-wide ones, and the resident split
+    step's tile workspace, so time loops with different settings run side by
+    side.  One handle serves one time loop (one thread, one stream)."""
+
+    def __init__(self, seg_rows: Optional[int] = None, split_mode: Optional[str] = None,
+                 dry_rows: Optional[int] = None):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        _lib.check(L.dclaw_create(ctypes.byref(h)))
+        self._h = h
+        if seg_rows is not None:
+            self.set_seg_rows(seg_rows)
+        if split_mode is not None:
+            self.set_split_mode(split_mode)
+        if dry_rows is not None:
+            _lib.check(min(0, L.dclaw_handle_set_dry_rows(self._h, int(dry_rows))))
+
+    def set_seg_rows(self, rows: int) -> int:
+        """Segment height of this handle's launches; returns the previous one."""
+        prev = _lib.lib().dclaw_handle_set_seg_rows(self._h, int(rows))
+        _lib.check(min(0, prev))
+        return prev
+
+    def set_split_mode(self, mode: str) -> None:
+        """"single" (tile order), "ordered" (wet-first, dry-tile skipping),
+        "dry" (a dry pass first) or "hybrid" (as stepgrid_split)."""
+        if mode not in SPLIT_MODES:
+            raise ValueError(f"mode must be one of {SPLIT_MODES}")
+        _lib.check(min(0, _lib.lib().dclaw_handle_set_split_mode(self._h,
+                                                                  SPLIT_MODES.index(mode))))
+
+    def tile_count(self, mx: int, my: int) -> int:
+        n = _lib.lib().dclaw_handle_tile_count(self._h, mx, my)
+        _lib.check(min(0, n))
+        return n
+
+    def step(self, q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int,
+             mx: int, my: int, dx: float, dy: float, bc, dt_dev: torch.Tensor,
+             params_dev: torch.Tensor, cfl_dev: torch.Tensor) -> None:
+        """The split step with this handle's settings and workspace (bc None:
+        stored ghosts)."""
+        arr = None if bc is None else (ctypes.c_int * 4)(*bc)
+        _lib.check(_lib.lib().dclaw_handle_stepgrid_dev(
+            self._h, _ptr(q_in), _ptr(q_out), _ptr(aux), maux, mx, my, dx, dy, arr,
+            _ptr(dt_dev), _ptr(params_dev), _ptr(cfl_dev), _stream()))
+
+    def state_written(self) -> None:
+        """The loop's state was written outside its steps."""
+        _lib.check(_lib.lib().dclaw_handle_state_written(self._h, _stream()))
+
+    def close(self) -> None:
+        h, self._h = self._h, None
+        if h is not None and h.value:
+            _lib.check(_lib.lib().dclaw_destroy(h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
 def stepgrid_curvature(q_in: torch.Tensor, q_out: torch.Tensor, aux: torch.Tensor, maux: int,
                        mx: int, my: int, dx: float, dy: float, bc, dt_dev: torch.Tensor,
                        params_dev: torch.Tensor, cfl_dev: torch.Tensor,
@@ -240,8 +304,12 @@ class PatchRunner:
 
     def __init__(self, q: np.ndarray, aux: np.ndarray, params: DigParams, dx: float, dy: float,
                  bc=(BC_EXTRAP,) * 4, device="cuda", split_src2: bool = True,
-                 split_mode: str = "ordered"):
+                 split_mode: str = "ordered", handle: Optional[Handle] = None):
+        """handle: step through that library handle (its own settings and
+        workspace; split_mode is then the handle's) instead of the library-wide
+        settings and this runner's workspace."""
         self.split_src2 = split_src2
+        self.handle = handle
         if split_mode not in SPLIT_MODES:
             raise ValueError(f"split_mode must be one of {SPLIT_MODES}")
         self.split_mode = split_mode
@@ -276,6 +344,8 @@ class PatchRunner:
         skipping (which assumes the state changes only through step) starts
         over."""
         self.flags.zero_()
+        if self.handle is not None:
+            self.handle.state_written()
 
     def step(self, fused_bc: bool = True) -> None:
         """bound + stepgrid (boundary conditions applied on load when fused_bc);
@@ -286,6 +356,9 @@ class PatchRunner:
             stepgrid_curvature(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx,
                                self.dy, self.bc if fused_bc else None, self.dt, self.params,
                                self.cfl, self.curv_work)
+        elif fused_bc and self.split_src2 and self.handle is not None:
+            self.handle.step(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx,
+                             self.dy, self.bc, self.dt, self.params, self.cfl)
         elif fused_bc and self.split_src2:
             stepgrid_split(self.qa, self.qb, self.bed, self.maux, self.mx, self.my, self.dx, self.dy,
                            self.bc, self.dt, self.params, self.cfl, self.flags, self.split_mode)
@@ -447,8 +520,12 @@ class Ensemble:
         self.ntiles = nt
         self.tiles = torch.tensor(list(tiles), dtype=torch.int32, device=device)
         # src2 flags, plus (ordered) the tile order built from the previous
-        # step's flags and its counters
-        self.flags = torch.zeros(2 * nt + 4 if ordered else nt, dtype=torch.int32, device=device)
+        # step's flags, its counters and the dry-tile bookkeeping of the
+        # resident step (this class swaps each member's buffers every step
+        # and writes neither in between; state_written() otherwise)
+        nws = L.dclaw_batched_resident_size(nt, self.n) if ordered else nt
+        _lib.check(min(0, nws))
+        self.flags = torch.zeros(nws, dtype=torch.int32, device=device)
         self.dx = [m.patch.dx for m in members]
         self.dy = [m.patch.dy for m in members]
         self._patches = [self._patch_table(False, device), self._patch_table(True, device)]
@@ -467,6 +544,16 @@ class Ensemble:
 
     def current(self, k: int) -> torch.Tensor:
         return self.qb[k] if self.flip else self.qa[k]
+
+    def state_written(self) -> None:
+        """Call after writing a member's state directly: the batched step's
+        dry-tile skipping (which assumes the states change only through step)
+        starts over (the four ints after the order's counters say whether
+        the bookkeeping describes the last call; the previous flags still
+        give the next step its wet-first order)."""
+        if self.flags.numel() > 2 * self.ntiles + 4:
+            self.flags[2 * self.ntiles + 4:2 * self.ntiles + 8].zero_()
+        self._plain_dirty = True
 
     def step(self) -> None:
         """One launch for all members; boundary conditions applied on load."""

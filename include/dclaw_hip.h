@@ -232,6 +232,55 @@ int dclaw_stepgrid_split_static_dev(const double* q_in, double* q_out, const dou
 /* Ints of the static workspace of dclaw_stepgrid_split_static_dev. */
 int dclaw_static_workspace_ints(void);
 
+/* Handle-based use of the single-patch step: a dclaw_handle carries its own
+ * tuning settings (segment height, split mode, dry-pass rows; at creation a
+ * copy of the library-wide values that dclaw_set_seg_rows, dclaw_set_split_mode
+ * and dclaw_set_dry_rows set) and owns the resident split step's tile
+ * workspace, so time loops with different settings run side by side and no
+ * caller passes a workspace.  The library-wide calls above stay as they are.
+ * Replaces the module-global configuration of the reference's per-patch step
+ * (the reference reads it from digclaw_module, digclaw_module.f90:15-52, and
+ * GeoClaw's geo_data): the physical parameters still come in the
+ * dclaw_params record of each call.  One handle serves one time loop: its
+ * calls come from one thread, in order, on one stream.  Every function
+ * returns a negative error for a null handle. */
+typedef struct dclaw_handle_s* dclaw_handle;
+
+/* New handle in *out (settings copied from the library-wide values; no
+ * device memory until its first step). */
+int dclaw_create(dclaw_handle* out);
+
+/* Frees the handle and its device workspace (waiting for the device's work);
+ * a null handle is accepted. */
+int dclaw_destroy(dclaw_handle h);
+
+/* The handle's settings; each returns the previous value (the ranges of the
+ * library-wide setters). */
+int dclaw_handle_set_seg_rows(dclaw_handle h, int seg_rows);
+int dclaw_handle_set_split_mode(dclaw_handle h, int mode);
+int dclaw_handle_set_dry_rows(dclaw_handle h, int rows);
+
+/* Tiles of an mx x my patch at the handle's segment height. */
+int dclaw_handle_tile_count(dclaw_handle h, int mx, int my);
+
+/* The split step of dclaw_stepgrid_split_dev with the handle's settings and
+ * its own resident workspace (allocated, or grown, on the current device at
+ * the first step that needs it): the wet-first tile order and, for a loop
+ * that swaps q_in and q_out every call with boundary conditions on load
+ * (every bc code nonzero), the skipping of tiles that stayed dry.  Same
+ * result as every other form of the step.  Replaces GeoClaw stepgrid
+ * (b4step2.f90:7, flux2fw.f:8, rpn2_dclaw.f:24, rpt2_dclaw.f:16,
+ * src2.f90:27). */
+int dclaw_handle_stepgrid_dev(dclaw_handle h, const double* q_in, double* q_out,
+                              const double* aux, int maux, int mx, int my, double dx, double dy,
+                              const int* bc, const double* dt_dev,
+                              const dclaw_params* params_dev, double* cfl_dev, void* stream);
+
+/* Tells the handle that its loop's state was written outside its steps (the
+ * dry-tile skipping starts over at the next step).  No reference
+ * counterpart. */
+int dclaw_handle_state_written(dclaw_handle h, void* stream);
+
 /* The step with src2's curvature term (params.curvature = 1; src2.f90:133-156).
  * The reference sets gz once per src2 call (:87) and adds each wet cell's
  * curvature acceleration max(u^2 b_xx + v^2 b_yy + 2uv b_xy + u^2 dtheta, 0)
@@ -279,8 +328,19 @@ int dclaw_step_batched(const dclaw_patch* patches_dev, int npatch, const int32_t
 /* dclaw_step_batched with the workspace size given: with n_ws >= 2 * ntiles +
  * 4 ints in tile_flags_dev the tiles the previous call flagged are
  * dispatched first (keep the workspace between steps; any contents give a
- * valid order), else as dclaw_step_batched.  Same result.  No reference
- * counterpart (a GPU scheduling choice). */
+ * valid order), else as dclaw_step_batched.  With n_ws >=
+ * dclaw_batched_resident_size(ntiles, npatch) (a "resident" workspace) and
+ * boundary conditions on load (every bc code nonzero), the tiles of a patch
+ * whose neighbourhood stayed dry on that patch's last three inputs are
+ * neither read nor written, as in the single-patch split step: for a loop
+ * that swaps each patch's q_in and q_out every call and changes neither in
+ * between, with the work list of dclaw_worklist_build (the library checks
+ * each patch's swap, the segment height and the list's shape on the device
+ * and starts over where they break; a caller that writes a state between
+ * calls zeroes the workspace, or only its four ints from 2 * ntiles + 4,
+ * which keeps the wet-first order).  Same result in every case.  No
+ * reference counterpart (a GPU scheduling choice). */
+int dclaw_batched_resident_size(int ntiles, int npatch);
 int dclaw_step_batched_ws(const dclaw_patch* patches_dev, int npatch, const int32_t* tiles_dev,
                           int ntiles, int seg_rows, const int* bc, const double* dt_dev,
                           const dclaw_params* params_dev, double* cfl_dev, int* tile_flags_dev,

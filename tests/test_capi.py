@@ -120,3 +120,33 @@ def test_adjoint_workspace_and_validation(L):
     assert L.dclaw_adjoint_step_dev(*args(1, 10, n, mx=1)) == -1
     assert b"mx, my >= 2" in L.dclaw_last_error()
     assert L.dclaw_gauges_adj_dev(1, 2, 1, 10, 6, 0.0, 0.0, 1.0, 1.0, 1e-3, 3, 0, 4, 5, None) == -1
+
+
+def test_handle_settings(L):
+    """Handles take their settings from the library-wide values at creation,
+    keep their own afterwards and reject bad values; no device memory is
+    touched before a step."""
+    import ctypes
+    prev = L.dclaw_set_seg_rows(12)
+    try:
+        h = ctypes.c_void_p()
+        assert L.dclaw_create(ctypes.byref(h)) == 0 and h.value
+        assert L.dclaw_handle_tile_count(h, 300, 200) == 6 * 17
+        assert L.dclaw_handle_set_seg_rows(h, 4) == 12
+        assert L.dclaw_set_seg_rows(12) == 12          # the library-wide value is unchanged
+        assert L.dclaw_handle_tile_count(h, 300, 200) == 6 * 50
+        assert L.dclaw_handle_set_seg_rows(h, 0) == -1 and b"seg_rows" in L.dclaw_last_error()
+        assert L.dclaw_handle_set_split_mode(h, 3) == 1
+        assert L.dclaw_handle_set_split_mode(h, 4) == -1
+        assert L.dclaw_handle_set_dry_rows(h, 16) == 48
+        assert L.dclaw_handle_tile_count(h, 0, 5) == -1
+        assert L.dclaw_handle_state_written(h, None) == 0     # nothing allocated yet
+        assert L.dclaw_handle_stepgrid_dev(None, 1, 2, 3, 1, 10, 10, 1.0, 1.0, None, 4, 5, 6,
+                                           None) == -1
+        assert b"null handle" in L.dclaw_last_error()
+        assert L.dclaw_destroy(h) == 0
+        assert L.dclaw_destroy(None) == 0
+        assert L.dclaw_handle_set_seg_rows(None, 4) == -1
+        assert L.dclaw_create(None) == -1
+    finally:
+        L.dclaw_set_seg_rows(prev)

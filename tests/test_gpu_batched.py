@@ -100,6 +100,63 @@ def test_c2_ensemble_256(dev, oracle_mod):
         assert (qo[0][2:-2, 2:-2] > pt.params.dry_tolerance).sum() > 100
 
 
+@pytest.mark.parametrize("bc,seg", [((1, 1, 1, 1), 8), ((3, 1, 3, 3), 2)])
+def test_batched_resident_skips_dry_tiles(dev, bc, seg):
+    """The resident batched split step (Ensemble's workspace) leaves each
+    member's tiles whose neighbourhood stayed dry unread and unwritten, and
+    its states and dt equal the same ensemble's without skipping (a workspace
+    of the ordered step only) bit for bit over 160 steps, through a state
+    written from outside in the middle (the bookkeeping starts over)."""
+    from dclaw_amd import _lib
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    L = _lib.lib()
+    members = [W.Member(W.fractal_sources(300, 200, dx=2.0, seed=5, nsrc=6), 0),
+               W.Member(W.fractal_sources(130, 250, dx=2.0, seed=6, nsrc=3), 1)]
+    members += _members(n=3, seed=11)
+    print(f"[resident batched] bc {bc} seg {seg}", flush=True)
+    a = S.Ensemble(members, bc=bc, seg_rows=seg)
+    b = S.Ensemble(members, bc=bc, seg_rows=seg)
+    nt = a.ntiles
+    assert a.flags.numel() == L.dclaw_batched_resident_size(nt, len(members))
+    b.flags = dev.zeros(2 * nt + 4, dtype=dev.int32, device="cuda")
+    for e in (a, b):
+        e.dt.fill_(1e-3)
+    for n in range(80):
+        for e in (a, b):
+            e.step()
+            e.cfl.zero_()
+        if n == 40:
+            for e in (a, b):
+                e.current(1).copy_(e.current(1).clone())
+            a.state_written()
+    a.run(80, a.dt.clone(), 0.9, 1e9)
+    b.run(80, b.dt.clone(), 0.9, 1e9)
+    dev.cuda.synchronize()
+    for k in range(len(members)):
+        qa, qb = a.q(k), b.q(k)
+        assert np.array_equal(qa, qb), f"member {k}: {(qa != qb).sum()} values differ"
+    assert dev.equal(a.dt, b.dt)
+    # member 0's far dry tiles are skipped: a marker put into the buffer the
+    # next step writes survives that step
+    npatch = len(members)
+    age = a.flags[2 * nt + 8 + 5 * npatch:3 * nt + 8 + 5 * npatch].cpu().numpy()
+    tiles = a.tiles.cpu().numpy().reshape(-1, 4)
+    mine = np.flatnonzero(tiles[:, 0] == 0)
+    old = mine[age[mine] >= 3]
+    assert old.size > 0 and (age[mine] < 3).any()
+    busy = tiles[mine[age[mine] < 3]]
+    far = max(old, key=lambda t: min(abs(tiles[t, 1] - bx) * 58 + abs(tiles[t, 2] - bg) * seg
+                                     for _, bx, bg, _ in busy))
+    i = 1 + tiles[far, 1] * 58
+    j = 1 + tiles[far, 2] * seg
+    nxt = a.qa[0] if a.flip else a.qb[0]
+    nxt[j + 1, i + 1, 6] = 12345.0
+    a.step()
+    dev.cuda.synchronize()
+    assert a.current(0)[j + 1, i + 1, 6].item() == 12345.0
+
+
 def test_fused_bc_equals_bound_then_step(dev):
     from dclaw_amd import step as S
     pt = slope_patch(90, 70, slope=0.8, phi_deg=25.0, seed=2)
