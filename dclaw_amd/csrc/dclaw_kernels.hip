@@ -496,7 +496,22 @@ __global__ void k_skip_meta(int* meta, const double* q_in, const double* q_out, 
     meta[6] = (int)(unsigned)(in >> 32);
 }
 
-// Age of each tile: consecutive inputs with no flagged tile within the read
+// Whether the part of neighbour tile (dx, dy) (strips, segments; (0, 0)
+// the tile itself) that a tile's march reads holds no cell flagged wet, from
+// the neighbour's flag: with segments of at least STRIP_HALO rows a march
+// reads 3 columns of each side neighbour, 3 rows of the segments below and
+// above, and their corners -- the neighbour's bands facing the tile.
+__device__ __forceinline__ bool halo_band_dry(int f, int dx, int dy)
+{
+    if (dx == 0 && dy == 0) return f == 0;
+    const int bx = dx < 0 ? TF_EAST : TF_WEST;      // the columns facing the tile
+    const int by = dy < 0 ? TF_NORTH : TF_SOUTH;    // the rows facing the tile
+    if (dy == 0) return (f & bx) == 0;
+    if (dx == 0) return (f & by) == 0;
+    return (f & bx) == 0 || (f & by) == 0;          // the corner lies in both bands
+}
+
+// Age of each tile: consecutive inputs with no flagged cell within the read
 // halo (0 when the flags do not describe this call's input).
 __global__ void k_dry_age(const int* __restrict__ flags, int* age, const int* __restrict__ meta,
                           int nstrips, int nseg, int ry)
@@ -509,9 +524,15 @@ __global__ void k_dry_age(const int* __restrict__ flags, int* age, const int* __
     }
     const int strip = t % nstrips, seg = t / nstrips;
     bool dry = true;
-    for (int s = max(0, seg - ry); s <= min(nseg - 1, seg + ry); s++)
-        for (int x = max(0, strip - 1); x <= min(nstrips - 1, strip + 1); x++)
-            dry = dry && flags[s * nstrips + x] == 0;
+    if (ry == 1) {
+        for (int s = max(0, seg - 1); s <= min(nseg - 1, seg + 1); s++)
+            for (int x = max(0, strip - 1); x <= min(nstrips - 1, strip + 1); x++)
+                dry = dry && halo_band_dry(flags[s * nstrips + x], x - strip, s - seg);
+    } else {
+        for (int s = max(0, seg - ry); s <= min(nseg - 1, seg + ry); s++)
+            for (int x = max(0, strip - 1); x <= min(nstrips - 1, strip + 1); x++)
+                dry = dry && flags[s * nstrips + x] == 0;
+    }
     age[t] = dry ? min(age[t] + 1, 3) : 0;
 }
 
@@ -611,7 +632,8 @@ __global__ void k_bdry_age(const dclaw_patch* __restrict__ patches,
             const long nb = base + (long)x * ng + sg;
             const bool placed = nb >= 0 && nb < ntiles && tiles[4 * nb + 0] == p &&
                                 tiles[4 * nb + 1] == x && tiles[4 * nb + 2] == sg;
-            dry = dry && placed && flags[nb] == 0;
+            dry = dry && placed &&
+                  (ry == 1 ? halo_band_dry(flags[nb], x - s, sg - g) : flags[nb] == 0);
         }
     age[t] = dry ? min(age[t] + 1, 3) : 0;
 }
@@ -701,8 +723,10 @@ __global__ __launch_bounds__(64) void k_stepgrid_hybrid(const double* __restrict
     const double cfl = strip_step_dispatch<MODE_STEP_NOSRC2>(L, P, A);
     wave_max_to(cfl, cfl_dev);
     if (nsub > 1 && threadIdx.x == 0) {
-        const int v = *flag0;
-        for (int k = 1; k < nsub; k++) flag0[k * nstrips] = v;
+        // the group's bands are not its segments': every band of each
+        // segment counts as wet
+        const int v = *flag0 ? TF_ALL : 0;
+        for (int k = 0; k < nsub; k++) flag0[k * nstrips] = v;
     }
 }
 

@@ -42,6 +42,18 @@ constexpr int STRIP_LANES = 64;
 constexpr int DRY_CHUNK = 8;       // rows per load batch of the whole-segment dry check
 constexpr int CLASSIFY_CHUNK = 4;  // rows per load batch of the split step's dry pass
 constexpr int STRIP_HALO = 3;
+
+// Tile flag of the split step (MODE_STEP_NOSRC2): 0 when no output cell of
+// the tile needs src2, else TF_ANY plus a bit for each band of the tile
+// that holds such a cell -- its first and last STRIP_HALO columns (west:
+// i0..i0+2, east: i0+55..i0+57) and rows (south: jA..jA+2, north:
+// jB-2..jB), the cells a neighbouring tile's march reads.
+constexpr int TF_ANY = 1;
+constexpr int TF_WEST = 2;
+constexpr int TF_EAST = 4;
+constexpr int TF_SOUTH = 8;
+constexpr int TF_NORTH = 16;
+constexpr int TF_ALL = TF_ANY | TF_WEST | TF_EAST | TF_SOUTH | TF_NORTH;
 constexpr int STRIP_W = STRIP_LANES - 2 * STRIP_HALO;  // 58 interior columns
 
 struct StripArgs {
@@ -54,7 +66,8 @@ struct StripArgs {
     int i0, jA, jB;        // interior strip start column, segment rows [jA, jB]
     int bc[4];             // bc codes (xlo, xhi, ylo, yhi) applied on load; 0 = use stored ghosts
     double *fm, *fp, *gm, *gp;   // step2 outputs (FLUX mode; same shape as q, zero-filled)
-    int* tile_flag;              // MODE_STEP_NOSRC2: set to 1 if src2 has wet cells here, else 0
+    int* tile_flag;              // MODE_STEP_NOSRC2: the tile flag (TF_* bits), 0 if no cell
+                                 // here needs src2
     bool known_wet = false;      // skip the whole-segment dry check (strip_classify found the
                                  // segment not dry and already wrote the dry state)
 };

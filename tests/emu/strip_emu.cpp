@@ -181,6 +181,43 @@ extern "C" int emu_stepgrid(const dclaw_params* P, const double* q_in, double* q
     return 0;
 }
 
+// The strip pass of the split step alone: q_out receives the state before
+// src2 (the cells src2 would zero already zeroed) and flags[segment *
+// strips + strip] each tile's flag (TF_* bits).
+extern "C" int emu_strip_flags(const dclaw_params* P, const double* q_in, double* q_out,
+                               const double* aux, int maux, int mx, int my, double dx, double dy,
+                               double dt, int seg_rows, const int* bc, int* flags)
+{
+    double cflgrid = 0.0;
+    const int nstrips = (mx + STRIP_W - 1) / STRIP_W;
+    const int nseg = (my + seg_rows - 1) / seg_rows;
+    for (int t = 0; t < nstrips * nseg; t++) flags[t] = -1;
+    for (int s = 0; s < nstrips; s++)
+        for (int g = 0; g < nseg; g++) {
+            StripArgs A;
+            A.q_in = q_in;
+            A.q_out = q_out;
+            A.aux = aux;
+            A.aux_out = nullptr;
+            A.maux = maux;
+            A.mx = mx;
+            A.my = my;
+            A.dx = dx;
+            A.dy = dy;
+            A.dt = dt;
+            A.i0 = 1 + s * STRIP_W;
+            A.jA = 1 + g * seg_rows;
+            A.jB = std::min(my, A.jA + seg_rows - 1);
+            for (int k = 0; k < 4; k++) A.bc[k] = bc ? bc[k] : 0;
+            A.fm = A.fp = A.gm = A.gp = nullptr;
+            A.tile_flag = &flags[(size_t)g * nstrips + s];
+            run_strip(*P, A, &cflgrid, MODE_STEP_NOSRC2);
+        }
+    for (int t = 0; t < nstrips * nseg; t++)
+        if (flags[t] < 0) return -1;       // a tile did not report
+    return 0;
+}
+
 // The step with src2 split out: the strip body without src2 flags its tiles,
 // then src2 runs over the interior cells of the flagged tiles.
 extern "C" int emu_stepgrid_split(const dclaw_params* P, const double* q_in, double* q_out,

@@ -388,3 +388,49 @@ def test_dry_scan_alignment(oracle_mod, emu_lib, in_off, out_off):
             if ref is None:
                 ref = qa
             assert np.array_equal(qa, ref)
+
+
+@pytest.mark.parametrize("mx,my,seg,bc", [
+    (130, 70, 8, (3, 3, 3, 3)), (200, 90, 3, (1, 1, 1, 1)), (59, 33, 4, (1, 3, 3, 1)),
+    (175, 61, 5, None), (120, 40, 2, (1, 1, 1, 1)),
+])
+def test_tile_flag_bands(oracle_mod, emu_lib, mx, my, seg, bc):
+    """The split step's tile flag is 0 when no owned cell needs src2, else
+    TF_ANY plus one bit per band (first / last 3 columns of the strip, first
+    / last 3 rows of the segment) holding such a cell: checked against the
+    wet mask of the strip pass's own output (before src2) on every tile."""
+    from dclaw_amd.params import to_c
+    O = oracle_mod
+    pt = slope_patch(mx, my, slope=0.8, phi_deg=25.0, seed=mx + 3 * my)
+    q, aux, dt = advance_oracle(O, pt, 8, bc=bc or (1, 1, 1, 1))
+    dp = ctypes.POINTER(ctypes.c_double)
+    qi = np.array(q, order="F", copy=True)
+    qe = np.full_like(qi, np.nan, order="F")
+    auxf = np.array(aux, order="F", copy=True)
+    nstrips, nseg = -(-mx // 58), -(-my // seg)
+    flags = np.full(nstrips * nseg, -7, dtype=np.int32)
+    cp = to_c(pt.params)
+    bca = None if bc is None else (ctypes.c_int * 4)(*bc)
+    rc = emu_lib.emu_strip_flags(ctypes.byref(cp), qi.ctypes.data_as(dp), qe.ctypes.data_as(dp),
+                                 auxf.ctypes.data_as(dp), auxf.shape[0], mx, my,
+                                 ctypes.c_double(pt.dx), ctypes.c_double(pt.dy),
+                                 ctypes.c_double(dt), seg, bca,
+                                 flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    assert rc == 0
+    wet = qe[0][2:-2, 2:-2] > pt.params.dry_tolerance      # [i - 1, j - 1]
+    assert wet.any() and not wet.all()
+    nbands = 0
+    for g in range(nseg):
+        for s in range(nstrips):
+            i0, j0 = s * 58, g * seg
+            w = wet[i0:min(mx, i0 + 58), j0:min(my, j0 + seg)]
+            want = 0
+            if w.any():
+                want = 1
+                want |= 2 if w[:3].any() else 0
+                want |= 4 if w[55:58].any() else 0
+                want |= 8 if w[:, :3].any() else 0
+                want |= 16 if w[:, -3:].any() else 0
+                nbands += want != 31
+            assert flags[g * nstrips + s] == want, (s, g, flags[g * nstrips + s], want)
+    assert nbands > 0

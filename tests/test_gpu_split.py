@@ -8,6 +8,13 @@ from helpers import TOL_STEP, advance_oracle, mask_equal, rel_linf_wet, slope_pa
 pytestmark = pytest.mark.gpu
 
 
+def valid_flags(f) -> bool:
+    """Tile flags: 0, or bit 0 (a cell needs src2) with any of the four band
+    bits."""
+    f = np.asarray(f)
+    return bool(np.all((f == 0) | ((f & 1) == 1)) and np.all((f >= 0) & (f < 32)))
+
+
 @pytest.fixture(scope="module")
 def dev():
     import torch
@@ -51,7 +58,7 @@ def test_split_equals_fused(dev, oracle_mod, mx, my, seg, bc):
     assert np.array_equal(a, b), f"{(a != b).sum()} values differ"
     assert cs.item() == cf.item()
     fl = flags.cpu().numpy()
-    assert set(fl.tolist()) <= {0, 1} and fl.sum() > 0
+    assert valid_flags(fl) and (fl > 0).any()
 
 
 def test_split_runner_matches_oracle_long(dev, oracle_mod):
@@ -195,7 +202,7 @@ def test_dry_pass_equals_fused(dev, oracle_mod, mx, my, seg, dry_rows, bc):
     nwet = int(w[2 * ntiles])
     assert 0 < nwet <= ntiles
     assert w[2 * ntiles + 1] >= nwet                      # the work cursor passed the list
-    assert set(w[:ntiles].tolist()) <= {0, 1}
+    assert valid_flags(w[:ntiles])
     listed = w[ntiles:ntiles + nwet]
     assert len(set(listed.tolist())) == nwet and listed.min() >= 0 and listed.max() < ntiles
     # unlisted tiles are dry: flag 0
@@ -316,7 +323,7 @@ def test_ordered_equals_fused(dev, oracle_mod, mx, my, seg, bc):
     assert np.array_equal(np.sort(order), np.arange(ntiles))
     assert nfirst == int(prev_flags.sum())
     assert set(order[:nfirst].tolist()) == set(np.flatnonzero(prev_flags).tolist())
-    assert set(w[:ntiles].tolist()) <= {0, 1}
+    assert valid_flags(w[:ntiles])
 
 
 @pytest.mark.parametrize("mx,my,seg,bc", [(130, 70, 8, (3, 3, 3, 3)), (59, 33, 4, (1, 3, 3, 1)),
@@ -380,7 +387,7 @@ def test_hybrid_equals_fused(dev, oracle_mod, mx, my, seg, bc):
     assert set(wet_list.tolist()) == flagged and len(wet_list) == len(flagged)
     assert len(dry_list) + len({t // nstrips // 4 * nstrips + t % nstrips for t in flagged}) \
         == nstrips * ngi
-    assert set(w[:ntiles].tolist()) <= {0, 1}
+    assert valid_flags(w[:ntiles])
 
 
 @pytest.mark.parametrize("mode", ["single", "ordered", "dry", "hybrid"])
