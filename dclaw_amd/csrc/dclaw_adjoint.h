@@ -145,7 +145,16 @@ struct AdjArgs {
     int ld;                  // plane row stride, plane_ld(mx)
     const dclaw_params* Pd;  // device copies of P and dt read by each GPU thread (or null)
     const double* dt_dev;
+    // Tile-sparse stepping (the GPU's tile passes): the marks of the tiles
+    // the passes run this call (TILE_W x TILE_H cells, tntx per row; a
+    // cell (i, j) is in tile ((j + 1) / TILE_H) * tntx + (i + 1) / TILE_W),
+    // or null when every cell runs every stage (the host emulator).
+    const int* tflags = nullptr;
+    int tntx = 0;
 };
+
+constexpr int TILE_W = 16;   // tile of the sparse passes: one wavefront of cells
+constexpr int TILE_H = 4;
 
 DC_HD int plane_ld(int mx) { return (mx + 4 + 15) / 16 * 16; }
 DC_HD size_t plane_size(int mx, int my) { return (size_t)plane_ld(mx) * (size_t)(my + 4); }
@@ -2638,6 +2647,46 @@ DC_HD void rev_prepare(const AdjArgs& A, const TP& Pz, int i, int j)
 // B7: lambda_n.  With the boundary map on load a ghost cell's q is its source
 // cell's (normal momentum negated at a wall), so its adjoint goes to the
 // source; the stored ghosts of q_n are not read and get 0.
+// The ghost cells whose source under the boundary map is interior cell
+// (i, j) -- columns -1, 0 / mx+1, mx+2 and rows -1, 0 / my+1, my+2 around
+// the cell's own -- in a fixed order: f(ci, cj, fu, fv).
+template <class F>
+DC_HD void for_ghost_images(const AdjArgs& A, int i, int j, F f)
+{
+    int gi[5], gj[5], ni = 0, nj = 0;
+    gi[ni++] = i;
+    if (i <= 2) { gi[ni++] = -1; gi[ni++] = 0; }
+    if (i >= A.mx - 1) { gi[ni++] = A.mx + 1; gi[ni++] = A.mx + 2; }
+    gj[nj++] = j;
+    if (j <= 2) { gj[nj++] = -1; gj[nj++] = 0; }
+    if (j >= A.my - 1) { gj[nj++] = A.my + 1; gj[nj++] = A.my + 2; }
+    for (int b = 0; b < nj; b++)
+        for (int a = 0; a < ni; a++) {
+            const int ci = gi[a], cj = gj[b];
+            if (interior(A, ci, cj)) continue;
+            int si, sj;
+            double fu, fv;
+            bc_source(A.bc, A.mx, A.my, ci, cj, si, sj, fu, fv);
+            if (si != i || sj != j) continue;
+            f(ci, cj, fu, fv);
+        }
+}
+
+// Whether the tile passes ran on cell (i, j) this call.
+DC_HD bool cell_live(const AdjArgs& A, int i, int j)
+{
+    return !A.tflags || A.tflags[((j + 1) / TILE_H) * A.tntx + (i + 1) / TILE_W] != 0;
+}
+
+// The bdif adjoint a dry ghost cell sends to its source (i, j) (bdif raw
+// value b there): the prepare pass's value far from wet cells, where the
+// ghost's Q0 is the dry state and only its output's bdif adjoint reaches
+// max(bdif, 0).
+DC_HD double dry_ghost_bdif(const AdjArgs& A, int ci, int cj, double b)
+{
+    return b > 0.0 ? A.lam_out[qidx(6, ci, cj, A.mx)] : 0.0;
+}
+
 DC_HD void rev_bc(const AdjArgs& A, int i, int j)
 {
     if (!bc_gathers(A, i, j)) return;   // written by the prepare pass
@@ -2647,29 +2696,18 @@ DC_HD void rev_bc(const AdjArgs& A, int i, int j)
     if (!interior(A, i, j)) {
         for (int m = 0; m < 7; m++) l[m] = 0.0;
     } else {
-        // ghost cells whose source is (i, j): columns -1, 0 / mx+1, mx+2
-        // and rows -1, 0 / my+1, my+2 around the cell's own
-        int gi[5], gj[5], ni = 0, nj = 0;
-        gi[ni++] = i;
-        if (i <= 2) { gi[ni++] = -1; gi[ni++] = 0; }
-        if (i >= A.mx - 1) { gi[ni++] = A.mx + 1; gi[ni++] = A.mx + 2; }
-        gj[nj++] = j;
-        if (j <= 2) { gj[nj++] = -1; gj[nj++] = 0; }
-        if (j >= A.my - 1) { gj[nj++] = A.my + 1; gj[nj++] = A.my + 2; }
-        for (int b = 0; b < nj; b++)
-            for (int a = 0; a < ni; a++) {
-                const int ci = gi[a], cj = gj[b];
-                if (interior(A, ci, cj)) continue;
-                int si, sj;
-                double fu, fv;
-                bc_source(A.bc, A.mx, A.my, ci, cj, si, sj, fu, fv);
-                if (si != i || sj != j) continue;
+        for_ghost_images(A, i, j, [&](int ci, int cj, double fu, double fv) {
+            if (cell_live(A, ci, cj)) {
                 const size_t c = cix(A, ci, cj);
                 for (int m = 0; m < 7; m++) {
                     const double s = m == 1 ? fu : (m == 2 ? fv : 1.0);
                     l[m] += s * wv(A, W_LQ7 + m, c);
                 }
+            } else {
+                // a ghost in a tile the passes skipped (far from wet cells)
+                l[6] += dry_ghost_bdif(A, ci, cj, A.q_in[qidx(6, i, j, A.mx)]);
             }
+        });
     }
     double* o = A.lam_in + qidx(0, i, j, A.mx);
     for (int m = 0; m < 7; m++) o[m] = l[m];

@@ -1267,8 +1267,8 @@ enum AdjStage {
 // Tiles are 16 cells wide and 4 high (64 cells: one wavefront): a tile row
 // of a workspace plane is one 128-byte line (the planes' row stride is a
 // multiple of 16 doubles), so a tile reads whole lines of its own cells.
-constexpr int ADJ_TW = 16;
-constexpr int ADJ_TH = 4;
+constexpr int ADJ_TW = dclaw_adj::TILE_W;
+constexpr int ADJ_TH = dclaw_adj::TILE_H;
 constexpr int ADJ_HALO = 5;       // cells around a wet cell whose tiles are active
 constexpr int ADJ_BLOCK = 256;    // dense passes
 constexpr int ADJ_GRID = 8192;    // persistent grid of the tile passes
@@ -1300,7 +1300,7 @@ inline unsigned adj_grid(long long ntiles)
 
 struct AdjTiles {
     int ntx, nty;
-    int* flags;       // ntx * nty, 0 between calls
+    int* flags;       // ntx * nty: the marks of this call's tiles (zeroed at its start)
     int* list;        // active tiles of the last call, in tile order
     int* count;       // [0]: their number
     int* bcount;      // marked tiles per block of 64 tiles, then their list offsets
@@ -1394,9 +1394,11 @@ __device__ __forceinline__ void adj_clear_cell(const dclaw_adj::AdjArgs& A, size
 
 // Dense work of cell n: bdif's plane (max(bdif, 0) of the loaded state), the
 // dry state as q_out (if wanted) and the adjoint of the dry map as lam_in
-// (the tile passes overwrite both in active tiles); marks the tiles within
-// ADJ_HALO of a wet cell and the tiles of the border band (i <= 2, i >= mx-1,
-// j <= 2, j >= my-1), or every tile when dry_tolerance <= 0.
+// (the tile passes overwrite both in active tiles): with the boundary map on
+// load a ghost cell gets 0 and an interior cell of the border band also the
+// bdif adjoints its dry ghost images send it (the boundary pass's sum, in its
+// order); marks the tiles within ADJ_HALO of a wet cell, or every tile when
+// dry_tolerance <= 0.
 __device__ __forceinline__ void adj_dense_cell(const dclaw_adj::AdjArgs& A, const AdjTiles& T,
                                                size_t n)
 {
@@ -1413,17 +1415,21 @@ __device__ __forceinline__ void adj_dense_cell(const dclaw_adj::AdjArgs& A, cons
         o[6] = bd;
     }
     {
-        const double* lo = A.lam_out + qidx(0, i, j, A.mx);
-        const double l6 = q7[6] > 0.0 ? lo[6] : 0.0;
+        double l6 = 0.0;
+        if (A.bc[0] == 0 || dclaw_adj::interior(A, i, j)) {
+            const double* lo = A.lam_out + qidx(0, i, j, A.mx);
+            l6 = q7[6] > 0.0 ? lo[6] : 0.0;
+            if (dclaw_adj::bc_gathers(A, i, j))
+                dclaw_adj::for_ghost_images(A, i, j, [&](int ci, int cj, double, double) {
+                    l6 += dclaw_adj::dry_ghost_bdif(A, ci, cj, q7[6]);
+                });
+        }
         double* li = A.lam_in + qidx(0, i, j, A.mx);
         for (int m = 0; m < 6; m++) li[m] = 0.0;
         li[6] = l6;
     }
-    const bool border = i <= 2 || i >= A.mx - 1 || j <= 2 || j >= A.my - 1;
     const bool wet = !(drytol > 0.0) || q7[0] > drytol;
-    if (border && !wet) {
-        T.flags[((j + 1) / ADJ_TH) * T.ntx + (i + 1) / ADJ_TW] = 1;
-    } else if (wet) {
+    if (wet) {
         const int ia = max(i - ADJ_HALO, -1), ib = min(i + ADJ_HALO, A.mx + 2);
         const int ja = max(j - ADJ_HALO, -1), jb = min(j + ADJ_HALO, A.my + 2);
         for (int ty = (ja + 1) / ADJ_TH; ty <= (jb + 1) / ADJ_TH; ty++)
@@ -1439,6 +1445,8 @@ __global__ __launch_bounds__(64) void k_adjoint(dclaw_adj::AdjArgs A, AdjTiles T
 {
     A.P = *A.Pd;
     A.dt = *A.dt_dev;
+    A.tflags = T.flags;
+    A.tntx = T.ntx;
     const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
     int k, end, step;
     adj_span(*T.count, k, end, step);
@@ -1455,6 +1463,8 @@ __global__ __launch_bounds__(64, WAVES) void k_adjointw(dclaw_adj::AdjArgs A, Ad
 {
     A.P = *A.Pd;
     A.dt = *A.dt_dev;
+    A.tflags = T.flags;
+    A.tntx = T.ntx;
     const dclaw_adj::TP Pz = dclaw_adj::zero_params(A.P);
     int k, end, step;
     adj_span(*T.count, k, end, step);
@@ -1486,6 +1496,32 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint_dense(dclaw_adj::AdjArgs 
     adj_dense_cell(A, T, n);
 }
 
+// With the boundary map on load, a tile of the border band (holding ghost
+// cells or interior cells within 2 of the edge) next to a tile marked for a
+// wet cell is marked too (2): the boundary pass of a cell gathers its ghost
+// images' adjoints, which lie in its own tile or a neighbour, and a tile the
+// passes skip has them from the dense pass, which knows only the dry ones.
+// A tile marked 2 is beyond ADJ_HALO of every wet cell, so its ghosts' values
+// are the dry ones too.
+__device__ __forceinline__ void adj_mark_border(const AdjTiles& T, int mx, int my, int t)
+{
+    const int tx = t % T.ntx, ty = t / T.ntx;
+    const bool bx = ADJ_TW * tx <= 3 || ADJ_TW * tx + ADJ_TW - 1 >= mx;   // i + 1 <= 3, >= mx
+    const bool by = ADJ_TH * ty <= 3 || ADJ_TH * ty + ADJ_TH - 1 >= my;
+    if (!(bx || by) || T.flags[t] != 0) return;
+    bool near = false;
+    for (int y = max(0, ty - 1); y <= min(T.nty - 1, ty + 1); y++)
+        for (int x = max(0, tx - 1); x <= min(T.ntx - 1, tx + 1); x++)
+            near = near || T.flags[y * T.ntx + x] == 1;
+    if (near) T.flags[t] = 2;
+}
+
+__global__ __launch_bounds__(256) void k_adjoint_border(AdjTiles T, int mx, int my)
+{
+    const int t = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (t < T.ntx * T.nty) adj_mark_border(T, mx, my, t);
+}
+
 // Marked tiles per block of 64 (pass 1 of the ordered compaction).
 __global__ __launch_bounds__(64) void k_adjoint_count_marks(AdjTiles T, int ntiles)
 {
@@ -1495,7 +1531,7 @@ __global__ __launch_bounds__(64) void k_adjoint_count_marks(AdjTiles T, int ntil
 }
 
 // Write the marked tiles into the list in tile order (T.bcount holds each
-// block's offset) and reset the marks.  Every tile pass writes only its own
+// block's offset); the marks stay for the boundary pass.  Every tile pass writes only its own
 // cells, so the order only affects cache locality.
 __global__ __launch_bounds__(64) void k_adjoint_list(AdjTiles T, int ntiles)
 {
@@ -1503,7 +1539,6 @@ __global__ __launch_bounds__(64) void k_adjoint_list(AdjTiles T, int ntiles)
     const bool f = t < ntiles && T.flags[t] != 0;
     const unsigned long long b = __ballot(f);
     if (f) {
-        T.flags[t] = 0;
         const unsigned long long below = b & ((1ull << threadIdx.x) - 1ull);
         T.list[T.bcount[blockIdx.x] + (int)__popcll(below)] = t;
     }
@@ -1621,6 +1656,8 @@ __device__ dclaw_adj::AdjArgs adj_member(const AdjBatch& B, int m, AdjTiles& T)
     A.N = dclaw_adj::plane_size(p.mx, p.my);
     A.ld = dclaw_adj::plane_ld(p.mx);
     T = adj_tiles(p.work, p.mx, p.my, A.N);
+    A.tflags = T.flags;
+    A.tntx = T.ntx;
     return A;
 }
 
@@ -1696,6 +1733,25 @@ __global__ __launch_bounds__(64) void k_adjb_clear_stale(AdjBatch B)
     }
 }
 
+// Zero member blockIdx.y's tile marks (start of a call).
+__global__ __launch_bounds__(256) void k_adjb_zero_flags(AdjBatch B)
+{
+    const dclaw_adj_patch& p = B.pt[blockIdx.y];
+    const AdjTiles T = adj_tiles(p.work, p.mx, p.my, dclaw_adj::plane_size(p.mx, p.my));
+    for (int t = (int)(blockIdx.x * 256 + threadIdx.x); t < T.ntx * T.nty;
+         t += (int)(gridDim.x * 256))
+        T.flags[t] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_adjb_border(AdjBatch B)
+{
+    const dclaw_adj_patch& p = B.pt[blockIdx.y];
+    const AdjTiles T = adj_tiles(p.work, p.mx, p.my, dclaw_adj::plane_size(p.mx, p.my));
+    for (int t = (int)(blockIdx.x * 256 + threadIdx.x); t < T.ntx * T.nty;
+         t += (int)(gridDim.x * 256))
+        adj_mark_border(T, p.mx, p.my, t);
+}
+
 __global__ __launch_bounds__(256) void k_adjb_count_zero(AdjBatch B)
 {
     const int m = blockIdx.x * 256 + threadIdx.x;
@@ -1756,7 +1812,6 @@ __global__ __launch_bounds__(64) void k_adjb_list(AdjBatch B, long long ntiles)
     if ((int)threadIdx.x == lead) base = atomicAdd(T.count, (int)__popcll(ba));
     base = __shfl(base, lead);
     if (!f) return;
-    T.flags[t] = 0;
     if (agg) T.list[base + (int)__popcll(ba & below)] = t;
     else T.list[atomicAdd(T.count, 1)] = t;
     const int slot = B.bcount[blockIdx.x] + (int)__popcll(b & below);
@@ -2220,7 +2275,12 @@ int dclaw_adjoint_step_dev(const double* q_in, const double* aux, int maux, int 
     const int ntiles = T.ntx * T.nty;
     hipStream_t st = (hipStream_t)stream;
     const dim3 tg(adj_grid(ntiles)), tb(64);
+    hipLaunchKernelGGL(k_zero_ints, dim3((ntiles + 255) / 256), dim3(256), 0, st, T.flags,
+                       ntiles);
     hipLaunchKernelGGL(k_adjoint_dense, dim3((unsigned)nb), dim3(ADJ_BLOCK), 0, st, A, T);
+    if (A.bc[0] != 0)
+        hipLaunchKernelGGL(k_adjoint_border, dim3((ntiles + 255) / 256), dim3(256), 0, st, T,
+                           mx, my);
     hipLaunchKernelGGL(k_adjoint_clear_stale, tg, tb, 0, st, A, T);
     const int nblk = (ntiles + 63) / 64;
     hipLaunchKernelGGL(k_adjoint_count_marks, dim3(nblk), dim3(64), 0, st, T, ntiles);
@@ -2318,8 +2378,11 @@ int dclaw_adjoint_batched(const dclaw_adj_patch* patches_dev, const dclaw_adj_pa
     hipStream_t st = (hipStream_t)stream;
     const dim3 tg(adj_grid(ntiles)), tb(64);
     hipLaunchKernelGGL(k_adjb_prefix, dim3(1), dim3(1024), 0, st, B);
+    hipLaunchKernelGGL(k_adjb_zero_flags, dim3(64, npatch), dim3(256), 0, st, B);
     hipLaunchKernelGGL(k_adjb_dense, dim3((unsigned)((ncells + ADJ_BLOCK - 1) / ADJ_BLOCK)),
                        dim3(ADJ_BLOCK), 0, st, B, ncells);
+    if (B.bc[0] != 0)
+        hipLaunchKernelGGL(k_adjb_border, dim3(64, npatch), dim3(256), 0, st, B);
     hipLaunchKernelGGL(k_adjb_clear_stale, dim3(32, npatch), tb, 0, st, B);
     hipLaunchKernelGGL(k_adjb_count_zero, dim3((npatch + 255) / 256), dim3(256), 0, st, B);
     hipLaunchKernelGGL(k_adjb_count_marks, dim3((unsigned)nblk), tb, 0, st, B, ntiles);

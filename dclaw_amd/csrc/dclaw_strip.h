@@ -210,14 +210,19 @@ DC_HD void dry_row_write(const StripArgs& A, const DryScan& D, int c, int lane, 
 }
 
 // src2 on the interior cells of one (strip, segment) tile: cell k of the
-// tile's 58 x (jB - jA + 1) block, k = column offset + 58 * row offset.
+// tile's 58 x (jB - jA + 1) block, k = column offset + 58 * row offset.  q
+// is the strip pass's output, in which a cell with h <= dry_tolerance is
+// already what src2 makes of it (h ... hchi zeroed, bdif kept: src2.f90:
+// 103-111), so such a cell is neither read further nor stored.
 DC_HD void src2_tile_cell(const dclaw_params& P, double dt, double* q, const double* aux,
                           int maux, int mx, int i0, int jA, int jB, int k)
 {
     const int i = i0 + k % STRIP_W, j = jA + k / STRIP_W;
     if (i > mx || j > jB) return;
     double* c = q + qidx(0, i, j, mx);
-    double h = c[0], hu = c[1], hv = c[2], hm = c[3], p = c[4], hchi = c[5], bdif = c[6];
+    double h = c[0];
+    if (h <= P.dry_tolerance) return;
+    double hu = c[1], hv = c[2], hm = c[3], p = c[4], hchi = c[5], bdif = c[6];
     const double h_e = P.entrainment ? aux[aidx(i, j, mx, maux) + 6] : 0.0;
     src2_cell(P, dt, h, hu, hv, hm, p, hchi, bdif, h_e);
     c[0] = h; c[1] = hu; c[2] = hv; c[3] = hm; c[4] = p; c[5] = hchi; c[6] = bdif;
