@@ -439,16 +439,22 @@ class EnsembleRollout:
         grad = torch.zeros((n_m, self.adj.np), dtype=torch.float64, device=e.dt.device)
         lams = [lam, lam_b]
 
+        last = [None]
+
         def advance(x, n):
             # one batched step from the states x (the ensemble's own buffers
-            # serve as scratch); returns copies of q_{n+1}
-            for k in range(n_m):
-                e.current(k).copy_(x[k])
-            e.state_written()
+            # serve as scratch); returns copies of q_{n+1}.  When x is the
+            # previous call's result the buffers still hold it: no copy, and
+            # the batched step goes on skipping the tiles that stayed dry
+            if x is not last[0]:
+                for k in range(n_m):
+                    e.current(k).copy_(x[k])
+                e.state_written()
             e.dt.copy_(self.dts[n])
             e.cfl.zero_()
             e.step()
-            return [y.clone() for y in self._states()]
+            last[0] = [y.clone() for y in self._states()]
+            return last[0]
 
         def reverse_step(n, qn, qn1):
             lo, li = lams

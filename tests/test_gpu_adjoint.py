@@ -277,9 +277,11 @@ def test_autograd_rollout(dev):
 
 def test_sparse_tiles_and_workspace_reuse(dev, emu_lib):
     """Small wet blobs in a 200 x 150 dry patch with random bdif (the tile
-    passes cover a few tiles; elsewhere only the dry map's bdif adjoint): one
-    workspace serves three calls (blob A, blob B elsewhere, A again), each
-    equal to the emulator's dense stages, the third bit for bit the first."""
+    passes cover a few tiles; elsewhere only the dry map's bdif adjoint, with
+    the border band's dry ghost images gathered by the dense pass): one
+    workspace serves four calls (blob A, blob B elsewhere, A again, a blob
+    across the wall near a corner), each equal to the emulator's dense
+    stages, the third bit for bit the first."""
     from dclaw_amd import adjoint as AD
     from dclaw_amd import step as S
     from dclaw_amd import tangent as T
@@ -299,6 +301,9 @@ def test_sparse_tiles_and_workspace_reuse(dev, emu_lib):
         return q
     states = [blob(50, 40, 10), blob(150, 110, 14)]
     states.append(states[0])
+    # a blob over the walled x-high edge near a corner: border tiles beside it
+    # run the passes, the rest of the band keeps the dense pass's dry gather
+    states.append(blob(197, 4, 9))
     aux = np.asfortranarray(pt.aux)
     bc = (1, 3, 1, 1)
     dPs = [CParams(phi=1.0), CParams(kref=1e-10)]
