@@ -1009,6 +1009,17 @@ def bench_c4(args):
         for line in open(tpath).read().splitlines():
             if line.startswith("traffic_bytes_per_step"):
                 traffic = float(line.split("=")[1])
+    # FP64 VALU work of one pair from the committed counter summary taken on
+    # this configuration (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes)
+    fp64 = None
+    fpath = os.path.join(ROOT, "profiles", f"fwd_adjoint_batched_c4_{len(members)}x{n}_fp64.txt")
+    if os.path.exists(fpath) and run_cfg in open(fpath).read():
+        for line in open(fpath).read().splitlines():
+            if line.startswith("fp64_ops_per_step"):
+                ops = float(line.split("=")[1])
+                rate = ops / pair_gpu_s / 1e12
+                fp64 = {"ops_per_step": ops, "achieved": rate, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": rate / FP64_PEAK_TFLOPS}
     del adj, lam, lin
     torch.cuda.empty_cache()
 
@@ -1070,6 +1081,7 @@ def bench_c4(args):
                                "dclaw_step_batched_ws (k_order_tiles, k_step_batched<1>, "
                                "k_src2_batched)",
                      "kernel_avg_ms": adj_ms + fwd_ms},
+        "fp64": fp64,
         "c1_forward": c1,
         "c2_pair": c2,
         "cpu_baseline": cpu,
