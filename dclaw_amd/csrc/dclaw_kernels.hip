@@ -310,7 +310,7 @@ __global__ __launch_bounds__(64) void k_step_batched_tl(const dclaw_patch* __res
 // patch; tile t = segment * nstrips + strip).  One wavefront per task, a task
 // being a 64-cell chunk of a tile, so the wet tiles spread over the whole chip;
 // grid-stride over the tasks.
-__global__ __launch_bounds__(64) void k_src2_tiles(double* q, const double* __restrict__ aux,
+__global__ __launch_bounds__(64, 4) void k_src2_tiles(double* q, const double* __restrict__ aux,
                                                    int maux, int mx, int my, int seg_rows,
                                                    int nstrips, int ntiles,
                                                    const int* __restrict__ tile_flags,
@@ -737,7 +737,7 @@ __global__ void k_zero_ints(int* a, int n)
 }
 
 // The same over a batched work list (each tile's patch, parameters and dt).
-__global__ __launch_bounds__(64) void k_src2_batched(const dclaw_patch* __restrict__ patches,
+__global__ __launch_bounds__(64, 4) void k_src2_batched(const dclaw_patch* __restrict__ patches,
                                                      const int32_t* __restrict__ tiles,
                                                      int ntiles, int seg_rows,
                                                      const int* __restrict__ tile_flags,
