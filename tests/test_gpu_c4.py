@@ -1,6 +1,7 @@
 """GPU: the batched-adjoint workload at full member size -- 64 members of
 1024 x 1024 cells (seeded fractal beds, one Gaussian source each, perturbed
-phi, m0, kref), each with its own gauges and gauge-misfit loss.  The batched
+phi, m0, kref), each with its own gauges and gauge-misfit loss.  Every
+member's forward run equals the oracle's bit for bit.  The batched
 reverse sweep (EnsembleRollout, two checkpoint levels, block checkpoints in
 host memory) gives the first and
 the last member exactly the gradient of that member's own Rollout, and the
@@ -50,6 +51,21 @@ def test_c4_batched_rollout_gradient(dev):
     er = AD.EnsembleRollout(members, nsteps, gxy, parameters=("phi", "kref"), checkpoint_every=6,
                             sub_every=2, seg_rows=32, host_checkpoints=True)
     series, qf = er.forward()
+    # the forward run of every member against the oracle (its OpenMP build,
+    # members spread over the host threads), bit for bit
+    import os
+    from oracle import oracle as O
+    O.build()
+    print("[c4] all 64 members' forward run against the oracle", flush=True)
+    with O.libm("omp"):
+        qos, _ = O.run_patches([O.params_from(m.patch.params) for m in members],
+                               [m.patch.q for m in members], [m.patch.aux for m in members],
+                               [m.patch.dx for m in members], [m.patch.dy for m in members],
+                               nsteps, 1e-3, 0.9, 1e9, nthreads=min(16, os.cpu_count() or 1))
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    for k in range(len(members)):
+        assert np.array_equal(S.to_host_q(qf[k])[I], qos[k][I]), k
+    del qos
     gs = [torch.zeros_like(s) for s in series]
     for k in range(len(members)):
         gs[k][:, :, 0] = 2.0 * (series[k][:, :, 0] - tg[k])
