@@ -80,3 +80,40 @@ def test_handles_side_by_side(dev):
     finally:
         L.dclaw_set_seg_rows(prev)
         L.dclaw_set_split_mode(prev_mode)
+
+
+def test_handles_in_two_threads(dev):
+    """Two host threads, each with its own handle (different segment heights
+    and split modes) and its own HIP stream, step their patches at the same
+    time; each result equals the same loop run alone, bit for bit."""
+    import threading
+    from dclaw_amd import step as S
+    from dclaw_amd import workloads as W
+    pa = W.fractal_sources(300, 200, dx=2.0, seed=7, nsrc=6)
+    pb = slope_patch(130, 70, slope=0.8, phi_deg=25.0, seed=201)
+    cases = [(pa, (1, 1, 1, 1), 4, "ordered"), (pb, (3, 1, 3, 3), 12, "hybrid")]
+
+    def run(p, bc, seg, mode, out, k):
+        s = dev.cuda.Stream()
+        with dev.cuda.stream(s):
+            h = S.Handle(seg_rows=seg, split_mode=mode)
+            r = S.PatchRunner(p.q, p.aux, p.params, p.dx, p.dy, bc=bc, handle=h)
+            r.run(50, 1e-3, 0.9, 1e9)
+            s.synchronize()
+            out[k] = (r.q(), r.dt.item())
+            h.close()
+
+    alone = [None, None]
+    for k, c in enumerate(cases):
+        run(*c, alone, k)
+    together = [None, None]
+    print("[handle] two threads, 50 steps each", flush=True)
+    ts = [threading.Thread(target=run, args=(*c, together, k)) for k, c in enumerate(cases)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for k in range(2):
+        assert together[k] is not None, k
+        assert np.array_equal(together[k][0], alone[k][0]), k
+        assert together[k][1] == alone[k][1], k
