@@ -105,6 +105,35 @@ def inclined_plane_mound(n: int = 200, dx: float = 1.0) -> Patch:
     return make_patch(b, h, 0.62, 0.8, p, dx, dx)
 
 
+BOWL_A, BOWL_SIGMA, BOWL_H0 = 1.0, 0.5, 0.1
+
+
+def bowl_slosh(n: int = 41, dry_tolerance: float = 1.0e-3) -> Patch:
+    """Clear water sloshing in a parabolic bowl (Thacker's planar surface),
+    the reference's tests/bowl-slosh case: domain [-2, 2]^2 in n x n cells
+    (setrun.py:70-79), bed B = h0((x^2 + y^2)/a^2 - 1) with a = 1, h0 = 0.1
+    (maketopo.py), surface eta = (sigma h0/a^2)(2x - sigma) with sigma = 0.5,
+    h = max(0, eta - B), hu = 0, hv = sigma omega h, omega = sqrt(2 g h0)/a
+    (qinit.f90:75-87), m = 0 and pb hydrostatic (init_ptype 0,
+    qinit.f90:236-247), and the D-Claw parameters of setrun.py:346-396
+    (phi 32 deg, mu 0.005, m0 = 0, kref 1e-8, alpha_c 0.05, delta 0.001).
+    The bed is the cell average of the paraboloid, which the reference's
+    topo integral over its 200 x 200 topo file approximates; depth and
+    momentum are set at cell centres, as qinit.f90 sets them."""
+    p = DigParams(grav=9.81, dry_tolerance=dry_tolerance, rho_f=1000.0, rho_s=2700.0,
+                  phi_deg=32.0, mu=0.005, mref=0.0, m_crit=0.64, kref=1.0e-8, alpha_c=0.05,
+                  delta=0.001, sigma_0=1.0e3, c1=1.0)
+    a, sig, h0 = BOWL_A, BOWL_SIGMA, BOWL_H0
+    dx = 4.0 / n
+    x = -2.0 + (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    b = h0 * ((X * X + Y * Y + dx * dx / 6.0) / (a * a) - 1.0)
+    h = np.maximum(0.0, sig * h0 / (a * a) * (2.0 * X - sig) - b)
+    pt = make_patch(b, h, 0.0, 1.0, p, dx, dx)
+    pt.q[2] = sig * math.sqrt(2.0 * p.grav * h0) / a * pt.q[0]
+    return pt
+
+
 def wet_plane(n: int = 2048, dx: float = 2.0, slope_deg: float = 20.0, h: float = 2.0) -> Patch:
     """Fully wet n x n patch: a plane inclined slope_deg in x under a uniform
     layer of depth h (m = 0.6, pb at 0.8 of hydrostatic), starting at rest.

@@ -33,6 +33,33 @@ def slope_patch(mx=40, my=30, slope=0.3, phi_deg=40.0, seed=0, h0=3.0, m0=0.62, 
     return W.make_patch(b, h, m0, pratio, p, 1.0, 1.0)
 
 
+def thacker_bowl(n: int, t: float):
+    """Thacker's exact planar-surface solution in the bowl of
+    workloads.bowl_slosh at time t, at the n x n cell centres of [-2, 2]^2
+    (docs/src/tests.rst:64-84): (h, u, v, bed)."""
+    a, sig, h0, g = W.BOWL_A, W.BOWL_SIGMA, W.BOWL_H0, 9.81
+    om = math.sqrt(2.0 * g * h0) / a
+    dx = 4.0 / n
+    x = -2.0 + (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    b = h0 * ((X * X + Y * Y + dx * dx / 6.0) / (a * a) - 1.0)
+    eta = sig * h0 / (a * a) * (2.0 * X * math.cos(om * t) + 2.0 * Y * math.sin(om * t) - sig)
+    return (np.maximum(0.0, eta - b), -sig * om * math.sin(om * t), sig * om * math.cos(om * t),
+            b)
+
+
+def bowl_errors(q: np.ndarray, n: int, t: float, deep: float = 0.02):
+    """Distance of an interior bowl state q (7, n, n) from Thacker's solution
+    at time t: mean |h - h_exact| over the domain and the largest velocity
+    error over the cells deeper than `deep` in both."""
+    he, ue, ve, _ = thacker_bowl(n, t)
+    h = q[0]
+    d = (he > deep) & (h > deep)
+    u = q[1][d] / h[d]
+    v = q[2][d] / h[d]
+    return float(np.abs(h - he).mean()), float(max(np.abs(u - ue).max(), np.abs(v - ve).max()))
+
+
 def rel_linf_wet(a: np.ndarray, b: np.ndarray, drytol: float) -> float:
     """max over components of ||a-b||_inf / ||b||_inf over cells wet in b."""
     wet = b[0] > drytol

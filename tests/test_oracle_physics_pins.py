@@ -23,7 +23,10 @@ and of the numerical method that a plausible transcription slip breaks:
   D, the critical-state m_eq) at vanishing time steps (Richardson
   extrapolation over two steps);
 * b4step2's speed limit (b4step2.f90:74-81) and the pore-pressure bounds of
-  qfix_cmass (digclaw_module.f90:353-387).
+  qfix_cmass (digclaw_module.f90:353-387);
+* the reference's bowl-slosh case (tests/bowl-slosh): clear water in a
+  paraboloid bowl converges to Thacker's exact planar-surface solution
+  (curved bed, moving shoreline, rotating flow; docs/src/tests.rst:56-84).
 
 tests/test_oracle_mutants.py compiles altered copies of the oracle (one
 plausible slip each) and checks that these pins reject every one of them.
@@ -552,13 +555,61 @@ def pin_still_impermeable_cell_keeps_pressure(O):
         assert 0.0 <= out[4, 2, 2] <= rho2 * G * h2 * (1 + 1e-14)
 
 
+# ------------------------------------------------ Thacker's bowl (tests/bowl-slosh)
+def _bowl_run(O, n, T, dry_tolerance, cfl=0.75, dt0=1e-4):
+    """workloads.bowl_slosh on n x n cells to time T with the reference
+    case's time-step rule (cfl_desired 0.75, dt_initial 1e-4: setrun.py:155-167)."""
+    pt = W.bowl_slosh(n, dry_tolerance)
+    prm = O.params_from(pt.params)
+    q, aux, t, dt = pt.q, pt.aux, 0.0, dt0
+    while t < T * (1 - 1e-14):
+        dt = min(dt, T - t)
+        q = O.bc_fill(q, (1, 1, 1, 1))
+        q, aux, c = O.stepgrid(prm, q, aux, pt.dx, pt.dy, dt)
+        t += dt
+        dt = dt * cfl / c if c > 0 else 2 * dt
+    return pt, q[:, 2:-2, 2:-2]
+
+
+def pin_bowl_slosh_thacker(O):
+    """Clear water (m = 0, hydrostatic pb) sloshing in a paraboloid bowl
+    converges to Thacker's planar-surface solution: over a quarter period
+    the mean depth error and the velocity error of the deep cells fall with
+    the grid (first order is what a moving shoreline allows), mass is kept
+    when the dry tolerance is negligible, and the source terms leave m = 0
+    and pb = rho_f g h exactly (zero effective stress: no Coulomb friction,
+    no relaxation, no dilatancy).  At the reference case's own 41 x 41 grid
+    and dry tolerance 1e-3, the state at its output time t = 1 stays close
+    to the solution and qfix's zeroing of cells thinner than the tolerance
+    (digclaw_module.f90 qfix) only removes mass."""
+    from helpers import bowl_errors
+    T = 0.25 * 2 * math.pi / math.sqrt(2 * G * W.BOWL_H0)
+    eh, ev = [], []
+    for n in (40, 80, 160):
+        pt, q = _bowl_run(O, n, T, 1e-6)
+        q0 = pt.q[:, 2:-2, 2:-2]
+        assert not q[3].any()
+        assert np.abs(q[4] - pt.params.rho_f * G * q[0]).max() <= 1e-10 * 1000.0 * G * 0.1
+        assert abs(q[0].sum() / q0[0].sum() - 1) < 1e-5
+        a, b = bowl_errors(q, n, T)
+        eh.append(a)
+        ev.append(b)
+    assert eh[0] > eh[1] > eh[2] and eh[0] / eh[2] > 3 and eh[2] < 2e-4, eh
+    assert ev[0] > ev[1] > ev[2] and ev[0] / ev[2] > 2.5 and ev[2] < 0.06, ev
+    pt, q = _bowl_run(O, 41, 1.0, 1e-3)
+    a, b = bowl_errors(q, 41, 1.0)
+    assert a < 1e-3 and b < 0.25, (a, b)
+    loss = q[0].sum() / pt.q[0, 2:-2, 2:-2].sum() - 1
+    assert -0.03 < loss <= 1e-12, loss
+
+
 PINS = [pin_rpt2_gravity_eigenvectors, pin_rpt2_uses_the_receiving_cell,
         pin_solver_waves_against_the_flow,
         pin_solver_stationary_bed_step, pin_rpn2_fluctuations, pin_basal_friction_rates,
         pin_pressure_relaxation_rate, pin_dilatancy_rate, pin_speed_limit,
         pin_still_impermeable_cell_keeps_pressure, pin_static_heap_threshold,
         pin_calc_taudir_still_cell, pin_fe4quad_rates, pin_curvature_running_gravity,
-        pin_second_order_1d, pin_second_order_2d]
+        pin_bowl_slosh_thacker, pin_second_order_1d, pin_second_order_2d]
 
 
 @pytest.mark.parametrize("pin", PINS, ids=[f.__name__[4:] for f in PINS])
